@@ -1,0 +1,923 @@
+// fsg_api.hip -- the C ABI (include/fsg.h): device state of one FluidDomain +
+// FluidSolver pair and the per-step orchestration of the gfx950 kernels.
+//
+// State carried between steps is exactly the reference's: front and back u, v
+// (swapped by index, src/MacGrid.cpp:89-93), prev and diff grids, labels, the
+// extension masks and the particles.  Every phase writes the same subset of
+// faces as the reference does, so faces the reference leaves stale stay stale.
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/fsg.h"
+#include "fsg_kernels.h"
+
+using namespace fsg;
+
+namespace {
+
+const char kValidateMsg[] =
+    "Memory pool and fluid domain does not match. Initialize the fluid solver with a "
+    "memory pool corresponding to the right fluid domain!";
+
+// Per-kernel HIP-event timer on the handle's stream.
+struct Timer {
+  int on = 0;  // 0 off, 1 every kernel, 2 only the pressure-solver kernels (cg_*)
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<double, long long>> acc;
+  std::vector<std::string> names;  // stable storage for fsg_get_timing
+  hipEvent_t get() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+  void collect() {
+    for (auto& p : pending) {
+      float ms = 0.f;
+      (void)hipEventSynchronize(p.b);
+      (void)hipEventElapsedTime(&ms, p.a, p.b);
+      auto& slot = acc[p.name];
+      slot.first += ms;
+      slot.second += 1;
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+  ~Timer() {
+    for (auto& p : pending) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+
+}  // namespace
+
+struct fsg_domain {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Geom g{};
+  float density = 0.f, pic_ratio = 0.f;
+  fsg_params prm{};
+  // velocity double buffers + prev/diff
+  float* u[2] = {nullptr, nullptr};
+  float* v[2] = {nullptr, nullptr};
+  int front = 0;
+  float *u_prev = nullptr, *v_prev = nullptr, *u_diff = nullptr, *v_diff = nullptr;
+  uint8_t* label = nullptr;
+  uint8_t* solid = nullptr;  // optional
+  float* phi = nullptr;      // optional
+  float4* acc = nullptr;
+  uint8_t* mask[4] = {nullptr, nullptr, nullptr, nullptr};  // u0 u1 v0 v1
+  int mask_front = 0;
+  uint8_t *info_u = nullptr, *info_v = nullptr;
+  // particles
+  float4* pt = nullptr;
+  int64_t np = 0, cap = 0;
+  uint8_t* reject = nullptr;
+  // pressure system
+  int* counts = nullptr;
+  int* offsets = nullptr;
+  int* n_fluid = nullptr;
+  int* fluid_index = nullptr;
+  int* cell_of = nullptr;
+  float* b = nullptr;
+  uint8_t* info = nullptr;
+  float* r = nullptr;
+  float* ap = nullptr;
+  float* xg = nullptr;
+  float* zg = nullptr;
+  float* pg0 = nullptr;
+  float* pg1 = nullptr;
+  double* x64 = nullptr;
+  double* partials = nullptr;
+  unsigned* bar = nullptr;
+  CgScalars* sc = nullptr;
+  int persistent_grid = 0;
+  // host-side statistics of the plain solver
+  fsg_stats host_stats{};
+  std::string err;
+  Timer timer;
+  std::vector<void*> allocations;
+};
+
+namespace {
+
+int fail(fsg_domain* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+#define HIPCHK(h, expr)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail((h), FSG_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(fsg_domain* h, T** p, size_t n) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, n * sizeof(T) + 16);
+  if (e != hipSuccess) return e;
+  h->allocations.push_back(q);
+  *p = reinterpret_cast<T*>(q);
+  return hipMemsetAsync(q, 0, n * sizeof(T) + 16, h->stream);
+}
+
+bool is_pow2(float x) {
+  if (!(x > 0.f) || !isfinite(x)) return false;
+  int e;
+  const float m = frexpf(x, &e);
+  return m == 0.5f;
+}
+
+// Timed launch helper: wraps `body` in events when timing is on.
+template <typename F>
+void timed(fsg_domain* h, const char* name, F&& body) {
+  if (!h->timer.on || (h->timer.on == 2 && strncmp(name, "cg_", 3) != 0)) {
+    body();
+    return;
+  }
+  Timer::Pending p{name, h->timer.get(), h->timer.get()};
+  (void)hipEventRecord(p.a, h->stream);
+  body();
+  (void)hipEventRecord(p.b, h->stream);
+  h->timer.pending.push_back(p);
+  if (h->timer.pending.size() > 4096) h->timer.collect();
+}
+
+inline size_t ncell(const fsg_domain* h) { return (size_t)h->g.nx * h->g.ny; }
+inline float* UF(fsg_domain* h) { return h->u[h->front]; }
+inline float* VF(fsg_domain* h) { return h->v[h->front]; }
+inline float* UB(fsg_domain* h) { return h->u[1 - h->front]; }
+inline float* VB(fsg_domain* h) { return h->v[1 - h->front]; }
+inline void swap_vel(fsg_domain* h) { h->front ^= 1; }
+inline uint8_t* MXF(fsg_domain* h) { return h->mask[h->mask_front ? 1 : 0]; }
+inline uint8_t* MXB(fsg_domain* h) { return h->mask[h->mask_front ? 0 : 1]; }
+inline uint8_t* MYF(fsg_domain* h) { return h->mask[h->mask_front ? 3 : 2]; }
+inline uint8_t* MYB(fsg_domain* h) { return h->mask[h->mask_front ? 2 : 3]; }
+
+CgParams make_cg_params(const fsg_domain* h, float tol) {
+  CgParams cp{};
+  cp.max_iter = h->prm.cg_max_iter;
+  cp.tol = tol;
+  const double dx = (double)h->g.dx;
+  cp.off = (float)(1 / pow(dx, 2));  // src/FluidSolver.cpp:381-402
+  for (int n = 0; n <= 4; ++n) {
+    cp.diag_tab[n] = (float)(-n / pow(dx, 2));  // :409-410
+    cp.invd_tab[n] = cp.diag_tab[n] != 0.f ? 1.0f / cp.diag_tab[n] : 1.0f;
+  }
+  cp.project_mean = h->prm.project_mean;
+  return cp;
+}
+
+// ---------------------------------------------------------------- phases
+int ph_classify(fsg_domain* h) {
+  timed(h, "cell_init", [&] { launch_cell_init(h->stream, h->g, h->label, h->solid, h->acc, 1, 0); });
+  timed(h, "particle_scatter", [&] {
+    launch_particle_scatter(h->stream, h->g, h->pt, h->np, h->label, h->acc, 1, 0);
+  });
+  return FSG_OK;
+}
+
+// Fused classify + P2G splat (one read of the particles), then the per-face
+// normalise [+ prev + gravity + Dirichlet + extension init] on the new front.
+void classify_and_splat(fsg_domain* h) {
+  timed(h, "cell_init", [&] { launch_cell_init(h->stream, h->g, h->label, h->solid, h->acc, 1, 1); });
+  timed(h, "particle_scatter", [&] {
+    launch_particle_scatter(h->stream, h->g, h->pt, h->np, h->label, h->acc, 1, 1);
+  });
+}
+
+void face_update(fsg_domain* h, int flags, float dt, bool record_info) {
+  const float gx_dt = h->prm.gravity_x * dt;
+  const float gy_dt = h->prm.gravity_y * dt;
+  timed(h, "face_update", [&] {
+    launch_face_update(h->stream, h->g, flags, UF(h), VF(h), h->acc, h->u_prev, h->v_prev, h->label,
+                       gx_dt, gy_dt, UB(h), VB(h), MXF(h), MXB(h), MYF(h), MYB(h),
+                       record_info ? h->info_u : nullptr, record_info ? h->info_v : nullptr);
+  });
+}
+
+void extend_sweeps(fsg_domain* h, bool record_info) {
+  for (int it = 0; it < h->prm.ext_layers; ++it) {
+    timed(h, "extend_sweep", [&] {
+      launch_extend_sweep(h->stream, h->g, h->label, UB(h), VB(h), MXF(h), MXB(h), MYF(h), MYB(h),
+                          record_info ? h->info_u : nullptr, record_info ? h->info_v : nullptr,
+                          it + 1);
+    });
+    h->mask_front ^= 1;  // FluidSolverMemoryPool::swapValidMaskBuffer
+  }
+  swap_vel(h);
+}
+
+int solve_pressure(fsg_domain* h) {
+  const int nb = (int)((ncell(h) + kScanChunk - 1) / kScanChunk);
+  timed(h, "count_liquid", [&] { launch_count_liquid(h->stream, h->g, h->label, h->counts); });
+  timed(h, "scan_counts", [&] { launch_scan_counts(h->stream, nb, h->counts, h->offsets, h->n_fluid); });
+  timed(h, "compact", [&] {
+    launch_compact(h->stream, h->g, h->label, h->offsets, h->fluid_index, h->cell_of, UF(h), VF(h),
+                   h->b, h->info);
+  });
+  const int solver = h->prm.solver;
+  if (solver == FSG_SOLVER_PLAIN) {
+    int F = 0;
+    HIPCHK(h, hipMemcpyAsync(&F, h->n_fluid, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->host_stats.n_fluid = F;
+    h->host_stats.cg_iterations = 0;
+    h->host_stats.cg_error = 0.0;
+    h->host_stats.refine_sweeps = 0;
+    if (F == 0) return FSG_OK;
+    const CgParams cp = make_cg_params(h, (float)h->prm.cg_tol);
+    if (cp.project_mean)
+      timed(h, "cg_mean", [&] { launch_cg_mean(h->stream, F, h->b, h->partials, h->sc); });
+    timed(h, "cg_init", [&] {
+      launch_cg_init(h->stream, F, cp, h->cell_of, h->info, h->b, h->r, h->pg0, h->xg, h->partials,
+                     h->sc);
+    });
+    const int check = h->prm.check_every > 0 ? h->prm.check_every : 32;
+    int launched = 0;
+    CgScalars hs{};
+    while (true) {
+      HIPCHK(h, hipMemcpyAsync(&hs, h->sc, sizeof(hs), hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      if (hs.done || launched >= cp.max_iter) break;
+      const int n = std::min(check, cp.max_iter - launched);
+      for (int q = 0; q < n; ++q)
+        timed(h, "cg_iteration", [&] {
+          launch_cg_iteration(h->stream, F, h->g.nx, cp, h->cell_of, h->info, h->r, h->ap, h->pg0,
+                              h->xg, h->partials, h->sc);
+        });
+      launched += n;
+    }
+    h->host_stats.cg_iterations = hs.iter;
+    h->host_stats.cg_error = hs.err;
+    h->host_stats.total_unknown_iterations += (int64_t)F * hs.spmv;
+    h->host_stats.total_solves += 1;
+    return FSG_OK;
+  }
+  // persistent / refined: one cooperative launch, no host decision
+  PersistentArgs a{};
+  a.nx = h->g.nx;
+  a.F_max = (int)ncell(h);
+  a.n_fluid = h->n_fluid;
+  a.cell_of = h->cell_of;
+  a.info = h->info;
+  a.b = h->b;
+  a.r = h->r;
+  a.xg = h->xg;
+  a.zg = h->zg;
+  a.pg0 = h->pg0;
+  a.pg1 = h->pg1;
+  a.x64 = h->x64;
+  a.e32 = h->ap;
+  a.partials = h->partials;
+  a.bar = h->bar;
+  a.sc = h->sc;
+  a.refined = solver == FSG_SOLVER_REFINED;
+  a.cp = make_cg_params(h, a.refined ? (float)h->prm.refine_inner_tol : (float)h->prm.cg_tol);
+  a.max_outer = h->prm.refine_max_outer;
+  a.inner_tol = (float)h->prm.refine_inner_tol;
+  a.outer_tol = h->prm.cg_tol;
+  timed(h, "zero_barrier", [&] { launch_fill(h->stream, reinterpret_cast<float*>(h->bar), 4, 0.f); });
+  hipError_t e = hipSuccess;
+  timed(h, a.refined ? "cg_refined" : "cg_persistent",
+        [&] { e = launch_cg_persistent(h->stream, a, h->persistent_grid); });
+  HIPCHK(h, e);
+  return FSG_OK;
+}
+
+int ph_pressure(fsg_domain* h, float dt, bool dirichlet, bool diff) {
+  int rc = solve_pressure(h);
+  if (rc != FSG_OK) return rc;
+  const float dt_over_density = dt / h->density;
+  timed(h, "pressure_update", [&] {
+    launch_pressure_update(h->stream, h->g, h->n_fluid, h->label, h->xg, UF(h), VF(h), UB(h), VB(h),
+                           dt_over_density, dirichlet ? 1 : 0, diff ? 1 : 0, h->u_prev, h->v_prev,
+                           h->u_diff, h->v_diff);
+  });
+  swap_vel(h);
+  return FSG_OK;
+}
+
+int check_validate(fsg_domain* h) {
+  // FluidSolver::validate against the pool the solver copied: its copy constructor
+  // passes deltaX twice (src/FluidSolver.cpp:56-63), so the check reduces to
+  // |dx - dx| < 1e-7 and |dy - dx| < 1e-7 (in float, compared with a double).
+  float d = h->g.dy - h->g.dx;
+  if (d < 0) d = -d;
+  if (!((double)d < 0.0000001)) return fail(h, FSG_E_VALIDATE, kValidateMsg);
+  return FSG_OK;
+}
+
+int step_flip_family(fsg_domain* h, int scheme, float dt) {
+  const bool flip = scheme != FSG_SCHEME_PIC;
+  const bool rec = h->prm.record_ext_info != 0;
+  classify_and_splat(h);
+  // P2G writes the back buffer, then swaps (src/FluidSolver.cpp:898-918)
+  swap_vel(h);
+  int flags = FU_P2G | FU_GRAVITY | FU_DIRICHLET | FU_EXTINIT;
+  if (flip) flags |= FU_PREV;
+  // the P2G normalise must read the stale values of the buffer it writes: after the
+  // swap that buffer is the front, and the extension init writes the new back.
+  face_update(h, flags, dt, rec);
+  extend_sweeps(h, rec);
+  int rc = ph_pressure(h, dt, /*dirichlet=*/true, /*diff=*/flip);
+  if (rc != FSG_OK) return rc;
+  const int g2p = scheme == FSG_SCHEME_PIC ? 1 : scheme == FSG_SCHEME_FLIP ? 2 : 3;
+  const int move = scheme == FSG_SCHEME_PICFLIP ? 2 : 1;
+  timed(h, "particle_gather", [&] {
+    launch_particle_gather(h->stream, h->g, h->pt, h->np, UF(h), VF(h), h->u_diff, h->v_diff,
+                           h->pic_ratio, g2p, move, dt, h->label, h->reject);
+  });
+  return FSG_OK;
+}
+
+int step_semi_lagrangian(fsg_domain* h, float dt) {
+  int rc = ph_classify(h);
+  if (rc) return rc;
+  timed(h, "fill", [&] {
+    launch_fill(h->stream, UB(h), (int64_t)ncell(h), 0.f);
+    launch_fill(h->stream, VB(h), (int64_t)ncell(h), 0.f);
+  });
+  timed(h, "sl_scatter", [&] {
+    launch_sl_scatter(h->stream, h->g, h->label, UF(h), VF(h), UB(h), VB(h), h->prm.integrator, dt);
+  });
+  face_update(h, FU_GRAVITY | FU_DIRICHLET, dt, false);
+  rc = ph_pressure(h, dt, true, false);
+  if (rc) return rc;
+  timed(h, "advect_grid", [&] {
+    launch_advect_grid(h->stream, h->g, h->pt, h->np, UF(h), VF(h), h->prm.integrator, dt);
+  });
+  return FSG_OK;
+}
+
+int step_grid_advect(fsg_domain* h, float dt) {
+  if (!h->phi) return fail(h, FSG_E_INVALID, "grid-only scheme needs a level set (fsg_set_level_set)");
+  timed(h, "classify_levelset",
+        [&] { launch_classify_levelset(h->stream, h->g, h->phi, h->solid, h->label); });
+  timed(h, "sl_gather", [&] {
+    launch_sl_gather(h->stream, h->g, h->label, UF(h), VF(h), UB(h), VB(h), h->prm.integrator, dt);
+  });
+  swap_vel(h);
+  face_update(h, FU_GRAVITY | FU_DIRICHLET, dt, false);
+  return ph_pressure(h, dt, true, false);
+}
+
+int check_grid_ptr(fsg_domain* h, const void* p) {
+  if (!p) return fail(h, FSG_E_INVALID, "null array");
+  return FSG_OK;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+const char* fsg_version(void) { return "fsg 0.1 (gfx950)"; }
+
+void fsg_params_default(fsg_params* p) {
+  if (!p) return;
+  memset(p, 0, sizeof(*p));
+  p->gravity_x = 0.0f;
+  p->gravity_y = (float)-9.82;
+  p->cg_tol = (double)FLT_EPSILON;
+  p->cg_max_iter = 100;
+  p->ext_layers = 2;
+  p->integrator = FSG_RK3;
+  p->solver = FSG_SOLVER_PLAIN;
+  p->project_mean = 0;
+  p->check_every = 32;
+  p->refine_inner_tol = 1e-3;
+  p->refine_max_outer = 20;
+  p->record_ext_info = 0;
+}
+
+int fsg_create(fsg_domain** out, int32_t nx, int32_t ny, float length_x, float length_y,
+               float density, float pic_ratio) {
+  if (!out) return FSG_E_INVALID;
+  *out = nullptr;
+  if (nx < 3 || ny < 3 || (int64_t)nx * ny > (int64_t)1 << 30) return FSG_E_INVALID;
+  if (!(length_x > 0.f) || !(length_y > 0.f)) return FSG_E_INVALID;
+  fsg_domain* h = new (std::nothrow) fsg_domain();
+  if (!h) return FSG_E_NOMEM;
+  auto bail = [&](int code) {
+    fsg_destroy(h);
+    return code;
+  };
+  if (hipGetDevice(&h->device) != hipSuccess) return bail(FSG_E_HIP);
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return bail(FSG_E_HIP);
+  fsg_params_default(&h->prm);
+  Geom& g = h->g;
+  g.nx = nx;
+  g.ny = ny;
+  g.dx = length_x / (float)nx;  // src/MacGrid.cpp:8
+  g.dy = length_y / (float)ny;
+  g.pow2_x = is_pow2(g.dx);
+  g.pow2_y = is_pow2(g.dy);
+  g.inv_dx = g.pow2_x ? 1.0f / g.dx : 0.f;
+  g.inv_dy = g.pow2_y ? 1.0f / g.dy : 0.f;
+  g.len_x = (float)nx * g.dx;   // GridInterface::lengthX, include/Grid.h:54
+  g.len_y = (float)ny * g.dy;
+  h->density = density;
+  h->pic_ratio = pic_ratio;     // the constructor stores it unclamped (src/FluidDomain.cpp:54-68)
+  const size_t C = (size_t)nx * ny;
+  const int nb = (int)((C + kScanChunk - 1) / kScanChunk);
+  bool ok = true;
+  ok &= dalloc(h, &h->u[0], C) == hipSuccess;
+  ok &= dalloc(h, &h->u[1], C) == hipSuccess;
+  ok &= dalloc(h, &h->v[0], C) == hipSuccess;
+  ok &= dalloc(h, &h->v[1], C) == hipSuccess;
+  ok &= dalloc(h, &h->u_prev, C) == hipSuccess;
+  ok &= dalloc(h, &h->v_prev, C) == hipSuccess;
+  ok &= dalloc(h, &h->u_diff, C) == hipSuccess;
+  ok &= dalloc(h, &h->v_diff, C) == hipSuccess;
+  ok &= dalloc(h, &h->label, C) == hipSuccess;
+  ok &= dalloc(h, &h->acc, C) == hipSuccess;
+  for (int q = 0; q < 4; ++q) ok &= dalloc(h, &h->mask[q], C) == hipSuccess;
+  ok &= dalloc(h, &h->info_u, C) == hipSuccess;
+  ok &= dalloc(h, &h->info_v, C) == hipSuccess;
+  ok &= dalloc(h, &h->counts, (size_t)nb) == hipSuccess;
+  ok &= dalloc(h, &h->offsets, (size_t)nb) == hipSuccess;
+  ok &= dalloc(h, &h->n_fluid, 4) == hipSuccess;
+  ok &= dalloc(h, &h->fluid_index, C) == hipSuccess;
+  ok &= dalloc(h, &h->cell_of, C) == hipSuccess;
+  ok &= dalloc(h, &h->b, C) == hipSuccess;
+  ok &= dalloc(h, &h->info, C) == hipSuccess;
+  ok &= dalloc(h, &h->r, C) == hipSuccess;
+  ok &= dalloc(h, &h->ap, C) == hipSuccess;
+  ok &= dalloc(h, &h->xg, C) == hipSuccess;
+  ok &= dalloc(h, &h->zg, C) == hipSuccess;
+  ok &= dalloc(h, &h->pg0, C) == hipSuccess;
+  ok &= dalloc(h, &h->pg1, C) == hipSuccess;
+  ok &= dalloc(h, &h->x64, C) == hipSuccess;
+  ok &= dalloc(h, &h->partials, (size_t)4 * 4096) == hipSuccess;
+  ok &= dalloc(h, &h->bar, 16) == hipSuccess;
+  ok &= dalloc(h, &h->sc, 1) == hipSuccess;
+  if (!ok) return bail(FSG_E_NOMEM);
+  // labels start as clearCellTypeBuffer leaves them (MacGrid ctor, src/MacGrid.cpp:24)
+  launch_cell_init(h->stream, h->g, h->label, nullptr, h->acc, 1, 0);
+  h->persistent_grid = persistent_grid_size(h->device);
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(FSG_E_HIP);
+  *out = h;
+  return FSG_OK;
+}
+
+int fsg_destroy(fsg_domain* h) {
+  if (!h) return FSG_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  h->timer.collect();
+  for (void* p : h->allocations) (void)hipFree(p);
+  if (h->pt) (void)hipFree(h->pt);
+  if (h->reject) (void)hipFree(h->reject);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return FSG_OK;
+}
+
+const char* fsg_last_error(const fsg_domain* h) { return h ? h->err.c_str() : "null handle"; }
+
+int fsg_set_params(fsg_domain* h, const fsg_params* p) {
+  if (!h || !p) return FSG_E_INVALID;
+  if (p->cg_max_iter < 0 || p->ext_layers < 0 || p->ext_layers > 14 || p->integrator < 0 ||
+      p->integrator > 2 || p->solver < 0 || p->solver > 2 || !(p->cg_tol >= 0.0))
+    return fail(h, FSG_E_INVALID, "fsg_set_params: value out of range");
+  if (p->solver != FSG_SOLVER_PLAIN && h->persistent_grid <= 0)
+    return fail(h, FSG_E_INVALID, "persistent solver unavailable on this device");
+  h->prm = *p;
+  return FSG_OK;
+}
+
+int fsg_get_params(const fsg_domain* h, fsg_params* p) {
+  if (!h || !p) return FSG_E_INVALID;
+  *p = h->prm;
+  return FSG_OK;
+}
+
+int fsg_set_pic_ratio(fsg_domain* h, float r) {
+  if (!h) return FSG_E_INVALID;
+  // CLAMP(pic_ratio, 0, 1) (src/FluidDomain.cpp:99-102)
+  const float t = r < 0.f ? 0.f : r;
+  h->pic_ratio = t > 1.f ? 1.f : t;
+  return FSG_OK;
+}
+
+int fsg_get_geometry(const fsg_domain* h, int32_t* nx, int32_t* ny, float* dx, float* dy,
+                     float* density, float* pic_ratio) {
+  if (!h) return FSG_E_INVALID;
+  if (nx) *nx = h->g.nx;
+  if (ny) *ny = h->g.ny;
+  if (dx) *dx = h->g.dx;
+  if (dy) *dy = h->g.dy;
+  if (density) *density = h->density;
+  if (pic_ratio) *pic_ratio = h->pic_ratio;
+  return FSG_OK;
+}
+
+int fsg_synchronize(fsg_domain* h) {
+  if (!h) return FSG_E_INVALID;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+static int reserve_particles(fsg_domain* h, int64_t n) {
+  if (n <= h->cap) return FSG_OK;
+  int64_t cap = h->cap > 0 ? h->cap : 1024;
+  while (cap < n) cap *= 2;
+  float4* np = nullptr;
+  uint8_t* nr = nullptr;
+  HIPCHK(h, hipMalloc(&np, (size_t)cap * sizeof(float4)));
+  HIPCHK(h, hipMalloc(&nr, (size_t)cap));
+  HIPCHK(h, hipMemsetAsync(nr, 0, (size_t)cap, h->stream));
+  if (h->np > 0) {
+    HIPCHK(h, hipMemcpyAsync(np, h->pt, (size_t)h->np * sizeof(float4), hipMemcpyDeviceToDevice,
+                             h->stream));
+    HIPCHK(h, hipMemcpyAsync(nr, h->reject, (size_t)h->np, hipMemcpyDeviceToDevice, h->stream));
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (h->pt) (void)hipFree(h->pt);
+  if (h->reject) (void)hipFree(h->reject);
+  h->pt = np;
+  h->reject = nr;
+  h->cap = cap;
+  return FSG_OK;
+}
+
+int fsg_set_particles(fsg_domain* h, const float* xyuv, int64_t n) {
+  if (!h || n < 0 || (n > 0 && !xyuv)) return FSG_E_INVALID;
+  h->np = 0;
+  return fsg_add_particles(h, xyuv, n);
+}
+
+int fsg_add_particles(fsg_domain* h, const float* xyuv, int64_t n) {
+  if (!h || n < 0 || (n > 0 && !xyuv)) return FSG_E_INVALID;
+  if (n == 0) return FSG_OK;
+  int rc = reserve_particles(h, h->np + n);
+  if (rc) return rc;
+  HIPCHK(h, hipMemcpyAsync(h->pt + h->np, xyuv, (size_t)n * sizeof(float4), hipMemcpyHostToDevice,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->np += n;
+  return FSG_OK;
+}
+
+int fsg_num_particles(const fsg_domain* h, int64_t* n) {
+  if (!h || !n) return FSG_E_INVALID;
+  *n = h->np;
+  return FSG_OK;
+}
+
+int fsg_get_particles(fsg_domain* h, float* xyuv, int64_t capacity) {
+  if (!h || (h->np > 0 && !xyuv)) return FSG_E_INVALID;
+  if (capacity < h->np) return fail(h, FSG_E_INVALID, "fsg_get_particles: capacity too small");
+  if (h->np == 0) return FSG_OK;
+  HIPCHK(h, hipMemcpyAsync(xyuv, h->pt, (size_t)h->np * sizeof(float4), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+static float* vel_ptr(fsg_domain* h, int which, int comp) {
+  switch (which) {
+    case FSG_VEL_FRONT: return comp ? VF(h) : UF(h);
+    case FSG_VEL_BACK: return comp ? VB(h) : UB(h);
+    case FSG_VEL_PREV: return comp ? h->v_prev : h->u_prev;
+    case FSG_VEL_DIFF: return comp ? h->v_diff : h->u_diff;
+    default: return nullptr;
+  }
+}
+
+int fsg_set_velocity(fsg_domain* h, int32_t which, const float* u, const float* v) {
+  if (!h) return FSG_E_INVALID;
+  float* du = vel_ptr(h, which, 0);
+  float* dv = vel_ptr(h, which, 1);
+  if (!du) return fail(h, FSG_E_INVALID, "fsg_set_velocity: bad buffer id");
+  const size_t bytes = ncell(h) * sizeof(float);
+  if (u) HIPCHK(h, hipMemcpyAsync(du, u, bytes, hipMemcpyHostToDevice, h->stream));
+  if (v) HIPCHK(h, hipMemcpyAsync(dv, v, bytes, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+int fsg_get_velocity(fsg_domain* h, int32_t which, float* u, float* v) {
+  if (!h) return FSG_E_INVALID;
+  float* du = vel_ptr(h, which, 0);
+  float* dv = vel_ptr(h, which, 1);
+  if (!du) return fail(h, FSG_E_INVALID, "fsg_get_velocity: bad buffer id");
+  const size_t bytes = ncell(h) * sizeof(float);
+  if (u) HIPCHK(h, hipMemcpyAsync(u, du, bytes, hipMemcpyDeviceToHost, h->stream));
+  if (v) HIPCHK(h, hipMemcpyAsync(v, dv, bytes, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+int fsg_swap_velocity(fsg_domain* h) {
+  if (!h) return FSG_E_INVALID;
+  swap_vel(h);
+  return FSG_OK;
+}
+
+int fsg_set_labels(fsg_domain* h, const uint8_t* labels) {
+  if (!h || check_grid_ptr(h, labels)) return FSG_E_INVALID;
+  const int nx = h->g.nx, ny = h->g.ny;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const uint8_t l = labels[(size_t)i + (size_t)j * nx];
+      if (l > FSG_SOLID) return fail(h, FSG_E_INVALID, "fsg_set_labels: label out of range");
+      const bool border = i == 0 || j == 0 || i == nx - 1 || j == ny - 1;
+      if (border && l != FSG_SOLID)
+        return fail(h, FSG_E_INVALID, "fsg_set_labels: border cells must be SOLID");
+    }
+  HIPCHK(h, hipMemcpyAsync(h->label, labels, ncell(h), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+int fsg_get_labels(fsg_domain* h, uint8_t* labels) {
+  if (!h || check_grid_ptr(h, labels)) return FSG_E_INVALID;
+  HIPCHK(h, hipMemcpyAsync(labels, h->label, ncell(h), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+int fsg_set_solid_mask(fsg_domain* h, const uint8_t* mask) {
+  if (!h) return FSG_E_INVALID;
+  if (!mask) {
+    h->solid = nullptr;  // the allocation stays owned by the handle
+    return FSG_OK;
+  }
+  static_assert(sizeof(uint8_t) == 1, "");
+  uint8_t* d = nullptr;
+  HIPCHK(h, dalloc(h, &d, ncell(h)));
+  HIPCHK(h, hipMemcpyAsync(d, mask, ncell(h), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->solid = d;
+  return FSG_OK;
+}
+
+int fsg_set_level_set(fsg_domain* h, const float* phi) {
+  if (!h || check_grid_ptr(h, phi)) return FSG_E_INVALID;
+  if (!h->phi) HIPCHK(h, dalloc(h, &h->phi, ncell(h)));
+  HIPCHK(h, hipMemcpyAsync(h->phi, phi, ncell(h) * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+int fsg_validate_pool(fsg_domain* h, int32_t pool_nx, int32_t pool_ny, float pool_dx,
+                      float pool_dy) {
+  if (!h) return FSG_E_INVALID;
+  float ddx = h->g.dx - pool_dx, ddy = h->g.dy - pool_dy;
+  if (ddx < 0) ddx = -ddx;
+  if (ddy < 0) ddy = -ddy;
+  const bool ok = h->g.nx == pool_nx && h->g.ny == pool_ny && (double)ddx < 0.0000001 &&
+                  (double)ddy < 0.0000001;
+  return ok ? FSG_OK : fail(h, FSG_E_VALIDATE, kValidateMsg);
+}
+
+int fsg_step(fsg_domain* h, int32_t scheme, float dt) {
+  if (!h) return FSG_E_INVALID;
+  if (scheme != FSG_SCHEME_GRID_ADVECT) {
+    int rc = check_validate(h);
+    if (rc) return rc;
+  }
+  int rc;
+  switch (scheme) {
+    case FSG_SCHEME_SEMI_LAGRANGIAN: rc = step_semi_lagrangian(h, dt); break;
+    case FSG_SCHEME_PIC:
+    case FSG_SCHEME_FLIP:
+    case FSG_SCHEME_PICFLIP: rc = step_flip_family(h, scheme, dt); break;
+    case FSG_SCHEME_GRID_ADVECT: rc = step_grid_advect(h, dt); break;
+    default: return fail(h, FSG_E_INVALID, "fsg_step: unknown scheme");
+  }
+  if (rc) return rc;
+  HIPCHK(h, hipGetLastError());
+  return FSG_OK;
+}
+
+int fsg_run_phase(fsg_domain* h, int32_t phase, float dt) {
+  if (!h) return FSG_E_INVALID;
+  const bool rec = true;
+  switch (phase) {
+    case FSG_PH_CLASSIFY: ph_classify(h); break;
+    case FSG_PH_P2G:
+      timed(h, "cell_init", [&] { launch_cell_init(h->stream, h->g, h->label, h->solid, h->acc, 0, 1); });
+      timed(h, "particle_scatter", [&] {
+        launch_particle_scatter(h->stream, h->g, h->pt, h->np, h->label, h->acc, 0, 1);
+      });
+      swap_vel(h);  // the normalise writes the (old) back, which is now the front
+      face_update(h, FU_P2G, dt, false);
+      break;
+    case FSG_PH_SAVE_PREV: face_update(h, FU_PREV, dt, false); break;
+    case FSG_PH_GRAVITY: face_update(h, FU_GRAVITY, dt, false); break;
+    case FSG_PH_DIRICHLET: face_update(h, FU_DIRICHLET, dt, false); break;
+    case FSG_PH_EXTEND:
+      face_update(h, FU_EXTINIT, dt, rec);
+      extend_sweeps(h, rec);
+      break;
+    case FSG_PH_PRESSURE: {
+      int rc = ph_pressure(h, dt, false, false);
+      if (rc) return rc;
+      break;
+    }
+    case FSG_PH_DIFF:
+      timed(h, "diff", [&] {
+        launch_diff(h->stream, h->g, UF(h), VF(h), h->u_prev, h->v_prev, h->u_diff, h->v_diff);
+      });
+      break;
+    case FSG_PH_G2P_PIC:
+    case FSG_PH_G2P_FLIP:
+    case FSG_PH_G2P_PICFLIP: {
+      const int mode = phase == FSG_PH_G2P_PIC ? 1 : phase == FSG_PH_G2P_FLIP ? 2 : 3;
+      timed(h, "particle_gather", [&] {
+        launch_particle_gather(h->stream, h->g, h->pt, h->np, UF(h), VF(h), h->u_diff, h->v_diff,
+                               h->pic_ratio, mode, 0, dt, h->label, h->reject);
+      });
+      break;
+    }
+    case FSG_PH_ADVECT:
+    case FSG_PH_ADVECT_REJECT: {
+      const int move = phase == FSG_PH_ADVECT ? 1 : 2;
+      timed(h, "particle_gather", [&] {
+        launch_particle_gather(h->stream, h->g, h->pt, h->np, UF(h), VF(h), h->u_diff, h->v_diff,
+                               h->pic_ratio, 0, move, dt, h->label, h->reject);
+      });
+      break;
+    }
+    case FSG_PH_SL_SCATTER:
+      timed(h, "fill", [&] {
+        launch_fill(h->stream, UB(h), (int64_t)ncell(h), 0.f);
+        launch_fill(h->stream, VB(h), (int64_t)ncell(h), 0.f);
+      });
+      timed(h, "sl_scatter", [&] {
+        launch_sl_scatter(h->stream, h->g, h->label, UF(h), VF(h), UB(h), VB(h), h->prm.integrator,
+                          dt);
+      });
+      break;
+    case FSG_PH_SL_GATHER:
+      timed(h, "sl_gather", [&] {
+        launch_sl_gather(h->stream, h->g, h->label, UF(h), VF(h), UB(h), VB(h), h->prm.integrator, dt);
+      });
+      swap_vel(h);
+      break;
+    case FSG_PH_ADVECT_GRID:
+      timed(h, "advect_grid", [&] {
+        launch_advect_grid(h->stream, h->g, h->pt, h->np, UF(h), VF(h), h->prm.integrator, dt);
+      });
+      break;
+    case FSG_PH_CLASSIFY_LEVELSET:
+      if (!h->phi) return fail(h, FSG_E_INVALID, "no level set (fsg_set_level_set)");
+      timed(h, "classify_levelset",
+            [&] { launch_classify_levelset(h->stream, h->g, h->phi, h->solid, h->label); });
+      break;
+    default: return fail(h, FSG_E_INVALID, "fsg_run_phase: unknown phase");
+  }
+  HIPCHK(h, hipGetLastError());
+  return FSG_OK;
+}
+
+int fsg_get_pressure(fsg_domain* h, float* p) {
+  if (!h || check_grid_ptr(h, p)) return FSG_E_INVALID;
+  const size_t C = ncell(h);
+  std::vector<int> idx(C);
+  HIPCHK(h, hipMemcpyAsync(p, h->xg, C * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(idx.data(), h->fluid_index, C * sizeof(int), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (size_t c = 0; c < C; ++c)
+    if (idx[c] < 0) p[c] = 0.f;
+  return FSG_OK;
+}
+
+int fsg_get_rhs(fsg_domain* h, float* bout) {
+  if (!h || check_grid_ptr(h, bout)) return FSG_E_INVALID;
+  const size_t C = ncell(h);
+  std::vector<int> idx(C);
+  std::vector<float> bc(C);
+  HIPCHK(h, hipMemcpyAsync(idx.data(), h->fluid_index, C * sizeof(int), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipMemcpyAsync(bc.data(), h->b, C * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (size_t c = 0; c < C; ++c) bout[c] = idx[c] >= 0 ? bc[(size_t)idx[c]] : 0.f;
+  return FSG_OK;
+}
+
+int fsg_get_fluid_indices(fsg_domain* h, int32_t* idx) {
+  if (!h || check_grid_ptr(h, idx)) return FSG_E_INVALID;
+  HIPCHK(h, hipMemcpyAsync(idx, h->fluid_index, ncell(h) * sizeof(int), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+int fsg_get_valid_masks(fsg_domain* h, uint8_t* mu, uint8_t* mv) {
+  if (!h) return FSG_E_INVALID;
+  if (mu) HIPCHK(h, hipMemcpyAsync(mu, MXF(h), ncell(h), hipMemcpyDeviceToHost, h->stream));
+  if (mv) HIPCHK(h, hipMemcpyAsync(mv, MYF(h), ncell(h), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+int fsg_get_ext_info(fsg_domain* h, uint8_t* iu, uint8_t* iv) {
+  if (!h) return FSG_E_INVALID;
+  if (iu) HIPCHK(h, hipMemcpyAsync(iu, h->info_u, ncell(h), hipMemcpyDeviceToHost, h->stream));
+  if (iv) HIPCHK(h, hipMemcpyAsync(iv, h->info_v, ncell(h), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+int fsg_get_reject_flags(fsg_domain* h, uint8_t* flags) {
+  if (!h || (h->np > 0 && !flags)) return FSG_E_INVALID;
+  if (h->np == 0) return FSG_OK;
+  HIPCHK(h, hipMemcpyAsync(flags, h->reject, (size_t)h->np, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+int fsg_get_stats(fsg_domain* h, fsg_stats* s) {
+  if (!h || !s) return FSG_E_INVALID;
+  if (h->prm.solver == FSG_SOLVER_PLAIN) {
+    *s = h->host_stats;
+    return FSG_OK;
+  }
+  CgScalars hs{};
+  int F = 0;
+  HIPCHK(h, hipMemcpyAsync(&hs, h->sc, sizeof(hs), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(&F, h->n_fluid, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  memset(s, 0, sizeof(*s));
+  s->n_fluid = F;
+  s->cg_iterations = hs.iter;
+  s->cg_error = hs.err;
+  s->refine_sweeps = hs.refine_sweeps;
+  s->total_unknown_iterations = (int64_t)hs.total_unknown_iters + h->host_stats.total_unknown_iterations;
+  s->total_solves = hs.total_solves + h->host_stats.total_solves;
+  return FSG_OK;
+}
+
+int fsg_reset_stats(fsg_domain* h) {
+  if (!h) return FSG_E_INVALID;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  CgScalars hs{};
+  HIPCHK(h, hipMemcpyAsync(&hs, h->sc, sizeof(hs), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  hs.total_unknown_iters = 0;
+  hs.total_solves = 0;
+  HIPCHK(h, hipMemcpyAsync(h->sc, &hs, sizeof(hs), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->host_stats.total_unknown_iterations = 0;
+  h->host_stats.total_solves = 0;
+  return FSG_OK;
+}
+
+int fsg_set_timing(fsg_domain* h, int32_t enable) {
+  if (!h) return FSG_E_INVALID;
+  if (enable < 0 || enable > 2) return fail(h, FSG_E_INVALID, "fsg_set_timing: mode 0, 1 or 2");
+  h->timer.on = enable;
+  return FSG_OK;
+}
+
+int fsg_get_timing(fsg_domain* h, int32_t cap, const char** names, double* ms, int64_t* launches) {
+  if (!h) return FSG_E_INVALID;
+  h->timer.collect();
+  h->timer.names.clear();
+  for (auto& kv : h->timer.acc) h->timer.names.push_back(kv.first);
+  int q = 0;
+  for (auto& kv : h->timer.acc) {
+    if (q >= cap) break;
+    if (names) names[q] = h->timer.names[(size_t)q].c_str();
+    if (ms) ms[q] = kv.second.first;
+    if (launches) launches[q] = kv.second.second;
+    ++q;
+  }
+  return (int)h->timer.acc.size();
+}
+
+int fsg_reset_timing(fsg_domain* h) {
+  if (!h) return FSG_E_INVALID;
+  h->timer.collect();
+  h->timer.acc.clear();
+  return FSG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,151 @@
+// fsg_device.h -- device helpers shared by the gfx950 kernels.
+//
+// Every helper rounds exactly like the reference's float build (no contraction:
+// the library is compiled with -ffp-contract=off and correctly rounded fp32
+// division), so per-element results can be compared bit-for-bit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fsg {
+
+constexpr uint8_t kLiquid = 0, kAir = 1, kSolid = 2;  // include/MacGrid.h:14-17
+
+// Grid geometry passed by value to kernels.
+struct Geom {
+  int nx, ny;
+  float dx, dy;          // MacGrid spacing (length / size in float, src/MacGrid.cpp:8)
+  float inv_dx, inv_dy;  // exact reciprocals when the spacing is a power of two
+  int pow2_x, pow2_y;    // 1: x / dx == x * inv_dx exactly
+  float len_x, len_y;    // GridInterface::lengthX() = size * delta (include/Grid.h:54-55)
+};
+
+// MathDefinitions.h:16-19: CLAMP is a MyFloat function, integer clamps go through
+// float.  Exact for |v| < 2^24, which covers every index here.
+__device__ __forceinline__ int clampi(int v, int lo, int hi) {
+  float d = (float)v;
+  const float t = d < (float)lo ? (float)lo : d;
+  return (int)(t > (float)hi ? (float)hi : t);
+}
+
+__device__ __forceinline__ float div_x(float a, const Geom& g) {
+  return g.pow2_x ? a * g.inv_dx : a / g.dx;
+}
+__device__ __forceinline__ float div_y(float a, const Geom& g) {
+  return g.pow2_y ? a * g.inv_dy : a / g.dy;
+}
+
+// Grid<T>::valueInterpolated (include/Grid.h:117-144) on a row-major grid:
+// truncating casts, fraction before clamping, i+1 clamped after i, x then y lerp.
+// xs, ys are the coordinates already divided by the spacing.
+__device__ __forceinline__ float interp_scaled(const float* __restrict__ g, int nx, int ny,
+                                               float xs, float ys) {
+  int i = (int)xs;
+  int j = (int)ys;
+  const float fi = xs - (float)i;
+  const float fj = ys - (float)j;
+  i = clampi(i, 0, nx - 1);
+  j = clampi(j, 0, ny - 1);
+  const int i1 = clampi(i + 1, 0, nx - 1);
+  const int j1 = clampi(j + 1, 0, ny - 1);
+  const float v00 = g[i + j * nx];
+  const float v10 = g[i1 + j * nx];
+  const float v01 = g[i + j1 * nx];
+  const float v11 = g[i1 + j1 * nx];
+  const float v0 = (1.0f - fi) * v00 + fi * v10;
+  const float v1 = (1.0f - fi) * v01 + fi * v11;
+  return (1.0f - fj) * v0 + fj * v1;
+}
+
+// MacGrid::velXInterpolated: sample u at (x, y - dy*0.5); the offset is formed in
+// double in the reference and rounded once to float, which equals the float
+// subtraction of the exact half spacing (a single correctly rounded operation).
+__device__ __forceinline__ float vel_x_at(const float* __restrict__ u, const Geom& g, float x,
+                                          float y) {
+  const float yy = y - g.dy * 0.5f;
+  return interp_scaled(u, g.nx, g.ny, div_x(x, g), div_y(yy, g));
+}
+// MacGrid::velYInterpolated: sample v at (x - dx*0.5, y).
+__device__ __forceinline__ float vel_y_at(const float* __restrict__ v, const Geom& g, float x,
+                                          float y) {
+  const float xx = x - g.dx * 0.5f;
+  return interp_scaled(v, g.nx, g.ny, div_x(xx, g), div_y(y, g));
+}
+
+// Grid<T>::addToValueInterpolated (include/Grid.h:152-184) with hardware fp32
+// atomics: i+1 taken before the clamp, spread in y then x.
+__device__ __forceinline__ void splat_scaled(float* g, int nx, int ny, float xs, float ys,
+                                             float value) {
+  int i = (int)xs;
+  int j = (int)ys;
+  int i1 = i + 1, j1 = j + 1;
+  const float fi = xs - (float)i;
+  const float fj = ys - (float)j;
+  i = clampi(i, 0, nx - 1);
+  j = clampi(j, 0, ny - 1);
+  i1 = clampi(i1, 0, nx - 1);
+  j1 = clampi(j1, 0, ny - 1);
+  const float val0 = (1.0f - fj) * value;
+  const float val1 = fj * value;
+  unsafeAtomicAdd(&g[i + j * nx], (1.0f - fi) * val0);
+  unsafeAtomicAdd(&g[i1 + j * nx], fi * val0);
+  unsafeAtomicAdd(&g[i + j1 * nx], (1.0f - fi) * val1);
+  unsafeAtomicAdd(&g[i1 + j1 * nx], fi * val1);
+}
+
+// Velocity field sampler used by the ODE steps (FluidSolver::getAdvectedPosition,
+// src/FluidSolver.cpp:793-814).
+struct VelField {
+  const float* __restrict__ u;
+  const float* __restrict__ v;
+  Geom g;
+  __device__ __forceinline__ void at(float x, float y, float& ox, float& oy) const {
+    ox = vel_x_at(u, g, x, y);
+    oy = vel_y_at(v, g, x, y);
+  }
+};
+
+constexpr int kRK3 = 0, kEuler = 1, kRK2 = 2;
+
+// Displacement of one ODE step in Vec2 float ops (include/FluidSolver.h:120-141).
+//  RK3 (include/OdeSolver.h:102-113): ((k1*2 + k2*3 + k3*4) * h * 1) / 9 with k2 at
+//      x + ((k1*h)*1)/2 and k3 at x + ((k2*h)*3)/4.
+//  Euler (include/OdeSolver.h:78-86): f(x + h) * h, the scalar added to both axes.
+//  RK2 (new, explicit midpoint): k2 at x + (k1*h)*0.5, displacement k2*h.
+__device__ __forceinline__ void ode_disp(const VelField& f, int integrator, float x, float y,
+                                         float h, float& ddx, float& ddy) {
+  if (integrator == kEuler) {
+    float kx, ky;
+    f.at(x + h, y + h, kx, ky);
+    ddx = kx * h;
+    ddy = ky * h;
+    return;
+  }
+  float k1x, k1y;
+  f.at(x, y, k1x, k1y);
+  if (integrator == kRK2) {
+    float k2x, k2y;
+    f.at(x + k1x * h * 0.5f, y + k1y * h * 0.5f, k2x, k2y);
+    ddx = k2x * h;
+    ddy = k2y * h;
+    return;
+  }
+  float k2x, k2y, k3x, k3y;
+  f.at(x + k1x * h * 1.0f / 2.0f, y + k1y * h * 1.0f / 2.0f, k2x, k2y);
+  f.at(x + k2x * h * 3.0f / 4.0f, y + k2y * h * 3.0f / 4.0f, k3x, k3y);
+  ddx = (k1x * 2.0f + k2x * 3.0f + k3x * 4.0f) * h * 1.0f / 9.0f;
+  ddy = (k1y * 2.0f + k2y * 3.0f + k3y * 4.0f) * h * 1.0f / 9.0f;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).  Map
+// dispatch order to logical tiles so that each XCD gets one contiguous band of
+// tiles (rows of the grid / runs of particles) and its L2 sees the halo rows.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  constexpr int kXcd = 8;
+  const int per = nb / kXcd;          // full tiles per XCD
+  const int full = per * kXcd;
+  if (b >= full) return b;            // the ragged tail stays in place
+  return (b % kXcd) * per + b / kXcd;
+}
+
+}  // namespace fsg

@@ -1,0 +1,122 @@
+// fsg_kernels.h -- host-side launchers of the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fsg_device.h"
+
+namespace fsg {
+
+// k_face_update phase flags (applied in this order)
+enum { FU_P2G = 1, FU_PREV = 2, FU_GRAVITY = 4, FU_DIRICHLET = 8, FU_EXTINIT = 16 };
+
+void launch_cell_init(hipStream_t s, const Geom& g, uint8_t* label, const uint8_t* solid,
+                      float4* acc, int do_labels, int do_acc);
+void launch_particle_scatter(hipStream_t s, const Geom& g, const float4* pt, int64_t np,
+                             uint8_t* label, float4* acc, int do_classify, int do_p2g);
+void launch_face_update(hipStream_t s, const Geom& g, int flags, float* u, float* v,
+                        const float4* acc, float* prev_u, float* prev_v, const uint8_t* label,
+                        float gx_dt, float gy_dt, float* ext_u, float* ext_v, uint8_t* mxf,
+                        uint8_t* mxb, uint8_t* myf, uint8_t* myb, uint8_t* info_u,
+                        uint8_t* info_v);
+void launch_extend_sweep(hipStream_t s, const Geom& g, const uint8_t* label, float* ub,
+                         float* vb, const uint8_t* mxf, uint8_t* mxb, const uint8_t* myf,
+                         uint8_t* myb, uint8_t* info_u, uint8_t* info_v, int layer);
+void launch_classify_levelset(hipStream_t s, const Geom& g, const float* phi,
+                              const uint8_t* solid, uint8_t* label);
+void launch_pressure_update(hipStream_t s, const Geom& g, const int* n_fluid,
+                            const uint8_t* label, const float* xg, float* au, float* av,
+                            float* bu, float* bv, float dt_over_density, int do_dirichlet,
+                            int do_diff, const float* prev_u, const float* prev_v, float* diff_u,
+                            float* diff_v);
+void launch_diff(hipStream_t s, const Geom& g, const float* u, const float* v, const float* pu,
+                 const float* pv, float* du, float* dv);
+void launch_particle_gather(hipStream_t s, const Geom& g, float4* pt, int64_t np, const float* u,
+                            const float* v, const float* du, const float* dv, float pic_ratio,
+                            int g2p, int move, float dt, const uint8_t* label, uint8_t* reject);
+void launch_sl_scatter(hipStream_t s, const Geom& g, const uint8_t* label, const float* uf,
+                       const float* vf, float* ub, float* vb, int integrator, float dt);
+void launch_sl_gather(hipStream_t s, const Geom& g, const uint8_t* label, const float* uf,
+                      const float* vf, float* ub, float* vb, int integrator, float dt);
+void launch_advect_grid(hipStream_t s, const Geom& g, float4* pt, int64_t np, const float* uf,
+                        const float* vf, int integrator, float dt);
+void launch_fill(hipStream_t s, float* a, int64_t n, float val);
+
+// ---------------------------------------------------------------- pressure
+constexpr int kScanChunk = 2048;  // cells per block of the liquid count / compaction
+
+// Device-resident scalars of one solve (written by the last block of each
+// reduction kernel, read by every later kernel of the solve).
+struct CgScalars {
+  double rhs_norm2;      // |b|^2 (after the optional mean removal)
+  double threshold;      // max(tol^2 |b|^2, min normal)
+  double abs_new;        // r.z of the current direction
+  double rr;             // |r|^2 after the last update
+  double pap;            // p.Ap of the current direction
+  double mean;           // mean of b (project_mean)
+  float alpha, beta;
+  int iter;              // Eigen's iteration count
+  int spmv;              // SpMV applications
+  int done;              // 1 once the solve has stopped
+  int refine_sweeps;
+  unsigned counter;      // last-block ticket
+  unsigned pad;
+  double err;            // sqrt(|r|^2 / |b|^2)
+  double b64_norm2;      // refined form: |b|^2 in FP64
+  double res64_norm2;    // refined form: |b - A x|^2 in FP64
+  unsigned long long total_unknown_iters;
+  long long total_solves;
+};
+
+struct CgParams {
+  int max_iter;
+  float tol;             // in float: Eigen's RealScalar tolerance
+  float off;             // 1/dx^2 as float (src/FluidSolver.cpp:381-402)
+  float diag_tab[5];     // -n/dx^2 as float for n non-SOLID neighbours (:409-410)
+  float invd_tab[5];     // Eigen's DiagonalPreconditioner: 1/diag, or 1 where diag == 0
+  int project_mean;
+};
+
+void launch_count_liquid(hipStream_t s, const Geom& g, const uint8_t* label, int* counts);
+void launch_scan_counts(hipStream_t s, int nb, const int* counts, int* offsets, int* n_fluid);
+void launch_compact(hipStream_t s, const Geom& g, const uint8_t* label, const int* offsets,
+                    int* fluid_index, int* cell_of, const float* uf, const float* vf, float* b,
+                    uint8_t* info);
+
+// plain form
+void launch_cg_mean(hipStream_t s, int F, const float* b, double* partials, CgScalars* sc);
+void launch_cg_init(hipStream_t s, int F, const CgParams& cp, const int* cell_of,
+                    const uint8_t* info, const float* b, float* r, float* pg, float* xg,
+                    double* partials, CgScalars* sc);
+void launch_cg_iteration(hipStream_t s, int F, int nx, const CgParams& cp, const int* cell_of,
+                         const uint8_t* info, float* r, float* ap, float* pg, float* xg,
+                         double* partials, CgScalars* sc);
+
+// persistent / refined forms
+struct PersistentArgs {
+  int nx;
+  int F_max;
+  const int* n_fluid;
+  const int* cell_of;
+  const uint8_t* info;
+  const float* b;
+  float* r;        // FP32 residual of the inner solve (compact)
+  float* xg;       // solution, grid layout
+  float* zg;       // z = M^-1 r, grid layout
+  float* pg0;      // search directions, grid layout, ping-pong
+  float* pg1;
+  double* x64;     // refined form: FP64 solution (compact)
+  float* e32;      // refined form: FP32 correction (compact)
+  double* partials;
+  unsigned* bar;   // barrier counters
+  CgScalars* sc;
+  CgParams cp;
+  int refined;
+  int max_outer;
+  float inner_tol;
+  double outer_tol;
+};
+int persistent_grid_size(int device);
+hipError_t launch_cg_persistent(hipStream_t s, const PersistentArgs& a, int grid);
+
+}  // namespace fsg

@@ -1,0 +1,526 @@
+// kernels_grid.hip -- gfx950 kernels for every per-step phase except the CG solve.
+//
+// Layout in HBM: face velocities are row-major float32 grids of nx*ny (u(i,j) is
+// the left face of cell (i,j), v(i,j) the bottom face); labels and extension masks
+// are uint8 grids; the P2G accumulators are interleaved (sum, weight) float2 grids;
+// particles are AoS float4 (x, y, vx, vy).  Grid kernels run one thread per cell
+// (both faces of the cell), 256 threads per block over a 1-D tile order remapped
+// so that each XCD owns a contiguous band of rows.
+#include "fsg_device.h"
+#include "fsg_kernels.h"
+
+namespace fsg {
+
+// ---------------------------------------------------------------- cell init
+// MacGrid::clearCellTypeBuffer (src/MacGrid.cpp:32-50) with optional static solids,
+// fused with zeroing the P2G accumulators (src/FluidSolver.cpp:877-886).
+__global__ void __launch_bounds__(256) k_cell_init(Geom g, uint8_t* __restrict__ label,
+                                                   const uint8_t* __restrict__ solid,
+                                                   float4* __restrict__ acc, int do_labels,
+                                                   int do_acc) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int c = t * 256 + threadIdx.x;
+  const int n = g.nx * g.ny;
+  if (c >= n) return;
+  if (do_labels) {
+    const int i = c % g.nx, j = c / g.nx;
+    const bool border = (i == 0 || j == 0 || i == g.nx - 1 || j == g.ny - 1);
+    uint8_t l = border ? kSolid : kAir;
+    if (solid && solid[c]) l = kSolid;
+    label[c] = l;
+  }
+  if (do_acc) acc[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// ------------------------------------------------------- particle scatter pass
+// classify: FluidDomain::classifyCells(MarkerParticleSet&) (src/FluidDomain.cpp:
+// 150-180): cell = (int)((x / lengthX) * sizeX), clamped, := LIQUID unless the
+// cell is SOLID (the reference re-marks border / solid cells afterwards).
+// p2g:      transferVelocityToGridSpread splat (src/FluidSolver.cpp:887-897): u and
+// weight at (x, y - dy/2), v and weight at (x - dx/2, y), on the pool grids whose
+// y spacing is dx (src/FluidSolver.cpp:56-63).  acc = (u sum, u weight, v sum,
+// v weight) per face.
+__global__ void __launch_bounds__(256) k_particle_scatter(Geom g, const float4* __restrict__ pt,
+                                                          int64_t np, uint8_t* __restrict__ label,
+                                                          float* __restrict__ acc, int do_classify,
+                                                          int do_p2g) {
+  const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t p = t * 256 + threadIdx.x;
+  if (p >= np) return;
+  const float4 q = pt[p];
+  if (do_classify) {
+    int x = (int)((q.x / g.len_x) * (float)g.nx);
+    int y = (int)((q.y / g.len_y) * (float)g.ny);
+    x = clampi(x, 0, g.nx - 1);
+    y = clampi(y, 0, g.ny - 1);
+    const int c = x + y * g.nx;
+    if (label[c] != kSolid) label[c] = kLiquid;
+  }
+  if (do_p2g) {
+    // pool spacing: (dx, dx)
+    const float yu = q.y - g.dy * 0.5f;
+    const float xv = q.x - g.dx * 0.5f;
+    const float xs_u = div_x(q.x, g), ys_u = div_x(yu, g);
+    const float xs_v = div_x(xv, g), ys_v = div_x(q.y, g);
+    const int nx = g.nx, ny = g.ny;
+    // u and its weight
+    {
+      int i = (int)xs_u, j = (int)ys_u;
+      int i1 = i + 1, j1 = j + 1;
+      const float fi = xs_u - (float)i, fj = ys_u - (float)j;
+      i = clampi(i, 0, nx - 1); j = clampi(j, 0, ny - 1);
+      i1 = clampi(i1, 0, nx - 1); j1 = clampi(j1, 0, ny - 1);
+      const float a0 = (1.0f - fj) * q.z, a1 = fj * q.z;
+      const float w0 = (1.0f - fj) * 1.0f, w1 = fj * 1.0f;
+      float* b00 = acc + 4 * (i + j * nx);
+      float* b10 = acc + 4 * (i1 + j * nx);
+      float* b01 = acc + 4 * (i + j1 * nx);
+      float* b11 = acc + 4 * (i1 + j1 * nx);
+      unsafeAtomicAdd(b00 + 0, (1.0f - fi) * a0);
+      unsafeAtomicAdd(b10 + 0, fi * a0);
+      unsafeAtomicAdd(b01 + 0, (1.0f - fi) * a1);
+      unsafeAtomicAdd(b11 + 0, fi * a1);
+      unsafeAtomicAdd(b00 + 1, (1.0f - fi) * w0);
+      unsafeAtomicAdd(b10 + 1, fi * w0);
+      unsafeAtomicAdd(b01 + 1, (1.0f - fi) * w1);
+      unsafeAtomicAdd(b11 + 1, fi * w1);
+    }
+    // v and its weight
+    {
+      int i = (int)xs_v, j = (int)ys_v;
+      int i1 = i + 1, j1 = j + 1;
+      const float fi = xs_v - (float)i, fj = ys_v - (float)j;
+      i = clampi(i, 0, nx - 1); j = clampi(j, 0, ny - 1);
+      i1 = clampi(i1, 0, nx - 1); j1 = clampi(j1, 0, ny - 1);
+      const float a0 = (1.0f - fj) * q.w, a1 = fj * q.w;
+      const float w0 = (1.0f - fj) * 1.0f, w1 = fj * 1.0f;
+      float* b00 = acc + 4 * (i + j * nx);
+      float* b10 = acc + 4 * (i1 + j * nx);
+      float* b01 = acc + 4 * (i + j1 * nx);
+      float* b11 = acc + 4 * (i1 + j1 * nx);
+      unsafeAtomicAdd(b00 + 2, (1.0f - fi) * a0);
+      unsafeAtomicAdd(b10 + 2, fi * a0);
+      unsafeAtomicAdd(b01 + 2, (1.0f - fi) * a1);
+      unsafeAtomicAdd(b11 + 2, fi * a1);
+      unsafeAtomicAdd(b00 + 3, (1.0f - fi) * w0);
+      unsafeAtomicAdd(b10 + 3, fi * w0);
+      unsafeAtomicAdd(b01 + 3, (1.0f - fi) * w1);
+      unsafeAtomicAdd(b11 + 3, fi * w1);
+    }
+  }
+}
+
+// -------------------------------------------------------------- face update
+// One thread per cell (its u and v face), applying in order the per-face phases
+// selected by `flags` to the buffer pair (u, v) in place:
+//   FU_P2G      back := sum / weight where weight > 1e-6 (src/FluidSolver.cpp:898-917)
+//   FU_PREV     prev := value (src/MacGrid.cpp:52-56)
+//   FU_GRAVITY  value += a*dt at LIQUID cells (src/FluidSolver.cpp:286-291)
+//   FU_DIRICHLET zero flow into SOLID (src/FluidSolver.cpp:300-320)
+//   FU_EXTINIT  extension init (src/FluidSolver.cpp:490-530): ext := value on valid
+//               faces else 0; masks; value.u := 0 where u- OR v-invalid (the v
+//               branch writes the x buffer, :527)
+// The weight test `w > 0.000001` compares the float weight with a double constant;
+// the float test w > 1e-6f selects the same floats (no float lies in between).
+__global__ void __launch_bounds__(256) k_face_update(
+    Geom g, int flags, float* __restrict__ u, float* __restrict__ v,
+    const float4* __restrict__ acc, float* __restrict__ prev_u, float* __restrict__ prev_v,
+    const uint8_t* __restrict__ label, float gx_dt, float gy_dt, float* __restrict__ ext_u,
+    float* __restrict__ ext_v, uint8_t* __restrict__ mxf, uint8_t* __restrict__ mxb,
+    uint8_t* __restrict__ myf, uint8_t* __restrict__ myb, uint8_t* __restrict__ info_u,
+    uint8_t* __restrict__ info_v) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int c = t * 256 + threadIdx.x;
+  if (c >= g.nx * g.ny) return;
+  const int i = c % g.nx, j = c / g.nx;
+  float uu = u[c], vv = v[c];
+  if (flags & FU_P2G) {
+    const float4 a = acc[c];
+    if (a.y > 0.000001f) uu = a.x / a.y;
+    if (a.w > 0.000001f) vv = a.z / a.w;
+  }
+  if (flags & FU_PREV) {
+    prev_u[c] = uu;
+    prev_v[c] = vv;
+  }
+  const uint8_t L = label[c];
+  const uint8_t Lim = label[clampi(i - 1, 0, g.nx - 1) + j * g.nx];
+  const uint8_t Ljm = label[i + clampi(j - 1, 0, g.ny - 1) * g.nx];
+  if ((flags & FU_GRAVITY) && L == kLiquid) {
+    uu = uu + gx_dt;
+    vv = vv + gy_dt;
+  }
+  if (flags & FU_DIRICHLET) {
+    if ((Lim == kSolid && uu < 0.f) || (L == kSolid && uu > 0.f)) uu = 0.f;
+    if ((Ljm == kSolid && vv < 0.f) || (L == kSolid && vv > 0.f)) vv = 0.f;
+  }
+  if (flags & FU_EXTINIT) {
+    const bool vu = (L == kLiquid || Lim == kLiquid);
+    const bool vv_ok = (L == kLiquid || Ljm == kLiquid);
+    ext_u[c] = vu ? uu : 0.f;
+    ext_v[c] = vv_ok ? vv : 0.f;
+    mxf[c] = vu; mxb[c] = vu;
+    myf[c] = vv_ok; myb[c] = vv_ok;
+    if (info_u) info_u[c] = vu ? 0 : 0xF0;
+    if (info_v) info_v[c] = vv_ok ? 0 : 0xF0;
+    if (!vu || !vv_ok) uu = 0.f;
+  }
+  u[c] = uu;
+  v[c] = vv;
+}
+
+// ------------------------------------------------------------ extension sweep
+// One sweep of extendVelocityIndividual (src/FluidSolver.cpp:533-617): an invalid
+// face whose two cells are not SOLID takes the mean of the back-buffer values of
+// its valid neighbours, summed (i-1,j), (i,j-1), (i,j+1), (i+1,j), and is marked
+// valid in the back mask.  Reads touch only front-valid faces and writes only
+// front-invalid ones, so the sweep is race-free in place.  The interior guard only
+// matters for labels whose border is not SOLID (the reference would assert).
+__global__ void __launch_bounds__(256) k_extend_sweep(
+    Geom g, const uint8_t* __restrict__ label, float* __restrict__ ub, float* __restrict__ vb,
+    const uint8_t* __restrict__ mxf, uint8_t* __restrict__ mxb, const uint8_t* __restrict__ myf,
+    uint8_t* __restrict__ myb, uint8_t* __restrict__ info_u, uint8_t* __restrict__ info_v,
+    int layer) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int c = t * 256 + threadIdx.x;
+  const int nx = g.nx;
+  if (c >= nx * g.ny) return;
+  const int i = c % nx, j = c / nx;
+  if (i < 1 || j < 1 || i > nx - 2 || j > g.ny - 2) return;
+  const uint8_t L = label[c];
+  if (L == kSolid) return;
+  if (mxf[c] == 0 && label[c - 1] != kSolid) {
+    float acc = 0.f;
+    int n = 0, bits = 0;
+    if (mxf[c - 1] == 1) { acc += ub[c - 1]; n++; bits |= 1; }
+    if (mxf[c - nx] == 1) { acc += ub[c - nx]; n++; bits |= 2; }
+    if (mxf[c + nx] == 1) { acc += ub[c + nx]; n++; bits |= 4; }
+    if (mxf[c + 1] == 1) { acc += ub[c + 1]; n++; bits |= 8; }
+    if (n > 0) {
+      acc /= (float)n;
+      ub[c] = acc;
+      mxb[c] = 1;
+      if (info_u) info_u[c] = (uint8_t)((layer << 4) | bits);
+    }
+  }
+  if (myf[c] == 0 && label[c - nx] != kSolid) {
+    float acc = 0.f;
+    int n = 0, bits = 0;
+    if (myf[c - 1] == 1) { acc += vb[c - 1]; n++; bits |= 1; }
+    if (myf[c - nx] == 1) { acc += vb[c - nx]; n++; bits |= 2; }
+    if (myf[c + nx] == 1) { acc += vb[c + nx]; n++; bits |= 4; }
+    if (myf[c + 1] == 1) { acc += vb[c + 1]; n++; bits |= 8; }
+    if (n > 0) {
+      acc /= (float)n;
+      vb[c] = acc;
+      myb[c] = 1;
+      if (info_v) info_v[c] = (uint8_t)((layer << 4) | bits);
+    }
+  }
+}
+
+// -------------------------------------------------------- level-set classify
+// FluidDomain::classifyCells(LevelSet&) (src/FluidDomain.cpp:129-148).
+__global__ void __launch_bounds__(256) k_classify_levelset(Geom g, const float* __restrict__ phi,
+                                                           const uint8_t* __restrict__ solid,
+                                                           uint8_t* __restrict__ label) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int c = t * 256 + threadIdx.x;
+  if (c >= g.nx * g.ny) return;
+  const int i = c % g.nx, j = c / g.nx;
+  uint8_t l = phi[c] < 0.f ? kLiquid : kAir;
+  const bool border = (i == 0 || j == 0 || i == g.nx - 1 || j == g.ny - 1);
+  if (border || (solid && solid[c])) l = kSolid;
+  label[c] = l;
+}
+
+// ------------------------------------------------------------ pressure update
+// pressureSolve's gradient update (src/FluidSolver.cpp:427-482) fused with the
+// following enforceDirichlet (:240) and updateVelocityDiffBuffer (src/MacGrid.cpp:
+// 58-70).  A = front (extended) velocity, B = back; the host swaps afterwards.
+//  F > 0: B := A - dt/density * (p - p_left) / dx at faces where (i,j), (i-1,j)
+//         or (i,j-1) is LIQUID, p = x for LIQUID cells and 0 otherwise; other
+//         faces keep B.  Then Dirichlet on B (the front after the swap).
+//  F = 0: the reference returns before solving and does not swap (:347-350); the
+//         same state is produced by exchanging A and B here (B := Dirichlet(A),
+//         A := B) ahead of the host's swap.
+__global__ void __launch_bounds__(256) k_pressure_update(
+    Geom g, const int* __restrict__ n_fluid, const uint8_t* __restrict__ label,
+    const float* __restrict__ xg, float* __restrict__ au, float* __restrict__ av,
+    float* __restrict__ bu, float* __restrict__ bv, float dt_over_density, int do_dirichlet,
+    int do_diff, const float* __restrict__ prev_u, const float* __restrict__ prev_v,
+    float* __restrict__ diff_u, float* __restrict__ diff_v) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int c = t * 256 + threadIdx.x;
+  if (c >= g.nx * g.ny) return;
+  const int i = c % g.nx, j = c / g.nx;
+  const int cim = clampi(i - 1, 0, g.nx - 1) + j * g.nx;
+  const int cjm = i + clampi(j - 1, 0, g.ny - 1) * g.nx;
+  const uint8_t L = label[c], Lim = label[cim], Ljm = label[cjm];
+  const float a_u = au[c], a_v = av[c];
+  float nu, nv;
+  if (*n_fluid > 0) {
+    nu = bu[c];
+    nv = bv[c];
+    if (L == kLiquid || Lim == kLiquid || Ljm == kLiquid) {
+      // + k * (...) with k = 0 (src/FluidSolver.cpp:443-460) kept as + 0
+      const float p = L == kLiquid ? xg[c] + 0.0f : 0.f;
+      const float p_im = Lim == kLiquid ? xg[cim] + 0.0f : 0.f;
+      const float p_jm = Ljm == kLiquid ? xg[cjm] + 0.0f : 0.f;
+      const float pdx = p - p_im;
+      const float pdy = p - p_jm;
+      nu = a_u - dt_over_density * pdx / g.dx;
+      nv = a_v - dt_over_density * pdy / g.dy;
+    }
+  } else {
+    nu = a_u;
+    nv = a_v;
+    au[c] = bu[c];
+    av[c] = bv[c];
+  }
+  if (do_dirichlet) {
+    if ((Lim == kSolid && nu < 0.f) || (L == kSolid && nu > 0.f)) nu = 0.f;
+    if ((Ljm == kSolid && nv < 0.f) || (L == kSolid && nv > 0.f)) nv = 0.f;
+  }
+  bu[c] = nu;
+  bv[c] = nv;
+  if (do_diff) {
+    diff_u[c] = nu - prev_u[c];
+    diff_v[c] = nv - prev_v[c];
+  }
+}
+
+// ------------------------------------------------------------ diff (alone)
+__global__ void __launch_bounds__(256) k_diff(Geom g, const float* __restrict__ u,
+                                              const float* __restrict__ v,
+                                              const float* __restrict__ pu,
+                                              const float* __restrict__ pv,
+                                              float* __restrict__ du, float* __restrict__ dv) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int c = t * 256 + threadIdx.x;
+  if (c >= g.nx * g.ny) return;
+  du[c] = u[c] - pu[c];
+  dv[c] = v[c] - pv[c];
+}
+
+// ------------------------------------------------------ particle gather pass
+// G2P (src/FluidSolver.cpp:921-963) + particle motion (src/MarkerParticleSet.cpp:
+// 40-62).  g2p: 0 none, 1 PIC, 2 FLIP, 3 PICFLIP; move: 0 none, 1 advect,
+// 2 advect and step back out of SOLID cells, tested at (int)(pos / delta).
+__global__ void __launch_bounds__(256) k_particle_gather(
+    Geom g, float4* __restrict__ pt, int64_t np, const float* __restrict__ u,
+    const float* __restrict__ v, const float* __restrict__ du, const float* __restrict__ dv,
+    float pic_ratio, int g2p, int move, float dt, const uint8_t* __restrict__ label,
+    uint8_t* __restrict__ reject) {
+  const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t p = t * 256 + threadIdx.x;
+  if (p >= np) return;
+  float4 q = pt[p];
+  if (g2p) {
+    const float xs_u = div_x(q.x, g), ys_u = div_y(q.y - g.dy * 0.5f, g);
+    const float xs_v = div_x(q.x - g.dx * 0.5f, g), ys_v = div_y(q.y, g);
+    if (g2p == 1) {
+      q.z = interp_scaled(u, g.nx, g.ny, xs_u, ys_u);
+      q.w = interp_scaled(v, g.nx, g.ny, xs_v, ys_v);
+    } else if (g2p == 2) {
+      q.z = q.z + interp_scaled(du, g.nx, g.ny, xs_u, ys_u);
+      q.w = q.w + interp_scaled(dv, g.nx, g.ny, xs_v, ys_v);
+    } else {
+      const float pu = interp_scaled(u, g.nx, g.ny, xs_u, ys_u);
+      const float pv = interp_scaled(v, g.nx, g.ny, xs_v, ys_v);
+      const float fu = q.z + interp_scaled(du, g.nx, g.ny, xs_u, ys_u);
+      const float fv = q.w + interp_scaled(dv, g.nx, g.ny, xs_v, ys_v);
+      q.z = pu * pic_ratio + fu * (1.0f - pic_ratio);
+      q.w = pv * pic_ratio + fv * (1.0f - pic_ratio);
+    }
+  }
+  if (move) {
+    q.x += q.z * dt;
+    q.y += q.w * dt;
+    uint8_t flag = 0;
+    if (move == 2) {
+      int x = (int)div_x(q.x, g);
+      int y = (int)div_y(q.y, g);
+      x = clampi(x, 0, g.nx - 1);
+      y = clampi(y, 0, g.ny - 1);
+      if (label[x + y * g.nx] == kSolid) {
+        q.x += q.z * -dt;
+        q.y += q.w * -dt;
+        flag = 1;
+      }
+    }
+    if (reject) reject[p] = flag;
+  }
+  pt[p] = q;
+}
+
+// ---------------------------------------------------- semi-Lagrangian advection
+// Reference scatter form, advectVelocitySemiLagrangian (src/FluidSolver.cpp:
+// 709-772): for each face next to LIQUID, backtrace its position by -dt through the
+// front field, sample the face's own velocity at its own position and splat it
+// into the back buffer (zeroed beforehand) at the backtraced point.  No swap.
+__global__ void __launch_bounds__(256) k_sl_scatter(Geom g, const uint8_t* __restrict__ label,
+                                                    const float* __restrict__ uf,
+                                                    const float* __restrict__ vf,
+                                                    float* __restrict__ ub, float* __restrict__ vb,
+                                                    int integrator, float dt) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int c = t * 256 + threadIdx.x;
+  if (c >= g.nx * g.ny) return;
+  const int i = c % g.nx, j = c / g.nx;
+  const uint8_t L = label[c];
+  const uint8_t Lim = label[clampi(i - 1, 0, g.nx - 1) + j * g.nx];
+  const uint8_t Ljm = label[i + clampi(j - 1, 0, g.ny - 1) * g.nx];
+  const VelField f{uf, vf, g};
+  if (L == kLiquid || Lim == kLiquid) {
+    const float xp = (float)i * g.dx;
+    const float yp = ((float)j + 0.5f) * g.dy;
+    float ddx, ddy;
+    ode_disp(f, integrator, xp, yp, -dt, ddx, ddy);
+    const float xb = xp + ddx, yb = yp + ddy;
+    const float val = vel_x_at(uf, g, xp, yp);
+    splat_scaled(ub, g.nx, g.ny, div_x(xb, g), div_y(yb - g.dy * 0.5f, g), val);
+  }
+  if (L == kLiquid || Ljm == kLiquid) {
+    const float xp = ((float)i + 0.5f) * g.dx;
+    const float yp = (float)j * g.dy;
+    float ddx, ddy;
+    ode_disp(f, integrator, xp, yp, -dt, ddx, ddy);
+    const float xb = xp + ddx, yb = yp + ddy;
+    const float val = vel_y_at(vf, g, xp, yp);
+    splat_scaled(vb, g.nx, g.ny, div_x(xb - g.dx * 0.5f, g), div_y(yb, g), val);
+  }
+}
+
+// Gather form (new): back := front component sampled at the backtraced face
+// position for faces next to LIQUID, back := front elsewhere; the host swaps.
+__global__ void __launch_bounds__(256) k_sl_gather(Geom g, const uint8_t* __restrict__ label,
+                                                   const float* __restrict__ uf,
+                                                   const float* __restrict__ vf,
+                                                   float* __restrict__ ub, float* __restrict__ vb,
+                                                   int integrator, float dt) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int c = t * 256 + threadIdx.x;
+  if (c >= g.nx * g.ny) return;
+  const int i = c % g.nx, j = c / g.nx;
+  const uint8_t L = label[c];
+  const uint8_t Lim = label[clampi(i - 1, 0, g.nx - 1) + j * g.nx];
+  const uint8_t Ljm = label[i + clampi(j - 1, 0, g.ny - 1) * g.nx];
+  const VelField f{uf, vf, g};
+  float nu = uf[c], nv = vf[c];
+  if (L == kLiquid || Lim == kLiquid) {
+    const float xp = (float)i * g.dx;
+    const float yp = ((float)j + 0.5f) * g.dy;
+    float ddx, ddy;
+    ode_disp(f, integrator, xp, yp, -dt, ddx, ddy);
+    nu = vel_x_at(uf, g, xp + ddx, yp + ddy);
+  }
+  if (L == kLiquid || Ljm == kLiquid) {
+    const float xp = ((float)i + 0.5f) * g.dx;
+    const float yp = (float)j * g.dy;
+    float ddx, ddy;
+    ode_disp(f, integrator, xp, yp, -dt, ddx, ddy);
+    nv = vel_y_at(vf, g, xp + ddx, yp + ddy);
+  }
+  ub[c] = nu;
+  vb[c] = nv;
+}
+
+// FluidSolver::advectParticlesWithGrid (src/FluidSolver.cpp:774-791).
+__global__ void __launch_bounds__(256) k_advect_grid(Geom g, float4* __restrict__ pt, int64_t np,
+                                                     const float* __restrict__ uf,
+                                                     const float* __restrict__ vf,
+                                                     int integrator, float dt) {
+  const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t p = t * 256 + threadIdx.x;
+  if (p >= np) return;
+  float4 q = pt[p];
+  const VelField f{uf, vf, g};
+  float ddx, ddy;
+  ode_disp(f, integrator, q.x, q.y, dt, ddx, ddy);
+  q.x = q.x + ddx;
+  q.y = q.y + ddy;
+  pt[p] = q;
+}
+
+// Plain fill used instead of hipMemsetAsync (stays valid inside captured graphs).
+__global__ void __launch_bounds__(256) k_fill_f32(float* __restrict__ a, int64_t n, float val) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) a[i] = val;
+}
+
+// ---------------------------------------------------------------- launchers
+static inline int nblocks(int64_t n) { return (int)((n + 255) / 256); }
+
+void launch_cell_init(hipStream_t s, const Geom& g, uint8_t* label, const uint8_t* solid,
+                      float4* acc, int do_labels, int do_acc) {
+  k_cell_init<<<nblocks((int64_t)g.nx * g.ny), 256, 0, s>>>(g, label, solid, acc, do_labels,
+                                                             do_acc);
+}
+void launch_particle_scatter(hipStream_t s, const Geom& g, const float4* pt, int64_t np,
+                             uint8_t* label, float4* acc, int do_classify, int do_p2g) {
+  if (np == 0) return;
+  k_particle_scatter<<<nblocks(np), 256, 0, s>>>(g, pt, np, label, (float*)acc, do_classify,
+                                                 do_p2g);
+}
+void launch_face_update(hipStream_t s, const Geom& g, int flags, float* u, float* v,
+                        const float4* acc, float* prev_u, float* prev_v, const uint8_t* label,
+                        float gx_dt, float gy_dt, float* ext_u, float* ext_v, uint8_t* mxf,
+                        uint8_t* mxb, uint8_t* myf, uint8_t* myb, uint8_t* info_u,
+                        uint8_t* info_v) {
+  k_face_update<<<nblocks((int64_t)g.nx * g.ny), 256, 0, s>>>(
+      g, flags, u, v, acc, prev_u, prev_v, label, gx_dt, gy_dt, ext_u, ext_v, mxf, mxb, myf,
+      myb, info_u, info_v);
+}
+void launch_extend_sweep(hipStream_t s, const Geom& g, const uint8_t* label, float* ub,
+                         float* vb, const uint8_t* mxf, uint8_t* mxb, const uint8_t* myf,
+                         uint8_t* myb, uint8_t* info_u, uint8_t* info_v, int layer) {
+  k_extend_sweep<<<nblocks((int64_t)g.nx * g.ny), 256, 0, s>>>(g, label, ub, vb, mxf, mxb, myf,
+                                                                myb, info_u, info_v, layer);
+}
+void launch_classify_levelset(hipStream_t s, const Geom& g, const float* phi,
+                              const uint8_t* solid, uint8_t* label) {
+  k_classify_levelset<<<nblocks((int64_t)g.nx * g.ny), 256, 0, s>>>(g, phi, solid, label);
+}
+void launch_pressure_update(hipStream_t s, const Geom& g, const int* n_fluid,
+                            const uint8_t* label, const float* xg, float* au, float* av,
+                            float* bu, float* bv, float dt_over_density, int do_dirichlet,
+                            int do_diff, const float* prev_u, const float* prev_v, float* diff_u,
+                            float* diff_v) {
+  k_pressure_update<<<nblocks((int64_t)g.nx * g.ny), 256, 0, s>>>(
+      g, n_fluid, label, xg, au, av, bu, bv, dt_over_density, do_dirichlet, do_diff, prev_u,
+      prev_v, diff_u, diff_v);
+}
+void launch_diff(hipStream_t s, const Geom& g, const float* u, const float* v, const float* pu,
+                 const float* pv, float* du, float* dv) {
+  k_diff<<<nblocks((int64_t)g.nx * g.ny), 256, 0, s>>>(g, u, v, pu, pv, du, dv);
+}
+void launch_particle_gather(hipStream_t s, const Geom& g, float4* pt, int64_t np, const float* u,
+                            const float* v, const float* du, const float* dv, float pic_ratio,
+                            int g2p, int move, float dt, const uint8_t* label,
+                            uint8_t* reject) {
+  if (np == 0) return;
+  k_particle_gather<<<nblocks(np), 256, 0, s>>>(g, pt, np, u, v, du, dv, pic_ratio, g2p, move,
+                                                dt, label, reject);
+}
+void launch_sl_scatter(hipStream_t s, const Geom& g, const uint8_t* label, const float* uf,
+                       const float* vf, float* ub, float* vb, int integrator, float dt) {
+  k_sl_scatter<<<nblocks((int64_t)g.nx * g.ny), 256, 0, s>>>(g, label, uf, vf, ub, vb,
+                                                              integrator, dt);
+}
+void launch_sl_gather(hipStream_t s, const Geom& g, const uint8_t* label, const float* uf,
+                      const float* vf, float* ub, float* vb, int integrator, float dt) {
+  k_sl_gather<<<nblocks((int64_t)g.nx * g.ny), 256, 0, s>>>(g, label, uf, vf, ub, vb,
+                                                             integrator, dt);
+}
+void launch_advect_grid(hipStream_t s, const Geom& g, float4* pt, int64_t np, const float* uf,
+                        const float* vf, int integrator, float dt) {
+  if (np == 0) return;
+  k_advect_grid<<<nblocks(np), 256, 0, s>>>(g, pt, np, uf, vf, integrator, dt);
+}
+void launch_fill(hipStream_t s, float* a, int64_t n, float val) {
+  if (n == 0) return;
+  k_fill_f32<<<nblocks(n), 256, 0, s>>>(a, n, val);
+}
+
+}  // namespace fsg

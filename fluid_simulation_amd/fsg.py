@@ -1,0 +1,344 @@
+"""ctypes binding of libfsg.so (include/fsg.h).
+
+``Domain`` mirrors the reference's FluidDomain + FluidSolver pair
+(include/FluidDomain.h:37-76, include/FluidSolver.h:45-54) with the state held on
+the GPU.  There is no CPU fallback: if libfsg.so is missing or cannot be loaded,
+``load()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfsg.so")
+_LIB = None
+_LOCK = threading.Lock()
+
+OK = 0
+LIQUID, AIR, SOLID = 0, 1, 2
+SEMI_LAGRANGIAN, PIC, FLIP, PICFLIP, GRID_ADVECT = 0, 1, 2, 3, 4
+RK3, EULER, RK2 = 0, 1, 2
+SOLVER_PLAIN, SOLVER_PERSISTENT, SOLVER_REFINED = 0, 1, 2
+VEL_FRONT, VEL_BACK, VEL_PREV, VEL_DIFF = 0, 1, 2, 3
+PH = dict(CLASSIFY=1, P2G=2, SAVE_PREV=3, GRAVITY=4, DIRICHLET=5, EXTEND=6,
+          PRESSURE=7, DIFF=8, G2P_PIC=9, G2P_FLIP=10, G2P_PICFLIP=11, ADVECT=12,
+          ADVECT_REJECT=13, SL_SCATTER=14, SL_GATHER=15, ADVECT_GRID=16,
+          CLASSIFY_LEVELSET=17)
+
+# every entry point declared in include/fsg.h
+EXPORTS = [
+    "fsg_version", "fsg_params_default", "fsg_create", "fsg_destroy", "fsg_last_error",
+    "fsg_set_params", "fsg_get_params", "fsg_set_pic_ratio", "fsg_get_geometry",
+    "fsg_synchronize", "fsg_set_particles", "fsg_add_particles", "fsg_num_particles",
+    "fsg_get_particles", "fsg_set_velocity", "fsg_get_velocity", "fsg_swap_velocity",
+    "fsg_set_labels", "fsg_get_labels", "fsg_set_solid_mask", "fsg_set_level_set",
+    "fsg_step", "fsg_run_phase", "fsg_validate_pool", "fsg_get_pressure", "fsg_get_rhs",
+    "fsg_get_fluid_indices", "fsg_get_valid_masks", "fsg_get_ext_info",
+    "fsg_get_reject_flags", "fsg_get_stats", "fsg_reset_stats", "fsg_set_timing",
+    "fsg_get_timing", "fsg_reset_timing",
+]
+
+
+class FsgError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("gravity_x", C.c_float), ("gravity_y", C.c_float), ("cg_tol", C.c_double),
+        ("cg_max_iter", C.c_int32), ("ext_layers", C.c_int32), ("integrator", C.c_int32),
+        ("solver", C.c_int32), ("project_mean", C.c_int32), ("check_every", C.c_int32),
+        ("refine_inner_tol", C.c_double), ("refine_max_outer", C.c_int32),
+        ("record_ext_info", C.c_int32), ("reserved", C.c_int32 * 4),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("n_fluid", C.c_int32), ("cg_iterations", C.c_int32), ("cg_error", C.c_double),
+        ("refine_sweeps", C.c_int32), ("reserved0", C.c_int32),
+        ("total_unknown_iterations", C.c_int64), ("total_solves", C.c_int64),
+        ("reserved", C.c_int64 * 4),
+    ]
+
+
+def load(path: str | None = None):
+    """Load libfsg.so (built by __graft_entry__.build()); raise if unavailable."""
+    global _LIB
+    with _LOCK:
+        if _LIB is not None:
+            return _LIB
+        p = path or LIB_PATH
+        if not os.path.exists(p):
+            raise FsgError(f"libfsg.so not found at {p}: build it first "
+                           "(python -c 'import __graft_entry__ as g; g.build()')")
+        L = C.CDLL(p)
+        H = C.c_void_p
+        P = C.POINTER
+        sig = {
+            "fsg_version": ([], C.c_char_p),
+            "fsg_params_default": ([P(Params)], None),
+            "fsg_create": ([P(H), C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float,
+                            C.c_float], C.c_int),
+            "fsg_destroy": ([H], C.c_int),
+            "fsg_last_error": ([H], C.c_char_p),
+            "fsg_set_params": ([H, P(Params)], C.c_int),
+            "fsg_get_params": ([H, P(Params)], C.c_int),
+            "fsg_set_pic_ratio": ([H, C.c_float], C.c_int),
+            "fsg_get_geometry": ([H, P(C.c_int32), P(C.c_int32), P(C.c_float), P(C.c_float),
+                                  P(C.c_float), P(C.c_float)], C.c_int),
+            "fsg_synchronize": ([H], C.c_int),
+            "fsg_set_particles": ([H, C.c_void_p, C.c_int64], C.c_int),
+            "fsg_add_particles": ([H, C.c_void_p, C.c_int64], C.c_int),
+            "fsg_num_particles": ([H, P(C.c_int64)], C.c_int),
+            "fsg_get_particles": ([H, C.c_void_p, C.c_int64], C.c_int),
+            "fsg_set_velocity": ([H, C.c_int32, C.c_void_p, C.c_void_p], C.c_int),
+            "fsg_get_velocity": ([H, C.c_int32, C.c_void_p, C.c_void_p], C.c_int),
+            "fsg_swap_velocity": ([H], C.c_int),
+            "fsg_set_labels": ([H, C.c_void_p], C.c_int),
+            "fsg_get_labels": ([H, C.c_void_p], C.c_int),
+            "fsg_set_solid_mask": ([H, C.c_void_p], C.c_int),
+            "fsg_set_level_set": ([H, C.c_void_p], C.c_int),
+            "fsg_step": ([H, C.c_int32, C.c_float], C.c_int),
+            "fsg_run_phase": ([H, C.c_int32, C.c_float], C.c_int),
+            "fsg_validate_pool": ([H, C.c_int32, C.c_int32, C.c_float, C.c_float], C.c_int),
+            "fsg_get_pressure": ([H, C.c_void_p], C.c_int),
+            "fsg_get_rhs": ([H, C.c_void_p], C.c_int),
+            "fsg_get_fluid_indices": ([H, C.c_void_p], C.c_int),
+            "fsg_get_valid_masks": ([H, C.c_void_p, C.c_void_p], C.c_int),
+            "fsg_get_ext_info": ([H, C.c_void_p, C.c_void_p], C.c_int),
+            "fsg_get_reject_flags": ([H, C.c_void_p], C.c_int),
+            "fsg_get_stats": ([H, P(Stats)], C.c_int),
+            "fsg_reset_stats": ([H], C.c_int),
+            "fsg_set_timing": ([H, C.c_int32], C.c_int),
+            "fsg_get_timing": ([H, C.c_int32, P(C.c_char_p), P(C.c_double), P(C.c_int64)],
+                               C.c_int),
+            "fsg_reset_timing": ([H], C.c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _LIB = L
+        return L
+
+
+def default_params() -> Params:
+    p = Params()
+    load().fsg_params_default(C.byref(p))
+    return p
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None and a.shape != shape:
+        a = a.reshape(shape)
+    return a
+
+
+class Domain:
+    """FluidDomain + FluidSolver on the GPU (one HIP stream per domain)."""
+
+    def __init__(self, nx: int, ny: int, length_x: float, length_y: float, density: float,
+                 pic_ratio: float):
+        self._L = load()
+        self._h = C.c_void_p()
+        rc = self._L.fsg_create(C.byref(self._h), nx, ny, length_x, length_y, density, pic_ratio)
+        if rc != OK:
+            raise FsgError(f"fsg_create failed ({rc})")
+        self.nx, self.ny = nx, ny
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._L.fsg_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != OK:
+            msg = self._L.fsg_last_error(self._h)
+            raise FsgError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    # -- params ---------------------------------------------------------------
+    @property
+    def params(self) -> Params:
+        p = Params()
+        self._chk(self._L.fsg_get_params(self._h, C.byref(p)), "fsg_get_params")
+        return p
+
+    def set_params(self, **kw):
+        p = self.params
+        for k, v in kw.items():
+            setattr(p, k, v)
+        self._chk(self._L.fsg_set_params(self._h, C.byref(p)), "fsg_set_params")
+
+    def geometry(self):
+        nx, ny = C.c_int32(), C.c_int32()
+        dx, dy, rho, r = C.c_float(), C.c_float(), C.c_float(), C.c_float()
+        self._chk(self._L.fsg_get_geometry(self._h, C.byref(nx), C.byref(ny), C.byref(dx),
+                                           C.byref(dy), C.byref(rho), C.byref(r)),
+                  "fsg_get_geometry")
+        return dict(nx=nx.value, ny=ny.value, dx=dx.value, dy=dy.value, density=rho.value,
+                    pic_ratio=r.value)
+
+    def set_pic_ratio(self, r):
+        self._chk(self._L.fsg_set_pic_ratio(self._h, r), "fsg_set_pic_ratio")
+
+    # -- particles ------------------------------------------------------------
+    def set_particles(self, parts):
+        a = _f32(parts).reshape(-1, 4)
+        self._chk(self._L.fsg_set_particles(self._h, a.ctypes.data, len(a)), "fsg_set_particles")
+
+    def add_particles(self, parts):
+        a = _f32(parts).reshape(-1, 4)
+        self._chk(self._L.fsg_add_particles(self._h, a.ctypes.data, len(a)), "fsg_add_particles")
+
+    def num_particles(self) -> int:
+        n = C.c_int64()
+        self._chk(self._L.fsg_num_particles(self._h, C.byref(n)), "fsg_num_particles")
+        return n.value
+
+    def particles(self) -> np.ndarray:
+        n = self.num_particles()
+        out = np.zeros((n, 4), dtype=np.float32)
+        self._chk(self._L.fsg_get_particles(self._h, out.ctypes.data, n), "fsg_get_particles")
+        return out
+
+    # -- grids ----------------------------------------------------------------
+    def set_velocity(self, u, v, which=VEL_FRONT):
+        sh = (self.ny, self.nx)
+        uu = _f32(u, sh) if u is not None else None
+        vv = _f32(v, sh) if v is not None else None
+        self._chk(self._L.fsg_set_velocity(self._h, which, uu.ctypes.data if uu is not None else None,
+                                           vv.ctypes.data if vv is not None else None),
+                  "fsg_set_velocity")
+
+    def velocity(self, which=VEL_FRONT):
+        u = np.zeros((self.ny, self.nx), np.float32)
+        v = np.zeros((self.ny, self.nx), np.float32)
+        self._chk(self._L.fsg_get_velocity(self._h, which, u.ctypes.data, v.ctypes.data),
+                  "fsg_get_velocity")
+        return u, v
+
+    def swap_velocity(self):
+        self._chk(self._L.fsg_swap_velocity(self._h), "fsg_swap_velocity")
+
+    def set_labels(self, lab):
+        a = np.ascontiguousarray(lab, dtype=np.uint8).reshape(self.ny, self.nx)
+        self._chk(self._L.fsg_set_labels(self._h, a.ctypes.data), "fsg_set_labels")
+
+    def labels(self) -> np.ndarray:
+        a = np.zeros((self.ny, self.nx), np.uint8)
+        self._chk(self._L.fsg_get_labels(self._h, a.ctypes.data), "fsg_get_labels")
+        return a
+
+    def set_solid_mask(self, mask):
+        if mask is None:
+            self._chk(self._L.fsg_set_solid_mask(self._h, None), "fsg_set_solid_mask")
+            return
+        a = np.ascontiguousarray(mask, dtype=np.uint8).reshape(self.ny, self.nx)
+        self._chk(self._L.fsg_set_solid_mask(self._h, a.ctypes.data), "fsg_set_solid_mask")
+
+    def set_level_set(self, phi):
+        a = _f32(phi, (self.ny, self.nx))
+        self._chk(self._L.fsg_set_level_set(self._h, a.ctypes.data), "fsg_set_level_set")
+
+    # -- stepping -------------------------------------------------------------
+    def step(self, scheme: int, dt: float):
+        self._chk(self._L.fsg_step(self._h, scheme, dt), "fsg_step")
+
+    def run_phase(self, phase, dt: float = 0.0):
+        ph = PH[phase] if isinstance(phase, str) else phase
+        self._chk(self._L.fsg_run_phase(self._h, ph, dt), f"fsg_run_phase({phase})")
+
+    def validate_pool(self, nx, ny, dx, dy):
+        return self._L.fsg_validate_pool(self._h, nx, ny, dx, dy)
+
+    def synchronize(self):
+        self._chk(self._L.fsg_synchronize(self._h), "fsg_synchronize")
+
+    # -- inspection -----------------------------------------------------------
+    def _grid(self, fn, dtype):
+        a = np.zeros((self.ny, self.nx), dtype)
+        self._chk(getattr(self._L, fn)(self._h, a.ctypes.data), fn)
+        return a
+
+    def pressure(self):
+        return self._grid("fsg_get_pressure", np.float32)
+
+    def rhs(self):
+        return self._grid("fsg_get_rhs", np.float32)
+
+    def fluid_indices(self):
+        return self._grid("fsg_get_fluid_indices", np.int32)
+
+    def valid_masks(self):
+        mu = np.zeros((self.ny, self.nx), np.uint8)
+        mv = np.zeros((self.ny, self.nx), np.uint8)
+        self._chk(self._L.fsg_get_valid_masks(self._h, mu.ctypes.data, mv.ctypes.data),
+                  "fsg_get_valid_masks")
+        return mu, mv
+
+    def ext_info(self):
+        iu = np.zeros((self.ny, self.nx), np.uint8)
+        iv = np.zeros((self.ny, self.nx), np.uint8)
+        self._chk(self._L.fsg_get_ext_info(self._h, iu.ctypes.data, iv.ctypes.data),
+                  "fsg_get_ext_info")
+        return iu, iv
+
+    def reject_flags(self):
+        n = self.num_particles()
+        a = np.zeros(n, np.uint8)
+        self._chk(self._L.fsg_get_reject_flags(self._h, a.ctypes.data), "fsg_get_reject_flags")
+        return a
+
+    def stats(self) -> Stats:
+        s = Stats()
+        self._chk(self._L.fsg_get_stats(self._h, C.byref(s)), "fsg_get_stats")
+        return s
+
+    def reset_stats(self):
+        self._chk(self._L.fsg_reset_stats(self._h), "fsg_reset_stats")
+
+    def set_timing(self, mode):
+        """0 off, 1 (or True) every kernel, 2 only the pressure-solver kernels."""
+        self._chk(self._L.fsg_set_timing(self._h, int(mode)), "fsg_set_timing")
+
+    def timing(self) -> dict:
+        n = self._L.fsg_get_timing(self._h, 0, None, None, None)
+        if n < 0:
+            self._chk(n, "fsg_get_timing")
+        names = (C.c_char_p * max(n, 1))()
+        ms = (C.c_double * max(n, 1))()
+        cnt = (C.c_int64 * max(n, 1))()
+        self._L.fsg_get_timing(self._h, n, names, ms, cnt)
+        return {names[q].decode(): (ms[q], cnt[q]) for q in range(n)}
+
+    def reset_timing(self):
+        self._chk(self._L.fsg_reset_timing(self._h), "fsg_reset_timing")
+
+    # -- construction from a synthetic scene -----------------------------------
+    @classmethod
+    def from_scene(cls, scene, **params):
+        d = cls(scene.nx, scene.ny, scene.length_x, scene.length_y, scene.density,
+                scene.pic_ratio)
+        if params:
+            d.set_params(**params)
+        if len(scene.particles):
+            d.set_particles(scene.particles)
+        if scene.u is not None:
+            d.set_velocity(scene.u, scene.v)
+        if scene.solid_mask is not None:
+            d.set_solid_mask(scene.solid_mask)
+        if scene.phi is not None:
+            d.set_level_set(scene.phi)
+        return d

@@ -1,0 +1,198 @@
+/*
+ * fsg.h -- C ABI of the MI355X (gfx950) FLIP / semi-Lagrangian / pressure
+ * projection hot path (libfsg.so).
+ *
+ * The reference (kbladin/Fluid_Simulation) has no FFI: its boundary is the C++
+ * class API in include/{FluidDomain,MacGrid,MarkerParticleSet,FluidSolver}.h.
+ * Each entry point below names the reference member it replaces.  The C++ drop-in
+ * classes (fluid_simulation_amd/cpp/include) are written on top of this ABI.
+ *
+ * Conventions
+ *  - Every function returns FSG_OK (0) or a negative FSG_E* status; the message of
+ *    the last failure on a handle is fsg_last_error(h).
+ *  - Grids are row-major float32 / uint8 arrays of nx*ny values, index i + j*nx
+ *    (include/Grid.h:27-31).  u(i,j) is the left face of cell (i,j), v(i,j) the
+ *    bottom face (include/MacGrid.h:66-79).
+ *  - Particles are AoS float32 (x, y, vx, vy) (include/MarkerParticleSet.h:50-56).
+ *  - Host arrays passed in are copied; the handle owns all device memory.
+ *  - One handle owns one HIP stream on the device current at fsg_create.  A handle
+ *    is not thread-safe; distinct handles are independent.
+ *  - Calls are stream-ordered: fsg_step / fsg_run_phase return once the work is
+ *    enqueued (except where a form needs a host decision, see fsg_params.solver);
+ *    the fsg_get_* calls synchronise the stream.
+ */
+#ifndef FSG_H
+#define FSG_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSG_OK 0
+#define FSG_E_INVALID (-1)   /* bad argument */
+#define FSG_E_HIP (-2)       /* HIP runtime failure */
+#define FSG_E_VALIDATE (-3)  /* FluidSolver::validate failure (src/FluidSolver.cpp:101-107) */
+#define FSG_E_NOMEM (-4)
+
+/* Cell labels: enum CellType (include/MacGrid.h:14-17). */
+#define FSG_LIQUID 0
+#define FSG_AIR 1
+#define FSG_SOLID 2
+
+/* Step schemes: FluidSolver::step* (include/FluidSolver.h:51-54) + grid-only. */
+#define FSG_SCHEME_SEMI_LAGRANGIAN 0 /* stepSemiLagrangian, src/FluidSolver.cpp:99-134 */
+#define FSG_SCHEME_PIC 1             /* stepPIC, src/FluidSolver.cpp:136-170 */
+#define FSG_SCHEME_FLIP 2            /* stepFLIP, src/FluidSolver.cpp:173-209 */
+#define FSG_SCHEME_PICFLIP 3         /* stepPICFLIP, src/FluidSolver.cpp:211-251 */
+#define FSG_SCHEME_GRID_ADVECT 4     /* new: labels from the level set, gather
+                                        advection, gravity, Dirichlet, projection */
+
+/* Phases for fsg_run_phase (one reference member each). */
+#define FSG_PH_CLASSIFY 1          /* FluidDomain::classifyCells(MarkerParticleSet&), src/FluidDomain.cpp:150-180 */
+#define FSG_PH_P2G 2               /* transferVelocityToGridSpread, src/FluidSolver.cpp:873-919 */
+#define FSG_PH_SAVE_PREV 3         /* MacGrid::updatePreviousVelocityBuffer, src/MacGrid.cpp:52-56 */
+#define FSG_PH_GRAVITY 4           /* addExternalAcceleration, src/FluidSolver.cpp:276-295 */
+#define FSG_PH_DIRICHLET 5         /* enforceDirichlet, src/FluidSolver.cpp:297-321 */
+#define FSG_PH_EXTEND 6            /* extendVelocityIndividual, src/FluidSolver.cpp:485-622 */
+#define FSG_PH_PRESSURE 7          /* pressureSolve, src/FluidSolver.cpp:323-483 */
+#define FSG_PH_DIFF 8              /* MacGrid::updateVelocityDiffBuffer, src/MacGrid.cpp:58-70 */
+#define FSG_PH_G2P_PIC 9           /* transferVelocityToParticlesPIC, src/FluidSolver.cpp:921-931 */
+#define FSG_PH_G2P_FLIP 10         /* transferVelocityToParticlesFLIP, src/FluidSolver.cpp:933-943 */
+#define FSG_PH_G2P_PICFLIP 11      /* transferVelocityToParticlesPICFLIP, src/FluidSolver.cpp:945-963 */
+#define FSG_PH_ADVECT 12           /* MarkerParticleSet::advect, src/MarkerParticleSet.cpp:40-46 */
+#define FSG_PH_ADVECT_REJECT 13    /* advectAndEnsureOutsideObstacles, src/MarkerParticleSet.cpp:48-62 */
+#define FSG_PH_SL_SCATTER 14       /* advectVelocitySemiLagrangian, src/FluidSolver.cpp:709-772 */
+#define FSG_PH_SL_GATHER 15        /* new: gather-form semi-Lagrangian advection + swap */
+#define FSG_PH_ADVECT_GRID 16      /* advectParticlesWithGrid, src/FluidSolver.cpp:774-791 */
+#define FSG_PH_CLASSIFY_LEVELSET 17 /* FluidDomain::classifyCells(LevelSet&), src/FluidDomain.cpp:129-148 */
+
+/* Integrators for getAdvectedPosition (include/FluidSolver.h:143-145). */
+#define FSG_RK3 0    /* RK3::step, include/OdeSolver.h:102-113 (the reference's choice) */
+#define FSG_EULER 1  /* EulerExplicit::step, include/OdeSolver.h:78-86 */
+#define FSG_RK2 2    /* new: explicit midpoint */
+
+/* Pressure solver forms (all replace Eigen::ConjugateGradient, FluidSolver.h:114). */
+#define FSG_SOLVER_PLAIN 0      /* FP32 Jacobi-PCG, 3 kernels per iteration, host loop */
+#define FSG_SOLVER_PERSISTENT 1 /* FP32 Jacobi-PCG in one persistent cooperative kernel */
+#define FSG_SOLVER_REFINED 2    /* FP32 PCG inside FP64 iterative refinement, FP64
+                                   residual and reductions, persistent kernel */
+
+/* Velocity buffers for fsg_{get,set}_velocity (include/MacGrid.h:162-176). */
+#define FSG_VEL_FRONT 0
+#define FSG_VEL_BACK 1
+#define FSG_VEL_PREV 2
+#define FSG_VEL_DIFF 3
+
+typedef struct fsg_domain fsg_domain;
+
+/* Tunables; fsg_params_default gives the reference's hard-coded values. */
+typedef struct fsg_params {
+  float gravity_x;        /* 0      (src/FluidSolver.cpp:232) */
+  float gravity_y;        /* -9.82f (src/FluidSolver.cpp:232) */
+  double cg_tol;          /* FLT_EPSILON: Eigen's default for float (no setTolerance call) */
+  int32_t cg_max_iter;    /* 100    (src/FluidSolver.cpp:81) */
+  int32_t ext_layers;     /* 2      (src/FluidSolver.cpp:234) */
+  int32_t integrator;     /* FSG_RK3 */
+  int32_t solver;         /* FSG_SOLVER_PLAIN */
+  int32_t project_mean;   /* 0: remove the mean of b (fully Neumann systems) */
+  int32_t check_every;    /* plain solver: iterations enqueued between host checks */
+  double refine_inner_tol;/* refined solver: relative tolerance of each FP32 inner solve */
+  int32_t refine_max_outer;/* refined solver: maximum FP64 refinement sweeps */
+  int32_t record_ext_info; /* 1: fsg_step also records the extension sources */
+  int32_t reserved[4];
+} fsg_params;
+
+/* Counters of the last pressure solve and running totals. */
+typedef struct fsg_stats {
+  int32_t n_fluid;        /* unknowns of the last solve */
+  int32_t cg_iterations;  /* iterations of the last solve (Eigen's count) */
+  double cg_error;        /* |r|/|b| reported by the last solve */
+  int32_t refine_sweeps;  /* FP64 sweeps of the last refined solve */
+  int32_t reserved0;
+  int64_t total_unknown_iterations; /* sum over solves of n_fluid * SpMV count */
+  int64_t total_solves;
+  int64_t reserved[4];
+} fsg_stats;
+
+/* ---- lifecycle ------------------------------------------------------------ */
+const char *fsg_version(void);
+void fsg_params_default(fsg_params *p);
+/* FluidDomain(size_x, size_y, length_x, length_y, density, pic_ratio)
+   (include/FluidDomain.h:40-46) + MacGrid(...) (include/MacGrid.h:22). */
+int fsg_create(fsg_domain **out, int32_t nx, int32_t ny, float length_x, float length_y,
+               float density, float pic_ratio);
+int fsg_destroy(fsg_domain *h);
+const char *fsg_last_error(const fsg_domain *h);
+int fsg_set_params(fsg_domain *h, const fsg_params *p);
+int fsg_get_params(const fsg_domain *h, fsg_params *p);
+/* FluidDomain::setPicRatio (src/FluidDomain.cpp:99-102, clamps to [0,1]). */
+int fsg_set_pic_ratio(fsg_domain *h, float pic_ratio);
+/* sizes, deltas, density, pic ratio: GridInterface getters (include/Grid.h:50-55),
+   FluidDomain::density/picRatio (src/FluidDomain.cpp:210-218). */
+int fsg_get_geometry(const fsg_domain *h, int32_t *nx, int32_t *ny, float *dx, float *dy,
+                     float *density, float *pic_ratio);
+int fsg_synchronize(fsg_domain *h);
+
+/* ---- particles: MarkerParticleSet (include/MarkerParticleSet.h:53-78) ------- */
+/* replaces the particle vector wholesale (FluidDomain::resetParticleSet + addParticle) */
+int fsg_set_particles(fsg_domain *h, const float *xyuv, int64_t n);
+/* MarkerParticleSet::addParticle (include/MarkerParticleSet.h:56), batched append */
+int fsg_add_particles(fsg_domain *h, const float *xyuv, int64_t n);
+/* MarkerParticleSet::size / iteration (include/MarkerParticleSet.h:65-74) */
+int fsg_num_particles(const fsg_domain *h, int64_t *n);
+int fsg_get_particles(fsg_domain *h, float *xyuv, int64_t capacity);
+
+/* ---- grid state: MacGrid (include/MacGrid.h:22-159) ------------------------ */
+int fsg_set_velocity(fsg_domain *h, int32_t which, const float *u, const float *v);
+int fsg_get_velocity(fsg_domain *h, int32_t which, float *u, float *v);
+/* MacGrid::swapVelocityBuffers (src/MacGrid.cpp:89-93) */
+int fsg_swap_velocity(fsg_domain *h);
+/* MacGrid::setCellType / cellType (include/MacGrid.h:92-97,134-137) */
+int fsg_set_labels(fsg_domain *h, const uint8_t *labels);
+int fsg_get_labels(fsg_domain *h, uint8_t *labels);
+/* new: static interior solids applied wherever the border is (NULL clears) */
+int fsg_set_solid_mask(fsg_domain *h, const uint8_t *mask);
+/* LevelSet values used by FSG_PH_CLASSIFY_LEVELSET (include/LevelSet.h:8-18) */
+int fsg_set_level_set(fsg_domain *h, const float *phi);
+
+/* ---- stepping: FluidSolver (include/FluidSolver.h:45-54) ------------------- */
+int fsg_step(fsg_domain *h, int32_t scheme, float dt);
+int fsg_run_phase(fsg_domain *h, int32_t phase, float dt);
+/* FluidSolver::validate (src/FluidSolver.cpp:89-97): the pool a solver was built
+   from must match the domain; returns FSG_OK or FSG_E_VALIDATE with the
+   reference's message. */
+int fsg_validate_pool(fsg_domain *h, int32_t pool_nx, int32_t pool_ny, float pool_dx,
+                      float pool_dy);
+
+/* ---- inspection (parity outputs) ------------------------------------------ */
+/* pressure x per cell (LIQUID cells; 0 elsewhere) and b per cell, last solve */
+int fsg_get_pressure(fsg_domain *h, float *p);
+int fsg_get_rhs(fsg_domain *h, float *b);
+/* FluidSolverMemoryPool::fluid_indices (include/FluidSolver.h:29) */
+int fsg_get_fluid_indices(fsg_domain *h, int32_t *idx);
+/* FluidSolverMemoryPool valid masks (front pair, include/FluidSolver.h:33-36) and
+   per-face extension sources: (layer << 4) | neighbour bits, neighbour bit order
+   (i-1,j) (i,j-1) (i,j+1) (i+1,j), layer 0 = valid before extension, 15 = never */
+int fsg_get_valid_masks(fsg_domain *h, uint8_t *mask_u, uint8_t *mask_v);
+int fsg_get_ext_info(fsg_domain *h, uint8_t *info_u, uint8_t *info_v);
+/* per particle: 1 where advectAndEnsureOutsideObstacles stepped back */
+int fsg_get_reject_flags(fsg_domain *h, uint8_t *flags);
+int fsg_get_stats(fsg_domain *h, fsg_stats *s);
+int fsg_reset_stats(fsg_domain *h);
+
+/* ---- timing --------------------------------------------------------------- */
+/* Per-kernel HIP-event timing on the handle's stream: 0 off (default), 1 every
+   kernel, 2 only the pressure-solver kernels. */
+int fsg_set_timing(fsg_domain *h, int32_t enable);
+/* Kernel names and their summed durations (ms) and launch counts since the last
+   reset; returns the number of kernels (writes at most `cap` entries). */
+int fsg_get_timing(fsg_domain *h, int32_t cap, const char **names, double *ms,
+                   int64_t *launches);
+int fsg_reset_timing(fsg_domain *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

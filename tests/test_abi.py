@@ -1,0 +1,40 @@
+"""The C ABI library loads and exports every entry point include/fsg.h declares
+(no device calls: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+from fluid_simulation_amd import fsg
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "fsg.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fsg_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_matches_binding_list():
+    assert declared_functions() == sorted(fsg.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(fsg.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_params_struct_layout_and_defaults():
+    # layout: 2 floats, double, 6 int32, double, 2 int32, 4 int32 reserved
+    assert ctypes.sizeof(fsg.Params) == 4 * 2 + 8 + 4 * 6 + 8 + 4 * 2 + 4 * 4
+    p = fsg.default_params()
+    assert p.gravity_y == ctypes.c_float(-9.82).value      # src/FluidSolver.cpp:232
+    assert p.cg_max_iter == 100                              # src/FluidSolver.cpp:81
+    assert p.ext_layers == 2                                 # src/FluidSolver.cpp:234
+    assert abs(p.cg_tol - 2.0 ** -23) == 0                   # Eigen default for float
+    assert p.integrator == fsg.RK3                           # include/FluidSolver.h:144
+
+
+def test_version_string():
+    assert b"gfx950" in fsg.load().fsg_version()

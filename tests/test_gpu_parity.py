@@ -1,0 +1,264 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle, phase by phase.
+
+Inputs are the oracle's own state after a few steps of the 128x128 dam break
+(the CPU-runnable workload), copied into the GPU domain, so both sides start from
+bit-identical front/back/prev/diff grids, labels and particles.
+
+Bars: discrete outputs (labels, fluid indices, valid masks, extension sources,
+rejection flags) bit-exact; phases without reductions bit-exact; phases with
+fp32 atomics (P2G, semi-Lagrangian scatter) within 1e-5 relative; the pressure
+solve against the FP64 oracle solved to 1e-12 within the tolerance stated at each
+test.
+"""
+import numpy as np
+import pytest
+
+from fluid_simulation_amd import fsg, scenes
+from helpers import copy_state, gpu_from_oracle, oracle_after, rel_err
+
+pytestmark = pytest.mark.gpu
+
+PICFLIP = 3
+
+
+@pytest.fixture(scope="module", params=[0, 4])
+def state(request):
+    sc, sim = oracle_after(128, steps=request.param)
+    return sc, sim
+
+
+def _pair(state, **params):
+    sc, sim = state
+    from oracle import OracleSim
+    o = OracleSim(sim.nx, sim.ny, sc.length_x, sc.length_y, sc.density, sc.pic_ratio, "f32")
+    copy_state(sim, o)
+    o.set_params(cg_tol=float(sim.s.cg_tol), cg_max_iter=int(sim.s.cg_max_iter))
+    g = gpu_from_oracle(o, sc, **params)
+    return sc, o, g
+
+
+def _assert_vel_equal(g, o, which="front"):
+    u, v = g.velocity(fsg.VEL_FRONT if which == "front" else fsg.VEL_BACK)
+    uo, vo = o.vel(which)
+    assert np.array_equal(u.view(np.uint32), uo.view(np.uint32)), f"u {which} differs"
+    assert np.array_equal(v.view(np.uint32), vo.view(np.uint32)), f"v {which} differs"
+
+
+def test_classify_bit_exact(state):
+    sc, o, g = _pair(state)
+    o.run_phase("CLASSIFY")
+    g.run_phase("CLASSIFY")
+    assert np.array_equal(g.labels(), o.label)
+
+
+def test_classify_edge_particles():
+    """Particles on cell boundaries, just below them and outside [0, L)."""
+    from oracle import OracleSim
+    pts = np.array([[2.0, 3.0, 0, 0], [2.9999998, 3.5, 0, 0], [5.0000005, 5.0, 0, 0],
+                    [-0.5, 4.0, 0, 0], [8.5, 2.0, 0, 0], [7.9, 7.9, 0, 0],
+                    [1e-7, 5.0, 0, 0], [4.5, -3.0, 0, 0], [3.25, 6.999999, 0, 0]],
+                   dtype=np.float32)
+    o = OracleSim(8, 8, 8.0, 8.0, 0.01, 0.05, "f32")
+    o.set_particles(pts)
+    o.run_phase("CLASSIFY")
+    g = fsg.Domain(8, 8, 8.0, 8.0, 0.01, 0.05)
+    g.set_particles(pts)
+    g.run_phase("CLASSIFY")
+    assert np.array_equal(g.labels(), o.label)
+    liquid = {(int(i), int(j)) for j, i in zip(*np.nonzero(o.label == 0))}
+    assert liquid == {(2, 3), (2, 3), (5, 5), (3, 6)}
+
+
+def test_p2g_within_1e5(state):
+    sc, o, g = _pair(state)
+    for s in (o, g):
+        s.run_phase("CLASSIFY")
+        s.run_phase("P2G")
+    u, v = g.velocity(fsg.VEL_FRONT)
+    uo, vo = o.vel("front")
+    eu, ev = rel_err(u, uo), rel_err(v, vo)
+    print(f"p2g rel err u {eu:.2e} v {ev:.2e}")
+    assert eu <= 1e-5 and ev <= 1e-5
+    # the buffer P2G did not write is untouched
+    _assert_vel_equal(g, o, "back")
+
+
+@pytest.mark.parametrize("phase", ["SAVE_PREV", "GRAVITY", "DIRICHLET", "DIFF"])
+def test_face_phases_bit_exact(state, phase):
+    sc, o, g = _pair(state)
+    rng = np.random.default_rng(7)
+    u = rng.standard_normal((o.ny, o.nx)).astype(np.float32)
+    v = rng.standard_normal((o.ny, o.nx)).astype(np.float32)
+    o.set_vel(u, v, "front")
+    g.set_velocity(u, v, fsg.VEL_FRONT)
+    o.run_phase(phase, sc.dt)
+    g.run_phase(phase, sc.dt)
+    _assert_vel_equal(g, o, "front")
+    if phase == "SAVE_PREV":
+        pu, pv = g.velocity(fsg.VEL_PREV)
+        assert np.array_equal(pu, o.u_prev) and np.array_equal(pv, o.v_prev)
+    if phase == "DIFF":
+        du, dv = g.velocity(fsg.VEL_DIFF)
+        assert np.array_equal(du.view(np.uint32), o.u_diff.view(np.uint32))
+        assert np.array_equal(dv.view(np.uint32), o.v_diff.view(np.uint32))
+
+
+def test_extend_bit_exact(state):
+    sc, o, g = _pair(state)
+    for s in (o, g):
+        s.run_phase("CLASSIFY")
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal((o.ny, o.nx)).astype(np.float32)
+    v = rng.standard_normal((o.ny, o.nx)).astype(np.float32)
+    o.set_vel(u, v, "front")
+    g.set_velocity(u, v, fsg.VEL_FRONT)
+    o.run_phase("EXTEND")
+    g.run_phase("EXTEND")
+    _assert_vel_equal(g, o, "front")
+    _assert_vel_equal(g, o, "back")
+    mu, mv = g.valid_masks()
+    mou, mov = o.valid_masks()
+    assert np.array_equal(mu, mou) and np.array_equal(mv, mov)
+    iu, iv = g.ext_info()
+    assert np.array_equal(iu, o.ext_info_u) and np.array_equal(iv, o.ext_info_v)
+    # two layers were actually added
+    assert (o.ext_info_u >> 4 == 2).any() and (o.ext_info_v >> 4 == 1).any()
+
+
+def _pressure_inputs(state):
+    sc, o, g = _pair(state)
+    for s in (o, g):
+        s.run_phase("CLASSIFY")
+        s.run_phase("P2G")
+        s.run_phase("SAVE_PREV")
+        s.run_phase("GRAVITY", sc.dt)
+        s.run_phase("DIRICHLET")
+        s.run_phase("EXTEND")
+    # continue from the oracle's (not the GPU's) P2G output on both sides
+    uf, vf = o.vel("front")
+    ub, vb = o.vel("back")
+    g.set_velocity(uf, vf, fsg.VEL_FRONT)
+    g.set_velocity(ub, vb, fsg.VEL_BACK)
+    return sc, o, g
+
+
+def _f64_twin(o, sc, tol=1e-12):
+    from oracle import OracleSim
+    d = OracleSim(o.nx, o.ny, sc.length_x, sc.length_y, sc.density, sc.pic_ratio, "f64")
+    copy_state(o, d)
+    d.set_params(cg_tol=tol, cg_max_iter=100000)
+    return d
+
+
+@pytest.mark.parametrize("solver,tol,bar_p", [
+    (fsg.SOLVER_PLAIN, 1e-6, 2e-5),
+    (fsg.SOLVER_PERSISTENT, 1e-6, 2e-5),
+    (fsg.SOLVER_REFINED, 1e-10, 1e-5),
+])
+def test_pressure_against_fp64(state, solver, tol, bar_p):
+    sc, o, g = _pressure_inputs(state)
+    g.set_params(solver=solver, cg_tol=tol, cg_max_iter=20000, refine_inner_tol=1e-3)
+    d = _f64_twin(o, sc)
+    o.set_params(cg_tol=tol if tol >= 1e-7 else 1e-7, cg_max_iter=20000)
+    for s in (o, d):
+        s.run_phase("PRESSURE", sc.dt)
+    g.run_phase("PRESSURE", sc.dt)
+    # discrete outputs and b are computed without reductions: bit-exact vs f32
+    assert np.array_equal(g.fluid_indices(), o.fluid_index)
+    assert np.array_equal(g.rhs().view(np.uint32), o.rhs.view(np.uint32))
+    st = g.stats()
+    ep = rel_err(g.pressure(), d.pressure)
+    u, v = g.velocity(fsg.VEL_FRONT)
+    ud, vd = d.vel("front")
+    eu, ev = rel_err(u, ud), rel_err(v, vd)
+    ep32 = rel_err(o.pressure, d.pressure)
+    print(f"solver {solver}: F {st.n_fluid} iters {st.cg_iterations} sweeps "
+          f"{st.refine_sweeps} |r|/|b| {st.cg_error:.2e}; rel err vs fp64: p {ep:.2e} "
+          f"u {eu:.2e} v {ev:.2e} (fp32 oracle p {ep32:.2e})")
+    assert st.n_fluid == o.s.n_fluid
+    assert st.cg_error <= max(tol, 1e-7) * 1.01
+    assert ep <= bar_p
+    assert max(eu, ev) <= bar_p
+
+
+def test_g2p_and_advect_bit_exact(state):
+    sc, o, g = _pair(state)
+    for phase in ("G2P_PICFLIP", "ADVECT_REJECT"):
+        o.run_phase(phase, sc.dt)
+        g.run_phase(phase, sc.dt)
+    assert np.array_equal(g.particles().view(np.uint32), o.particles.view(np.uint32))
+    assert np.array_equal(g.reject_flags(), o.reject_flag)
+
+
+@pytest.mark.parametrize("phase", ["G2P_PIC", "G2P_FLIP", "ADVECT", "ADVECT_GRID"])
+def test_particle_phases_bit_exact(state, phase):
+    sc, o, g = _pair(state)
+    o.run_phase(phase, sc.dt)
+    g.run_phase(phase, sc.dt)
+    assert np.array_equal(g.particles().view(np.uint32), o.particles.view(np.uint32))
+
+
+def test_sl_scatter_within_1e5(state):
+    sc, o, g = _pair(state)
+    for s in (o, g):
+        s.run_phase("CLASSIFY")
+        s.run_phase("SL_SCATTER", sc.dt)
+    _assert_vel_equal(g, o, "front")
+    ub, vb = g.velocity(fsg.VEL_BACK)
+    uo, vo = o.vel("back")
+    assert rel_err(ub, uo) <= 1e-5 and rel_err(vb, vo) <= 1e-5
+
+
+@pytest.mark.parametrize("integrator", [0, 1, 2])
+def test_sl_gather_bit_exact(state, integrator):
+    sc, o, g = _pair(state)
+    o.set_params(integrator=integrator)
+    g.set_params(integrator=integrator)
+    for s in (o, g):
+        s.run_phase("CLASSIFY")
+        s.run_phase("SL_GATHER", sc.dt)
+    _assert_vel_equal(g, o, "front")
+    _assert_vel_equal(g, o, "back")
+
+
+@pytest.mark.parametrize("scheme", [fsg.PIC, fsg.FLIP, fsg.PICFLIP, fsg.SEMI_LAGRANGIAN])
+def test_full_step(state, scheme):
+    """One whole step against the FP32 oracle; CG to 1e-7 on both sides."""
+    sc, o, g = _pair(state)
+    o.set_params(cg_tol=1e-7, cg_max_iter=20000)
+    g.set_params(cg_tol=1e-7, cg_max_iter=20000, solver=fsg.SOLVER_PLAIN)
+    o.step(scheme, sc.dt)
+    g.step(scheme, sc.dt)
+    assert np.array_equal(g.labels(), o.label)
+    ep = rel_err(g.particles(), o.particles)
+    u, v = g.velocity(fsg.VEL_FRONT)
+    uo, vo = o.vel("front")
+    ub, vb = g.velocity(fsg.VEL_BACK)
+    ubo, vbo = o.vel("back")
+    errs = [ep, rel_err(u, uo), rel_err(v, vo), rel_err(ub, ubo), rel_err(vb, vbo)]
+    print(f"scheme {scheme}: particles {errs[0]:.2e} front u/v {errs[1]:.2e}/{errs[2]:.2e} "
+          f"back u/v {errs[3]:.2e}/{errs[4]:.2e}")
+    assert max(errs) <= 1e-4
+
+
+def test_no_fluid_keeps_buffers_unswapped():
+    """pressureSolve returns before the swap when no cell is LIQUID
+    (src/FluidSolver.cpp:347-350): front stays the extended field."""
+    from oracle import OracleSim
+    o = OracleSim(16, 16, 16.0, 16.0, 0.01, 0.05, "f32")
+    rng = np.random.default_rng(1)
+    for which in ("front", "back"):
+        o.set_vel(rng.standard_normal((16, 16)), rng.standard_normal((16, 16)), which)
+    sc = scenes.Scene(16, 16, 16.0, 16.0, 0.01, 0.05, 0.01, np.zeros((0, 4), np.float32))
+    g = gpu_from_oracle(o, sc)
+    o.step(PICFLIP, 0.01)
+    g.step(fsg.PICFLIP, 0.01)
+    _assert_vel_equal(g, o, "front")
+    _assert_vel_equal(g, o, "back")
+
+
+def test_validate_error_message():
+    """dx != dy fails like FluidSolver::validate (src/FluidSolver.cpp:101-107)."""
+    g = fsg.Domain(16, 8, 16.0, 16.0, 0.01, 0.05)   # dx = 1, dy = 2
+    with pytest.raises(fsg.FsgError, match="Memory pool and fluid domain does not match"):
+        g.step(fsg.PICFLIP, 0.01)
