@@ -107,7 +107,7 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=1024)
-    ap.add_argument("--solver", default="refined", choices=["plain", "persistent", "refined"])
+    ap.add_argument("--solver", default="persistent", choices=["plain", "persistent", "refined"])
     ap.add_argument("--plain-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-fp64-check", action="store_true")

@@ -105,14 +105,16 @@ struct fsg_domain {
   float* r = nullptr;
   float* ap = nullptr;
   float* xg = nullptr;
-  float* zg = nullptr;
   float* pg0 = nullptr;
-  float* pg1 = nullptr;
+  int2* updn = nullptr;
+  float* exch = nullptr;
   double* x64 = nullptr;
   double* partials = nullptr;
   unsigned* bar = nullptr;
   CgScalars* sc = nullptr;
-  int persistent_grid = 0;
+  int max_coop_blocks = 0;  // co-resident 1024-thread blocks of the persistent solver
+  size_t pcg_lds_max = 0;   // dynamic LDS the persistent solver may use per block
+  int last_F = 0, last_grid = 0, last_kpt = 0;
   // host-side statistics of the plain solver
   fsg_stats host_stats{};
   std::string err;
@@ -239,19 +241,19 @@ int solve_pressure(fsg_domain* h) {
     launch_compact(h->stream, h->g, h->label, h->offsets, h->fluid_index, h->cell_of, UF(h), VF(h),
                    h->b, h->info);
   });
+  // the launch geometry of both solver forms depends on the number of unknowns
+  int F = 0;
+  HIPCHK(h, hipMemcpyAsync(&F, h->n_fluid, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->host_stats.n_fluid = F;
+  h->host_stats.cg_iterations = 0;
+  h->host_stats.cg_error = 0.0;
+  h->host_stats.refine_sweeps = 0;
+  h->last_F = F;
+  if (F == 0) return FSG_OK;
   const int solver = h->prm.solver;
   if (solver == FSG_SOLVER_PLAIN) {
-    int F = 0;
-    HIPCHK(h, hipMemcpyAsync(&F, h->n_fluid, sizeof(int), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    h->host_stats.n_fluid = F;
-    h->host_stats.cg_iterations = 0;
-    h->host_stats.cg_error = 0.0;
-    h->host_stats.refine_sweeps = 0;
-    if (F == 0) return FSG_OK;
     const CgParams cp = make_cg_params(h, (float)h->prm.cg_tol);
-    if (cp.project_mean)
-      timed(h, "cg_mean", [&] { launch_cg_mean(h->stream, F, h->b, h->partials, h->sc); });
     timed(h, "cg_init", [&] {
       launch_cg_init(h->stream, F, cp, h->cell_of, h->info, h->b, h->r, h->pg0, h->xg, h->partials,
                      h->sc);
@@ -259,17 +261,23 @@ int solve_pressure(fsg_domain* h) {
     const int check = h->prm.check_every > 0 ? h->prm.check_every : 32;
     int launched = 0;
     CgScalars hs{};
-    while (true) {
+    while (launched < cp.max_iter) {
+      const int n = std::min(check, cp.max_iter - launched);
+      for (int q = 0; q < n; ++q) {
+        const int it = launched + q;
+        timed(h, "cg_iteration", [&] {
+          launch_cg_iteration(h->stream, F, h->g.nx, it, cp, h->cell_of, h->info, h->r, h->ap,
+                              h->pg0, h->xg, h->partials, h->sc);
+        });
+      }
+      launched += n;
       HIPCHK(h, hipMemcpyAsync(&hs, h->sc, sizeof(hs), hipMemcpyDeviceToHost, h->stream));
       HIPCHK(h, hipStreamSynchronize(h->stream));
-      if (hs.done || launched >= cp.max_iter) break;
-      const int n = std::min(check, cp.max_iter - launched);
-      for (int q = 0; q < n; ++q)
-        timed(h, "cg_iteration", [&] {
-          launch_cg_iteration(h->stream, F, h->g.nx, cp, h->cell_of, h->info, h->r, h->ap, h->pg0,
-                              h->xg, h->partials, h->sc);
-        });
-      launched += n;
+      if (hs.done) break;
+    }
+    if (cp.max_iter == 0) {
+      HIPCHK(h, hipMemcpyAsync(&hs, h->sc, sizeof(hs), hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(h, hipStreamSynchronize(h->stream));
     }
     h->host_stats.cg_iterations = hs.iter;
     h->host_stats.cg_error = hs.err;
@@ -277,21 +285,49 @@ int solve_pressure(fsg_domain* h) {
     h->host_stats.total_solves += 1;
     return FSG_OK;
   }
-  // persistent / refined: one cooperative launch, no host decision
+  // persistent / refined: one cooperative launch, every iteration on the device
+  // Launch shape: G blocks of 1024 threads, KPT unknowns per thread.  Fewer blocks
+  // make the grid barrier cheaper (measured ~1 us at 32 blocks, ~3.6 us at 256);
+  // more unknowns per thread cost registers and LDS (a window of per + 2 nx values
+  // of three arrays).  The default takes the largest KPT whose LDS window fits;
+  // fsg_params.pcg_kpt / pcg_blocks override.
+  int kpt = 0, grid = 0;
+  const int gmax = h->max_coop_blocks;
+  const int kfix = h->prm.pcg_kpt;
+  // spill-free register budgets at 1024 threads: KPT <= 4 (FP32 form), KPT <= 2
+  // (refined); measured fastest at 1024^2: KPT 2 for both
+  const bool refined = solver == FSG_SOLVER_REFINED;
+  const int order_plain[4] = {2, 4, 8, 1}, order_ref[4] = {2, 4, 8, 1};
+  for (int k : refined ? order_ref : order_plain) {
+    if (kfix > 0 && k != kfix) continue;
+    const int gneed = (F + 1024 * k - 1) / (1024 * k);
+    if (gneed > gmax) continue;
+    int g = gneed;
+    if (h->prm.pcg_blocks > 0) g = std::min(std::max(gneed, h->prm.pcg_blocks), gmax);
+    if (pcg_lds_bytes(F, g, h->g.nx) > h->pcg_lds_max) continue;
+    kpt = k;
+    grid = g;
+    break;
+  }
+  if (kpt == 0)
+    return fail(h, FSG_E_INVALID,
+                "pressure system too large for the register-resident persistent solver "
+                "(use FSG_SOLVER_PLAIN)");
+  timed(h, "updown", [&] {
+    launch_updown(h->stream, h->g.nx, F, h->n_fluid, h->cell_of, h->info, h->fluid_index, h->updn);
+  });
   PersistentArgs a{};
   a.nx = h->g.nx;
   a.F_max = (int)ncell(h);
+  a.F_actual = F;
   a.n_fluid = h->n_fluid;
   a.cell_of = h->cell_of;
   a.info = h->info;
+  a.updn = h->updn;
   a.b = h->b;
-  a.r = h->r;
   a.xg = h->xg;
-  a.zg = h->zg;
-  a.pg0 = h->pg0;
-  a.pg1 = h->pg1;
+  a.exch = h->exch;
   a.x64 = h->x64;
-  a.e32 = h->ap;
   a.partials = h->partials;
   a.bar = h->bar;
   a.sc = h->sc;
@@ -302,9 +338,10 @@ int solve_pressure(fsg_domain* h) {
   a.outer_tol = h->prm.cg_tol;
   timed(h, "zero_barrier", [&] { launch_fill(h->stream, reinterpret_cast<float*>(h->bar), 4, 0.f); });
   hipError_t e = hipSuccess;
-  timed(h, a.refined ? "cg_refined" : "cg_persistent",
-        [&] { e = launch_cg_persistent(h->stream, a, h->persistent_grid); });
+  timed(h, a.refined ? "cg_refined" : "cg_persistent", [&] { e = launch_pcg(h->stream, a, grid, kpt); });
   HIPCHK(h, e);
+  h->last_grid = grid;
+  h->last_kpt = kpt;
   return FSG_OK;
 }
 
@@ -468,9 +505,9 @@ int fsg_create(fsg_domain** out, int32_t nx, int32_t ny, float length_x, float l
   ok &= dalloc(h, &h->r, C) == hipSuccess;
   ok &= dalloc(h, &h->ap, C) == hipSuccess;
   ok &= dalloc(h, &h->xg, C) == hipSuccess;
-  ok &= dalloc(h, &h->zg, C) == hipSuccess;
   ok &= dalloc(h, &h->pg0, C) == hipSuccess;
-  ok &= dalloc(h, &h->pg1, C) == hipSuccess;
+  ok &= dalloc(h, &h->updn, C) == hipSuccess;
+  ok &= dalloc(h, &h->exch, 3 * C) == hipSuccess;
   ok &= dalloc(h, &h->x64, C) == hipSuccess;
   ok &= dalloc(h, &h->partials, (size_t)4 * 4096) == hipSuccess;
   ok &= dalloc(h, &h->bar, 16) == hipSuccess;
@@ -478,7 +515,17 @@ int fsg_create(fsg_domain** out, int32_t nx, int32_t ny, float length_x, float l
   if (!ok) return bail(FSG_E_NOMEM);
   // labels start as clearCellTypeBuffer leaves them (MacGrid ctor, src/MacGrid.cpp:24)
   launch_cell_init(h->stream, h->g, h->label, nullptr, h->acc, 1, 0);
-  h->persistent_grid = persistent_grid_size(h->device);
+  {
+    // the persistent solver's LDS window: all of a CU's LDS but the static part
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, h->device) == hipSuccess) {
+      size_t lds = prop.maxSharedMemoryPerMultiProcessor;
+      if (lds == 0 || lds > 160 * 1024) lds = 160 * 1024;
+      h->pcg_lds_max = lds - 4096;
+      if (pcg_set_lds_limit(h->pcg_lds_max) == hipSuccess)
+        h->max_coop_blocks = std::min(prop.multiProcessorCount, 1024);
+    }
+  }
   if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(FSG_E_HIP);
   *out = h;
   return FSG_OK;
@@ -503,7 +550,7 @@ int fsg_set_params(fsg_domain* h, const fsg_params* p) {
   if (p->cg_max_iter < 0 || p->ext_layers < 0 || p->ext_layers > 14 || p->integrator < 0 ||
       p->integrator > 2 || p->solver < 0 || p->solver > 2 || !(p->cg_tol >= 0.0))
     return fail(h, FSG_E_INVALID, "fsg_set_params: value out of range");
-  if (p->solver != FSG_SOLVER_PLAIN && h->persistent_grid <= 0)
+  if (p->solver != FSG_SOLVER_PLAIN && h->max_coop_blocks <= 0)
     return fail(h, FSG_E_INVALID, "persistent solver unavailable on this device");
   h->prm = *p;
   return FSG_OK;
@@ -535,10 +582,22 @@ int fsg_get_geometry(const fsg_domain* h, int32_t* nx, int32_t* ny, float* dx, f
   return FSG_OK;
 }
 
+// The persistent solver raises bar[1] when a block abandons a grid-barrier wait
+// (bounded spin); every block then leaves the kernel and the solution is invalid.
+static int check_solver_abort(fsg_domain* h) {
+  unsigned flag = 0;
+  HIPCHK(h, hipMemcpyAsync(&flag, h->bar + 1, sizeof(flag), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (flag)
+    return fail(h, FSG_E_HIP,
+                "persistent pressure solver abandoned a grid-barrier wait: results are invalid");
+  return FSG_OK;
+}
+
 int fsg_synchronize(fsg_domain* h) {
   if (!h) return FSG_E_INVALID;
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  return FSG_OK;
+  return check_solver_abort(h);
 }
 
 static int reserve_particles(fsg_domain* h, int64_t n) {
@@ -856,6 +915,7 @@ int fsg_get_reject_flags(fsg_domain* h, uint8_t* flags) {
 
 int fsg_get_stats(fsg_domain* h, fsg_stats* s) {
   if (!h || !s) return FSG_E_INVALID;
+  if (int rc = check_solver_abort(h)) return rc;
   if (h->prm.solver == FSG_SOLVER_PLAIN) {
     *s = h->host_stats;
     return FSG_OK;

@@ -45,23 +45,19 @@ void launch_fill(hipStream_t s, float* a, int64_t n, float val);
 // ---------------------------------------------------------------- pressure
 constexpr int kScanChunk = 2048;  // cells per block of the liquid count / compaction
 
-// Device-resident scalars of one solve (written by the last block of each
-// reduction kernel, read by every later kernel of the solve).
+// Device-resident scalars of one solve.
 struct CgScalars {
   double rhs_norm2;      // |b|^2 (after the optional mean removal)
   double threshold;      // max(tol^2 |b|^2, min normal)
-  double abs_new;        // r.z of the current direction
   double rr;             // |r|^2 after the last update
-  double pap;            // p.Ap of the current direction
-  double mean;           // mean of b (project_mean)
-  float alpha, beta;
-  int iter;              // Eigen's iteration count
+  double err;            // sqrt(|r|^2 / |b|^2)
+  float rz[2];           // r.z of the current / next direction (iteration parity)
+  float beta;
+  int iter;              // iterations (Eigen's count for the plain form)
   int spmv;              // SpMV applications
   int done;              // 1 once the solve has stopped
   int refine_sweeps;
-  unsigned counter;      // last-block ticket
-  unsigned pad;
-  double err;            // sqrt(|r|^2 / |b|^2)
+  int pad;
   double b64_norm2;      // refined form: |b|^2 in FP64
   double res64_norm2;    // refined form: |b - A x|^2 in FP64
   unsigned long long total_unknown_iters;
@@ -82,33 +78,32 @@ void launch_scan_counts(hipStream_t s, int nb, const int* counts, int* offsets, 
 void launch_compact(hipStream_t s, const Geom& g, const uint8_t* label, const int* offsets,
                     int* fluid_index, int* cell_of, const float* uf, const float* vf, float* b,
                     uint8_t* info);
+void launch_updown(hipStream_t s, int nx, int F_bound, const int* n_fluid, const int* cell_of,
+                   const uint8_t* info, const int* fluid_index, int2* updn);
 
-// plain form
-void launch_cg_mean(hipStream_t s, int F, const float* b, double* partials, CgScalars* sc);
+// plain form (partials: >= 12288 doubles)
 void launch_cg_init(hipStream_t s, int F, const CgParams& cp, const int* cell_of,
                     const uint8_t* info, const float* b, float* r, float* pg, float* xg,
                     double* partials, CgScalars* sc);
-void launch_cg_iteration(hipStream_t s, int F, int nx, const CgParams& cp, const int* cell_of,
-                         const uint8_t* info, float* r, float* ap, float* pg, float* xg,
-                         double* partials, CgScalars* sc);
+void launch_cg_iteration(hipStream_t s, int F, int nx, int it, const CgParams& cp,
+                         const int* cell_of, const uint8_t* info, float* r, float* ap, float* pg,
+                         float* xg, double* partials, CgScalars* sc);
 
-// persistent / refined forms
+// persistent forms (pipelined PCG, optionally inside FP64 refinement)
 struct PersistentArgs {
   int nx;
-  int F_max;
+  int F_max;             // capacity of the compact arrays
+  int F_actual;          // unknowns of this solve (host copy, sizes the launch)
   const int* n_fluid;
   const int* cell_of;
   const uint8_t* info;
+  const int2* updn;
   const float* b;
-  float* r;        // FP32 residual of the inner solve (compact)
-  float* xg;       // solution, grid layout
-  float* zg;       // z = M^-1 r, grid layout
-  float* pg0;      // search directions, grid layout, ping-pong
-  float* pg1;
-  double* x64;     // refined form: FP64 solution (compact)
-  float* e32;      // refined form: FP32 correction (compact)
-  double* partials;
-  unsigned* bar;   // barrier counters
+  float* xg;             // solution, grid layout
+  float* exch;           // 3 x F_max floats: device-coherent z and p (x2) exchange
+  double* x64;           // F_max doubles: FP64 solution exchange (refined form)
+  double* partials;      // 2 x grid x 4 doubles
+  unsigned* bar;         // [0] arrival counter, [1] abort flag
   CgScalars* sc;
   CgParams cp;
   int refined;
@@ -116,7 +111,9 @@ struct PersistentArgs {
   float inner_tol;
   double outer_tol;
 };
-int persistent_grid_size(int device);
-hipError_t launch_cg_persistent(hipStream_t s, const PersistentArgs& a, int grid);
+int pcg_max_kpt();
+size_t pcg_lds_bytes(int F, int grid, int nx);
+hipError_t launch_pcg(hipStream_t s, const PersistentArgs& a, int grid, int kpt);
+hipError_t pcg_set_lds_limit(size_t max_bytes);
 
 }  // namespace fsg

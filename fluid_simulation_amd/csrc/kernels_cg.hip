@@ -6,92 +6,28 @@
 // fluid_indices (src/FluidSolver.cpp:331-346), built by a three-kernel scan.  The
 // operator is applied matrix-free: one info byte per unknown holds the four
 // LIQUID-neighbour bits and the count of non-SOLID neighbours, so A's row is
-// diag_tab[count] at the centre and off = 1/dx^2 at each LIQUID neighbour.
-// Vectors that neighbours read (the search direction p and z = M^-1 r) live in
-// grid layout (nx*ny floats, indexed by cell); everything else is compact.
+// diag_tab[count] at the centre and off = 1/dx^2 at each LIQUID neighbour.  Left
+// and right neighbours of unknown k are k-1 and k+1 (row-major order); the up and
+// down neighbours' indices are stored per unknown.
 //
-// Reductions are deterministic: per-thread partial sums, a fixed wave/block tree,
-// per-block partials in FP64, and a fixed-order final sum.
+// Synchronisation on gfx950 (measured): a contended returning atomic costs ~90 ns
+// per block and agent-scope release/acquire fences write back and invalidate L2.
+// So no kernel here uses a last-block ticket or an agent-scope fence:
+//  - the plain form lets every block re-sum the previous kernel's per-block
+//    partials in a fixed order (kernel boundaries order the data);
+//  - the persistent forms use a fence-free grid barrier: data that crosses blocks
+//    (halo values, partial sums) is written with agent-scope atomic stores and read
+//    with agent-scope atomic loads, both of which go to the device coherence point,
+//    and the arrival counter takes one non-returning atomic add per block.
+// Every sum is deterministic (fixed per-thread order, fixed wave/block tree, fixed
+// block order), so every block of a persistent launch derives identical scalars.
 #include <float.h>
 
+#include "fsg_common.h"
 #include "fsg_device.h"
 #include "fsg_kernels.h"
 
 namespace fsg {
-
-// ------------------------------------------------------------------ helpers
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double* sm) {
-  // sm: NV * 4 doubles (256 threads = 4 waves of 64)
-#pragma unroll
-  for (int q = 0; q < NV; ++q)
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_down(v[q], off, 64);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0)
-#pragma unroll
-    for (int q = 0; q < NV; ++q) sm[q * 4 + wave] = v[q];
-  __syncthreads();
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int q = 0; q < NV; ++q) v[q] = ((sm[q * 4 + 0] + sm[q * 4 + 1]) + sm[q * 4 + 2]) + sm[q * 4 + 3];
-  __syncthreads();
-}
-
-// Block partials + last-block-done total.  Returns true in the last block, where
-// thread 0 holds the totals in v (summed over blocks in block order).
-template <int NV>
-__device__ __forceinline__ bool last_block_total(double (&v)[NV], double* partials,
-                                                 unsigned* counter) {
-  __shared__ double sm[NV * 4];
-  __shared__ int is_last;
-  block_sum<NV>(v, sm);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int q = 0; q < NV; ++q) partials[blockIdx.x * NV + q] = v[q];
-    __threadfence();
-    const unsigned t = atomicAdd(counter, 1u);
-    is_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-  double acc[NV];
-#pragma unroll
-  for (int q = 0; q < NV; ++q) acc[q] = 0.0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-#pragma unroll
-    for (int q = 0; q < NV; ++q)
-      acc[q] += __hip_atomic_load(&partials[b * NV + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  block_sum<NV>(acc, sm);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int q = 0; q < NV; ++q) v[q] = acc[q];
-    *counter = 0;
-  }
-  return true;
-}
-
-__device__ __forceinline__ float sel5(int n, const float (&t)[5]) {
-  float r = t[0];
-  r = n == 1 ? t[1] : r;
-  r = n == 2 ? t[2] : r;
-  r = n == 3 ? t[3] : r;
-  r = n == 4 ? t[4] : r;
-  return r;
-}
-
-// Row of the Laplacian at cell c: diag * p[c] first, then the LIQUID neighbours in
-// the order (i-1), (i+1), (j-1), (j+1) (src/FluidSolver.cpp:379-410 insert order).
-__device__ __forceinline__ float apply_row(const float* __restrict__ pg, int c, int nx,
-                                           uint8_t inf, const CgParams& cp) {
-  float acc = sel5((inf >> 4) & 7, cp.diag_tab) * pg[c];
-  if (inf & 1) acc += cp.off * pg[c - 1];
-  if (inf & 2) acc += cp.off * pg[c + 1];
-  if (inf & 4) acc += cp.off * pg[c - nx];
-  if (inf & 8) acc += cp.off * pg[c + nx];
-  return acc;
-}
 
 // ------------------------------------------------------------ system setup
 // Per 2048-cell chunk: number of LIQUID cells (8 cells per thread, 8-byte loads).
@@ -112,7 +48,7 @@ __global__ void __launch_bounds__(256) k_count_liquid(int n, const uint8_t* __re
   }
   double v[1] = {(double)cnt};
   __shared__ double sm[4];
-  block_sum<1>(v, sm);
+  block_sum<1, 256>(v, sm);
   if (threadIdx.x == 0) counts[blockIdx.x] = (int)v[0];
 }
 
@@ -202,28 +138,73 @@ __global__ void __launch_bounds__(256) k_compact(Geom g, const uint8_t* __restri
   }
 }
 
+// Compact indices of the (j-1) and (j+1) neighbours of every unknown (-1 where the
+// neighbour is not LIQUID).  Runs after k_compact has written fluid_index.
+__global__ void __launch_bounds__(256) k_updown(int nx, const int* __restrict__ n_fluid,
+                                                const int* __restrict__ cell_of,
+                                                const uint8_t* __restrict__ info,
+                                                const int* __restrict__ fluid_index,
+                                                int2* __restrict__ updn) {
+  const int F = *n_fluid;
+  for (int k = blockIdx.x * 256 + threadIdx.x; k < F; k += gridDim.x * 256) {
+    const int c = cell_of[k];
+    const uint8_t inf = info[k];
+    updn[k] = make_int2((inf & 4) ? fluid_index[c - nx] : -1, (inf & 8) ? fluid_index[c + nx] : -1);
+  }
+}
+
 // ---------------------------------------------------------------- plain form
 // Eigen's conjugate_gradient restated (x0 = 0; stop when |r|^2 < max(tol^2 |b|^2,
 // min normal) or after max_iter iterations), one iteration = 3 kernels:
-//   spmv:    Ap = A p, p.Ap                       -> alpha = r.z / p.Ap
-//   update:  x += alpha p, r -= alpha Ap, |r|^2, z = r / diag, r.z -> stop / beta
-//   pupdate: p = z + beta p
-// Scalars are formed in float like Eigen's RealScalar; sums are reduced in FP64.
+//   spmv:    Ap = A p, partial p.Ap
+//   update:  alpha = r.z / p.Ap; x += alpha p, r -= alpha Ap; partial |r|^2, r.z
+//   pupdate: stop test, beta = r.z_new / r.z; p = z + beta p
+// Each kernel's blocks re-sum the previous kernel's partials in block order;
+// block 0 records the scalars for the host.  Scalars are formed in float like
+// Eigen's RealScalar; sums are reduced in FP64.
 
-__global__ void __launch_bounds__(256) k_cg_mean(int F, const float* __restrict__ b,
-                                                 double* partials, CgScalars* sc) {
+constexpr int kPlainThreads = 256;
+
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double* part, int nb, double (&v)[NV],
+                                             double* sm) {
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kPlainThreads)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] += part[b * NV + q];
+  block_sum<NV, kPlainThreads>(v, sm);
+}
+
+template <int NV>
+__device__ __forceinline__ void write_partials(double (&v)[NV], double* part, double* sm) {
+  block_sum<NV, kPlainThreads>(v, sm);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) part[blockIdx.x * NV + q] = v[q];
+}
+
+__global__ void __launch_bounds__(256) k_cg_sum_b(int F, const float* __restrict__ b,
+                                                  double* __restrict__ part) {
+  __shared__ double sm[4];
   double v[1] = {0.0};
   for (int k = blockIdx.x * 256 + threadIdx.x; k < F; k += gridDim.x * 256) v[0] += b[k];
-  if (last_block_total<1>(v, partials, &sc->counter) && threadIdx.x == 0)
-    sc->mean = (double)((float)v[0] / (float)F);
+  write_partials<1>(v, part, sm);
 }
 
 __global__ void __launch_bounds__(256) k_cg_init(int F, CgParams cp, const int* __restrict__ cell_of,
                                                  const uint8_t* __restrict__ info,
                                                  const float* __restrict__ b, float* __restrict__ r,
                                                  float* __restrict__ pg, float* __restrict__ xg,
-                                                 double* partials, CgScalars* sc) {
-  const float mean = cp.project_mean ? (float)sc->mean : 0.f;
+                                                 const double* __restrict__ mean_part,
+                                                 double* __restrict__ part, CgScalars* sc) {
+  __shared__ double sm[2 * 4];
+  float mean = 0.f;
+  if (cp.project_mean) {
+    double m[1];
+    sum_partials<1>(mean_part, gridDim.x, m, sm);
+    mean = (float)m[0] / (float)F;
+  }
   double v[2] = {0.0, 0.0};
   for (int k = blockIdx.x * 256 + threadIdx.x; k < F; k += gridDim.x * 256) {
     const int c = cell_of[k];
@@ -235,56 +216,71 @@ __global__ void __launch_bounds__(256) k_cg_init(int F, CgParams cp, const int* 
     v[0] += (double)(rk * rk);
     v[1] += (double)(rk * z);
   }
-  if (last_block_total<2>(v, partials, &sc->counter) && threadIdx.x == 0) {
-    const float rhs = (float)v[0];
-    sc->rhs_norm2 = rhs;
+  write_partials<2>(v, part, sm);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->done = 0;
     sc->iter = 0;
     sc->spmv = 0;
     sc->err = 0.0;
-    if (rhs == 0.f) {
-      sc->done = 1;
-      return;
-    }
-    float thr = cp.tol * cp.tol * rhs;
-    if (thr < FLT_MIN) thr = FLT_MIN;
-    sc->threshold = thr;
-    sc->rr = rhs;
-    sc->abs_new = (float)v[1];
-    sc->done = 0;
-    if (rhs < thr) {
-      sc->done = 1;
-      sc->err = 1.0;
-    }
   }
 }
 
-__global__ void __launch_bounds__(256) k_cg_spmv(int F, int nx, CgParams cp,
+// part0: (|r|^2, r.z) of the previous update (or init, it == 0); part1: p.Ap out
+__global__ void __launch_bounds__(256) k_cg_spmv(int F, int nx, int it, CgParams cp,
                                                  const int* __restrict__ cell_of,
                                                  const uint8_t* __restrict__ info,
                                                  const float* __restrict__ pg, float* __restrict__ ap,
-                                                 double* partials, CgScalars* sc) {
-  if (sc->done) return;
+                                                 const double* __restrict__ part_rz,
+                                                 double* __restrict__ part_pap, CgScalars* sc) {
+  __shared__ double sm[2 * 4];
+  if (it == 0) {
+    // Eigen's entry checks: |b|^2 == 0 -> x = 0; |r0|^2 < threshold -> done
+    double v[2];
+    sum_partials<2>(part_rz, gridDim.x, v, sm);
+    const float rhs = (float)v[0];
+    float thr = cp.tol * cp.tol * rhs;
+    if (thr < FLT_MIN) thr = FLT_MIN;
+    const bool stop = rhs == 0.f || rhs < thr;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->rhs_norm2 = rhs;
+      sc->threshold = thr;
+      sc->rz[0] = (float)v[1];
+      sc->rr = rhs;
+      sc->err = rhs == 0.f ? 0.0 : (stop ? 1.0 : 0.0);
+      if (stop) sc->done = 1;
+    }
+    if (stop) return;
+  } else if (sc->done) {
+    return;
+  }
   double v[1] = {0.0};
   for (int k = blockIdx.x * 256 + threadIdx.x; k < F; k += gridDim.x * 256) {
     const int c = cell_of[k];
-    const float a = apply_row(pg, c, nx, info[k], cp);
-    ap[k] = a;
-    v[0] += (double)(pg[c] * a);
+    const uint8_t inf = info[k];
+    const float pc = pg[c];
+    float acc = sel5((inf >> 4) & 7, cp.diag_tab) * pc;
+    if (inf & 1) acc += cp.off * pg[c - 1];
+    if (inf & 2) acc += cp.off * pg[c + 1];
+    if (inf & 4) acc += cp.off * pg[c - nx];
+    if (inf & 8) acc += cp.off * pg[c + nx];
+    ap[k] = acc;
+    v[0] += (double)(pc * acc);
   }
-  if (last_block_total<1>(v, partials, &sc->counter) && threadIdx.x == 0) {
-    sc->pap = (float)v[0];
-    sc->alpha = (float)sc->abs_new / (float)v[0];
-    sc->spmv += 1;
-  }
+  write_partials<1>(v, part_pap, sm);
 }
 
-__global__ void __launch_bounds__(256) k_cg_update(int F, CgParams cp, const int* __restrict__ cell_of,
+__global__ void __launch_bounds__(256) k_cg_update(int F, int it, CgParams cp,
+                                                   const int* __restrict__ cell_of,
                                                    const uint8_t* __restrict__ info,
                                                    float* __restrict__ r, const float* __restrict__ ap,
                                                    const float* __restrict__ pg, float* __restrict__ xg,
-                                                   double* partials, CgScalars* sc) {
+                                                   const double* __restrict__ part_pap,
+                                                   double* __restrict__ part_rz, CgScalars* sc) {
+  __shared__ double sm[2 * 4];
   if (sc->done) return;
-  const float alpha = sc->alpha;
+  double pap[1];
+  sum_partials<1>(part_pap, gridDim.x, pap, sm);
+  const float alpha = sc->rz[it & 1] / (float)pap[0];
   double v[2] = {0.0, 0.0};
   for (int k = blockIdx.x * 256 + threadIdx.x; k < F; k += gridDim.x * 256) {
     const int c = cell_of[k];
@@ -295,29 +291,39 @@ __global__ void __launch_bounds__(256) k_cg_update(int F, CgParams cp, const int
     v[0] += (double)(rk * rk);
     v[1] += (double)(rk * z);
   }
-  if (last_block_total<2>(v, partials, &sc->counter) && threadIdx.x == 0) {
-    const float rr = (float)v[0];
-    sc->rr = rr;
-    sc->err = sqrt((double)rr / sc->rhs_norm2);
-    if (rr < (float)sc->threshold) {
-      sc->done = 1;
-      return;
-    }
-    const float abs_old = (float)sc->abs_new;
-    const float abs_new = (float)v[1];
-    sc->abs_new = abs_new;
-    sc->beta = abs_new / abs_old;
-    sc->iter += 1;
-    if (sc->iter >= cp.max_iter) sc->done = 1;
-  }
+  write_partials<2>(v, part_rz, sm);
 }
 
-__global__ void __launch_bounds__(256) k_cg_pupdate(int F, CgParams cp, const int* __restrict__ cell_of,
+__global__ void __launch_bounds__(256) k_cg_pupdate(int F, int it, CgParams cp,
+                                                    const int* __restrict__ cell_of,
                                                     const uint8_t* __restrict__ info,
                                                     const float* __restrict__ r, float* __restrict__ pg,
-                                                    const CgScalars* sc) {
+                                                    const double* __restrict__ part_rz, CgScalars* sc) {
+  __shared__ double sm[2 * 4];
   if (sc->done) return;
-  const float beta = sc->beta;
+  double v[2];
+  sum_partials<2>(part_rz, gridDim.x, v, sm);
+  const float rr = (float)v[0];
+  const bool conv = rr < (float)sc->threshold;
+  const float abs_old = sc->rz[it & 1];
+  const float abs_new = (float)v[1];
+  const float beta = abs_new / abs_old;
+  const bool last = !conv && it + 1 >= cp.max_iter;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->rr = rr;
+    sc->err = sqrt((double)rr / sc->rhs_norm2);
+    sc->spmv = it + 1;
+    if (conv) {
+      sc->iter = it;
+      sc->done = 1;
+    } else {
+      sc->iter = it + 1;
+      sc->rz[(it + 1) & 1] = abs_new;
+      sc->beta = beta;
+      if (last) sc->done = 1;
+    }
+  }
+  if (conv || last) return;
   for (int k = blockIdx.x * 256 + threadIdx.x; k < F; k += gridDim.x * 256) {
     const int c = cell_of[k];
     const float z = sel5((info[k] >> 4) & 7, cp.invd_tab) * r[k];
@@ -328,7 +334,7 @@ __global__ void __launch_bounds__(256) k_cg_pupdate(int F, CgParams cp, const in
 static inline int cg_blocks(int F) {
   int nb = (F + 1023) / 1024;  // ~4 unknowns per thread
   if (nb < 1) nb = 1;
-  if (nb > 2048) nb = 2048;
+  if (nb > 1024) nb = 1024;
   return nb;
 }
 
@@ -346,315 +352,32 @@ void launch_compact(hipStream_t s, const Geom& g, const uint8_t* label, const in
   k_compact<<<(n + kScanChunk - 1) / kScanChunk, 256, 0, s>>>(g, label, offsets, fluid_index,
                                                                cell_of, uf, vf, b, info);
 }
-void launch_cg_mean(hipStream_t s, int F, const float* b, double* partials, CgScalars* sc) {
-  k_cg_mean<<<cg_blocks(F), 256, 0, s>>>(F, b, partials, sc);
+void launch_updown(hipStream_t s, int nx, int F_bound, const int* n_fluid, const int* cell_of,
+                   const uint8_t* info, const int* fluid_index, int2* updn) {
+  int nb = (F_bound + 255) / 256;
+  if (nb > 4096) nb = 4096;
+  if (nb < 1) nb = 1;
+  k_updown<<<nb, 256, 0, s>>>(nx, n_fluid, cell_of, info, fluid_index, updn);
 }
 void launch_cg_init(hipStream_t s, int F, const CgParams& cp, const int* cell_of,
                     const uint8_t* info, const float* b, float* r, float* pg, float* xg,
                     double* partials, CgScalars* sc) {
-  k_cg_init<<<cg_blocks(F), 256, 0, s>>>(F, cp, cell_of, info, b, r, pg, xg, partials, sc);
-}
-void launch_cg_iteration(hipStream_t s, int F, int nx, const CgParams& cp, const int* cell_of,
-                         const uint8_t* info, float* r, float* ap, float* pg, float* xg,
-                         double* partials, CgScalars* sc) {
   const int nb = cg_blocks(F);
-  k_cg_spmv<<<nb, 256, 0, s>>>(F, nx, cp, cell_of, info, pg, ap, partials, sc);
-  k_cg_update<<<nb, 256, 0, s>>>(F, cp, cell_of, info, r, ap, pg, xg, partials, sc);
-  k_cg_pupdate<<<nb, 256, 0, s>>>(F, cp, cell_of, info, r, pg, sc);
+  // partials layout: [0, 4096) mean, [4096, 8192) r.z sets (2 x 2 values), [12288,...) p.Ap
+  double* mean_part = partials;
+  if (cp.project_mean) k_cg_sum_b<<<nb, 256, 0, s>>>(F, b, mean_part);
+  k_cg_init<<<nb, 256, 0, s>>>(F, cp, cell_of, info, b, r, pg, xg, mean_part, partials + 4096, sc);
 }
-
-// ----------------------------------------------------------- persistent form
-// All iterations of one solve in one cooperative launch.  Each block owns a
-// contiguous range of unknowns.  One iteration = two grid-wide reduce-barriers:
-//   A: p_k = z_k + beta p_{k-1} for own cells and (recomputed) for neighbours,
-//      publish p_k, Ap_k, partial p.Ap                        -> barrier, alpha
-//   B: x += alpha p, r -= alpha Ap, z = r/diag published, partial |r|^2, r.z
-//                                                             -> barrier, beta
-// Phase A reads z (published before barrier B) and p_{k-1} (published in the
-// previous phase A, before barrier A), and writes p_k into the other ping-pong
-// buffer, so no third barrier is needed.
-//
-// The refined form wraps the FP32 inner solve in FP64 iterative refinement:
-// r64 = b - A x64 (FP64 SpMV on x64 published in grid layout), inner solve
-// A e = r64 in FP32 to a relative tolerance, x64 += e.
-
-// Barrier + sum of NV doubles over all blocks.  Slots are double-buffered by the
-// barrier's parity with a fixed stride of kSlotNV values per block, so the two sets
-// never overlap whatever NV the neighbouring barriers use: a block cannot reuse a
-// set before every block has left the barrier that read it.  Returns the totals in
-// every thread.
-constexpr int kSlotNV = 4;
-template <int NV>
-__device__ __forceinline__ void grid_sum(double (&v)[NV], double* slots, unsigned* bar,
-                                         unsigned& epoch, double* sm) {
-  static_assert(NV <= kSlotNV, "grid_sum: too many values");
-  block_sum<NV>(v, sm);
-  const unsigned G = gridDim.x;
-  double* set = slots + (size_t)(epoch & 1) * G * kSlotNV;
-  epoch += 1;
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int q = 0; q < NV; ++q) set[blockIdx.x * NV + q] = v[q];
-    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned target = epoch * G;
-    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
-      __builtin_amdgcn_s_sleep(1);
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  }
-  __syncthreads();
-  // every block sums the same slots in the same order
-  double acc[NV];
-#pragma unroll
-  for (int q = 0; q < NV; ++q) acc[q] = 0.0;
-  for (int b = threadIdx.x; b < (int)G; b += blockDim.x)
-#pragma unroll
-    for (int q = 0; q < NV; ++q)
-      acc[q] += __hip_atomic_load(&set[b * NV + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  block_sum<NV>(acc, sm);
-  __shared__ double bc[NV];
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int q = 0; q < NV; ++q) bc[q] = acc[q];
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < NV; ++q) v[q] = bc[q];
-  __syncthreads();
-}
-
-// Plain barrier (no data): used where only publication order matters.
-__device__ __forceinline__ void grid_barrier(unsigned* bar, unsigned& epoch) {
-  __syncthreads();
-  epoch += 1;
-  if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned target = epoch * gridDim.x;
-    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
-      __builtin_amdgcn_s_sleep(1);
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  }
-  __syncthreads();
-}
-
-// One FP32 Jacobi-PCG solve of A e = rhs over this block's unknowns [k0, k1).
-// rhs is r (compact, overwritten with the running residual); the solution is
-// accumulated in xg (grid layout, zeroed here).  Returns the iteration count in
-// Eigen's convention; sets *spmv_out and the final relative residual.
-__device__ int pcg_inner(const PersistentArgs& a, int F, int k0, int k1, float tol, int max_iter,
-                         unsigned& epoch, double* sm, int* spmv_out, double* err_out,
-                         double* rhs_out) {
-  const CgParams& cp = a.cp;
-  const int nx = a.nx;
-  double v2[2] = {0.0, 0.0};
-  for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
-    const int c = a.cell_of[k];
-    const float rk = a.r[k];
-    const float z = sel5((a.info[k] >> 4) & 7, cp.invd_tab) * rk;
-    a.zg[c] = z;
-    a.xg[c] = 0.f;
-    v2[0] += (double)(rk * rk);
-    v2[1] += (double)(rk * z);
-  }
-  grid_sum<2>(v2, a.partials, a.bar, epoch, sm);
-  const float rhs = (float)v2[0];
-  *rhs_out = v2[0];
-  *spmv_out = 0;
-  if (rhs == 0.f) {
-    *err_out = 0.0;
-    return 0;
-  }
-  float thr = tol * tol * rhs;
-  if (thr < FLT_MIN) thr = FLT_MIN;
-  float rr = rhs;
-  if (rr < thr) {
-    *err_out = 1.0;
-    return 0;
-  }
-  float abs_new = (float)v2[1];
-  float beta = 0.f;
-  float* pold = a.pg0;
-  float* pnew = a.pg1;
-  int it = 0, spmv = 0;
-  while (true) {
-    // phase A
-    double v1[1] = {0.0};
-    for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
-      const int c = a.cell_of[k];
-      const uint8_t inf = a.info[k];
-      float pc, acc;
-      if (it == 0) {
-        pc = a.zg[c];
-        acc = sel5((inf >> 4) & 7, cp.diag_tab) * pc;
-        if (inf & 1) acc += cp.off * a.zg[c - 1];
-        if (inf & 2) acc += cp.off * a.zg[c + 1];
-        if (inf & 4) acc += cp.off * a.zg[c - nx];
-        if (inf & 8) acc += cp.off * a.zg[c + nx];
-      } else {
-        pc = a.zg[c] + beta * pold[c];
-        acc = sel5((inf >> 4) & 7, cp.diag_tab) * pc;
-        if (inf & 1) acc += cp.off * (a.zg[c - 1] + beta * pold[c - 1]);
-        if (inf & 2) acc += cp.off * (a.zg[c + 1] + beta * pold[c + 1]);
-        if (inf & 4) acc += cp.off * (a.zg[c - nx] + beta * pold[c - nx]);
-        if (inf & 8) acc += cp.off * (a.zg[c + nx] + beta * pold[c + nx]);
-      }
-      pnew[c] = pc;
-      a.e32[k] = acc;  // Ap, read back by the same thread in phase B
-      v1[0] += (double)(pc * acc);
-    }
-    grid_sum<1>(v1, a.partials, a.bar, epoch, sm);
-    ++spmv;
-    const float alpha = abs_new / (float)v1[0];
-    // phase B
-    v2[0] = 0.0;
-    v2[1] = 0.0;
-    for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
-      const int c = a.cell_of[k];
-      const float pc = pnew[c];
-      a.xg[c] = a.xg[c] + alpha * pc;
-      const float rk = a.r[k] - alpha * a.e32[k];
-      a.r[k] = rk;
-      const float z = sel5((a.info[k] >> 4) & 7, cp.invd_tab) * rk;
-      a.zg[c] = z;
-      v2[0] += (double)(rk * rk);
-      v2[1] += (double)(rk * z);
-    }
-    grid_sum<2>(v2, a.partials, a.bar, epoch, sm);
-    rr = (float)v2[0];
-    if (rr < thr) break;
-    const float abs_old = abs_new;
-    abs_new = (float)v2[1];
-    beta = abs_new / abs_old;
-    ++it;
-    if (it >= max_iter) break;
-    float* t = pold;
-    pold = pnew;
-    pnew = t;
-  }
-  *spmv_out = spmv;
-  *err_out = sqrt((double)rr / (double)rhs);
-  return it;
-}
-
-__global__ void __launch_bounds__(256) k_cg_persistent(PersistentArgs a) {
-  __shared__ double sm[8];
-  const int F = *a.n_fluid;
-  CgScalars* sc = a.sc;
-  if (F == 0) return;  // uniform across the grid: no barrier is entered
-  const int G = gridDim.x;
-  const int per = (F + G - 1) / G;
-  const int k0 = min(F, (int)blockIdx.x * per);
-  const int k1 = min(F, k0 + per);
-  unsigned epoch = 0;
-  const CgParams& cp = a.cp;
-  const int nx = a.nx;
-
-  // optional mean removal of b (fully Neumann systems)
-  float mean = 0.f;
-  if (cp.project_mean) {
-    double v[1] = {0.0};
-    for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) v[0] += a.b[k];
-    grid_sum<1>(v, a.partials, a.bar, epoch, sm);
-    mean = (float)v[0] / (float)F;
-  }
-
-  if (!a.refined) {
-    for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x)
-      a.r[k] = cp.project_mean ? a.b[k] - mean : a.b[k];
-    int spmv = 0;
-    double err = 0.0, rhs = 0.0;
-    const int it = pcg_inner(a, F, k0, k1, cp.tol, cp.max_iter, epoch, sm, &spmv, &err, &rhs);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      sc->iter = it;
-      sc->spmv = spmv;
-      sc->err = err;
-      sc->rhs_norm2 = rhs;
-      sc->refine_sweeps = 0;
-      sc->total_unknown_iters += (unsigned long long)F * (unsigned long long)spmv;
-      sc->total_solves += 1;
-    }
-    return;
-  }
-
-  // ---- FP64 iterative refinement
-  double v[1] = {0.0};
-  for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
-    const double bk = (double)(cp.project_mean ? a.b[k] - mean : a.b[k]);
-    a.x64[k] = 0.0;
-    a.r[k] = (float)bk;
-    v[0] += bk * bk;
-  }
-  grid_sum<1>(v, a.partials, a.bar, epoch, sm);
-  const double b2 = v[0];
-  double res2 = b2;
-  int total_it = 0, total_spmv = 0, sweeps = 0;
-  if (b2 > 0.0) {
-    const double thr2 = a.outer_tol * a.outer_tol * b2;
-    while (res2 >= thr2 && sweeps < a.max_outer) {
-      int spmv = 0;
-      double err = 0.0, rhs = 0.0;
-      total_it += pcg_inner(a, F, k0, k1, a.inner_tol, cp.max_iter, epoch, sm, &spmv, &err, &rhs);
-      total_spmv += spmv;
-      ++sweeps;
-      // x64 += e ; publish x64 in grid layout via zg/pg0 pair (hi/lo split)
-      for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
-        const int c = a.cell_of[k];
-        const double x = a.x64[k] + (double)a.xg[c];
-        a.x64[k] = x;
-        const float hi = (float)x;
-        a.zg[c] = hi;
-        a.pg0[c] = (float)(x - (double)hi);
-      }
-      grid_barrier(a.bar, epoch);
-      // r64 = b - A x64 in FP64
-      double w[1] = {0.0};
-      const double off = (double)cp.off;
-      for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
-        const int c = a.cell_of[k];
-        const uint8_t inf = a.info[k];
-        auto X = [&](int cc) { return (double)a.zg[cc] + (double)a.pg0[cc]; };
-        double ax = (double)sel5((inf >> 4) & 7, cp.diag_tab) * X(c);
-        if (inf & 1) ax += off * X(c - 1);
-        if (inf & 2) ax += off * X(c + 1);
-        if (inf & 4) ax += off * X(c - nx);
-        if (inf & 8) ax += off * X(c + nx);
-        const double bk = (double)(cp.project_mean ? a.b[k] - mean : a.b[k]);
-        const double rk = bk - ax;
-        a.r[k] = (float)rk;
-        w[0] += rk * rk;
-      }
-      grid_sum<1>(w, a.partials, a.bar, epoch, sm);
-      res2 = w[0];
-      total_spmv += 1;
-    }
-  }
-  // final solution into xg (FP32) for the gradient update
-  for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) a.xg[a.cell_of[k]] = (float)a.x64[k];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    sc->iter = total_it;
-    sc->spmv = total_spmv;
-    sc->err = b2 > 0.0 ? sqrt(res2 / b2) : 0.0;
-    sc->rhs_norm2 = b2;
-    sc->b64_norm2 = b2;
-    sc->res64_norm2 = res2;
-    sc->refine_sweeps = sweeps;
-    sc->total_unknown_iters += (unsigned long long)F * (unsigned long long)total_spmv;
-    sc->total_solves += 1;
-  }
-}
-
-int persistent_grid_size(int device) {
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 0;
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cg_persistent, 256, 0) !=
-      hipSuccess)
-    return 0;
-  if (per_cu < 1) return 0;
-  return prop.multiProcessorCount * (per_cu > 1 ? 1 : per_cu);
-}
-
-hipError_t launch_cg_persistent(hipStream_t s, const PersistentArgs& a, int grid) {
-  void* args[] = {(void*)&a};
-  return hipLaunchCooperativeKernel((const void*)k_cg_persistent, dim3(grid), dim3(256), args, 0, s);
+void launch_cg_iteration(hipStream_t s, int F, int nx, int it, const CgParams& cp,
+                         const int* cell_of, const uint8_t* info, float* r, float* ap, float* pg,
+                         float* xg, double* partials, CgScalars* sc) {
+  const int nb = cg_blocks(F);
+  double* rz_prev = partials + 4096 + (it & 1) * 2048;
+  double* rz_next = partials + 4096 + ((it + 1) & 1) * 2048;
+  double* pap = partials + 8192;
+  k_cg_spmv<<<nb, 256, 0, s>>>(F, nx, it, cp, cell_of, info, pg, ap, rz_prev, pap, sc);
+  k_cg_update<<<nb, 256, 0, s>>>(F, it, cp, cell_of, info, r, ap, pg, xg, pap, rz_next, sc);
+  k_cg_pupdate<<<nb, 256, 0, s>>>(F, it, cp, cell_of, info, r, pg, rz_next, sc);
 }
 
 }  // namespace fsg

@@ -53,7 +53,8 @@ class Params(C.Structure):
         ("cg_max_iter", C.c_int32), ("ext_layers", C.c_int32), ("integrator", C.c_int32),
         ("solver", C.c_int32), ("project_mean", C.c_int32), ("check_every", C.c_int32),
         ("refine_inner_tol", C.c_double), ("refine_max_outer", C.c_int32),
-        ("record_ext_info", C.c_int32), ("reserved", C.c_int32 * 4),
+        ("record_ext_info", C.c_int32), ("pcg_blocks", C.c_int32), ("pcg_kpt", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 

@@ -101,7 +101,9 @@ typedef struct fsg_params {
   double refine_inner_tol;/* refined solver: relative tolerance of each FP32 inner solve */
   int32_t refine_max_outer;/* refined solver: maximum FP64 refinement sweeps */
   int32_t record_ext_info; /* 1: fsg_step also records the extension sources */
-  int32_t reserved[4];
+  int32_t pcg_blocks;     /* persistent solvers: blocks (0: automatic) */
+  int32_t pcg_kpt;        /* persistent solvers: unknowns per thread, 1/2/4/8 (0: automatic) */
+  int32_t reserved[2];
 } fsg_params;
 
 /* Counters of the last pressure solve and running totals. */
