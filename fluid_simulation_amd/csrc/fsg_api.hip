@@ -336,6 +336,7 @@ int solve_pressure(fsg_domain* h) {
   a.max_outer = h->prm.refine_max_outer;
   a.inner_tol = (float)h->prm.refine_inner_tol;
   a.outer_tol = h->prm.cg_tol;
+  a.warm_start = h->prm.warm_start != 0;
   timed(h, "zero_barrier", [&] { launch_fill(h->stream, reinterpret_cast<float*>(h->bar), 4, 0.f); });
   hipError_t e = hipSuccess;
   timed(h, a.refined ? "cg_refined" : "cg_persistent", [&] { e = launch_pcg(h->stream, a, grid, kpt); });

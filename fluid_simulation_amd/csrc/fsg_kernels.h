@@ -25,7 +25,7 @@ void launch_extend_sweep(hipStream_t s, const Geom& g, const uint8_t* label, flo
 void launch_classify_levelset(hipStream_t s, const Geom& g, const float* phi,
                               const uint8_t* solid, uint8_t* label);
 void launch_pressure_update(hipStream_t s, const Geom& g, const int* n_fluid,
-                            const uint8_t* label, const float* xg, float* au, float* av,
+                            const uint8_t* label, float* xg, float* au, float* av,
                             float* bu, float* bv, float dt_over_density, int do_dirichlet,
                             int do_diff, const float* prev_u, const float* prev_v, float* diff_u,
                             float* diff_v);
@@ -110,6 +110,7 @@ struct PersistentArgs {
   int max_outer;
   float inner_tol;
   double outer_tol;
+  int warm_start;        // 1: x0 = previous solution (xg), else x0 = 0
 };
 int pcg_max_kpt();
 size_t pcg_lds_bytes(int F, int grid, int nx);

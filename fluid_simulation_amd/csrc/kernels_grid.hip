@@ -244,9 +244,11 @@ __global__ void __launch_bounds__(256) k_classify_levelset(Geom g, const float* 
 //  F = 0: the reference returns before solving and does not swap (:347-350); the
 //         same state is produced by exchanging A and B here (B := Dirichlet(A),
 //         A := B) ahead of the host's swap.
+// xg is zeroed at cells that are not LIQUID, so that it can seed the next solve
+// (each thread writes only its own cell, and only where no thread reads it).
 __global__ void __launch_bounds__(256) k_pressure_update(
     Geom g, const int* __restrict__ n_fluid, const uint8_t* __restrict__ label,
-    const float* __restrict__ xg, float* __restrict__ au, float* __restrict__ av,
+    float* __restrict__ xg, float* __restrict__ au, float* __restrict__ av,
     float* __restrict__ bu, float* __restrict__ bv, float dt_over_density, int do_dirichlet,
     int do_diff, const float* __restrict__ prev_u, const float* __restrict__ prev_v,
     float* __restrict__ diff_u, float* __restrict__ diff_v) {
@@ -278,6 +280,7 @@ __global__ void __launch_bounds__(256) k_pressure_update(
     au[c] = bu[c];
     av[c] = bv[c];
   }
+  if (L != kLiquid) xg[c] = 0.f;
   if (do_dirichlet) {
     if ((Lim == kSolid && nu < 0.f) || (L == kSolid && nu > 0.f)) nu = 0.f;
     if ((Ljm == kSolid && nv < 0.f) || (L == kSolid && nv > 0.f)) nv = 0.f;
@@ -483,7 +486,7 @@ void launch_classify_levelset(hipStream_t s, const Geom& g, const float* phi,
   k_classify_levelset<<<nblocks((int64_t)g.nx * g.ny), 256, 0, s>>>(g, phi, solid, label);
 }
 void launch_pressure_update(hipStream_t s, const Geom& g, const int* n_fluid,
-                            const uint8_t* label, const float* xg, float* au, float* av,
+                            const uint8_t* label, float* xg, float* au, float* av,
                             float* bu, float* bv, float dt_over_density, int do_dirichlet,
                             int do_diff, const float* prev_u, const float* prev_v, float* diff_u,
                             float* diff_v) {

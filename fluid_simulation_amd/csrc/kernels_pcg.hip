@@ -161,25 +161,40 @@ __global__ void __launch_bounds__(kPcgThreads) k_pcg(PersistentArgs a) {
   float x[KPT], r[KPT], ap[KPT];
   int total_it = 0, total_spmv = 0, sweeps = 0;
   double err = 0.0, b2 = 0.0, res2 = 0.0, thr2 = 0.0;
-  if (REFINED) {
-    double v[1] = {0.0};
+  // Start: x0 = 0 (Eigen's solve()), or with warm_start the previous solution kept
+  // in xg (zero at cells that were not LIQUID).  r0 = b - A x0 is formed in FP64
+  // from xg, which nothing writes during the solve.
+  {
+    const double off = (double)cp.off;
+    double v[2] = {0.0, 0.0};
 #pragma unroll
     for (int j = 0; j < KPT; ++j) {
+      r[j] = 0.f;
       if (!FSG_VALID(j)) continue;
       const int k = FSG_K(j);
       const double bk = (double)(cp.project_mean ? a.b[k] - mean : a.b[k]);
-      st_dev(a.x64 + k, 0.0);
+      double rk = bk;
+      double x0 = 0.0;
+      if (a.warm_start) {
+        const int c = a.cell_of[k];
+        const unsigned f = (infw >> (4 * j)) & 15u;
+        x0 = (double)a.xg[c];
+        double ax = (double)sel5((nnsw >> (3 * j)) & 7, cp.diag_tab) * x0;
+        if (f & 1u) ax += off * (double)a.xg[c - 1];
+        if (f & 2u) ax += off * (double)a.xg[c + 1];
+        if (f & 4u) ax += off * (double)a.xg[c - a.nx];
+        if (f & 8u) ax += off * (double)a.xg[c + a.nx];
+        rk = bk - ax;
+      }
+      if (REFINED) st_dev(a.x64 + k, x0);
+      r[j] = (float)rk;
       v[0] += bk * bk;
+      v[1] += rk * rk;
     }
-    if (!grid_reduce<1>(bs, v, sm)) return;
+    if (!grid_reduce<2>(bs, v, sm)) return;
     b2 = v[0];
-    res2 = b2;
+    res2 = v[1];
     thr2 = a.outer_tol * a.outer_tol * b2;
-  }
-#pragma unroll
-  for (int j = 0; j < KPT; ++j) {
-    const int k = FSG_K(j);
-    r[j] = FSG_VALID(j) ? (cp.project_mean ? a.b[k] - mean : a.b[k]) : 0.f;
   }
 
   const float tol = REFINED ? a.inner_tol : cp.tol;
@@ -201,9 +216,12 @@ __global__ void __launch_bounds__(kPcgThreads) k_pcg(PersistentArgs a) {
     const float rhs = (float)v2[0];
     int it = 0, spmv = 0;
     float rr = rhs;
-    float thr = tol * tol * rhs;
+    // Eigen's test |r| < tol |b|; the refined form's inner solves are relative to
+    // their own right-hand side (the current residual)
+    const float ref = REFINED ? rhs : (float)b2;
+    float thr = tol * tol * ref;
     if (thr < FLT_MIN) thr = FLT_MIN;
-    if (rhs != 0.f && !(rr < thr)) {
+    if (ref != 0.f && rhs != 0.f && !(rr < thr)) {
       float abs_new = (float)v2[1];
       float beta = 0.f;
       int par = 0;  // p_k goes to buffer par, p_{k-1} is read from par ^ 1
@@ -277,7 +295,7 @@ __global__ void __launch_bounds__(kPcgThreads) k_pcg(PersistentArgs a) {
     }
     total_it += it;
     total_spmv += spmv;
-    err = rhs != 0.f ? sqrt((double)rr / (double)rhs) : 0.0;
+    err = ref != 0.f ? sqrt((double)rr / (double)ref) : 0.0;
     if (!REFINED) break;
     // ------------- FP64 refinement: x64 += e; r = b - A x64 -------------
     ++sweeps;
@@ -318,7 +336,12 @@ __global__ void __launch_bounds__(kPcgThreads) k_pcg(PersistentArgs a) {
   for (int j = 0; j < KPT; ++j) {
     if (!FSG_VALID(j)) continue;
     const int k = FSG_K(j);
-    a.xg[a.cell_of[k]] = REFINED ? (float)ld_dev(a.x64 + k) : x[j];
+    const int c = a.cell_of[k];
+    float xk;
+    if (b2 == 0.0) xk = 0.f;  // Eigen: |b| = 0 gives x = 0
+    else if (REFINED) xk = (float)ld_dev(a.x64 + k);
+    else xk = a.warm_start ? a.xg[c] + x[j] : x[j];
+    a.xg[c] = xk;
   }
 #undef FSG_VALID
 #undef FSG_K

@@ -54,7 +54,7 @@ class Params(C.Structure):
         ("solver", C.c_int32), ("project_mean", C.c_int32), ("check_every", C.c_int32),
         ("refine_inner_tol", C.c_double), ("refine_max_outer", C.c_int32),
         ("record_ext_info", C.c_int32), ("pcg_blocks", C.c_int32), ("pcg_kpt", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("warm_start", C.c_int32), ("reserved", C.c_int32 * 1),
     ]
 
 
@@ -326,6 +326,25 @@ class Domain:
 
     def reset_timing(self):
         self._chk(self._L.fsg_reset_timing(self._h), "fsg_reset_timing")
+
+    # -- checkpoint ---------------------------------------------------------------
+    def save_state(self) -> dict:
+        """The state carried from one step to the next: particles, front and back
+        velocity buffers, prev and diff grids, labels (the buffers the reference
+        keeps alive across steps, include/MacGrid.h:162-176)."""
+        st = {"particles": self.particles(), "labels": self.labels()}
+        for name, which in (("front", VEL_FRONT), ("back", VEL_BACK), ("prev", VEL_PREV),
+                            ("diff", VEL_DIFF)):
+            st[name] = self.velocity(which)
+        return st
+
+    def load_state(self, st: dict):
+        self.set_particles(st["particles"])
+        for name, which in (("front", VEL_FRONT), ("back", VEL_BACK), ("prev", VEL_PREV),
+                            ("diff", VEL_DIFF)):
+            u, v = st[name]
+            self.set_velocity(u, v, which)
+        self.set_labels(st["labels"])
 
     # -- construction from a synthetic scene -----------------------------------
     @classmethod

@@ -103,7 +103,10 @@ typedef struct fsg_params {
   int32_t record_ext_info; /* 1: fsg_step also records the extension sources */
   int32_t pcg_blocks;     /* persistent solvers: blocks (0: automatic) */
   int32_t pcg_kpt;        /* persistent solvers: unknowns per thread, 1/2/4/8 (0: automatic) */
-  int32_t reserved[2];
+  int32_t warm_start;     /* persistent solvers: 1 starts CG from the previous step's
+                             pressure (same stopping test |r| < tol |b|); 0 (default)
+                             starts from x0 = 0 like Eigen's solve() */
+  int32_t reserved[1];
 } fsg_params;
 
 /* Counters of the last pressure solve and running totals. */

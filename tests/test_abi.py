@@ -38,3 +38,21 @@ def test_params_struct_layout_and_defaults():
 
 def test_version_string():
     assert b"gfx950" in fsg.load().fsg_version()
+
+
+def test_cpp_layer_loads_and_exports_reference_api():
+    """libfluidsim_amd.so (the drop-in C++ classes) loads on top of libfsg.so and
+    exports the reference's public C++ entry points (include/FluidSolver.h:51-54,
+    include/FluidDomain.h:40-46, include/MacGrid.h)."""
+    path = os.path.join(ROOT, "fluid_simulation_amd", "libfluidsim_amd.so")
+    assert os.path.exists(path), "libfluidsim_amd.so is not built"
+    lib = ctypes.CDLL(path)
+    for sym in ("_ZN11FluidSolver18stepSemiLagrangianER11FluidDomainf",
+                "_ZN11FluidSolver7stepPICER11FluidDomainf",
+                "_ZN11FluidSolver8stepFLIPER11FluidDomainf",
+                "_ZN11FluidSolver11stepPICFLIPER11FluidDomainf",
+                "_ZN11FluidDomainC1Eiiffff",
+                "_ZN7MacGridC1Eiiff",
+                "_ZN11FluidDomain13classifyCellsER17MarkerParticleSet",
+                "_ZN17MarkerParticleSet6advectEf"):
+        assert hasattr(lib, sym), sym
