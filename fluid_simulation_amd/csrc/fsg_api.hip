@@ -115,6 +115,10 @@ struct fsg_domain {
   int max_coop_blocks = 0;  // co-resident 1024-thread blocks of the persistent solver
   size_t pcg_lds_max = 0;   // dynamic LDS the persistent solver may use per block
   int last_F = 0, last_grid = 0, last_kpt = 0;
+  // multigrid form: hierarchy allocated on first use; iterations of the last solve
+  MgHost mg{};
+  bool mg_ready = false;
+  int mg_last_iters = 0;
   // host-side statistics of the plain solver
   fsg_stats host_stats{};
   std::string err;
@@ -233,6 +237,109 @@ void extend_sweeps(fsg_domain* h, bool record_info) {
   swap_vel(h);
 }
 
+// Multigrid hierarchy: halve (rounding up) until both sides are <= 4 cells; the
+// levels from the first one of <= kMgCoarseCells cells run in one workgroup.
+int mg_prepare(fsg_domain* h) {
+  if (h->mg_ready) return FSG_OK;
+  MgHost& m = h->mg;
+  m.nx[0] = h->g.nx;
+  m.ny[0] = h->g.ny;
+  m.nlev = 1;
+  while ((m.nx[m.nlev - 1] > 4 || m.ny[m.nlev - 1] > 4) && m.nlev < kMgMaxLevels) {
+    m.nx[m.nlev] = (m.nx[m.nlev - 1] + 1) / 2;
+    m.ny[m.nlev] = (m.ny[m.nlev - 1] + 1) / 2;
+    ++m.nlev;
+  }
+  m.lc = 0;
+  while ((size_t)m.nx[m.lc] * m.ny[m.lc] > (size_t)kMgCoarseCells) ++m.lc;
+  if (m.lc >= m.nlev)
+    return fail(h, FSG_E_INVALID, "multigrid solver: grid too elongated for the hierarchy");
+  bool ok = true;
+  for (int l = 0; l < m.nlev; ++l) {
+    const size_t n = (size_t)m.nx[l] * m.ny[l];
+    if (l > 0) ok &= dalloc(h, &m.lab[l], n) == hipSuccess;
+    ok &= dalloc(h, &m.inf[l], n) == hipSuccess;
+    ok &= dalloc(h, &m.rl[l], n) == hipSuccess;
+    ok &= dalloc(h, &m.xs[l], n) == hipSuccess;
+    ok &= dalloc(h, &m.xo[l], n) == hipSuccess;
+  }
+  const size_t C = ncell(h);
+  ok &= dalloc(h, &m.b, C) == hipSuccess;
+  ok &= dalloc(h, &m.x[0], C) == hipSuccess;
+  ok &= dalloc(h, &m.x[1], C) == hipSuccess;
+  ok &= dalloc(h, &m.p[0], C) == hipSuccess;
+  ok &= dalloc(h, &m.p[1], C) == hipSuccess;
+  ok &= dalloc(h, &m.part_sum, kMgMaxVecBlocks) == hipSuccess;
+  ok &= dalloc(h, &m.part_cnt, kMgMaxVecBlocks) == hipSuccess;
+  ok &= dalloc(h, &m.part_bb, kMgMaxVecBlocks) == hipSuccess;
+  ok &= dalloc(h, &m.part_rr, kMgMaxVecBlocks) == hipSuccess;
+  ok &= dalloc(h, &m.part_pq, kMgMaxVecBlocks) == hipSuccess;
+  ok &= dalloc(h, &m.part_rz, (size_t)std::max(mg_up_blocks(m.nx[0], m.ny[0]), 1)) == hipSuccess;
+  if (!ok) return fail(h, FSG_E_NOMEM, "multigrid solver: device allocation failed");
+  m.lab[0] = h->label;
+  MgCoarseArgs& a = m.coarse;
+  a.lc = m.lc;
+  a.nlev = m.nlev;
+  a.omega = 0.8f;
+  a.sweeps = 2 * std::max(m.nx[m.nlev - 1], m.ny[m.nlev - 1]) + 8;
+  int off = 0;
+  for (int l = 0; l < m.nlev; ++l) {
+    a.nx[l] = m.nx[l];
+    a.ny[l] = m.ny[l];
+    a.inf[l] = m.inf[l];
+    a.off[l] = off;
+    if (l >= m.lc) off += m.nx[l] * m.ny[l];
+  }
+  // offsets count from level lc
+  for (int l = 0; l < m.nlev; ++l) a.off[l] = l >= m.lc ? a.off[l] : 0;
+  a.total = off;
+  a.r_in = m.rl[m.lc];
+  a.x_out = m.xo[m.lc];
+  // The one-workgroup levels hold at most ~2 x kMgCoarseCells cells (13 bytes each);
+  // the kernel's LDS limit is raised once, process-wide, to a fixed 128 KiB.
+  constexpr size_t kCoarseLdsLimit = 128 * 1024;
+  m.coarse_lds = (size_t)off * (3 * sizeof(float) + 1);
+  if (m.coarse_lds > kCoarseLdsLimit)
+    return fail(h, FSG_E_INVALID, "multigrid solver: coarse levels exceed the LDS budget");
+  if (mg_set_lds_limit(kCoarseLdsLimit) != hipSuccess)
+    return fail(h, FSG_E_HIP, "multigrid solver: cannot raise the LDS limit of the coarse kernel");
+  h->mg_ready = true;
+  return FSG_OK;
+}
+
+// CG preconditioned by one V-cycle per iteration (FSG_SOLVER_MG).  Iterations are
+// enqueued in batches (the first sized by the previous solve's count); every
+// kernel returns at once after the device-side stop test has fired.
+int mg_solve(fsg_domain* h, int F) {
+  if (int rc = mg_prepare(h)) return rc;
+  MgHost& m = h->mg;
+  timed(h, "cg_mg_setup", [&] {
+    launch_mg_setup(h->stream, h->g, m, h->label, UF(h), VF(h), h->prm.project_mean, h->sc);
+  });
+  const int max_iter = h->prm.cg_max_iter;
+  const double tol = h->prm.cg_tol;
+  const float omega = m.coarse.omega;
+  int it = 0, batch = std::max(h->mg_last_iters + 1, 4);
+  CgScalars hs{};
+  for (;;) {
+    for (int q = 0; q < batch && it <= max_iter; ++q, ++it)
+      timed(h, "cg_mg_iteration",
+            [&] { launch_mg_iteration(h->stream, m, it, omega, max_iter, tol, h->sc); });
+    HIPCHK(h, hipMemcpyAsync(&hs, h->sc, sizeof(hs), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (hs.done || it > max_iter) break;
+    batch = 2;
+  }
+  timed(h, "cg_mg_finish",
+        [&] { launch_mg_finish(h->stream, m, h->prm.project_mean, h->sc, h->xg); });
+  h->mg_last_iters = hs.iter;
+  h->host_stats.cg_iterations = hs.iter;
+  h->host_stats.cg_error = hs.err;
+  h->host_stats.total_unknown_iterations += (int64_t)F * hs.iter;
+  h->host_stats.total_solves += 1;
+  return FSG_OK;
+}
+
 int solve_pressure(fsg_domain* h) {
   const int nb = (int)((ncell(h) + kScanChunk - 1) / kScanChunk);
   timed(h, "count_liquid", [&] { launch_count_liquid(h->stream, h->g, h->label, h->counts); });
@@ -252,6 +359,7 @@ int solve_pressure(fsg_domain* h) {
   h->last_F = F;
   if (F == 0) return FSG_OK;
   const int solver = h->prm.solver;
+  if (solver == FSG_SOLVER_MG) return mg_solve(h, F);
   if (solver == FSG_SOLVER_PLAIN) {
     const CgParams cp = make_cg_params(h, (float)h->prm.cg_tol);
     timed(h, "cg_init", [&] {
@@ -549,9 +657,10 @@ const char* fsg_last_error(const fsg_domain* h) { return h ? h->err.c_str() : "n
 int fsg_set_params(fsg_domain* h, const fsg_params* p) {
   if (!h || !p) return FSG_E_INVALID;
   if (p->cg_max_iter < 0 || p->ext_layers < 0 || p->ext_layers > 14 || p->integrator < 0 ||
-      p->integrator > 2 || p->solver < 0 || p->solver > 2 || !(p->cg_tol >= 0.0))
+      p->integrator > 2 || p->solver < 0 || p->solver > 3 || !(p->cg_tol >= 0.0))
     return fail(h, FSG_E_INVALID, "fsg_set_params: value out of range");
-  if (p->solver != FSG_SOLVER_PLAIN && h->max_coop_blocks <= 0)
+  if ((p->solver == FSG_SOLVER_PERSISTENT || p->solver == FSG_SOLVER_REFINED) &&
+      h->max_coop_blocks <= 0)
     return fail(h, FSG_E_INVALID, "persistent solver unavailable on this device");
   h->prm = *p;
   return FSG_OK;
@@ -917,7 +1026,7 @@ int fsg_get_reject_flags(fsg_domain* h, uint8_t* flags) {
 int fsg_get_stats(fsg_domain* h, fsg_stats* s) {
   if (!h || !s) return FSG_E_INVALID;
   if (int rc = check_solver_abort(h)) return rc;
-  if (h->prm.solver == FSG_SOLVER_PLAIN) {
+  if (h->prm.solver == FSG_SOLVER_PLAIN || h->prm.solver == FSG_SOLVER_MG) {
     *s = h->host_stats;
     return FSG_OK;
   }

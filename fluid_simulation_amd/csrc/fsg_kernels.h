@@ -62,6 +62,7 @@ struct CgScalars {
   double res64_norm2;    // refined form: |b - A x|^2 in FP64
   unsigned long long total_unknown_iters;
   long long total_solves;
+  double rho[2];         // multigrid form: r.z of the current / previous direction
 };
 
 struct CgParams {
@@ -116,5 +117,46 @@ int pcg_max_kpt();
 size_t pcg_lds_bytes(int F, int grid, int nx);
 hipError_t launch_pcg(hipStream_t s, const PersistentArgs& a, int grid, int kpt);
 hipError_t pcg_set_lds_limit(size_t max_bytes);
+
+// multigrid-preconditioned CG with FP64 solution and residual (dense cell grids)
+constexpr int kMgMaxLevels = 16;
+constexpr int kMgMaxVecBlocks = 1024;  // blocks of the vector kernels (partials per kernel)
+constexpr int kMgCoarseCells = 4096;   // levels at or below this size run in one workgroup
+
+struct MgCoarseArgs {
+  int lc, nlev, total, sweeps;
+  float omega;
+  int nx[kMgMaxLevels], ny[kMgMaxLevels], off[kMgMaxLevels];
+  const uint8_t* inf[kMgMaxLevels];
+  const float* r_in;  // right-hand side of level lc
+  float* x_out;       // cycle result of level lc
+};
+
+struct MgHost {
+  int nlev = 0, lc = 0;
+  int nx[kMgMaxLevels] = {}, ny[kMgMaxLevels] = {};
+  uint8_t* lab[kMgMaxLevels] = {};  // [0]: the domain's labels
+  uint8_t* inf[kMgMaxLevels] = {};
+  float* rl[kMgMaxLevels] = {};     // right-hand side per level ([0]: the CG residual r)
+  float* xs[kMgMaxLevels] = {};     // after pre-smoothing
+  float* xo[kMgMaxLevels] = {};     // after the cycle ([0]: z)
+  double* b = nullptr;              // b' = -dx^2 b (FP64)
+  double* x[2] = {nullptr, nullptr};  // FP64 solution (ping-pong)
+  float* p[2] = {nullptr, nullptr};   // search direction (ping-pong)
+  double *part_sum = nullptr, *part_cnt = nullptr, *part_bb = nullptr, *part_rr = nullptr,
+         *part_pq = nullptr, *part_rz = nullptr;
+  MgCoarseArgs coarse{};
+  size_t coarse_lds = 0;
+};
+
+int mg_vec_blocks(int C);
+int mg_up_blocks(int nx, int ny);
+hipError_t mg_set_lds_limit(size_t bytes);
+void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_t* label,
+                     const float* uf, const float* vf, int project_mean, CgScalars* sc);
+void launch_mg_iteration(hipStream_t s, const MgHost& m, int it, float omega, int max_iter,
+                         double tol, CgScalars* sc);
+void launch_mg_finish(hipStream_t s, const MgHost& m, int project_mean, CgScalars* sc,
+                      float* xg);
 
 }  // namespace fsg

@@ -6,6 +6,8 @@
 // particles are AoS float4 (x, y, vx, vy).  Grid kernels run one thread per cell
 // (both faces of the cell), 256 threads per block over a 1-D tile order remapped
 // so that each XCD owns a contiguous band of rows.
+#include <climits>
+
 #include "fsg_device.h"
 #include "fsg_kernels.h"
 
@@ -40,15 +42,62 @@ __global__ void __launch_bounds__(256) k_cell_init(Geom g, uint8_t* __restrict__
 // weight at (x, y - dy/2), v and weight at (x - dx/2, y), on the pool grids whose
 // y spacing is dx (src/FluidSolver.cpp:56-63).  acc = (u sum, u weight, v sum,
 // v weight) per face.
+//
+// The 16 splat terms of a particle go to an LDS copy of the block's footprint
+// when the bounding box of the block's 256 particles' splat cells fits in
+// kSplatTile cells (particles of one block are usually neighbours: emitters and
+// the initial fill are row-major); the tile is then added to the grid with one
+// atomic per touched component.  Otherwise every term is a global atomic.  Both
+// give the reference's terms; only the order of the float additions differs,
+// which atomic accumulation leaves unspecified anyway.
+constexpr int kSplatTile = 2048;  // cells (float4 each, 32 KiB of LDS)
+
+struct SplatCorners {
+  int i, j, i1, j1;
+  float fi, fj;
+};
+
+__device__ __forceinline__ SplatCorners splat_corners(float xs, float ys, int nx, int ny) {
+  SplatCorners s;
+  s.i = (int)xs;
+  s.j = (int)ys;
+  s.i1 = s.i + 1;
+  s.j1 = s.j + 1;
+  s.fi = xs - (float)s.i;
+  s.fj = ys - (float)s.j;
+  s.i = clampi(s.i, 0, nx - 1);
+  s.j = clampi(s.j, 0, ny - 1);
+  s.i1 = clampi(s.i1, 0, nx - 1);
+  s.j1 = clampi(s.j1, 0, ny - 1);
+  return s;
+}
+
+// Adds the 8 terms of one splat (value and weight, Grid.h:152-184 order of
+// operations) through `add(cell_i, cell_j, component, term)`.
+template <typename Add>
+__device__ __forceinline__ void splat_terms(const SplatCorners& s, float value, int comp,
+                                            Add&& add) {
+  const float a0 = (1.0f - s.fj) * value, a1 = s.fj * value;
+  const float w0 = (1.0f - s.fj) * 1.0f, w1 = s.fj * 1.0f;
+  add(s.i, s.j, comp, (1.0f - s.fi) * a0);
+  add(s.i1, s.j, comp, s.fi * a0);
+  add(s.i, s.j1, comp, (1.0f - s.fi) * a1);
+  add(s.i1, s.j1, comp, s.fi * a1);
+  add(s.i, s.j, comp + 1, (1.0f - s.fi) * w0);
+  add(s.i1, s.j, comp + 1, s.fi * w0);
+  add(s.i, s.j1, comp + 1, (1.0f - s.fi) * w1);
+  add(s.i1, s.j1, comp + 1, s.fi * w1);
+}
+
 __global__ void __launch_bounds__(256) k_particle_scatter(Geom g, const float4* __restrict__ pt,
                                                           int64_t np, uint8_t* __restrict__ label,
                                                           float* __restrict__ acc, int do_classify,
                                                           int do_p2g) {
   const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
   const int64_t p = t * 256 + threadIdx.x;
-  if (p >= np) return;
-  const float4 q = pt[p];
-  if (do_classify) {
+  const bool live = p < np;
+  const float4 q = live ? pt[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+  if (do_classify && live) {
     int x = (int)((q.x / g.len_x) * (float)g.nx);
     int y = (int)((q.y / g.len_y) * (float)g.ny);
     x = clampi(x, 0, g.nx - 1);
@@ -56,57 +105,67 @@ __global__ void __launch_bounds__(256) k_particle_scatter(Geom g, const float4* 
     const int c = x + y * g.nx;
     if (label[c] != kSolid) label[c] = kLiquid;
   }
-  if (do_p2g) {
-    // pool spacing: (dx, dx)
-    const float yu = q.y - g.dy * 0.5f;
-    const float xv = q.x - g.dx * 0.5f;
-    const float xs_u = div_x(q.x, g), ys_u = div_x(yu, g);
-    const float xs_v = div_x(xv, g), ys_v = div_x(q.y, g);
-    const int nx = g.nx, ny = g.ny;
-    // u and its weight
-    {
-      int i = (int)xs_u, j = (int)ys_u;
-      int i1 = i + 1, j1 = j + 1;
-      const float fi = xs_u - (float)i, fj = ys_u - (float)j;
-      i = clampi(i, 0, nx - 1); j = clampi(j, 0, ny - 1);
-      i1 = clampi(i1, 0, nx - 1); j1 = clampi(j1, 0, ny - 1);
-      const float a0 = (1.0f - fj) * q.z, a1 = fj * q.z;
-      const float w0 = (1.0f - fj) * 1.0f, w1 = fj * 1.0f;
-      float* b00 = acc + 4 * (i + j * nx);
-      float* b10 = acc + 4 * (i1 + j * nx);
-      float* b01 = acc + 4 * (i + j1 * nx);
-      float* b11 = acc + 4 * (i1 + j1 * nx);
-      unsafeAtomicAdd(b00 + 0, (1.0f - fi) * a0);
-      unsafeAtomicAdd(b10 + 0, fi * a0);
-      unsafeAtomicAdd(b01 + 0, (1.0f - fi) * a1);
-      unsafeAtomicAdd(b11 + 0, fi * a1);
-      unsafeAtomicAdd(b00 + 1, (1.0f - fi) * w0);
-      unsafeAtomicAdd(b10 + 1, fi * w0);
-      unsafeAtomicAdd(b01 + 1, (1.0f - fi) * w1);
-      unsafeAtomicAdd(b11 + 1, fi * w1);
-    }
-    // v and its weight
-    {
-      int i = (int)xs_v, j = (int)ys_v;
-      int i1 = i + 1, j1 = j + 1;
-      const float fi = xs_v - (float)i, fj = ys_v - (float)j;
-      i = clampi(i, 0, nx - 1); j = clampi(j, 0, ny - 1);
-      i1 = clampi(i1, 0, nx - 1); j1 = clampi(j1, 0, ny - 1);
-      const float a0 = (1.0f - fj) * q.w, a1 = fj * q.w;
-      const float w0 = (1.0f - fj) * 1.0f, w1 = fj * 1.0f;
-      float* b00 = acc + 4 * (i + j * nx);
-      float* b10 = acc + 4 * (i1 + j * nx);
-      float* b01 = acc + 4 * (i + j1 * nx);
-      float* b11 = acc + 4 * (i1 + j1 * nx);
-      unsafeAtomicAdd(b00 + 2, (1.0f - fi) * a0);
-      unsafeAtomicAdd(b10 + 2, fi * a0);
-      unsafeAtomicAdd(b01 + 2, (1.0f - fi) * a1);
-      unsafeAtomicAdd(b11 + 2, fi * a1);
-      unsafeAtomicAdd(b00 + 3, (1.0f - fi) * w0);
-      unsafeAtomicAdd(b10 + 3, fi * w0);
-      unsafeAtomicAdd(b01 + 3, (1.0f - fi) * w1);
-      unsafeAtomicAdd(b11 + 3, fi * w1);
-    }
+  if (!do_p2g) return;
+  const int nx = g.nx, ny = g.ny;
+  // pool spacing: (dx, dx)
+  const float yu = q.y - g.dy * 0.5f;
+  const float xv = q.x - g.dx * 0.5f;
+  const SplatCorners su = splat_corners(div_x(q.x, g), div_x(yu, g), nx, ny);
+  const SplatCorners sv = splat_corners(div_x(xv, g), div_x(q.y, g), nx, ny);
+  // footprint of the block: min / max corner over live particles
+  int i0 = live ? min(su.i, sv.i) : INT_MAX, j0 = live ? min(su.j, sv.j) : INT_MAX;
+  int i1 = live ? max(su.i1, sv.i1) : INT_MIN, j1 = live ? max(su.j1, sv.j1) : INT_MIN;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    i0 = min(i0, __shfl_xor(i0, off, 64));
+    j0 = min(j0, __shfl_xor(j0, off, 64));
+    i1 = max(i1, __shfl_xor(i1, off, 64));
+    j1 = max(j1, __shfl_xor(j1, off, 64));
+  }
+  __shared__ int s_box[4][4];
+  __shared__ float4 s_tile[kSplatTile];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) {
+    s_box[0][wave] = i0;
+    s_box[1][wave] = j0;
+    s_box[2][wave] = i1;
+    s_box[3][wave] = j1;
+  }
+  __syncthreads();
+  i0 = min(min(s_box[0][0], s_box[0][1]), min(s_box[0][2], s_box[0][3]));
+  j0 = min(min(s_box[1][0], s_box[1][1]), min(s_box[1][2], s_box[1][3]));
+  i1 = max(max(s_box[2][0], s_box[2][1]), max(s_box[2][2], s_box[2][3]));
+  j1 = max(max(s_box[3][0], s_box[3][1]), max(s_box[3][2], s_box[3][3]));
+  const int bw = i1 - i0 + 1, bh = j1 - j0 + 1;
+  const bool tiled = bw > 0 && bh > 0 && (int64_t)bw * bh <= kSplatTile;
+  if (!tiled) {
+    if (!live) return;
+    auto gadd = [&](int i, int j, int comp, float v) {
+      unsafeAtomicAdd(acc + 4 * (i + j * nx) + comp, v);
+    };
+    splat_terms(su, q.z, 0, gadd);
+    splat_terms(sv, q.w, 2, gadd);
+    return;
+  }
+  const int area = bw * bh;
+  for (int c = threadIdx.x; c < area; c += 256) s_tile[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+  __syncthreads();
+  if (live) {
+    float* tile = reinterpret_cast<float*>(s_tile);
+    auto ladd = [&](int i, int j, int comp, float v) {
+      atomicAdd(tile + 4 * ((i - i0) + (j - j0) * bw) + comp, v);
+    };
+    splat_terms(su, q.z, 0, ladd);
+    splat_terms(sv, q.w, 2, ladd);
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < area; c += 256) {
+    const float4 v = s_tile[c];
+    float* dst = acc + 4 * ((i0 + c % bw) + (j0 + c / bw) * nx);
+    if (v.x != 0.f) unsafeAtomicAdd(dst + 0, v.x);
+    if (v.y != 0.f) unsafeAtomicAdd(dst + 1, v.y);
+    if (v.z != 0.f) unsafeAtomicAdd(dst + 2, v.z);
+    if (v.w != 0.f) unsafeAtomicAdd(dst + 3, v.w);
   }
 }
 

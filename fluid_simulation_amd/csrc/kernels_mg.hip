@@ -1,0 +1,781 @@
+// kernels_mg.hip -- multigrid-preconditioned CG with FP64 solution and residual
+// (FSG_SOLVER_MG): the fast form of the pressure solve that replaces
+// Eigen::ConjugateGradient (include/FluidSolver.h:114, src/FluidSolver.cpp:81,
+// 362-425).  It solves the same system as the reference -- A x = b over the
+// LIQUID cells, A_ii = -n/dx^2 (n non-SOLID neighbours), A_ij = 1/dx^2 for LIQUID
+// neighbours, b the cell divergence -- to a relative FP64 residual
+// |b - A x| <= tol |b|, so its result is the converged pressure, not Eigen's
+// iterate after a fixed iteration count.
+//
+// Layout: every vector is a dense cell grid (row-major i + j*nx) holding zero at
+// cells that are not unknowns, so the 5-point operator needs no neighbour lists.
+// The system is scaled to the integer stencil K = -dx^2 A (K_ii = n, K_ij = -1),
+// right-hand side b' = -dx^2 b, kept in FP64.
+//
+// Precision: the solution x and the residual r = b' - K x are FP64 (the residual
+// is recomputed from x at every iteration, never updated recursively); the search
+// direction, the preconditioned residual and the whole V-cycle are FP32; every dot
+// product is summed in FP64.
+//
+// Preconditioner: one V-cycle of cell-centred geometric multigrid
+// (McAdams, Sifakis, Teran, "A parallel multigrid Poisson solver for fluids
+// simulation on large grids", SCA 2010): a coarse cell is AIR if any child is AIR,
+// else LIQUID if any child is LIQUID, else SOLID; each level is the same integer
+// stencil on its own labels; prolongation is cell-centred bilinear (weights 9, 3,
+// 3, 1 / 16) and restriction its transpose; smoothing is two damped-Jacobi sweeps
+// before and after the coarse correction.  Every component is a symmetric linear
+// operator, so the V-cycle is an SPD preconditioner for CG.  In the scaled form a
+// level-(l+1) right-hand side is P^T of the level-l residual (the 4^-l scale of the
+// rediscretised operators cancels), so every level uses the unit stencil.
+//
+// Kernels per CG iteration: mg_down (one per fine level: 2 pre-smoothing sweeps,
+// residual and restriction fused in one LDS tile), mg_coarse (one workgroup: the
+// V-cycle of the levels of <= 4096 cells in LDS), mg_up (one per fine level:
+// prolongation, correction and 2 post-smoothing sweeps fused), mg_dir (p = z +
+// beta p, K p and p.Kp), mg_xr (x += alpha p in FP64 and r = b' - K x).  Every
+// kernel returns at once when the solve is done, and each sums the block partials
+// of the previous kernel itself in a fixed order, so the scalars are identical in
+// every block and the solve is deterministic.
+#include <float.h>
+
+#include "fsg_common.h"
+#include "fsg_device.h"
+#include "fsg_kernels.h"
+
+namespace fsg {
+
+namespace {
+
+constexpr int kT = 256;         // threads of the tile and vector kernels
+constexpr uint8_t kAct = 0x80;  // info bit: the cell is an unknown
+
+__device__ __forceinline__ bool is_act(uint8_t f) { return (f & kAct) != 0; }
+__device__ __forceinline__ float nns_of(uint8_t f) { return (float)(f & 7); }
+
+// Unknown flag and non-SOLID neighbour count of cell (i, j) of a label grid
+// (neighbours outside the grid count as SOLID).  A LIQUID cell whose four
+// neighbours are SOLID has an all-zero row and is left out of the system.
+__device__ __forceinline__ uint8_t make_info(const uint8_t* __restrict__ lab, int nx, int ny,
+                                             int i, int j) {
+  const int c = i + j * nx;
+  int n = 0;
+  n += i > 0 && lab[c - 1] != kSolid;
+  n += i < nx - 1 && lab[c + 1] != kSolid;
+  n += j > 0 && lab[c - nx] != kSolid;
+  n += j < ny - 1 && lab[c + nx] != kSolid;
+  const bool act = lab[c] == kLiquid && n > 0;
+  return (uint8_t)((act ? kAct : 0) | n);
+}
+
+// Sum of n block partials, the same fixed order in every block (NT threads;
+// sm: NT / 64 doubles).
+template <int NT = kT>
+__device__ __forceinline__ double sum_part(const double* __restrict__ part, int n, double* sm) {
+  double v[1] = {0.0};
+  for (int b = threadIdx.x; b < n; b += NT) v[0] += part[b];
+  block_sum<1, NT>(v, sm);
+  return v[0];
+}
+
+template <int NT = kT>
+__device__ __forceinline__ void write_part(double val, double* __restrict__ part, double* sm) {
+  double v[1] = {val};
+  block_sum<1, NT>(v, sm);
+  if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- setup
+// Level-0 info, b' = -dx^2 * div (the divergence exactly as the compact path forms
+// it), and per-block partials (sum b', number of unknowns).
+__global__ void __launch_bounds__(kT) k_mg_setup0(Geom g, const uint8_t* __restrict__ label,
+                                                   const float* __restrict__ uf,
+                                                   const float* __restrict__ vf,
+                                                   uint8_t* __restrict__ inf,
+                                                   double* __restrict__ b,
+                                                   double* __restrict__ part_sum,
+                                                   double* __restrict__ part_cnt) {
+  __shared__ double sm[4];
+  const int nx = g.nx, ny = g.ny, C = nx * ny;
+  const double scale = -(double)g.dx * (double)g.dx;
+  double s = 0.0, cnt = 0.0;
+  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT) {
+    const int i = c % nx, j = c / nx;
+    const uint8_t f = make_info(label, nx, ny, i, j);
+    inf[c] = f;
+    double bv = 0.0;
+    if (is_act(f)) {
+      const float uxp = i < nx - 1 ? uf[c + 1] : 0.f;
+      const float vyp = j < ny - 1 ? vf[c + nx] : 0.f;
+      const float dvx = div_x(uxp - uf[c], g);
+      const float dvy = div_y(vyp - vf[c], g);
+      bv = scale * (double)(dvx + dvy);
+      s += bv;
+      cnt += 1.0;
+    }
+    b[c] = bv;
+  }
+  write_part<kT>(s, part_sum, sm);
+  write_part<kT>(cnt, part_cnt, sm);
+}
+
+// Optional mean removal (singular all-Neumann systems), x0 = 0, r0 = b', |b'|^2.
+__global__ void __launch_bounds__(kT) k_mg_setup1(int C, int project_mean,
+                                                   const uint8_t* __restrict__ inf,
+                                                   double* __restrict__ b,
+                                                   double* __restrict__ x0,
+                                                   float* __restrict__ r,
+                                                   const double* __restrict__ part_sum,
+                                                   const double* __restrict__ part_cnt, int n_part,
+                                                   double* __restrict__ part_bb, CgScalars* sc) {
+  __shared__ double sm[4];
+  double mean = 0.0;
+  if (project_mean) {
+    const double s = sum_part<kT>(part_sum, n_part, sm);
+    const double n = sum_part<kT>(part_cnt, n_part, sm);
+    mean = n > 0.0 ? s / n : 0.0;
+  }
+  double acc = 0.0;
+  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT) {
+    double bv = b[c];
+    if (project_mean && is_act(inf[c])) {
+      bv -= mean;
+      b[c] = bv;
+    }
+    x0[c] = 0.0;
+    r[c] = (float)bv;
+    acc += bv * bv;
+  }
+  write_part<kT>(acc, part_bb, sm);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->done = 0;
+    sc->iter = 0;
+    sc->spmv = 0;
+    sc->err = 0.0;
+    sc->rho[0] = sc->rho[1] = 0.0;
+  }
+}
+
+// Coarse labels of level l+1 from level l, and (finf != null) the info of level l.
+__global__ void __launch_bounds__(kT) k_mg_coarsen(int fnx, int fny, const uint8_t* __restrict__ flab,
+                                                    uint8_t* __restrict__ finf, int cnx, int cny,
+                                                    uint8_t* __restrict__ clab) {
+  const int nf = fnx * fny, nc = cnx * cny;
+  const int n = nf > nc ? nf : nc;
+  for (int t = blockIdx.x * kT + threadIdx.x; t < n; t += gridDim.x * kT) {
+    if (t < nc) {
+      const int I = t % cnx, J = t / cnx;
+      bool air = false, liq = false;
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          const int i = 2 * I + a, j = 2 * J + b;
+          if (i < fnx && j < fny) {
+            const uint8_t L = flab[i + j * fnx];
+            air |= L == kAir;
+            liq |= L == kLiquid;
+          }
+        }
+      clab[t] = air ? kAir : (liq ? kLiquid : kSolid);
+    }
+    if (finf && t < nf) finf[t] = make_info(flab, fnx, fny, t % fnx, t / fnx);
+  }
+}
+
+// ---------------------------------------------------------------- V-cycle
+// Down: level l (fine) -> level l+1.  A block owns 16 x 16 coarse cells (32 x 32
+// fine cells).  In LDS: r and info on the fine halo [-3, 35), x1 = w r / n there,
+// x2 (second sweep) on [-2, 34), the residual on [-1, 33), then the restriction
+// (1D weights 1/4, 3/4, 3/4, 1/4 over fine 2I-1 .. 2I+2).  Level 0 first checks
+// convergence on the partials of |r|^2 (the stop test of the CG iteration).
+struct MgCheck {
+  CgScalars* sc;
+  const double* part_rr;
+  int n_rr;
+  const double* part_bb;
+  int n_bb;
+  double tol2;
+  int max_iter;
+};
+
+// Every block takes the same decision from the same partials.  `done` is read by
+// one thread and broadcast, since block 0 may set it while other blocks start.
+template <int NT>
+__device__ __forceinline__ bool mg_check(const MgCheck& k, int it, double* sm) {
+  __shared__ int s_done;
+  if (threadIdx.x == 0) s_done = ld_dev(&k.sc->done);
+  __syncthreads();
+  if (s_done) return true;
+  const double rr = sum_part<NT>(k.part_rr, k.n_rr, sm);
+  double bb, thr;
+  if (it == 0) {
+    bb = sum_part<NT>(k.part_bb, k.n_bb, sm);
+    thr = k.tol2 * bb;
+    if (thr < DBL_MIN) thr = DBL_MIN;
+  } else {
+    bb = k.sc->rhs_norm2;
+    thr = k.sc->threshold;
+  }
+  const bool stop = bb == 0.0 || rr < thr || it >= k.max_iter;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (it == 0) {
+      k.sc->rhs_norm2 = bb;
+      k.sc->threshold = thr;
+    }
+    k.sc->rr = rr;
+    k.sc->err = bb > 0.0 ? sqrt(rr / bb) : 0.0;
+    k.sc->iter = it;
+    if (stop) st_dev(&k.sc->done, 1);
+  }
+  return stop;
+}
+
+struct MgLevel {
+  int nx, ny;
+  const uint8_t* inf;
+  float* r;   // right-hand side of this level (level 0: the CG residual)
+  float* xs;  // after pre-smoothing
+  float* xo;  // after the full cycle (level 0: z)
+};
+
+template <bool L0>
+__global__ void __launch_bounds__(kT) k_mg_down(MgLevel f, MgLevel c, float omega, int it,
+                                                 MgCheck chk) {
+  __shared__ double sm[4];
+  if (L0) {
+    if (mg_check<kT>(chk, it, sm)) return;
+  } else if (chk.sc->done) {
+    return;
+  }
+  constexpr int W3 = 38, W2 = 36, W1 = 34;
+  __shared__ float s_r[W3 * W3], s_x1[W3 * W3], s_x2[W2 * W2], s_res[W1 * W1];
+  __shared__ uint8_t s_inf[W3 * W3];
+  const int tiles_x = (c.nx + 15) >> 4;
+  const int bx = blockIdx.x % tiles_x, by = blockIdx.x / tiles_x;
+  const int fi0 = bx * 32, fj0 = by * 32;
+  for (int t = threadIdx.x; t < W3 * W3; t += kT) {
+    const int i = fi0 + t % W3 - 3, j = fj0 + t / W3 - 3;
+    float rv = 0.f;
+    uint8_t fv = 0;
+    if (i >= 0 && i < f.nx && j >= 0 && j < f.ny) {
+      const int q = i + j * f.nx;
+      fv = f.inf[q];
+      rv = f.r[q];
+    }
+    s_inf[t] = fv;
+    s_r[t] = rv;
+    s_x1[t] = is_act(fv) ? omega * rv / nns_of(fv) : 0.f;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < W2 * W2; t += kT) {
+    const int q = (t % W2 + 1) + (t / W2 + 1) * W3;
+    const uint8_t fv = s_inf[q];
+    float v = 0.f;
+    if (is_act(fv)) {
+      const float n = nns_of(fv), x = s_x1[q];
+      const float kx = n * x - (s_x1[q - 1] + s_x1[q + 1] + s_x1[q - W3] + s_x1[q + W3]);
+      v = x + omega * (s_r[q] - kx) / n;
+    }
+    s_x2[t] = v;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < W1 * W1; t += kT) {
+    const int li = t % W1, lj = t / W1;  // fine offset li - 1, lj - 1
+    const int q = (li + 1) + (lj + 1) * W2;
+    const int q3 = (li + 2) + (lj + 2) * W3;
+    const uint8_t fv = s_inf[q3];
+    const float x = s_x2[q];
+    float res = 0.f;
+    if (is_act(fv)) {
+      const float kx = nns_of(fv) * x - (s_x2[q - 1] + s_x2[q + 1] + s_x2[q - W2] + s_x2[q + W2]);
+      res = s_r[q3] - kx;
+    }
+    s_res[t] = res;
+    const int oi = li - 1, oj = lj - 1;
+    if (oi >= 0 && oi < 32 && oj >= 0 && oj < 32) {
+      const int i = fi0 + oi, j = fj0 + oj;
+      if (i < f.nx && j < f.ny) f.xs[i + j * f.nx] = x;
+    }
+  }
+  __syncthreads();
+  {
+    const int ci = threadIdx.x & 15, cj = threadIdx.x >> 4;
+    const int I = bx * 16 + ci, J = by * 16 + cj;
+    if (I < c.nx && J < c.ny) {
+      const int cc = I + J * c.nx;
+      float v = 0.f;
+      if (is_act(c.inf[cc])) {
+        // fine offsets 2ci-1 .. 2ci+2 are s_res columns 2ci .. 2ci+3
+        const float w[4] = {0.25f, 0.75f, 0.75f, 0.25f};
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const float* row = s_res + (2 * cj + b) * W1 + 2 * ci;
+          const float s = 0.25f * row[0] + 0.75f * row[1] + 0.75f * row[2] + 0.25f * row[3];
+          v += w[b] * s;
+        }
+      }
+      c.r[cc] = v;
+    }
+  }
+}
+
+// Up: level l+1 -> level l.  A block owns 32 x 32 fine cells.  In LDS: the coarse
+// solution around the tile, r and info on [-2, 34), the corrected x = xs + P e
+// there, the first post-sweep on [-1, 33), the second on the tile, written to xo.
+// Level 0 also forms the partial of r.z (z = xo) for the next direction.
+template <bool L0>
+__global__ void __launch_bounds__(kT) k_mg_up(MgLevel f, MgLevel c, float omega,
+                                               const CgScalars* sc, double* __restrict__ part_rz) {
+  if (sc->done) return;
+  constexpr int W2 = 36, W1 = 34, WC = 20;
+  __shared__ float s_r[W2 * W2], s_xc[W2 * W2], s_y1[W1 * W1], s_e[WC * WC];
+  __shared__ uint8_t s_inf[W2 * W2];
+  __shared__ double sm[4];
+  const int tiles_x = (f.nx + 31) >> 5;
+  const int bx = blockIdx.x % tiles_x, by = blockIdx.x / tiles_x;
+  const int fi0 = bx * 32, fj0 = by * 32;
+  const int ci0 = (fi0 >> 1) - 2, cj0 = (fj0 >> 1) - 2;  // coarse frame origin
+  for (int t = threadIdx.x; t < WC * WC; t += kT) {
+    const int I = ci0 + t % WC, J = cj0 + t / WC;
+    s_e[t] = (I >= 0 && I < c.nx && J >= 0 && J < c.ny) ? c.xo[I + J * c.nx] : 0.f;
+  }
+  for (int t = threadIdx.x; t < W2 * W2; t += kT) {
+    const int i = fi0 + t % W2 - 2, j = fj0 + t / W2 - 2;
+    float rv = 0.f, xv = 0.f;
+    uint8_t fv = 0;
+    if (i >= 0 && i < f.nx && j >= 0 && j < f.ny) {
+      const int q = i + j * f.nx;
+      fv = f.inf[q];
+      rv = f.r[q];
+      xv = f.xs[q];
+    }
+    s_inf[t] = fv;
+    s_r[t] = rv;
+    s_xc[t] = xv;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < W2 * W2; t += kT) {
+    if (!is_act(s_inf[t])) {
+      s_xc[t] = 0.f;
+      continue;
+    }
+    const int i = fi0 + t % W2 - 2, j = fj0 + t / W2 - 2;
+    const int I = i >> 1, J = j >> 1;  // floor for the -2 .. -1 halo too
+    const int In = (i & 1) ? I + 1 : I - 1;
+    const int Jn = (j & 1) ? J + 1 : J - 1;
+    const float* e = s_e;
+    const float e00 = e[(I - ci0) + (J - cj0) * WC];
+    const float e10 = e[(In - ci0) + (J - cj0) * WC];
+    const float e01 = e[(I - ci0) + (Jn - cj0) * WC];
+    const float e11 = e[(In - ci0) + (Jn - cj0) * WC];
+    s_xc[t] += 0.5625f * e00 + 0.1875f * e10 + 0.1875f * e01 + 0.0625f * e11;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < W1 * W1; t += kT) {
+    const int q = (t % W1 + 1) + (t / W1 + 1) * W2;
+    const uint8_t fv = s_inf[q];
+    float v = 0.f;
+    if (is_act(fv)) {
+      const float n = nns_of(fv), x = s_xc[q];
+      const float kx = n * x - (s_xc[q - 1] + s_xc[q + 1] + s_xc[q - W2] + s_xc[q + W2]);
+      v = x + omega * (s_r[q] - kx) / n;
+    }
+    s_y1[t] = v;
+  }
+  __syncthreads();
+  double rz = 0.0;
+  for (int t = threadIdx.x; t < 32 * 32; t += kT) {
+    const int oi = t & 31, oj = t >> 5;
+    const int i = fi0 + oi, j = fj0 + oj;
+    if (i >= f.nx || j >= f.ny) continue;
+    const int q = (oi + 1) + (oj + 1) * W1;
+    const int q2 = (oi + 2) + (oj + 2) * W2;
+    const uint8_t fv = s_inf[q2];
+    float v = 0.f;
+    if (is_act(fv)) {
+      const float n = nns_of(fv), x = s_y1[q];
+      const float kx = n * x - (s_y1[q - 1] + s_y1[q + 1] + s_y1[q - W1] + s_y1[q + W1]);
+      v = x + omega * (s_r[q2] - kx) / n;
+    }
+    f.xo[i + j * f.nx] = v;
+    if (L0) rz += (double)s_r[q2] * (double)v;
+  }
+  if (L0) write_part<kT>(rz, part_rz, sm);
+}
+
+// The levels lc .. nlev-1 (each <= 4096 cells) in one workgroup, all in LDS:
+// the same V-cycle, with `sweeps` damped-Jacobi sweeps from zero on the coarsest.
+__device__ __forceinline__ float nbr_sum(const float* a, int c, int i, int j, int nx, int ny) {
+  float s = 0.f;
+  if (i > 0) s += a[c - 1];
+  if (i < nx - 1) s += a[c + 1];
+  if (j > 0) s += a[c - nx];
+  if (j < ny - 1) s += a[c + nx];
+  return s;
+}
+
+constexpr int kCoarseThreads = 1024;
+
+// With check != 0 (the hierarchy starts in this kernel) it runs level 0's
+// convergence test first and forms the r.z partial at the end.
+__global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, MgCheck chk, int it,
+                                                             int check,
+                                                             double* __restrict__ part_rz) {
+  __shared__ double sm[2 * 16];
+  if (check) {
+    if (mg_check<kCoarseThreads>(chk, it, sm)) return;
+  } else if (chk.sc->done) {
+    return;
+  }
+  extern __shared__ float lds[];
+  const int tot = a.total;
+  float* R = lds;
+  float* X = R + tot;
+  float* T = X + tot;
+  uint8_t* INF = reinterpret_cast<uint8_t*>(T + tot);
+  for (int l = a.lc; l < a.nlev; ++l) {
+    const int n = a.nx[l] * a.ny[l], o = a.off[l];
+    for (int q = threadIdx.x; q < n; q += kCoarseThreads) {
+      INF[o + q] = a.inf[l][q];
+      R[o + q] = l == a.lc ? a.r_in[q] : 0.f;
+      X[o + q] = 0.f;
+      T[o + q] = 0.f;
+    }
+  }
+  __syncthreads();
+  const float w = a.omega;
+  for (int l = a.lc; l < a.nlev - 1; ++l) {
+    const int nx = a.nx[l], ny = a.ny[l], n = nx * ny, o = a.off[l];
+    float *r = R + o, *x = X + o, *t = T + o;
+    const uint8_t* inf = INF + o;
+    for (int q = threadIdx.x; q < n; q += kCoarseThreads)
+      t[q] = is_act(inf[q]) ? w * r[q] / nns_of(inf[q]) : 0.f;
+    __syncthreads();
+    for (int q = threadIdx.x; q < n; q += kCoarseThreads) {
+      float v = 0.f;
+      if (is_act(inf[q])) {
+        const float nn = nns_of(inf[q]);
+        v = t[q] + w * (r[q] - (nn * t[q] - nbr_sum(t, q, q % nx, q / nx, nx, ny))) / nn;
+      }
+      x[q] = v;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < n; q += kCoarseThreads) {
+      float v = 0.f;
+      if (is_act(inf[q]))
+        v = r[q] - (nns_of(inf[q]) * x[q] - nbr_sum(x, q, q % nx, q / nx, nx, ny));
+      t[q] = v;
+    }
+    __syncthreads();
+    const int cnx = a.nx[l + 1], cny = a.ny[l + 1], co = a.off[l + 1];
+    for (int q = threadIdx.x; q < cnx * cny; q += kCoarseThreads) {
+      const int I = q % cnx, J = q / cnx;
+      float v = 0.f;
+      if (is_act(INF[co + q])) {
+        const float wt[4] = {0.25f, 0.75f, 0.75f, 0.25f};
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const int j = 2 * J - 1 + b;
+          if (j < 0 || j >= ny) continue;
+          float s = 0.f;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int i = 2 * I - 1 + e;
+            if (i >= 0 && i < nx) s += wt[e] * t[i + j * nx];
+          }
+          v += wt[b] * s;
+        }
+      }
+      R[co + q] = v;
+    }
+    __syncthreads();
+  }
+  {
+    const int l = a.nlev - 1;
+    const int nx = a.nx[l], ny = a.ny[l], n = nx * ny, o = a.off[l];
+    float *r = R + o, *x = X + o, *t = T + o;
+    const uint8_t* inf = INF + o;
+    for (int s = 0; s < a.sweeps; ++s) {
+      float* src = (s & 1) ? t : x;
+      float* dst = (s & 1) ? x : t;
+      for (int q = threadIdx.x; q < n; q += kCoarseThreads) {
+        float v = 0.f;
+        if (is_act(inf[q])) {
+          const float nn = nns_of(inf[q]);
+          v = src[q] + w * (r[q] - (nn * src[q] - nbr_sum(src, q, q % nx, q / nx, nx, ny))) / nn;
+        }
+        dst[q] = v;
+      }
+      __syncthreads();
+    }
+    if (a.sweeps & 1) {
+      for (int q = threadIdx.x; q < n; q += kCoarseThreads) x[q] = t[q];
+      __syncthreads();
+    }
+  }
+  for (int l = a.nlev - 2; l >= a.lc; --l) {
+    const int nx = a.nx[l], ny = a.ny[l], n = nx * ny, o = a.off[l];
+    const int cnx = a.nx[l + 1], cny = a.ny[l + 1], co = a.off[l + 1];
+    float *r = R + o, *x = X + o, *t = T + o;
+    const uint8_t* inf = INF + o;
+    const float* e = X + co;
+    for (int q = threadIdx.x; q < n; q += kCoarseThreads) {
+      float v = 0.f;
+      if (is_act(inf[q])) {
+        const int i = q % nx, j = q / nx;
+        const int I = i >> 1, J = j >> 1;
+        const int In = (i & 1) ? I + 1 : I - 1;
+        const int Jn = (j & 1) ? J + 1 : J - 1;
+        const bool iok = In >= 0 && In < cnx, jok = Jn >= 0 && Jn < cny;
+        const float e00 = e[I + J * cnx];
+        const float e10 = iok ? e[In + J * cnx] : 0.f;
+        const float e01 = jok ? e[I + Jn * cnx] : 0.f;
+        const float e11 = iok && jok ? e[In + Jn * cnx] : 0.f;
+        v = x[q] + (0.5625f * e00 + 0.1875f * e10 + 0.1875f * e01 + 0.0625f * e11);
+      }
+      t[q] = v;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < n; q += kCoarseThreads) {
+      float v = 0.f;
+      if (is_act(inf[q])) {
+        const float nn = nns_of(inf[q]);
+        v = t[q] + w * (r[q] - (nn * t[q] - nbr_sum(t, q, q % nx, q / nx, nx, ny))) / nn;
+      }
+      x[q] = v;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < n; q += kCoarseThreads) {
+      float v = 0.f;
+      if (is_act(inf[q])) {
+        const float nn = nns_of(inf[q]);
+        v = x[q] + w * (r[q] - (nn * x[q] - nbr_sum(x, q, q % nx, q / nx, nx, ny))) / nn;
+      }
+      t[q] = v;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < n; q += kCoarseThreads) x[q] = t[q];
+    __syncthreads();
+  }
+  {
+    const int n = a.nx[a.lc] * a.ny[a.lc], o = a.off[a.lc];
+    double rz = 0.0;
+    for (int q = threadIdx.x; q < n; q += kCoarseThreads) {
+      a.x_out[q] = X[o + q];
+      rz += (double)R[o + q] * (double)X[o + q];
+    }
+    if (part_rz) {
+      double v[1] = {rz};
+      block_sum<1, kCoarseThreads>(v, sm);
+      if (threadIdx.x == 0) part_rz[0] = v[0];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- CG vectors
+// p = z + beta p_old (beta = 0 at the first iteration), K p, partial p.Kp.
+// rho = r.z is summed from the V-cycle's partials; block 0 records it.
+__global__ void __launch_bounds__(kT) k_mg_dir(int nx, int ny, const uint8_t* __restrict__ inf,
+                                                const float* __restrict__ z,
+                                                const float* __restrict__ p_old,
+                                                float* __restrict__ p_new, int it,
+                                                const double* __restrict__ part_rz, int n_rz,
+                                                double* __restrict__ part_pq, CgScalars* sc) {
+  __shared__ double sm[4];
+  if (sc->done) return;
+  const double rz = sum_part<kT>(part_rz, n_rz, sm);
+  const double beta = it == 0 ? 0.0 : rz / sc->rho[(it - 1) & 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->rho[it & 1] = rz;
+  const float bf = (float)beta;
+  const bool first = it == 0;
+  const int C = nx * ny;
+  double acc = 0.0;
+  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT) {
+    const uint8_t f = inf[c];
+    float pc = 0.f;
+    if (is_act(f)) {
+      const int i = c % nx, j = c / nx;
+      auto pn = [&](int q) { return first ? z[q] : z[q] + bf * p_old[q]; };
+      pc = pn(c);
+      float s = 0.f;
+      if (i > 0) s += pn(c - 1);
+      if (i < nx - 1) s += pn(c + 1);
+      if (j > 0) s += pn(c - nx);
+      if (j < ny - 1) s += pn(c + nx);
+      const float kp = nns_of(f) * pc - s;
+      acc += (double)pc * (double)kp;
+    }
+    p_new[c] = pc;
+  }
+  write_part<kT>(acc, part_pq, sm);
+}
+
+// x_new = x_old + alpha p (FP64), r = b' - K x_new (FP64, stored as FP32),
+// partial |r|^2 (FP64).
+__global__ void __launch_bounds__(kT) k_mg_xr(int nx, int ny, const uint8_t* __restrict__ inf,
+                                               const float* __restrict__ p,
+                                               const double* __restrict__ x_old,
+                                               double* __restrict__ x_new,
+                                               const double* __restrict__ b, float* __restrict__ r,
+                                               int it, const double* __restrict__ part_pq, int n_pq,
+                                               double* __restrict__ part_rr, CgScalars* sc) {
+  __shared__ double sm[4];
+  if (sc->done) return;
+  const double pq = sum_part<kT>(part_pq, n_pq, sm);
+  const double alpha = sc->rho[it & 1] / pq;
+  const int C = nx * ny;
+  double acc = 0.0;
+  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT) {
+    const uint8_t f = inf[c];
+    double xn = 0.0;
+    float rv = 0.f;
+    if (is_act(f)) {
+      const int i = c % nx, j = c / nx;
+      auto xv = [&](int q) { return x_old[q] + alpha * (double)p[q]; };
+      xn = xv(c);
+      double s = 0.0;
+      if (i > 0) s += xv(c - 1);
+      if (i < nx - 1) s += xv(c + 1);
+      if (j > 0) s += xv(c - nx);
+      if (j < ny - 1) s += xv(c + nx);
+      const double r64 = b[c] - ((double)(f & 7) * xn - s);
+      rv = (float)r64;
+      acc += r64 * r64;
+    }
+    x_new[c] = xn;
+    r[c] = rv;
+  }
+  write_part<kT>(acc, part_rr, sm);
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->spmv = it + 1;
+}
+
+// Partials of sum(n_i x_i) and sum(n_i) over the unknowns (n_i = K_ii).
+__global__ void __launch_bounds__(kT) k_mg_wsum(int C, const uint8_t* __restrict__ inf,
+                                                 const double* __restrict__ x0,
+                                                 const double* __restrict__ x1,
+                                                 const CgScalars* sc, double* __restrict__ part_nx,
+                                                 double* __restrict__ part_n) {
+  __shared__ double sm[4];
+  const double* x = (sc->iter & 1) ? x1 : x0;
+  double a = 0.0, n = 0.0;
+  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT) {
+    const uint8_t f = inf[c];
+    if (is_act(f)) {
+      a += (double)(f & 7) * x[c];
+      n += (double)(f & 7);
+    }
+  }
+  write_part<kT>(a, part_nx, sm);
+  write_part<kT>(n, part_n, sm);
+}
+
+// Pressure grid for the velocity update: x at unknowns, 0 elsewhere.  With the
+// mean of b removed (all-Neumann systems, defined up to a constant) the constant
+// is fixed as Jacobi-preconditioned CG from x0 = 0 fixes it: every Krylov vector of
+// D^-1 K has zero D-weighted sum, so that solve ends with sum(n_i x_i) = 0.  The
+// constant matters: the update treats SOLID neighbours as p = 0.
+__global__ void __launch_bounds__(kT) k_mg_finish(int C, const uint8_t* __restrict__ inf,
+                                                   const double* __restrict__ x0,
+                                                   const double* __restrict__ x1,
+                                                   const CgScalars* sc,
+                                                   const double* __restrict__ part_nx,
+                                                   const double* __restrict__ part_n, int n_part,
+                                                   float* __restrict__ xg) {
+  __shared__ double sm[4];
+  double shift = 0.0;
+  if (part_nx) {
+    const double a = sum_part<kT>(part_nx, n_part, sm);
+    const double n = sum_part<kT>(part_n, n_part, sm);
+    shift = n > 0.0 ? a / n : 0.0;
+  }
+  const double* x = (sc->iter & 1) ? x1 : x0;
+  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT)
+    xg[c] = is_act(inf[c]) ? (float)(x[c] - shift) : 0.f;
+}
+
+// ---------------------------------------------------------------- launchers
+static inline int vec_blocks(int C) {
+  int nb = (C + 4 * kT - 1) / (4 * kT);
+  if (nb > kMgMaxVecBlocks) nb = kMgMaxVecBlocks;
+  return nb < 1 ? 1 : nb;
+}
+
+int mg_vec_blocks(int C) { return vec_blocks(C); }
+int mg_up_blocks(int nx, int ny) { return ((nx + 31) / 32) * ((ny + 31) / 32); }
+
+hipError_t mg_set_lds_limit(size_t bytes) {
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_coarse),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_t* label,
+                     const float* uf, const float* vf, int project_mean, CgScalars* sc) {
+  const int C = g.nx * g.ny, nb = vec_blocks(C);
+  k_mg_setup0<<<nb, kT, 0, s>>>(g, label, uf, vf, m.inf[0], m.b, m.part_sum, m.part_cnt);
+  k_mg_setup1<<<nb, kT, 0, s>>>(C, project_mean, m.inf[0], m.b, m.x[0], m.rl[0], m.part_sum,
+                                m.part_cnt, nb, m.part_bb, sc);
+  for (int l = 1; l < m.nlev; ++l) {
+    const int nf = m.nx[l - 1] * m.ny[l - 1], ncl = m.nx[l] * m.ny[l];
+    const int n = nf > ncl ? nf : ncl;
+    int blocks = (n + kT - 1) / kT;
+    if (blocks > 2048) blocks = 2048;
+    k_mg_coarsen<<<blocks, kT, 0, s>>>(m.nx[l - 1], m.ny[l - 1], m.lab[l - 1],
+                                       l - 1 >= 1 ? m.inf[l - 1] : nullptr, m.nx[l], m.ny[l],
+                                       m.lab[l]);
+  }
+  if (m.nlev > 1) {
+    const int l = m.nlev - 1, n = m.nx[l] * m.ny[l];
+    // info of the last level: a coarsening call with no coarse output
+    k_mg_coarsen<<<(n + kT - 1) / kT, kT, 0, s>>>(m.nx[l], m.ny[l], m.lab[l], m.inf[l], 0, 0,
+                                                  nullptr);
+  }
+}
+
+void launch_mg_iteration(hipStream_t s, const MgHost& m, int it, float omega, int max_iter,
+                         double tol, CgScalars* sc) {
+  const int C = m.nx[0] * m.ny[0], nb = vec_blocks(C);
+  // |r_0|^2 = |b'|^2 is in part_bb; later residuals are k_mg_xr's partials
+  const MgCheck chk{sc, it == 0 ? m.part_bb : m.part_rr, nb, m.part_bb, nb, tol * tol, max_iter};
+  auto level = [&](int l) { return MgLevel{m.nx[l], m.ny[l], m.inf[l], m.rl[l], m.xs[l], m.xo[l]}; };
+  // ---- V-cycle: z = B r (level 0's down kernel, or the one-workgroup kernel when the
+  // hierarchy starts there, runs the convergence test of this iteration first)
+  for (int l = 0; l < m.lc; ++l) {
+    const int blocks = ((m.nx[l + 1] + 15) / 16) * ((m.ny[l + 1] + 15) / 16);
+    if (l == 0)
+      k_mg_down<true><<<blocks, kT, 0, s>>>(level(0), level(1), omega, it, chk);
+    else
+      k_mg_down<false><<<blocks, kT, 0, s>>>(level(l), level(l + 1), omega, it, chk);
+  }
+  k_mg_coarse<<<1, kCoarseThreads, m.coarse_lds, s>>>(m.coarse, chk, it, m.lc == 0 ? 1 : 0,
+                                                      m.lc == 0 ? m.part_rz : nullptr);
+  for (int l = m.lc - 1; l >= 0; --l) {
+    const int blocks = mg_up_blocks(m.nx[l], m.ny[l]);
+    if (l == 0)
+      k_mg_up<true><<<blocks, kT, 0, s>>>(level(0), level(1), omega, sc, m.part_rz);
+    else
+      k_mg_up<false><<<blocks, kT, 0, s>>>(level(l), level(l + 1), omega, sc, nullptr);
+  }
+  // ---- CG direction and update
+  const int n_rz = m.lc == 0 ? 1 : mg_up_blocks(m.nx[0], m.ny[0]);
+  k_mg_dir<<<nb, kT, 0, s>>>(m.nx[0], m.ny[0], m.inf[0], m.xo[0], m.p[it & 1], m.p[(it + 1) & 1],
+                             it, m.part_rz, n_rz, m.part_pq, sc);
+  k_mg_xr<<<nb, kT, 0, s>>>(m.nx[0], m.ny[0], m.inf[0], m.p[(it + 1) & 1], m.x[it & 1],
+                            m.x[(it + 1) & 1], m.b, m.rl[0], it, m.part_pq, nb, m.part_rr, sc);
+}
+
+void launch_mg_finish(hipStream_t s, const MgHost& m, int project_mean, CgScalars* sc,
+                      float* xg) {
+  const int C = m.nx[0] * m.ny[0], nb = vec_blocks(C);
+  if (project_mean) {
+    // part_sum / part_cnt are free again after the setup
+    k_mg_wsum<<<nb, kT, 0, s>>>(C, m.inf[0], m.x[0], m.x[1], sc, m.part_sum, m.part_cnt);
+    k_mg_finish<<<nb, kT, 0, s>>>(C, m.inf[0], m.x[0], m.x[1], sc, m.part_sum, m.part_cnt, nb,
+                                  xg);
+  } else {
+    k_mg_finish<<<nb, kT, 0, s>>>(C, m.inf[0], m.x[0], m.x[1], sc, nullptr, nullptr, 0, xg);
+  }
+}
+
+}  // namespace fsg

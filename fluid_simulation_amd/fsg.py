@@ -22,7 +22,7 @@ OK = 0
 LIQUID, AIR, SOLID = 0, 1, 2
 SEMI_LAGRANGIAN, PIC, FLIP, PICFLIP, GRID_ADVECT = 0, 1, 2, 3, 4
 RK3, EULER, RK2 = 0, 1, 2
-SOLVER_PLAIN, SOLVER_PERSISTENT, SOLVER_REFINED = 0, 1, 2
+SOLVER_PLAIN, SOLVER_PERSISTENT, SOLVER_REFINED, SOLVER_MG = 0, 1, 2, 3
 VEL_FRONT, VEL_BACK, VEL_PREV, VEL_DIFF = 0, 1, 2, 3
 PH = dict(CLASSIFY=1, P2G=2, SAVE_PREV=3, GRAVITY=4, DIRICHLET=5, EXTEND=6,
           PRESSURE=7, DIFF=8, G2P_PIC=9, G2P_FLIP=10, G2P_PICFLIP=11, ADVECT=12,

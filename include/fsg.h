@@ -78,6 +78,10 @@ extern "C" {
 #define FSG_SOLVER_PERSISTENT 1 /* FP32 Jacobi-PCG in one persistent cooperative kernel */
 #define FSG_SOLVER_REFINED 2    /* FP32 PCG inside FP64 iterative refinement, FP64
                                    residual and reductions, persistent kernel */
+#define FSG_SOLVER_MG 3         /* CG preconditioned by a geometric-multigrid V-cycle
+                                   (FP32), FP64 solution, residual recomputed in FP64
+                                   every iteration, FP64 reductions; stops on the FP64
+                                   residual |b - A x| <= cg_tol |b| */
 
 /* Velocity buffers for fsg_{get,set}_velocity (include/MacGrid.h:162-176). */
 #define FSG_VEL_FRONT 0

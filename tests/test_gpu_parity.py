@@ -154,6 +154,8 @@ def _f64_twin(o, sc, tol=1e-12):
     (fsg.SOLVER_PLAIN, 1e-6, 2e-5),
     (fsg.SOLVER_PERSISTENT, 1e-6, 2e-5),
     (fsg.SOLVER_REFINED, 1e-10, 1e-5),
+    (fsg.SOLVER_MG, 1e-8, 1e-5),
+    (fsg.SOLVER_MG, 1e-10, 1e-5),
 ])
 def test_pressure_against_fp64(state, solver, tol, bar_p):
     sc, o, g = _pressure_inputs(state)
