@@ -33,6 +33,26 @@ HBM_PEAK_GBS = 8000.0
 CG_BYTES_PER_UNKNOWN_ITER = 47.0
 
 
+def kernel_bytes(name, C, P):
+    """Algorithmic HBM bytes of one launch of a kernel (each array read or written
+    once): C cells, P particles."""
+    table = {
+        # particles 16 B read + label 1 B; accumulators (float4) read + written once
+        "particle_scatter": 17.0 * P + 32.0 * C,
+        # info 1, p 4, x_old 8, b 8 read; x_new 8, r 4 written (dense level-0 cells)
+        "cg_mg_xr": 33.0 * C,
+        # info 1, z 4, p_old 4 read; p_new 4 written
+        "cg_mg_dir": 13.0 * C,
+        # info 1, r 4 read, xs 4 written; level-1 info 0.25, r 1 (a quarter of the cells)
+        "cg_mg_down0": 10.25 * C,
+        # info 1, r 4, xs 4 read, xo 4 written; level-1 xo 1
+        "cg_mg_up0": 14.0 * C,
+        # particles 16 B read + written, 4 grids sampled (16 B per cell), labels
+        "particle_gather": 32.0 * P + 17.0 * C,
+    }
+    return table.get(name)
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 

@@ -29,7 +29,8 @@ const char kValidateMsg[] =
 
 // Per-kernel HIP-event timer on the handle's stream.
 struct Timer {
-  int on = 0;  // 0 off, 1 every kernel, 2 only the pressure-solver kernels (cg_*)
+  int on = 0;  // 0 off, 1 every kernel, 2 only names starting with `prefix`
+  std::string prefix = "cg_";
   struct Pending {
     std::string name;
     hipEvent_t a, b;
@@ -118,6 +119,7 @@ struct fsg_domain {
   // multigrid form: hierarchy allocated on first use; iterations of the last solve
   MgHost mg{};
   bool mg_ready = false;
+  bool mg_solved = false;  // the last solve was the multigrid form (its FP64 x is valid)
   int mg_last_iters = 0;
   // host-side statistics of the plain solver
   fsg_stats host_stats{};
@@ -160,7 +162,8 @@ bool is_pow2(float x) {
 // Timed launch helper: wraps `body` in events when timing is on.
 template <typename F>
 void timed(fsg_domain* h, const char* name, F&& body) {
-  if (!h->timer.on || (h->timer.on == 2 && strncmp(name, "cg_", 3) != 0)) {
+  if (!h->timer.on ||
+      (h->timer.on == 2 && strncmp(name, h->timer.prefix.c_str(), h->timer.prefix.size()) != 0)) {
     body();
     return;
   }
@@ -318,13 +321,22 @@ int mg_solve(fsg_domain* h, int F) {
   });
   const int max_iter = h->prm.cg_max_iter;
   const double tol = h->prm.cg_tol;
-  const float omega = m.coarse.omega;
   int it = 0, batch = std::max(h->mg_last_iters + 1, 4);
   CgScalars hs{};
+  static const char* const kDownNames[4] = {"cg_mg_down0", "cg_mg_down1", "cg_mg_down2",
+                                            "cg_mg_down3+"};
+  static const char* const kUpNames[4] = {"cg_mg_up0", "cg_mg_up1", "cg_mg_up2", "cg_mg_up3+"};
   for (;;) {
-    for (int q = 0; q < batch && it <= max_iter; ++q, ++it)
-      timed(h, "cg_mg_iteration",
-            [&] { launch_mg_iteration(h->stream, m, it, omega, max_iter, tol, h->sc); });
+    for (int q = 0; q < batch && it <= max_iter; ++q, ++it) {
+      for (int l = 0; l < m.lc; ++l)
+        timed(h, kDownNames[std::min(l, 3)],
+              [&] { launch_mg_down(h->stream, m, l, it, max_iter, tol, h->sc); });
+      timed(h, "cg_mg_coarse", [&] { launch_mg_coarse(h->stream, m, it, max_iter, tol, h->sc); });
+      for (int l = m.lc - 1; l >= 0; --l)
+        timed(h, kUpNames[std::min(l, 3)], [&] { launch_mg_up(h->stream, m, l, h->sc); });
+      timed(h, "cg_mg_dir", [&] { launch_mg_dir(h->stream, m, it, h->sc); });
+      timed(h, "cg_mg_xr", [&] { launch_mg_xr(h->stream, m, it, h->sc); });
+    }
     HIPCHK(h, hipMemcpyAsync(&hs, h->sc, sizeof(hs), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     if (hs.done || it > max_iter) break;
@@ -333,6 +345,7 @@ int mg_solve(fsg_domain* h, int F) {
   timed(h, "cg_mg_finish",
         [&] { launch_mg_finish(h->stream, m, h->prm.project_mean, h->sc, h->xg); });
   h->mg_last_iters = hs.iter;
+  h->mg_solved = true;
   h->host_stats.cg_iterations = hs.iter;
   h->host_stats.cg_error = hs.err;
   h->host_stats.total_unknown_iterations += (int64_t)F * hs.iter;
@@ -352,6 +365,7 @@ int solve_pressure(fsg_domain* h) {
   int F = 0;
   HIPCHK(h, hipMemcpyAsync(&F, h->n_fluid, sizeof(int), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->mg_solved = false;
   h->host_stats.n_fluid = F;
   h->host_stats.cg_iterations = 0;
   h->host_stats.cg_error = 0.0;
@@ -459,7 +473,8 @@ int ph_pressure(fsg_domain* h, float dt, bool dirichlet, bool diff) {
   if (rc != FSG_OK) return rc;
   const float dt_over_density = dt / h->density;
   timed(h, "pressure_update", [&] {
-    launch_pressure_update(h->stream, h->g, h->n_fluid, h->label, h->xg, UF(h), VF(h), UB(h), VB(h),
+    launch_pressure_update(h->stream, h->g, h->n_fluid, h->label, h->xg,
+                           h->mg_solved ? h->mg.x[0] : nullptr, UF(h), VF(h), UB(h), VB(h),
                            dt_over_density, dirichlet ? 1 : 0, diff ? 1 : 0, h->u_prev, h->v_prev,
                            h->u_diff, h->v_diff);
   });
@@ -1064,6 +1079,13 @@ int fsg_set_timing(fsg_domain* h, int32_t enable) {
   if (!h) return FSG_E_INVALID;
   if (enable < 0 || enable > 2) return fail(h, FSG_E_INVALID, "fsg_set_timing: mode 0, 1 or 2");
   h->timer.on = enable;
+  return FSG_OK;
+}
+
+int fsg_set_timing_prefix(fsg_domain* h, const char* prefix) {
+  if (!h || !prefix || strlen(prefix) > 63)
+    return fail(h, FSG_E_INVALID, "fsg_set_timing_prefix: prefix of at most 63 characters");
+  h->timer.prefix = prefix;
   return FSG_OK;
 }
 

@@ -25,7 +25,8 @@ void launch_extend_sweep(hipStream_t s, const Geom& g, const uint8_t* label, flo
 void launch_classify_levelset(hipStream_t s, const Geom& g, const float* phi,
                               const uint8_t* solid, uint8_t* label);
 void launch_pressure_update(hipStream_t s, const Geom& g, const int* n_fluid,
-                            const uint8_t* label, float* xg, float* au, float* av,
+                            const uint8_t* label, float* xg, const double* x64, float* au,
+                            float* av,
                             float* bu, float* bv, float dt_over_density, int do_dirichlet,
                             int do_diff, const float* prev_u, const float* prev_v, float* diff_u,
                             float* diff_v);
@@ -154,8 +155,15 @@ int mg_up_blocks(int nx, int ny);
 hipError_t mg_set_lds_limit(size_t bytes);
 void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_t* label,
                      const float* uf, const float* vf, int project_mean, CgScalars* sc);
-void launch_mg_iteration(hipStream_t s, const MgHost& m, int it, float omega, int max_iter,
-                         double tol, CgScalars* sc);
+// one CG iteration: down(l) for l < lc, coarse, up(l) for l = lc-1 .. 0, dir, xr
+void launch_mg_down(hipStream_t s, const MgHost& m, int l, int it, int max_iter, double tol,
+                    CgScalars* sc);
+void launch_mg_coarse(hipStream_t s, const MgHost& m, int it, int max_iter, double tol,
+                      CgScalars* sc);
+void launch_mg_up(hipStream_t s, const MgHost& m, int l, CgScalars* sc);
+void launch_mg_dir(hipStream_t s, const MgHost& m, int it, CgScalars* sc);
+void launch_mg_xr(hipStream_t s, const MgHost& m, int it, CgScalars* sc);
+// x at the unknowns -> xg (FP32) and m.x[0] (FP64, read by the pressure update)
 void launch_mg_finish(hipStream_t s, const MgHost& m, int project_mean, CgScalars* sc,
                       float* xg);
 

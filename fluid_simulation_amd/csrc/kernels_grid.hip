@@ -305,9 +305,14 @@ __global__ void __launch_bounds__(256) k_classify_levelset(Geom g, const float* 
 //         A := B) ahead of the host's swap.
 // xg is zeroed at cells that are not LIQUID, so that it can seed the next solve
 // (each thread writes only its own cell, and only where no thread reads it).
+// x64 != null (the multigrid form's FP64 solution): the pressure differences are
+// formed in FP64 and the update rounded once to float.  The pressure of a large
+// domain is ~1e6 while its gradient is O(1), so differencing FP32 pressures would
+// cost ~1e-4 relative accuracy that the FP64 solve has.
 __global__ void __launch_bounds__(256) k_pressure_update(
     Geom g, const int* __restrict__ n_fluid, const uint8_t* __restrict__ label,
-    float* __restrict__ xg, float* __restrict__ au, float* __restrict__ av,
+    float* __restrict__ xg, const double* __restrict__ x64, float* __restrict__ au,
+    float* __restrict__ av,
     float* __restrict__ bu, float* __restrict__ bv, float dt_over_density, int do_dirichlet,
     int do_diff, const float* __restrict__ prev_u, const float* __restrict__ prev_v,
     float* __restrict__ diff_u, float* __restrict__ diff_v) {
@@ -323,7 +328,13 @@ __global__ void __launch_bounds__(256) k_pressure_update(
   if (*n_fluid > 0) {
     nu = bu[c];
     nv = bv[c];
-    if (L == kLiquid || Lim == kLiquid || Ljm == kLiquid) {
+    if (x64 && (L == kLiquid || Lim == kLiquid || Ljm == kLiquid)) {
+      const double p = L == kLiquid ? x64[c] : 0.0;
+      const double p_im = Lim == kLiquid ? x64[cim] : 0.0;
+      const double p_jm = Ljm == kLiquid ? x64[cjm] : 0.0;
+      nu = (float)((double)a_u - (double)dt_over_density * (p - p_im) / (double)g.dx);
+      nv = (float)((double)a_v - (double)dt_over_density * (p - p_jm) / (double)g.dy);
+    } else if (L == kLiquid || Lim == kLiquid || Ljm == kLiquid) {
       // + k * (...) with k = 0 (src/FluidSolver.cpp:443-460) kept as + 0
       const float p = L == kLiquid ? xg[c] + 0.0f : 0.f;
       const float p_im = Lim == kLiquid ? xg[cim] + 0.0f : 0.f;
@@ -545,12 +556,13 @@ void launch_classify_levelset(hipStream_t s, const Geom& g, const float* phi,
   k_classify_levelset<<<nblocks((int64_t)g.nx * g.ny), 256, 0, s>>>(g, phi, solid, label);
 }
 void launch_pressure_update(hipStream_t s, const Geom& g, const int* n_fluid,
-                            const uint8_t* label, float* xg, float* au, float* av,
+                            const uint8_t* label, float* xg, const double* x64, float* au,
+                            float* av,
                             float* bu, float* bv, float dt_over_density, int do_dirichlet,
                             int do_diff, const float* prev_u, const float* prev_v, float* diff_u,
                             float* diff_v) {
   k_pressure_update<<<nblocks((int64_t)g.nx * g.ny), 256, 0, s>>>(
-      g, n_fluid, label, xg, au, av, bu, bv, dt_over_density, do_dirichlet, do_diff, prev_u,
+      g, n_fluid, label, xg, x64, au, av, bu, bv, dt_over_density, do_dirichlet, do_diff, prev_u,
       prev_v, diff_u, diff_v);
 }
 void launch_diff(hipStream_t s, const Geom& g, const float* u, const float* v, const float* pu,

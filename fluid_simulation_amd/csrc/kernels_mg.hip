@@ -22,7 +22,11 @@
 // simulation on large grids", SCA 2010): a coarse cell is AIR if any child is AIR,
 // else LIQUID if any child is LIQUID, else SOLID; each level is the same integer
 // stencil on its own labels; prolongation is cell-centred bilinear (weights 9, 3,
-// 3, 1 / 16) and restriction its transpose; smoothing is two damped-Jacobi sweeps
+// 3, 1 / 16), except that the weight of a coarse neighbour that is SOLID or
+// outside the grid goes to the fine cell's own coarse cell (constant extrapolation
+// across a wall: the Neumann condition; interpolating a zero there acts like a
+// Dirichlet wall and costs 2-4x the iterations on closed domains), and restriction
+// is its exact transpose; smoothing is two damped-Jacobi sweeps
 // before and after the coarse correction.  Every component is a symmetric linear
 // operator, so the V-cycle is an SPD preconditioner for CG.  In the scaled form a
 // level-(l+1) right-hand side is P^T of the level-l residual (the 4^-l scale of the
@@ -47,9 +51,30 @@ namespace fsg {
 namespace {
 
 constexpr int kT = 256;         // threads of the tile and vector kernels
-constexpr uint8_t kAct = 0x80;  // info bit: the cell is an unknown
+constexpr uint8_t kAct = 0x80;    // info bit: the cell is an unknown
+constexpr uint8_t kWall = 0x40;   // info bit: SOLID (also used for cells outside the grid)
 
 __device__ __forceinline__ bool is_act(uint8_t f) { return (f & kAct) != 0; }
+__device__ __forceinline__ bool is_wall(uint8_t f) { return (f & kWall) != 0; }
+
+// Bilinear prolongation of one fine cell from its coarse cell (value e0) and the
+// x-, y- and diagonal neighbours (e, info f): a wall neighbour's weight is moved to
+// the fine cell's own coarse cell.
+__device__ __forceinline__ float prolong_value(float e0, float ex, uint8_t fx, float ey, uint8_t fy,
+                                               float exy, uint8_t fxy) {
+  float w0 = 0.5625f, v = 0.f;
+  if (is_wall(fx)) w0 += 0.1875f; else v += 0.1875f * ex;
+  if (is_wall(fy)) w0 += 0.1875f; else v += 0.1875f * ey;
+  if (is_wall(fxy)) w0 += 0.0625f; else v += 0.0625f * exy;
+  return v + w0 * e0;
+}
+
+// Extra restriction weight of a fine cell whose own coarse cell is the target:
+// the weights its prolongation moved from wall neighbours (fx, fy, fxy).
+__device__ __forceinline__ float moved_weight(uint8_t fx, uint8_t fy, uint8_t fxy) {
+  return (is_wall(fx) ? 0.1875f : 0.f) + (is_wall(fy) ? 0.1875f : 0.f) +
+         (is_wall(fxy) ? 0.0625f : 0.f);
+}
 __device__ __forceinline__ float nns_of(uint8_t f) { return (float)(f & 7); }
 
 // Unknown flag and non-SOLID neighbour count of cell (i, j) of a label grid
@@ -64,7 +89,7 @@ __device__ __forceinline__ uint8_t make_info(const uint8_t* __restrict__ lab, in
   n += j > 0 && lab[c - nx] != kSolid;
   n += j < ny - 1 && lab[c + nx] != kSolid;
   const bool act = lab[c] == kLiquid && n > 0;
-  return (uint8_t)((act ? kAct : 0) | n);
+  return (uint8_t)((act ? kAct : 0) | (lab[c] == kSolid ? kWall : 0) | n);
 }
 
 // Sum of n block partials, the same fixed order in every block (NT threads;
@@ -250,11 +275,16 @@ __global__ void __launch_bounds__(kT) k_mg_down(MgLevel f, MgLevel c, float omeg
     return;
   }
   constexpr int W3 = 38, W2 = 36, W1 = 34;
+  constexpr int WC = 18;  // coarse cells of the tile and their neighbours
   __shared__ float s_r[W3 * W3], s_x1[W3 * W3], s_x2[W2 * W2], s_res[W1 * W1];
-  __shared__ uint8_t s_inf[W3 * W3];
+  __shared__ uint8_t s_inf[W3 * W3], s_cinf[WC * WC];
   const int tiles_x = (c.nx + 15) >> 4;
   const int bx = blockIdx.x % tiles_x, by = blockIdx.x / tiles_x;
   const int fi0 = bx * 32, fj0 = by * 32;
+  for (int t = threadIdx.x; t < WC * WC; t += kT) {
+    const int I = bx * 16 + t % WC - 1, J = by * 16 + t / WC - 1;
+    s_cinf[t] = (I >= 0 && I < c.nx && J >= 0 && J < c.ny) ? c.inf[I + J * c.nx] : kWall;
+  }
   for (int t = threadIdx.x; t < W3 * W3; t += kT) {
     const int i = fi0 + t % W3 - 3, j = fj0 + t / W3 - 3;
     float rv = 0.f;
@@ -306,7 +336,7 @@ __global__ void __launch_bounds__(kT) k_mg_down(MgLevel f, MgLevel c, float omeg
     if (I < c.nx && J < c.ny) {
       const int cc = I + J * c.nx;
       float v = 0.f;
-      if (is_act(c.inf[cc])) {
+      if (is_act(s_cinf[(ci + 1) + (cj + 1) * WC])) {
         // fine offsets 2ci-1 .. 2ci+2 are s_res columns 2ci .. 2ci+3
         const float w[4] = {0.25f, 0.75f, 0.75f, 0.25f};
 #pragma unroll
@@ -315,6 +345,16 @@ __global__ void __launch_bounds__(kT) k_mg_down(MgLevel f, MgLevel c, float omeg
           const float s = 0.25f * row[0] + 0.75f * row[1] + 0.75f * row[2] + 0.25f * row[3];
           v += w[b] * s;
         }
+        // the four children moved the weight of their wall neighbours here
+        auto ci_at = [&](int di, int dj) { return s_cinf[(ci + 1 + di) + (cj + 1 + dj) * WC]; };
+#pragma unroll
+        for (int b = 1; b <= 2; ++b)
+#pragma unroll
+          for (int a = 1; a <= 2; ++a) {
+            const int dx = a == 1 ? -1 : 1, dy = b == 1 ? -1 : 1;
+            const float m = moved_weight(ci_at(dx, 0), ci_at(0, dy), ci_at(dx, dy));
+            v += m * s_res[(2 * cj + b) * W1 + 2 * ci + a];
+          }
       }
       c.r[cc] = v;
     }
@@ -331,7 +371,7 @@ __global__ void __launch_bounds__(kT) k_mg_up(MgLevel f, MgLevel c, float omega,
   if (sc->done) return;
   constexpr int W2 = 36, W1 = 34, WC = 20;
   __shared__ float s_r[W2 * W2], s_xc[W2 * W2], s_y1[W1 * W1], s_e[WC * WC];
-  __shared__ uint8_t s_inf[W2 * W2];
+  __shared__ uint8_t s_inf[W2 * W2], s_cinf[WC * WC];
   __shared__ double sm[4];
   const int tiles_x = (f.nx + 31) >> 5;
   const int bx = blockIdx.x % tiles_x, by = blockIdx.x / tiles_x;
@@ -339,7 +379,9 @@ __global__ void __launch_bounds__(kT) k_mg_up(MgLevel f, MgLevel c, float omega,
   const int ci0 = (fi0 >> 1) - 2, cj0 = (fj0 >> 1) - 2;  // coarse frame origin
   for (int t = threadIdx.x; t < WC * WC; t += kT) {
     const int I = ci0 + t % WC, J = cj0 + t / WC;
-    s_e[t] = (I >= 0 && I < c.nx && J >= 0 && J < c.ny) ? c.xo[I + J * c.nx] : 0.f;
+    const bool in = I >= 0 && I < c.nx && J >= 0 && J < c.ny;
+    s_e[t] = in ? c.xo[I + J * c.nx] : 0.f;
+    s_cinf[t] = in ? c.inf[I + J * c.nx] : kWall;
   }
   for (int t = threadIdx.x; t < W2 * W2; t += kT) {
     const int i = fi0 + t % W2 - 2, j = fj0 + t / W2 - 2;
@@ -365,12 +407,10 @@ __global__ void __launch_bounds__(kT) k_mg_up(MgLevel f, MgLevel c, float omega,
     const int I = i >> 1, J = j >> 1;  // floor for the -2 .. -1 halo too
     const int In = (i & 1) ? I + 1 : I - 1;
     const int Jn = (j & 1) ? J + 1 : J - 1;
-    const float* e = s_e;
-    const float e00 = e[(I - ci0) + (J - cj0) * WC];
-    const float e10 = e[(In - ci0) + (J - cj0) * WC];
-    const float e01 = e[(I - ci0) + (Jn - cj0) * WC];
-    const float e11 = e[(In - ci0) + (Jn - cj0) * WC];
-    s_xc[t] += 0.5625f * e00 + 0.1875f * e10 + 0.1875f * e01 + 0.0625f * e11;
+    const int q00 = (I - ci0) + (J - cj0) * WC, q10 = (In - ci0) + (J - cj0) * WC;
+    const int q01 = (I - ci0) + (Jn - cj0) * WC, q11 = (In - ci0) + (Jn - cj0) * WC;
+    s_xc[t] += prolong_value(s_e[q00], s_e[q10], s_cinf[q10], s_e[q01], s_cinf[q01], s_e[q11],
+                             s_cinf[q11]);
   }
   __syncthreads();
   for (int t = threadIdx.x; t < W1 * W1; t += kT) {
@@ -487,6 +527,19 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, Mg
           }
           v += wt[b] * s;
         }
+        auto cinf = [&](int a, int b) -> uint8_t {
+          return (a >= 0 && a < cnx && b >= 0 && b < cny) ? INF[co + a + b * cnx] : kWall;
+        };
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int e = 0; e < 2; ++e) {
+            const int i = 2 * I + e, j = 2 * J + b;
+            if (i >= nx || j >= ny) continue;
+            const int dx = e == 0 ? -1 : 1, dy = b == 0 ? -1 : 1;
+            const float m = moved_weight(cinf(I + dx, J), cinf(I, J + dy), cinf(I + dx, J + dy));
+            v += m * t[i + j * nx];
+          }
       }
       R[co + q] = v;
     }
@@ -529,11 +582,15 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, Mg
         const int In = (i & 1) ? I + 1 : I - 1;
         const int Jn = (j & 1) ? J + 1 : J - 1;
         const bool iok = In >= 0 && In < cnx, jok = Jn >= 0 && Jn < cny;
+        const uint8_t* cinf = INF + co;
         const float e00 = e[I + J * cnx];
         const float e10 = iok ? e[In + J * cnx] : 0.f;
         const float e01 = jok ? e[I + Jn * cnx] : 0.f;
         const float e11 = iok && jok ? e[In + Jn * cnx] : 0.f;
-        v = x[q] + (0.5625f * e00 + 0.1875f * e10 + 0.1875f * e01 + 0.0625f * e11);
+        const uint8_t f10 = iok ? cinf[In + J * cnx] : kWall;
+        const uint8_t f01 = jok ? cinf[I + Jn * cnx] : kWall;
+        const uint8_t f11 = iok && jok ? cinf[In + Jn * cnx] : kWall;
+        v = x[q] + prolong_value(e00, e10, f10, e01, f01, e11, f11);
       }
       t[q] = v;
     }
@@ -682,6 +739,7 @@ __global__ void __launch_bounds__(kT) k_mg_finish(int C, const uint8_t* __restri
                                                    const CgScalars* sc,
                                                    const double* __restrict__ part_nx,
                                                    const double* __restrict__ part_n, int n_part,
+                                                   double* __restrict__ xout,
                                                    float* __restrict__ xg) {
   __shared__ double sm[4];
   double shift = 0.0;
@@ -691,8 +749,11 @@ __global__ void __launch_bounds__(kT) k_mg_finish(int C, const uint8_t* __restri
     shift = n > 0.0 ? a / n : 0.0;
   }
   const double* x = (sc->iter & 1) ? x1 : x0;
-  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT)
-    xg[c] = is_act(inf[c]) ? (float)(x[c] - shift) : 0.f;
+  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT) {
+    const double v = is_act(inf[c]) ? x[c] - shift : 0.0;
+    xout[c] = v;  // may be x[c] itself: each thread reads and writes only its cells
+    xg[c] = (float)v;
+  }
 }
 
 // ---------------------------------------------------------------- launchers
@@ -733,34 +794,56 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
   }
 }
 
-void launch_mg_iteration(hipStream_t s, const MgHost& m, int it, float omega, int max_iter,
-                         double tol, CgScalars* sc) {
-  const int C = m.nx[0] * m.ny[0], nb = vec_blocks(C);
+MgCheck mg_check_args(const MgHost& m, int it, int max_iter, double tol, CgScalars* sc) {
+  const int nb = vec_blocks(m.nx[0] * m.ny[0]);
   // |r_0|^2 = |b'|^2 is in part_bb; later residuals are k_mg_xr's partials
-  const MgCheck chk{sc, it == 0 ? m.part_bb : m.part_rr, nb, m.part_bb, nb, tol * tol, max_iter};
-  auto level = [&](int l) { return MgLevel{m.nx[l], m.ny[l], m.inf[l], m.rl[l], m.xs[l], m.xo[l]}; };
-  // ---- V-cycle: z = B r (level 0's down kernel, or the one-workgroup kernel when the
-  // hierarchy starts there, runs the convergence test of this iteration first)
-  for (int l = 0; l < m.lc; ++l) {
-    const int blocks = ((m.nx[l + 1] + 15) / 16) * ((m.ny[l + 1] + 15) / 16);
-    if (l == 0)
-      k_mg_down<true><<<blocks, kT, 0, s>>>(level(0), level(1), omega, it, chk);
-    else
-      k_mg_down<false><<<blocks, kT, 0, s>>>(level(l), level(l + 1), omega, it, chk);
-  }
+  return MgCheck{sc, it == 0 ? m.part_bb : m.part_rr, nb, m.part_bb, nb, tol * tol, max_iter};
+}
+
+static MgLevel mg_level(const MgHost& m, int l) {
+  return MgLevel{m.nx[l], m.ny[l], m.inf[l], m.rl[l], m.xs[l], m.xo[l]};
+}
+
+// One CG iteration = V-cycle (z = B r) + direction + update.  Level 0's down kernel
+// (or the one-workgroup kernel when the hierarchy starts there) runs the stop test
+// of this iteration first.
+void launch_mg_down(hipStream_t s, const MgHost& m, int l, int it, int max_iter, double tol,
+                    CgScalars* sc) {
+  const int blocks = ((m.nx[l + 1] + 15) / 16) * ((m.ny[l + 1] + 15) / 16);
+  const MgCheck chk = mg_check_args(m, it, max_iter, tol, sc);
+  if (l == 0)
+    k_mg_down<true><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega, it, chk);
+  else
+    k_mg_down<false><<<blocks, kT, 0, s>>>(mg_level(m, l), mg_level(m, l + 1), m.coarse.omega, it,
+                                           chk);
+}
+
+void launch_mg_coarse(hipStream_t s, const MgHost& m, int it, int max_iter, double tol,
+                      CgScalars* sc) {
+  const MgCheck chk = mg_check_args(m, it, max_iter, tol, sc);
   k_mg_coarse<<<1, kCoarseThreads, m.coarse_lds, s>>>(m.coarse, chk, it, m.lc == 0 ? 1 : 0,
                                                       m.lc == 0 ? m.part_rz : nullptr);
-  for (int l = m.lc - 1; l >= 0; --l) {
-    const int blocks = mg_up_blocks(m.nx[l], m.ny[l]);
-    if (l == 0)
-      k_mg_up<true><<<blocks, kT, 0, s>>>(level(0), level(1), omega, sc, m.part_rz);
-    else
-      k_mg_up<false><<<blocks, kT, 0, s>>>(level(l), level(l + 1), omega, sc, nullptr);
-  }
-  // ---- CG direction and update
+}
+
+void launch_mg_up(hipStream_t s, const MgHost& m, int l, CgScalars* sc) {
+  const int blocks = mg_up_blocks(m.nx[l], m.ny[l]);
+  if (l == 0)
+    k_mg_up<true><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega, sc,
+                                        m.part_rz);
+  else
+    k_mg_up<false><<<blocks, kT, 0, s>>>(mg_level(m, l), mg_level(m, l + 1), m.coarse.omega, sc,
+                                         nullptr);
+}
+
+void launch_mg_dir(hipStream_t s, const MgHost& m, int it, CgScalars* sc) {
+  const int nb = vec_blocks(m.nx[0] * m.ny[0]);
   const int n_rz = m.lc == 0 ? 1 : mg_up_blocks(m.nx[0], m.ny[0]);
   k_mg_dir<<<nb, kT, 0, s>>>(m.nx[0], m.ny[0], m.inf[0], m.xo[0], m.p[it & 1], m.p[(it + 1) & 1],
                              it, m.part_rz, n_rz, m.part_pq, sc);
+}
+
+void launch_mg_xr(hipStream_t s, const MgHost& m, int it, CgScalars* sc) {
+  const int nb = vec_blocks(m.nx[0] * m.ny[0]);
   k_mg_xr<<<nb, kT, 0, s>>>(m.nx[0], m.ny[0], m.inf[0], m.p[(it + 1) & 1], m.x[it & 1],
                             m.x[(it + 1) & 1], m.b, m.rl[0], it, m.part_pq, nb, m.part_rr, sc);
 }
@@ -772,9 +855,10 @@ void launch_mg_finish(hipStream_t s, const MgHost& m, int project_mean, CgScalar
     // part_sum / part_cnt are free again after the setup
     k_mg_wsum<<<nb, kT, 0, s>>>(C, m.inf[0], m.x[0], m.x[1], sc, m.part_sum, m.part_cnt);
     k_mg_finish<<<nb, kT, 0, s>>>(C, m.inf[0], m.x[0], m.x[1], sc, m.part_sum, m.part_cnt, nb,
-                                  xg);
+                                  m.x[0], xg);
   } else {
-    k_mg_finish<<<nb, kT, 0, s>>>(C, m.inf[0], m.x[0], m.x[1], sc, nullptr, nullptr, 0, xg);
+    k_mg_finish<<<nb, kT, 0, s>>>(C, m.inf[0], m.x[0], m.x[1], sc, nullptr, nullptr, 0, m.x[0],
+                                  xg);
   }
 }
 

@@ -39,7 +39,7 @@ EXPORTS = [
     "fsg_step", "fsg_run_phase", "fsg_validate_pool", "fsg_get_pressure", "fsg_get_rhs",
     "fsg_get_fluid_indices", "fsg_get_valid_masks", "fsg_get_ext_info",
     "fsg_get_reject_flags", "fsg_get_stats", "fsg_reset_stats", "fsg_set_timing",
-    "fsg_get_timing", "fsg_reset_timing",
+    "fsg_get_timing", "fsg_reset_timing", "fsg_set_timing_prefix",
 ]
 
 
@@ -119,6 +119,7 @@ def load(path: str | None = None):
             "fsg_get_timing": ([H, C.c_int32, P(C.c_char_p), P(C.c_double), P(C.c_int64)],
                                C.c_int),
             "fsg_reset_timing": ([H], C.c_int),
+            "fsg_set_timing_prefix": ([H, C.c_char_p], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -313,6 +314,11 @@ class Domain:
     def set_timing(self, mode):
         """0 off, 1 (or True) every kernel, 2 only the pressure-solver kernels."""
         self._chk(self._L.fsg_set_timing(self._h, int(mode)), "fsg_set_timing")
+
+    def set_timing_prefix(self, prefix: str):
+        """Timing mode 2 times only kernels whose name starts with `prefix`."""
+        self._chk(self._L.fsg_set_timing_prefix(self._h, prefix.encode()),
+                  "fsg_set_timing_prefix")
 
     def timing(self) -> dict:
         n = self._L.fsg_get_timing(self._h, 0, None, None, None)

@@ -193,8 +193,11 @@ int fsg_reset_stats(fsg_domain *h);
 
 /* ---- timing --------------------------------------------------------------- */
 /* Per-kernel HIP-event timing on the handle's stream: 0 off (default), 1 every
-   kernel, 2 only the pressure-solver kernels. */
+   kernel, 2 only the kernels whose timing name starts with the prefix set by
+   fsg_set_timing_prefix (default "cg_": the pressure-solver kernels). */
 int fsg_set_timing(fsg_domain *h, int32_t enable);
+/* Name prefix selected by timing mode 2 (at most 63 characters). */
+int fsg_set_timing_prefix(fsg_domain *h, const char *prefix);
 /* Kernel names and their summed durations (ms) and launch counts since the last
    reset; returns the number of kernels (writes at most `cap` entries). */
 int fsg_get_timing(fsg_domain *h, int32_t cap, const char **names, double *ms,
