@@ -74,6 +74,10 @@ class MarkerParticleSet {
 
   // FluidDomain attaches its set to its device context
   void attach(FsgContext* ctx);
+  // Appends the lattice (xs[i], ys[j], vx, vy), j outer, i inner (FluidSource's
+  // emission order); on the GPU when attached.
+  void addLattice(const std::vector<MyFloat>& xs, const std::vector<MyFloat>& ys, MyFloat vx,
+                  MyFloat vy);
 
  private:
   std::vector<MarkerParticle>& hostRead() const;

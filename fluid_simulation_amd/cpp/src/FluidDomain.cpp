@@ -17,15 +17,18 @@ FluidSource::~FluidSource() {}
 
 // Every time_step seconds, emit a lattice of particles over the area with spacing
 // delta / 2.5 (about 6.25 per cell), rows in y, columns in x, accumulated in
-// MyFloat -- the reference emitter's lattice (src/FluidDomain.cpp:29-52).
+// MyFloat -- the reference emitter's lattice (src/FluidDomain.cpp:29-52).  The two
+// axes are accumulated here (each value is a chain of float additions); the
+// lattice itself is appended on the GPU when the set belongs to a domain.
 void FluidSource::update(MarkerParticleSet& particle_set, MyFloat dt) {
   if (isFinished()) return;
   if (_time_since_last >= _time_step) {
     const MyFloat x_step = _delta_x / 2.5;
     const MyFloat y_step = _delta_y / 2.5;
-    for (MyFloat y = _area.y_min; y < _area.y_max; y += y_step)
-      for (MyFloat x = _area.x_min; x < _area.x_max; x += x_step)
-        particle_set.addParticle(MarkerParticle(x, y, _x_velocity, _y_velocity));
+    std::vector<MyFloat> xs, ys;
+    for (MyFloat x = _area.x_min; x < _area.x_max; x += x_step) xs.push_back(x);
+    for (MyFloat y = _area.y_min; y < _area.y_max; y += y_step) ys.push_back(y);
+    particle_set.addLattice(xs, ys, _x_velocity, _y_velocity);
     _time_since_last = 0;
     _n_spawns++;
   }
@@ -69,29 +72,17 @@ const MarkerParticleSet& FluidDomain::markerParticleSet() const { return _partic
 const MyFloat FluidDomain::density() const { return _density; }
 const MyFloat FluidDomain::picRatio() const { return _pic_ratio; }
 
-// Forward-Euler upwind advection of the level set by the cell-centred velocity
-// (host), with the reference's loop bounds (j over sizeX, i over sizeY) and its
-// factor 5 on the step.  The reference's cell-centred average reads one face past
-// the last column / row (an assert in its debug build); that face is clamped here.
+// FluidDomain::advectLevelSet (src/FluidDomain.cpp:104-127) on the GPU with the
+// front velocity: forward Euler with the reference's factor 5 and first-order
+// upwind gradients.  The face past the last column / row, which the reference
+// reads out of range (an assert in its -g build), is clamped; on a non-square grid
+// only the min(nx, ny) square its swapped loop bounds reach is advected and the
+// rest becomes 0, as in the reference's fresh level set.
 void FluidDomain::advectLevelSet(MyFloat dt) {
-  LevelSet next(_level_set.sizeX(), _level_set.sizeY(), _level_set.lengthX(),
-                _level_set.lengthY());
-  const FsgContext::GridMirror& g = _ctx->grids();
-  const int nx = _SIZE_X, ny = _SIZE_Y;
-  auto U = [&](int i, int j) { return g.u[FSG_VEL_FRONT][(size_t)i + (size_t)j * nx]; };
-  auto V = [&](int i, int j) { return g.v[FSG_VEL_FRONT][(size_t)i + (size_t)j * nx]; };
-  for (int j = 0; j < _level_set.sizeX(); ++j)
-    for (int i = 0; i < _level_set.sizeY(); ++i) {
-      const int i1 = i + 1 < nx ? i + 1 : nx - 1;
-      const int j1 = j + 1 < ny ? j + 1 : ny - 1;
-      const MyFloat vel_x = (U(i, j) + U(i1, j)) / 2;
-      const MyFloat vel_y = (V(i, j) + V(i, j1)) / 2;
-      const MyFloat gx = _level_set.computeUpwindGradientX(i, j, vel_x);
-      const MyFloat gy = _level_set.computeUpwindGradientY(i, j, vel_y);
-      const MyFloat rate = -(vel_x * gx + vel_y * gy);
-      next(i, j) = _level_set(i, j) + rate * dt * 5;
-    }
-  _level_set = std::move(next);
+  _ctx->beforeDevice();
+  _ctx->check(fsg_set_level_set(_ctx->handle(), _level_set.raw()), "fsg_set_level_set");
+  _ctx->check(fsg_advect_level_set(_ctx->handle(), dt), "fsg_advect_level_set");
+  _ctx->check(fsg_get_level_set(_ctx->handle(), _level_set.raw()), "fsg_get_level_set");
 }
 
 void FluidDomain::classifyCells(LevelSet& level_set) {

@@ -33,6 +33,20 @@ std::vector<MarkerParticle>& MarkerParticleSet::hostWrite() {
 }
 
 void MarkerParticleSet::addParticle(MarkerParticle p) { hostWrite().push_back(p); }
+
+void MarkerParticleSet::addLattice(const std::vector<MyFloat>& xs, const std::vector<MyFloat>& ys,
+                                   MyFloat vx, MyFloat vy) {
+  if (_ctx) {
+    _ctx->beforeDevice();
+    _ctx->check(fsg_emit_lattice(_ctx->handle(), xs.data(), (int32_t)xs.size(), ys.data(),
+                                 (int32_t)ys.size(), vx, vy),
+                "fsg_emit_lattice");
+    _ctx->afterDevice();
+    return;
+  }
+  for (MyFloat y : ys)
+    for (MyFloat x : xs) _particles.push_back(MarkerParticle(x, y, vx, vy));
+}
 void MarkerParticleSet::reserve(int particle_count) { _particles.reserve(particle_count); }
 void MarkerParticleSet::clear() { hostWrite().clear(); }
 

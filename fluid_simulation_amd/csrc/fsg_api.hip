@@ -118,6 +118,10 @@ struct fsg_domain {
   int last_F = 0, last_grid = 0, last_kpt = 0;
   // multigrid form: hierarchy allocated on first use; iterations of the last solve
   MgHost mg{};
+  int* pcounts = nullptr;   // particles per cell (particle-pressure term), on first use
+  float* phi2 = nullptr;    // second level-set buffer (advection), on first use
+  float* emit_buf = nullptr;  // emitter coordinates
+  size_t emit_cap = 0;
   bool mg_ready = false;
   bool mg_solved = false;  // the last solve was the multigrid form (its FP64 x is valid)
   int mg_last_iters = 0;
@@ -471,10 +475,21 @@ int solve_pressure(fsg_domain* h) {
 int ph_pressure(fsg_domain* h, float dt, bool dirichlet, bool diff) {
   int rc = solve_pressure(h);
   if (rc != FSG_OK) return rc;
+  // particle-count pressure term (the reference multiplies it by k = 0)
+  const float ppk = h->prm.particle_pressure_k;
+  int* counts = nullptr;
+  if (ppk != 0.f) {
+    if (!h->pcounts && dalloc(h, &h->pcounts, ncell(h)) != hipSuccess)
+      return fail(h, FSG_E_NOMEM, "particle counts: device allocation failed");
+    counts = h->pcounts;
+    timed(h, "count_particles",
+          [&] { launch_count_particles(h->stream, h->g, h->pt, h->np, counts); });
+  }
   const float dt_over_density = dt / h->density;
   timed(h, "pressure_update", [&] {
     launch_pressure_update(h->stream, h->g, h->n_fluid, h->label, h->xg,
-                           h->mg_solved ? h->mg.x[0] : nullptr, UF(h), VF(h), UB(h), VB(h),
+                           h->mg_solved ? h->mg.x[0] : nullptr, counts, ppk, UF(h), VF(h), UB(h),
+                           VB(h),
                            dt_over_density, dirichlet ? 1 : 0, diff ? 1 : 0, h->u_prev, h->v_prev,
                            h->u_diff, h->v_diff);
   });
@@ -661,6 +676,7 @@ int fsg_destroy(fsg_domain* h) {
   h->timer.collect();
   for (void* p : h->allocations) (void)hipFree(p);
   if (h->pt) (void)hipFree(h->pt);
+  if (h->emit_buf) (void)hipFree(h->emit_buf);
   if (h->reject) (void)hipFree(h->reject);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -672,7 +688,8 @@ const char* fsg_last_error(const fsg_domain* h) { return h ? h->err.c_str() : "n
 int fsg_set_params(fsg_domain* h, const fsg_params* p) {
   if (!h || !p) return FSG_E_INVALID;
   if (p->cg_max_iter < 0 || p->ext_layers < 0 || p->ext_layers > 14 || p->integrator < 0 ||
-      p->integrator > 2 || p->solver < 0 || p->solver > 3 || !(p->cg_tol >= 0.0))
+      p->integrator > 3 || p->solver < 0 || p->solver > 3 || !(p->cg_tol >= 0.0) ||
+      !std::isfinite(p->particle_pressure_k))
     return fail(h, FSG_E_INVALID, "fsg_set_params: value out of range");
   if ((p->solver == FSG_SOLVER_PERSISTENT || p->solver == FSG_SOLVER_REFINED) &&
       h->max_coop_blocks <= 0)
@@ -741,6 +758,7 @@ static int reserve_particles(fsg_domain* h, int64_t n) {
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));
   if (h->pt) (void)hipFree(h->pt);
+  if (h->emit_buf) (void)hipFree(h->emit_buf);
   if (h->reject) (void)hipFree(h->reject);
   h->pt = np;
   h->reject = nr;
@@ -763,6 +781,54 @@ int fsg_add_particles(fsg_domain* h, const float* xyuv, int64_t n) {
                            h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->np += n;
+  return FSG_OK;
+}
+
+int fsg_emit_lattice(fsg_domain* h, const float* xs, int32_t nxp, const float* ys, int32_t nyp,
+                     float vx, float vy) {
+  if (!h || nxp < 0 || nyp < 0 || (nxp > 0 && !xs) || (nyp > 0 && !ys)) return FSG_E_INVALID;
+  const int64_t n = (int64_t)nxp * nyp;
+  if (n == 0) return FSG_OK;
+  int rc = reserve_particles(h, h->np + n);
+  if (rc) return rc;
+  const size_t need = (size_t)nxp + (size_t)nyp;
+  if (need > h->emit_cap) {
+    if (h->emit_buf) HIPCHK(h, hipFree(h->emit_buf));
+    h->emit_buf = nullptr;
+    h->emit_cap = 0;
+    HIPCHK(h, hipMalloc(&h->emit_buf, need * sizeof(float)));
+    h->emit_cap = need;
+  }
+  HIPCHK(h, hipMemcpyAsync(h->emit_buf, xs, (size_t)nxp * sizeof(float), hipMemcpyHostToDevice,
+                           h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->emit_buf + nxp, ys, (size_t)nyp * sizeof(float),
+                           hipMemcpyHostToDevice, h->stream));
+  timed(h, "emit_lattice", [&] {
+    launch_emit_lattice(h->stream, h->pt, h->np, h->emit_buf, nxp, h->emit_buf + nxp, nyp, vx, vy);
+  });
+  // the host arrays may be released once the copies are done
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->np += n;
+  return FSG_OK;
+}
+
+int fsg_advect_level_set(fsg_domain* h, float dt) {
+  if (!h) return FSG_E_INVALID;
+  if (!h->phi) return fail(h, FSG_E_INVALID, "fsg_advect_level_set: no level set (fsg_set_level_set)");
+  if (!h->phi2 && dalloc(h, &h->phi2, ncell(h)) != hipSuccess)
+    return fail(h, FSG_E_NOMEM, "level set: device allocation failed");
+  timed(h, "advect_level_set",
+        [&] { launch_advect_level_set(h->stream, h->g, h->phi, h->phi2, UF(h), VF(h), dt); });
+  std::swap(h->phi, h->phi2);
+  return FSG_OK;
+}
+
+int fsg_get_level_set(fsg_domain* h, float* phi) {
+  if (!h || check_grid_ptr(h, phi)) return FSG_E_INVALID;
+  if (!h->phi) return fail(h, FSG_E_INVALID, "fsg_get_level_set: no level set");
+  HIPCHK(h, hipMemcpyAsync(phi, h->phi, ncell(h) * sizeof(float), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
   return FSG_OK;
 }
 

@@ -105,15 +105,30 @@ struct VelField {
   }
 };
 
-constexpr int kRK3 = 0, kEuler = 1, kRK2 = 2;
+constexpr int kRK3 = 0, kEuler = 1, kRK2 = 2, kRK4 = 3;
 
 // Displacement of one ODE step in Vec2 float ops (include/FluidSolver.h:120-141).
 //  RK3 (include/OdeSolver.h:102-113): ((k1*2 + k2*3 + k3*4) * h * 1) / 9 with k2 at
 //      x + ((k1*h)*1)/2 and k3 at x + ((k2*h)*3)/4.
 //  Euler (include/OdeSolver.h:78-86): f(x + h) * h, the scalar added to both axes.
 //  RK2 (new, explicit midpoint): k2 at x + (k1*h)*0.5, displacement k2*h.
+//  RK4 (include/OdeSolver.h:117-142): the template's stages with its scalar shift of
+//      both coordinates (Vec2 + MyFloat, like Euler): f1 = f(x), f2 = f3 = f(x + h/2),
+//      f4 = f(x + h), displacement (f1 + f2*2 + f3*2 + f4) * h / 6.  (The template
+//      cannot be instantiated with the solver's Vec2, which has no MyFloat * Vec2;
+//      the Vec2 * MyFloat order is used.)
 __device__ __forceinline__ void ode_disp(const VelField& f, int integrator, float x, float y,
                                          float h, float& ddx, float& ddy) {
+  if (integrator == kRK4) {
+    float f1x, f1y, f2x, f2y, f4x, f4y;
+    f.at(x, y, f1x, f1y);
+    const float hh = h / 2.0f;
+    f.at(x + hh, y + hh, f2x, f2y);
+    f.at(x + h, y + h, f4x, f4y);
+    ddx = (f1x + f2x * 2.0f + f2x * 2.0f + f4x) * h / 6.0f;
+    ddy = (f1y + f2y * 2.0f + f2y * 2.0f + f4y) * h / 6.0f;
+    return;
+  }
   if (integrator == kEuler) {
     float kx, ky;
     f.at(x + h, y + h, kx, ky);

@@ -25,8 +25,8 @@ void launch_extend_sweep(hipStream_t s, const Geom& g, const uint8_t* label, flo
 void launch_classify_levelset(hipStream_t s, const Geom& g, const float* phi,
                               const uint8_t* solid, uint8_t* label);
 void launch_pressure_update(hipStream_t s, const Geom& g, const int* n_fluid,
-                            const uint8_t* label, float* xg, const double* x64, float* au,
-                            float* av,
+                            const uint8_t* label, float* xg, const double* x64,
+                            const int* counts, float ppk, float* au, float* av,
                             float* bu, float* bv, float dt_over_density, int do_dirichlet,
                             int do_diff, const float* prev_u, const float* prev_v, float* diff_u,
                             float* diff_v);
@@ -42,6 +42,12 @@ void launch_sl_gather(hipStream_t s, const Geom& g, const uint8_t* label, const 
 void launch_advect_grid(hipStream_t s, const Geom& g, float4* pt, int64_t np, const float* uf,
                         const float* vf, int integrator, float dt);
 void launch_fill(hipStream_t s, float* a, int64_t n, float val);
+void launch_count_particles(hipStream_t s, const Geom& g, const float4* pt, int64_t np,
+                            int* counts);
+void launch_emit_lattice(hipStream_t s, float4* pt, int64_t base, const float* xs, int nxp,
+                         const float* ys, int nyp, float vx, float vy);
+void launch_advect_level_set(hipStream_t s, const Geom& g, const float* phi, float* out,
+                             const float* u, const float* v, float dt);
 
 // ---------------------------------------------------------------- pressure
 constexpr int kScanChunk = 2048;  // cells per block of the liquid count / compaction

@@ -311,8 +311,8 @@ __global__ void __launch_bounds__(256) k_classify_levelset(Geom g, const float* 
 // cost ~1e-4 relative accuracy that the FP64 solve has.
 __global__ void __launch_bounds__(256) k_pressure_update(
     Geom g, const int* __restrict__ n_fluid, const uint8_t* __restrict__ label,
-    float* __restrict__ xg, const double* __restrict__ x64, float* __restrict__ au,
-    float* __restrict__ av,
+    float* __restrict__ xg, const double* __restrict__ x64, const int* __restrict__ counts,
+    float ppk, float* __restrict__ au, float* __restrict__ av,
     float* __restrict__ bu, float* __restrict__ bv, float dt_over_density, int do_dirichlet,
     int do_diff, const float* __restrict__ prev_u, const float* __restrict__ prev_v,
     float* __restrict__ diff_u, float* __restrict__ diff_v) {
@@ -328,17 +328,25 @@ __global__ void __launch_bounds__(256) k_pressure_update(
   if (*n_fluid > 0) {
     nu = bu[c];
     nv = bv[c];
+    // particle pressure k * (n > 9 ? n - 8 : 0) (src/FluidSolver.cpp:441-453); the
+    // reference's k is 0 (then the term is + 0, as there)
+    float pp = 0.f, pp_im = 0.f, pp_jm = 0.f;
+    if (counts) {
+      const int n0 = counts[c], n1 = counts[cim], n2 = counts[cjm];
+      pp = ppk * (float)(n0 > 9 ? n0 - 8 : 0);
+      pp_im = ppk * (float)(n1 > 9 ? n1 - 8 : 0);
+      pp_jm = ppk * (float)(n2 > 9 ? n2 - 8 : 0);
+    }
     if (x64 && (L == kLiquid || Lim == kLiquid || Ljm == kLiquid)) {
-      const double p = L == kLiquid ? x64[c] : 0.0;
-      const double p_im = Lim == kLiquid ? x64[cim] : 0.0;
-      const double p_jm = Ljm == kLiquid ? x64[cjm] : 0.0;
+      const double p = L == kLiquid ? x64[c] + (double)pp : 0.0;
+      const double p_im = Lim == kLiquid ? x64[cim] + (double)pp_im : 0.0;
+      const double p_jm = Ljm == kLiquid ? x64[cjm] + (double)pp_jm : 0.0;
       nu = (float)((double)a_u - (double)dt_over_density * (p - p_im) / (double)g.dx);
       nv = (float)((double)a_v - (double)dt_over_density * (p - p_jm) / (double)g.dy);
     } else if (L == kLiquid || Lim == kLiquid || Ljm == kLiquid) {
-      // + k * (...) with k = 0 (src/FluidSolver.cpp:443-460) kept as + 0
-      const float p = L == kLiquid ? xg[c] + 0.0f : 0.f;
-      const float p_im = Lim == kLiquid ? xg[cim] + 0.0f : 0.f;
-      const float p_jm = Ljm == kLiquid ? xg[cjm] + 0.0f : 0.f;
+      const float p = L == kLiquid ? xg[c] + pp : 0.f;
+      const float p_im = Lim == kLiquid ? xg[cim] + pp_im : 0.f;
+      const float p_jm = Ljm == kLiquid ? xg[cjm] + pp_jm : 0.f;
       const float pdx = p - p_im;
       const float pdy = p - p_jm;
       nu = a_u - dt_over_density * pdx / g.dx;
@@ -361,6 +369,72 @@ __global__ void __launch_bounds__(256) k_pressure_update(
     diff_u[c] = nu - prev_u[c];
     diff_v[c] = nv - prev_v[c];
   }
+}
+
+// ----------------------------------------------- particle-pressure counts
+// Particles per cell for the volume term of the pressure update
+// (src/FluidSolver.cpp:352-360): cell = (int)((x / lengthX) * sizeX), the
+// reference's unclamped index clamped into the grid.
+__global__ void __launch_bounds__(256) k_count_particles(Geom g, const float4* __restrict__ pt,
+                                                         int64_t np, int* __restrict__ counts) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= np) return;
+  const float4 q = pt[p];
+  int x = (int)((q.x / g.len_x) * (float)g.nx);
+  int y = (int)((q.y / g.len_y) * (float)g.ny);
+  x = clampi(x, 0, g.nx - 1);
+  y = clampi(y, 0, g.ny - 1);
+  atomicAdd(counts + x + y * g.nx, 1);
+}
+
+// ---------------------------------------------------- FluidSource lattice
+// FluidSource::update (src/FluidDomain.cpp:29-52) appended on the device: particle
+// base + j*nxp + i = (xs[i], ys[j], vx, vy).  xs and ys are the emitter's MyFloat
+// accumulations (x += delta/2.5 while x < max), formed on the host because each
+// value is the result of a sequence of float additions.
+__global__ void __launch_bounds__(256) k_emit_lattice(float4* __restrict__ pt, int64_t base,
+                                                      const float* __restrict__ xs, int nxp,
+                                                      const float* __restrict__ ys, int nyp,
+                                                      float vx, float vy) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (int64_t)nxp * nyp) return;
+  const int i = (int)(t % nxp), j = (int)(t / nxp);
+  pt[base + t] = make_float4(xs[i], ys[j], vx, vy);
+}
+
+// ------------------------------------------------------- level-set advection
+// FluidDomain::advectLevelSet (src/FluidDomain.cpp:104-127): forward Euler with the
+// reference's factor 5, first-order upwind gradients (LevelSet::
+// computeUpwindGradientX/Y, src/LevelSet.cpp:23-59) and the cell-centred velocity
+// (u(i,j) + u(i+1,j)) / 2 (MacGrid::velX, include/MacGrid.h:29-38).  Differences:
+// the face past the last column / row, which the reference reads out of range (an
+// assert in its -g build), is clamped; the reference's loops run j over sizeX and
+// i over sizeY, so on a non-square grid only the min(nx, ny) square is computed and
+// the rest of the new level set keeps its fresh value 0.
+__global__ void __launch_bounds__(256) k_advect_level_set(Geom g, const float* __restrict__ phi,
+                                                          float* __restrict__ out,
+                                                          const float* __restrict__ u,
+                                                          const float* __restrict__ v, float dt) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  const int nx = g.nx, ny = g.ny;
+  if (c >= nx * ny) return;
+  const int i = c % nx, j = c / nx;
+  const int m = nx < ny ? nx : ny;
+  if (i >= m || j >= m) {
+    out[c] = 0.f;
+    return;
+  }
+  const int i1 = i + 1 < nx ? i + 1 : nx - 1;
+  const int j1 = j + 1 < ny ? j + 1 : ny - 1;
+  const float vel_x = (u[c] + u[i1 + j * nx]) / 2.0f;
+  const float vel_y = (v[c] + v[i + j1 * nx]) / 2.0f;
+  const int ip = clampi(i + 1, 0, nx - 1), im = clampi(i - 1, 0, nx - 1);
+  const int jp = clampi(j + 1, 0, ny - 1), jm = clampi(j - 1, 0, ny - 1);
+  const float p0 = phi[c];
+  const float gx = vel_x < 0.f ? phi[ip + j * nx] - p0 : p0 - phi[im + j * nx];
+  const float gy = vel_y < 0.f ? phi[i + jp * nx] - p0 : p0 - phi[i + jm * nx];
+  const float rate = -(vel_x * gx + vel_y * gy);
+  out[c] = p0 + rate * dt * 5.0f;
 }
 
 // ------------------------------------------------------------ diff (alone)
@@ -556,14 +630,31 @@ void launch_classify_levelset(hipStream_t s, const Geom& g, const float* phi,
   k_classify_levelset<<<nblocks((int64_t)g.nx * g.ny), 256, 0, s>>>(g, phi, solid, label);
 }
 void launch_pressure_update(hipStream_t s, const Geom& g, const int* n_fluid,
-                            const uint8_t* label, float* xg, const double* x64, float* au,
-                            float* av,
+                            const uint8_t* label, float* xg, const double* x64,
+                            const int* counts, float ppk, float* au, float* av,
                             float* bu, float* bv, float dt_over_density, int do_dirichlet,
                             int do_diff, const float* prev_u, const float* prev_v, float* diff_u,
                             float* diff_v) {
   k_pressure_update<<<nblocks((int64_t)g.nx * g.ny), 256, 0, s>>>(
-      g, n_fluid, label, xg, x64, au, av, bu, bv, dt_over_density, do_dirichlet, do_diff, prev_u,
-      prev_v, diff_u, diff_v);
+      g, n_fluid, label, xg, x64, counts, ppk, au, av, bu, bv, dt_over_density, do_dirichlet,
+      do_diff, prev_u, prev_v, diff_u, diff_v);
+}
+void launch_count_particles(hipStream_t s, const Geom& g, const float4* pt, int64_t np,
+                            int* counts) {
+  launch_fill(s, reinterpret_cast<float*>(counts), (int64_t)g.nx * g.ny, 0.f);  // int 0 bits
+  if (np == 0) return;
+  k_count_particles<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(g, pt, np, counts);
+}
+void launch_emit_lattice(hipStream_t s, float4* pt, int64_t base, const float* xs, int nxp,
+                         const float* ys, int nyp, float vx, float vy) {
+  const int64_t n = (int64_t)nxp * nyp;
+  if (n == 0) return;
+  k_emit_lattice<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(pt, base, xs, nxp, ys, nyp, vx, vy);
+}
+void launch_advect_level_set(hipStream_t s, const Geom& g, const float* phi, float* out,
+                             const float* u, const float* v, float dt) {
+  const int n = g.nx * g.ny;
+  k_advect_level_set<<<(n + 255) / 256, 256, 0, s>>>(g, phi, out, u, v, dt);
 }
 void launch_diff(hipStream_t s, const Geom& g, const float* u, const float* v, const float* pu,
                  const float* pv, float* du, float* dv) {

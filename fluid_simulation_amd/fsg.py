@@ -21,7 +21,7 @@ _LOCK = threading.Lock()
 OK = 0
 LIQUID, AIR, SOLID = 0, 1, 2
 SEMI_LAGRANGIAN, PIC, FLIP, PICFLIP, GRID_ADVECT = 0, 1, 2, 3, 4
-RK3, EULER, RK2 = 0, 1, 2
+RK3, EULER, RK2, RK4 = 0, 1, 2, 3
 SOLVER_PLAIN, SOLVER_PERSISTENT, SOLVER_REFINED, SOLVER_MG = 0, 1, 2, 3
 VEL_FRONT, VEL_BACK, VEL_PREV, VEL_DIFF = 0, 1, 2, 3
 PH = dict(CLASSIFY=1, P2G=2, SAVE_PREV=3, GRAVITY=4, DIRICHLET=5, EXTEND=6,
@@ -39,7 +39,8 @@ EXPORTS = [
     "fsg_step", "fsg_run_phase", "fsg_validate_pool", "fsg_get_pressure", "fsg_get_rhs",
     "fsg_get_fluid_indices", "fsg_get_valid_masks", "fsg_get_ext_info",
     "fsg_get_reject_flags", "fsg_get_stats", "fsg_reset_stats", "fsg_set_timing",
-    "fsg_get_timing", "fsg_reset_timing", "fsg_set_timing_prefix",
+    "fsg_get_timing", "fsg_reset_timing", "fsg_set_timing_prefix", "fsg_emit_lattice",
+    "fsg_advect_level_set", "fsg_get_level_set",
 ]
 
 
@@ -54,7 +55,7 @@ class Params(C.Structure):
         ("solver", C.c_int32), ("project_mean", C.c_int32), ("check_every", C.c_int32),
         ("refine_inner_tol", C.c_double), ("refine_max_outer", C.c_int32),
         ("record_ext_info", C.c_int32), ("pcg_blocks", C.c_int32), ("pcg_kpt", C.c_int32),
-        ("warm_start", C.c_int32), ("reserved", C.c_int32 * 1),
+        ("warm_start", C.c_int32), ("particle_pressure_k", C.c_float),
     ]
 
 
@@ -120,6 +121,10 @@ def load(path: str | None = None):
                                C.c_int),
             "fsg_reset_timing": ([H], C.c_int),
             "fsg_set_timing_prefix": ([H, C.c_char_p], C.c_int),
+            "fsg_emit_lattice": ([H, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_float,
+                                  C.c_float], C.c_int),
+            "fsg_advect_level_set": ([H, C.c_float], C.c_int),
+            "fsg_get_level_set": ([H, C.c_void_p], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -310,6 +315,19 @@ class Domain:
 
     def reset_stats(self):
         self._chk(self._L.fsg_reset_stats(self._h), "fsg_reset_stats")
+
+    def emit_lattice(self, xs, ys, vx=0.0, vy=0.0):
+        """FluidSource::update's lattice (xs[i], ys[j]) appended on the device."""
+        xs = _f32(np.asarray(xs, np.float32).ravel())
+        ys = _f32(np.asarray(ys, np.float32).ravel())
+        self._chk(self._L.fsg_emit_lattice(self._h, xs.ctypes.data, len(xs), ys.ctypes.data,
+                                           len(ys), float(vx), float(vy)), "fsg_emit_lattice")
+
+    def advect_level_set(self, dt):
+        self._chk(self._L.fsg_advect_level_set(self._h, float(dt)), "fsg_advect_level_set")
+
+    def level_set(self):
+        return self._grid("fsg_get_level_set", np.float32)
 
     def set_timing(self, mode):
         """0 off, 1 (or True) every kernel, 2 only the pressure-solver kernels."""

@@ -72,6 +72,8 @@ extern "C" {
 #define FSG_RK3 0    /* RK3::step, include/OdeSolver.h:102-113 (the reference's choice) */
 #define FSG_EULER 1  /* EulerExplicit::step, include/OdeSolver.h:78-86 */
 #define FSG_RK2 2    /* new: explicit midpoint */
+#define FSG_RK4 3    /* RK4::step, include/OdeSolver.h:117-142 (with the Euler-style scalar
+                        shift of both coordinates its Vec2 form implies) */
 
 /* Pressure solver forms (all replace Eigen::ConjugateGradient, FluidSolver.h:114). */
 #define FSG_SOLVER_PLAIN 0      /* FP32 Jacobi-PCG, 3 kernels per iteration, host loop */
@@ -110,7 +112,8 @@ typedef struct fsg_params {
   int32_t warm_start;     /* persistent solvers: 1 starts CG from the previous step's
                              pressure (same stopping test |r| < tol |b|); 0 (default)
                              starts from x0 = 0 like Eigen's solve() */
-  int32_t reserved[1];
+  float particle_pressure_k; /* 0 (src/FluidSolver.cpp:443): k of the particle-count
+                                pressure term k * (n > 9 ? n - 8 : 0) */
 } fsg_params;
 
 /* Counters of the last pressure solve and running totals. */
@@ -190,6 +193,17 @@ int fsg_get_ext_info(fsg_domain *h, uint8_t *info_u, uint8_t *info_v);
 int fsg_get_reject_flags(fsg_domain *h, uint8_t *flags);
 int fsg_get_stats(fsg_domain *h, fsg_stats *s);
 int fsg_reset_stats(fsg_domain *h);
+
+/* ---- emission and level set ------------------------------------------------ */
+/* FluidSource::update (src/FluidDomain.cpp:29-52) on the device: appends
+   nxp * nyp particles (xs[i], ys[j], vx, vy), j outer, i inner.  xs / ys are the
+   emitter's MyFloat accumulations (x = x_min; x < x_max; x += delta_x / 2.5). */
+int fsg_emit_lattice(fsg_domain *h, const float *xs, int32_t nxp, const float *ys, int32_t nyp,
+                     float vx, float vy);
+/* FluidDomain::advectLevelSet (src/FluidDomain.cpp:104-127) on the level set set by
+   fsg_set_level_set, with the front velocity. */
+int fsg_advect_level_set(fsg_domain *h, float dt);
+int fsg_get_level_set(fsg_domain *h, float *phi);
 
 /* ---- timing --------------------------------------------------------------- */
 /* Per-kernel HIP-event timing on the handle's stream: 0 off (default), 1 every

@@ -13,7 +13,7 @@ _LIB = None
 _LOCK = threading.Lock()
 
 LIQUID, AIR, SOLID = 0, 1, 2
-RK3, EULER, RK2 = 0, 1, 2
+RK3, EULER, RK2, RK4 = 0, 1, 2, 3
 SCHEME_SEMI_LAGRANGIAN, SCHEME_PIC, SCHEME_FLIP, SCHEME_PICFLIP, SCHEME_GRID = range(5)
 PH = dict(CLASSIFY=1, P2G=2, SAVE_PREV=3, GRAVITY=4, DIRICHLET=5, EXTEND=6,
           PRESSURE=7, DIFF=8, G2P_PIC=9, G2P_FLIP=10, G2P_PICFLIP=11, ADVECT=12,
@@ -47,7 +47,7 @@ def _state_type(R):
             ("ext_info_u", U8), ("ext_info_v", U8),
             ("fluid_index", C.POINTER(C.c_int32)), ("pressure", P), ("rhs", P),
             ("n_fluid", C.c_int32), ("cg_iterations", C.c_int32),
-            ("cg_error", C.c_double),
+            ("cg_error", C.c_double), ("particle_pressure_k", R),
         ]
     return State
 
@@ -81,6 +81,8 @@ def lib():
                 getattr(L, "orc_rk_displacement" + sfx).argtypes = [
                     C.POINTER(S), R, R, R, C.POINTER(R), C.POINTER(R)]
                 getattr(L, "orc_rk_displacement" + sfx).restype = None
+                getattr(L, "orc_advect_level_set" + sfx).argtypes = [C.POINTER(S), C.POINTER(R), R]
+                getattr(L, "orc_advect_level_set" + sfx).restype = None
             _LIB = L
     return _LIB
 
@@ -160,6 +162,11 @@ class OracleSim:
     def set_phi(self, phi):
         self.phi = np.array(phi, dtype=self.R, copy=True, order="C")
         self.s.phi = _ptr(self.phi, self.cR)
+
+    def advect_level_set(self, dt):
+        """FluidDomain::advectLevelSet on this sim's level set (set_phi)."""
+        getattr(lib(), "orc_advect_level_set" + self.sfx)(C.byref(self.s),
+                                                          _ptr(self.phi, self.cR), dt)
 
     def vel(self, which="front"):
         f = self.s.front

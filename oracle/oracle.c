@@ -425,9 +425,10 @@ static void cg_solve(const lap_t *L, const REAL *b, REAL *x, int max_iter, doubl
  * (:347-350); b = divVelX + divVelY (:413, unscaled); solve; write the back
  * buffer at faces where (i,j), (i-1,j) or (i,j-1) is LIQUID (:439) with
  * p = 0 for non-LIQUID cells (:455-460) and vel - dt/density * dp / dx
- * (:471-474); swap (:482).  The per-cell particle count (:352-360) is dead (it is
- * multiplied by k = 0, :443) and is not restated; its `+ k * ...` term is kept as
- * `+ 0`. */
+ * (:471-474); swap (:482).  The per-cell particle count (:352-360) feeds the term
+ * k * (n > 9 ? n - 8 : 0) added to each LIQUID pressure (:441-453); the
+ * reference's k is 0.  Its count index (int)((x / lengthX) * sizeX) and the
+ * (i-1, j) / (i, j-1) lookups are clamped here (the reference asserts on them). */
 static void pressure(STATE *s, REAL dt) {
   const int nx = s->nx, ny = s->ny;
   const size_t C = (size_t)nx * ny;
@@ -479,7 +480,15 @@ static void pressure(STATE *s, REAL dt) {
     for (int k = 0; k < F_; ++k) s->rhs[cell_of[k]] = b[k];
 
   REAL *ub = U_B(s), *vb = V_B(s);
-  const REAL kvol = (REAL)0.0;
+  const REAL kvol = s->particle_pressure_k;
+  int32_t *npart = (int32_t *)calloc(C, sizeof(int32_t));
+  for (int64_t q = 0; q < s->n_particles; ++q) {
+    const REAL *pt = s->particles + 4 * q;
+    int cx = (int)((pt[0] / ((REAL)nx * s->dx)) * (REAL)nx);
+    int cy = (int)((pt[1] / ((REAL)ny * s->dy)) * (REAL)ny);
+    npart[IDX(s, clampi(cx, 0, nx - 1), clampi(cy, 0, ny - 1))]++;
+  }
+#define PPRESS(c) (kvol * (REAL)(npart[c] > 9 ? npart[c] - 8 : 0))
   for (int j = 0; j < ny; ++j)
     for (int i = 0; i < nx; ++i) {
       int im = clampi(i - 1, 0, nx - 1);
@@ -488,9 +497,9 @@ static void pressure(STATE *s, REAL dt) {
       int idx_im = s->fluid_index[IDX(s, im, j)];
       int idx_jm = s->fluid_index[IDX(s, i, jm)];
       if (idx >= 0 || idx_im >= 0 || idx_jm >= 0) {
-        REAL p = idx >= 0 ? x[idx] + kvol * 0 : 0;
-        REAL p_im = idx_im >= 0 ? x[idx_im] + kvol * 0 : 0;
-        REAL p_jm = idx_jm >= 0 ? x[idx_jm] + kvol * 0 : 0;
+        REAL p = idx >= 0 ? x[idx] + PPRESS(IDX(s, i, j)) : 0;
+        REAL p_im = idx_im >= 0 ? x[idx_im] + PPRESS(IDX(s, im, j)) : 0;
+        REAL p_jm = idx_jm >= 0 ? x[idx_jm] + PPRESS(IDX(s, i, jm)) : 0;
         REAL pdx = p - p_im;
         REAL pdy = p - p_jm;
         size_t c = IDX(s, i, j);
@@ -500,6 +509,8 @@ static void pressure(STATE *s, REAL dt) {
       }
     }
   swap_vel(s);
+#undef PPRESS
+  free(npart);
   free(cell_of); free(diag); free(b); free(x);
 }
 
@@ -562,10 +573,23 @@ static void advect_particles(STATE *s, REAL dt, int reject) {
  *   Euler include/OdeSolver.h:78-86: f(x + h) * h with the scalar shift of both
  *         coordinates (Vec2::operator+(MyFloat))
  *   RK2   midpoint (new; no counterpart in the reference): k2 at x + k1*h*(1/2),
- *         displacement k2*h. */
+ *         displacement k2*h.
+ *   RK4   include/OdeSolver.h:117-142 with the Vec2 + MyFloat scalar shift its
+ *         stages imply: f1 = f(x), f2 = f3 = f(x + h/2), f4 = f(x + h),
+ *         (f1 + f2*2 + f3*2 + f4) * h / 6 (Vec2 * MyFloat order; the template itself
+ *         does not compile with the solver's Vec2). */
 static void rk_disp(const STATE *s, const REAL *u, const REAL *v, REAL x, REAL y, REAL h,
                     REAL *dx_out, REAL *dy_out) {
   const REAL one = (REAL)1.0;
+  if (s->integrator == ORC_RK4) {
+    const REAL hh = h / (REAL)2.0;
+    REAL f1x = vel_x_at(s, u, x, y), f1y = vel_y_at(s, v, x, y);
+    REAL f2x = vel_x_at(s, u, x + hh, y + hh), f2y = vel_y_at(s, v, x + hh, y + hh);
+    REAL f4x = vel_x_at(s, u, x + h, y + h), f4y = vel_y_at(s, v, x + h, y + h);
+    *dx_out = (f1x + f2x * (REAL)2 + f2x * (REAL)2 + f4x) * h / (REAL)6;
+    *dy_out = (f1y + f2y * (REAL)2 + f2y * (REAL)2 + f4y) * h / (REAL)6;
+    return;
+  }
   if (s->integrator == ORC_EULER) {
     REAL px = x + h, py = y + h;
     *dx_out = vel_x_at(s, u, px, py) * h;
@@ -586,6 +610,34 @@ static void rk_disp(const STATE *s, const REAL *u, const REAL *v, REAL x, REAL y
   REAL k3x = vel_x_at(s, u, bx, by), k3y = vel_y_at(s, v, bx, by);
   *dx_out = (k1x * (REAL)2 + k2x * (REAL)3 + k3x * (REAL)4) * h * one / (REAL)9;
   *dy_out = (k1y * (REAL)2 + k2y * (REAL)3 + k3y * (REAL)4) * h * one / (REAL)9;
+}
+
+/* FluidDomain::advectLevelSet, src/FluidDomain.cpp:104-127, on phi (nx*ny, in
+ * place): forward Euler, change rate -(vx gx + vy gy) times dt times 5, upwind
+ * gradients of LevelSet::computeUpwindGradientX/Y (src/LevelSet.cpp:23-59, clamped
+ * indices), cell-centred front velocity (u(i)+u(i+1))/2 (MacGrid::velX).  The
+ * reference reads u(nx, j) / v(i, ny) out of range (an assert in its -g build):
+ * clamped here.  Its loops run j over sizeX and i over sizeY; cells outside the
+ * min(nx, ny) square keep the new level set's initial 0. */
+void F(orc_advect_level_set)(STATE *s, REAL *phi, REAL dt) {
+  const int nx = s->nx, ny = s->ny, m = nx < ny ? nx : ny;
+  REAL *out = (REAL *)calloc((size_t)nx * ny, sizeof(REAL));
+  const REAL *u = U_F(s), *v = V_F(s);
+  for (int j = 0; j < m; ++j)
+    for (int i = 0; i < m; ++i) {
+      const int i1 = i + 1 < nx ? i + 1 : nx - 1, j1 = j + 1 < ny ? j + 1 : ny - 1;
+      const REAL vx = (u[IDX(s, i, j)] + u[IDX(s, i1, j)]) / 2;
+      const REAL vy = (v[IDX(s, i, j)] + v[IDX(s, i, j1)]) / 2;
+      const REAL p0 = phi[IDX(s, i, j)];
+      const REAL gx = vx < 0 ? phi[IDX(s, clampi(i + 1, 0, nx - 1), j)] - p0
+                             : p0 - phi[IDX(s, clampi(i - 1, 0, nx - 1), j)];
+      const REAL gy = vy < 0 ? phi[IDX(s, i, clampi(j + 1, 0, ny - 1))] - p0
+                             : p0 - phi[IDX(s, i, clampi(j - 1, 0, ny - 1))];
+      const REAL rate = -(vx * gx + vy * gy);
+      out[IDX(s, i, j)] = p0 + rate * dt * 5;
+    }
+  memcpy(phi, out, sizeof(REAL) * (size_t)nx * ny);
+  free(out);
 }
 
 void F(orc_rk_displacement)(const STATE *s, REAL x, REAL y, REAL h, REAL *ox, REAL *oy) {

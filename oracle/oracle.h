@@ -25,7 +25,7 @@ extern "C" {
 #endif
 
 enum { ORC_LIQUID = 0, ORC_AIR = 1, ORC_SOLID = 2 };       /* include/MacGrid.h:14-17 */
-enum { ORC_RK3 = 0, ORC_EULER = 1, ORC_RK2 = 2 };          /* include/OdeSolver.h:78-113 */
+enum { ORC_RK3 = 0, ORC_EULER = 1, ORC_RK2 = 2, ORC_RK4 = 3 }; /* include/OdeSolver.h:78-142 */
 enum { ORC_SCHEME_SEMI_LAGRANGIAN = 0, ORC_SCHEME_PIC = 1, ORC_SCHEME_FLIP = 2,
        ORC_SCHEME_PICFLIP = 3, ORC_SCHEME_GRID_ADVECT = 4 };
 
@@ -67,6 +67,7 @@ enum {
     R *rhs;                   /* per cell: b for LIQUID cells, else 0 */            \
     int32_t n_fluid, cg_iterations;                                                 \
     double cg_error;                                                                \
+    R particle_pressure_k;    /* 0 (FluidSolver.cpp:443) */                         \
   } orc_state##SFX;                                                                 \
   void orc_defaults##SFX(orc_state##SFX *s, int32_t nx, int32_t ny, R length_x,     \
                          R length_y, R density, R pic_ratio);                       \
@@ -77,7 +78,8 @@ enum {
   R orc_interp##SFX(const R *grid, int32_t nx, int32_t ny, R dx, R dy, R x, R y);   \
   void orc_splat##SFX(R *grid, int32_t nx, int32_t ny, R dx, R dy, R x, R y, R val);\
   void orc_rk_displacement##SFX(const orc_state##SFX *s, R x, R y, R h, R *ox,      \
-                                R *oy);
+                                R *oy);                                             \
+  void orc_advect_level_set##SFX(orc_state##SFX *s, R *phi, R dt);
 
 ORC_DECLARE(float, _f32)
 ORC_DECLARE(double, _f64)

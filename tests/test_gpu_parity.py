@@ -243,7 +243,7 @@ def test_sl_scatter_within_1e5(state):
     assert rel_err(ub, uo) <= 1e-5 and rel_err(vb, vo) <= 1e-5
 
 
-@pytest.mark.parametrize("integrator", [0, 1, 2])
+@pytest.mark.parametrize("integrator", [0, 1, 2, 3])
 def test_sl_gather_bit_exact(state, integrator):
     sc, o, g = _pair(state)
     o.set_params(integrator=integrator)
