@@ -1,19 +1,23 @@
 """Benchmark: FLIP steps/s on the 1024x1024 dam break (one MI355X).
 
 One step = FluidSolver::stepPICFLIP (src/FluidSolver.cpp:211-251) on the GPU:
-classify, P2G, gravity, Dirichlet, 2-layer extension, Jacobi-PCG pressure
-projection to relative residual 1e-6, Dirichlet, PIC/FLIP G2P (0.95 FLIP),
-advection with obstacle rejection.  Workload: 1024x1024 MAC grid, unit cells,
+classify, P2G, gravity, Dirichlet, 2-layer extension, pressure projection to a
+relative residual of 1e-6, Dirichlet, PIC/FLIP G2P (0.95 FLIP), advection with
+obstacle rejection.  The chosen pressure solver is the multigrid-preconditioned
+CG with FP64 solution and residual; the plain form is FP32 Jacobi-PCG.  Workload: 1024x1024 MAC grid, unit cells,
 fluid block 409 x 614 cells at (1,1), 4 jittered particles per fluid cell
 (1,004,504 particles), dt = density = 0.01.  Inputs are resident in HBM before
 the timed region; W warmup steps are the first W steps of the trajectory and the
 K timed steps continue it.
 
 The printed JSON line reports the chosen solver form ("value"), the plain FP32
-3-kernel form timed over the same trajectory segment, the CG kernel's achieved
-bandwidth against the 8 TB/s HBM peak (47 algorithmic bytes per unknown per CG
-iteration, timed with HIP events on the solver's stream), the single-step error
-of the chosen form against the FP64 oracle, and the FP32 CPU oracle on one core.
+3-kernel form timed over the same trajectory segment, the dominant kernel's
+achieved bandwidth against the 8 TB/s HBM peak (its algorithmic bytes per launch
+over its average duration, timed with HIP events on the solver's stream inside the
+timed region; the dominant kernel is picked from a per-kernel profile of a few
+untimed steps before it), the pressure solver's aggregate bandwidth, the
+single-step error of the chosen form against the FP64 oracle, and the FP32 CPU
+oracle on one core.
 """
 from __future__ import annotations
 
@@ -30,27 +34,45 @@ sys.path.insert(0, ROOT)
 
 METRIC = "FLIP steps/s @1024² (chosen vs plain GPU, ≤1e-5 rel err) + CG HBM GB/s"
 HBM_PEAK_GBS = 8000.0
-CG_BYTES_PER_UNKNOWN_ITER = 47.0
 
 
-def kernel_bytes(name, C, P):
-    """Algorithmic HBM bytes of one launch of a kernel (each array read or written
-    once): C cells, P particles."""
+def mg_levels(nx, ny, coarse_cells=4096):
+    """Level sizes of the multigrid hierarchy (halved, rounded up, to <= 4 cells a
+    side) and the first level solved in one workgroup (<= 4096 cells)."""
+    lv = [(nx, ny)]
+    while lv[-1][0] > 4 or lv[-1][1] > 4:
+        lv.append(((lv[-1][0] + 1) // 2, (lv[-1][1] + 1) // 2))
+    lc = next(l for l, (a, b) in enumerate(lv) if a * b <= coarse_cells)
+    return [a * b for a, b in lv], lc
+
+
+def kernel_bytes(name, nx, ny, P):
+    """Algorithmic HBM bytes of one launch of a kernel, each array read or written
+    once (dense cell grids): nx * ny cells, P particles."""
+    C = nx * ny
+    cells, lc = mg_levels(nx, ny)
     table = {
-        # particles 16 B read + label 1 B; accumulators (float4) read + written once
+        # particles 16 B read + label 1 B; float4 accumulators read and written once
         "particle_scatter": 17.0 * P + 32.0 * C,
-        # info 1, p 4, x_old 8, b 8 read; x_new 8, r 4 written (dense level-0 cells)
+        # particles 16 B read + written; 4 grids sampled (16 B per cell); labels 1 B
+        "particle_gather": 32.0 * P + 17.0 * C,
+        # info 1, p 4, x_old 8, b 8 read; x_new 8, r 4 written
         "cg_mg_xr": 33.0 * C,
         # info 1, z 4, p_old 4 read; p_new 4 written
         "cg_mg_dir": 13.0 * C,
-        # info 1, r 4 read, xs 4 written; level-1 info 0.25, r 1 (a quarter of the cells)
-        "cg_mg_down0": 10.25 * C,
-        # info 1, r 4, xs 4 read, xo 4 written; level-1 xo 1
-        "cg_mg_up0": 14.0 * C,
-        # particles 16 B read + written, 4 grids sampled (16 B per cell), labels
-        "particle_gather": 32.0 * P + 17.0 * C,
+        # right-hand side and info of the one-workgroup levels read, solution written
+        "cg_mg_coarse": 8.0 * cells[lc] + float(sum(cells[lc:])),
     }
-    return table.get(name)
+    if name in table:
+        return table[name]
+    for pre, fine_b, coarse_b in (("cg_mg_down", 9.0, 5.0), ("cg_mg_up", 13.0, 5.0)):
+        if name.startswith(pre):
+            tail = name[len(pre):]
+            ls = range(3, lc) if tail.endswith("+") else [int(tail)]
+            # a "3+" timer covers every launch of levels >= 3: average over them
+            ls = list(ls) or [0]
+            return sum(fine_b * cells[l] + coarse_b * cells[l + 1] for l in ls) / len(ls)
+    return None
 
 
 def log(*a):
@@ -58,9 +80,8 @@ def log(*a):
 
 
 def make_domain(fsg, scene, solver, tol, max_iter):
-    d = fsg.Domain.from_scene(scene, cg_tol=tol, cg_max_iter=max_iter, solver=solver,
-                              refine_inner_tol=1e-3, refine_max_outer=20, check_every=64)
-    return d
+    return fsg.Domain.from_scene(scene, cg_tol=tol, cg_max_iter=max_iter, solver=solver,
+                                 refine_inner_tol=1e-3, refine_max_outer=20, check_every=64)
 
 
 def run_steps(fsg, d, n, dt):
@@ -68,19 +89,35 @@ def run_steps(fsg, d, n, dt):
         d.step(fsg.PICFLIP, dt)
 
 
-def timed_segment(fsg, d, warmup, steps, dt, timing_mode=0):
-    run_steps(fsg, d, warmup, dt)
+def kernel_profile(fsg, d, steps, dt):
+    """Per-kernel HIP-event times over `steps` steps (continuing the trajectory):
+    {name: (ms per launch, launches per step)}."""
+    d.synchronize()
+    d.reset_timing()
+    d.set_timing(1)
+    run_steps(fsg, d, steps, dt)
+    d.synchronize()
+    tm = d.timing()
+    d.set_timing(0)
+    return {k: (ms / max(n, 1), n / steps) for k, (ms, n) in tm.items()}
+
+
+def timed_segment(fsg, d, steps, dt, kernel=None):
+    """K steps bracketed by synchronisations; with `kernel`, that kernel (and only
+    it) is timed with HIP events on the solver's stream inside the region."""
     d.synchronize()
     d.reset_stats()
     d.reset_timing()
-    d.set_timing(timing_mode)
+    if kernel:
+        d.set_timing_prefix(kernel)
+        d.set_timing(2)
     d.synchronize()
     t0 = time.perf_counter()
     run_steps(fsg, d, steps, dt)
     d.synchronize()
     t1 = time.perf_counter()
     st = d.stats()
-    tm = d.timing() if timing_mode else {}
+    tm = d.timing() if kernel else {}
     d.set_timing(0)
     return t1 - t0, st, tm
 
@@ -130,19 +167,24 @@ def cpu_baseline(scene, steps):
     return steps / dt, o.s.cg_iterations
 
 
+SOLVERS = {"plain": 0, "persistent": 1, "refined": 2, "mg": 3}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=1024)
-    ap.add_argument("--solver", default="persistent", choices=["plain", "persistent", "refined"])
+    ap.add_argument("--solver", default="mg", choices=list(SOLVERS))
+    ap.add_argument("--tol", type=float, default=1e-6,
+                    help="pressure solve: relative residual (FP64 residual for mg / refined)")
     ap.add_argument("--plain-steps", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--plain-tol", type=float, default=1e-6)
+    ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-fp64-check", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tol", type=float, default=1e-6,
-                    help="CG relative residual (for the refined form: of the FP64 outer loop)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -153,48 +195,67 @@ def main():
     fsg.load()
     scene = scenes.dam_break(args.n)
     dt = scene.dt
-    tol, max_iter = args.tol, 10000
-    solver = {"plain": fsg.SOLVER_PLAIN, "persistent": fsg.SOLVER_PERSISTENT,
-              "refined": fsg.SOLVER_REFINED}[args.solver]
-    log(f"workload dam_break_{args.n}: {len(scene.particles)} particles; solver {args.solver}")
+    P = len(scene.particles)
+    solver = SOLVERS[args.solver]
+    max_iter = 200 if solver == fsg.SOLVER_MG else 20000
+    log(f"workload dam_break_{args.n}: {P} particles; solver {args.solver} tol {args.tol:g}")
 
-    # ---- chosen form: timed region (CG kernel timed with HIP events on its stream)
-    d = make_domain(fsg, scene, solver, tol, max_iter)
-    elapsed, st, tm = timed_segment(fsg, d, args.warmup, args.steps, dt, timing_mode=2)
+    # ---- chosen form: warmup, per-kernel profile (untimed), timed region
+    d = make_domain(fsg, scene, solver, args.tol, max_iter)
+    run_steps(fsg, d, args.warmup, dt)
+    prof = kernel_profile(fsg, d, args.profile_steps, dt) if args.profile_steps > 0 else {}
+    per_step = {k: ms * n for k, (ms, n) in prof.items()}
+    dominant = max(per_step, key=per_step.get) if per_step else None
+    log("kernel profile (ms per step): " +
+        ", ".join(f"{k} {v:.3f}" for k, v in sorted(per_step.items(), key=lambda kv: -kv[1])[:12]))
+    elapsed, st, tm = timed_segment(fsg, d, args.steps, dt, kernel=dominant)
     state = d.save_state() if not args.no_fp64_check else None
     d.close()
     sps = args.steps / elapsed
-    cg_name = [k for k in tm if k.startswith("cg_")]
-    cg_ms = sum(tm[k][0] for k in cg_name)
-    cg_launches = sum(tm[k][1] for k in cg_name)
-    unknown_iters = st.total_unknown_iterations
-    cg_bytes = CG_BYTES_PER_UNKNOWN_ITER * unknown_iters
-    cg_gbs = cg_bytes / (cg_ms * 1e-3) / 1e9 if cg_ms > 0 else 0.0
-    log(f"chosen: {sps:.3f} steps/s ({1e3 * elapsed / args.steps:.2f} ms/step), "
-        f"CG {cg_ms / max(cg_launches, 1):.3f} ms/launch, {cg_gbs:.1f} GB/s, "
-        f"unknown-iterations {unknown_iters}")
+    ms_per_step = 1e3 * elapsed / args.steps
+    log(f"chosen: {sps:.3f} steps/s ({ms_per_step:.3f} ms/step), dominant kernel {dominant}")
 
-    # ---- plain FP32 form over the same trajectory segment
+    # roofline of the dominant kernel, timed live in the timed region
+    roof = None
+    if dominant and dominant in tm:
+        k_ms = tm[dominant][0] / max(tm[dominant][1], 1)
+        k_bytes = kernel_bytes(dominant, args.n, args.n, P)
+        ach = k_bytes / (k_ms * 1e-3) / 1e9 if k_bytes else None
+        roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS if ach else None, "traffic": None,
+                "kernel": dominant, "kernel_us_per_launch": 1e3 * k_ms,
+                "algorithmic_bytes_per_launch": k_bytes}
+    # "CG HBM GB/s": algorithmic bytes of every pressure-solver kernel over their time
+    cg_bytes = cg_ms = 0.0
+    for k, (ms, n) in prof.items():
+        b = kernel_bytes(k, args.n, args.n, P) if k.startswith("cg_") else None
+        if b is not None:
+            cg_bytes += b * n
+            cg_ms += ms * n
+    cg_gbs = cg_bytes / (cg_ms * 1e-3) / 1e9 if cg_ms > 0 else None
+
+    # ---- plain FP32 Jacobi-PCG form over the same trajectory segment
     plain_sps = None
     if args.plain_steps > 0 and solver != fsg.SOLVER_PLAIN:
-        dp = make_domain(fsg, scene, fsg.SOLVER_PLAIN, tol, max_iter)
+        dp = make_domain(fsg, scene, fsg.SOLVER_PLAIN, args.plain_tol, 20000)
+        run_steps(fsg, dp, args.warmup + args.profile_steps, dt)
         pk = min(args.plain_steps, args.steps)
-        e_p, st_p, _ = timed_segment(fsg, dp, args.warmup, pk, dt)
+        e_p, st_p, _ = timed_segment(fsg, dp, pk, dt)
         dp.close()
         plain_sps = pk / e_p
-        log(f"plain: {plain_sps:.3f} steps/s over {pk} steps")
+        log(f"plain: {plain_sps:.3f} steps/s over {pk} steps ({st_p.cg_iterations} iterations "
+            f"in the last solve)")
     elif solver == fsg.SOLVER_PLAIN:
         plain_sps = sps
 
-    # ---- single-step error against the FP64 oracle
-    # (from the state the timed trajectory reached, where the flow is developed; at
-    # step 0 the projected field is hydrostatic, ~0, and relative errors of a
-    # near-zero field say little)
+    # ---- single-step error against the FP64 oracle, from the state the timed run
+    # reached (at step 0 the projected field is hydrostatic, ~0, and relative
+    # errors of a near-zero field say little)
     errs, same_labels = (None, None)
     if not args.no_fp64_check:
         t0 = time.perf_counter()
-        errs, same_labels = fp64_single_step_error(fsg, scene, solver, tol, state)
-        log(f"fp64 check at step {args.warmup + args.steps} "
+        errs, same_labels = fp64_single_step_error(fsg, scene, solver, args.tol, state)
+        log(f"fp64 check at step {args.warmup + args.profile_steps + args.steps} "
             f"({time.perf_counter() - t0:.1f} s): {errs} labels equal {same_labels}")
 
     # ---- CPU baseline: the FP32 oracle, one thread, bounded sample
@@ -205,10 +266,9 @@ def main():
         log(f"cpu oracle: {cpu_sps:.4f} steps/s ({time.perf_counter() - t0:.1f} s, "
             f"{cpu_its} CG iterations in the last step)")
         cpu = {"value": cpu_sps, "unit": "steps/s", "cores": 1, "kind": "port",
-               "sample": f"first {args.cpu_steps} steps of the same 1024^2 dam break, FP32 "
-                         f"oracle (C restatement, gcc -O2), CG rel tol 1e-6, 1 thread"}
+               "sample": f"first {args.cpu_steps} steps of the same {args.n}^2 dam break, FP32 "
+                         f"oracle (C restatement, gcc -O2), Jacobi-PCG rel tol 1e-6, 1 thread"}
 
-    ms_per_step = 1e3 * elapsed / args.steps
     out = {
         "metric": METRIC,
         "value": sps,
@@ -224,25 +284,20 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"dam_break_{args.n}: stepPICFLIP on a {args.n}x{args.n} MAC grid, "
-                        f"{len(scene.particles)} particles, dt 0.01, CG rel tol {tol:g}",
+                        f"{P} particles, dt 0.01, density 0.01, 0.95 FLIP",
             "solver": args.solver,
+            "tol": args.tol,
+            "plain_tol": args.plain_tol,
             "plain_steps_per_s": plain_sps,
             "speedup_vs_plain": (sps / plain_sps) if plain_sps else None,
             "cg_hbm_gbs": cg_gbs,
-            "cg_unknown_iterations": unknown_iters,
+            "cg_iterations_last_solve": st.cg_iterations,
             "rel_err_vs_fp64_single_step": errs,
+            "max_rel_err_vs_fp64": max(errs.values()) if errs else None,
             "labels_equal_fp64": same_labels,
+            "kernel_ms_per_step": {k: round(v, 4) for k, v in per_step.items()},
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": cg_gbs,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": cg_gbs / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": ",".join(cg_name),
-            "kernel_ms_per_launch": cg_ms / max(cg_launches, 1),
-        },
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
