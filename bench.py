@@ -36,9 +36,9 @@ METRIC = "FLIP steps/s @1024² (chosen vs plain GPU, ≤1e-5 rel err) + CG HBM G
 HBM_PEAK_GBS = 8000.0
 
 
-def mg_levels(nx, ny, coarse_cells=4096):
+def mg_levels(nx, ny, coarse_cells=1024):
     """Level sizes of the multigrid hierarchy (halved, rounded up, to <= 4 cells a
-    side) and the first level solved in one workgroup (<= 4096 cells)."""
+    side) and the first level solved in one workgroup (<= 1024 cells)."""
     lv = [(nx, ny)]
     while lv[-1][0] > 4 or lv[-1][1] > 4:
         lv.append(((lv[-1][0] + 1) // 2, (lv[-1][1] + 1) // 2))
@@ -185,7 +185,20 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-fp64-check", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-sized device buffer 5 times (calibrates the "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe * (1 << 20) // 4
+        a = torch.ones(n, device="cuda", dtype=torch.float32)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)

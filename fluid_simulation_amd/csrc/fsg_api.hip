@@ -257,15 +257,31 @@ int mg_prepare(fsg_domain* h) {
     m.ny[m.nlev] = (m.ny[m.nlev - 1] + 1) / 2;
     ++m.nlev;
   }
+  // the coarsest level is swept by one wavefront (a cell per lane)
+  if (m.nx[m.nlev - 1] * m.ny[m.nlev - 1] > 64)
+    return fail(h, FSG_E_INVALID, "multigrid solver: grid too elongated for the hierarchy");
   m.lc = 0;
   while ((size_t)m.nx[m.lc] * m.ny[m.lc] > (size_t)kMgCoarseCells) ++m.lc;
   if (m.lc >= m.nlev)
     return fail(h, FSG_E_INVALID, "multigrid solver: grid too elongated for the hierarchy");
+  // cell offsets of the one-workgroup levels (its LDS layout and its info array)
+  MgCoarseArgs& a = m.coarse;
+  int off = 0;
+  for (int l = 0; l < m.nlev; ++l) {
+    a.off[l] = l >= m.lc ? off : 0;
+    if (l >= m.lc) off += m.nx[l] * m.ny[l];
+  }
+  a.total = (off + 3) & ~3;
   bool ok = true;
+  uint8_t* coarse_inf = nullptr;
+  ok &= dalloc(h, &coarse_inf, (size_t)a.total) == hipSuccess;
   for (int l = 0; l < m.nlev; ++l) {
     const size_t n = (size_t)m.nx[l] * m.ny[l];
     if (l > 0) ok &= dalloc(h, &m.lab[l], n) == hipSuccess;
-    ok &= dalloc(h, &m.inf[l], n) == hipSuccess;
+    if (l < m.lc)
+      ok &= dalloc(h, &m.inf[l], n) == hipSuccess;
+    else if (coarse_inf)
+      m.inf[l] = coarse_inf + a.off[l];
     ok &= dalloc(h, &m.rl[l], n) == hipSuccess;
     ok &= dalloc(h, &m.xs[l], n) == hipSuccess;
     ok &= dalloc(h, &m.xo[l], n) == hipSuccess;
@@ -284,28 +300,21 @@ int mg_prepare(fsg_domain* h) {
   ok &= dalloc(h, &m.part_rz, (size_t)std::max(mg_up_blocks(m.nx[0], m.ny[0]), 1)) == hipSuccess;
   if (!ok) return fail(h, FSG_E_NOMEM, "multigrid solver: device allocation failed");
   m.lab[0] = h->label;
-  MgCoarseArgs& a = m.coarse;
   a.lc = m.lc;
   a.nlev = m.nlev;
   a.omega = 0.8f;
   a.sweeps = 2 * std::max(m.nx[m.nlev - 1], m.ny[m.nlev - 1]) + 8;
-  int off = 0;
   for (int l = 0; l < m.nlev; ++l) {
     a.nx[l] = m.nx[l];
     a.ny[l] = m.ny[l];
-    a.inf[l] = m.inf[l];
-    a.off[l] = off;
-    if (l >= m.lc) off += m.nx[l] * m.ny[l];
   }
-  // offsets count from level lc
-  for (int l = 0; l < m.nlev; ++l) a.off[l] = l >= m.lc ? a.off[l] : 0;
-  a.total = off;
+  a.inf_all = coarse_inf;
   a.r_in = m.rl[m.lc];
   a.x_out = m.xo[m.lc];
   // The one-workgroup levels hold at most ~2 x kMgCoarseCells cells (13 bytes each);
   // the kernel's LDS limit is raised once, process-wide, to a fixed 128 KiB.
   constexpr size_t kCoarseLdsLimit = 128 * 1024;
-  m.coarse_lds = (size_t)off * (3 * sizeof(float) + 1);
+  m.coarse_lds = (size_t)a.total * (3 * sizeof(float) + 1);
   if (m.coarse_lds > kCoarseLdsLimit)
     return fail(h, FSG_E_INVALID, "multigrid solver: coarse levels exceed the LDS budget");
   if (mg_set_lds_limit(kCoarseLdsLimit) != hipSuccess)
@@ -321,7 +330,8 @@ int mg_solve(fsg_domain* h, int F) {
   if (int rc = mg_prepare(h)) return rc;
   MgHost& m = h->mg;
   timed(h, "cg_mg_setup", [&] {
-    launch_mg_setup(h->stream, h->g, m, h->label, UF(h), VF(h), h->prm.project_mean, h->sc);
+    launch_mg_setup(h->stream, h->g, m, h->label, UF(h), VF(h), h->prm.project_mean,
+                    h->prm.warm_start != 0, h->sc);
   });
   const int max_iter = h->prm.cg_max_iter;
   const double tol = h->prm.cg_tol;

@@ -128,13 +128,13 @@ hipError_t pcg_set_lds_limit(size_t max_bytes);
 // multigrid-preconditioned CG with FP64 solution and residual (dense cell grids)
 constexpr int kMgMaxLevels = 16;
 constexpr int kMgMaxVecBlocks = 1024;  // blocks of the vector kernels (partials per kernel)
-constexpr int kMgCoarseCells = 4096;   // levels at or below this size run in one workgroup
+constexpr int kMgCoarseCells = 1024;   // levels at or below this size run in one workgroup
 
 struct MgCoarseArgs {
   int lc, nlev, total, sweeps;
   float omega;
   int nx[kMgMaxLevels], ny[kMgMaxLevels], off[kMgMaxLevels];
-  const uint8_t* inf[kMgMaxLevels];
+  const uint8_t* inf_all;  // info of levels lc .. nlev-1, contiguous (offsets off[l] - off[lc])
   const float* r_in;  // right-hand side of level lc
   float* x_out;       // cycle result of level lc
 };
@@ -160,7 +160,8 @@ int mg_vec_blocks(int C);
 int mg_up_blocks(int nx, int ny);
 hipError_t mg_set_lds_limit(size_t bytes);
 void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_t* label,
-                     const float* uf, const float* vf, int project_mean, CgScalars* sc);
+                     const float* uf, const float* vf, int project_mean, int warm,
+                     CgScalars* sc);
 // one CG iteration: down(l) for l < lc, coarse, up(l) for l = lc-1 .. 0, dir, xr
 void launch_mg_down(hipStream_t s, const MgHost& m, int l, int it, int max_iter, double tol,
                     CgScalars* sc);

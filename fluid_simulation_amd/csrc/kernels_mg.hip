@@ -34,7 +34,7 @@
 //
 // Kernels per CG iteration: mg_down (one per fine level: 2 pre-smoothing sweeps,
 // residual and restriction fused in one LDS tile), mg_coarse (one workgroup: the
-// V-cycle of the levels of <= 4096 cells in LDS), mg_up (one per fine level:
+// V-cycle of the levels of <= 1024 cells in LDS), mg_up (one per fine level:
 // prolongation, correction and 2 post-smoothing sweeps fused), mg_dir (p = z +
 // beta p, K p and p.Kp), mg_xr (x += alpha p in FP64 and r = b' - K x).  Every
 // kernel returns at once when the solve is done, and each sums the block partials
@@ -50,12 +50,23 @@ namespace fsg {
 
 namespace {
 
-constexpr int kT = 256;         // threads of the tile and vector kernels
+constexpr int kT = 256;         // threads of the vector kernels and the level-0 tiles
+constexpr int kTileBig = 1024;  // threads of the tiles of the smaller levels
 constexpr uint8_t kAct = 0x80;    // info bit: the cell is an unknown
 constexpr uint8_t kWall = 0x40;   // info bit: SOLID (also used for cells outside the grid)
 
 __device__ __forceinline__ bool is_act(uint8_t f) { return (f & kAct) != 0; }
 __device__ __forceinline__ bool is_wall(uint8_t f) { return (f & kWall) != 0; }
+// 1 / n for the Jacobi smoother (a fixed diagonal scaling: any consistent rounding
+// keeps the smoother symmetric)
+__device__ __forceinline__ float inv_nns(uint8_t f) {
+  const int n = f & 7;
+  float r = 1.0f;
+  r = n == 2 ? 0.5f : r;
+  r = n == 3 ? 0.333333343f : r;
+  r = n == 4 ? 0.25f : r;
+  return r;
+}
 
 // Bilinear prolongation of one fine cell from its coarse cell (value e0) and the
 // x-, y- and diagonal neighbours (e, info f): a wall neighbour's weight is moved to
@@ -92,6 +103,34 @@ __device__ __forceinline__ uint8_t make_info(const uint8_t* __restrict__ lab, in
   return (uint8_t)((act ? kAct : 0) | (lab[c] == kSolid ? kWall : 0) | n);
 }
 
+// (i, j) of a cell index that advances by a fixed stride: one division at the
+// start, none per step (grid-stride loops over row-major cells).
+struct CellWalk {
+  int i, j, di, dj, nx;
+  __device__ __forceinline__ CellWalk(int c, int stride, int nx_) : nx(nx_) {
+    j = c / nx;
+    i = c - j * nx;
+    dj = stride / nx;
+    di = stride - dj * nx;
+  }
+  __device__ __forceinline__ void next() {
+    i += di;
+    j += dj;
+    if (i >= nx) {
+      i -= nx;
+      ++j;
+    }
+  }
+};
+
+// i = q % nx, j = q / nx for q < 2^12 via a float reciprocal: (q + 0.5) / nx is
+// at least 0.5 / nx away from an integer and the product's rounding error is below
+// 2^-12 / nx, so the truncation is exact.
+__device__ __forceinline__ void small_divmod(int q, int nx, float inv_nx, int& i, int& j) {
+  j = (int)(((float)q + 0.5f) * inv_nx);
+  i = q - j * nx;
+}
+
 // Sum of n block partials, the same fixed order in every block (NT threads;
 // sm: NT / 64 doubles).
 template <int NT = kT>
@@ -125,8 +164,9 @@ __global__ void __launch_bounds__(kT) k_mg_setup0(Geom g, const uint8_t* __restr
   const int nx = g.nx, ny = g.ny, C = nx * ny;
   const double scale = -(double)g.dx * (double)g.dx;
   double s = 0.0, cnt = 0.0;
-  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT) {
-    const int i = c % nx, j = c / nx;
+  CellWalk w(blockIdx.x * kT + threadIdx.x, gridDim.x * kT, nx);
+  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT, w.next()) {
+    const int i = w.i, j = w.j;
     const uint8_t f = make_info(label, nx, ny, i, j);
     inf[c] = f;
     double bv = 0.0;
@@ -145,34 +185,57 @@ __global__ void __launch_bounds__(kT) k_mg_setup0(Geom g, const uint8_t* __restr
   write_part<kT>(cnt, part_cnt, sm);
 }
 
-// Optional mean removal (singular all-Neumann systems), x0 = 0, r0 = b', |b'|^2.
-__global__ void __launch_bounds__(kT) k_mg_setup1(int C, int project_mean,
+// Optional mean removal (singular all-Neumann systems), x0 and r0 = b' - K x0,
+// |b'|^2 and |r0|^2.  Cold start: x0 = 0 (xw == null).  Warm start: x0 = xw, the
+// previous solution already zeroed off this solve's unknowns (k_mg_warm), read with
+// its neighbours from xw and copied to x0.
+__global__ void __launch_bounds__(kT) k_mg_setup1(int nx, int ny, int project_mean,
                                                    const uint8_t* __restrict__ inf,
                                                    double* __restrict__ b,
+                                                   const double* __restrict__ xw,
                                                    double* __restrict__ x0,
                                                    float* __restrict__ r,
                                                    const double* __restrict__ part_sum,
                                                    const double* __restrict__ part_cnt, int n_part,
-                                                   double* __restrict__ part_bb, CgScalars* sc) {
+                                                   double* __restrict__ part_bb,
+                                                   double* __restrict__ part_rr, CgScalars* sc) {
   __shared__ double sm[4];
+  const int C = nx * ny;
   double mean = 0.0;
   if (project_mean) {
     const double s = sum_part<kT>(part_sum, n_part, sm);
     const double n = sum_part<kT>(part_cnt, n_part, sm);
     mean = n > 0.0 ? s / n : 0.0;
   }
-  double acc = 0.0;
-  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT) {
+  double acc_b = 0.0, acc_r = 0.0;
+  CellWalk w(blockIdx.x * kT + threadIdx.x, gridDim.x * kT, nx);
+  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT, w.next()) {
+    const uint8_t f = inf[c];
     double bv = b[c];
-    if (project_mean && is_act(inf[c])) {
+    if (project_mean && is_act(f)) {
       bv -= mean;
       b[c] = bv;
     }
-    x0[c] = 0.0;
-    r[c] = (float)bv;
-    acc += bv * bv;
+    double rv = bv, xv = 0.0;
+    if (xw) {
+      xv = xw[c];
+      if (is_act(f)) {
+        const int i = w.i, j = w.j;
+        double sn = 0.0;
+        if (i > 0) sn += xw[c - 1];
+        if (i < nx - 1) sn += xw[c + 1];
+        if (j > 0) sn += xw[c - nx];
+        if (j < ny - 1) sn += xw[c + nx];
+        rv = bv - ((double)(f & 7) * xv - sn);
+      }
+    }
+    x0[c] = xv;
+    r[c] = (float)rv;
+    acc_b += bv * bv;
+    acc_r += rv * rv;
   }
-  write_part<kT>(acc, part_bb, sm);
+  write_part<kT>(acc_b, part_bb, sm);
+  write_part<kT>(acc_r, part_rr, sm);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     sc->done = 0;
     sc->iter = 0;
@@ -180,6 +243,14 @@ __global__ void __launch_bounds__(kT) k_mg_setup1(int C, int project_mean,
     sc->err = 0.0;
     sc->rho[0] = sc->rho[1] = 0.0;
   }
+}
+
+// Warm start: the previous solution at this solve's unknowns, 0 elsewhere.
+__global__ void __launch_bounds__(kT) k_mg_warm(int C, const uint8_t* __restrict__ inf,
+                                                 const double* __restrict__ prev,
+                                                 double* __restrict__ xw) {
+  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT)
+    xw[c] = is_act(inf[c]) ? prev[c] : 0.0;
 }
 
 // Coarse labels of level l+1 from level l, and (finf != null) the info of level l.
@@ -265,12 +336,12 @@ struct MgLevel {
   float* xo;  // after the full cycle (level 0: z)
 };
 
-template <bool L0>
-__global__ void __launch_bounds__(kT) k_mg_down(MgLevel f, MgLevel c, float omega, int it,
+template <bool L0, int NT>
+__global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omega, int it,
                                                  MgCheck chk) {
-  __shared__ double sm[4];
+  __shared__ double sm[NT / 64];
   if (L0) {
-    if (mg_check<kT>(chk, it, sm)) return;
+    if (mg_check<NT>(chk, it, sm)) return;
   } else if (chk.sc->done) {
     return;
   }
@@ -281,11 +352,11 @@ __global__ void __launch_bounds__(kT) k_mg_down(MgLevel f, MgLevel c, float omeg
   const int tiles_x = (c.nx + 15) >> 4;
   const int bx = blockIdx.x % tiles_x, by = blockIdx.x / tiles_x;
   const int fi0 = bx * 32, fj0 = by * 32;
-  for (int t = threadIdx.x; t < WC * WC; t += kT) {
+  for (int t = threadIdx.x; t < WC * WC; t += NT) {
     const int I = bx * 16 + t % WC - 1, J = by * 16 + t / WC - 1;
     s_cinf[t] = (I >= 0 && I < c.nx && J >= 0 && J < c.ny) ? c.inf[I + J * c.nx] : kWall;
   }
-  for (int t = threadIdx.x; t < W3 * W3; t += kT) {
+  for (int t = threadIdx.x; t < W3 * W3; t += NT) {
     const int i = fi0 + t % W3 - 3, j = fj0 + t / W3 - 3;
     float rv = 0.f;
     uint8_t fv = 0;
@@ -296,22 +367,22 @@ __global__ void __launch_bounds__(kT) k_mg_down(MgLevel f, MgLevel c, float omeg
     }
     s_inf[t] = fv;
     s_r[t] = rv;
-    s_x1[t] = is_act(fv) ? omega * rv / nns_of(fv) : 0.f;
+    s_x1[t] = is_act(fv) ? omega * rv * inv_nns(fv) : 0.f;
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < W2 * W2; t += kT) {
+  for (int t = threadIdx.x; t < W2 * W2; t += NT) {
     const int q = (t % W2 + 1) + (t / W2 + 1) * W3;
     const uint8_t fv = s_inf[q];
     float v = 0.f;
     if (is_act(fv)) {
       const float n = nns_of(fv), x = s_x1[q];
       const float kx = n * x - (s_x1[q - 1] + s_x1[q + 1] + s_x1[q - W3] + s_x1[q + W3]);
-      v = x + omega * (s_r[q] - kx) / n;
+      v = x + omega * (s_r[q] - kx) * inv_nns(fv);
     }
     s_x2[t] = v;
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < W1 * W1; t += kT) {
+  for (int t = threadIdx.x; t < W1 * W1; t += NT) {
     const int li = t % W1, lj = t / W1;  // fine offset li - 1, lj - 1
     const int q = (li + 1) + (lj + 1) * W2;
     const int q3 = (li + 2) + (lj + 2) * W3;
@@ -330,7 +401,7 @@ __global__ void __launch_bounds__(kT) k_mg_down(MgLevel f, MgLevel c, float omeg
     }
   }
   __syncthreads();
-  {
+  if (threadIdx.x < 256) {
     const int ci = threadIdx.x & 15, cj = threadIdx.x >> 4;
     const int I = bx * 16 + ci, J = by * 16 + cj;
     if (I < c.nx && J < c.ny) {
@@ -365,25 +436,25 @@ __global__ void __launch_bounds__(kT) k_mg_down(MgLevel f, MgLevel c, float omeg
 // solution around the tile, r and info on [-2, 34), the corrected x = xs + P e
 // there, the first post-sweep on [-1, 33), the second on the tile, written to xo.
 // Level 0 also forms the partial of r.z (z = xo) for the next direction.
-template <bool L0>
-__global__ void __launch_bounds__(kT) k_mg_up(MgLevel f, MgLevel c, float omega,
+template <bool L0, int NT>
+__global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega,
                                                const CgScalars* sc, double* __restrict__ part_rz) {
   if (sc->done) return;
   constexpr int W2 = 36, W1 = 34, WC = 20;
   __shared__ float s_r[W2 * W2], s_xc[W2 * W2], s_y1[W1 * W1], s_e[WC * WC];
   __shared__ uint8_t s_inf[W2 * W2], s_cinf[WC * WC];
-  __shared__ double sm[4];
+  __shared__ double sm[NT / 64];
   const int tiles_x = (f.nx + 31) >> 5;
   const int bx = blockIdx.x % tiles_x, by = blockIdx.x / tiles_x;
   const int fi0 = bx * 32, fj0 = by * 32;
   const int ci0 = (fi0 >> 1) - 2, cj0 = (fj0 >> 1) - 2;  // coarse frame origin
-  for (int t = threadIdx.x; t < WC * WC; t += kT) {
+  for (int t = threadIdx.x; t < WC * WC; t += NT) {
     const int I = ci0 + t % WC, J = cj0 + t / WC;
     const bool in = I >= 0 && I < c.nx && J >= 0 && J < c.ny;
     s_e[t] = in ? c.xo[I + J * c.nx] : 0.f;
     s_cinf[t] = in ? c.inf[I + J * c.nx] : kWall;
   }
-  for (int t = threadIdx.x; t < W2 * W2; t += kT) {
+  for (int t = threadIdx.x; t < W2 * W2; t += NT) {
     const int i = fi0 + t % W2 - 2, j = fj0 + t / W2 - 2;
     float rv = 0.f, xv = 0.f;
     uint8_t fv = 0;
@@ -398,7 +469,7 @@ __global__ void __launch_bounds__(kT) k_mg_up(MgLevel f, MgLevel c, float omega,
     s_xc[t] = xv;
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < W2 * W2; t += kT) {
+  for (int t = threadIdx.x; t < W2 * W2; t += NT) {
     if (!is_act(s_inf[t])) {
       s_xc[t] = 0.f;
       continue;
@@ -413,20 +484,20 @@ __global__ void __launch_bounds__(kT) k_mg_up(MgLevel f, MgLevel c, float omega,
                              s_cinf[q11]);
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < W1 * W1; t += kT) {
+  for (int t = threadIdx.x; t < W1 * W1; t += NT) {
     const int q = (t % W1 + 1) + (t / W1 + 1) * W2;
     const uint8_t fv = s_inf[q];
     float v = 0.f;
     if (is_act(fv)) {
       const float n = nns_of(fv), x = s_xc[q];
       const float kx = n * x - (s_xc[q - 1] + s_xc[q + 1] + s_xc[q - W2] + s_xc[q + W2]);
-      v = x + omega * (s_r[q] - kx) / n;
+      v = x + omega * (s_r[q] - kx) * inv_nns(fv);
     }
     s_y1[t] = v;
   }
   __syncthreads();
   double rz = 0.0;
-  for (int t = threadIdx.x; t < 32 * 32; t += kT) {
+  for (int t = threadIdx.x; t < 32 * 32; t += NT) {
     const int oi = t & 31, oj = t >> 5;
     const int i = fi0 + oi, j = fj0 + oj;
     if (i >= f.nx || j >= f.ny) continue;
@@ -437,16 +508,19 @@ __global__ void __launch_bounds__(kT) k_mg_up(MgLevel f, MgLevel c, float omega,
     if (is_act(fv)) {
       const float n = nns_of(fv), x = s_y1[q];
       const float kx = n * x - (s_y1[q - 1] + s_y1[q + 1] + s_y1[q - W1] + s_y1[q + W1]);
-      v = x + omega * (s_r[q2] - kx) / n;
+      v = x + omega * (s_r[q2] - kx) * inv_nns(fv);
     }
     f.xo[i + j * f.nx] = v;
     if (L0) rz += (double)s_r[q2] * (double)v;
   }
-  if (L0) write_part<kT>(rz, part_rz, sm);
+  if (L0) write_part<NT>(rz, part_rz, sm);
 }
 
-// The levels lc .. nlev-1 (each <= 4096 cells) in one workgroup, all in LDS:
-// the same V-cycle, with `sweeps` damped-Jacobi sweeps from zero on the coarsest.
+// The levels lc .. nlev-1 (each <= 1024 cells) in one workgroup, all in LDS: the
+// same V-cycle as the tile kernels.  The info bytes of all these levels are one
+// contiguous array, loaded in one pass.  The coarsest level (<= 4 x 4 cells) is
+// swept by one wavefront, a cell per lane, neighbours by lane shuffles, with no
+// barrier between sweeps.
 __device__ __forceinline__ float nbr_sum(const float* a, int c, int i, int j, int nx, int ny) {
   float s = 0.f;
   if (i > 0) s += a[c - 1];
@@ -470,48 +544,58 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, Mg
     return;
   }
   extern __shared__ float lds[];
-  const int tot = a.total;
+  const int tot = a.total;  // a multiple of 4
   float* R = lds;
   float* X = R + tot;
   float* T = X + tot;
   uint8_t* INF = reinterpret_cast<uint8_t*>(T + tot);
-  for (int l = a.lc; l < a.nlev; ++l) {
-    const int n = a.nx[l] * a.ny[l], o = a.off[l];
-    for (int q = threadIdx.x; q < n; q += kCoarseThreads) {
-      INF[o + q] = a.inf[l][q];
-      R[o + q] = l == a.lc ? a.r_in[q] : 0.f;
-      X[o + q] = 0.f;
-      T[o + q] = 0.f;
-    }
+  {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(a.inf_all);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(INF);
+    for (int q = threadIdx.x; q < tot / 4; q += kCoarseThreads) dst[q] = src[q];
+    const int n0 = a.nx[a.lc] * a.ny[a.lc];
+    for (int q = threadIdx.x; q < n0; q += kCoarseThreads) R[q] = a.r_in[q];
   }
   __syncthreads();
   const float w = a.omega;
   for (int l = a.lc; l < a.nlev - 1; ++l) {
-    const int nx = a.nx[l], ny = a.ny[l], n = nx * ny, o = a.off[l];
+    const int nx = a.nx[l], ny = a.ny[l], n = nx * ny, o = a.off[l] - a.off[a.lc];
     float *r = R + o, *x = X + o, *t = T + o;
     const uint8_t* inf = INF + o;
-    for (int q = threadIdx.x; q < n; q += kCoarseThreads)
-      t[q] = is_act(inf[q]) ? w * r[q] / nns_of(inf[q]) : 0.f;
-    __syncthreads();
+    const float inv_nx = 1.0f / (float)nx;
+    // two sweeps from zero; the first sweep's neighbour values w r / n formed on the fly
     for (int q = threadIdx.x; q < n; q += kCoarseThreads) {
       float v = 0.f;
       if (is_act(inf[q])) {
-        const float nn = nns_of(inf[q]);
-        v = t[q] + w * (r[q] - (nn * t[q] - nbr_sum(t, q, q % nx, q / nx, nx, ny))) / nn;
+        int i, j;
+        small_divmod(q, nx, inv_nx, i, j);
+        auto x1 = [&](int k) { return is_act(inf[k]) ? w * r[k] * inv_nns(inf[k]) : 0.f; };
+        float sn = 0.f;
+        if (i > 0) sn += x1(q - 1);
+        if (i < nx - 1) sn += x1(q + 1);
+        if (j > 0) sn += x1(q - nx);
+        if (j < ny - 1) sn += x1(q + nx);
+        const float nn = nns_of(inf[q]), xc = w * r[q] * inv_nns(inf[q]);
+        v = xc + w * (r[q] - (nn * xc - sn)) * inv_nns(inf[q]);
       }
       x[q] = v;
     }
     __syncthreads();
     for (int q = threadIdx.x; q < n; q += kCoarseThreads) {
       float v = 0.f;
-      if (is_act(inf[q]))
-        v = r[q] - (nns_of(inf[q]) * x[q] - nbr_sum(x, q, q % nx, q / nx, nx, ny));
+      if (is_act(inf[q])) {
+        int i, j;
+        small_divmod(q, nx, inv_nx, i, j);
+        v = r[q] - (nns_of(inf[q]) * x[q] - nbr_sum(x, q, i, j, nx, ny));
+      }
       t[q] = v;
     }
     __syncthreads();
-    const int cnx = a.nx[l + 1], cny = a.ny[l + 1], co = a.off[l + 1];
+    const int cnx = a.nx[l + 1], cny = a.ny[l + 1], co = a.off[l + 1] - a.off[a.lc];
+    const float inv_cnx = 1.0f / (float)cnx;
     for (int q = threadIdx.x; q < cnx * cny; q += kCoarseThreads) {
-      const int I = q % cnx, J = q / cnx;
+      int I, J;
+      small_divmod(q, cnx, inv_cnx, I, J);
       float v = 0.f;
       if (is_act(INF[co + q])) {
         const float wt[4] = {0.25f, 0.75f, 0.75f, 0.25f};
@@ -519,16 +603,16 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, Mg
         for (int b = 0; b < 4; ++b) {
           const int j = 2 * J - 1 + b;
           if (j < 0 || j >= ny) continue;
-          float s = 0.f;
+          float srow = 0.f;
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             const int i = 2 * I - 1 + e;
-            if (i >= 0 && i < nx) s += wt[e] * t[i + j * nx];
+            if (i >= 0 && i < nx) srow += wt[e] * t[i + j * nx];
           }
-          v += wt[b] * s;
+          v += wt[b] * srow;
         }
-        auto cinf = [&](int a, int b) -> uint8_t {
-          return (a >= 0 && a < cnx && b >= 0 && b < cny) ? INF[co + a + b * cnx] : kWall;
+        auto cinf = [&](int ii, int jj) -> uint8_t {
+          return (ii >= 0 && ii < cnx && jj >= 0 && jj < cny) ? INF[co + ii + jj * cnx] : kWall;
         };
 #pragma unroll
         for (int b = 0; b < 2; ++b)
@@ -546,43 +630,52 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, Mg
     __syncthreads();
   }
   {
+    // coarsest level: <= 16 cells (checked on the host), one wavefront
     const int l = a.nlev - 1;
-    const int nx = a.nx[l], ny = a.ny[l], n = nx * ny, o = a.off[l];
-    float *r = R + o, *x = X + o, *t = T + o;
-    const uint8_t* inf = INF + o;
-    for (int s = 0; s < a.sweeps; ++s) {
-      float* src = (s & 1) ? t : x;
-      float* dst = (s & 1) ? x : t;
-      for (int q = threadIdx.x; q < n; q += kCoarseThreads) {
-        float v = 0.f;
-        if (is_act(inf[q])) {
-          const float nn = nns_of(inf[q]);
-          v = src[q] + w * (r[q] - (nn * src[q] - nbr_sum(src, q, q % nx, q / nx, nx, ny))) / nn;
-        }
-        dst[q] = v;
+    const int nx = a.nx[l], ny = a.ny[l], n = nx * ny, o = a.off[l] - a.off[a.lc];
+    if (threadIdx.x < 64) {
+      const int q = threadIdx.x;
+      const bool in = q < n;
+      const uint8_t f = in ? INF[o + q] : 0;
+      const float r = in ? R[o + q] : 0.f;
+      const int i = in ? q % nx : 0, j = in ? q / nx : 0;
+      const float nn = nns_of(f), rn = inv_nns(f);
+      float x = 0.f;
+      for (int sw = 0; sw < a.sweeps; ++sw) {
+        const float xl = __shfl(x, q - 1 < 0 ? 0 : q - 1, 64);
+        const float xr = __shfl(x, q + 1 > 63 ? 63 : q + 1, 64);
+        const float xd = __shfl(x, q - nx < 0 ? 0 : q - nx, 64);
+        const float xu = __shfl(x, q + nx > 63 ? 63 : q + nx, 64);
+        float s = 0.f;
+        if (i > 0) s += xl;
+        if (i < nx - 1) s += xr;
+        if (j > 0) s += xd;
+        if (j < ny - 1) s += xu;
+        x = is_act(f) ? x + w * (r - (nn * x - s)) * rn : 0.f;
       }
-      __syncthreads();
+      if (in) X[o + q] = x;
     }
-    if (a.sweeps & 1) {
-      for (int q = threadIdx.x; q < n; q += kCoarseThreads) x[q] = t[q];
-      __syncthreads();
-    }
+    __syncthreads();
   }
+  // up: t = xs + P e, x = sweep(t), t = sweep(x); a level's result is in T (the
+  // coarsest level's in X)
   for (int l = a.nlev - 2; l >= a.lc; --l) {
-    const int nx = a.nx[l], ny = a.ny[l], n = nx * ny, o = a.off[l];
-    const int cnx = a.nx[l + 1], cny = a.ny[l + 1], co = a.off[l + 1];
+    const int nx = a.nx[l], ny = a.ny[l], n = nx * ny, o = a.off[l] - a.off[a.lc];
+    const int cnx = a.nx[l + 1], cny = a.ny[l + 1], co = a.off[l + 1] - a.off[a.lc];
     float *r = R + o, *x = X + o, *t = T + o;
     const uint8_t* inf = INF + o;
-    const float* e = X + co;
+    const float* e = (l + 1 == a.nlev - 1 ? X : T) + co;
+    const uint8_t* cinf = INF + co;
+    const float inv_nx = 1.0f / (float)nx;
     for (int q = threadIdx.x; q < n; q += kCoarseThreads) {
       float v = 0.f;
       if (is_act(inf[q])) {
-        const int i = q % nx, j = q / nx;
+        int i, j;
+        small_divmod(q, nx, inv_nx, i, j);
         const int I = i >> 1, J = j >> 1;
         const int In = (i & 1) ? I + 1 : I - 1;
         const int Jn = (j & 1) ? J + 1 : J - 1;
         const bool iok = In >= 0 && In < cnx, jok = Jn >= 0 && Jn < cny;
-        const uint8_t* cinf = INF + co;
         const float e00 = e[I + J * cnx];
         const float e10 = iok ? e[In + J * cnx] : 0.f;
         const float e01 = jok ? e[I + Jn * cnx] : 0.f;
@@ -599,7 +692,9 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, Mg
       float v = 0.f;
       if (is_act(inf[q])) {
         const float nn = nns_of(inf[q]);
-        v = t[q] + w * (r[q] - (nn * t[q] - nbr_sum(t, q, q % nx, q / nx, nx, ny))) / nn;
+        int i, j;
+        small_divmod(q, nx, inv_nx, i, j);
+        v = t[q] + w * (r[q] - (nn * t[q] - nbr_sum(t, q, i, j, nx, ny))) * inv_nns(inf[q]);
       }
       x[q] = v;
     }
@@ -608,20 +703,21 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, Mg
       float v = 0.f;
       if (is_act(inf[q])) {
         const float nn = nns_of(inf[q]);
-        v = x[q] + w * (r[q] - (nn * x[q] - nbr_sum(x, q, q % nx, q / nx, nx, ny))) / nn;
+        int i, j;
+        small_divmod(q, nx, inv_nx, i, j);
+        v = x[q] + w * (r[q] - (nn * x[q] - nbr_sum(x, q, i, j, nx, ny))) * inv_nns(inf[q]);
       }
       t[q] = v;
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < n; q += kCoarseThreads) x[q] = t[q];
-    __syncthreads();
   }
   {
-    const int n = a.nx[a.lc] * a.ny[a.lc], o = a.off[a.lc];
+    const int n = a.nx[a.lc] * a.ny[a.lc];
+    const float* res = a.lc == a.nlev - 1 ? X : T;
     double rz = 0.0;
     for (int q = threadIdx.x; q < n; q += kCoarseThreads) {
-      a.x_out[q] = X[o + q];
-      rz += (double)R[o + q] * (double)X[o + q];
+      a.x_out[q] = res[q];
+      rz += (double)R[q] * (double)res[q];
     }
     if (part_rz) {
       double v[1] = {rz};
@@ -649,11 +745,12 @@ __global__ void __launch_bounds__(kT) k_mg_dir(int nx, int ny, const uint8_t* __
   const bool first = it == 0;
   const int C = nx * ny;
   double acc = 0.0;
-  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT) {
+  CellWalk w(blockIdx.x * kT + threadIdx.x, gridDim.x * kT, nx);
+  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT, w.next()) {
     const uint8_t f = inf[c];
     float pc = 0.f;
     if (is_act(f)) {
-      const int i = c % nx, j = c / nx;
+      const int i = w.i, j = w.j;
       auto pn = [&](int q) { return first ? z[q] : z[q] + bf * p_old[q]; };
       pc = pn(c);
       float s = 0.f;
@@ -684,12 +781,13 @@ __global__ void __launch_bounds__(kT) k_mg_xr(int nx, int ny, const uint8_t* __r
   const double alpha = sc->rho[it & 1] / pq;
   const int C = nx * ny;
   double acc = 0.0;
-  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT) {
+  CellWalk w(blockIdx.x * kT + threadIdx.x, gridDim.x * kT, nx);
+  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT, w.next()) {
     const uint8_t f = inf[c];
     double xn = 0.0;
     float rv = 0.f;
     if (is_act(f)) {
-      const int i = c % nx, j = c / nx;
+      const int i = w.i, j = w.j;
       auto xv = [&](int q) { return x_old[q] + alpha * (double)p[q]; };
       xn = xv(c);
       double s = 0.0;
@@ -772,11 +870,16 @@ hipError_t mg_set_lds_limit(size_t bytes) {
 }
 
 void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_t* label,
-                     const float* uf, const float* vf, int project_mean, CgScalars* sc) {
+                     const float* uf, const float* vf, int project_mean, int warm,
+                     CgScalars* sc) {
   const int C = g.nx * g.ny, nb = vec_blocks(C);
   k_mg_setup0<<<nb, kT, 0, s>>>(g, label, uf, vf, m.inf[0], m.b, m.part_sum, m.part_cnt);
-  k_mg_setup1<<<nb, kT, 0, s>>>(C, project_mean, m.inf[0], m.b, m.x[0], m.rl[0], m.part_sum,
-                                m.part_cnt, nb, m.part_bb, sc);
+  // x[0] holds the previous solve's solution (k_mg_finish); x[1] is free until the
+  // first update of this solve
+  if (warm) k_mg_warm<<<nb, kT, 0, s>>>(C, m.inf[0], m.x[0], m.x[1]);
+  k_mg_setup1<<<nb, kT, 0, s>>>(g.nx, g.ny, project_mean, m.inf[0], m.b, warm ? m.x[1] : nullptr,
+                                m.x[0], m.rl[0], m.part_sum, m.part_cnt, nb, m.part_bb,
+                                m.part_rr, sc);
   for (int l = 1; l < m.nlev; ++l) {
     const int nf = m.nx[l - 1] * m.ny[l - 1], ncl = m.nx[l] * m.ny[l];
     const int n = nf > ncl ? nf : ncl;
@@ -796,8 +899,8 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
 
 MgCheck mg_check_args(const MgHost& m, int it, int max_iter, double tol, CgScalars* sc) {
   const int nb = vec_blocks(m.nx[0] * m.ny[0]);
-  // |r_0|^2 = |b'|^2 is in part_bb; later residuals are k_mg_xr's partials
-  return MgCheck{sc, it == 0 ? m.part_bb : m.part_rr, nb, m.part_bb, nb, tol * tol, max_iter};
+  // |r|^2: k_mg_setup1's partials (it == 0) or k_mg_xr's; |b'|^2 from k_mg_setup1
+  return MgCheck{sc, m.part_rr, nb, m.part_bb, nb, tol * tol, max_iter};
 }
 
 static MgLevel mg_level(const MgHost& m, int l) {
@@ -811,11 +914,14 @@ void launch_mg_down(hipStream_t s, const MgHost& m, int l, int it, int max_iter,
                     CgScalars* sc) {
   const int blocks = ((m.nx[l + 1] + 15) / 16) * ((m.ny[l + 1] + 15) / 16);
   const MgCheck chk = mg_check_args(m, it, max_iter, tol, sc);
+  // level 0: 256-thread blocks (many tiles); smaller levels: 1024 threads, so that
+  // each LDS phase of the few tiles takes one or two passes
   if (l == 0)
-    k_mg_down<true><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega, it, chk);
+    k_mg_down<true, kT><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega, it,
+                                              chk);
   else
-    k_mg_down<false><<<blocks, kT, 0, s>>>(mg_level(m, l), mg_level(m, l + 1), m.coarse.omega, it,
-                                           chk);
+    k_mg_down<false, kTileBig><<<blocks, kTileBig, 0, s>>>(mg_level(m, l), mg_level(m, l + 1),
+                                                           m.coarse.omega, it, chk);
 }
 
 void launch_mg_coarse(hipStream_t s, const MgHost& m, int it, int max_iter, double tol,
@@ -828,11 +934,11 @@ void launch_mg_coarse(hipStream_t s, const MgHost& m, int it, int max_iter, doub
 void launch_mg_up(hipStream_t s, const MgHost& m, int l, CgScalars* sc) {
   const int blocks = mg_up_blocks(m.nx[l], m.ny[l]);
   if (l == 0)
-    k_mg_up<true><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega, sc,
-                                        m.part_rz);
+    k_mg_up<true, kT><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega, sc,
+                                            m.part_rz);
   else
-    k_mg_up<false><<<blocks, kT, 0, s>>>(mg_level(m, l), mg_level(m, l + 1), m.coarse.omega, sc,
-                                         nullptr);
+    k_mg_up<false, kTileBig><<<blocks, kTileBig, 0, s>>>(mg_level(m, l), mg_level(m, l + 1),
+                                                         m.coarse.omega, sc, nullptr);
 }
 
 void launch_mg_dir(hipStream_t s, const MgHost& m, int it, CgScalars* sc) {

@@ -186,6 +186,7 @@ def test_pressure_against_fp64(state, solver, tol, bar_p):
 @pytest.mark.parametrize("solver,tol,bar_p", [
     (fsg.SOLVER_PERSISTENT, 1e-6, 2e-5),
     (fsg.SOLVER_REFINED, 1e-10, 1e-5),
+    (fsg.SOLVER_MG, 1e-10, 1e-5),
 ])
 def test_warm_start_resolve(state, solver, tol, bar_p):
     """A second solve of the same system warm-started from the first solution
