@@ -75,6 +75,22 @@ def kernel_bytes(name, nx, ny, P):
     return None
 
 
+# HBM traffic per launch on the 1024^2 dam break (bytes), from two rocprofv3 --pmc
+# passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) of
+#   bench.py --steps 5 --warmup 2 --plain-steps 0 --profile-steps 1 --no-fp64-check --no-cpu
+# scaled by a 1 GiB device copy (bench.py --copy-probe 1024) read by the same counters:
+# FETCH_SIZE reads 0.5 GiB for its 1 GiB read (factor 2.0), WRITE_SIZE 1 GiB (1.0).
+# traffic = (2.0 * FETCH_SIZE + 1.0 * WRITE_SIZE) * 1024.
+MEASURED_TRAFFIC_1024 = {
+    "cg_mg_coarse": (2.0 * 13.6 + 3.6) * 1024,
+    "cg_mg_down0": (2.0 * 2586.1 + 4602.3) * 1024,
+    "cg_mg_up0": (2.0 * 4890.8 + 3710.6) * 1024,
+    "cg_mg_xr": (2.0 * 4242.4 + 11074.2) * 1024,
+    "cg_mg_dir": (2.0 * 2350.7 + 3710.6) * 1024,
+    "particle_scatter": (2.0 * 8038.9 + 39819.9) * 1024,
+}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -235,7 +251,8 @@ def main():
         k_bytes = kernel_bytes(dominant, args.n, args.n, P)
         ach = k_bytes / (k_ms * 1e-3) / 1e9 if k_bytes else None
         roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBS if ach else None, "traffic": None,
+                "frac": ach / HBM_PEAK_GBS if ach else None,
+                "traffic": MEASURED_TRAFFIC_1024.get(dominant) if args.n == 1024 else None,
                 "kernel": dominant, "kernel_us_per_launch": 1e3 * k_ms,
                 "algorithmic_bytes_per_launch": k_bytes}
     # "CG HBM GB/s": algorithmic bytes of every pressure-solver kernel over their time

@@ -350,7 +350,9 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
   __shared__ float s_r[W3 * W3], s_x1[W3 * W3], s_x2[W2 * W2], s_res[W1 * W1];
   __shared__ uint8_t s_inf[W3 * W3], s_cinf[WC * WC];
   const int tiles_x = (c.nx + 15) >> 4;
-  const int bx = blockIdx.x % tiles_x, by = blockIdx.x / tiles_x;
+  // each XCD takes one band of consecutive tiles, so halo rows are shared in its L2
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = tile % tiles_x, by = tile / tiles_x;
   const int fi0 = bx * 32, fj0 = by * 32;
   for (int t = threadIdx.x; t < WC * WC; t += NT) {
     const int I = bx * 16 + t % WC - 1, J = by * 16 + t / WC - 1;
@@ -445,7 +447,9 @@ __global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega,
   __shared__ uint8_t s_inf[W2 * W2], s_cinf[WC * WC];
   __shared__ double sm[NT / 64];
   const int tiles_x = (f.nx + 31) >> 5;
-  const int bx = blockIdx.x % tiles_x, by = blockIdx.x / tiles_x;
+  // each XCD takes one band of consecutive tiles, so halo rows are shared in its L2
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = tile % tiles_x, by = tile / tiles_x;
   const int fi0 = bx * 32, fj0 = by * 32;
   const int ci0 = (fi0 >> 1) - 2, cj0 = (fj0 >> 1) - 2;  // coarse frame origin
   for (int t = threadIdx.x; t < WC * WC; t += NT) {
