@@ -4,8 +4,11 @@
 // and `steps` solver steps of dt = 0.01, each preceded by FluidDomain::update.
 //
 // usage: fsg_dam_break n steps scheme out.bin [pic_ratio] [solver] [cg_tol] [cg_max_iter]
+//                      [frame.ppm]
 //   scheme: 0 semi-Lagrangian, 1 PIC, 2 FLIP, 3 PIC/FLIP; solver, cg_tol and
-//   cg_max_iter set the fsg_params fields of those names (defaults: the reference's)
+//   cg_max_iter set the fsg_params fields of those names (defaults: the reference's);
+//   frame.ppm: the final frame as the example renders it (a 512 x 512 canvas, cells
+//   then particles, examples/simple.cpp:75-80)
 //
 // out.bin (little endian): int32 n, steps, scheme, n_initial, n_final; float32
 // initial particles (n_initial x 4: x, y, vx, vy); float32 final particles; float32
@@ -17,6 +20,7 @@
 
 #include "FluidDomain.h"
 #include "FluidSolver.h"
+#include "Renderer.h"
 
 namespace {
 void dumpParticles(const MarkerParticleSet& set, std::vector<float>& out) {
@@ -32,7 +36,10 @@ void dumpParticles(const MarkerParticleSet& set, std::vector<float>& out) {
 
 int main(int argc, char** argv) {
   if (argc < 5) {
-    std::fprintf(stderr, "usage: %s n steps scheme out.bin [pic_ratio] [solver] [cg_tol] [cg_max_iter]\n", argv[0]);
+    std::fprintf(stderr,
+                 "usage: %s n steps scheme out.bin [pic_ratio] [solver] [cg_tol] [cg_max_iter] "
+                 "[frame.ppm]\n",
+                 argv[0]);
     return 2;
   }
   const int n = std::atoi(argv[1]);
@@ -91,6 +98,14 @@ int main(int argc, char** argv) {
     std::fwrite(v.data(), sizeof(float), v.size(), f);
     std::fwrite(lab.data(), 1, lab.size(), f);
     std::fclose(f);
+    if (argc > 9) {
+      Renderer renderer({0, 1, 0, 1});
+      Canvas canvas(512, 512);
+      renderer.clearCanvas(canvas);
+      renderer.renderGridCellsToCanvas(fluid_domain.macGrid(), canvas);
+      renderer.renderParticlesToCanvas(fluid_domain.markerParticleSet(), canvas);
+      renderer.writeCanvasToPpm(argv[9], canvas);
+    }
     std::printf("dam_break n=%d steps=%d scheme=%d particles=%d\n", n, steps, scheme, hdr[4]);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "%s\n", e.what());

@@ -995,6 +995,20 @@ int fsg_run_phase(fsg_domain* h, int32_t phase, float dt) {
       face_update(h, FU_EXTINIT, dt, rec);
       extend_sweeps(h, rec);
       break;
+    case FSG_PH_EXTEND_AVERAGE:
+      timed(h, "extend_average_init", [&] {
+        launch_extend_average(h->stream, h->g, h->label, UF(h), VF(h), UB(h), VB(h), MXF(h),
+                              MXB(h), 1);
+      });
+      for (int it = 0; it < h->prm.ext_layers; ++it) {
+        timed(h, "extend_average_sweep", [&] {
+          launch_extend_average(h->stream, h->g, h->label, UF(h), VF(h), UB(h), VB(h), MXF(h),
+                                MXB(h), 0);
+        });
+        h->mask_front ^= 1;  // FluidSolverMemoryPool::swapValidMaskBuffer
+      }
+      swap_vel(h);
+      break;
     case FSG_PH_PRESSURE: {
       int rc = ph_pressure(h, dt, false, false);
       if (rc) return rc;

@@ -44,6 +44,10 @@ void launch_advect_grid(hipStream_t s, const Geom& g, float4* pt, int64_t np, co
 void launch_fill(hipStream_t s, float* a, int64_t n, float val);
 void launch_count_particles(hipStream_t s, const Geom& g, const float4* pt, int64_t np,
                             int* counts);
+// init != 0: masks + back := front; else one sweep (front mask mf, back mask mb)
+void launch_extend_average(hipStream_t s, const Geom& g, const uint8_t* label, const float* uf,
+                           const float* vf, float* ub, float* vb, uint8_t* mf, uint8_t* mb,
+                           int init);
 void launch_emit_lattice(hipStream_t s, float4* pt, int64_t base, const float* xs, int nxp,
                          const float* ys, int nyp, float vx, float vy);
 void launch_advect_level_set(hipStream_t s, const Geom& g, const float* phi, float* out,

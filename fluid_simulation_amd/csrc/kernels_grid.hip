@@ -371,6 +371,84 @@ __global__ void __launch_bounds__(256) k_pressure_update(
   }
 }
 
+// ------------------------------------------------ averaging extension
+// FluidSolver::extendVelocityAvarageing (src/FluidSolver.cpp:625-707), which the
+// reference's steps do not call.  Init: one mask per cell (the x pair), 1 at LIQUID
+// cells in both; back := front.
+__global__ void __launch_bounds__(256) k_extavg_init(Geom g, const uint8_t* __restrict__ label,
+                                                     const float* __restrict__ uf,
+                                                     const float* __restrict__ vf,
+                                                     float* __restrict__ ub, float* __restrict__ vb,
+                                                     uint8_t* __restrict__ mf,
+                                                     uint8_t* __restrict__ mb) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= g.nx * g.ny) return;
+  const uint8_t m = label[c] == kLiquid ? 1 : 0;
+  mf[c] = m;
+  mb[c] = m;
+  ub[c] = uf[c];
+  vb[c] = vf[c];
+}
+
+// One sweep.  The reference visits cells in raster order and writes into the back
+// buffer it reads, so a cell sees the writes of cells before it.  Cell (i, j) reads
+// faces that only (i-2, j), (i-1, j-1), (i+1, j-1) and (i, j-2) write earlier, and
+// no two cells on a line i + 2j = t touch each other's faces, so sweeping the lines
+// t = 0, 1, ... in order, the cells of a line in parallel, reproduces the raster
+// result exactly.  One workgroup; the barrier between lines orders the global
+// memory writes within it.
+__global__ void __launch_bounds__(1024) k_extavg_sweep(Geom g, const uint8_t* __restrict__ label,
+                                                       float* __restrict__ ub,
+                                                       float* __restrict__ vb,
+                                                       const uint8_t* __restrict__ mf,
+                                                       uint8_t* __restrict__ mb) {
+  const int nx = g.nx, ny = g.ny;
+  const int tmax = (nx - 1) + 2 * (ny - 1);
+  for (int t = 0; t <= tmax; ++t) {
+    for (int j = threadIdx.x; j < ny; j += 1024) {
+      const int i = t - 2 * j;
+      if (i < 0 || i >= nx) continue;
+      const int c = i + j * nx;
+      if (mf[c] != 0 || label[c] == kSolid) continue;
+      float su = 0.f, sv = 0.f;
+      int n = 0;
+      const int ni[4] = {i - 1, i, i, i + 1}, nj[4] = {j, j - 1, j + 1, j};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int a = ni[q], b = nj[q];
+        if (a < 0 || a >= nx || b < 0 || b >= ny) continue;
+        const int k = a + b * nx;
+        if (mf[k] != 1) continue;
+        const float u1 = a + 1 < nx ? ub[k + 1] : ub[k];
+        const float v1 = b + 1 < ny ? vb[k + nx] : vb[k];
+        su += (ub[k] + u1) / 2.0f;
+        sv += (vb[k] + v1) / 2.0f;
+        ++n;
+      }
+      if (n > 0) {
+        su /= (float)n;
+        sv /= (float)n;
+        ub[c] = su;
+        if (i + 1 < nx) ub[c + 1] = su;
+        vb[c] = sv;
+        if (j + 1 < ny) vb[c + nx] = sv;
+        mb[c] = 1;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+void launch_extend_average(hipStream_t s, const Geom& g, const uint8_t* label, const float* uf,
+                           const float* vf, float* ub, float* vb, uint8_t* mf, uint8_t* mb,
+                           int init) {
+  if (init) {
+    k_extavg_init<<<(g.nx * g.ny + 255) / 256, 256, 0, s>>>(g, label, uf, vf, ub, vb, mf, mb);
+    return;
+  }
+  k_extavg_sweep<<<1, 1024, 0, s>>>(g, label, ub, vb, mf, mb);
+}
+
 // ----------------------------------------------- particle-pressure counts
 // Particles per cell for the volume term of the pressure update
 // (src/FluidSolver.cpp:352-360): cell = (int)((x / lengthX) * sizeX), the

@@ -27,7 +27,7 @@ VEL_FRONT, VEL_BACK, VEL_PREV, VEL_DIFF = 0, 1, 2, 3
 PH = dict(CLASSIFY=1, P2G=2, SAVE_PREV=3, GRAVITY=4, DIRICHLET=5, EXTEND=6,
           PRESSURE=7, DIFF=8, G2P_PIC=9, G2P_FLIP=10, G2P_PICFLIP=11, ADVECT=12,
           ADVECT_REJECT=13, SL_SCATTER=14, SL_GATHER=15, ADVECT_GRID=16,
-          CLASSIFY_LEVELSET=17)
+          CLASSIFY_LEVELSET=17, EXTEND_AVERAGE=18)
 
 # every entry point declared in include/fsg.h
 EXPORTS = [

@@ -67,6 +67,7 @@ extern "C" {
 #define FSG_PH_SL_GATHER 15        /* new: gather-form semi-Lagrangian advection + swap */
 #define FSG_PH_ADVECT_GRID 16      /* advectParticlesWithGrid, src/FluidSolver.cpp:774-791 */
 #define FSG_PH_CLASSIFY_LEVELSET 17 /* FluidDomain::classifyCells(LevelSet&), src/FluidDomain.cpp:129-148 */
+#define FSG_PH_EXTEND_AVERAGE 18   /* extendVelocityAvarageing(grid, ext_layers), src/FluidSolver.cpp:625-707 */
 
 /* Integrators for getAdvectedPosition (include/FluidSolver.h:143-145). */
 #define FSG_RK3 0    /* RK3::step, include/OdeSolver.h:102-113 (the reference's choice) */

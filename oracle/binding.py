@@ -18,7 +18,7 @@ SCHEME_SEMI_LAGRANGIAN, SCHEME_PIC, SCHEME_FLIP, SCHEME_PICFLIP, SCHEME_GRID = r
 PH = dict(CLASSIFY=1, P2G=2, SAVE_PREV=3, GRAVITY=4, DIRICHLET=5, EXTEND=6,
           PRESSURE=7, DIFF=8, G2P_PIC=9, G2P_FLIP=10, G2P_PICFLIP=11, ADVECT=12,
           ADVECT_REJECT=13, SL_SCATTER=14, SL_GATHER=15, ADVECT_GRID=16,
-          CLASSIFY_LEVELSET=17)
+          CLASSIFY_LEVELSET=17, EXTEND_AVERAGE=18)
 
 
 def build() -> str:

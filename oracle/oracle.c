@@ -254,6 +254,64 @@ static void dirichlet(STATE *s) {
 
 /* ------------------------------------------------------------------ extension */
 
+/* FluidSolver::extendVelocityAvarageing, src/FluidSolver.cpp:625-707 (not called by
+ * the reference's steps).  One mask per cell, the x pair (:631-643): 1 at LIQUID
+ * cells in both masks; back := front for u and v everywhere.  Each sweep, in raster
+ * order (j outer, i inner), a non-SOLID cell with front mask 0 averages the
+ * cell-centred back velocities ((f(i)+f(i+1))/2 per axis, MacGrid::velXBackBuffer)
+ * of its front-mask-1 neighbours in the order (i-1,j), (i,j-1), (i,j+1), (i+1,j),
+ * writes the average to both of its faces per axis (setVelXBackBuffer: (i,j) and
+ * (i+1,j); setVelYBackBuffer: (i,j) and (i,j+1)) and sets its back mask.  The
+ * writes land in the buffer the sweep reads, so later cells see them.  Masks swap
+ * per sweep (both pairs, :72-76), velocities at the end.  Neighbours and faces
+ * outside the grid (reachable only through non-SOLID border cells, an assert in
+ * the reference) are skipped. */
+static void extend_average(STATE *s, int n_iter) {
+  const int nx = s->nx, ny = s->ny;
+  REAL *uf = U_F(s), *vf = V_F(s), *ub = U_B(s), *vb = V_B(s);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      size_t c = IDX(s, i, j);
+      uint8_t m = cell(s, i, j) == ORC_LIQUID ? 1 : 0;
+      (s->mask_front ? s->mask_u1 : s->mask_u0)[c] = m;
+      (s->mask_front ? s->mask_u0 : s->mask_u1)[c] = m;
+      ub[c] = uf[c];
+      vb[c] = vf[c];
+    }
+  for (int it = 0; it < n_iter; ++it) {
+    const uint8_t *mf = s->mask_front ? s->mask_u1 : s->mask_u0;
+    uint8_t *mb = s->mask_front ? s->mask_u0 : s->mask_u1;
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        size_t c = IDX(s, i, j);
+        if (mf[c] != 0 || cell(s, i, j) == ORC_SOLID) continue;
+        REAL su = 0, sv = 0;
+        int n = 0;
+        const int ni[4] = {i - 1, i, i, i + 1}, nj[4] = {j, j - 1, j + 1, j};
+        for (int q = 0; q < 4; ++q) {
+          const int a = ni[q], b = nj[q];
+          if (a < 0 || a >= nx || b < 0 || b >= ny || mf[IDX(s, a, b)] != 1) continue;
+          const REAL u1 = a + 1 < nx ? ub[IDX(s, a + 1, b)] : ub[IDX(s, a, b)];
+          const REAL v1 = b + 1 < ny ? vb[IDX(s, a, b + 1)] : vb[IDX(s, a, b)];
+          su += (ub[IDX(s, a, b)] + u1) / 2;
+          sv += (vb[IDX(s, a, b)] + v1) / 2;
+          n++;
+        }
+        if (n > 0) {
+          su /= n;
+          sv /= n;
+          ub[c] = su;
+          if (i + 1 < nx) ub[IDX(s, i + 1, j)] = su;
+          vb[c] = sv;
+          if (j + 1 < ny) vb[IDX(s, i, j + 1)] = sv;
+          mb[c] = 1;
+        }
+      }
+    s->mask_front ^= 1;
+  }
+  swap_vel(s);
+}
+
 /* FluidSolver::extendVelocityIndividual, src/FluidSolver.cpp:485-622.
  * Init (:490-530) including the :527 quirk (the v-invalid branch zeroes the front
  * *u*), then n sweeps of neighbour averaging over the back buffer with the front
@@ -756,6 +814,7 @@ int F(orc_run_phase)(STATE *s, int32_t phase, REAL dt) {
     case ORC_PH_GRAVITY: gravity(s, dt); return 0;
     case ORC_PH_DIRICHLET: dirichlet(s); return 0;
     case ORC_PH_EXTEND: extend(s, s->ext_layers); return 0;
+    case ORC_PH_EXTEND_AVERAGE: extend_average(s, s->ext_layers); return 0;
     case ORC_PH_PRESSURE: pressure(s, dt); return 0;
     case ORC_PH_DIFF: save_diff(s); return 0;
     case ORC_PH_G2P_PIC: g2p(s, 0); return 0;

@@ -35,7 +35,8 @@ enum {
   ORC_PH_DIRICHLET = 5, ORC_PH_EXTEND = 6, ORC_PH_PRESSURE = 7, ORC_PH_DIFF = 8,
   ORC_PH_G2P_PIC = 9, ORC_PH_G2P_FLIP = 10, ORC_PH_G2P_PICFLIP = 11,
   ORC_PH_ADVECT = 12, ORC_PH_ADVECT_REJECT = 13, ORC_PH_SL_SCATTER = 14,
-  ORC_PH_SL_GATHER = 15, ORC_PH_ADVECT_GRID = 16, ORC_PH_CLASSIFY_LEVELSET = 17
+  ORC_PH_SL_GATHER = 15, ORC_PH_ADVECT_GRID = 16, ORC_PH_CLASSIFY_LEVELSET = 17,
+  ORC_PH_EXTEND_AVERAGE = 18
 };
 
 #define ORC_DECLARE(R, SFX)                                                         \

@@ -24,13 +24,15 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "fluid_simulation_amd", "fsg_dam_break")
 
 
-def _run_example(tmp_path, n, steps, scheme, pic=0.05, solver=0, tol=1e-7, max_iter=20000):
+def _run_example(tmp_path, n, steps, scheme, pic=0.05, solver=0, tol=1e-7, max_iter=20000,
+                 ppm=None):
     if not os.path.exists(EXE):
         pytest.fail("fsg_dam_break is not built (run __graft_entry__.build())")
     out = os.path.join(str(tmp_path), f"dam_{n}_{scheme}.bin")
     print(f"fsg_dam_break n={n} steps={steps} scheme={scheme} solver={solver}", flush=True)
-    r = subprocess.run([EXE, str(n), str(steps), str(scheme), out, repr(pic), str(solver),
-                        repr(tol), str(max_iter)], capture_output=True, text=True, timeout=120)
+    cmd = [EXE, str(n), str(steps), str(scheme), out, repr(pic), str(solver), repr(tol),
+           str(max_iter)] + ([ppm] if ppm else [])
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     raw = open(out, "rb").read()
     hdr = np.frombuffer(raw[:20], np.int32)
@@ -107,3 +109,43 @@ def test_cpp_reference_defaults_run(tmp_path):
                                         max_iter=100)
     assert len(fin) == len(init) > 0
     assert np.isfinite(fin).all() and np.isfinite(u).all() and np.isfinite(v).all()
+
+
+def _render_reference_frame(n, parts, lab, w=512):
+    """The example's frame (Renderer::clearCanvas, renderGridCellsToCanvas,
+    renderParticlesToCanvas, writeCanvasToPpm; src/Renderer.cpp:14-56,141-162,
+    217-248) for a 1 x 1 world drawn on a w x w canvas, computed independently with
+    numpy: cells are rectangles [int(i c), int((i+1) c)] (c = w / n pixels) with a
+    white outline, particles 3 x 3 squares at int(w x)."""
+    f32 = np.float32
+    img = np.ones((w, w, 3), np.float32)
+    c = w // n
+    fill = {0: (0.7, 0.7, 1.0), 1: (1.0, 1.0, 1.0), 2: (0.5, 0.5, 0.5)}
+    cols = np.array([fill[int(k)] for k in range(3)], np.float32)
+    yy, xx = np.mgrid[0:w, 0:w]
+    cell_lab = lab[np.minimum(yy // c, n - 1), np.minimum(xx // c, n - 1)]
+    img[...] = cols[cell_lab]
+    edge = (xx % c == 0) | (yy % c == 0) | (xx == w - 1) | (yy == w - 1)
+    img[edge] = 1.0
+    px = (f32(w) * parts[:, 0].astype(np.float32)).astype(np.int64)
+    py = (f32(w) * parts[:, 1].astype(np.float32)).astype(np.int64)
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            x = np.clip(px + dx, 0, w - 1)
+            y = np.clip(py + dy, 0, w - 1)
+            img[y, x] = (f32(0.3), f32(0.6), f32(0.9))
+    return (np.clip(img, 0, 1) * f32(255)).astype(np.uint8)
+
+
+def test_cpp_render_frame(tmp_path):
+    """The C++ Renderer draws the host mirrors after GPU steps; the PPM matches an
+    independent numpy rendering of the same particles and labels pixel for pixel."""
+    ppm = os.path.join(str(tmp_path), "frame.ppm")
+    init, fin, u, v, lab = _run_example(tmp_path, 64, 3, fsg.PICFLIP, ppm=ppm)
+    raw = open(ppm, "rb").read()
+    head = b"P6\n512 512\n255\n"
+    assert raw.startswith(head)
+    img = np.frombuffer(raw[len(head):], np.uint8).reshape(512, 512, 3)
+    ref = _render_reference_frame(64, fin, lab)
+    assert np.array_equal(img, ref), int((img != ref).any(-1).sum())
+    assert (img == (76, 153, 229)).all(-1).sum() > len(fin)   # particle colour present

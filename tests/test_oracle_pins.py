@@ -443,6 +443,32 @@ def test_particle_pressure_term_closed_form():
                 assert abs((ub[j, i] - ua[j, i]) - d) < 1e-9
 
 
+def test_extend_average_hand_computed():
+    """extendVelocityAvarageing (src/FluidSolver.cpp:625-707), one sweep on a 5x4
+    grid with one LIQUID cell (1,1): raster order, and each average written to both
+    faces of its cell in the buffer being read -- so (1,2), after (2,1), averages the
+    already updated u(2,1) = 16: (11 + 16) / 2 = 13.5 (a Jacobi sweep would give 16)."""
+    s = OracleSim(5, 4, 5.0, 4.0, 0.01, 0.05, "f64")
+    s.set_params(ext_layers=1)
+    s.label[...] = SOLID
+    s.label[1:3, 1:4] = AIR
+    s.label[1, 1] = LIQUID
+    jj, ii = np.mgrid[0:4, 0:5]
+    u = (10.0 * ii + jj).astype(np.float64)
+    v = (100.0 * ii + jj).astype(np.float64)
+    s.set_vel(u, v, "front")
+    s.run_phase("EXTEND_AVERAGE")
+    uo, vo = s.vel("front")
+    eu, ev = u.copy(), v.copy()
+    eu[1, 2] = eu[1, 3] = 16.0        # cell (2,1): (u(1,1) + u(2,1)) / 2
+    ev[1, 2] = ev[2, 2] = 101.5       # (v(1,1) + v(1,2)) / 2
+    eu[2, 1] = eu[2, 2] = 13.5        # cell (1,2): (u(1,1) + updated u(2,1)) / 2
+    ev[2, 1] = ev[3, 1] = 101.5
+    assert np.array_equal(uo, eu) and np.array_equal(vo, ev)
+    mu, _ = s.valid_masks()
+    assert set(zip(*np.nonzero(mu))) == {(1, 1), (1, 2), (2, 1)}
+
+
 def test_g2p_closed_forms_on_affine_fields():
     n = 32
     s = OracleSim(n, n, float(n), float(n), 0.01, 0.3, "f64")
