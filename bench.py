@@ -186,6 +186,65 @@ def cpu_baseline(scene, steps):
 SOLVERS = {"plain": 0, "persistent": 1, "refined": 2, "mg": 3}
 
 
+def batch_leg(fsg, scenes, args):
+    """The packed batch of scenes: ``--batch-scenes`` non-square dam breaks of sides
+    ``--batch-lo..--batch-hi`` with 1-4 obstacles each, stepped together (one launch
+    per phase, one multigrid-preconditioned solve bounding every scene's relative
+    residual).  For comparison, the same scenes stepped one Domain at a time over a
+    sample of ``--batch-seq-scenes`` scenes."""
+    scs = scenes.dam_break_batch(args.batch_scenes, args.batch_lo, args.batch_hi)
+    cells = sum(sc.nx * sc.ny for sc in scs)
+    parts = sum(len(sc.particles) for sc in scs)
+    b = fsg.Batch.from_scenes(scs, cg_tol=args.tol, cg_max_iter=200)
+    log(f"batch: {len(scs)} scenes, {cells} cells, {parts} particles, packed {b.nx}x{b.ny}")
+    run_steps(fsg, b, args.batch_warmup, scs[0].dt)
+    prof = kernel_profile(fsg, b, 2, scs[0].dt)
+    per_step = {k: ms * n for k, (ms, n) in prof.items()}
+    elapsed, st, _ = timed_segment(fsg, b, args.batch_steps, scs[0].dt)
+    b.close()
+    sps = args.batch_steps / elapsed
+    log(f"batch: {sps:.3f} steps/s = {sps * len(scs):.1f} scene-steps/s "
+        f"({st.cg_iterations} MG iterations in the last solve)")
+    seq = None
+    k = min(args.batch_seq_scenes, len(scs))
+    if k > 0:
+        doms = [make_domain(fsg, sc, fsg.SOLVER_MG, args.tol, 200) for sc in scs[:k]]
+        for d in doms:
+            run_steps(fsg, d, args.batch_warmup, scs[0].dt)
+        for d in doms:
+            d.synchronize()
+        nsteps = 5
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            for d in doms:
+                d.step(fsg.PICFLIP, scs[0].dt)
+        for d in doms:
+            d.synchronize()
+        t1 = time.perf_counter()
+        cells_k = sum(sc.nx * sc.ny for sc in scs[:k])
+        # cell-steps/s of the one-at-a-time form, converted to the batch's scene-steps
+        seq = {"scenes": k, "steps": nsteps,
+               "scene_steps_per_s": k * nsteps / (t1 - t0),
+               "cell_steps_per_s": cells_k * nsteps / (t1 - t0)}
+        for d in doms:
+            d.close()
+        log(f"one scene at a time ({k} scenes): {seq['scene_steps_per_s']:.1f} scene-steps/s")
+    return {
+        "workload": f"dam_break_batch: {len(scs)} non-square scenes, sides {args.batch_lo}.."
+                    f"{args.batch_hi}, 1-4 obstacles each, {cells} cells, {parts} particles, "
+                    f"stepPICFLIP dt 0.01, 0.95 FLIP, per-scene rel tol {args.tol:g}",
+        "steps": args.batch_steps,
+        "steps_per_s": sps,
+        "ms_per_step": 1e3 / sps,
+        "scene_steps_per_s": sps * len(scs),
+        "cell_steps_per_s": sps * cells,
+        "cg_iterations_last_solve": st.cg_iterations,
+        "one_scene_at_a_time": seq,
+        "speedup_vs_one_at_a_time": (sps * cells / seq["cell_steps_per_s"]) if seq else None,
+        "kernel_ms_per_step": {k: round(v, 4) for k, v in per_step.items()},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,6 +258,12 @@ def main():
     ap.add_argument("--plain-tol", type=float, default=1e-6)
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--batch-lo", type=int, default=64)
+    ap.add_argument("--batch-hi", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=50)
+    ap.add_argument("--batch-warmup", type=int, default=3)
+    ap.add_argument("--batch-seq-scenes", type=int, default=32)
     ap.add_argument("--no-fp64-check", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -288,6 +353,9 @@ def main():
         log(f"fp64 check at step {args.warmup + args.profile_steps + args.steps} "
             f"({time.perf_counter() - t0:.1f} s): {errs} labels equal {same_labels}")
 
+    # ---- packed batch of scenes
+    batch = batch_leg(fsg, scenes, args) if args.batch_scenes > 0 else None
+
     # ---- CPU baseline: the FP32 oracle, one thread, bounded sample
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
@@ -326,6 +394,7 @@ def main():
             "max_rel_err_vs_fp64": max(errs.values()) if errs else None,
             "labels_equal_fp64": same_labels,
             "kernel_ms_per_step": {k: round(v, 4) for k, v in per_step.items()},
+            "batch": batch,
         },
         "roofline": roof,
         "cpu_baseline": cpu,

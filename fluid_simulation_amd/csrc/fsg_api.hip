@@ -73,6 +73,32 @@ struct Timer {
 
 }  // namespace
 
+// Packed batch of scenes (fsg_batch_create): scene s occupies the rectangle
+// (ox, oy, nx, ny) of the packed grid; everything outside every scene is SOLID.
+struct BatchInfo {
+  int S = 0;
+  std::vector<int> nx, ny, ox, oy;
+  std::vector<int64_t> np, part_base;
+  std::vector<uint8_t> solid;  // host copy of the packed grid's solid mask
+  fsg::BatchScene* d_scene = nullptr;
+  int2* d_ptile = nullptr;
+  int n_ptile = 0;
+  int4* d_rect = nullptr;
+  int* d_sid = nullptr;
+  double* d_norm = nullptr;
+  double* d_inv = nullptr;
+  // multigrid: packed levels 0 .. la, then every scene's own levels (one workgroup each)
+  int la = 1;
+  std::vector<fsg::SceneCoarse> coarse;
+  fsg::SceneCoarse* d_coarse = nullptr;
+  uint8_t* d_sinf = nullptr;
+  int stride = 0;
+  fsg::BatchTables tables() const { return fsg::BatchTables{d_scene, d_ptile, n_ptile}; }
+  fsg::MgBatchScale scale() const {
+    return fsg::MgBatchScale{S, d_rect, d_sid, d_norm, d_inv};
+  }
+};
+
 struct fsg_domain {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -85,7 +111,8 @@ struct fsg_domain {
   int front = 0;
   float *u_prev = nullptr, *v_prev = nullptr, *u_diff = nullptr, *v_diff = nullptr;
   uint8_t* label = nullptr;
-  uint8_t* solid = nullptr;  // optional
+  uint8_t* solid = nullptr;  // optional (points at solid_buf when set)
+  uint8_t* solid_buf = nullptr;
   float* phi = nullptr;      // optional
   float4* acc = nullptr;
   uint8_t* mask[4] = {nullptr, nullptr, nullptr, nullptr};  // u0 u1 v0 v1
@@ -118,6 +145,7 @@ struct fsg_domain {
   int last_F = 0, last_grid = 0, last_kpt = 0;
   // multigrid form: hierarchy allocated on first use; iterations of the last solve
   MgHost mg{};
+  BatchInfo* batch = nullptr;  // packed batch of scenes (fsg_batch_create), else null
   int* pcounts = nullptr;   // particles per cell (particle-pressure term), on first use
   float* phi2 = nullptr;    // second level-set buffer (advection), on first use
   float* emit_buf = nullptr;  // emitter coordinates
@@ -161,6 +189,23 @@ bool is_pow2(float x) {
   int e;
   const float m = frexpf(x, &e);
   return m == 0.5f;
+}
+
+// MacGrid geometry: spacing = length / size in float (src/MacGrid.cpp:8), lengths
+// = size * spacing (GridInterface::lengthX, include/Grid.h:54-55).
+Geom make_geom(int nx, int ny, float length_x, float length_y) {
+  Geom g{};
+  g.nx = nx;
+  g.ny = ny;
+  g.dx = length_x / (float)nx;
+  g.dy = length_y / (float)ny;
+  g.pow2_x = is_pow2(g.dx);
+  g.pow2_y = is_pow2(g.dy);
+  g.inv_dx = g.pow2_x ? 1.0f / g.dx : 0.f;
+  g.inv_dy = g.pow2_y ? 1.0f / g.dy : 0.f;
+  g.len_x = (float)nx * g.dx;
+  g.len_y = (float)ny * g.dy;
+  return g;
 }
 
 // Timed launch helper: wraps `body` in events when timing is on.
@@ -218,7 +263,10 @@ int ph_classify(fsg_domain* h) {
 void classify_and_splat(fsg_domain* h) {
   timed(h, "cell_init", [&] { launch_cell_init(h->stream, h->g, h->label, h->solid, h->acc, 1, 1); });
   timed(h, "particle_scatter", [&] {
-    launch_particle_scatter(h->stream, h->g, h->pt, h->np, h->label, h->acc, 1, 1);
+    fsg::BatchTables bt;
+    if (h->batch) bt = h->batch->tables();
+    launch_particle_scatter(h->stream, h->g, h->pt, h->np, h->label, h->acc, 1, 1,
+                            h->batch ? &bt : nullptr);
   });
 }
 
@@ -252,18 +300,27 @@ int mg_prepare(fsg_domain* h) {
   m.nx[0] = h->g.nx;
   m.ny[0] = h->g.ny;
   m.nlev = 1;
-  while ((m.nx[m.nlev - 1] > 4 || m.ny[m.nlev - 1] > 4) && m.nlev < kMgMaxLevels) {
+  // a batch: packed levels 0 .. la only, each scene's own levels below (per-scene kernel)
+  const int last = h->batch ? h->batch->la : kMgMaxLevels;
+  while ((m.nx[m.nlev - 1] > 4 || m.ny[m.nlev - 1] > 4) && m.nlev < kMgMaxLevels &&
+         m.nlev <= last) {
     m.nx[m.nlev] = (m.nx[m.nlev - 1] + 1) / 2;
     m.ny[m.nlev] = (m.ny[m.nlev - 1] + 1) / 2;
     ++m.nlev;
   }
-  // the coarsest level is swept by one wavefront (a cell per lane)
-  if (m.nx[m.nlev - 1] * m.ny[m.nlev - 1] > 64)
-    return fail(h, FSG_E_INVALID, "multigrid solver: grid too elongated for the hierarchy");
-  m.lc = 0;
-  while ((size_t)m.nx[m.lc] * m.ny[m.lc] > (size_t)kMgCoarseCells) ++m.lc;
-  if (m.lc >= m.nlev)
-    return fail(h, FSG_E_INVALID, "multigrid solver: grid too elongated for the hierarchy");
+  if (h->batch) {
+    if (m.nlev != last + 1)
+      return fail(h, FSG_E_INVALID, "multigrid solver: packed grid smaller than its split level");
+    m.lc = last;
+  } else {
+    // the coarsest level is swept by one wavefront (a cell per lane)
+    if (m.nx[m.nlev - 1] * m.ny[m.nlev - 1] > 64)
+      return fail(h, FSG_E_INVALID, "multigrid solver: grid too elongated for the hierarchy");
+    m.lc = 0;
+    while ((size_t)m.nx[m.lc] * m.ny[m.lc] > (size_t)kMgCoarseCells) ++m.lc;
+    if (m.lc >= m.nlev)
+      return fail(h, FSG_E_INVALID, "multigrid solver: grid too elongated for the hierarchy");
+  }
   // cell offsets of the one-workgroup levels (its LDS layout and its info array)
   MgCoarseArgs& a = m.coarse;
   int off = 0;
@@ -315,12 +372,29 @@ int mg_prepare(fsg_domain* h) {
   // the kernel's LDS limit is raised once, process-wide, to a fixed 128 KiB.
   constexpr size_t kCoarseLdsLimit = 128 * 1024;
   m.coarse_lds = (size_t)a.total * (3 * sizeof(float) + 1);
+  if (h->batch) m.coarse_lds = (size_t)h->batch->stride * (3 * sizeof(float) + 1);
   if (m.coarse_lds > kCoarseLdsLimit)
     return fail(h, FSG_E_INVALID, "multigrid solver: coarse levels exceed the LDS budget");
   if (mg_set_lds_limit(kCoarseLdsLimit) != hipSuccess)
     return fail(h, FSG_E_HIP, "multigrid solver: cannot raise the LDS limit of the coarse kernel");
   h->mg_ready = true;
   return FSG_OK;
+}
+
+fsg::SceneCoarseArgs scene_coarse_args(const fsg_domain* h) {
+  const BatchInfo& b = *h->batch;
+  const MgHost& m = h->mg;
+  fsg::SceneCoarseArgs a;
+  a.tab = b.d_coarse;
+  a.n_scenes = b.S;
+  a.pitch = m.nx[b.la];
+  a.stride = b.stride;
+  a.inf_all = b.d_sinf;
+  a.lab = m.lab[b.la];
+  a.r_in = m.rl[b.la];
+  a.x_out = m.xo[b.la];
+  a.omega = m.coarse.omega;
+  return a;
 }
 
 // CG preconditioned by one V-cycle per iteration (FSG_SOLVER_MG).  Iterations are
@@ -330,11 +404,16 @@ int mg_solve(fsg_domain* h, int F) {
   if (int rc = mg_prepare(h)) return rc;
   MgHost& m = h->mg;
   timed(h, "cg_mg_setup", [&] {
+    fsg::MgBatchScale bs;
+    if (h->batch) bs = h->batch->scale();
     launch_mg_setup(h->stream, h->g, m, h->label, UF(h), VF(h), h->prm.project_mean,
-                    h->prm.warm_start != 0, h->sc);
+                    h->prm.warm_start != 0 && !h->batch, h->sc, h->batch ? &bs : nullptr);
+    if (h->batch) launch_mg_scene_setup(h->stream, scene_coarse_args(h));
   });
   const int max_iter = h->prm.cg_max_iter;
-  const double tol = h->prm.cg_tol;
+  // a batch: every scene's b' has unit norm, so |r| <= (tol / sqrt(S)) |b| bounds every
+  // scene's |r_s| / |b'_s| by tol
+  const double tol = h->batch ? h->prm.cg_tol / sqrt((double)h->batch->S) : h->prm.cg_tol;
   int it = 0, batch = std::max(h->mg_last_iters + 1, 4);
   CgScalars hs{};
   static const char* const kDownNames[4] = {"cg_mg_down0", "cg_mg_down1", "cg_mg_down2",
@@ -345,7 +424,10 @@ int mg_solve(fsg_domain* h, int F) {
       for (int l = 0; l < m.lc; ++l)
         timed(h, kDownNames[std::min(l, 3)],
               [&] { launch_mg_down(h->stream, m, l, it, max_iter, tol, h->sc); });
-      timed(h, "cg_mg_coarse", [&] { launch_mg_coarse(h->stream, m, it, max_iter, tol, h->sc); });
+      if (h->batch)
+        timed(h, "cg_mg_coarse", [&] { launch_mg_scene_coarse(h->stream, scene_coarse_args(h), h->sc); });
+      else
+        timed(h, "cg_mg_coarse", [&] { launch_mg_coarse(h->stream, m, it, max_iter, tol, h->sc); });
       for (int l = m.lc - 1; l >= 0; --l)
         timed(h, kUpNames[std::min(l, 3)], [&] { launch_mg_up(h->stream, m, l, h->sc); });
       timed(h, "cg_mg_dir", [&] { launch_mg_dir(h->stream, m, it, h->sc); });
@@ -357,7 +439,12 @@ int mg_solve(fsg_domain* h, int F) {
     batch = 2;
   }
   timed(h, "cg_mg_finish",
-        [&] { launch_mg_finish(h->stream, m, h->prm.project_mean, h->sc, h->xg); });
+        [&] {
+          fsg::MgBatchScale bs;
+          if (h->batch) bs = h->batch->scale();
+          launch_mg_finish(h->stream, m, h->prm.project_mean, h->sc, h->xg,
+                           h->batch ? &bs : nullptr);
+        });
   h->mg_last_iters = hs.iter;
   h->mg_solved = true;
   h->host_stats.cg_iterations = hs.iter;
@@ -534,8 +621,11 @@ int step_flip_family(fsg_domain* h, int scheme, float dt) {
   const int g2p = scheme == FSG_SCHEME_PIC ? 1 : scheme == FSG_SCHEME_FLIP ? 2 : 3;
   const int move = scheme == FSG_SCHEME_PICFLIP ? 2 : 1;
   timed(h, "particle_gather", [&] {
+    fsg::BatchTables bt;
+    if (h->batch) bt = h->batch->tables();
     launch_particle_gather(h->stream, h->g, h->pt, h->np, UF(h), VF(h), h->u_diff, h->v_diff,
-                           h->pic_ratio, g2p, move, dt, h->label, h->reject);
+                           h->pic_ratio, g2p, move, dt, h->label, h->reject,
+                           h->batch ? &bt : nullptr);
   });
   return FSG_OK;
 }
@@ -615,17 +705,7 @@ int fsg_create(fsg_domain** out, int32_t nx, int32_t ny, float length_x, float l
   if (hipGetDevice(&h->device) != hipSuccess) return bail(FSG_E_HIP);
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return bail(FSG_E_HIP);
   fsg_params_default(&h->prm);
-  Geom& g = h->g;
-  g.nx = nx;
-  g.ny = ny;
-  g.dx = length_x / (float)nx;  // src/MacGrid.cpp:8
-  g.dy = length_y / (float)ny;
-  g.pow2_x = is_pow2(g.dx);
-  g.pow2_y = is_pow2(g.dy);
-  g.inv_dx = g.pow2_x ? 1.0f / g.dx : 0.f;
-  g.inv_dy = g.pow2_y ? 1.0f / g.dy : 0.f;
-  g.len_x = (float)nx * g.dx;   // GridInterface::lengthX, include/Grid.h:54
-  g.len_y = (float)ny * g.dy;
+  h->g = make_geom(nx, ny, length_x, length_y);
   h->density = density;
   h->pic_ratio = pic_ratio;     // the constructor stores it unclamped (src/FluidDomain.cpp:54-68)
   const size_t C = (size_t)nx * ny;
@@ -683,6 +763,7 @@ int fsg_create(fsg_domain** out, int32_t nx, int32_t ny, float length_x, float l
 int fsg_destroy(fsg_domain* h) {
   if (!h) return FSG_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  delete h->batch;  // its device arrays are in h->allocations
   h->timer.collect();
   for (void* p : h->allocations) (void)hipFree(p);
   if (h->pt) (void)hipFree(h->pt);
@@ -768,7 +849,6 @@ static int reserve_particles(fsg_domain* h, int64_t n) {
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));
   if (h->pt) (void)hipFree(h->pt);
-  if (h->emit_buf) (void)hipFree(h->emit_buf);
   if (h->reject) (void)hipFree(h->reject);
   h->pt = np;
   h->reject = nr;
@@ -927,12 +1007,10 @@ int fsg_set_solid_mask(fsg_domain* h, const uint8_t* mask) {
     h->solid = nullptr;  // the allocation stays owned by the handle
     return FSG_OK;
   }
-  static_assert(sizeof(uint8_t) == 1, "");
-  uint8_t* d = nullptr;
-  HIPCHK(h, dalloc(h, &d, ncell(h)));
-  HIPCHK(h, hipMemcpyAsync(d, mask, ncell(h), hipMemcpyHostToDevice, h->stream));
+  if (!h->solid_buf) HIPCHK(h, dalloc(h, &h->solid_buf, ncell(h)));
+  HIPCHK(h, hipMemcpyAsync(h->solid_buf, mask, ncell(h), hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  h->solid = d;
+  h->solid = h->solid_buf;
   return FSG_OK;
 }
 
@@ -957,6 +1035,13 @@ int fsg_validate_pool(fsg_domain* h, int32_t pool_nx, int32_t pool_ny, float poo
 
 int fsg_step(fsg_domain* h, int32_t scheme, float dt) {
   if (!h) return FSG_E_INVALID;
+  if (h->batch) {
+    if (scheme != FSG_SCHEME_PIC && scheme != FSG_SCHEME_FLIP && scheme != FSG_SCHEME_PICFLIP)
+      return fail(h, FSG_E_INVALID, "fsg_step: a batch steps PIC, FLIP or PICFLIP");
+    if (h->prm.solver != FSG_SOLVER_MG || h->prm.particle_pressure_k != 0.f)
+      return fail(h, FSG_E_INVALID,
+                  "fsg_step: a batch needs the multigrid solver and particle_pressure_k = 0");
+  }
   if (scheme != FSG_SCHEME_GRID_ADVECT) {
     int rc = check_validate(h);
     if (rc) return rc;
@@ -977,6 +1062,7 @@ int fsg_step(fsg_domain* h, int32_t scheme, float dt) {
 
 int fsg_run_phase(fsg_domain* h, int32_t phase, float dt) {
   if (!h) return FSG_E_INVALID;
+  if (h->batch) return fail(h, FSG_E_INVALID, "fsg_run_phase: not available for a batch");
   const bool rec = true;
   switch (phase) {
     case FSG_PH_CLASSIFY: ph_classify(h); break;
@@ -1200,6 +1286,252 @@ int fsg_reset_timing(fsg_domain* h) {
   h->timer.collect();
   h->timer.acc.clear();
   return FSG_OK;
+}
+
+// ============================================================= packed batch
+// Scenes packed side by side into one grid ("shelf" rows, origins on multiples of
+// 8 cells so that the first multigrid levels never mix scenes); the rest of the
+// grid is SOLID.  Grid phases and the pressure solve run on the packed grid (one
+// launch per phase for all scenes); particle phases use each scene's own geometry
+// and scene-local coordinates.
+int fsg_batch_create(fsg_domain** out, int32_t n_scenes, const int32_t* nx, const int32_t* ny,
+                     float delta, float density, float pic_ratio) {
+  if (!out || n_scenes < 1 || !nx || !ny || !(delta > 0.f)) return FSG_E_INVALID;
+  *out = nullptr;
+  auto up = [](int a, int m) { return (a + m - 1) / m * m; };
+  for (int s = 0; s < n_scenes; ++s)
+    if (nx[s] < 3 || ny[s] < 3 || nx[s] > (1 << 15) || ny[s] > (1 << 15)) return FSG_E_INVALID;
+  // Multigrid split: packed levels 0 .. la, then each scene's own hierarchy from its
+  // rectangle of level la (at most kSceneCoarseCells cells, one workgroup).  Slots are
+  // aligned to 2^la (and to 8 cells) and leave at least one all-SOLID cell of level la
+  // after every scene in x and y, so no level of the packed hierarchy couples scenes.
+  int la = 1;
+  for (bool fits = false; !fits; ) {
+    fits = true;
+    for (int s = 0; s < n_scenes && fits; ++s)
+      fits = (int64_t)((nx[s] + (1 << la) - 1) >> la) * ((ny[s] + (1 << la) - 1) >> la) <=
+             fsg::kSceneCoarseCells;
+    if (!fits) ++la;
+  }
+  const int A = 1 << la, U = std::max(8, A);
+  auto slot = [&](int n) { return up(A * ((n + A - 1) / A + 1), U); };
+  int64_t area = 0;
+  int wmax = 0;
+  for (int s = 0; s < n_scenes; ++s) {
+    area += (int64_t)slot(nx[s]) * slot(ny[s]);
+    wmax = std::max(wmax, slot(nx[s]));
+  }
+  const int W = std::max(wmax, up((int)ceil(sqrt(1.1 * (double)area)), U));
+  std::vector<int> order(n_scenes);
+  for (int s = 0; s < n_scenes; ++s) order[s] = s;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ny[a] > ny[b]; });
+  std::vector<int> ox(n_scenes), oy(n_scenes);
+  int x = 0, y = 0, shelf = 0;
+  for (int s : order) {
+    const int w = slot(nx[s]), hh = slot(ny[s]);
+    if (x + w > W) {
+      y += shelf;
+      x = 0;
+      shelf = 0;
+    }
+    ox[s] = x;
+    oy[s] = y;
+    x += w;
+    shelf = std::max(shelf, hh);
+  }
+  const int H = y + shelf;
+  if ((int64_t)W * H > ((int64_t)1 << 30)) return FSG_E_INVALID;
+  fsg_domain* h = nullptr;
+  int rc = fsg_create(&h, W, H, (float)W * delta, (float)H * delta, density, pic_ratio);
+  if (rc) return rc;
+  auto bail = [&](int code, const char* msg) {
+    fsg_destroy(h);
+    (void)msg;
+    return code;
+  };
+  BatchInfo* b = new (std::nothrow) BatchInfo();
+  if (!b) return bail(FSG_E_NOMEM, "");
+  h->batch = b;
+  b->S = n_scenes;
+  b->la = la;
+  b->nx.assign(nx, nx + n_scenes);
+  b->ny.assign(ny, ny + n_scenes);
+  b->ox = ox;
+  b->oy = oy;
+  b->np.assign(n_scenes, 0);
+  b->part_base.assign(n_scenes, 0);
+  b->solid.assign((size_t)W * H, 1);
+  std::vector<int> sid((size_t)W * H, -1);
+  std::vector<int4> rect(n_scenes);
+  std::vector<fsg::BatchScene> table(n_scenes);
+  for (int s = 0; s < n_scenes; ++s) {
+    const Geom gs = make_geom(nx[s], ny[s], (float)nx[s] * delta, (float)ny[s] * delta);
+    if (gs.dx != h->g.dx || gs.dy != h->g.dy)
+      return bail(FSG_E_INVALID, "scene spacing differs from the packed grid's");
+    for (int j = 0; j < ny[s]; ++j)
+      for (int i = 0; i < nx[s]; ++i) {
+        const size_t c = (size_t)(ox[s] + i) + (size_t)(oy[s] + j) * W;
+        sid[c] = s;
+        const bool border = i == 0 || j == 0 || i == nx[s] - 1 || j == ny[s] - 1;
+        b->solid[c] = border ? 1 : 0;
+      }
+    rect[s] = make_int4(ox[s], oy[s], nx[s], ny[s]);
+    table[s] = fsg::BatchScene{gs, 0, 0, ox[s] + oy[s] * W, W};
+    // the scene's own multigrid levels: its rectangle of level la, halved until 4 x 4
+    fsg::SceneCoarse c{};
+    c.cx = ox[s] >> la;
+    c.cy = oy[s] >> la;
+    int w = (nx[s] + A - 1) >> la, hh = (ny[s] + A - 1) >> la, off = 0;
+    for (;;) {
+      c.nx[c.nl] = w;
+      c.ny[c.nl] = hh;
+      c.off[c.nl] = off;
+      off += w * hh;
+      ++c.nl;
+      if ((w <= 4 && hh <= 4) || c.nl == fsg::kSceneMaxLevels) break;
+      w = (w + 1) / 2;
+      hh = (hh + 1) / 2;
+    }
+    if (c.nx[c.nl - 1] * c.ny[c.nl - 1] > 64) return bail(FSG_E_INVALID, "scene hierarchy");
+    c.total = (off + 3) & ~3;
+    b->stride = std::max(b->stride, c.total);
+    b->coarse.push_back(c);
+  }
+  bool ok = true;
+  ok &= dalloc(h, &b->d_scene, (size_t)n_scenes) == hipSuccess;
+  ok &= dalloc(h, &b->d_rect, (size_t)n_scenes) == hipSuccess;
+  ok &= dalloc(h, &b->d_sid, (size_t)W * H) == hipSuccess;
+  ok &= dalloc(h, &b->d_norm, (size_t)n_scenes) == hipSuccess;
+  ok &= dalloc(h, &b->d_inv, (size_t)n_scenes) == hipSuccess;
+  ok &= dalloc(h, &b->d_coarse, (size_t)n_scenes) == hipSuccess;
+  ok &= dalloc(h, &b->d_sinf, (size_t)n_scenes * b->stride) == hipSuccess;
+  if (!ok) return bail(FSG_E_NOMEM, "");
+  if (hipMemcpyAsync(b->d_scene, table.data(), sizeof(fsg::BatchScene) * n_scenes,
+                     hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+      hipMemcpyAsync(b->d_rect, rect.data(), sizeof(int4) * n_scenes, hipMemcpyHostToDevice,
+                     h->stream) != hipSuccess ||
+      hipMemcpyAsync(b->d_sid, sid.data(), sizeof(int) * sid.size(), hipMemcpyHostToDevice,
+                     h->stream) != hipSuccess ||
+      hipMemcpyAsync(b->d_coarse, b->coarse.data(), sizeof(fsg::SceneCoarse) * n_scenes,
+                     hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+      hipStreamSynchronize(h->stream) != hipSuccess)
+    return bail(FSG_E_HIP, "");
+  if ((rc = fsg_set_solid_mask(h, b->solid.data())) != FSG_OK) return bail(rc, "");
+  // labels as MacGrid's constructor leaves them, with the scene borders SOLID
+  launch_cell_init(h->stream, h->g, h->label, h->solid, h->acc, 1, 0);
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(FSG_E_HIP, "");
+  h->prm.solver = FSG_SOLVER_MG;
+  *out = h;
+  return FSG_OK;
+}
+
+int fsg_batch_num_scenes(const fsg_domain* h, int32_t* n) {
+  if (!h || !h->batch || !n) return FSG_E_INVALID;
+  *n = h->batch->S;
+  return FSG_OK;
+}
+
+int fsg_batch_scene_rect(const fsg_domain* h, int32_t scene, int32_t* rect4) {
+  if (!h || !h->batch || !rect4 || scene < 0 || scene >= h->batch->S) return FSG_E_INVALID;
+  const BatchInfo& b = *h->batch;
+  rect4[0] = b.ox[scene];
+  rect4[1] = b.oy[scene];
+  rect4[2] = b.nx[scene];
+  rect4[3] = b.ny[scene];
+  return FSG_OK;
+}
+
+// All scenes' particles at once: counts[s] particles of scene s, concatenated, in
+// scene-local coordinates.
+int fsg_batch_set_particles(fsg_domain* h, const int64_t* counts, const float* xyuv) {
+  if (!h || !h->batch || !counts) return FSG_E_INVALID;
+  BatchInfo& b = *h->batch;
+  int64_t total = 0;
+  std::vector<int2> ptile;
+  std::vector<fsg::BatchScene> table(b.S);
+  HIPCHK(h, hipMemcpy(table.data(), b.d_scene, sizeof(fsg::BatchScene) * b.S,
+                      hipMemcpyDeviceToHost));
+  for (int s = 0; s < b.S; ++s) {
+    if (counts[s] < 0) return fail(h, FSG_E_INVALID, "fsg_batch_set_particles: negative count");
+    b.part_base[s] = total;
+    b.np[s] = counts[s];
+    table[s].part_base = total;
+    table[s].np = counts[s];
+    for (int64_t t = 0; t < (counts[s] + 255) / 256; ++t) ptile.push_back(make_int2(s, (int)t));
+    total += counts[s];
+  }
+  if (total > 0 && !xyuv) return FSG_E_INVALID;
+  if (int rc = fsg_set_particles(h, xyuv, total)) return rc;
+  if ((int)ptile.size() > b.n_ptile) {
+    HIPCHK(h, dalloc(h, &b.d_ptile, ptile.size()));
+  }
+  b.n_ptile = (int)ptile.size();
+  if (b.n_ptile > 0)
+    HIPCHK(h, hipMemcpyAsync(b.d_ptile, ptile.data(), sizeof(int2) * ptile.size(),
+                             hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(b.d_scene, table.data(), sizeof(fsg::BatchScene) * b.S,
+                           hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+// Interior SOLID cells of one scene (its ny x nx mask, row-major); its border stays SOLID.
+int fsg_batch_set_solid_mask(fsg_domain* h, int32_t scene, const uint8_t* mask) {
+  if (!h || !h->batch || !mask || scene < 0 || scene >= h->batch->S) return FSG_E_INVALID;
+  BatchInfo& b = *h->batch;
+  const int W = h->g.nx, nx = b.nx[scene], ny = b.ny[scene];
+  for (int j = 1; j < ny - 1; ++j)
+    for (int i = 1; i < nx - 1; ++i)
+      b.solid[(size_t)(b.ox[scene] + i) + (size_t)(b.oy[scene] + j) * W] =
+          mask[(size_t)i + (size_t)j * nx] ? 1 : 0;
+  return fsg_set_solid_mask(h, b.solid.data());
+}
+
+int fsg_batch_get_particles(fsg_domain* h, int32_t scene, float* xyuv, int64_t capacity) {
+  if (!h || !h->batch || scene < 0 || scene >= h->batch->S) return FSG_E_INVALID;
+  const BatchInfo& b = *h->batch;
+  if (capacity < b.np[scene]) return fail(h, FSG_E_INVALID, "fsg_batch_get_particles: capacity");
+  if (b.np[scene] == 0) return FSG_OK;
+  HIPCHK(h, hipMemcpyAsync(xyuv, h->pt + b.part_base[scene], sizeof(float4) * b.np[scene],
+                           hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+int fsg_batch_num_particles(const fsg_domain* h, int32_t scene, int64_t* n) {
+  if (!h || !h->batch || !n || scene < 0 || scene >= h->batch->S) return FSG_E_INVALID;
+  *n = h->batch->np[scene];
+  return FSG_OK;
+}
+
+// One scene's rectangle of a packed grid into a dense nx * ny host array.
+static int batch_copy_rect(fsg_domain* h, int scene, const void* src, size_t elem, void* dst) {
+  const BatchInfo& b = *h->batch;
+  const size_t W = (size_t)h->g.nx;
+  const char* base =
+      static_cast<const char*>(src) + ((size_t)b.ox[scene] + (size_t)b.oy[scene] * W) * elem;
+  HIPCHK(h, hipMemcpy2DAsync(dst, (size_t)b.nx[scene] * elem, base, W * elem,
+                             (size_t)b.nx[scene] * elem, (size_t)b.ny[scene],
+                             hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+int fsg_batch_get_velocity(fsg_domain* h, int32_t scene, int32_t which, float* u, float* v) {
+  if (!h || !h->batch || scene < 0 || scene >= h->batch->S) return FSG_E_INVALID;
+  float* du = vel_ptr(h, which, 0);
+  float* dv = vel_ptr(h, which, 1);
+  if (!du) return fail(h, FSG_E_INVALID, "fsg_batch_get_velocity: bad buffer id");
+  if (u)
+    if (int rc = batch_copy_rect(h, scene, du, sizeof(float), u)) return rc;
+  if (v)
+    if (int rc = batch_copy_rect(h, scene, dv, sizeof(float), v)) return rc;
+  return FSG_OK;
+}
+
+int fsg_batch_get_labels(fsg_domain* h, int32_t scene, uint8_t* labels) {
+  if (!h || !h->batch || !labels || scene < 0 || scene >= h->batch->S) return FSG_E_INVALID;
+  return batch_copy_rect(h, scene, h->label, 1, labels);
 }
 
 }  // extern "C"

@@ -37,9 +37,11 @@ __device__ __forceinline__ float div_y(float a, const Geom& g) {
 
 // Grid<T>::valueInterpolated (include/Grid.h:117-144) on a row-major grid:
 // truncating casts, fraction before clamping, i+1 clamped after i, x then y lerp.
-// xs, ys are the coordinates already divided by the spacing.
-__device__ __forceinline__ float interp_scaled(const float* __restrict__ g, int nx, int ny,
-                                               float xs, float ys) {
+// xs, ys are the coordinates already divided by the spacing; nx, ny bound the
+// indices and `pitch` is the row stride (nx for a grid of its own, larger for one
+// scene inside a packed grid).
+__device__ __forceinline__ float interp_pitched(const float* __restrict__ g, int nx, int ny,
+                                                int pitch, float xs, float ys) {
   int i = (int)xs;
   int j = (int)ys;
   const float fi = xs - (float)i;
@@ -48,13 +50,18 @@ __device__ __forceinline__ float interp_scaled(const float* __restrict__ g, int 
   j = clampi(j, 0, ny - 1);
   const int i1 = clampi(i + 1, 0, nx - 1);
   const int j1 = clampi(j + 1, 0, ny - 1);
-  const float v00 = g[i + j * nx];
-  const float v10 = g[i1 + j * nx];
-  const float v01 = g[i + j1 * nx];
-  const float v11 = g[i1 + j1 * nx];
+  const float v00 = g[i + j * pitch];
+  const float v10 = g[i1 + j * pitch];
+  const float v01 = g[i + j1 * pitch];
+  const float v11 = g[i1 + j1 * pitch];
   const float v0 = (1.0f - fi) * v00 + fi * v10;
   const float v1 = (1.0f - fi) * v01 + fi * v11;
   return (1.0f - fj) * v0 + fj * v1;
+}
+
+__device__ __forceinline__ float interp_scaled(const float* __restrict__ g, int nx, int ny,
+                                               float xs, float ys) {
+  return interp_pitched(g, nx, ny, nx, xs, ys);
 }
 
 // MacGrid::velXInterpolated: sample u at (x, y - dy*0.5); the offset is formed in

@@ -7,13 +7,32 @@
 
 namespace fsg {
 
+// Batched form: scenes packed into one grid.  Scene s occupies the rectangle whose
+// first cell is `cell_off` with row stride `pitch` (the packed grid's nx); its
+// geometry `g` is the scene's own (sizes, spacing, lengths), and its particles are
+// pt[part_base, part_base + np) in scene-local coordinates.
+struct BatchScene {
+  Geom g;
+  int64_t part_base;
+  int64_t np;
+  int cell_off;
+  int pitch;
+};
+// Particle tile t (256 particles) belongs to scene ptile[t].x, local tile ptile[t].y.
+struct BatchTables {
+  const BatchScene* scene;
+  const int2* ptile;
+  int n_ptile;
+};
+
 // k_face_update phase flags (applied in this order)
 enum { FU_P2G = 1, FU_PREV = 2, FU_GRAVITY = 4, FU_DIRICHLET = 8, FU_EXTINIT = 16 };
 
 void launch_cell_init(hipStream_t s, const Geom& g, uint8_t* label, const uint8_t* solid,
                       float4* acc, int do_labels, int do_acc);
 void launch_particle_scatter(hipStream_t s, const Geom& g, const float4* pt, int64_t np,
-                             uint8_t* label, float4* acc, int do_classify, int do_p2g);
+                             uint8_t* label, float4* acc, int do_classify, int do_p2g,
+                             const BatchTables* batch = nullptr);
 void launch_face_update(hipStream_t s, const Geom& g, int flags, float* u, float* v,
                         const float4* acc, float* prev_u, float* prev_v, const uint8_t* label,
                         float gx_dt, float gy_dt, float* ext_u, float* ext_v, uint8_t* mxf,
@@ -34,7 +53,8 @@ void launch_diff(hipStream_t s, const Geom& g, const float* u, const float* v, c
                  const float* pv, float* du, float* dv);
 void launch_particle_gather(hipStream_t s, const Geom& g, float4* pt, int64_t np, const float* u,
                             const float* v, const float* du, const float* dv, float pic_ratio,
-                            int g2p, int move, float dt, const uint8_t* label, uint8_t* reject);
+                            int g2p, int move, float dt, const uint8_t* label, uint8_t* reject,
+                            const BatchTables* batch = nullptr);
 void launch_sl_scatter(hipStream_t s, const Geom& g, const uint8_t* label, const float* uf,
                        const float* vf, float* ub, float* vb, int integrator, float dt);
 void launch_sl_gather(hipStream_t s, const Geom& g, const uint8_t* label, const float* uf,
@@ -143,6 +163,26 @@ struct MgCoarseArgs {
   float* x_out;       // cycle result of level lc
 };
 
+// Packed batch of scenes: scene s's own hierarchy below level `la` of the packed one.
+constexpr int kSceneCoarseCells = 4096;  // cells of a scene's rectangle at level la, at most
+constexpr int kSceneMaxLevels = 8;
+struct SceneCoarse {
+  int cx, cy;     // first cell of the scene's rectangle in the packed grid of level la
+  int nl, total;  // its levels (the first is the rectangle); cells over all, a multiple of 4
+  int nx[kSceneMaxLevels], ny[kSceneMaxLevels], off[kSceneMaxLevels];
+};
+struct SceneCoarseArgs {
+  const SceneCoarse* tab = nullptr;
+  int n_scenes = 0;
+  int pitch = 0;              // width of the packed grid of level la
+  int stride = 0;             // bytes of info per scene in inf_all (>= every total, mult. of 4)
+  uint8_t* inf_all = nullptr; // scene s's info at s * stride
+  const uint8_t* lab = nullptr;  // packed labels of level la
+  const float* r_in = nullptr;   // packed right-hand side of level la
+  float* x_out = nullptr;        // packed cycle result of level la
+  float omega = 0.8f;
+};
+
 struct MgHost {
   int nlev = 0, lc = 0;
   int nx[kMgMaxLevels] = {}, ny[kMgMaxLevels] = {};
@@ -160,22 +200,36 @@ struct MgHost {
   size_t coarse_lds = 0;
 };
 
+// Packed batch of scenes: each scene's right-hand side is scaled to unit norm before
+// the joint solve and its solution scaled back, so that one stopping test bounds
+// every scene's own relative residual.
+struct MgBatchScale {
+  int n_scenes;
+  const int4* rect;  // per scene: first cell (x, y) in the packed grid, nx, ny
+  const int* sid;    // per packed cell: scene, or -1 outside every scene
+  double* norm;      // per scene: |b'_s|
+  double* inv;       // per scene: 1 / |b'_s| (0 when b'_s = 0)
+};
+
 int mg_vec_blocks(int C);
 int mg_up_blocks(int nx, int ny);
 hipError_t mg_set_lds_limit(size_t bytes);
 void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_t* label,
                      const float* uf, const float* vf, int project_mean, int warm,
-                     CgScalars* sc);
+                     CgScalars* sc, const MgBatchScale* batch = nullptr);
 // one CG iteration: down(l) for l < lc, coarse, up(l) for l = lc-1 .. 0, dir, xr
 void launch_mg_down(hipStream_t s, const MgHost& m, int l, int it, int max_iter, double tol,
                     CgScalars* sc);
 void launch_mg_coarse(hipStream_t s, const MgHost& m, int it, int max_iter, double tol,
                       CgScalars* sc);
+// a batch: every scene's own levels (labels / info once per solve, a cycle per iteration)
+void launch_mg_scene_setup(hipStream_t s, const SceneCoarseArgs& a);
+void launch_mg_scene_coarse(hipStream_t s, const SceneCoarseArgs& a, CgScalars* sc);
 void launch_mg_up(hipStream_t s, const MgHost& m, int l, CgScalars* sc);
 void launch_mg_dir(hipStream_t s, const MgHost& m, int it, CgScalars* sc);
 void launch_mg_xr(hipStream_t s, const MgHost& m, int it, CgScalars* sc);
 // x at the unknowns -> xg (FP32) and m.x[0] (FP64, read by the pressure update)
 void launch_mg_finish(hipStream_t s, const MgHost& m, int project_mean, CgScalars* sc,
-                      float* xg);
+                      float* xg, const MgBatchScale* batch = nullptr);
 
 }  // namespace fsg

@@ -89,20 +89,56 @@ __device__ __forceinline__ void splat_terms(const SplatCorners& s, float value, 
   add(s.i1, s.j1, comp + 1, s.fi * w1);
 }
 
-__global__ void __launch_bounds__(256) k_particle_scatter(Geom g, const float4* __restrict__ pt,
+// Particle block -> (geometry, particles, grid rectangle): the whole grid for one
+// domain; the block's scene (tiles never straddle scenes) for the packed batch.
+struct PartTile {
+  Geom g;
+  int64_t p;       // this thread's particle
+  int64_t np_end;  // end of the block's particle range
+  int off, pitch;  // cell (i, j) of the block's grid is off + i + j * pitch
+};
+
+template <bool BATCH>
+__device__ __forceinline__ PartTile part_tile(const Geom& g0, int64_t np, const BatchTables& tb) {
+  PartTile pt;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  if (BATCH) {
+    const int2 e = tb.ptile[t];
+    const BatchScene sc = tb.scene[e.x];
+    pt.g = sc.g;
+    pt.p = sc.part_base + (int64_t)e.y * 256 + threadIdx.x;
+    pt.np_end = sc.part_base + sc.np;
+    pt.off = sc.cell_off;
+    pt.pitch = sc.pitch;
+  } else {
+    pt.g = g0;
+    pt.p = (int64_t)t * 256 + threadIdx.x;
+    pt.np_end = np;
+    pt.off = 0;
+    pt.pitch = g0.nx;
+  }
+  return pt;
+}
+
+template <bool BATCH>
+__global__ void __launch_bounds__(256) k_particle_scatter(Geom g0, const float4* __restrict__ pt,
                                                           int64_t np, uint8_t* __restrict__ label,
                                                           float* __restrict__ acc, int do_classify,
-                                                          int do_p2g) {
-  const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
-  const int64_t p = t * 256 + threadIdx.x;
-  const bool live = p < np;
+                                                          int do_p2g, BatchTables tb) {
+  const PartTile T = part_tile<BATCH>(g0, np, tb);
+  const Geom& g = T.g;
+  const int64_t p = T.p;
+  const bool live = p < T.np_end;
+  label += T.off;
+  acc += 4 * (int64_t)T.off;
+  const int pitch = T.pitch;
   const float4 q = live ? pt[p] : make_float4(0.f, 0.f, 0.f, 0.f);
   if (do_classify && live) {
     int x = (int)((q.x / g.len_x) * (float)g.nx);
     int y = (int)((q.y / g.len_y) * (float)g.ny);
     x = clampi(x, 0, g.nx - 1);
     y = clampi(y, 0, g.ny - 1);
-    const int c = x + y * g.nx;
+    const int c = x + y * pitch;
     if (label[c] != kSolid) label[c] = kLiquid;
   }
   if (!do_p2g) return;
@@ -141,7 +177,7 @@ __global__ void __launch_bounds__(256) k_particle_scatter(Geom g, const float4* 
   if (!tiled) {
     if (!live) return;
     auto gadd = [&](int i, int j, int comp, float v) {
-      unsafeAtomicAdd(acc + 4 * (i + j * nx) + comp, v);
+      unsafeAtomicAdd(acc + 4 * (i + j * pitch) + comp, v);
     };
     splat_terms(su, q.z, 0, gadd);
     splat_terms(sv, q.w, 2, gadd);
@@ -161,7 +197,7 @@ __global__ void __launch_bounds__(256) k_particle_scatter(Geom g, const float4* 
   __syncthreads();
   for (int c = threadIdx.x; c < area; c += 256) {
     const float4 v = s_tile[c];
-    float* dst = acc + 4 * ((i0 + c % bw) + (j0 + c / bw) * nx);
+    float* dst = acc + 4 * ((i0 + c % bw) + (j0 + c / bw) * pitch);
     if (v.x != 0.f) unsafeAtomicAdd(dst + 0, v.x);
     if (v.y != 0.f) unsafeAtomicAdd(dst + 1, v.y);
     if (v.z != 0.f) unsafeAtomicAdd(dst + 2, v.z);
@@ -532,29 +568,37 @@ __global__ void __launch_bounds__(256) k_diff(Geom g, const float* __restrict__ 
 // G2P (src/FluidSolver.cpp:921-963) + particle motion (src/MarkerParticleSet.cpp:
 // 40-62).  g2p: 0 none, 1 PIC, 2 FLIP, 3 PICFLIP; move: 0 none, 1 advect,
 // 2 advect and step back out of SOLID cells, tested at (int)(pos / delta).
+template <bool BATCH>
 __global__ void __launch_bounds__(256) k_particle_gather(
-    Geom g, float4* __restrict__ pt, int64_t np, const float* __restrict__ u,
+    Geom g0, float4* __restrict__ pt, int64_t np, const float* __restrict__ u,
     const float* __restrict__ v, const float* __restrict__ du, const float* __restrict__ dv,
     float pic_ratio, int g2p, int move, float dt, const uint8_t* __restrict__ label,
-    uint8_t* __restrict__ reject) {
-  const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
-  const int64_t p = t * 256 + threadIdx.x;
-  if (p >= np) return;
+    uint8_t* __restrict__ reject, BatchTables tb) {
+  const PartTile T = part_tile<BATCH>(g0, np, tb);
+  const Geom& g = T.g;
+  const int64_t p = T.p;
+  if (p >= T.np_end) return;
+  const int pitch = T.pitch;
+  u += T.off;
+  v += T.off;
+  du += T.off;
+  dv += T.off;
+  label += T.off;
   float4 q = pt[p];
   if (g2p) {
     const float xs_u = div_x(q.x, g), ys_u = div_y(q.y - g.dy * 0.5f, g);
     const float xs_v = div_x(q.x - g.dx * 0.5f, g), ys_v = div_y(q.y, g);
     if (g2p == 1) {
-      q.z = interp_scaled(u, g.nx, g.ny, xs_u, ys_u);
-      q.w = interp_scaled(v, g.nx, g.ny, xs_v, ys_v);
+      q.z = interp_pitched(u, g.nx, g.ny, pitch, xs_u, ys_u);
+      q.w = interp_pitched(v, g.nx, g.ny, pitch, xs_v, ys_v);
     } else if (g2p == 2) {
-      q.z = q.z + interp_scaled(du, g.nx, g.ny, xs_u, ys_u);
-      q.w = q.w + interp_scaled(dv, g.nx, g.ny, xs_v, ys_v);
+      q.z = q.z + interp_pitched(du, g.nx, g.ny, pitch, xs_u, ys_u);
+      q.w = q.w + interp_pitched(dv, g.nx, g.ny, pitch, xs_v, ys_v);
     } else {
-      const float pu = interp_scaled(u, g.nx, g.ny, xs_u, ys_u);
-      const float pv = interp_scaled(v, g.nx, g.ny, xs_v, ys_v);
-      const float fu = q.z + interp_scaled(du, g.nx, g.ny, xs_u, ys_u);
-      const float fv = q.w + interp_scaled(dv, g.nx, g.ny, xs_v, ys_v);
+      const float pu = interp_pitched(u, g.nx, g.ny, pitch, xs_u, ys_u);
+      const float pv = interp_pitched(v, g.nx, g.ny, pitch, xs_v, ys_v);
+      const float fu = q.z + interp_pitched(du, g.nx, g.ny, pitch, xs_u, ys_u);
+      const float fv = q.w + interp_pitched(dv, g.nx, g.ny, pitch, xs_v, ys_v);
       q.z = pu * pic_ratio + fu * (1.0f - pic_ratio);
       q.w = pv * pic_ratio + fv * (1.0f - pic_ratio);
     }
@@ -568,7 +612,7 @@ __global__ void __launch_bounds__(256) k_particle_gather(
       int y = (int)div_y(q.y, g);
       x = clampi(x, 0, g.nx - 1);
       y = clampi(y, 0, g.ny - 1);
-      if (label[x + y * g.nx] == kSolid) {
+      if (label[x + y * pitch] == kSolid) {
         q.x += q.z * -dt;
         q.y += q.w * -dt;
         flag = 1;
@@ -683,10 +727,17 @@ void launch_cell_init(hipStream_t s, const Geom& g, uint8_t* label, const uint8_
                                                              do_acc);
 }
 void launch_particle_scatter(hipStream_t s, const Geom& g, const float4* pt, int64_t np,
-                             uint8_t* label, float4* acc, int do_classify, int do_p2g) {
+                             uint8_t* label, float4* acc, int do_classify, int do_p2g,
+                             const BatchTables* batch) {
+  if (batch) {
+    if (batch->n_ptile == 0) return;
+    k_particle_scatter<true><<<batch->n_ptile, 256, 0, s>>>(g, pt, np, label, (float*)acc,
+                                                            do_classify, do_p2g, *batch);
+    return;
+  }
   if (np == 0) return;
-  k_particle_scatter<<<nblocks(np), 256, 0, s>>>(g, pt, np, label, (float*)acc, do_classify,
-                                                 do_p2g);
+  k_particle_scatter<false><<<nblocks(np), 256, 0, s>>>(g, pt, np, label, (float*)acc,
+                                                        do_classify, do_p2g, BatchTables{});
 }
 void launch_face_update(hipStream_t s, const Geom& g, int flags, float* u, float* v,
                         const float4* acc, float* prev_u, float* prev_v, const uint8_t* label,
@@ -740,11 +791,17 @@ void launch_diff(hipStream_t s, const Geom& g, const float* u, const float* v, c
 }
 void launch_particle_gather(hipStream_t s, const Geom& g, float4* pt, int64_t np, const float* u,
                             const float* v, const float* du, const float* dv, float pic_ratio,
-                            int g2p, int move, float dt, const uint8_t* label,
-                            uint8_t* reject) {
+                            int g2p, int move, float dt, const uint8_t* label, uint8_t* reject,
+                            const BatchTables* batch) {
+  if (batch) {
+    if (batch->n_ptile == 0) return;
+    k_particle_gather<true><<<batch->n_ptile, 256, 0, s>>>(g, pt, np, u, v, du, dv, pic_ratio,
+                                                           g2p, move, dt, label, reject, *batch);
+    return;
+  }
   if (np == 0) return;
-  k_particle_gather<<<nblocks(np), 256, 0, s>>>(g, pt, np, u, v, du, dv, pic_ratio, g2p, move,
-                                                dt, label, reject);
+  k_particle_gather<false><<<nblocks(np), 256, 0, s>>>(g, pt, np, u, v, du, dv, pic_ratio, g2p,
+                                                       move, dt, label, reject, BatchTables{});
 }
 void launch_sl_scatter(hipStream_t s, const Geom& g, const uint8_t* label, const float* uf,
                        const float* vf, float* ub, float* vb, int integrator, float dt) {

@@ -245,6 +245,38 @@ __global__ void __launch_bounds__(kT) k_mg_setup1(int nx, int ny, int project_me
   }
 }
 
+// Packed batch: |b'_s| over each scene's rectangle, one workgroup per scene (a
+// fixed summation order), and its reciprocal.
+__global__ void __launch_bounds__(1024) k_mg_scene_norm(const double* __restrict__ b, int pitch,
+                                                        const int4* __restrict__ rect,
+                                                        double* __restrict__ norm,
+                                                        double* __restrict__ inv) {
+  __shared__ double sm[16];
+  const int4 r = rect[blockIdx.x];
+  double v[1] = {0.0};
+  for (int q = threadIdx.x; q < r.z * r.w; q += 1024) {
+    const int i = q % r.z, j = q / r.z;
+    const double x = b[(r.x + i) + (int64_t)(r.y + j) * pitch];
+    v[0] += x * x;
+  }
+  block_sum<1, 1024>(v, sm);
+  if (threadIdx.x == 0) {
+    const double n = sqrt(v[0]);
+    norm[blockIdx.x] = n;
+    inv[blockIdx.x] = n > 0.0 ? 1.0 / n : 0.0;
+  }
+}
+
+// b'[c] *= factor[scene of c]
+__global__ void __launch_bounds__(kT) k_mg_scene_scale(int C, const int* __restrict__ sid,
+                                                        const double* __restrict__ factor,
+                                                        double* __restrict__ b) {
+  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT) {
+    const int s = sid[c];
+    if (s >= 0) b[c] *= factor[s];
+  }
+}
+
 // Warm start: the previous solution at this solve's unknowns, 0 elsewhere.
 __global__ void __launch_bounds__(kT) k_mg_warm(int C, const uint8_t* __restrict__ inf,
                                                  const double* __restrict__ prev,
@@ -536,34 +568,15 @@ __device__ __forceinline__ float nbr_sum(const float* a, int c, int i, int j, in
 
 constexpr int kCoarseThreads = 1024;
 
-// With check != 0 (the hierarchy starts in this kernel) it runs level 0's
-// convergence test first and forms the r.z partial at the end.
-__global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, MgCheck chk, int it,
-                                                             int check,
-                                                             double* __restrict__ part_rz) {
-  __shared__ double sm[2 * 16];
-  if (check) {
-    if (mg_check<kCoarseThreads>(chk, it, sm)) return;
-  } else if (chk.sc->done) {
-    return;
-  }
-  extern __shared__ float lds[];
-  const int tot = a.total;  // a multiple of 4
-  float* R = lds;
-  float* X = R + tot;
-  float* T = X + tot;
-  uint8_t* INF = reinterpret_cast<uint8_t*>(T + tot);
-  {
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(a.inf_all);
-    uint32_t* dst = reinterpret_cast<uint32_t*>(INF);
-    for (int q = threadIdx.x; q < tot / 4; q += kCoarseThreads) dst[q] = src[q];
-    const int n0 = a.nx[a.lc] * a.ny[a.lc];
-    for (int q = threadIdx.x; q < n0; q += kCoarseThreads) R[q] = a.r_in[q];
-  }
-  __syncthreads();
-  const float w = a.omega;
-  for (int l = a.lc; l < a.nlev - 1; ++l) {
-    const int nx = a.nx[l], ny = a.ny[l], n = nx * ny, o = a.off[l] - a.off[a.lc];
+// The one-workgroup V-cycle over the levels 0 .. nl-1 of a small hierarchy held in
+// LDS: R, X, T and INF, level l at offset off[l] (off[0] = 0).  R's level 0 holds
+// the right-hand side; the coarser right-hand sides are formed here.  Level 0's
+// result is left in X when nl == 1, else in T.
+__device__ void coarse_cycle(int nl, const int* __restrict__ lnx, const int* __restrict__ lny,
+                             const int* __restrict__ loff, float w, int sweeps, float* R,
+                             float* X, float* T, const uint8_t* INF) {
+  for (int l = 0; l < nl - 1; ++l) {
+    const int nx = lnx[l], ny = lny[l], n = nx * ny, o = loff[l];
     float *r = R + o, *x = X + o, *t = T + o;
     const uint8_t* inf = INF + o;
     const float inv_nx = 1.0f / (float)nx;
@@ -595,7 +608,7 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, Mg
       t[q] = v;
     }
     __syncthreads();
-    const int cnx = a.nx[l + 1], cny = a.ny[l + 1], co = a.off[l + 1] - a.off[a.lc];
+    const int cnx = lnx[l + 1], cny = lny[l + 1], co = loff[l + 1];
     const float inv_cnx = 1.0f / (float)cnx;
     for (int q = threadIdx.x; q < cnx * cny; q += kCoarseThreads) {
       int I, J;
@@ -635,8 +648,8 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, Mg
   }
   {
     // coarsest level: <= 16 cells (checked on the host), one wavefront
-    const int l = a.nlev - 1;
-    const int nx = a.nx[l], ny = a.ny[l], n = nx * ny, o = a.off[l] - a.off[a.lc];
+    const int l = nl - 1;
+    const int nx = lnx[l], ny = lny[l], n = nx * ny, o = loff[l];
     if (threadIdx.x < 64) {
       const int q = threadIdx.x;
       const bool in = q < n;
@@ -645,7 +658,7 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, Mg
       const int i = in ? q % nx : 0, j = in ? q / nx : 0;
       const float nn = nns_of(f), rn = inv_nns(f);
       float x = 0.f;
-      for (int sw = 0; sw < a.sweeps; ++sw) {
+      for (int sw = 0; sw < sweeps; ++sw) {
         const float xl = __shfl(x, q - 1 < 0 ? 0 : q - 1, 64);
         const float xr = __shfl(x, q + 1 > 63 ? 63 : q + 1, 64);
         const float xd = __shfl(x, q - nx < 0 ? 0 : q - nx, 64);
@@ -663,12 +676,12 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, Mg
   }
   // up: t = xs + P e, x = sweep(t), t = sweep(x); a level's result is in T (the
   // coarsest level's in X)
-  for (int l = a.nlev - 2; l >= a.lc; --l) {
-    const int nx = a.nx[l], ny = a.ny[l], n = nx * ny, o = a.off[l] - a.off[a.lc];
-    const int cnx = a.nx[l + 1], cny = a.ny[l + 1], co = a.off[l + 1] - a.off[a.lc];
+  for (int l = nl - 2; l >= 0; --l) {
+    const int nx = lnx[l], ny = lny[l], n = nx * ny, o = loff[l];
+    const int cnx = lnx[l + 1], cny = lny[l + 1], co = loff[l + 1];
     float *r = R + o, *x = X + o, *t = T + o;
     const uint8_t* inf = INF + o;
-    const float* e = (l + 1 == a.nlev - 1 ? X : T) + co;
+    const float* e = (l + 1 == nl - 1 ? X : T) + co;
     const uint8_t* cinf = INF + co;
     const float inv_nx = 1.0f / (float)nx;
     for (int q = threadIdx.x; q < n; q += kCoarseThreads) {
@@ -715,9 +728,46 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, Mg
     }
     __syncthreads();
   }
+}
+
+// With check != 0 (the hierarchy starts in this kernel) it runs level 0's
+// convergence test first and forms the r.z partial at the end.
+__global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, MgCheck chk, int it,
+                                                             int check,
+                                                             double* __restrict__ part_rz) {
+  __shared__ double sm[2 * 16];
+  __shared__ int s_lv[3 * kMgMaxLevels];
+  if (check) {
+    if (mg_check<kCoarseThreads>(chk, it, sm)) return;
+  } else if (chk.sc->done) {
+    return;
+  }
+  extern __shared__ float lds[];
+  const int tot = a.total;  // a multiple of 4
+  float* R = lds;
+  float* X = R + tot;
+  float* T = X + tot;
+  uint8_t* INF = reinterpret_cast<uint8_t*>(T + tot);
+  const int nl = a.nlev - a.lc;
+  if (threadIdx.x < nl) {
+    const int l = a.lc + threadIdx.x;
+    s_lv[threadIdx.x] = a.nx[l];
+    s_lv[kMgMaxLevels + threadIdx.x] = a.ny[l];
+    s_lv[2 * kMgMaxLevels + threadIdx.x] = a.off[l] - a.off[a.lc];
+  }
+  {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(a.inf_all);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(INF);
+    for (int q = threadIdx.x; q < tot / 4; q += kCoarseThreads) dst[q] = src[q];
+    const int n0 = a.nx[a.lc] * a.ny[a.lc];
+    for (int q = threadIdx.x; q < n0; q += kCoarseThreads) R[q] = a.r_in[q];
+  }
+  __syncthreads();
+  coarse_cycle(nl, s_lv, s_lv + kMgMaxLevels, s_lv + 2 * kMgMaxLevels, a.omega, a.sweeps, R, X,
+               T, INF);
   {
     const int n = a.nx[a.lc] * a.ny[a.lc];
-    const float* res = a.lc == a.nlev - 1 ? X : T;
+    const float* res = nl == 1 ? X : T;
     double rz = 0.0;
     for (int q = threadIdx.x; q < n; q += kCoarseThreads) {
       a.x_out[q] = res[q];
@@ -729,6 +779,87 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, Mg
       if (threadIdx.x == 0) part_rz[0] = v[0];
     }
   }
+}
+
+// ---------------------------------------------------------------- per-scene coarse
+// Packed batch of scenes: from level `la` of the packed hierarchy on, every scene
+// has a hierarchy of its own (its rectangle of level la, cells outside it count as
+// SOLID), solved by one workgroup per scene.  Scene rectangles are separated by at
+// least one all-SOLID cell of level la, so levels 0 .. la never couple two scenes
+// either, and the preconditioner is block-diagonal over the scenes.
+
+// Labels and info of every scene's own levels (one workgroup per scene; dynamic LDS:
+// the largest scene's cells over all its levels).
+__global__ void __launch_bounds__(kCoarseThreads) k_mg_scene_setup(SceneCoarseArgs a) {
+  __shared__ SceneCoarse t;
+  extern __shared__ uint8_t slab[];
+  if (threadIdx.x == 0) t = a.tab[blockIdx.x];
+  __syncthreads();
+  {
+    const int nx = t.nx[0], n = nx * t.ny[0];
+    for (int q = threadIdx.x; q < n; q += kCoarseThreads) {
+      const int i = q % nx, j = q / nx;
+      slab[q] = a.lab[(size_t)(t.cx + i) + (size_t)(t.cy + j) * a.pitch];
+    }
+  }
+  __syncthreads();
+  for (int l = 1; l < t.nl; ++l) {
+    const int fnx = t.nx[l - 1], fny = t.ny[l - 1], cnx = t.nx[l], n = cnx * t.ny[l];
+    const uint8_t* fl = slab + t.off[l - 1];
+    for (int q = threadIdx.x; q < n; q += kCoarseThreads) {
+      const int I = q % cnx, J = q / cnx;
+      bool air = false, liq = false;
+      for (int b = 0; b < 2; ++b)
+        for (int e = 0; e < 2; ++e) {
+          const int i = 2 * I + e, j = 2 * J + b;
+          if (i < fnx && j < fny) {
+            const uint8_t L = fl[i + j * fnx];
+            air |= L == kAir;
+            liq |= L == kLiquid;
+          }
+        }
+      slab[t.off[l] + q] = air ? kAir : (liq ? kLiquid : kSolid);
+    }
+    __syncthreads();
+  }
+  uint8_t* out = a.inf_all + (size_t)blockIdx.x * a.stride;
+  for (int l = 0; l < t.nl; ++l) {
+    const int nx = t.nx[l], n = nx * t.ny[l];
+    for (int q = threadIdx.x; q < n; q += kCoarseThreads)
+      out[t.off[l] + q] = make_info(slab + t.off[l], nx, t.ny[l], q % nx, q / nx);
+  }
+}
+
+// One V-cycle of every scene's own levels: right-hand side from the packed level la,
+// result back into it (cells outside every rectangle keep their zero).
+__global__ void __launch_bounds__(kCoarseThreads) k_mg_scene_coarse(SceneCoarseArgs a,
+                                                                   const CgScalars* sc) {
+  __shared__ SceneCoarse t;
+  if (sc->done) return;
+  if (threadIdx.x == 0) t = a.tab[blockIdx.x];
+  __syncthreads();
+  extern __shared__ float lds[];
+  const int tot = a.stride;  // a multiple of 4, >= t.total
+  float* R = lds;
+  float* X = R + tot;
+  float* T = X + tot;
+  uint8_t* INF = reinterpret_cast<uint8_t*>(T + tot);
+  const int nx = t.nx[0], n0 = nx * t.ny[0];
+  {
+    const uint32_t* src =
+        reinterpret_cast<const uint32_t*>(a.inf_all + (size_t)blockIdx.x * a.stride);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(INF);
+    for (int q = threadIdx.x; q < t.total / 4; q += kCoarseThreads) dst[q] = src[q];
+    for (int q = threadIdx.x; q < n0; q += kCoarseThreads)
+      R[q] = a.r_in[(size_t)(t.cx + q % nx) + (size_t)(t.cy + q / nx) * a.pitch];
+  }
+  __syncthreads();
+  const int cl = t.nl - 1;
+  const int sweeps = 2 * (t.nx[cl] > t.ny[cl] ? t.nx[cl] : t.ny[cl]) + 8;
+  coarse_cycle(t.nl, t.nx, t.ny, t.off, a.omega, sweeps, R, X, T, INF);
+  const float* res = t.nl == 1 ? X : T;
+  for (int q = threadIdx.x; q < n0; q += kCoarseThreads)
+    a.x_out[(size_t)(t.cx + q % nx) + (size_t)(t.cy + q / nx) * a.pitch] = res[q];
 }
 
 // ---------------------------------------------------------------- CG vectors
@@ -842,7 +973,9 @@ __global__ void __launch_bounds__(kT) k_mg_finish(int C, const uint8_t* __restri
                                                    const double* __restrict__ part_nx,
                                                    const double* __restrict__ part_n, int n_part,
                                                    double* __restrict__ xout,
-                                                   float* __restrict__ xg) {
+                                                   float* __restrict__ xg,
+                                                   const int* __restrict__ sid,
+                                                   const double* __restrict__ scene_norm) {
   __shared__ double sm[4];
   double shift = 0.0;
   if (part_nx) {
@@ -852,7 +985,8 @@ __global__ void __launch_bounds__(kT) k_mg_finish(int C, const uint8_t* __restri
   }
   const double* x = (sc->iter & 1) ? x1 : x0;
   for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT) {
-    const double v = is_act(inf[c]) ? x[c] - shift : 0.0;
+    double v = is_act(inf[c]) ? x[c] - shift : 0.0;
+    if (sid) v = sid[c] >= 0 ? v * scene_norm[sid[c]] : 0.0;
     xout[c] = v;  // may be x[c] itself: each thread reads and writes only its cells
     xg[c] = (float)v;
   }
@@ -869,15 +1003,32 @@ int mg_vec_blocks(int C) { return vec_blocks(C); }
 int mg_up_blocks(int nx, int ny) { return ((nx + 31) / 32) * ((ny + 31) / 32); }
 
 hipError_t mg_set_lds_limit(size_t bytes) {
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_coarse),
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_coarse),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_scene_coarse),
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+void launch_mg_scene_setup(hipStream_t s, const SceneCoarseArgs& a) {
+  k_mg_scene_setup<<<a.n_scenes, kCoarseThreads, a.stride, s>>>(a);
+}
+
+void launch_mg_scene_coarse(hipStream_t s, const SceneCoarseArgs& a, CgScalars* sc) {
+  k_mg_scene_coarse<<<a.n_scenes, kCoarseThreads, (size_t)a.stride * (3 * sizeof(float) + 1), s>>>(
+      a, sc);
 }
 
 void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_t* label,
                      const float* uf, const float* vf, int project_mean, int warm,
-                     CgScalars* sc) {
+                     CgScalars* sc, const MgBatchScale* batch) {
   const int C = g.nx * g.ny, nb = vec_blocks(C);
   k_mg_setup0<<<nb, kT, 0, s>>>(g, label, uf, vf, m.inf[0], m.b, m.part_sum, m.part_cnt);
+  if (batch) {
+    k_mg_scene_norm<<<batch->n_scenes, 1024, 0, s>>>(m.b, g.nx, batch->rect, batch->norm,
+                                                     batch->inv);
+    k_mg_scene_scale<<<nb, kT, 0, s>>>(C, batch->sid, batch->inv, m.b);
+  }
   // x[0] holds the previous solve's solution (k_mg_finish); x[1] is free until the
   // first update of this solve
   if (warm) k_mg_warm<<<nb, kT, 0, s>>>(C, m.inf[0], m.x[0], m.x[1]);
@@ -959,16 +1110,18 @@ void launch_mg_xr(hipStream_t s, const MgHost& m, int it, CgScalars* sc) {
 }
 
 void launch_mg_finish(hipStream_t s, const MgHost& m, int project_mean, CgScalars* sc,
-                      float* xg) {
+                      float* xg, const MgBatchScale* batch) {
   const int C = m.nx[0] * m.ny[0], nb = vec_blocks(C);
+  const int* sid = batch ? batch->sid : nullptr;
+  const double* norm = batch ? batch->norm : nullptr;
   if (project_mean) {
     // part_sum / part_cnt are free again after the setup
     k_mg_wsum<<<nb, kT, 0, s>>>(C, m.inf[0], m.x[0], m.x[1], sc, m.part_sum, m.part_cnt);
     k_mg_finish<<<nb, kT, 0, s>>>(C, m.inf[0], m.x[0], m.x[1], sc, m.part_sum, m.part_cnt, nb,
-                                  m.x[0], xg);
+                                  m.x[0], xg, sid, norm);
   } else {
     k_mg_finish<<<nb, kT, 0, s>>>(C, m.inf[0], m.x[0], m.x[1], sc, nullptr, nullptr, 0, m.x[0],
-                                  xg);
+                                  xg, sid, norm);
   }
 }
 

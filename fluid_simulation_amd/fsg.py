@@ -40,7 +40,10 @@ EXPORTS = [
     "fsg_get_fluid_indices", "fsg_get_valid_masks", "fsg_get_ext_info",
     "fsg_get_reject_flags", "fsg_get_stats", "fsg_reset_stats", "fsg_set_timing",
     "fsg_get_timing", "fsg_reset_timing", "fsg_set_timing_prefix", "fsg_emit_lattice",
-    "fsg_advect_level_set", "fsg_get_level_set",
+    "fsg_advect_level_set", "fsg_get_level_set", "fsg_batch_create", "fsg_batch_num_scenes",
+    "fsg_batch_scene_rect", "fsg_batch_set_particles", "fsg_batch_set_solid_mask",
+    "fsg_batch_num_particles", "fsg_batch_get_particles", "fsg_batch_get_velocity",
+    "fsg_batch_get_labels",
 ]
 
 
@@ -125,6 +128,16 @@ def load(path: str | None = None):
                                   C.c_float], C.c_int),
             "fsg_advect_level_set": ([H, C.c_float], C.c_int),
             "fsg_get_level_set": ([H, C.c_void_p], C.c_int),
+            "fsg_batch_create": ([P(H), C.c_int32, C.c_void_p, C.c_void_p, C.c_float, C.c_float,
+                                  C.c_float], C.c_int),
+            "fsg_batch_num_scenes": ([H, P(C.c_int32)], C.c_int),
+            "fsg_batch_scene_rect": ([H, C.c_int32, C.c_void_p], C.c_int),
+            "fsg_batch_set_particles": ([H, C.c_void_p, C.c_void_p], C.c_int),
+            "fsg_batch_set_solid_mask": ([H, C.c_int32, C.c_void_p], C.c_int),
+            "fsg_batch_num_particles": ([H, C.c_int32, P(C.c_int64)], C.c_int),
+            "fsg_batch_get_particles": ([H, C.c_int32, C.c_void_p, C.c_int64], C.c_int),
+            "fsg_batch_get_velocity": ([H, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p], C.c_int),
+            "fsg_batch_get_labels": ([H, C.c_int32, C.c_void_p], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -386,3 +399,87 @@ class Domain:
         if scene.phi is not None:
             d.set_level_set(scene.phi)
         return d
+
+
+class Batch(Domain):
+    """Many scenes packed into one grid and stepped together (fsg_batch_create):
+    one kernel launch per phase for all scenes; particles in scene-local
+    coordinates; the pressure solve bounds every scene's own relative residual."""
+
+    def __init__(self, sizes, delta: float, density: float, pic_ratio: float):
+        self._L = load()
+        self._h = C.c_void_p()
+        nx = np.ascontiguousarray([s[0] for s in sizes], np.int32)
+        ny = np.ascontiguousarray([s[1] for s in sizes], np.int32)
+        rc = self._L.fsg_batch_create(C.byref(self._h), len(sizes), nx.ctypes.data,
+                                      ny.ctypes.data, delta, density, pic_ratio)
+        if rc != OK:
+            raise FsgError(f"fsg_batch_create failed ({rc})")
+        self.sizes = [(int(a), int(b)) for a, b in zip(nx, ny)]
+        self.n_scenes = len(self.sizes)
+        geo = self.geometry()
+        self.nx, self.ny = geo["nx"], geo["ny"]  # the packed grid
+
+    @classmethod
+    def from_scenes(cls, scenes, **params):
+        """One batch holding ``scenes`` (all with unit spacing ratio length / n equal,
+        the same density and pic_ratio); scene s keeps index s."""
+        if not scenes:
+            raise FsgError("a batch needs at least one scene")
+        sc0 = scenes[0]
+        delta = sc0.length_x / sc0.nx
+        for sc in scenes:
+            if (sc.length_x / sc.nx != delta or sc.length_y / sc.ny != delta
+                    or sc.density != sc0.density or sc.pic_ratio != sc0.pic_ratio):
+                raise FsgError("batched scenes must share spacing, density and pic_ratio")
+        b = cls([(sc.nx, sc.ny) for sc in scenes], delta, sc0.density, sc0.pic_ratio)
+        if params:
+            b.set_params(**params)
+        b.set_scene_particles([sc.particles for sc in scenes])
+        for s, sc in enumerate(scenes):
+            if sc.solid_mask is not None:
+                b.set_scene_solid_mask(s, sc.solid_mask)
+        return b
+
+    def scene_rect(self, s):
+        r = (C.c_int32 * 4)()
+        self._chk(self._L.fsg_batch_scene_rect(self._h, s, r), "fsg_batch_scene_rect")
+        return tuple(int(v) for v in r)
+
+    def set_scene_particles(self, parts_list):
+        counts = np.ascontiguousarray([len(p) for p in parts_list], np.int64)
+        allp = _f32(np.concatenate([np.asarray(p, np.float32).reshape(-1, 4) for p in parts_list])
+                    if counts.sum() else np.zeros((0, 4), np.float32))
+        self._chk(self._L.fsg_batch_set_particles(self._h, counts.ctypes.data,
+                                                  allp.ctypes.data if len(allp) else None),
+                  "fsg_batch_set_particles")
+
+    def set_scene_solid_mask(self, s, mask):
+        m = np.ascontiguousarray(mask, np.uint8)
+        self._chk(self._L.fsg_batch_set_solid_mask(self._h, s, m.ctypes.data),
+                  "fsg_batch_set_solid_mask")
+
+    def scene_particles(self, s):
+        n = C.c_int64()
+        self._chk(self._L.fsg_batch_num_particles(self._h, s, C.byref(n)), "num_particles")
+        out = np.zeros((n.value, 4), np.float32)
+        if n.value:
+            self._chk(self._L.fsg_batch_get_particles(self._h, s, out.ctypes.data, n.value),
+                      "fsg_batch_get_particles")
+        return out
+
+    def scene_velocity(self, s, which=VEL_FRONT):
+        nx, ny = self.sizes[s]
+        u = np.zeros((ny, nx), np.float32)
+        v = np.zeros((ny, nx), np.float32)
+        self._chk(self._L.fsg_batch_get_velocity(self._h, s, which, u.ctypes.data, v.ctypes.data),
+                  "fsg_batch_get_velocity")
+        return u, v
+
+    def scene_labels(self, s):
+        nx, ny = self.sizes[s]
+        lab = np.zeros((ny, nx), np.uint8)
+        self._chk(self._L.fsg_batch_get_labels(self._h, s, lab.ctypes.data),
+                  "fsg_batch_get_labels")
+        return lab
+

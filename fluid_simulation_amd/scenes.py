@@ -109,6 +109,57 @@ def dam_break(n: int, seed: int = 0, fx: float = 0.4, fy: float = 0.6,
                  name=f"dam_break_{n}")
 
 
+def dam_break_batch(count: int, lo: int = 64, hi: int = 512, seed: int = 0,
+                    dt: float = 0.01, pic_ratio: float = 0.05) -> list[Scene]:
+    """``count`` non-square dam breaks of independent random sizes ``lo..hi`` (unit
+    cells, 1-cell solid border), for the packed batch of scenes.
+
+    Scene s draws from its own generator (seeded by ``(seed, s)``): nx != ny; 1-4
+    axis-aligned SOLID rectangles covering at most 15 % of the cells (redrawn until
+    every liquid region touches air, so no pressure system is singular); a fluid
+    rectangle at cell (1, 1) covering 20-60 % of the interior, minus the solids, with
+    4 jittered particles per cell (particle seed ``seed * 100003 + s``).  All
+    scenes share dt, density (= dt) and pic_ratio, as one batch must.
+    """
+    from scipy import ndimage
+    out = []
+    for s in range(count):
+        rng = np.random.default_rng([seed, s])
+        nx = ny = 0
+        while nx == ny:
+            nx, ny = (int(v) for v in rng.integers(lo, hi + 1, size=2))
+        f = rng.uniform(0.2, 0.6)
+        fx = rng.uniform(f, 1.0)
+        bw = max(1, math.ceil(fx * (nx - 2) - 1e-9))
+        bh = max(1, math.ceil(f / fx * (ny - 2) - 1e-9))
+        fluid = np.zeros((ny, nx), bool)
+        fluid[1:1 + bh, 1:1 + bw] = True
+        while True:
+            mask = np.zeros((ny, nx), bool)
+            for _ in range(int(rng.integers(1, 5))):
+                w = int(rng.integers(2, max(3, (nx - 2) // 4) + 1))
+                h = int(rng.integers(2, max(3, (ny - 2) // 4) + 1))
+                x0 = int(rng.integers(1, nx - 1 - w + 1))
+                y0 = int(rng.integers(1, ny - 1 - h + 1))
+                trial = mask.copy()
+                trial[y0:y0 + h, x0:x0 + w] = True
+                if trial.sum() <= 0.15 * nx * ny:
+                    mask = trial
+            liquid = fluid & ~mask
+            air = np.zeros((ny, nx), bool)
+            air[1:-1, 1:-1] = True
+            air &= ~mask & ~liquid
+            comp, n = ndimage.label(liquid)
+            touch = ndimage.binary_dilation(air) & liquid
+            if mask.any() and liquid.any() and len(np.unique(comp[touch])) == n:
+                break
+        jj, ii = np.nonzero(liquid)          # j outer, i inner
+        parts = jittered_particles(ii, jj, nx, 1.0, 1.0, seed * 100003 + s)
+        out.append(Scene(nx, ny, float(nx), float(ny), dt, pic_ratio, dt, parts,
+                         solid_mask=mask.astype(np.uint8), name=f"dam_break_batch_{s}"))
+    return out
+
+
 def interior_mask(nx: int, ny: int) -> np.ndarray:
     m = np.zeros((ny, nx), dtype=bool)
     m[1:-1, 1:-1] = True

@@ -207,6 +207,31 @@ int fsg_emit_lattice(fsg_domain *h, const float *xs, int32_t nxp, const float *y
 int fsg_advect_level_set(fsg_domain *h, float dt);
 int fsg_get_level_set(fsg_domain *h, float *phi);
 
+/* ---- packed batch of scenes (no counterpart in the reference) ---------------
+   Many independent scenes stepped together: each scene (nx[s] x ny[s] cells of
+   spacing delta, its own 1-cell SOLID border) gets a rectangle of one packed grid,
+   the rest of which is SOLID.  The handle is an fsg_domain over the packed grid:
+   fsg_set_params, fsg_step (PIC / FLIP / PICFLIP, FSG_SOLVER_MG, no particle
+   pressure term), fsg_get_stats and the timing calls apply to the whole batch, one
+   kernel launch per phase for all scenes.  Particles are kept in scene-local
+   coordinates.  The joint pressure solve normalises each scene's right-hand side
+   and divides cg_tol by sqrt(n_scenes), so every scene meets |r_s| <= cg_tol |b_s|.
+   A scene with no LIQUID cell still swaps with the others (the reference's
+   no-fluid early return is per packed grid). */
+int fsg_batch_create(fsg_domain **out, int32_t n_scenes, const int32_t *nx, const int32_t *ny,
+                     float delta, float density, float pic_ratio);
+int fsg_batch_num_scenes(const fsg_domain *h, int32_t *n);
+/* rect4 = first cell (x, y) in the packed grid, nx, ny */
+int fsg_batch_scene_rect(const fsg_domain *h, int32_t scene, int32_t *rect4);
+/* all scenes at once: counts[s] particles of scene s, concatenated in scene order */
+int fsg_batch_set_particles(fsg_domain *h, const int64_t *counts, const float *xyuv);
+/* interior SOLID cells of one scene (ny x nx, row-major) */
+int fsg_batch_set_solid_mask(fsg_domain *h, int32_t scene, const uint8_t *mask);
+int fsg_batch_num_particles(const fsg_domain *h, int32_t scene, int64_t *n);
+int fsg_batch_get_particles(fsg_domain *h, int32_t scene, float *xyuv, int64_t capacity);
+int fsg_batch_get_velocity(fsg_domain *h, int32_t scene, int32_t which, float *u, float *v);
+int fsg_batch_get_labels(fsg_domain *h, int32_t scene, uint8_t *labels);
+
 /* ---- timing --------------------------------------------------------------- */
 /* Per-kernel HIP-event timing on the handle's stream: 0 off (default), 1 every
    kernel, 2 only the kernels whose timing name starts with the prefix set by

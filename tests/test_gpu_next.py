@@ -44,6 +44,24 @@ def test_emit_lattice_appends_in_emitter_order():
     assert np.array_equal(p[1 + n1:, 0], np.tile(xs[:3], 2))
 
 
+def test_emit_lattice_after_growth():
+    """Emission, then an emission that grows the particle array, then another: the
+    coordinate staging buffer must survive the growth."""
+    d = fsg.Domain(64, 64, 64.0, 64.0, 0.01, 0.05)
+    xs = _lattice_axis(2.0, 10.0, 1.0)
+    ys = _lattice_axis(2.0, 10.0, 1.0)
+    d.emit_lattice(xs, ys)
+    big_x = _lattice_axis(2.0, 60.0, 1.0)
+    big_y = _lattice_axis(2.0, 60.0, 1.0)
+    d.emit_lattice(big_x, big_y)          # > 1024 particles: the array grows
+    d.emit_lattice(xs, ys, 1.0, 2.0)
+    p = d.particles()
+    n0, n1 = len(xs) * len(ys), len(big_x) * len(big_y)
+    assert len(p) == 2 * n0 + n1
+    assert np.array_equal(p[n0 + n1:, 0], np.tile(xs, len(ys)))
+    assert (p[n0 + n1:, 2] == 1.0).all()
+
+
 @pytest.mark.parametrize("nx,ny", [(64, 64), (48, 80), (80, 48)])
 def test_advect_level_set_bit_exact(nx, ny):
     from oracle import OracleSim
