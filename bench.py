@@ -18,6 +18,12 @@ timed region; the dominant kernel is picked from a per-kernel profile of a few
 untimed steps before it), the pressure solver's aggregate bandwidth, the
 single-step error of the chosen form against the FP64 oracle, and the FP32 CPU
 oracle on one core.
+
+Two more workloads are reported under "config" (not in "value"):
+- "batch": 256 non-square dam breaks of sides 64..512 with 1-4 obstacles each, stepped
+  together as one packed batch, beside the same scenes stepped one Domain at a time
+  and the CPU oracle on the first 16 scenes;
+- "grid_only": semi-Lagrangian advection + projection on the fully fluid 2048^2 grid.
 """
 from __future__ import annotations
 
@@ -46,11 +52,13 @@ def mg_levels(nx, ny, coarse_cells=1024):
     return [a * b for a, b in lv], lc
 
 
-def kernel_bytes(name, nx, ny, P):
+def kernel_bytes(name, nx, ny, P, levels=None):
     """Algorithmic HBM bytes of one launch of a kernel, each array read or written
-    once (dense cell grids): nx * ny cells, P particles."""
+    once (dense cell grids): nx * ny cells, P particles.  `levels` = (cells per level,
+    first one-workgroup level) overrides the single-grid hierarchy (a batch's packed
+    levels, whose last one is cycled scene by scene)."""
     C = nx * ny
-    cells, lc = mg_levels(nx, ny)
+    cells, lc = levels or mg_levels(nx, ny)
     table = {
         # particles 16 B read + label 1 B; float4 accumulators read and written once
         "particle_scatter": 17.0 * P + 32.0 * C,
@@ -100,25 +108,50 @@ def make_domain(fsg, scene, solver, tol, max_iter):
                                  refine_inner_tol=1e-3, refine_max_outer=20, check_every=64)
 
 
-def run_steps(fsg, d, n, dt):
+def run_steps(fsg, d, n, dt, scheme=None):
+    scheme = fsg.PICFLIP if scheme is None else scheme
     for _ in range(n):
-        d.step(fsg.PICFLIP, dt)
+        d.step(scheme, dt)
 
 
-def kernel_profile(fsg, d, steps, dt):
+def solver_gbs(prof, nx, ny, P, levels=None):
+    """The pressure solver's algorithmic bytes over its kernel time (GB/s)."""
+    tot_b = tot_ms = 0.0
+    for k, (ms, n) in prof.items():
+        b = kernel_bytes(k, nx, ny, P, levels) if k.startswith("cg_") else None
+        if b is not None:
+            tot_b += b * n
+            tot_ms += ms * n
+    return tot_b / (tot_ms * 1e-3) / 1e9 if tot_ms > 0 else None
+
+
+def batch_levels(sizes, W, H, coarse_cells=4096):
+    """A batch's packed multigrid levels: halved up to the first level `la` at which
+    every scene's rectangle has at most `coarse_cells` cells (>= 1)."""
+    la = 1
+    while any(((a + (1 << la) - 1) >> la) * ((b + (1 << la) - 1) >> la) > coarse_cells
+              for a, b in sizes):
+        la += 1
+    lv = [(W, H)]
+    for _ in range(la):
+        lv.append(((lv[-1][0] + 1) // 2, (lv[-1][1] + 1) // 2))
+    return [a * b for a, b in lv], la
+
+
+def kernel_profile(fsg, d, steps, dt, scheme=None):
     """Per-kernel HIP-event times over `steps` steps (continuing the trajectory):
     {name: (ms per launch, launches per step)}."""
     d.synchronize()
     d.reset_timing()
     d.set_timing(1)
-    run_steps(fsg, d, steps, dt)
+    run_steps(fsg, d, steps, dt, scheme)
     d.synchronize()
     tm = d.timing()
     d.set_timing(0)
     return {k: (ms / max(n, 1), n / steps) for k, (ms, n) in tm.items()}
 
 
-def timed_segment(fsg, d, steps, dt, kernel=None):
+def timed_segment(fsg, d, steps, dt, kernel=None, scheme=None):
     """K steps bracketed by synchronisations; with `kernel`, that kernel (and only
     it) is timed with HIP events on the solver's stream inside the region."""
     d.synchronize()
@@ -129,7 +162,7 @@ def timed_segment(fsg, d, steps, dt, kernel=None):
         d.set_timing(2)
     d.synchronize()
     t0 = time.perf_counter()
-    run_steps(fsg, d, steps, dt)
+    run_steps(fsg, d, steps, dt, scheme)
     d.synchronize()
     t1 = time.perf_counter()
     st = d.stats()
@@ -201,8 +234,10 @@ def batch_leg(fsg, scenes, args):
     prof = kernel_profile(fsg, b, 2, scs[0].dt)
     per_step = {k: ms * n for k, (ms, n) in prof.items()}
     elapsed, st, _ = timed_segment(fsg, b, args.batch_steps, scs[0].dt)
+    W, H = b.nx, b.ny
     b.close()
     sps = args.batch_steps / elapsed
+    gbs = solver_gbs(prof, W, H, parts, batch_levels([(sc.nx, sc.ny) for sc in scs], W, H))
     log(f"batch: {sps:.3f} steps/s = {sps * len(scs):.1f} scene-steps/s "
         f"({st.cg_iterations} MG iterations in the last solve)")
     seq = None
@@ -229,6 +264,25 @@ def batch_leg(fsg, scenes, args):
         for d in doms:
             d.close()
         log(f"one scene at a time ({k} scenes): {seq['scene_steps_per_s']:.1f} scene-steps/s")
+    cpu = None
+    k = min(args.batch_cpu_scenes, len(scs))
+    if k > 0 and not args.no_cpu:
+        from oracle import OracleSim
+        from oracle.binding import SCHEME_PICFLIP
+        t_cpu = 0.0
+        for sc in scs[:k]:
+            o = OracleSim.from_scene(sc, "f32")
+            o.set_params(cg_tol=1e-6, cg_max_iter=10000)
+            t0 = time.perf_counter()
+            o.step(SCHEME_PICFLIP, sc.dt)
+            t_cpu += time.perf_counter() - t0
+        cells_k = sum(sc.nx * sc.ny for sc in scs[:k])
+        cpu = {"value": k / t_cpu, "unit": "scene-steps/s", "cores": 1, "kind": "port",
+               "cell_steps_per_s": cells_k / t_cpu,
+               "batch_steps_per_s_extrapolated": cells_k / t_cpu / cells,
+               "sample": f"first step of the first {k} scenes, FP32 oracle, Jacobi-PCG rel "
+                         f"tol 1e-6, 1 thread; the batch rate is extrapolated by cells"}
+        log(f"batch cpu oracle: {k / t_cpu:.3f} scene-steps/s over {k} scenes ({t_cpu:.1f} s)")
     return {
         "workload": f"dam_break_batch: {len(scs)} non-square scenes, sides {args.batch_lo}.."
                     f"{args.batch_hi}, 1-4 obstacles each, {cells} cells, {parts} particles, "
@@ -239,8 +293,39 @@ def batch_leg(fsg, scenes, args):
         "scene_steps_per_s": sps * len(scs),
         "cell_steps_per_s": sps * cells,
         "cg_iterations_last_solve": st.cg_iterations,
+        "cg_hbm_gbs": gbs,
         "one_scene_at_a_time": seq,
+        "cpu_baseline": cpu,
         "speedup_vs_one_at_a_time": (sps * cells / seq["cell_steps_per_s"]) if seq else None,
+        "kernel_ms_per_step": {k: round(v, 4) for k, v in per_step.items()},
+    }
+
+
+def grid_leg(fsg, scenes, args):
+    """Grid-only semi-Lagrangian advection + projection on the fully fluid
+    ``--grid-n``^2 grid (divergence-free 64-mode Fourier velocity, RK2 gather
+    backtrace, dt = 0.5 / max|u|, mean-removed right-hand side, closed walls: the
+    reference's one-sided wall rule makes this closed box inject fluid through its
+    walls and overflow within a few steps)."""
+    sc = scenes.grid_only(args.grid_n)
+    d = fsg.Domain.from_scene(sc, cg_tol=args.tol, cg_max_iter=200, solver=fsg.SOLVER_MG,
+                              integrator=2, project_mean=1, closed_walls=1)
+    run_steps(fsg, d, 2, sc.dt, fsg.GRID_ADVECT)
+    prof = kernel_profile(fsg, d, 2, sc.dt, fsg.GRID_ADVECT)
+    per_step = {k: ms * n for k, (ms, n) in prof.items()}
+    elapsed, st, _ = timed_segment(fsg, d, args.grid_steps, sc.dt, scheme=fsg.GRID_ADVECT)
+    d.close()
+    sps = args.grid_steps / elapsed
+    log(f"grid-only {args.grid_n}: {sps:.2f} steps/s ({st.cg_iterations} MG iterations)")
+    return {
+        "workload": f"grid_only_{args.grid_n}: gather-RK2 semi-Lagrangian advection + "
+                    f"projection, fully fluid {args.grid_n}^2, closed walls, "
+                    f"dt {sc.dt:.4g}, rel tol {args.tol:g}",
+        "steps": args.grid_steps,
+        "steps_per_s": sps,
+        "ms_per_step": 1e3 / sps,
+        "cg_iterations_last_solve": st.cg_iterations,
+        "cg_hbm_gbs": solver_gbs(prof, args.grid_n, args.grid_n, 0),
         "kernel_ms_per_step": {k: round(v, 4) for k, v in per_step.items()},
     }
 
@@ -264,6 +349,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--batch-warmup", type=int, default=3)
     ap.add_argument("--batch-seq-scenes", type=int, default=32)
+    ap.add_argument("--batch-cpu-scenes", type=int, default=16)
+    ap.add_argument("--grid-n", type=int, default=2048)
+    ap.add_argument("--grid-steps", type=int, default=20)
     ap.add_argument("--no-fp64-check", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -321,13 +409,7 @@ def main():
                 "kernel": dominant, "kernel_us_per_launch": 1e3 * k_ms,
                 "algorithmic_bytes_per_launch": k_bytes}
     # "CG HBM GB/s": algorithmic bytes of every pressure-solver kernel over their time
-    cg_bytes = cg_ms = 0.0
-    for k, (ms, n) in prof.items():
-        b = kernel_bytes(k, args.n, args.n, P) if k.startswith("cg_") else None
-        if b is not None:
-            cg_bytes += b * n
-            cg_ms += ms * n
-    cg_gbs = cg_bytes / (cg_ms * 1e-3) / 1e9 if cg_ms > 0 else None
+    cg_gbs = solver_gbs(prof, args.n, args.n, P)
 
     # ---- plain FP32 Jacobi-PCG form over the same trajectory segment
     plain_sps = None
@@ -353,8 +435,9 @@ def main():
         log(f"fp64 check at step {args.warmup + args.profile_steps + args.steps} "
             f"({time.perf_counter() - t0:.1f} s): {errs} labels equal {same_labels}")
 
-    # ---- packed batch of scenes
+    # ---- packed batch of scenes; grid-only advection + projection
     batch = batch_leg(fsg, scenes, args) if args.batch_scenes > 0 else None
+    grid = grid_leg(fsg, scenes, args) if args.grid_n > 0 and args.grid_steps > 0 else None
 
     # ---- CPU baseline: the FP32 oracle, one thread, bounded sample
     cpu = None
@@ -395,6 +478,7 @@ def main():
             "labels_equal_fp64": same_labels,
             "kernel_ms_per_step": {k: round(v, 4) for k, v in per_step.items()},
             "batch": batch,
+            "grid_only": grid,
         },
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -271,6 +271,7 @@ void classify_and_splat(fsg_domain* h) {
 }
 
 void face_update(fsg_domain* h, int flags, float dt, bool record_info) {
+  if ((flags & FU_DIRICHLET) && h->prm.closed_walls) flags |= FU_CLOSED;
   const float gx_dt = h->prm.gravity_x * dt;
   const float gy_dt = h->prm.gravity_y * dt;
   timed(h, "face_update", [&] {
@@ -587,7 +588,8 @@ int ph_pressure(fsg_domain* h, float dt, bool dirichlet, bool diff) {
     launch_pressure_update(h->stream, h->g, h->n_fluid, h->label, h->xg,
                            h->mg_solved ? h->mg.x[0] : nullptr, counts, ppk, UF(h), VF(h), UB(h),
                            VB(h),
-                           dt_over_density, dirichlet ? 1 : 0, diff ? 1 : 0, h->u_prev, h->v_prev,
+                           dt_over_density, dirichlet ? (h->prm.closed_walls ? 2 : 1) : 0,
+                           diff ? 1 : 0, h->u_prev, h->v_prev,
                            h->u_diff, h->v_diff);
   });
   swap_vel(h);
@@ -780,7 +782,7 @@ int fsg_set_params(fsg_domain* h, const fsg_params* p) {
   if (!h || !p) return FSG_E_INVALID;
   if (p->cg_max_iter < 0 || p->ext_layers < 0 || p->ext_layers > 14 || p->integrator < 0 ||
       p->integrator > 3 || p->solver < 0 || p->solver > 3 || !(p->cg_tol >= 0.0) ||
-      !std::isfinite(p->particle_pressure_k))
+      !std::isfinite(p->particle_pressure_k) || p->closed_walls < 0 || p->closed_walls > 1)
     return fail(h, FSG_E_INVALID, "fsg_set_params: value out of range");
   if ((p->solver == FSG_SOLVER_PERSISTENT || p->solver == FSG_SOLVER_REFINED) &&
       h->max_coop_blocks <= 0)

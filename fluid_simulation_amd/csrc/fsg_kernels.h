@@ -26,7 +26,8 @@ struct BatchTables {
 };
 
 // k_face_update phase flags (applied in this order)
-enum { FU_P2G = 1, FU_PREV = 2, FU_GRAVITY = 4, FU_DIRICHLET = 8, FU_EXTINIT = 16 };
+enum { FU_P2G = 1, FU_PREV = 2, FU_GRAVITY = 4, FU_DIRICHLET = 8, FU_EXTINIT = 16,
+       FU_CLOSED = 32 };
 
 void launch_cell_init(hipStream_t s, const Geom& g, uint8_t* label, const uint8_t* solid,
                       float4* acc, int do_labels, int do_acc);

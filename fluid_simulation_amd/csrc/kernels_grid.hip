@@ -211,7 +211,8 @@ __global__ void __launch_bounds__(256) k_particle_scatter(Geom g0, const float4*
 //   FU_P2G      back := sum / weight where weight > 1e-6 (src/FluidSolver.cpp:898-917)
 //   FU_PREV     prev := value (src/MacGrid.cpp:52-56)
 //   FU_GRAVITY  value += a*dt at LIQUID cells (src/FluidSolver.cpp:286-291)
-//   FU_DIRICHLET zero flow into SOLID (src/FluidSolver.cpp:300-320)
+//   FU_DIRICHLET zero flow into SOLID (src/FluidSolver.cpp:300-320); with FU_CLOSED
+//               zero every face next to SOLID (fsg_params.closed_walls)
 //   FU_EXTINIT  extension init (src/FluidSolver.cpp:490-530): ext := value on valid
 //               faces else 0; masks; value.u := 0 where u- OR v-invalid (the v
 //               branch writes the x buffer, :527)
@@ -246,8 +247,13 @@ __global__ void __launch_bounds__(256) k_face_update(
     vv = vv + gy_dt;
   }
   if (flags & FU_DIRICHLET) {
-    if ((Lim == kSolid && uu < 0.f) || (L == kSolid && uu > 0.f)) uu = 0.f;
-    if ((Ljm == kSolid && vv < 0.f) || (L == kSolid && vv > 0.f)) vv = 0.f;
+    if (flags & FU_CLOSED) {
+      if (Lim == kSolid || L == kSolid) uu = 0.f;
+      if (Ljm == kSolid || L == kSolid) vv = 0.f;
+    } else {
+      if ((Lim == kSolid && uu < 0.f) || (L == kSolid && uu > 0.f)) uu = 0.f;
+      if ((Ljm == kSolid && vv < 0.f) || (L == kSolid && vv > 0.f)) vv = 0.f;
+    }
   }
   if (flags & FU_EXTINIT) {
     const bool vu = (L == kLiquid || Lim == kLiquid);
@@ -335,7 +341,8 @@ __global__ void __launch_bounds__(256) k_classify_levelset(Geom g, const float* 
 // 58-70).  A = front (extended) velocity, B = back; the host swaps afterwards.
 //  F > 0: B := A - dt/density * (p - p_left) / dx at faces where (i,j), (i-1,j)
 //         or (i,j-1) is LIQUID, p = x for LIQUID cells and 0 otherwise; other
-//         faces keep B.  Then Dirichlet on B (the front after the swap).
+//         faces keep B.  Then Dirichlet on B (the front after the swap):
+//         do_dirichlet 1 the reference's one-sided rule, 2 closed walls.
 //  F = 0: the reference returns before solving and does not swap (:347-350); the
 //         same state is produced by exchanging A and B here (B := Dirichlet(A),
 //         A := B) ahead of the host's swap.
@@ -395,7 +402,10 @@ __global__ void __launch_bounds__(256) k_pressure_update(
     av[c] = bv[c];
   }
   if (L != kLiquid) xg[c] = 0.f;
-  if (do_dirichlet) {
+  if (do_dirichlet == 2) {  // closed walls: no flow through any face next to SOLID
+    if (Lim == kSolid || L == kSolid) nu = 0.f;
+    if (Ljm == kSolid || L == kSolid) nv = 0.f;
+  } else if (do_dirichlet) {
     if ((Lim == kSolid && nu < 0.f) || (L == kSolid && nu > 0.f)) nu = 0.f;
     if ((Ljm == kSolid && nv < 0.f) || (L == kSolid && nv > 0.f)) nv = 0.f;
   }

@@ -116,6 +116,13 @@ typedef struct fsg_params {
                              Eigen's solve() */
   float particle_pressure_k; /* 0 (src/FluidSolver.cpp:443): k of the particle-count
                                 pressure term k * (n > 9 ? n - 8 : 0) */
+  int32_t closed_walls;   /* 0 (the reference): enforceDirichlet zeroes only flow into
+                             SOLID (src/FluidSolver.cpp:306-319); 1: every face next to
+                             a SOLID cell is zeroed, both directions (no-penetration).
+                             With the mean of b removed (closed all-fluid domains) the
+                             one-sided rule lets the update, which takes SOLID
+                             neighbours as p = 0, inject fluid through the walls, and
+                             the flow grows without bound; 1 keeps such runs bounded. */
 } fsg_params;
 
 /* Counters of the last pressure solve and running totals. */

@@ -48,6 +48,7 @@ def _state_type(R):
             ("fluid_index", C.POINTER(C.c_int32)), ("pressure", P), ("rhs", P),
             ("n_fluid", C.c_int32), ("cg_iterations", C.c_int32),
             ("cg_error", C.c_double), ("particle_pressure_k", R),
+            ("closed_walls", C.c_int32),
         ]
     return State
 

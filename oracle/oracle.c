@@ -235,7 +235,8 @@ static void gravity(STATE *s, REAL dt) {
       }
 }
 
-/* FluidSolver::enforceDirichlet, src/FluidSolver.cpp:297-321. */
+/* FluidSolver::enforceDirichlet, src/FluidSolver.cpp:297-321.  closed_walls (not in
+ * the reference): every face with a SOLID cell on either side is zeroed. */
 static void dirichlet(STATE *s) {
   REAL *u = U_F(s), *v = V_F(s);
   for (int j = 0; j < s->ny; ++j)
@@ -243,6 +244,11 @@ static void dirichlet(STATE *s) {
       int im = clampi(i - 1, 0, s->nx - 1);
       int jm = clampi(j - 1, 0, s->ny - 1);
       size_t c = IDX(s, i, j);
+      if (s->closed_walls) {
+        if (cell(s, im, j) == ORC_SOLID || cell(s, i, j) == ORC_SOLID) u[c] = 0;
+        if (cell(s, i, jm) == ORC_SOLID || cell(s, i, j) == ORC_SOLID) v[c] = 0;
+        continue;
+      }
       if ((cell(s, im, j) == ORC_SOLID && u[c] < 0) ||
           (cell(s, i, j) == ORC_SOLID && u[c] > 0))
         u[c] = 0;

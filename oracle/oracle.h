@@ -69,6 +69,7 @@ enum {
     int32_t n_fluid, cg_iterations;                                                 \
     double cg_error;                                                                \
     R particle_pressure_k;    /* 0 (FluidSolver.cpp:443) */                         \
+    int32_t closed_walls;     /* 1: Dirichlet zeroes every face next to SOLID */    \
   } orc_state##SFX;                                                                 \
   void orc_defaults##SFX(orc_state##SFX *s, int32_t nx, int32_t ny, R length_x,     \
                          R length_y, R density, R pic_ratio);                       \

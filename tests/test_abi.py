@@ -26,9 +26,11 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_params_struct_layout_and_defaults():
-    # layout: 2 floats, double, 6 int32, double, 2 int32, 3 int32, 1 float
-    assert ctypes.sizeof(fsg.Params) == 4 * 2 + 8 + 4 * 6 + 8 + 4 * 2 + 4 * 4
-    assert fsg.Params.particle_pressure_k.offset == ctypes.sizeof(fsg.Params) - 4
+    # layout: 2 floats, double, 6 int32, double, 2 int32, 3 int32, 1 float, 1 int32
+    # (+ 4 bytes of tail padding to the double alignment)
+    assert ctypes.sizeof(fsg.Params) == 4 * 2 + 8 + 4 * 6 + 8 + 4 * 2 + 4 * 4 + 4 + 4
+    assert fsg.Params.particle_pressure_k.offset == 68
+    assert fsg.Params.closed_walls.offset == 72
     p = fsg.default_params()
     assert p.gravity_y == ctypes.c_float(-9.82).value      # src/FluidSolver.cpp:232
     assert p.cg_max_iter == 100                              # src/FluidSolver.cpp:81
@@ -36,6 +38,7 @@ def test_params_struct_layout_and_defaults():
     assert abs(p.cg_tol - 2.0 ** -23) == 0                   # Eigen default for float
     assert p.integrator == fsg.RK3                           # include/FluidSolver.h:144
     assert p.particle_pressure_k == 0.0                      # src/FluidSolver.cpp:443
+    assert p.closed_walls == 0                               # src/FluidSolver.cpp:306-319
 
 
 def test_version_string():

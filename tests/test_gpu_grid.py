@@ -22,9 +22,10 @@ pytestmark = pytest.mark.gpu
 RK2 = 2
 
 
-def _domain(sc, solver, tol):
+def _domain(sc, solver, tol, closed_walls=0):
     d = fsg.Domain.from_scene(sc, cg_tol=tol, cg_max_iter=20000 if solver == fsg.SOLVER_PLAIN
-                              else 200, solver=solver, integrator=RK2, project_mean=1)
+                              else 200, solver=solver, integrator=RK2, project_mean=1,
+                              closed_walls=closed_walls)
     return d
 
 
@@ -102,3 +103,63 @@ def test_grid_2048_advection_bit_exact_and_projection():
         assert st.cg_iterations <= 30
         # residual 1e-8 in FP64; the FP32 velocities add rounding of ~1e-7 |u| per face
         assert nd <= 1e-4 * nb
+
+
+# The reference's one-sided Dirichlet rule keeps inflow through walls, and the update
+# takes SOLID neighbours as p = 0: in a closed all-fluid box (mean of b removed) the
+# walls then inject fluid every step and mean(b) grows ~250x per step (overflow by
+# step 5).  closed_walls = 1 zeroes every face next to SOLID: the textbook
+# no-penetration condition under which the Neumann system is consistent.
+def test_grid_closed_walls_against_fp64():
+    """Closed-wall grid-only steps at 128^2, each from the GPU's own state, against
+    the FP64 oracle stepping the same input (single-step errors)."""
+    from oracle import OracleSim
+    from oracle.binding import SCHEME_GRID
+    sc = scenes.grid_only(128)
+    o = OracleSim.from_scene(sc, "f64")
+    o.set_params(cg_tol=1e-13, cg_max_iter=200000, integrator=RK2, project_mean=1,
+                 closed_walls=1)
+    g = _domain(sc, fsg.SOLVER_MG, 1e-10, closed_walls=1)
+    print("grid-only 128 closed walls", flush=True)
+    for step in range(3):
+        for which, name in ((fsg.VEL_FRONT, "front"), (fsg.VEL_BACK, "back")):
+            o.set_vel(*g.velocity(which), name)
+        o.step(SCHEME_GRID, sc.dt)
+        g.step(fsg.GRID_ADVECT, sc.dt)
+        u, v = g.velocity(fsg.VEL_FRONT)
+        uo, vo = o.vel("front")
+        eu, ev = rel_err(u, uo), rel_err(v, vo)
+        print(f"step {step}: iters {g.stats().cg_iterations} u {eu:.2e} v {ev:.2e}")
+        assert max(eu, ev) <= 1e-5
+        assert not u[:, 0].any() and not u[:, 1].any() and not v[0].any() and not v[1].any()
+
+
+def test_grid_2048_closed_walls_bounded():
+    """20 closed-wall steps at 2048^2 (tol 1e-6): the flux through the walls is zero, so
+    mean(b) stays at rounding level, the speed stays bounded, the projected
+    divergence is the solver's residual and every solve converges in <= 30
+    iterations."""
+    n = 2048
+    sc = scenes.grid_only(n)
+    g = _domain(sc, fsg.SOLVER_MG, 1e-6, closed_walls=1)
+    u0, v0 = g.velocity(fsg.VEL_FRONT)
+    vmax0 = max(float(np.abs(u0).max()), float(np.abs(v0).max()))
+    print(f"grid-only {n} closed walls: 20 steps", flush=True)
+    for step in range(20):
+        g.step(fsg.GRID_ADVECT, sc.dt)
+        st = g.stats()
+        assert st.cg_iterations <= 30 and st.cg_error <= 1e-6
+        if step % 5 == 4:
+            b = g.rhs().astype(np.float64)
+            u, v = g.velocity(fsg.VEL_FRONT)
+            nb = float(np.sqrt((b ** 2).sum()))
+            mean_b = float(b[1:-1, 1:-1].mean())
+            div = _divergence(u, v)[1:-1, 1:-1]
+            nd = float(np.sqrt((div ** 2).sum()))
+            vmax = max(float(np.abs(u).max()), float(np.abs(v).max()))
+            print(f"step {step}: iters {st.cg_iterations} |r|/|b| {st.cg_error:.2e} "
+                  f"mean(b) {mean_b:.2e} |div| / |b| {nd / nb:.2e} max|u| {vmax:.3f}")
+            assert abs(mean_b) * (n - 2) ** 2 <= 1e-3 * nb * (n - 2)
+            assert nd <= 1e-4 * nb
+            assert vmax <= 1.5 * vmax0
+

@@ -161,6 +161,29 @@ def test_gravity_and_dirichlet_truth_table():
     np.testing.assert_array_equal(vf, np.array(case["v_dirichlet"], dtype=np.float32))
 
 
+def test_closed_walls_dirichlet_zeroes_every_solid_face():
+    """closed_walls = 1: a face is zeroed iff a SOLID cell lies on either side of it
+    (u: cells i-1, i; v: cells j-1, j; clamped at the grid edge); other faces keep
+    their value.  Random labels and velocities, checked face by face."""
+    rng = np.random.default_rng(4)
+    n = 9
+    lab = rng.choice([LIQUID, AIR, SOLID], size=(n, n)).astype(np.uint8)
+    u = rng.standard_normal((n, n)).astype(np.float32)
+    v = rng.standard_normal((n, n)).astype(np.float32)
+    s = OracleSim(n, n, float(n), float(n), 0.01, 0.05, "f32")
+    s.set_params(closed_walls=1)
+    s.label[...] = lab
+    s.set_vel(u, v, "front")
+    s.run_phase("DIRICHLET")
+    uf, vf = s.vel("front")
+    for j in range(n):
+        for i in range(n):
+            su = lab[j, max(i - 1, 0)] == SOLID or lab[j, i] == SOLID
+            sv = lab[max(j - 1, 0), i] == SOLID or lab[j, i] == SOLID
+            assert uf[j, i] == (0.0 if su else u[j, i])
+            assert vf[j, i] == (0.0 if sv else v[j, i])
+
+
 # ---------------------------------------------------------------- extension
 def test_extension_hand_computed():
     case = KAT["extension"]
