@@ -23,7 +23,9 @@ Two more workloads are reported under "config" (not in "value"):
 - "batch": 256 non-square dam breaks of sides 64..512 with 1-4 obstacles each, stepped
   together as one packed batch, beside the same scenes stepped one Domain at a time
   and the CPU oracle on the first 16 scenes;
-- "grid_only": semi-Lagrangian advection + projection on the fully fluid 2048^2 grid.
+- "grid_only": semi-Lagrangian advection + projection on the fully fluid 2048^2 grid;
+- "adjoint": the reverse-mode gradient of a 64-step 512^2 rollout (closed all-fluid
+  box, density loss) through the adjoint kernels, timed, beside FP32 differences.
 """
 from __future__ import annotations
 
@@ -330,6 +332,102 @@ def grid_leg(fsg, scenes, args):
     }
 
 
+def adjoint_leg(fsg, scenes, args):
+    """The differentiable form: a closed all-fluid ``--adj-n``^2 box (4 particles per
+    interior cell), initial velocities from a (2, 16, 16) control grid bilinearly
+    upsampled, ``--adj-steps`` PICFLIP steps with checkpoints every ``--adj-every``,
+    loss = sum((particle density - target)^2) with the target from control seed 1 and
+    the evaluation at seed 0.  Times the checkpointed forward and the reverse pass,
+    and compares the gradient with central FP32 differences (each ``--adj-eps``) along
+    ``--adj-dirs`` seeded unit directions, counting the label changes between each
+    pair of runs (a discrete decision the differences cross), beside the loss's
+    run-to-run spread at fixed controls (the differences' noise floor)."""
+    import torch
+    from fluid_simulation_amd import adjoint
+    n, steps, every, dt = args.adj_n, args.adj_steps, args.adj_every, 0.01
+    jj, ii = np.meshgrid(np.arange(1, n - 1), np.arange(1, n - 1), indexing="ij")
+    parts = scenes.jittered_particles(ii.ravel(), jj.ravel(), n, 1.0, 1.0, 0)
+    model = adjoint.ControlModel(n, n, float(n), float(n), parts[:, :2])
+    d = fsg.Domain(n, n, float(n), float(n), dt, 0.05)
+    d.set_params(solver=fsg.SOLVER_MG, cg_tol=args.tol, cg_max_iter=200, project_mean=1,
+                 closed_walls=1)
+    pos = torch.tensor(parts[:, :2].astype(np.float32))
+
+    def run(c, record=False):
+        vel, u, v = model(c)
+        adjoint.set_state(d, np.concatenate([parts[:, :2], vel.detach().numpy()], 1),
+                          u.detach().numpy(), v.detach().numpy())
+        labels = []
+        for _ in range(steps):
+            d.step(fsg.PICFLIP, dt)
+            if record:
+                labels.append(d.labels())
+        return d.density().astype(np.float64), labels
+
+    scale = args.adj_scale
+    target, _ = run(torch.randn(2, 16, 16, generator=torch.Generator().manual_seed(1)) * scale)
+    c = (torch.randn(2, 16, 16, generator=torch.Generator().manual_seed(0)) * scale)
+    c.requires_grad_(True)
+    d.synchronize()
+    t0 = time.perf_counter()
+    vel, u, v = model(c)
+    loss = adjoint.Rollout.apply(pos, vel, u, v, d, fsg.PICFLIP, dt, steps, every, target)
+    d.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    d.synchronize()
+    t2 = time.perf_counter()
+    grad = c.grad.detach().numpy().astype(np.float64)
+    log(f"adjoint {n}^2, {len(parts)} particles, {steps} steps: forward {1e3 * (t1 - t0):.1f} ms, "
+        f"reverse {1e3 * (t2 - t1):.1f} ms, loss {loss.item():.6e}, |grad| {np.linalg.norm(grad):.3e}")
+    # run-to-run spread of the loss at the same controls (float atomics in P2G)
+    with torch.no_grad():
+        l_a = float((((run(c)[0]) - target) ** 2).sum())
+        l_b = float((((run(c)[0]) - target) ** 2).sum())
+    noise = abs(l_a - l_b)
+    fd_out = {}
+    for eps in args.adj_eps:
+        rng = np.random.default_rng(16)
+        rows = []
+        for k in range(args.adj_dirs):
+            dirn = rng.standard_normal((2, 16, 16))
+            dirn /= np.linalg.norm(dirn)
+            with torch.no_grad():
+                rp, lp = run(c + eps * torch.tensor(dirn, dtype=torch.float32), record=True)
+                rm, lm = run(c - eps * torch.tensor(dirn, dtype=torch.float32), record=True)
+            flips = int(sum((a != b).sum() for a, b in zip(lp, lm)))
+            fd = (((rp - target) ** 2).sum() - ((rm - target) ** 2).sum()) / (2 * eps)
+            adj = float((grad * dirn).sum())
+            rel = abs(adj - fd) / max(abs(fd), abs(adj), 1e-30)
+            rows.append((adj, fd, rel, flips))
+        rels = [r[2] for r in rows]
+        clean = [r[2] for r in rows if r[3] == 0]
+        log(f"adjoint vs FP32 differences, eps {eps:g}, {len(rows)} directions: median rel "
+            f"{np.median(rels):.2e}, max {max(rels):.2e}; {len(clean)} without label changes; "
+            f"loss noise {noise:.2e} vs |dL| ~ {2 * eps * np.median([abs(r[0]) for r in rows]):.2e}")
+        fd_out[f"{eps:g}"] = {
+            "directions": [{"adjoint": a, "fd": f, "rel": r, "label_changes": fl}
+                           for a, f, r, fl in rows],
+            "rel_median": float(np.median(rels)),
+            "rel_max": float(max(rels)),
+            "directions_without_label_changes": len(clean),
+            "rel_max_without_label_changes": float(max(clean)) if clean else None,
+        }
+    d.close()
+    return {
+        "workload": f"closed all-fluid {n}^2 box, {len(parts)} particles, controls (2, 16, 16) "
+                    f"x {scale:g} bilinearly upsampled, {steps} PICFLIP steps dt {dt}, "
+                    f"checkpoints every {every}, loss sum((density - target)^2)",
+        "forward_ms": 1e3 * (t1 - t0),
+        "reverse_ms": 1e3 * (t2 - t1),
+        "reverse_ms_per_step": 1e3 * (t2 - t1) / steps,
+        "loss": loss.item(),
+        "grad_norm": float(np.linalg.norm(grad)),
+        "loss_run_to_run_spread": noise,
+        "fd": fd_out,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -352,6 +450,13 @@ def main():
     ap.add_argument("--batch-cpu-scenes", type=int, default=16)
     ap.add_argument("--grid-n", type=int, default=2048)
     ap.add_argument("--grid-steps", type=int, default=20)
+    ap.add_argument("--adj-n", type=int, default=512)
+    ap.add_argument("--adj-steps", type=int, default=64)
+    ap.add_argument("--adj-every", type=int, default=8)
+    ap.add_argument("--adj-dirs", type=int, default=16)
+    ap.add_argument("--adj-scale", type=float, default=4.0)
+    ap.add_argument("--adj-eps", type=lambda t: [float(x) for x in t.split(",")],
+                    default=[1e-3, 3e-2], help="comma-separated FD steps in control space")
     ap.add_argument("--no-fp64-check", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -438,6 +543,7 @@ def main():
     # ---- packed batch of scenes; grid-only advection + projection
     batch = batch_leg(fsg, scenes, args) if args.batch_scenes > 0 else None
     grid = grid_leg(fsg, scenes, args) if args.grid_n > 0 and args.grid_steps > 0 else None
+    adj = adjoint_leg(fsg, scenes, args) if args.adj_n > 0 and args.adj_steps > 0 else None
 
     # ---- CPU baseline: the FP32 oracle, one thread, bounded sample
     cpu = None
@@ -479,6 +585,7 @@ def main():
             "kernel_ms_per_step": {k: round(v, 4) for k, v in per_step.items()},
             "batch": batch,
             "grid_only": grid,
+            "adjoint": adj,
         },
         "roofline": roof,
         "cpu_baseline": cpu,

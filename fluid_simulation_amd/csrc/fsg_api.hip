@@ -99,6 +99,32 @@ struct BatchInfo {
   }
 };
 
+// Reverse mode (fsg_adjoint_*): the adjoint state carried between steps (particles
+// and the back buffer), the tape of one step, scratch grids; allocated on first use.
+struct AdjState {
+  int64_t pcap = 0;
+  float4* pbar = nullptr;  // per particle: (x bar, y bar, vx bar, vy bar)
+  float4* x0 = nullptr;    // tape: the particles at the step's start
+  float *bu = nullptr, *bv = nullptr;  // adjoint of the back buffer
+  float *su = nullptr, *sv = nullptr, *eu = nullptr, *ev = nullptr, *e2u = nullptr,
+        *e2v = nullptr, *g4u = nullptr, *g4v = nullptr, *g1u = nullptr, *g1v = nullptr,
+        *qu = nullptr, *qv = nullptr;
+  double* y = nullptr;     // adjoint of the pressure, then of the divergence
+  uint8_t* zmask = nullptr;  // Dirichlet decisions of the taped pressure update
+  std::vector<uint8_t*> rmask;  // extension read masks, one per sweep
+  bool grids = false;
+  bool taping = false;
+  int last_iters = 0;
+};
+
+// A checkpoint of the carried state: particles, front and back velocities.
+struct Checkpoint {
+  float4* pt = nullptr;
+  int64_t np = 0, cap = 0;
+  float* u[2] = {nullptr, nullptr};
+  float* v[2] = {nullptr, nullptr};
+};
+
 struct fsg_domain {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -150,6 +176,8 @@ struct fsg_domain {
   float* phi2 = nullptr;    // second level-set buffer (advection), on first use
   float* emit_buf = nullptr;  // emitter coordinates
   size_t emit_cap = 0;
+  AdjState adj;
+  std::vector<Checkpoint> ckpt;
   bool mg_ready = false;
   bool mg_solved = false;  // the last solve was the multigrid form (its FP64 x is valid)
   int mg_last_iters = 0;
@@ -401,14 +429,15 @@ fsg::SceneCoarseArgs scene_coarse_args(const fsg_domain* h) {
 // CG preconditioned by one V-cycle per iteration (FSG_SOLVER_MG).  Iterations are
 // enqueued in batches (the first sized by the previous solve's count); every
 // kernel returns at once after the device-side stop test has fired.
-int mg_solve(fsg_domain* h, int F) {
+int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
   if (int rc = mg_prepare(h)) return rc;
   MgHost& m = h->mg;
   timed(h, "cg_mg_setup", [&] {
     fsg::MgBatchScale bs;
     if (h->batch) bs = h->batch->scale();
     launch_mg_setup(h->stream, h->g, m, h->label, UF(h), VF(h), h->prm.project_mean,
-                    h->prm.warm_start != 0 && !h->batch, h->sc, h->batch ? &bs : nullptr);
+                    h->prm.warm_start != 0 && !h->batch && !adj_rhs, h->sc,
+                    h->batch ? &bs : nullptr, adj_rhs);
     if (h->batch) launch_mg_scene_setup(h->stream, scene_coarse_args(h));
   });
   const int max_iter = h->prm.cg_max_iter;
@@ -590,7 +619,7 @@ int ph_pressure(fsg_domain* h, float dt, bool dirichlet, bool diff) {
                            VB(h),
                            dt_over_density, dirichlet ? (h->prm.closed_walls ? 2 : 1) : 0,
                            diff ? 1 : 0, h->u_prev, h->v_prev,
-                           h->u_diff, h->v_diff);
+                           h->u_diff, h->v_diff, h->adj.taping ? h->adj.zmask : nullptr);
   });
   swap_vel(h);
   return FSG_OK;
@@ -613,7 +642,7 @@ int step_flip_family(fsg_domain* h, int scheme, float dt) {
   // P2G writes the back buffer, then swaps (src/FluidSolver.cpp:898-918)
   swap_vel(h);
   int flags = FU_P2G | FU_GRAVITY | FU_DIRICHLET | FU_EXTINIT;
-  if (flip) flags |= FU_PREV;
+  if (flip || h->adj.taping) flags |= FU_PREV;  // the adjoint reads the P2G grid from prev
   // the P2G normalise must read the stale values of the buffer it writes: after the
   // swap that buffer is the front, and the extension init writes the new back.
   face_update(h, flags, dt, rec);
@@ -771,6 +800,10 @@ int fsg_destroy(fsg_domain* h) {
   if (h->pt) (void)hipFree(h->pt);
   if (h->emit_buf) (void)hipFree(h->emit_buf);
   if (h->reject) (void)hipFree(h->reject);
+  if (h->adj.pbar) (void)hipFree(h->adj.pbar);
+  if (h->adj.x0) (void)hipFree(h->adj.x0);
+  for (Checkpoint& c : h->ckpt)
+    if (c.pt) (void)hipFree(c.pt);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return FSG_OK;
@@ -1534,6 +1567,251 @@ int fsg_batch_get_velocity(fsg_domain* h, int32_t scene, int32_t which, float* u
 int fsg_batch_get_labels(fsg_domain* h, int32_t scene, uint8_t* labels) {
   if (!h || !h->batch || !labels || scene < 0 || scene >= h->batch->S) return FSG_E_INVALID;
   return batch_copy_rect(h, scene, h->label, 1, labels);
+}
+
+// ---------------------------------------------------------------- checkpoints
+int fsg_checkpoint_save(fsg_domain* h, int32_t slot) {
+  if (!h || h->batch || slot < 0 || slot >= (1 << 16)) return FSG_E_INVALID;
+  if ((size_t)slot >= h->ckpt.size()) h->ckpt.resize((size_t)slot + 1);
+  Checkpoint& c = h->ckpt[(size_t)slot];
+  const size_t C = ncell(h);
+  if (!c.u[0]) {
+    HIPCHK(h, dalloc(h, &c.u[0], C));
+    HIPCHK(h, dalloc(h, &c.u[1], C));
+    HIPCHK(h, dalloc(h, &c.v[0], C));
+    HIPCHK(h, dalloc(h, &c.v[1], C));
+  }
+  if (c.cap < h->np) {
+    if (c.pt) (void)hipFree(c.pt);
+    c.pt = nullptr;
+    c.cap = 0;
+    HIPCHK(h, hipMalloc(&c.pt, (size_t)h->np * sizeof(float4)));
+    c.cap = h->np;
+  }
+  c.np = h->np;
+  if (h->np > 0)
+    HIPCHK(h, hipMemcpyAsync(c.pt, h->pt, (size_t)h->np * sizeof(float4),
+                             hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(c.u[0], UF(h), C * 4, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(c.v[0], VF(h), C * 4, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(c.u[1], UB(h), C * 4, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(c.v[1], VB(h), C * 4, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+int fsg_checkpoint_load(fsg_domain* h, int32_t slot) {
+  if (!h || h->batch || slot < 0 || (size_t)slot >= h->ckpt.size() || !h->ckpt[slot].u[0])
+    return FSG_E_INVALID;
+  const Checkpoint& c = h->ckpt[(size_t)slot];
+  const size_t C = ncell(h);
+  if (int rc = reserve_particles(h, c.np)) return rc;
+  h->np = c.np;
+  if (c.np > 0)
+    HIPCHK(h, hipMemcpyAsync(h->pt, c.pt, (size_t)c.np * sizeof(float4),
+                             hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(UF(h), c.u[0], C * 4, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(VF(h), c.v[0], C * 4, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(UB(h), c.u[1], C * 4, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(VB(h), c.v[1], C * 4, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+int fsg_checkpoint_clear(fsg_domain* h) {
+  if (!h) return FSG_E_INVALID;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  // the slots' grids stay with the handle's allocations; particle copies are freed
+  for (Checkpoint& c : h->ckpt) {
+    if (c.pt) (void)hipFree(c.pt);
+    c.pt = nullptr;
+    c.np = c.cap = 0;
+  }
+  return FSG_OK;
+}
+
+// ---------------------------------------------------------------- reverse mode
+static int adj_prepare(fsg_domain* h) {
+  AdjState& a = h->adj;
+  const size_t C = ncell(h);
+  if (!a.grids) {
+    float** grids[] = {&a.bu, &a.bv, &a.su, &a.sv, &a.eu, &a.ev, &a.e2u, &a.e2v,
+                       &a.g4u, &a.g4v, &a.g1u, &a.g1v, &a.qu, &a.qv};
+    for (float** g : grids) HIPCHK(h, dalloc(h, g, C));
+    HIPCHK(h, dalloc(h, &a.y, C));
+    HIPCHK(h, dalloc(h, &a.zmask, C));
+    a.grids = true;
+  }
+  while ((int)a.rmask.size() < std::max(h->prm.ext_layers, 1)) {
+    uint8_t* m = nullptr;
+    HIPCHK(h, dalloc(h, &m, C));
+    a.rmask.push_back(m);
+  }
+  const int64_t need = std::max<int64_t>(h->np, 1);
+  if (a.pcap < need) {
+    int64_t cap = a.pcap > 0 ? a.pcap : 1024;
+    while (cap < need) cap *= 2;
+    float4 *pb = nullptr, *x0 = nullptr;
+    HIPCHK(h, hipMalloc(&pb, (size_t)cap * sizeof(float4)));
+    HIPCHK(h, hipMalloc(&x0, (size_t)cap * sizeof(float4)));
+    HIPCHK(h, hipMemsetAsync(pb, 0, (size_t)cap * sizeof(float4), h->stream));
+    if (a.pcap > 0)
+      HIPCHK(h, hipMemcpyAsync(pb, a.pbar, (size_t)a.pcap * sizeof(float4),
+                               hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (a.pbar) (void)hipFree(a.pbar);
+    if (a.x0) (void)hipFree(a.x0);
+    a.pbar = pb;
+    a.x0 = x0;
+    a.pcap = cap;
+  }
+  return FSG_OK;
+}
+
+int fsg_adjoint_set(fsg_domain* h, const float* pbar_xyuv, const float* bbar_u,
+                    const float* bbar_v) {
+  if (!h || h->batch) return FSG_E_INVALID;
+  if (int rc = adj_prepare(h)) return rc;
+  AdjState& a = h->adj;
+  const size_t C = ncell(h);
+  if (h->np > 0) {
+    if (pbar_xyuv)
+      HIPCHK(h, hipMemcpyAsync(a.pbar, pbar_xyuv, (size_t)h->np * sizeof(float4),
+                               hipMemcpyHostToDevice, h->stream));
+    else
+      launch_fill(h->stream, reinterpret_cast<float*>(a.pbar), 4 * h->np, 0.f);
+  }
+  if (bbar_u)
+    HIPCHK(h, hipMemcpyAsync(a.bu, bbar_u, C * 4, hipMemcpyHostToDevice, h->stream));
+  else
+    launch_fill(h->stream, a.bu, (int64_t)C, 0.f);
+  if (bbar_v)
+    HIPCHK(h, hipMemcpyAsync(a.bv, bbar_v, C * 4, hipMemcpyHostToDevice, h->stream));
+  else
+    launch_fill(h->stream, a.bv, (int64_t)C, 0.f);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+int fsg_adjoint_get(fsg_domain* h, float* pbar_xyuv, float* bbar_u, float* bbar_v) {
+  if (!h || h->batch || !h->adj.grids) return FSG_E_INVALID;
+  const AdjState& a = h->adj;
+  const size_t C = ncell(h);
+  if (pbar_xyuv && h->np > a.pcap)
+    return fail(h, FSG_E_INVALID, "fsg_adjoint_get: more particles than the adjoint state holds");
+  if (pbar_xyuv && h->np > 0)
+    HIPCHK(h, hipMemcpyAsync(pbar_xyuv, a.pbar, (size_t)h->np * sizeof(float4),
+                             hipMemcpyDeviceToHost, h->stream));
+  if (bbar_u) HIPCHK(h, hipMemcpyAsync(bbar_u, a.bu, C * 4, hipMemcpyDeviceToHost, h->stream));
+  if (bbar_v) HIPCHK(h, hipMemcpyAsync(bbar_v, a.bv, C * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+int fsg_particle_density(fsg_domain* h, float* rho) {
+  if (!h || h->batch || !rho) return FSG_E_INVALID;
+  if (int rc = adj_prepare(h)) return rc;
+  const size_t C = ncell(h);
+  launch_fill(h->stream, h->adj.su, (int64_t)C, 0.f);
+  launch_density(h->stream, h->g, h->pt, h->np, h->adj.su);
+  HIPCHK(h, hipMemcpyAsync(rho, h->adj.su, C * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+int fsg_adjoint_seed_density(fsg_domain* h, const float* rho_bar) {
+  if (!h || h->batch || !rho_bar) return FSG_E_INVALID;
+  if (int rc = adj_prepare(h)) return rc;
+  AdjState& a = h->adj;
+  const size_t C = ncell(h);
+  HIPCHK(h, hipMemcpyAsync(a.su, rho_bar, C * 4, hipMemcpyHostToDevice, h->stream));
+  launch_density_adj(h->stream, h->g, h->pt, h->np, a.su, a.pbar);
+  launch_fill(h->stream, a.bu, (int64_t)C, 0.f);
+  launch_fill(h->stream, a.bv, (int64_t)C, 0.f);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+// One step forward from the handle's state, taped, then its adjoint: the adjoint
+// state (particles, back buffer) is carried from the step's end to its start.
+int fsg_adjoint_step(fsg_domain* h, int32_t scheme, float dt) {
+  if (!h) return FSG_E_INVALID;
+  if (h->batch) return fail(h, FSG_E_INVALID, "fsg_adjoint_step: not available for a batch");
+  if (scheme != FSG_SCHEME_PIC && scheme != FSG_SCHEME_FLIP && scheme != FSG_SCHEME_PICFLIP)
+    return fail(h, FSG_E_INVALID, "fsg_adjoint_step: PIC, FLIP or PICFLIP only");
+  if (h->prm.solver != FSG_SOLVER_MG || h->prm.warm_start != 0)
+    return fail(h, FSG_E_INVALID,
+                "fsg_adjoint_step: needs FSG_SOLVER_MG with warm_start = 0 (a replayable solve)");
+  if (int rc = check_validate(h)) return rc;
+  if (int rc = adj_prepare(h)) return rc;
+  AdjState& a = h->adj;
+  const int64_t np = h->np;
+  const size_t C = ncell(h);
+  if (np > 0)
+    HIPCHK(h, hipMemcpyAsync(a.x0, h->pt, (size_t)np * sizeof(float4), hipMemcpyDeviceToDevice,
+                             h->stream));
+  a.taping = true;
+  int rc = step_flip_family(h, scheme, dt);
+  a.taping = false;
+  if (rc) return rc;
+  if (h->last_F == 0)
+    return fail(h, FSG_E_INVALID,
+                "fsg_adjoint_step: no LIQUID cell (the no-fluid step has no adjoint here)");
+  const fsg_stats fwd = h->host_stats;
+  const bool flip = scheme != FSG_SCHEME_PIC;
+  const float r = scheme == FSG_SCHEME_PIC ? 1.0f : scheme == FSG_SCHEME_FLIP ? 0.0f : h->pic_ratio;
+  const float dt_over_density = dt / h->density;
+  // G2P + advect
+  timed(h, "adj_g2p", [&] {
+    launch_fill(h->stream, a.su, (int64_t)C, 0.f);
+    launch_fill(h->stream, a.sv, (int64_t)C, 0.f);
+    launch_adj_g2p(h->stream, h->g, a.x0, np, UF(h), VF(h), h->u_diff, h->v_diff, r, dt,
+                   h->reject, a.pbar, a.su, a.sv);
+  });
+  // pressure update: direct term, pressure adjoint, adjoint solve, divergence adjoint
+  timed(h, "adj_post", [&] {
+    launch_adj_post(h->stream, h->g, h->label, a.zmask, r, flip ? 1 : 0, a.su, a.sv, a.eu, a.ev,
+                    a.g4u, a.g4v, a.g1u, a.g1v);
+    launch_adj_pgrad(h->stream, h->g, h->label, a.eu, a.ev, dt_over_density, a.y);
+  });
+  rc = mg_solve(h, h->last_F, a.y);
+  if (rc) return rc;
+  a.last_iters = h->host_stats.cg_iterations;
+  h->host_stats = fwd;
+  timed(h, "adj_div", [&] {
+    if (h->prm.project_mean) launch_mg_remove_mean(h->stream, h->mg);
+    launch_adj_div(h->stream, h->g, h->mg.inf[0], h->mg.x[0], a.bu, a.bv, a.eu, a.ev);
+  });
+  // extension: replay the read masks, then the sweeps in reverse
+  const int n = h->prm.ext_layers;
+  float *cu = a.eu, *cv = a.ev, *ou = a.e2u, *ov = a.e2v;
+  timed(h, "adj_extend", [&] {
+    launch_adj_mask_init(h->stream, h->g, h->label, a.rmask[0], nullptr);
+    for (int k = 1; k < n; ++k)
+      launch_adj_mask_step(h->stream, h->g, h->label, a.rmask[k - 1],
+                           a.rmask[k >= 2 ? k - 2 : 0], a.rmask[k], nullptr);
+    for (int k = n; k >= 1; --k) {
+      launch_adj_ext_sweep(h->stream, h->g, h->label, a.rmask[k - 1], cu, cv, ou, ov);
+      std::swap(cu, ou);
+      std::swap(cv, ov);
+    }
+  });
+  // extension init, Dirichlet, gravity, the P2G split, then P2G per particle
+  timed(h, "adj_p2g", [&] {
+    launch_adj_pre(h->stream, h->g, h->label, a.rmask[0], cu, cv, a.g4u, a.g4v, a.g1u, a.g1v,
+                   h->u_prev, h->v_prev, h->acc, h->prm.gravity_x * dt, h->prm.gravity_y * dt,
+                   h->prm.closed_walls, a.qu, a.qv, a.bu, a.bv);
+    launch_adj_p2g(h->stream, h->g, a.x0, np, h->u_prev, h->v_prev, a.qu, a.qv, a.pbar);
+  });
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+int fsg_adjoint_last_iterations(const fsg_domain* h, int32_t* iters) {
+  if (!h || !iters) return FSG_E_INVALID;
+  *iters = h->adj.last_iters;
+  return FSG_OK;
 }
 
 }  // extern "C"

@@ -79,6 +79,43 @@ __device__ __forceinline__ float vel_y_at(const float* __restrict__ v, const Geo
   return interp_scaled(v, g.nx, g.ny, div_x(xx, g), div_y(y, g));
 }
 
+// Corners and fractions of Grid<T>::addToValueInterpolated (include/Grid.h:
+// 152-184) at scaled coordinates: i+1 taken before the clamp.
+struct SplatCorners {
+  int i, j, i1, j1;
+  float fi, fj;
+};
+
+__device__ __forceinline__ SplatCorners splat_corners(float xs, float ys, int nx, int ny) {
+  SplatCorners s;
+  s.i = (int)xs;
+  s.j = (int)ys;
+  s.i1 = s.i + 1;
+  s.j1 = s.j + 1;
+  s.fi = xs - (float)s.i;
+  s.fj = ys - (float)s.j;
+  s.i = clampi(s.i, 0, nx - 1);
+  s.j = clampi(s.j, 0, ny - 1);
+  s.i1 = clampi(s.i1, 0, nx - 1);
+  s.j1 = clampi(s.j1, 0, ny - 1);
+  return s;
+}
+
+// Corners and fractions of Grid<T>::valueInterpolated (include/Grid.h:117-144):
+// i+1 taken after the clamp (differs from the splat only below the first node).
+__device__ __forceinline__ SplatCorners interp_corners(float xs, float ys, int nx, int ny) {
+  SplatCorners s;
+  s.i = (int)xs;
+  s.j = (int)ys;
+  s.fi = xs - (float)s.i;
+  s.fj = ys - (float)s.j;
+  s.i = clampi(s.i, 0, nx - 1);
+  s.j = clampi(s.j, 0, ny - 1);
+  s.i1 = clampi(s.i + 1, 0, nx - 1);
+  s.j1 = clampi(s.j + 1, 0, ny - 1);
+  return s;
+}
+
 // Grid<T>::addToValueInterpolated (include/Grid.h:152-184) with hardware fp32
 // atomics: i+1 taken before the clamp, spread in y then x.
 __device__ __forceinline__ void splat_scaled(float* g, int nx, int ny, float xs, float ys,

@@ -49,7 +49,7 @@ void launch_pressure_update(hipStream_t s, const Geom& g, const int* n_fluid,
                             const int* counts, float ppk, float* au, float* av,
                             float* bu, float* bv, float dt_over_density, int do_dirichlet,
                             int do_diff, const float* prev_u, const float* prev_v, float* diff_u,
-                            float* diff_v);
+                            float* diff_v, uint8_t* zmask = nullptr);
 void launch_diff(hipStream_t s, const Geom& g, const float* u, const float* v, const float* pu,
                  const float* pv, float* du, float* dv);
 void launch_particle_gather(hipStream_t s, const Geom& g, float4* pt, int64_t np, const float* u,
@@ -215,14 +215,47 @@ struct MgBatchScale {
 int mg_vec_blocks(int C);
 int mg_up_blocks(int nx, int ny);
 hipError_t mg_set_lds_limit(size_t bytes);
+// adj_rhs != null: the right-hand side is the adjoint solve's cell field (the
+// pressure's adjoint) instead of the divergence of (uf, vf); see k_mg_adj_rhs0
 void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_t* label,
                      const float* uf, const float* vf, int project_mean, int warm,
-                     CgScalars* sc, const MgBatchScale* batch = nullptr);
+                     CgScalars* sc, const MgBatchScale* batch = nullptr,
+                     const double* adj_rhs = nullptr);
+// removes the plain mean of the FP64 solution over the unknowns (adjoint solves)
+void launch_mg_remove_mean(hipStream_t s, const MgHost& m);
 // one CG iteration: down(l) for l < lc, coarse, up(l) for l = lc-1 .. 0, dir, xr
 void launch_mg_down(hipStream_t s, const MgHost& m, int l, int it, int max_iter, double tol,
                     CgScalars* sc);
 void launch_mg_coarse(hipStream_t s, const MgHost& m, int it, int max_iter, double tol,
                       CgScalars* sc);
+// ---- adjoint (reverse-mode) kernels of the PIC / FLIP step, kernels_adj.hip
+void launch_adj_g2p(hipStream_t s, const Geom& g, const float4* x0, int64_t np, const float* hu,
+                    const float* hv, const float* du, const float* dv, float r, float dt,
+                    const uint8_t* reject, float4* pbar, float* su, float* sv);
+void launch_adj_post(hipStream_t s, const Geom& g, const uint8_t* label, const uint8_t* zmask,
+                     float r, int flip, const float* su, const float* sv, float* eu, float* ev,
+                     float* g4u, float* g4v, float* g1u, float* g1v);
+void launch_adj_pgrad(hipStream_t s, const Geom& g, const uint8_t* label, const float* eu,
+                      const float* ev, float dt_over_density, double* pbar);
+void launch_adj_div(hipStream_t s, const Geom& g, const uint8_t* inf, const double* bbar,
+                    const float* nu, const float* nv, float* eu, float* ev);
+void launch_adj_mask_init(hipStream_t s, const Geom& g, const uint8_t* label, uint8_t* r0,
+                          uint8_t* b0);
+void launch_adj_mask_step(hipStream_t s, const Geom& g, const uint8_t* label, const uint8_t* rk,
+                          const uint8_t* bk, uint8_t* rn, uint8_t* bn);
+void launch_adj_ext_sweep(hipStream_t s, const Geom& g, const uint8_t* label, const uint8_t* rk,
+                          const float* inu, const float* inv, float* outu, float* outv);
+void launch_adj_pre(hipStream_t s, const Geom& g, const uint8_t* label, const uint8_t* m0,
+                    const float* e0u, const float* e0v, const float* g4u, const float* g4v,
+                    const float* g1u, const float* g1v, const float* prev_u,
+                    const float* prev_v, const float4* acc, float gx_dt, float gy_dt,
+                    int closed, float* qu, float* qv, float* nu, float* nv);
+void launch_adj_p2g(hipStream_t s, const Geom& g, const float4* x0, int64_t np, const float* g1u,
+                    const float* g1v, const float* qu, const float* qv, float4* pbar);
+void launch_density(hipStream_t s, const Geom& g, const float4* pt, int64_t np, float* rho);
+void launch_density_adj(hipStream_t s, const Geom& g, const float4* pt, int64_t np,
+                        const float* rb, float4* pbar);
+
 // a batch: every scene's own levels (labels / info once per solve, a cycle per iteration)
 void launch_mg_scene_setup(hipStream_t s, const SceneCoarseArgs& a);
 void launch_mg_scene_coarse(hipStream_t s, const SceneCoarseArgs& a, CgScalars* sc);

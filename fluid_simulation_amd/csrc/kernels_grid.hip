@@ -52,26 +52,6 @@ __global__ void __launch_bounds__(256) k_cell_init(Geom g, uint8_t* __restrict__
 // which atomic accumulation leaves unspecified anyway.
 constexpr int kSplatTile = 2048;  // cells (float4 each, 32 KiB of LDS)
 
-struct SplatCorners {
-  int i, j, i1, j1;
-  float fi, fj;
-};
-
-__device__ __forceinline__ SplatCorners splat_corners(float xs, float ys, int nx, int ny) {
-  SplatCorners s;
-  s.i = (int)xs;
-  s.j = (int)ys;
-  s.i1 = s.i + 1;
-  s.j1 = s.j + 1;
-  s.fi = xs - (float)s.i;
-  s.fj = ys - (float)s.j;
-  s.i = clampi(s.i, 0, nx - 1);
-  s.j = clampi(s.j, 0, ny - 1);
-  s.i1 = clampi(s.i1, 0, nx - 1);
-  s.j1 = clampi(s.j1, 0, ny - 1);
-  return s;
-}
-
 // Adds the 8 terms of one splat (value and weight, Grid.h:152-184 order of
 // operations) through `add(cell_i, cell_j, component, term)`.
 template <typename Add>
@@ -358,7 +338,7 @@ __global__ void __launch_bounds__(256) k_pressure_update(
     float ppk, float* __restrict__ au, float* __restrict__ av,
     float* __restrict__ bu, float* __restrict__ bv, float dt_over_density, int do_dirichlet,
     int do_diff, const float* __restrict__ prev_u, const float* __restrict__ prev_v,
-    float* __restrict__ diff_u, float* __restrict__ diff_v) {
+    float* __restrict__ diff_u, float* __restrict__ diff_v, uint8_t* __restrict__ zmask) {
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int c = t * 256 + threadIdx.x;
   if (c >= g.nx * g.ny) return;
@@ -402,13 +382,18 @@ __global__ void __launch_bounds__(256) k_pressure_update(
     av[c] = bv[c];
   }
   if (L != kLiquid) xg[c] = 0.f;
+  bool zu = false, zv = false;
   if (do_dirichlet == 2) {  // closed walls: no flow through any face next to SOLID
-    if (Lim == kSolid || L == kSolid) nu = 0.f;
-    if (Ljm == kSolid || L == kSolid) nv = 0.f;
+    zu = Lim == kSolid || L == kSolid;
+    zv = Ljm == kSolid || L == kSolid;
   } else if (do_dirichlet) {
-    if ((Lim == kSolid && nu < 0.f) || (L == kSolid && nu > 0.f)) nu = 0.f;
-    if ((Ljm == kSolid && nv < 0.f) || (L == kSolid && nv > 0.f)) nv = 0.f;
+    zu = (Lim == kSolid && nu < 0.f) || (L == kSolid && nu > 0.f);
+    zv = (Ljm == kSolid && nv < 0.f) || (L == kSolid && nv > 0.f);
   }
+  if (zu) nu = 0.f;
+  if (zv) nv = 0.f;
+  // the adjoint's record of the Dirichlet decisions (bit 0: u zeroed, bit 1: v)
+  if (zmask) zmask[c] = (uint8_t)((zu ? 1 : 0) | (zv ? 2 : 0));
   bu[c] = nu;
   bv[c] = nv;
   if (do_diff) {
@@ -773,10 +758,10 @@ void launch_pressure_update(hipStream_t s, const Geom& g, const int* n_fluid,
                             const int* counts, float ppk, float* au, float* av,
                             float* bu, float* bv, float dt_over_density, int do_dirichlet,
                             int do_diff, const float* prev_u, const float* prev_v, float* diff_u,
-                            float* diff_v) {
+                            float* diff_v, uint8_t* zmask) {
   k_pressure_update<<<nblocks((int64_t)g.nx * g.ny), 256, 0, s>>>(
       g, n_fluid, label, xg, x64, counts, ppk, au, av, bu, bv, dt_over_density, do_dirichlet,
-      do_diff, prev_u, prev_v, diff_u, diff_v);
+      do_diff, prev_u, prev_v, diff_u, diff_v, zmask);
 }
 void launch_count_particles(hipStream_t s, const Geom& g, const float4* pt, int64_t np,
                             int* counts) {

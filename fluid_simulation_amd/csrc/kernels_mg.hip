@@ -185,6 +185,94 @@ __global__ void __launch_bounds__(kT) k_mg_setup0(Geom g, const uint8_t* __restr
   write_part<kT>(cnt, part_cnt, sm);
 }
 
+// Adjoint solves (the step's reverse pass): the right-hand side is a given cell
+// field y (the adjoint of the pressure) instead of the divergence.  With the mean
+// removed the forward solution is x = S A^+ P b (P: remove the mean of b, S: the
+// D-weighted constant of k_mg_finish), so the adjoint is P A^+ S^T y with
+// S^T y = y - n (sum y) / (sum n) over the unknowns (n: non-SOLID neighbours).
+// Pass 1: info and the partials of sum y and sum n (into part_y / part_n).
+__global__ void __launch_bounds__(kT) k_mg_adj_rhs0(Geom g, const uint8_t* __restrict__ label,
+                                                    const double* __restrict__ y,
+                                                    uint8_t* __restrict__ inf,
+                                                    double* __restrict__ part_y,
+                                                    double* __restrict__ part_n) {
+  __shared__ double sm[4];
+  const int nx = g.nx, ny = g.ny, C = nx * ny;
+  double sy = 0.0, sn = 0.0;
+  CellWalk w(blockIdx.x * kT + threadIdx.x, gridDim.x * kT, nx);
+  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT, w.next()) {
+    const uint8_t f = make_info(label, nx, ny, w.i, w.j);
+    inf[c] = f;
+    if (is_act(f)) {
+      sy += y[c];
+      sn += (double)(f & 7);
+    }
+  }
+  write_part<kT>(sy, part_y, sm);
+  write_part<kT>(sn, part_n, sm);
+}
+
+// Pass 2: b' = -dx^2 S^T y at the unknowns (S^T = I without the mean removal), and
+// the partials of sum b' and of the unknown count that k_mg_setup1 reads.
+__global__ void __launch_bounds__(kT) k_mg_adj_rhs1(int C, double scale, int project_mean,
+                                                    const uint8_t* __restrict__ inf,
+                                                    const double* __restrict__ y,
+                                                    const double* __restrict__ part_y,
+                                                    const double* __restrict__ part_n, int n_part,
+                                                    double* __restrict__ b,
+                                                    double* __restrict__ part_sum,
+                                                    double* __restrict__ part_cnt) {
+  __shared__ double sm[4];
+  double shift = 0.0;
+  if (project_mean) {
+    const double a = sum_part<kT>(part_y, n_part, sm);
+    const double n = sum_part<kT>(part_n, n_part, sm);
+    shift = n > 0.0 ? a / n : 0.0;
+  }
+  double s = 0.0, cnt = 0.0;
+  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT) {
+    const uint8_t f = inf[c];
+    double bv = 0.0;
+    if (is_act(f)) {
+      bv = scale * (y[c] - (double)(f & 7) * shift);
+      s += bv;
+      cnt += 1.0;
+    }
+    b[c] = bv;
+  }
+  write_part<kT>(s, part_sum, sm);
+  write_part<kT>(cnt, part_cnt, sm);
+}
+
+// P: the plain mean of x over the unknowns removed (after k_mg_finish).
+__global__ void __launch_bounds__(kT) k_mg_mean_part(int C, const uint8_t* __restrict__ inf,
+                                                     const double* __restrict__ x,
+                                                     double* __restrict__ part_sum,
+                                                     double* __restrict__ part_cnt) {
+  __shared__ double sm[4];
+  double s = 0.0, cnt = 0.0;
+  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT)
+    if (is_act(inf[c])) {
+      s += x[c];
+      cnt += 1.0;
+    }
+  write_part<kT>(s, part_sum, sm);
+  write_part<kT>(cnt, part_cnt, sm);
+}
+
+__global__ void __launch_bounds__(kT) k_mg_mean_sub(int C, const uint8_t* __restrict__ inf,
+                                                    double* __restrict__ x,
+                                                    const double* __restrict__ part_sum,
+                                                    const double* __restrict__ part_cnt,
+                                                    int n_part) {
+  __shared__ double sm[4];
+  const double a = sum_part<kT>(part_sum, n_part, sm);
+  const double n = sum_part<kT>(part_cnt, n_part, sm);
+  const double mean = n > 0.0 ? a / n : 0.0;
+  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT)
+    if (is_act(inf[c])) x[c] -= mean;
+}
+
 // Optional mean removal (singular all-Neumann systems), x0 and r0 = b' - K x0,
 // |b'|^2 and |r0|^2.  Cold start: x0 = 0 (xw == null).  Warm start: x0 = xw, the
 // previous solution already zeroed off this solve's unknowns (k_mg_warm), read with
@@ -1021,9 +1109,17 @@ void launch_mg_scene_coarse(hipStream_t s, const SceneCoarseArgs& a, CgScalars* 
 
 void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_t* label,
                      const float* uf, const float* vf, int project_mean, int warm,
-                     CgScalars* sc, const MgBatchScale* batch) {
+                     CgScalars* sc, const MgBatchScale* batch, const double* adj_rhs) {
   const int C = g.nx * g.ny, nb = vec_blocks(C);
-  k_mg_setup0<<<nb, kT, 0, s>>>(g, label, uf, vf, m.inf[0], m.b, m.part_sum, m.part_cnt);
+  if (adj_rhs) {
+    // part_bb / part_rr are free until k_mg_setup1 writes them
+    k_mg_adj_rhs0<<<nb, kT, 0, s>>>(g, label, adj_rhs, m.inf[0], m.part_bb, m.part_rr);
+    k_mg_adj_rhs1<<<nb, kT, 0, s>>>(C, -(double)g.dx * (double)g.dx, project_mean, m.inf[0],
+                                    adj_rhs, m.part_bb, m.part_rr, nb, m.b, m.part_sum,
+                                    m.part_cnt);
+  } else {
+    k_mg_setup0<<<nb, kT, 0, s>>>(g, label, uf, vf, m.inf[0], m.b, m.part_sum, m.part_cnt);
+  }
   if (batch) {
     k_mg_scene_norm<<<batch->n_scenes, 1024, 0, s>>>(m.b, g.nx, batch->rect, batch->norm,
                                                      batch->inv);
@@ -1107,6 +1203,12 @@ void launch_mg_xr(hipStream_t s, const MgHost& m, int it, CgScalars* sc) {
   const int nb = vec_blocks(m.nx[0] * m.ny[0]);
   k_mg_xr<<<nb, kT, 0, s>>>(m.nx[0], m.ny[0], m.inf[0], m.p[(it + 1) & 1], m.x[it & 1],
                             m.x[(it + 1) & 1], m.b, m.rl[0], it, m.part_pq, nb, m.part_rr, sc);
+}
+
+void launch_mg_remove_mean(hipStream_t s, const MgHost& m) {
+  const int C = m.nx[0] * m.ny[0], nb = vec_blocks(C);
+  k_mg_mean_part<<<nb, kT, 0, s>>>(C, m.inf[0], m.x[0], m.part_sum, m.part_cnt);
+  k_mg_mean_sub<<<nb, kT, 0, s>>>(C, m.inf[0], m.x[0], m.part_sum, m.part_cnt, nb);
 }
 
 void launch_mg_finish(hipStream_t s, const MgHost& m, int project_mean, CgScalars* sc,

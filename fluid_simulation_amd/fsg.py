@@ -43,7 +43,9 @@ EXPORTS = [
     "fsg_advect_level_set", "fsg_get_level_set", "fsg_batch_create", "fsg_batch_num_scenes",
     "fsg_batch_scene_rect", "fsg_batch_set_particles", "fsg_batch_set_solid_mask",
     "fsg_batch_num_particles", "fsg_batch_get_particles", "fsg_batch_get_velocity",
-    "fsg_batch_get_labels",
+    "fsg_batch_get_labels", "fsg_checkpoint_save", "fsg_checkpoint_load",
+    "fsg_checkpoint_clear", "fsg_adjoint_set", "fsg_adjoint_get", "fsg_adjoint_step",
+    "fsg_adjoint_last_iterations", "fsg_particle_density", "fsg_adjoint_seed_density",
 ]
 
 
@@ -139,6 +141,15 @@ def load(path: str | None = None):
             "fsg_batch_get_particles": ([H, C.c_int32, C.c_void_p, C.c_int64], C.c_int),
             "fsg_batch_get_velocity": ([H, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p], C.c_int),
             "fsg_batch_get_labels": ([H, C.c_int32, C.c_void_p], C.c_int),
+            "fsg_checkpoint_save": ([H, C.c_int32], C.c_int),
+            "fsg_checkpoint_load": ([H, C.c_int32], C.c_int),
+            "fsg_checkpoint_clear": ([H], C.c_int),
+            "fsg_adjoint_set": ([H, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+            "fsg_adjoint_get": ([H, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+            "fsg_adjoint_step": ([H, C.c_int32, C.c_float], C.c_int),
+            "fsg_adjoint_last_iterations": ([H, P(C.c_int32)], C.c_int),
+            "fsg_particle_density": ([H, C.c_void_p], C.c_int),
+            "fsg_adjoint_seed_density": ([H, C.c_void_p], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -400,6 +411,60 @@ class Domain:
         if scene.phi is not None:
             d.set_level_set(scene.phi)
         return d
+
+
+    # -- checkpoints and reverse mode ---------------------------------------------
+    def checkpoint_save(self, slot: int):
+        self._chk(self._L.fsg_checkpoint_save(self._h, slot), "fsg_checkpoint_save")
+
+    def checkpoint_load(self, slot: int):
+        self._chk(self._L.fsg_checkpoint_load(self._h, slot), "fsg_checkpoint_load")
+
+    def checkpoint_clear(self):
+        self._chk(self._L.fsg_checkpoint_clear(self._h), "fsg_checkpoint_clear")
+
+    def adjoint_set(self, pbar=None, bbar_u=None, bbar_v=None):
+        """Adjoint state: pbar (n, 4) per particle, the back buffer's (ny, nx) u and v
+        (None = zeros)."""
+        args = []
+        keep = []
+        for a, shape in ((pbar, (self.num_particles(), 4)), (bbar_u, (self.ny, self.nx)),
+                         (bbar_v, (self.ny, self.nx))):
+            if a is None:
+                args.append(None)
+            else:
+                arr = np.ascontiguousarray(a, np.float32).reshape(shape)
+                keep.append(arr)
+                args.append(arr.ctypes.data)
+        self._chk(self._L.fsg_adjoint_set(self._h, *args), "fsg_adjoint_set")
+
+    def adjoint_get(self):
+        n = self.num_particles()
+        pbar = np.zeros((n, 4), np.float32)
+        bu = np.zeros((self.ny, self.nx), np.float32)
+        bv = np.zeros((self.ny, self.nx), np.float32)
+        self._chk(self._L.fsg_adjoint_get(self._h, pbar.ctypes.data if n else None,
+                                          bu.ctypes.data, bv.ctypes.data), "fsg_adjoint_get")
+        return pbar, bu, bv
+
+    def adjoint_step(self, scheme: int, dt: float):
+        self._chk(self._L.fsg_adjoint_step(self._h, scheme, dt), "fsg_adjoint_step")
+
+    def adjoint_last_iterations(self) -> int:
+        it = C.c_int32()
+        self._chk(self._L.fsg_adjoint_last_iterations(self._h, C.byref(it)),
+                  "fsg_adjoint_last_iterations")
+        return it.value
+
+    def density(self) -> np.ndarray:
+        rho = np.zeros((self.ny, self.nx), np.float32)
+        self._chk(self._L.fsg_particle_density(self._h, rho.ctypes.data), "fsg_particle_density")
+        return rho
+
+    def adjoint_seed_density(self, rho_bar):
+        rb = np.ascontiguousarray(rho_bar, np.float32).reshape(self.ny, self.nx)
+        self._chk(self._L.fsg_adjoint_seed_density(self._h, rb.ctypes.data),
+                  "fsg_adjoint_seed_density")
 
 
 class Batch(Domain):
