@@ -93,7 +93,24 @@ struct BatchInfo {
   fsg::SceneCoarse* d_coarse = nullptr;
   uint8_t* d_sinf = nullptr;
   int stride = 0;
+  // per-scene CG: 64 x 4 vector tiles inside the scenes, and the scalars (5 S doubles)
+  int4* d_vtile = nullptr;
+  int n_vtile = 0;
+  double* d_cg = nullptr;
   fsg::BatchTables tables() const { return fsg::BatchTables{d_scene, d_ptile, n_ptile}; }
+  fsg::MgbArgs cg_args(int W) const {
+    fsg::MgbArgs a;
+    a.tile = d_vtile;
+    a.rect = d_rect;
+    a.n_tiles = n_vtile;
+    a.n_scenes = S;
+    a.W = W;
+    a.rz = d_cg;
+    a.pq = d_cg + 2 * S;
+    a.rr = d_cg + 3 * S;
+    a.bb = d_cg + 4 * S;
+    return a;
+  }
   fsg::MgBatchScale scale() const {
     return fsg::MgBatchScale{S, d_rect, d_sid, d_norm, d_inv};
   }
@@ -341,6 +358,7 @@ int mg_prepare(fsg_domain* h) {
     if (m.nlev != last + 1)
       return fail(h, FSG_E_INVALID, "multigrid solver: packed grid smaller than its split level");
     m.lc = last;
+    m.per_scene = true;
   } else {
     // the coarsest level is swept by one wavefront (a cell per lane)
     if (m.nx[m.nlev - 1] * m.ny[m.nlev - 1] > 64)
@@ -438,12 +456,14 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
     launch_mg_setup(h->stream, h->g, m, h->label, UF(h), VF(h), h->prm.project_mean,
                     h->prm.warm_start != 0 && !h->batch && !adj_rhs, h->sc,
                     h->batch ? &bs : nullptr, adj_rhs);
-    if (h->batch) launch_mg_scene_setup(h->stream, scene_coarse_args(h));
+    if (h->batch) {
+      launch_mg_scene_setup(h->stream, scene_coarse_args(h));
+      launch_mgb_init(h->stream, h->batch->cg_args(h->g.nx), h->batch->d_norm);
+    }
   });
   const int max_iter = h->prm.cg_max_iter;
-  // a batch: every scene's b' has unit norm, so |r| <= (tol / sqrt(S)) |b| bounds every
-  // scene's |r_s| / |b'_s| by tol
-  const double tol = h->batch ? h->prm.cg_tol / sqrt((double)h->batch->S) : h->prm.cg_tol;
+  // a batch runs one CG per scene in lock step, each stopping at |r_s| < tol |b'_s|
+  const double tol = h->prm.cg_tol;
   int it = 0, batch = std::max(h->mg_last_iters + 1, 4);
   CgScalars hs{};
   static const char* const kDownNames[4] = {"cg_mg_down0", "cg_mg_down1", "cg_mg_down2",
@@ -451,6 +471,10 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
   static const char* const kUpNames[4] = {"cg_mg_up0", "cg_mg_up1", "cg_mg_up2", "cg_mg_up3+"};
   for (;;) {
     for (int q = 0; q < batch && it <= max_iter; ++q, ++it) {
+      if (h->batch)
+        timed(h, "cg_mg_check", [&] {
+          launch_mgb_check(h->stream, h->batch->cg_args(h->g.nx), it, max_iter, tol, h->sc);
+        });
       for (int l = 0; l < m.lc; ++l)
         timed(h, kDownNames[std::min(l, 3)],
               [&] { launch_mg_down(h->stream, m, l, it, max_iter, tol, h->sc); });
@@ -460,8 +484,15 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
         timed(h, "cg_mg_coarse", [&] { launch_mg_coarse(h->stream, m, it, max_iter, tol, h->sc); });
       for (int l = m.lc - 1; l >= 0; --l)
         timed(h, kUpNames[std::min(l, 3)], [&] { launch_mg_up(h->stream, m, l, h->sc); });
-      timed(h, "cg_mg_dir", [&] { launch_mg_dir(h->stream, m, it, h->sc); });
-      timed(h, "cg_mg_xr", [&] { launch_mg_xr(h->stream, m, it, h->sc); });
+      if (h->batch) {
+        const fsg::MgbArgs ba = h->batch->cg_args(h->g.nx);
+        timed(h, "cg_mg_rz", [&] { launch_mgb_vectors(h->stream, ba, m, it, h->sc, 0); });
+        timed(h, "cg_mg_dir", [&] { launch_mgb_vectors(h->stream, ba, m, it, h->sc, 1); });
+        timed(h, "cg_mg_xr", [&] { launch_mgb_vectors(h->stream, ba, m, it, h->sc, 2); });
+      } else {
+        timed(h, "cg_mg_dir", [&] { launch_mg_dir(h->stream, m, it, h->sc); });
+        timed(h, "cg_mg_xr", [&] { launch_mg_xr(h->stream, m, it, h->sc); });
+      }
     }
     HIPCHK(h, hipMemcpyAsync(&hs, h->sc, sizeof(hs), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1440,6 +1471,13 @@ int fsg_batch_create(fsg_domain** out, int32_t n_scenes, const int32_t* nx, cons
   ok &= dalloc(h, &b->d_inv, (size_t)n_scenes) == hipSuccess;
   ok &= dalloc(h, &b->d_coarse, (size_t)n_scenes) == hipSuccess;
   ok &= dalloc(h, &b->d_sinf, (size_t)n_scenes * b->stride) == hipSuccess;
+  std::vector<int4> vtile;
+  for (int s = 0; s < n_scenes; ++s)
+    for (int y0 = 0; y0 < ny[s]; y0 += 4)
+      for (int x0 = 0; x0 < nx[s]; x0 += 64) vtile.push_back(make_int4(ox[s] + x0, oy[s] + y0, s, 0));
+  b->n_vtile = (int)vtile.size();
+  ok &= dalloc(h, &b->d_vtile, vtile.size()) == hipSuccess;
+  ok &= dalloc(h, &b->d_cg, (size_t)5 * n_scenes) == hipSuccess;
   if (!ok) return bail(FSG_E_NOMEM, "");
   if (hipMemcpyAsync(b->d_scene, table.data(), sizeof(fsg::BatchScene) * n_scenes,
                      hipMemcpyHostToDevice, h->stream) != hipSuccess ||
@@ -1448,6 +1486,8 @@ int fsg_batch_create(fsg_domain** out, int32_t n_scenes, const int32_t* nx, cons
       hipMemcpyAsync(b->d_sid, sid.data(), sizeof(int) * sid.size(), hipMemcpyHostToDevice,
                      h->stream) != hipSuccess ||
       hipMemcpyAsync(b->d_coarse, b->coarse.data(), sizeof(fsg::SceneCoarse) * n_scenes,
+                     hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+      hipMemcpyAsync(b->d_vtile, vtile.data(), sizeof(int4) * vtile.size(),
                      hipMemcpyHostToDevice, h->stream) != hipSuccess ||
       hipStreamSynchronize(h->stream) != hipSuccess)
     return bail(FSG_E_HIP, "");

@@ -184,7 +184,19 @@ struct SceneCoarseArgs {
   float omega = 0.8f;
 };
 
+// Packed batch: per-scene CG scalars (one CG per scene in lock step).
+struct MgbArgs {
+  const int4* tile = nullptr;  // (x0, y0, scene, 0): 64 x 4 tiles that lie in one scene
+  const int4* rect = nullptr;  // per scene: (x, y, nx, ny) in the packed grid
+  int n_tiles = 0, n_scenes = 0, W = 0;
+  double* rz = nullptr;  // [2][S] r.z by iteration parity
+  double* pq = nullptr;  // [S]
+  double* rr = nullptr;  // [S]
+  double* bb = nullptr;  // [S]
+};
+
 struct MgHost {
+  bool per_scene = false;  // a batch: stop test and CG scalars per scene (MgbArgs)
   int nlev = 0, lc = 0;
   int nx[kMgMaxLevels] = {}, ny[kMgMaxLevels] = {};
   uint8_t* lab[kMgMaxLevels] = {};  // [0]: the domain's labels
@@ -221,6 +233,13 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
                      const float* uf, const float* vf, int project_mean, int warm,
                      CgScalars* sc, const MgBatchScale* batch = nullptr,
                      const double* adj_rhs = nullptr);
+// a batch's per-scene CG: init after the setup; the stop test at the start of each
+// iteration; the vector kernels (which: 0 r.z, 1 direction, 2 x / r update)
+void launch_mgb_init(hipStream_t s, const MgbArgs& a, const double* norm);
+void launch_mgb_check(hipStream_t s, const MgbArgs& a, int it, int max_iter, double tol,
+                      CgScalars* sc);
+void launch_mgb_vectors(hipStream_t s, const MgbArgs& a, const MgHost& m, int it, CgScalars* sc,
+                        int which);
 // removes the plain mean of the FP64 solution over the unknowns (adjoint solves)
 void launch_mg_remove_mean(hipStream_t s, const MgHost& m);
 // one CG iteration: down(l) for l < lc, coarse, up(l) for l = lc-1 .. 0, dir, xr

@@ -414,6 +414,7 @@ struct MgCheck {
   int n_bb;
   double tol2;
   int max_iter;
+  int external;  // 1: the stop test ran in its own kernel (per-scene CG); read `done` only
 };
 
 // Every block takes the same decision from the same partials.  `done` is read by
@@ -423,7 +424,7 @@ __device__ __forceinline__ bool mg_check(const MgCheck& k, int it, double* sm) {
   __shared__ int s_done;
   if (threadIdx.x == 0) s_done = ld_dev(&k.sc->done);
   __syncthreads();
-  if (s_done) return true;
+  if (s_done || k.external) return s_done != 0;
   const double rr = sum_part<NT>(k.part_rr, k.n_rr, sm);
   double bb, thr;
   if (it == 0) {
@@ -1080,6 +1081,179 @@ __global__ void __launch_bounds__(kT) k_mg_finish(int C, const uint8_t* __restri
   }
 }
 
+// ---------------------------------------------------------------- batch: per-scene CG
+// A packed batch runs one CG per scene in lock step: every scene has its own alpha,
+// beta and stopping test (its own dot products), so each converges as its own solve
+// would, and the iteration ends once every scene meets |r_s| < tol |b'_s|.  The
+// vector kernels walk 64 x 4 tiles that each lie in one scene (a block per tile) and
+// add their block sums to that scene's accumulator, one atomic per block.  Cells
+// outside every scene are never unknowns and keep their zeros.
+__device__ __forceinline__ bool mgb_cell(const MgbArgs& a, int& i, int& j, int& s) {
+  const int4 t = a.tile[blockIdx.x];
+  const int4 r = a.rect[t.z];
+  s = t.z;
+  i = t.x + (threadIdx.x & 63);
+  j = t.y + (threadIdx.x >> 6);
+  return i < r.x + r.z && j < r.y + r.w;
+}
+
+__device__ __forceinline__ void mgb_add(double v, double* dst, double* sm) {
+  double t[1] = {v};
+  block_sum<1, kT>(t, sm);
+  if (threadIdx.x == 0 && t[0] != 0.0) unsafeAtomicAdd(dst, t[0]);
+}
+
+// |b'_s|^2 (1 after the per-scene normalisation, 0 for a scene without unknowns) and
+// |r_s|^2 of x0 = 0.
+__global__ void __launch_bounds__(kT) k_mgb_init(MgbArgs a, const double* __restrict__ norm) {
+  for (int s = threadIdx.x; s < a.n_scenes; s += kT) {
+    const double b = norm[s] > 0.0 ? 1.0 : 0.0;
+    a.bb[s] = b;
+    a.rr[s] = b;
+  }
+}
+
+// Stop test of iteration `it` on the scenes' |r_s|^2, then the accumulators of this
+// iteration cleared.  One workgroup.
+__global__ void __launch_bounds__(kT) k_mgb_check(MgbArgs a, int it, int max_iter, double tol2,
+                                                  CgScalars* sc) {
+  __shared__ int s_done;
+  __shared__ double s_w[kT / 64];
+  if (threadIdx.x == 0) s_done = ld_dev(&sc->done);
+  __syncthreads();
+  if (s_done) return;
+  const int S = a.n_scenes;
+  int ok = 1;
+  double worst = 0.0;
+  for (int s = threadIdx.x; s < S; s += kT) {
+    const double bb = a.bb[s], rr = a.rr[s];
+    if (bb > 0.0) {
+      double thr = tol2 * bb;
+      if (thr < DBL_MIN) thr = DBL_MIN;
+      if (!(rr < thr)) ok = 0;
+      worst = fmax(worst, rr / bb);
+    }
+    a.rr[s] = 0.0;
+    a.pq[s] = 0.0;
+    a.rz[(it & 1) * S + s] = 0.0;
+  }
+  const int all_ok = __syncthreads_and(ok);
+  for (int off = 32; off > 0; off >>= 1) worst = fmax(worst, __shfl_down(worst, off, 64));
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = worst;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kT / 64; ++w) worst = fmax(worst, s_w[w]);
+    sc->iter = it;
+    sc->err = sqrt(worst);
+    if (all_ok || it >= max_iter) st_dev(&sc->done, 1);
+  }
+}
+
+// r_s . z_s
+__global__ void __launch_bounds__(kT) k_mgb_rz(MgbArgs a, const uint8_t* __restrict__ inf,
+                                               const float* __restrict__ r,
+                                               const float* __restrict__ z, int it,
+                                               const CgScalars* sc) {
+  __shared__ double sm[kT / 64];
+  if (sc->done) return;
+  int i, j, s;
+  double v = 0.0;
+  if (mgb_cell(a, i, j, s)) {
+    const int c = i + j * a.W;
+    if (is_act(inf[c])) v = (double)r[c] * (double)z[c];
+  }
+  mgb_add(v, &a.rz[(it & 1) * a.n_scenes + s], sm);
+}
+
+// p_s = z + beta_s p_old (beta_s = r.z / previous r.z of the scene), K p, p_s . K p_s
+__global__ void __launch_bounds__(kT) k_mgb_dir(MgbArgs a, const uint8_t* __restrict__ inf,
+                                                const float* __restrict__ z,
+                                                const float* __restrict__ p_old,
+                                                float* __restrict__ p_new, int it,
+                                                const CgScalars* sc) {
+  __shared__ double sm[kT / 64];
+  if (sc->done) return;
+  int i, j, s;
+  const bool in = mgb_cell(a, i, j, s);
+  const int S = a.n_scenes, W = a.W;
+  const double rzp = it == 0 ? 0.0 : a.rz[((it - 1) & 1) * S + s];
+  const float bf = rzp != 0.0 ? (float)(a.rz[(it & 1) * S + s] / rzp) : 0.f;
+  const bool first = it == 0;
+  double acc = 0.0;
+  if (in) {
+    const int c = i + j * W;
+    const uint8_t f = inf[c];
+    float pc = 0.f;
+    if (is_act(f)) {
+      // neighbours across a scene's border are SOLID: z = p = 0 there
+      auto pn = [&](int q) { return first ? z[q] : z[q] + bf * p_old[q]; };
+      pc = pn(c);
+      const float sn = pn(c - 1) + pn(c + 1) + pn(c - W) + pn(c + W);
+      const float kp = nns_of(f) * pc - sn;
+      acc = (double)pc * (double)kp;
+    }
+    p_new[c] = pc;
+  }
+  mgb_add(acc, &a.pq[s], sm);
+}
+
+// x_new = x_old + alpha_s p, r = b' - K x_new (FP64), |r_s|^2
+__global__ void __launch_bounds__(kT) k_mgb_xr(MgbArgs a, const uint8_t* __restrict__ inf,
+                                               const float* __restrict__ p,
+                                               const double* __restrict__ x_old,
+                                               double* __restrict__ x_new,
+                                               const double* __restrict__ b,
+                                               float* __restrict__ r, int it, CgScalars* sc) {
+  __shared__ double sm[kT / 64];
+  if (sc->done) return;
+  int i, j, s;
+  const bool in = mgb_cell(a, i, j, s);
+  const int W = a.W;
+  const double pq = a.pq[s];
+  const double alpha = pq != 0.0 ? a.rz[(it & 1) * a.n_scenes + s] / pq : 0.0;
+  double acc = 0.0;
+  if (in) {
+    const int c = i + j * W;
+    const uint8_t f = inf[c];
+    double xn = 0.0;
+    float rv = 0.f;
+    if (is_act(f)) {
+      auto xv = [&](int q) { return x_old[q] + alpha * (double)p[q]; };
+      xn = xv(c);
+      const double sn = xv(c - 1) + xv(c + 1) + xv(c - W) + xv(c + W);
+      const double r64 = b[c] - ((double)(f & 7) * xn - sn);
+      rv = (float)r64;
+      acc = r64 * r64;
+    }
+    x_new[c] = xn;
+    r[c] = rv;
+  }
+  mgb_add(acc, &a.rr[s], sm);
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->spmv = it + 1;
+}
+
+void launch_mgb_init(hipStream_t s, const MgbArgs& a, const double* norm) {
+  k_mgb_init<<<1, kT, 0, s>>>(a, norm);
+}
+
+void launch_mgb_check(hipStream_t s, const MgbArgs& a, int it, int max_iter, double tol,
+                      CgScalars* sc) {
+  k_mgb_check<<<1, kT, 0, s>>>(a, it, max_iter, tol * tol, sc);
+}
+
+void launch_mgb_vectors(hipStream_t s, const MgbArgs& a, const MgHost& m, int it, CgScalars* sc,
+                        int which) {
+  if (a.n_tiles <= 0) return;
+  if (which == 0)
+    k_mgb_rz<<<a.n_tiles, kT, 0, s>>>(a, m.inf[0], m.rl[0], m.xo[0], it, sc);
+  else if (which == 1)
+    k_mgb_dir<<<a.n_tiles, kT, 0, s>>>(a, m.inf[0], m.xo[0], m.p[it & 1], m.p[(it + 1) & 1], it,
+                                        sc);
+  else
+    k_mgb_xr<<<a.n_tiles, kT, 0, s>>>(a, m.inf[0], m.p[(it + 1) & 1], m.x[it & 1],
+                                       m.x[(it + 1) & 1], m.b, m.rl[0], it, sc);
+}
+
 // ---------------------------------------------------------------- launchers
 static inline int vec_blocks(int C) {
   int nb = (C + 4 * kT - 1) / (4 * kT);
@@ -1151,7 +1325,7 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
 MgCheck mg_check_args(const MgHost& m, int it, int max_iter, double tol, CgScalars* sc) {
   const int nb = vec_blocks(m.nx[0] * m.ny[0]);
   // |r|^2: k_mg_setup1's partials (it == 0) or k_mg_xr's; |b'|^2 from k_mg_setup1
-  return MgCheck{sc, m.part_rr, nb, m.part_bb, nb, tol * tol, max_iter};
+  return MgCheck{sc, m.part_rr, nb, m.part_bb, nb, tol * tol, max_iter, m.per_scene ? 1 : 0};
 }
 
 static MgLevel mg_level(const MgHost& m, int l) {

@@ -221,8 +221,9 @@ int fsg_get_level_set(fsg_domain *h, float *phi);
    fsg_set_params, fsg_step (PIC / FLIP / PICFLIP, FSG_SOLVER_MG, no particle
    pressure term), fsg_get_stats and the timing calls apply to the whole batch, one
    kernel launch per phase for all scenes.  Particles are kept in scene-local
-   coordinates.  The joint pressure solve normalises each scene's right-hand side
-   and divides cg_tol by sqrt(n_scenes), so every scene meets |r_s| <= cg_tol |b_s|.
+   coordinates.  The pressure solve runs one multigrid-preconditioned CG per scene in
+   lock step (per-scene step lengths and stopping test), so every scene meets
+   |r_s| < cg_tol |b_s|; fsg_get_stats reports the largest scene's |r_s| / |b_s|.
    A scene with no LIQUID cell still swaps with the others (the reference's
    no-fluid early return is per packed grid). */
 int fsg_batch_create(fsg_domain **out, int32_t n_scenes, const int32_t *nx, const int32_t *ny,
