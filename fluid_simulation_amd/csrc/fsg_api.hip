@@ -195,6 +195,7 @@ struct fsg_domain {
   size_t emit_cap = 0;
   AdjState adj;
   std::vector<Checkpoint> ckpt;
+  bool kclock_on = false;  // k_mg_coarse's in-kernel clock (fsg_set_kernel_clock)
   bool mg_ready = false;
   bool mg_solved = false;  // the last solve was the multigrid form (its FP64 x is valid)
   int mg_last_iters = 0;
@@ -402,6 +403,7 @@ int mg_prepare(fsg_domain* h) {
   ok &= dalloc(h, &m.part_rr, kMgMaxVecBlocks) == hipSuccess;
   ok &= dalloc(h, &m.part_pq, kMgMaxVecBlocks) == hipSuccess;
   ok &= dalloc(h, &m.part_rz, (size_t)std::max(mg_up_blocks(m.nx[0], m.ny[0]), 1)) == hipSuccess;
+  ok &= dalloc(h, &m.clk, 2) == hipSuccess;
   if (!ok) return fail(h, FSG_E_NOMEM, "multigrid solver: device allocation failed");
   m.lab[0] = h->label;
   a.lc = m.lc;
@@ -450,6 +452,7 @@ fsg::SceneCoarseArgs scene_coarse_args(const fsg_domain* h) {
 int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
   if (int rc = mg_prepare(h)) return rc;
   MgHost& m = h->mg;
+  m.clk_on = h->kclock_on;
   timed(h, "cg_mg_setup", [&] {
     fsg::MgBatchScale bs;
     if (h->batch) bs = h->batch->scale();
@@ -1845,6 +1848,32 @@ int fsg_adjoint_step(fsg_domain* h, int32_t scheme, float dt) {
   });
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+int fsg_set_kernel_clock(fsg_domain* h, int32_t enable) {
+  if (!h) return FSG_E_INVALID;
+  h->kclock_on = enable != 0;
+  if (h->mg.clk) {
+    HIPCHK(h, hipMemsetAsync(h->mg.clk, 0, 2 * sizeof(unsigned long long), h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  return FSG_OK;
+}
+
+int fsg_get_kernel_clock(fsg_domain* h, double* us_total, int64_t* launches) {
+  if (!h || !us_total || !launches) return FSG_E_INVALID;
+  *us_total = 0.0;
+  *launches = 0;
+  if (!h->mg.clk) return FSG_OK;
+  unsigned long long v[2] = {0, 0};
+  HIPCHK(h, hipMemcpyAsync(v, h->mg.clk, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  int khz = 0;
+  HIPCHK(h, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, h->device));
+  if (khz <= 0) return fail(h, FSG_E_HIP, "fsg_get_kernel_clock: no device wall clock rate");
+  *us_total = (double)v[0] * 1000.0 / (double)khz;
+  *launches = (int64_t)v[1];
   return FSG_OK;
 }
 

@@ -197,6 +197,8 @@ struct MgbArgs {
 
 struct MgHost {
   bool per_scene = false;  // a batch: stop test and CG scalars per scene (MgbArgs)
+  unsigned long long* clk = nullptr;  // k_mg_coarse's own clock: ticks, launches
+  bool clk_on = false;
   int nlev = 0, lc = 0;
   int nx[kMgMaxLevels] = {}, ny[kMgMaxLevels] = {};
   uint8_t* lab[kMgMaxLevels] = {};  // [0]: the domain's labels

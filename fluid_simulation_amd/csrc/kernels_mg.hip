@@ -819,16 +819,31 @@ __device__ void coarse_cycle(int nl, const int* __restrict__ lnx, const int* __r
   }
 }
 
+// The kernel's own duration from the device wall clock (clk != null): clk[0] += end -
+// start (ticks), clk[1] += 1, by thread 0 after the workgroup's last barrier.
+__device__ __forceinline__ void kclock_end(unsigned long long* clk, unsigned long long t0) {
+  if (clk && threadIdx.x == 0) {
+    atomicAdd(&clk[0], wall_clock64() - t0);
+    atomicAdd(&clk[1], 1ull);
+  }
+}
+
 // With check != 0 (the hierarchy starts in this kernel) it runs level 0's
 // convergence test first and forms the r.z partial at the end.
 __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, MgCheck chk, int it,
                                                              int check,
-                                                             double* __restrict__ part_rz) {
+                                                             double* __restrict__ part_rz,
+                                                             unsigned long long* clk) {
+  const unsigned long long t0 = clk ? wall_clock64() : 0ull;
   __shared__ double sm[2 * 16];
   __shared__ int s_lv[3 * kMgMaxLevels];
   if (check) {
-    if (mg_check<kCoarseThreads>(chk, it, sm)) return;
+    if (mg_check<kCoarseThreads>(chk, it, sm)) {
+      kclock_end(clk, t0);
+      return;
+    }
   } else if (chk.sc->done) {
+    kclock_end(clk, t0);
     return;
   }
   extern __shared__ float lds[];
@@ -867,6 +882,10 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, Mg
       block_sum<1, kCoarseThreads>(v, sm);
       if (threadIdx.x == 0) part_rz[0] = v[0];
     }
+  }
+  if (clk) {
+    __syncthreads();
+    kclock_end(clk, t0);
   }
 }
 
@@ -1353,7 +1372,8 @@ void launch_mg_coarse(hipStream_t s, const MgHost& m, int it, int max_iter, doub
                       CgScalars* sc) {
   const MgCheck chk = mg_check_args(m, it, max_iter, tol, sc);
   k_mg_coarse<<<1, kCoarseThreads, m.coarse_lds, s>>>(m.coarse, chk, it, m.lc == 0 ? 1 : 0,
-                                                      m.lc == 0 ? m.part_rz : nullptr);
+                                                      m.lc == 0 ? m.part_rz : nullptr,
+                                                      m.clk_on ? m.clk : nullptr);
 }
 
 void launch_mg_up(hipStream_t s, const MgHost& m, int l, CgScalars* sc) {

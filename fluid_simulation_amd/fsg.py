@@ -46,6 +46,7 @@ EXPORTS = [
     "fsg_batch_get_labels", "fsg_checkpoint_save", "fsg_checkpoint_load",
     "fsg_checkpoint_clear", "fsg_adjoint_set", "fsg_adjoint_get", "fsg_adjoint_step",
     "fsg_adjoint_last_iterations", "fsg_particle_density", "fsg_adjoint_seed_density",
+    "fsg_set_kernel_clock", "fsg_get_kernel_clock",
 ]
 
 
@@ -150,6 +151,8 @@ def load(path: str | None = None):
             "fsg_adjoint_last_iterations": ([H, P(C.c_int32)], C.c_int),
             "fsg_particle_density": ([H, C.c_void_p], C.c_int),
             "fsg_adjoint_seed_density": ([H, C.c_void_p], C.c_int),
+            "fsg_set_kernel_clock": ([H, C.c_int32], C.c_int),
+            "fsg_get_kernel_clock": ([H, P(C.c_double), P(C.c_int64)], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -412,6 +415,17 @@ class Domain:
             d.set_level_set(scene.phi)
         return d
 
+
+    def set_kernel_clock(self, enable: bool):
+        self._chk(self._L.fsg_set_kernel_clock(self._h, 1 if enable else 0),
+                  "fsg_set_kernel_clock")
+
+    def kernel_clock(self):
+        """(microseconds, launches) of k_mg_coarse by its own device clock."""
+        us, n = C.c_double(), C.c_int64()
+        self._chk(self._L.fsg_get_kernel_clock(self._h, C.byref(us), C.byref(n)),
+                  "fsg_get_kernel_clock")
+        return us.value, n.value
 
     # -- checkpoints and reverse mode ---------------------------------------------
     def checkpoint_save(self, slot: int):
