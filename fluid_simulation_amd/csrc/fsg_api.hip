@@ -93,7 +93,7 @@ struct BatchInfo {
   fsg::SceneCoarse* d_coarse = nullptr;
   uint8_t* d_sinf = nullptr;
   int stride = 0;
-  // per-scene CG: 64 x 4 vector tiles inside the scenes, and the scalars (5 S doubles)
+  // per-scene CG: 64 x 16 vector tiles inside the scenes, and the scalars (5 S doubles)
   int4* d_vtile = nullptr;
   int n_vtile = 0;
   double* d_cg = nullptr;
@@ -1476,7 +1476,7 @@ int fsg_batch_create(fsg_domain** out, int32_t n_scenes, const int32_t* nx, cons
   ok &= dalloc(h, &b->d_sinf, (size_t)n_scenes * b->stride) == hipSuccess;
   std::vector<int4> vtile;
   for (int s = 0; s < n_scenes; ++s)
-    for (int y0 = 0; y0 < ny[s]; y0 += 4)
+    for (int y0 = 0; y0 < ny[s]; y0 += 16)
       for (int x0 = 0; x0 < nx[s]; x0 += 64) vtile.push_back(make_int4(ox[s] + x0, oy[s] + y0, s, 0));
   b->n_vtile = (int)vtile.size();
   ok &= dalloc(h, &b->d_vtile, vtile.size()) == hipSuccess;

@@ -1104,16 +1104,27 @@ __global__ void __launch_bounds__(kT) k_mg_finish(int C, const uint8_t* __restri
 // A packed batch runs one CG per scene in lock step: every scene has its own alpha,
 // beta and stopping test (its own dot products), so each converges as its own solve
 // would, and the iteration ends once every scene meets |r_s| < tol |b'_s|.  The
-// vector kernels walk 64 x 4 tiles that each lie in one scene (a block per tile) and
-// add their block sums to that scene's accumulator, one atomic per block.  Cells
-// outside every scene are never unknowns and keep their zeros.
-__device__ __forceinline__ bool mgb_cell(const MgbArgs& a, int& i, int& j, int& s) {
+// vector kernels walk 64 x 16 tiles that each lie in one scene (a block per tile, a
+// column of 4 rows per thread) and add their block sums to that scene's
+// accumulator, one atomic per block.  Cells outside every scene are never unknowns
+// and keep their zeros.
+constexpr int kMgbRows = 4;  // rows per thread; tiles are 64 x (4 * kMgbRows)
+
+struct MgbTile {
+  int i, j0, s, jend;  // column, first row, scene, end row (exclusive) of the scene
+  bool col;            // the column lies in the scene
+};
+
+__device__ __forceinline__ MgbTile mgb_tile(const MgbArgs& a) {
   const int4 t = a.tile[blockIdx.x];
   const int4 r = a.rect[t.z];
-  s = t.z;
-  i = t.x + (threadIdx.x & 63);
-  j = t.y + (threadIdx.x >> 6);
-  return i < r.x + r.z && j < r.y + r.w;
+  MgbTile m;
+  m.s = t.z;
+  m.i = t.x + (threadIdx.x & 63);
+  m.j0 = t.y + (threadIdx.x >> 6);
+  m.jend = r.y + r.w;
+  m.col = m.i < r.x + r.z;
+  return m;
 }
 
 __device__ __forceinline__ void mgb_add(double v, double* dst, double* sm) {
@@ -1175,13 +1186,14 @@ __global__ void __launch_bounds__(kT) k_mgb_rz(MgbArgs a, const uint8_t* __restr
                                                const CgScalars* sc) {
   __shared__ double sm[kT / 64];
   if (sc->done) return;
-  int i, j, s;
+  const MgbTile t = mgb_tile(a);
   double v = 0.0;
-  if (mgb_cell(a, i, j, s)) {
-    const int c = i + j * a.W;
-    if (is_act(inf[c])) v = (double)r[c] * (double)z[c];
-  }
-  mgb_add(v, &a.rz[(it & 1) * a.n_scenes + s], sm);
+  if (t.col)
+    for (int k = 0, j = t.j0; k < kMgbRows && j < t.jend; ++k, j += 4) {
+      const int c = t.i + j * a.W;
+      if (is_act(inf[c])) v += (double)r[c] * (double)z[c];
+    }
+  mgb_add(v, &a.rz[(it & 1) * a.n_scenes + t.s], sm);
 }
 
 // p_s = z + beta_s p_old (beta_s = r.z / previous r.z of the scene), K p, p_s . K p_s
@@ -1192,28 +1204,28 @@ __global__ void __launch_bounds__(kT) k_mgb_dir(MgbArgs a, const uint8_t* __rest
                                                 const CgScalars* sc) {
   __shared__ double sm[kT / 64];
   if (sc->done) return;
-  int i, j, s;
-  const bool in = mgb_cell(a, i, j, s);
+  const MgbTile t = mgb_tile(a);
   const int S = a.n_scenes, W = a.W;
-  const double rzp = it == 0 ? 0.0 : a.rz[((it - 1) & 1) * S + s];
-  const float bf = rzp != 0.0 ? (float)(a.rz[(it & 1) * S + s] / rzp) : 0.f;
+  const double rzp = it == 0 ? 0.0 : a.rz[((it - 1) & 1) * S + t.s];
+  const float bf = rzp != 0.0 ? (float)(a.rz[(it & 1) * S + t.s] / rzp) : 0.f;
   const bool first = it == 0;
   double acc = 0.0;
-  if (in) {
-    const int c = i + j * W;
-    const uint8_t f = inf[c];
-    float pc = 0.f;
-    if (is_act(f)) {
-      // neighbours across a scene's border are SOLID: z = p = 0 there
-      auto pn = [&](int q) { return first ? z[q] : z[q] + bf * p_old[q]; };
-      pc = pn(c);
-      const float sn = pn(c - 1) + pn(c + 1) + pn(c - W) + pn(c + W);
-      const float kp = nns_of(f) * pc - sn;
-      acc = (double)pc * (double)kp;
+  if (t.col)
+    for (int k = 0, j = t.j0; k < kMgbRows && j < t.jend; ++k, j += 4) {
+      const int c = t.i + j * W;
+      const uint8_t f = inf[c];
+      float pc = 0.f;
+      if (is_act(f)) {
+        // neighbours across a scene's border are SOLID: z = p = 0 there
+        auto pn = [&](int q) { return first ? z[q] : z[q] + bf * p_old[q]; };
+        pc = pn(c);
+        const float sn = pn(c - 1) + pn(c + 1) + pn(c - W) + pn(c + W);
+        const float kp = nns_of(f) * pc - sn;
+        acc += (double)pc * (double)kp;
+      }
+      p_new[c] = pc;
     }
-    p_new[c] = pc;
-  }
-  mgb_add(acc, &a.pq[s], sm);
+  mgb_add(acc, &a.pq[t.s], sm);
 }
 
 // x_new = x_old + alpha_s p, r = b' - K x_new (FP64), |r_s|^2
@@ -1225,29 +1237,29 @@ __global__ void __launch_bounds__(kT) k_mgb_xr(MgbArgs a, const uint8_t* __restr
                                                float* __restrict__ r, int it, CgScalars* sc) {
   __shared__ double sm[kT / 64];
   if (sc->done) return;
-  int i, j, s;
-  const bool in = mgb_cell(a, i, j, s);
+  const MgbTile t = mgb_tile(a);
   const int W = a.W;
-  const double pq = a.pq[s];
-  const double alpha = pq != 0.0 ? a.rz[(it & 1) * a.n_scenes + s] / pq : 0.0;
+  const double pq = a.pq[t.s];
+  const double alpha = pq != 0.0 ? a.rz[(it & 1) * a.n_scenes + t.s] / pq : 0.0;
   double acc = 0.0;
-  if (in) {
-    const int c = i + j * W;
-    const uint8_t f = inf[c];
-    double xn = 0.0;
-    float rv = 0.f;
-    if (is_act(f)) {
-      auto xv = [&](int q) { return x_old[q] + alpha * (double)p[q]; };
-      xn = xv(c);
-      const double sn = xv(c - 1) + xv(c + 1) + xv(c - W) + xv(c + W);
-      const double r64 = b[c] - ((double)(f & 7) * xn - sn);
-      rv = (float)r64;
-      acc = r64 * r64;
+  if (t.col)
+    for (int k = 0, j = t.j0; k < kMgbRows && j < t.jend; ++k, j += 4) {
+      const int c = t.i + j * W;
+      const uint8_t f = inf[c];
+      double xn = 0.0;
+      float rv = 0.f;
+      if (is_act(f)) {
+        auto xv = [&](int q) { return x_old[q] + alpha * (double)p[q]; };
+        xn = xv(c);
+        const double sn = xv(c - 1) + xv(c + 1) + xv(c - W) + xv(c + W);
+        const double r64 = b[c] - ((double)(f & 7) * xn - sn);
+        rv = (float)r64;
+        acc += r64 * r64;
+      }
+      x_new[c] = xn;
+      r[c] = rv;
     }
-    x_new[c] = xn;
-    r[c] = rv;
-  }
-  mgb_add(acc, &a.rr[s], sm);
+  mgb_add(acc, &a.rr[t.s], sm);
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->spmv = it + 1;
 }
 
