@@ -394,7 +394,7 @@ def adjoint_leg(fsg, scenes, args):
         l_b = float((((run(c)[0]) - target) ** 2).sum())
     noise = abs(l_a - l_b)
     fd_out = {}
-    for eps in args.adj_eps:
+    for eps in (args.adj_eps if args.adj_dirs > 0 else []):
         rng = np.random.default_rng(16)
         rows = []
         for k in range(args.adj_dirs):
