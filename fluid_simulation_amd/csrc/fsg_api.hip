@@ -97,6 +97,7 @@ struct BatchInfo {
   int4* d_vtile = nullptr;
   int n_vtile = 0;
   double* d_cg = nullptr;
+  uint8_t* d_vflag = nullptr;
   fsg::BatchTables tables() const { return fsg::BatchTables{d_scene, d_ptile, n_ptile}; }
   fsg::MgbArgs cg_args(int W) const {
     fsg::MgbArgs a;
@@ -109,6 +110,7 @@ struct BatchInfo {
     a.pq = d_cg + 2 * S;
     a.rr = d_cg + 3 * S;
     a.bb = d_cg + 4 * S;
+    a.vflag = d_vflag;
     return a;
   }
   fsg::MgBatchScale scale() const {
@@ -404,6 +406,7 @@ int mg_prepare(fsg_domain* h) {
   ok &= dalloc(h, &m.part_pq, kMgMaxVecBlocks) == hipSuccess;
   ok &= dalloc(h, &m.part_rz, (size_t)std::max(mg_up_blocks(m.nx[0], m.ny[0]), 1)) == hipSuccess;
   ok &= dalloc(h, &m.clk, 2) == hipSuccess;
+  if (m.lc > 0) ok &= dalloc(h, &m.tflag, (size_t)mg_up_blocks(m.nx[0], m.ny[0])) == hipSuccess;
   if (!ok) return fail(h, FSG_E_NOMEM, "multigrid solver: device allocation failed");
   m.lab[0] = h->label;
   a.lc = m.lc;
@@ -461,7 +464,7 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
                     h->batch ? &bs : nullptr, adj_rhs);
     if (h->batch) {
       launch_mg_scene_setup(h->stream, scene_coarse_args(h));
-      launch_mgb_init(h->stream, h->batch->cg_args(h->g.nx), h->batch->d_norm);
+      launch_mgb_init(h->stream, h->batch->cg_args(h->g.nx), h->batch->d_norm, m);
     }
   });
   const int max_iter = h->prm.cg_max_iter;
@@ -486,7 +489,7 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
       else
         timed(h, "cg_mg_coarse", [&] { launch_mg_coarse(h->stream, m, it, max_iter, tol, h->sc); });
       for (int l = m.lc - 1; l >= 0; --l)
-        timed(h, kUpNames[std::min(l, 3)], [&] { launch_mg_up(h->stream, m, l, h->sc); });
+        timed(h, kUpNames[std::min(l, 3)], [&] { launch_mg_up(h->stream, m, l, it, h->sc); });
       if (h->batch) {
         const fsg::MgbArgs ba = h->batch->cg_args(h->g.nx);
         timed(h, "cg_mg_rz", [&] { launch_mgb_vectors(h->stream, ba, m, it, h->sc, 0); });
@@ -1481,6 +1484,7 @@ int fsg_batch_create(fsg_domain** out, int32_t n_scenes, const int32_t* nx, cons
   b->n_vtile = (int)vtile.size();
   ok &= dalloc(h, &b->d_vtile, vtile.size()) == hipSuccess;
   ok &= dalloc(h, &b->d_cg, (size_t)5 * n_scenes) == hipSuccess;
+  ok &= dalloc(h, &b->d_vflag, vtile.size()) == hipSuccess;
   if (!ok) return bail(FSG_E_NOMEM, "");
   if (hipMemcpyAsync(b->d_scene, table.data(), sizeof(fsg::BatchScene) * n_scenes,
                      hipMemcpyHostToDevice, h->stream) != hipSuccess ||

@@ -193,12 +193,14 @@ struct MgbArgs {
   double* pq = nullptr;  // [S]
   double* rr = nullptr;  // [S]
   double* bb = nullptr;  // [S]
+  const uint8_t* vflag = nullptr;  // per tile: 1 if it holds an unknown (set each solve)
 };
 
 struct MgHost {
   bool per_scene = false;  // a batch: stop test and CG scalars per scene (MgbArgs)
   unsigned long long* clk = nullptr;  // k_mg_coarse's own clock: ticks, launches
   bool clk_on = false;
+  uint8_t* tflag = nullptr;  // level-0 tiles (32 x 32) holding an unknown on level 0 or 1
   int nlev = 0, lc = 0;
   int nx[kMgMaxLevels] = {}, ny[kMgMaxLevels] = {};
   uint8_t* lab[kMgMaxLevels] = {};  // [0]: the domain's labels
@@ -237,7 +239,7 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
                      const double* adj_rhs = nullptr);
 // a batch's per-scene CG: init after the setup; the stop test at the start of each
 // iteration; the vector kernels (which: 0 r.z, 1 direction, 2 x / r update)
-void launch_mgb_init(hipStream_t s, const MgbArgs& a, const double* norm);
+void launch_mgb_init(hipStream_t s, const MgbArgs& a, const double* norm, const MgHost& m);
 void launch_mgb_check(hipStream_t s, const MgbArgs& a, int it, int max_iter, double tol,
                       CgScalars* sc);
 void launch_mgb_vectors(hipStream_t s, const MgbArgs& a, const MgHost& m, int it, CgScalars* sc,
@@ -280,7 +282,7 @@ void launch_density_adj(hipStream_t s, const Geom& g, const float4* pt, int64_t 
 // a batch: every scene's own levels (labels / info once per solve, a cycle per iteration)
 void launch_mg_scene_setup(hipStream_t s, const SceneCoarseArgs& a);
 void launch_mg_scene_coarse(hipStream_t s, const SceneCoarseArgs& a, CgScalars* sc);
-void launch_mg_up(hipStream_t s, const MgHost& m, int l, CgScalars* sc);
+void launch_mg_up(hipStream_t s, const MgHost& m, int l, int it, CgScalars* sc);
 void launch_mg_dir(hipStream_t s, const MgHost& m, int it, CgScalars* sc);
 void launch_mg_xr(hipStream_t s, const MgHost& m, int it, CgScalars* sc);
 // x at the unknowns -> xg (FP32) and m.x[0] (FP64, read by the pressure update)

@@ -459,7 +459,7 @@ struct MgLevel {
 
 template <bool L0, int NT>
 __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omega, int it,
-                                                 MgCheck chk) {
+                                                 MgCheck chk, const uint8_t* __restrict__ tflag) {
   __shared__ double sm[NT / 64];
   if (L0) {
     if (mg_check<NT>(chk, it, sm)) return;
@@ -475,6 +475,21 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int bx = tile % tiles_x, by = tile / tiles_x;
   const int fi0 = bx * 32, fj0 = by * 32;
+  if (tflag && !tflag[tile]) {
+    // no unknown in the tile on either level: its outputs are zero; written once per
+    // solve (earlier solves may have left other values)
+    if (it == 0) {
+      for (int t = threadIdx.x; t < 32 * 32; t += NT) {
+        const int i = fi0 + (t & 31), j = fj0 + (t >> 5);
+        if (i < f.nx && j < f.ny) f.xs[i + j * f.nx] = 0.f;
+      }
+      for (int t = threadIdx.x; t < 16 * 16; t += NT) {
+        const int I = bx * 16 + (t & 15), J = by * 16 + (t >> 4);
+        if (I < c.nx && J < c.ny) c.r[I + J * c.nx] = 0.f;
+      }
+    }
+    return;
+  }
   for (int t = threadIdx.x; t < WC * WC; t += NT) {
     const int I = bx * 16 + t % WC - 1, J = by * 16 + t / WC - 1;
     s_cinf[t] = (I >= 0 && I < c.nx && J >= 0 && J < c.ny) ? c.inf[I + J * c.nx] : kWall;
@@ -561,7 +576,8 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
 // Level 0 also forms the partial of r.z (z = xo) for the next direction.
 template <bool L0, int NT>
 __global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega,
-                                               const CgScalars* sc, double* __restrict__ part_rz) {
+                                               const CgScalars* sc, double* __restrict__ part_rz,
+                                               int it, const uint8_t* __restrict__ tflag) {
   if (sc->done) return;
   constexpr int W2 = 36, W1 = 34, WC = 20;
   __shared__ float s_r[W2 * W2], s_xc[W2 * W2], s_y1[W1 * W1], s_e[WC * WC];
@@ -572,6 +588,16 @@ __global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega,
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int bx = tile % tiles_x, by = tile / tiles_x;
   const int fi0 = bx * 32, fj0 = by * 32;
+  if (tflag && !tflag[tile]) {
+    // no unknown in the tile: z = 0 there (written once per solve) and r.z adds 0
+    if (it == 0)
+      for (int t = threadIdx.x; t < 32 * 32; t += NT) {
+        const int i = fi0 + (t & 31), j = fj0 + (t >> 5);
+        if (i < f.nx && j < f.ny) f.xo[i + j * f.nx] = 0.f;
+      }
+    if (L0 && threadIdx.x == 0) part_rz[blockIdx.x] = 0.0;
+    return;
+  }
   const int ci0 = (fi0 >> 1) - 2, cj0 = (fj0 >> 1) - 2;  // coarse frame origin
   for (int t = threadIdx.x; t < WC * WC; t += NT) {
     const int I = ci0 + t % WC, J = cj0 + t / WC;
@@ -1100,6 +1126,27 @@ __global__ void __launch_bounds__(kT) k_mg_finish(int C, const uint8_t* __restri
   }
 }
 
+// Level-0 tiles (32 x 32 fine cells and their 16 x 16 coarse cells) with no unknown on
+// either level: the level-0 tile kernels skip them.  One block per tile.
+__global__ void __launch_bounds__(kT) k_mg_tile_flags(int nx, int ny,
+                                                      const uint8_t* __restrict__ inf0, int cnx,
+                                                      int cny, const uint8_t* __restrict__ inf1,
+                                                      uint8_t* __restrict__ flag) {
+  const int tiles_x = (nx + 31) >> 5;
+  const int bx = blockIdx.x % tiles_x, by = blockIdx.x / tiles_x;
+  int any = 0;
+  for (int t = threadIdx.x; t < 32 * 32; t += kT) {
+    const int i = bx * 32 + (t & 31), j = by * 32 + (t >> 5);
+    if (i < nx && j < ny && is_act(inf0[i + j * nx])) any = 1;
+  }
+  for (int t = threadIdx.x; t < 16 * 16; t += kT) {
+    const int I = bx * 16 + (t & 15), J = by * 16 + (t >> 4);
+    if (I < cnx && J < cny && is_act(inf1[I + J * cnx])) any = 1;
+  }
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0) flag[blockIdx.x] = any ? 1 : 0;
+}
+
 // ---------------------------------------------------------------- batch: per-scene CG
 // A packed batch runs one CG per scene in lock step: every scene has its own alpha,
 // beta and stopping test (its own dot products), so each converges as its own solve
@@ -1113,6 +1160,7 @@ constexpr int kMgbRows = 4;  // rows per thread; tiles are 64 x (4 * kMgbRows)
 struct MgbTile {
   int i, j0, s, jend;  // column, first row, scene, end row (exclusive) of the scene
   bool col;            // the column lies in the scene
+  bool live;           // the tile holds an unknown (tiles without one skip their work)
 };
 
 __device__ __forceinline__ MgbTile mgb_tile(const MgbArgs& a) {
@@ -1124,7 +1172,22 @@ __device__ __forceinline__ MgbTile mgb_tile(const MgbArgs& a) {
   m.j0 = t.y + (threadIdx.x >> 6);
   m.jend = r.y + r.w;
   m.col = m.i < r.x + r.z;
+  m.live = a.vflag == nullptr || a.vflag[blockIdx.x] != 0;
   return m;
+}
+
+// Vector tiles with no unknown (flags for the batch kernels).
+__global__ void __launch_bounds__(kT) k_mgb_flags(MgbArgs a, const uint8_t* __restrict__ inf,
+                                                  uint8_t* __restrict__ vflag) {
+  const int4 t4 = a.tile[blockIdx.x];
+  const int4 r = a.rect[t4.z];
+  const int i = t4.x + (threadIdx.x & 63);
+  int any = 0;
+  if (i < r.x + r.z)
+    for (int k = 0, j = t4.y + (threadIdx.x >> 6); k < kMgbRows && j < r.y + r.w; ++k, j += 4)
+      if (is_act(inf[i + j * a.W])) any = 1;
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0) vflag[blockIdx.x] = any ? 1 : 0;
 }
 
 __device__ __forceinline__ void mgb_add(double v, double* dst, double* sm) {
@@ -1187,6 +1250,7 @@ __global__ void __launch_bounds__(kT) k_mgb_rz(MgbArgs a, const uint8_t* __restr
   __shared__ double sm[kT / 64];
   if (sc->done) return;
   const MgbTile t = mgb_tile(a);
+  if (!t.live) return;  // r = z = 0 there
   double v = 0.0;
   if (t.col)
     for (int k = 0, j = t.j0; k < kMgbRows && j < t.jend; ++k, j += 4) {
@@ -1205,6 +1269,8 @@ __global__ void __launch_bounds__(kT) k_mgb_dir(MgbArgs a, const uint8_t* __rest
   __shared__ double sm[kT / 64];
   if (sc->done) return;
   const MgbTile t = mgb_tile(a);
+  // a tile without unknowns writes p = 0 into both ping-pong buffers (iterations 0, 1)
+  if (!t.live && it >= 2) return;
   const int S = a.n_scenes, W = a.W;
   const double rzp = it == 0 ? 0.0 : a.rz[((it - 1) & 1) * S + t.s];
   const float bf = rzp != 0.0 ? (float)(a.rz[(it & 1) * S + t.s] / rzp) : 0.f;
@@ -1238,6 +1304,8 @@ __global__ void __launch_bounds__(kT) k_mgb_xr(MgbArgs a, const uint8_t* __restr
   __shared__ double sm[kT / 64];
   if (sc->done) return;
   const MgbTile t = mgb_tile(a);
+  // a tile without unknowns: x[0] = r = 0 from the setup, x[1] = 0 from iteration 0
+  if (!t.live && it >= 1) return;
   const int W = a.W;
   const double pq = a.pq[t.s];
   const double alpha = pq != 0.0 ? a.rz[(it & 1) * a.n_scenes + t.s] / pq : 0.0;
@@ -1263,8 +1331,10 @@ __global__ void __launch_bounds__(kT) k_mgb_xr(MgbArgs a, const uint8_t* __restr
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->spmv = it + 1;
 }
 
-void launch_mgb_init(hipStream_t s, const MgbArgs& a, const double* norm) {
+void launch_mgb_init(hipStream_t s, const MgbArgs& a, const double* norm, const MgHost& m) {
   k_mgb_init<<<1, kT, 0, s>>>(a, norm);
+  if (a.vflag && a.n_tiles > 0)
+    k_mgb_flags<<<a.n_tiles, kT, 0, s>>>(a, m.inf[0], const_cast<uint8_t*>(a.vflag));
 }
 
 void launch_mgb_check(hipStream_t s, const MgbArgs& a, int it, int max_iter, double tol,
@@ -1351,6 +1421,9 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
     k_mg_coarsen<<<(n + kT - 1) / kT, kT, 0, s>>>(m.nx[l], m.ny[l], m.lab[l], m.inf[l], 0, 0,
                                                   nullptr);
   }
+  if (m.tflag && m.lc > 0)
+    k_mg_tile_flags<<<mg_up_blocks(m.nx[0], m.ny[0]), kT, 0, s>>>(
+        m.nx[0], m.ny[0], m.inf[0], m.nx[1], m.ny[1], m.inf[1], m.tflag);
 }
 
 MgCheck mg_check_args(const MgHost& m, int it, int max_iter, double tol, CgScalars* sc) {
@@ -1374,10 +1447,10 @@ void launch_mg_down(hipStream_t s, const MgHost& m, int l, int it, int max_iter,
   // each LDS phase of the few tiles takes one or two passes
   if (l == 0)
     k_mg_down<true, kT><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega, it,
-                                              chk);
+                                              chk, m.tflag);
   else
     k_mg_down<false, kTileBig><<<blocks, kTileBig, 0, s>>>(mg_level(m, l), mg_level(m, l + 1),
-                                                           m.coarse.omega, it, chk);
+                                                           m.coarse.omega, it, chk, nullptr);
 }
 
 void launch_mg_coarse(hipStream_t s, const MgHost& m, int it, int max_iter, double tol,
@@ -1388,14 +1461,15 @@ void launch_mg_coarse(hipStream_t s, const MgHost& m, int it, int max_iter, doub
                                                       m.clk_on ? m.clk : nullptr);
 }
 
-void launch_mg_up(hipStream_t s, const MgHost& m, int l, CgScalars* sc) {
+void launch_mg_up(hipStream_t s, const MgHost& m, int l, int it, CgScalars* sc) {
   const int blocks = mg_up_blocks(m.nx[l], m.ny[l]);
   if (l == 0)
     k_mg_up<true, kT><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega, sc,
-                                            m.part_rz);
+                                            m.part_rz, it, m.tflag);
   else
     k_mg_up<false, kTileBig><<<blocks, kTileBig, 0, s>>>(mg_level(m, l), mg_level(m, l + 1),
-                                                         m.coarse.omega, sc, nullptr);
+                                                         m.coarse.omega, sc, nullptr, it,
+                                                         nullptr);
 }
 
 void launch_mg_dir(hipStream_t s, const MgHost& m, int it, CgScalars* sc) {
