@@ -377,8 +377,9 @@ def adjoint_leg(fsg, scenes, args):
     c = (torch.randn(2, 16, 16, generator=torch.Generator().manual_seed(0)) * scale)
     c.requires_grad_(True)
     d.synchronize()
-    t0 = time.perf_counter()
+    tm0 = time.perf_counter()
     vel, u, v = model(c)
+    t0 = time.perf_counter()
     loss = adjoint.Rollout.apply(pos, vel, u, v, d, fsg.PICFLIP, dt, steps, every, target)
     d.synchronize()
     t1 = time.perf_counter()
@@ -386,7 +387,8 @@ def adjoint_leg(fsg, scenes, args):
     d.synchronize()
     t2 = time.perf_counter()
     grad = c.grad.detach().numpy().astype(np.float64)
-    log(f"adjoint {n}^2, {len(parts)} particles, {steps} steps: forward {1e3 * (t1 - t0):.1f} ms, "
+    log(f"adjoint {n}^2, {len(parts)} particles, {steps} steps: control model "
+        f"{1e3 * (t0 - tm0):.1f} ms, forward {1e3 * (t1 - t0):.1f} ms, "
         f"reverse {1e3 * (t2 - t1):.1f} ms, loss {loss.item():.6e}, |grad| {np.linalg.norm(grad):.3e}")
     # run-to-run spread of the loss at the same controls (float atomics in P2G)
     with torch.no_grad():
@@ -426,7 +428,9 @@ def adjoint_leg(fsg, scenes, args):
         "workload": f"closed all-fluid {n}^2 box, {len(parts)} particles, controls (2, 16, 16) "
                     f"x {scale:g} bilinearly upsampled, {steps} PICFLIP steps dt {dt}, "
                     f"checkpoints every {every}, loss sum((density - target)^2)",
+        "control_model_ms": 1e3 * (t0 - tm0),
         "forward_ms": 1e3 * (t1 - t0),
+        "forward_ms_per_step": 1e3 * (t1 - t0) / steps,
         "reverse_ms": 1e3 * (t2 - t1),
         "reverse_ms_per_step": 1e3 * (t2 - t1) / steps,
         "loss": loss.item(),

@@ -1629,6 +1629,7 @@ int fsg_checkpoint_save(fsg_domain* h, int32_t slot) {
     HIPCHK(h, dalloc(h, &c.v[1], C));
   }
   if (c.cap < h->np) {
+    HIPCHK(h, hipStreamSynchronize(h->stream));  // earlier copies may still use the slot
     if (c.pt) (void)hipFree(c.pt);
     c.pt = nullptr;
     c.cap = 0;
@@ -1643,8 +1644,7 @@ int fsg_checkpoint_save(fsg_domain* h, int32_t slot) {
   HIPCHK(h, hipMemcpyAsync(c.v[0], VF(h), C * 4, hipMemcpyDeviceToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(c.u[1], UB(h), C * 4, hipMemcpyDeviceToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(c.v[1], VB(h), C * 4, hipMemcpyDeviceToDevice, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  return FSG_OK;
+  return FSG_OK;  // stream-ordered: later work on the handle sees the copy
 }
 
 int fsg_checkpoint_load(fsg_domain* h, int32_t slot) {
@@ -1661,8 +1661,7 @@ int fsg_checkpoint_load(fsg_domain* h, int32_t slot) {
   HIPCHK(h, hipMemcpyAsync(VF(h), c.v[0], C * 4, hipMemcpyDeviceToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(UB(h), c.u[1], C * 4, hipMemcpyDeviceToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(VB(h), c.v[1], C * 4, hipMemcpyDeviceToDevice, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  return FSG_OK;
+  return FSG_OK;  // stream-ordered
 }
 
 int fsg_checkpoint_clear(fsg_domain* h) {
@@ -1851,8 +1850,7 @@ int fsg_adjoint_step(fsg_domain* h, int32_t scheme, float dt) {
     launch_adj_p2g(h->stream, h->g, a.x0, np, h->u_prev, h->v_prev, a.qu, a.qv, a.pbar);
   });
   HIPCHK(h, hipGetLastError());
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  return FSG_OK;
+  return FSG_OK;  // stream-ordered; fsg_adjoint_get synchronises
 }
 
 int fsg_set_kernel_clock(fsg_domain* h, int32_t enable) {
