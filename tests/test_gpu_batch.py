@@ -26,10 +26,12 @@ def _vel_errs(ub, vb, ur, vr):
     return float(np.abs(ub - ur).max()) / scale, float(np.abs(vb - vr).max()) / scale
 
 
-def test_batch_matches_single_domains():
-    """13 scenes (random sizes 24..96, obstacles, one without particles), 3 PICFLIP
-    steps: per scene, labels equal and particles / velocities within 1e-5 of a
-    single-scene Domain; both solve to |r| / |b| <= 1e-10 (every scene's own)."""
+@pytest.mark.parametrize("scheme", [fsg.PICFLIP, fsg.FLIP, fsg.PIC])
+def test_batch_matches_single_domains(scheme):
+    """13 scenes (random sizes 24..96, obstacles, one without particles), 3 steps of
+    each FLIP-family scheme: per scene, labels equal and particles / velocities within
+    1e-5 of a single-scene Domain; both solve to |r| / |b| <= 1e-10 (every scene's
+    own)."""
     scs = scenes.dam_break_batch(12, 24, 96, seed=3)
     scs.insert(5, _empty_scene(40, 30))
     tol = 1e-10
@@ -38,10 +40,10 @@ def test_batch_matches_single_domains():
                for sc in scs]
     print(f"batch of {len(scs)} scenes packed into {b.nx} x {b.ny}", flush=True)
     for step in range(3):
-        b.step(fsg.PICFLIP, 0.01)
+        b.step(scheme, 0.01)
         its = []
         for d in singles:
-            d.step(fsg.PICFLIP, 0.01)
+            d.step(scheme, 0.01)
             its.append(d.stats().cg_iterations)
         bi = b.stats().cg_iterations
         print(f"step {step}: batch MG iterations {bi}, single scenes {its}", flush=True)
