@@ -293,6 +293,11 @@ def batch_leg(fsg, scenes, args):
                "sample": f"first step of the first {k} scenes, FP32 oracle, Jacobi-PCG rel "
                          f"tol 1e-6, 1 thread; the batch rate is extrapolated by cells"}
         log(f"batch cpu oracle: {k / t_cpu:.3f} scene-steps/s over {k} scenes ({t_cpu:.1f} s)")
+    # the per-scene CG vector kernels: algorithmic bytes over the scenes' cells (each
+    # array read or written once) per launch, over their HIP-event time (which also
+    # counts launch gaps, so this is a lower bound on the bandwidth)
+    vec_bytes = {"cg_mg_xr": 33.0 * cells, "cg_mg_dir": 13.0 * cells, "cg_mg_rz": 9.0 * cells}
+    vec_gbs = {k: vec_bytes[k] / (prof[k][0] * 1e-3) / 1e9 for k in vec_bytes if k in prof}
     return {
         "workload": f"dam_break_batch: {len(scs)} non-square scenes, sides {args.batch_lo}.."
                     f"{args.batch_hi}, 1-4 obstacles each, {cells} cells, {parts} particles, "
@@ -304,6 +309,7 @@ def batch_leg(fsg, scenes, args):
         "cell_steps_per_s": sps * cells,
         "cg_iterations_last_solve": st.cg_iterations,
         "cg_hbm_gbs": gbs,
+        "cg_vector_kernel_gbs": vec_gbs,
         "one_scene_at_a_time": seq,
         "cpu_baseline": cpu,
         "speedup_vs_one_at_a_time": (sps * cells / seq["cell_steps_per_s"]) if seq else None,
