@@ -105,9 +105,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_domain(fsg, scene, solver, tol, max_iter):
+def make_domain(fsg, scene, solver, tol, max_iter, warm=0):
     return fsg.Domain.from_scene(scene, cg_tol=tol, cg_max_iter=max_iter, solver=solver,
-                                 refine_inner_tol=1e-3, refine_max_outer=20, check_every=64)
+                                 refine_inner_tol=1e-3, refine_max_outer=20, check_every=64,
+                                 warm_start=warm)
 
 
 def run_steps(fsg, d, n, dt, scheme=None):
@@ -459,6 +460,8 @@ def main():
     ap.add_argument("--plain-tol", type=float, default=1e-6)
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--warm-start", type=int, default=0,
+                    help="chosen form: 1 starts each solve from the previous pressure")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-lo", type=int, default=64)
     ap.add_argument("--batch-hi", type=int, default=512)
@@ -506,7 +509,7 @@ def main():
     log(f"workload dam_break_{args.n}: {P} particles; solver {args.solver} tol {args.tol:g}")
 
     # ---- chosen form: warmup, per-kernel profile (untimed), timed region
-    d = make_domain(fsg, scene, solver, args.tol, max_iter)
+    d = make_domain(fsg, scene, solver, args.tol, max_iter, args.warm_start)
     run_steps(fsg, d, args.warmup, dt)
     prof = kernel_profile(fsg, d, args.profile_steps, dt) if args.profile_steps > 0 else {}
     per_step = {k: ms * n for k, (ms, n) in prof.items()}

@@ -136,9 +136,45 @@ __device__ __forceinline__ void small_divmod(int q, int nx, float inv_nx, int& i
 template <int NT = kT>
 __device__ __forceinline__ double sum_part(const double* __restrict__ part, int n, double* sm) {
   double v[1] = {0.0};
-  for (int b = threadIdx.x; b < n; b += NT) v[0] += part[b];
+  int b = threadIdx.x;
+  // four loads in flight per thread; the additions keep the order b, b + NT, ...
+  for (; b + 3 * NT < n; b += 4 * NT) {
+    const double a0 = part[b], a1 = part[b + NT], a2 = part[b + 2 * NT], a3 = part[b + 3 * NT];
+    v[0] += a0;
+    v[0] += a1;
+    v[0] += a2;
+    v[0] += a3;
+  }
+  for (; b < n; b += NT) v[0] += part[b];
   block_sum<1, NT>(v, sm);
   return v[0];
+}
+
+// Cells per thread per pass of the CG vector kernels: the loads of all of them are
+// issued before the first is used, so a thread waits on memory once per pass.
+constexpr int kVecU = 4;
+
+// The 5-point stencil of cell c = (i, j) on an nx x ny grid: own, left, right, down, up.
+// A neighbour outside the grid (or any, for a cell past the end, in = false) points at
+// c itself and has its mask bit clear, so every index is in bounds.
+struct Stencil5 {
+  int q[5];
+  bool m[5];
+};
+
+__device__ __forceinline__ Stencil5 stencil5(int c, int i, int j, int nx, int ny, bool in) {
+  Stencil5 s;
+  s.m[0] = in;
+  s.m[1] = in && i > 0;
+  s.m[2] = in && i < nx - 1;
+  s.m[3] = in && j > 0;
+  s.m[4] = in && j < ny - 1;
+  s.q[0] = c;
+  s.q[1] = s.m[1] ? c - 1 : c;
+  s.q[2] = s.m[2] ? c + 1 : c;
+  s.q[3] = s.m[3] ? c - nx : c;
+  s.q[4] = s.m[4] ? c + nx : c;
+  return s;
 }
 
 template <int NT = kT>
@@ -490,22 +526,43 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
     }
     return;
   }
-  for (int t = threadIdx.x; t < WC * WC; t += NT) {
+  // every load of the tile is issued before the first LDS store (a load outside the
+  // grid reads cell 0 and is replaced by the boundary value)
+  constexpr int NC = (WC * WC + NT - 1) / NT, NF = (W3 * W3 + NT - 1) / NT;
+  uint8_t cv[NC], fv[NF];
+  float rv[NF];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    const int t = threadIdx.x + k * NT;
     const int I = bx * 16 + t % WC - 1, J = by * 16 + t / WC - 1;
-    s_cinf[t] = (I >= 0 && I < c.nx && J >= 0 && J < c.ny) ? c.inf[I + J * c.nx] : kWall;
+    const bool in = t < WC * WC && I >= 0 && I < c.nx && J >= 0 && J < c.ny;
+    const uint8_t v = c.inf[in ? I + J * c.nx : 0];
+    cv[k] = in ? v : kWall;
   }
-  for (int t = threadIdx.x; t < W3 * W3; t += NT) {
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    const int t = threadIdx.x + k * NT;
     const int i = fi0 + t % W3 - 3, j = fj0 + t / W3 - 3;
-    float rv = 0.f;
-    uint8_t fv = 0;
-    if (i >= 0 && i < f.nx && j >= 0 && j < f.ny) {
-      const int q = i + j * f.nx;
-      fv = f.inf[q];
-      rv = f.r[q];
+    const bool in = t < W3 * W3 && i >= 0 && i < f.nx && j >= 0 && j < f.ny;
+    const int q = in ? i + j * f.nx : 0;
+    const uint8_t fq = f.inf[q];
+    const float rq = f.r[q];
+    fv[k] = in ? fq : 0;
+    rv[k] = in ? rq : 0.f;
+  }
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    const int t = threadIdx.x + k * NT;
+    if (t < WC * WC) s_cinf[t] = cv[k];
+  }
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    const int t = threadIdx.x + k * NT;
+    if (t < W3 * W3) {
+      s_inf[t] = fv[k];
+      s_r[t] = rv[k];
+      s_x1[t] = is_act(fv[k]) ? omega * rv[k] * inv_nns(fv[k]) : 0.f;
     }
-    s_inf[t] = fv;
-    s_r[t] = rv;
-    s_x1[t] = is_act(fv) ? omega * rv * inv_nns(fv) : 0.f;
   }
   __syncthreads();
   for (int t = threadIdx.x; t < W2 * W2; t += NT) {
@@ -599,25 +656,50 @@ __global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega,
     return;
   }
   const int ci0 = (fi0 >> 1) - 2, cj0 = (fj0 >> 1) - 2;  // coarse frame origin
-  for (int t = threadIdx.x; t < WC * WC; t += NT) {
+  // every load of the tile is issued before the first LDS store (a load outside the
+  // grid reads cell 0 and is replaced by the boundary value)
+  constexpr int NC = (WC * WC + NT - 1) / NT, NF = (W2 * W2 + NT - 1) / NT;
+  uint8_t cv[NC], fv[NF];
+  float ev[NC], rv[NF], xv[NF];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    const int t = threadIdx.x + k * NT;
     const int I = ci0 + t % WC, J = cj0 + t / WC;
-    const bool in = I >= 0 && I < c.nx && J >= 0 && J < c.ny;
-    s_e[t] = in ? c.xo[I + J * c.nx] : 0.f;
-    s_cinf[t] = in ? c.inf[I + J * c.nx] : kWall;
+    const bool in = t < WC * WC && I >= 0 && I < c.nx && J >= 0 && J < c.ny;
+    const int q = in ? I + J * c.nx : 0;
+    const float e = c.xo[q];
+    const uint8_t ci = c.inf[q];
+    ev[k] = in ? e : 0.f;
+    cv[k] = in ? ci : kWall;
   }
-  for (int t = threadIdx.x; t < W2 * W2; t += NT) {
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    const int t = threadIdx.x + k * NT;
     const int i = fi0 + t % W2 - 2, j = fj0 + t / W2 - 2;
-    float rv = 0.f, xv = 0.f;
-    uint8_t fv = 0;
-    if (i >= 0 && i < f.nx && j >= 0 && j < f.ny) {
-      const int q = i + j * f.nx;
-      fv = f.inf[q];
-      rv = f.r[q];
-      xv = f.xs[q];
+    const bool in = t < W2 * W2 && i >= 0 && i < f.nx && j >= 0 && j < f.ny;
+    const int q = in ? i + j * f.nx : 0;
+    const uint8_t fq = f.inf[q];
+    const float rq = f.r[q], xq = f.xs[q];
+    fv[k] = in ? fq : 0;
+    rv[k] = in ? rq : 0.f;
+    xv[k] = in ? xq : 0.f;
+  }
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    const int t = threadIdx.x + k * NT;
+    if (t < WC * WC) {
+      s_e[t] = ev[k];
+      s_cinf[t] = cv[k];
     }
-    s_inf[t] = fv;
-    s_r[t] = rv;
-    s_xc[t] = xv;
+  }
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    const int t = threadIdx.x + k * NT;
+    if (t < W2 * W2) {
+      s_inf[t] = fv[k];
+      s_r[t] = rv[k];
+      s_xc[t] = xv[k];
+    }
   }
   __syncthreads();
   for (int t = threadIdx.x; t < W2 * W2; t += NT) {
@@ -997,6 +1079,58 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_scene_coarse(SceneCoarseA
 }
 
 // ---------------------------------------------------------------- CG vectors
+// The cells of one thread's pass of k_mg_dir (FIRST: p = z).
+template <bool FIRST>
+__device__ __forceinline__ double mg_dir_cells(int nx, int ny, const uint8_t* __restrict__ inf,
+                                               const float* __restrict__ z,
+                                               const float* __restrict__ p_old,
+                                               float* __restrict__ p_new, float bf) {
+  const int C = nx * ny, stride = gridDim.x * kT;
+  double acc = 0.0;
+  CellWalk w(blockIdx.x * kT + threadIdx.x, stride, nx);
+  for (int c0 = blockIdx.x * kT + threadIdx.x; c0 < C; c0 += kVecU * stride) {
+    Stencil5 st[kVecU];
+    uint8_t f[kVecU];
+    float zv[kVecU][5], pv[kVecU][5];
+#pragma unroll
+    for (int u = 0; u < kVecU; ++u) {
+      const int c = c0 + u * stride;
+      const bool in = c < C;
+      st[u] = stencil5(in ? c : C - 1, w.i, w.j, nx, ny, in);
+      w.next();
+      f[u] = inf[st[u].q[0]];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        zv[u][k] = z[st[u].q[k]];
+        if (!FIRST) pv[u][k] = p_old[st[u].q[k]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kVecU; ++u) {
+      if (!st[u].m[0]) break;
+      float pn[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const float v = FIRST ? zv[u][k] : zv[u][k] + bf * pv[u][k];
+        pn[k] = st[u].m[k] ? v : 0.f;
+      }
+      float pc = 0.f;
+      if (is_act(f[u])) {
+        pc = pn[0];
+        float sn = 0.f;
+        sn += pn[1];
+        sn += pn[2];
+        sn += pn[3];
+        sn += pn[4];
+        const float kp = nns_of(f[u]) * pc - sn;
+        acc += (double)pc * (double)kp;
+      }
+      p_new[st[u].q[0]] = pc;
+    }
+  }
+  return acc;
+}
+
 // p = z + beta p_old (beta = 0 at the first iteration), K p, partial p.Kp.
 // rho = r.z is summed from the V-cycle's partials; block 0 records it.
 __global__ void __launch_bounds__(kT) k_mg_dir(int nx, int ny, const uint8_t* __restrict__ inf,
@@ -1010,28 +1144,8 @@ __global__ void __launch_bounds__(kT) k_mg_dir(int nx, int ny, const uint8_t* __
   const double rz = sum_part<kT>(part_rz, n_rz, sm);
   const double beta = it == 0 ? 0.0 : rz / sc->rho[(it - 1) & 1];
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->rho[it & 1] = rz;
-  const float bf = (float)beta;
-  const bool first = it == 0;
-  const int C = nx * ny;
-  double acc = 0.0;
-  CellWalk w(blockIdx.x * kT + threadIdx.x, gridDim.x * kT, nx);
-  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT, w.next()) {
-    const uint8_t f = inf[c];
-    float pc = 0.f;
-    if (is_act(f)) {
-      const int i = w.i, j = w.j;
-      auto pn = [&](int q) { return first ? z[q] : z[q] + bf * p_old[q]; };
-      pc = pn(c);
-      float s = 0.f;
-      if (i > 0) s += pn(c - 1);
-      if (i < nx - 1) s += pn(c + 1);
-      if (j > 0) s += pn(c - nx);
-      if (j < ny - 1) s += pn(c + nx);
-      const float kp = nns_of(f) * pc - s;
-      acc += (double)pc * (double)kp;
-    }
-    p_new[c] = pc;
-  }
+  const double acc = it == 0 ? mg_dir_cells<true>(nx, ny, inf, z, p_old, p_new, 0.f)
+                             : mg_dir_cells<false>(nx, ny, inf, z, p_old, p_new, (float)beta);
   write_part<kT>(acc, part_pq, sm);
 }
 
@@ -1048,28 +1162,51 @@ __global__ void __launch_bounds__(kT) k_mg_xr(int nx, int ny, const uint8_t* __r
   if (sc->done) return;
   const double pq = sum_part<kT>(part_pq, n_pq, sm);
   const double alpha = sc->rho[it & 1] / pq;
-  const int C = nx * ny;
+  const int C = nx * ny, stride = gridDim.x * kT;
   double acc = 0.0;
-  CellWalk w(blockIdx.x * kT + threadIdx.x, gridDim.x * kT, nx);
-  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT, w.next()) {
-    const uint8_t f = inf[c];
-    double xn = 0.0;
-    float rv = 0.f;
-    if (is_act(f)) {
-      const int i = w.i, j = w.j;
-      auto xv = [&](int q) { return x_old[q] + alpha * (double)p[q]; };
-      xn = xv(c);
-      double s = 0.0;
-      if (i > 0) s += xv(c - 1);
-      if (i < nx - 1) s += xv(c + 1);
-      if (j > 0) s += xv(c - nx);
-      if (j < ny - 1) s += xv(c + nx);
-      const double r64 = b[c] - ((double)(f & 7) * xn - s);
-      rv = (float)r64;
-      acc += r64 * r64;
+  CellWalk w(blockIdx.x * kT + threadIdx.x, stride, nx);
+  for (int c0 = blockIdx.x * kT + threadIdx.x; c0 < C; c0 += kVecU * stride) {
+    Stencil5 st[kVecU];
+    uint8_t f[kVecU];
+    double xv[kVecU][5], bv[kVecU];
+    float pv[kVecU][5];
+#pragma unroll
+    for (int u = 0; u < kVecU; ++u) {
+      const int c = c0 + u * stride;
+      const bool in = c < C;
+      st[u] = stencil5(in ? c : C - 1, w.i, w.j, nx, ny, in);
+      w.next();
+      f[u] = inf[st[u].q[0]];
+      bv[u] = b[st[u].q[0]];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        xv[u][k] = x_old[st[u].q[k]];
+        pv[u][k] = p[st[u].q[k]];
+      }
     }
-    x_new[c] = xn;
-    r[c] = rv;
+#pragma unroll
+    for (int u = 0; u < kVecU; ++u) {
+      if (!st[u].m[0]) break;
+      double xn = 0.0;
+      float rv = 0.f;
+      if (is_act(f[u])) {
+        double xs[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+          xs[k] = st[u].m[k] ? xv[u][k] + alpha * (double)pv[u][k] : 0.0;
+        xn = xs[0];
+        double sn = 0.0;
+        sn += xs[1];
+        sn += xs[2];
+        sn += xs[3];
+        sn += xs[4];
+        const double r64 = bv[u] - ((double)(f[u] & 7) * xn - sn);
+        rv = (float)r64;
+        acc += r64 * r64;
+      }
+      x_new[st[u].q[0]] = xn;
+      r[st[u].q[0]] = rv;
+    }
   }
   write_part<kT>(acc, part_rr, sm);
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->spmv = it + 1;
@@ -1242,6 +1379,31 @@ __global__ void __launch_bounds__(kT) k_mgb_check(MgbArgs a, int it, int max_ite
   }
 }
 
+// The rows k = 0 .. kMgbRows-1 of a thread's column in its tile: cell index (0 for a
+// row outside the scene, in[k] = false) and info byte.  Loads first, then use.
+struct MgbCells {
+  int c[kMgbRows];
+  bool in[kMgbRows];
+  uint8_t f[kMgbRows];
+};
+
+__device__ __forceinline__ MgbCells mgb_cells(const MgbTile& t, int W,
+                                              const uint8_t* __restrict__ inf) {
+  MgbCells m;
+#pragma unroll
+  for (int k = 0; k < kMgbRows; ++k) {
+    const int j = t.j0 + 4 * k;
+    m.in[k] = t.col && j < t.jend;
+    m.c[k] = m.in[k] ? t.i + j * W : 0;
+  }
+#pragma unroll
+  for (int k = 0; k < kMgbRows; ++k) {
+    const uint8_t f = inf[m.c[k]];
+    m.f[k] = m.in[k] ? f : 0;
+  }
+  return m;
+}
+
 // r_s . z_s
 __global__ void __launch_bounds__(kT) k_mgb_rz(MgbArgs a, const uint8_t* __restrict__ inf,
                                                const float* __restrict__ r,
@@ -1251,12 +1413,17 @@ __global__ void __launch_bounds__(kT) k_mgb_rz(MgbArgs a, const uint8_t* __restr
   if (sc->done) return;
   const MgbTile t = mgb_tile(a);
   if (!t.live) return;  // r = z = 0 there
+  const MgbCells m = mgb_cells(t, a.W, inf);
+  float rv[kMgbRows], zv[kMgbRows];
+#pragma unroll
+  for (int k = 0; k < kMgbRows; ++k) {
+    rv[k] = r[m.c[k]];
+    zv[k] = z[m.c[k]];
+  }
   double v = 0.0;
-  if (t.col)
-    for (int k = 0, j = t.j0; k < kMgbRows && j < t.jend; ++k, j += 4) {
-      const int c = t.i + j * a.W;
-      if (is_act(inf[c])) v += (double)r[c] * (double)z[c];
-    }
+#pragma unroll
+  for (int k = 0; k < kMgbRows; ++k)
+    if (m.in[k] && is_act(m.f[k])) v += (double)rv[k] * (double)zv[k];
   mgb_add(v, &a.rz[(it & 1) * a.n_scenes + t.s], sm);
 }
 
@@ -1275,22 +1442,37 @@ __global__ void __launch_bounds__(kT) k_mgb_dir(MgbArgs a, const uint8_t* __rest
   const double rzp = it == 0 ? 0.0 : a.rz[((it - 1) & 1) * S + t.s];
   const float bf = rzp != 0.0 ? (float)(a.rz[(it & 1) * S + t.s] / rzp) : 0.f;
   const bool first = it == 0;
-  double acc = 0.0;
-  if (t.col)
-    for (int k = 0, j = t.j0; k < kMgbRows && j < t.jend; ++k, j += 4) {
-      const int c = t.i + j * W;
-      const uint8_t f = inf[c];
-      float pc = 0.f;
-      if (is_act(f)) {
-        // neighbours across a scene's border are SOLID: z = p = 0 there
-        auto pn = [&](int q) { return first ? z[q] : z[q] + bf * p_old[q]; };
-        pc = pn(c);
-        const float sn = pn(c - 1) + pn(c + 1) + pn(c - W) + pn(c + W);
-        const float kp = nns_of(f) * pc - sn;
-        acc += (double)pc * (double)kp;
-      }
-      p_new[c] = pc;
+  const MgbCells m = mgb_cells(t, W, inf);
+  // neighbours of an unknown lie in its scene (its border is SOLID: z = p = 0 there);
+  // other cells read their own value only
+  float zv[kMgbRows][5], pv[kMgbRows][5];
+#pragma unroll
+  for (int k = 0; k < kMgbRows; ++k) {
+    const int c = m.c[k];
+    const bool act = is_act(m.f[k]);
+    const int q[5] = {c, act ? c - 1 : c, act ? c + 1 : c, act ? c - W : c, act ? c + W : c};
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      zv[k][e] = z[q[e]];
+      pv[k][e] = first ? 0.f : p_old[q[e]];
     }
+  }
+  double acc = 0.0;
+#pragma unroll
+  for (int k = 0; k < kMgbRows; ++k) {
+    if (!m.in[k]) break;
+    float pc = 0.f;
+    if (is_act(m.f[k])) {
+      float pn[5];
+#pragma unroll
+      for (int e = 0; e < 5; ++e) pn[e] = first ? zv[k][e] : zv[k][e] + bf * pv[k][e];
+      pc = pn[0];
+      const float sn = pn[1] + pn[2] + pn[3] + pn[4];
+      const float kp = nns_of(m.f[k]) * pc - sn;
+      acc += (double)pc * (double)kp;
+    }
+    p_new[m.c[k]] = pc;
+  }
   mgb_add(acc, &a.pq[t.s], sm);
 }
 
@@ -1309,24 +1491,40 @@ __global__ void __launch_bounds__(kT) k_mgb_xr(MgbArgs a, const uint8_t* __restr
   const int W = a.W;
   const double pq = a.pq[t.s];
   const double alpha = pq != 0.0 ? a.rz[(it & 1) * a.n_scenes + t.s] / pq : 0.0;
-  double acc = 0.0;
-  if (t.col)
-    for (int k = 0, j = t.j0; k < kMgbRows && j < t.jend; ++k, j += 4) {
-      const int c = t.i + j * W;
-      const uint8_t f = inf[c];
-      double xn = 0.0;
-      float rv = 0.f;
-      if (is_act(f)) {
-        auto xv = [&](int q) { return x_old[q] + alpha * (double)p[q]; };
-        xn = xv(c);
-        const double sn = xv(c - 1) + xv(c + 1) + xv(c - W) + xv(c + W);
-        const double r64 = b[c] - ((double)(f & 7) * xn - sn);
-        rv = (float)r64;
-        acc += r64 * r64;
-      }
-      x_new[c] = xn;
-      r[c] = rv;
+  const MgbCells m = mgb_cells(t, W, inf);
+  double xv[kMgbRows][5], bv[kMgbRows];
+  float pv[kMgbRows][5];
+#pragma unroll
+  for (int k = 0; k < kMgbRows; ++k) {
+    const int c = m.c[k];
+    const bool act = is_act(m.f[k]);
+    const int q[5] = {c, act ? c - 1 : c, act ? c + 1 : c, act ? c - W : c, act ? c + W : c};
+    bv[k] = b[c];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      xv[k][e] = x_old[q[e]];
+      pv[k][e] = p[q[e]];
     }
+  }
+  double acc = 0.0;
+#pragma unroll
+  for (int k = 0; k < kMgbRows; ++k) {
+    if (!m.in[k]) break;
+    double xn = 0.0;
+    float rv = 0.f;
+    if (is_act(m.f[k])) {
+      double xs[5];
+#pragma unroll
+      for (int e = 0; e < 5; ++e) xs[e] = xv[k][e] + alpha * (double)pv[k][e];
+      xn = xs[0];
+      const double sn = xs[1] + xs[2] + xs[3] + xs[4];
+      const double r64 = bv[k] - ((double)(m.f[k] & 7) * xn - sn);
+      rv = (float)r64;
+      acc += r64 * r64;
+    }
+    x_new[m.c[k]] = xn;
+    r[m.c[k]] = rv;
+  }
   mgb_add(acc, &a.rr[t.s], sm);
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->spmv = it + 1;
 }

@@ -502,8 +502,8 @@ class Batch(Domain):
 
     @classmethod
     def from_scenes(cls, scenes, **params):
-        """One batch holding ``scenes`` (all with unit spacing ratio length / n equal,
-        the same density and pic_ratio); scene s keeps index s."""
+        """One batch holding ``scenes`` (all with the same cell width length / n,
+        density and pic_ratio); scene s keeps index s."""
         if not scenes:
             raise FsgError("a batch needs at least one scene")
         sc0 = scenes[0]
