@@ -754,12 +754,17 @@ __global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega,
 // contiguous array, loaded in one pass.  The coarsest level (<= 4 x 4 cells) is
 // swept by one wavefront, a cell per lane, neighbours by lane shuffles, with no
 // barrier between sweeps.
+// Neighbour sum of cell c = (i, j) of an nx x ny level in LDS; a neighbour outside the
+// level adds 0.  Branch-free: the four reads are issued together.
 __device__ __forceinline__ float nbr_sum(const float* a, int c, int i, int j, int nx, int ny) {
+  const bool ml = i > 0, mr = i < nx - 1, md = j > 0, mu = j < ny - 1;
+  const float vl = a[ml ? c - 1 : c], vr = a[mr ? c + 1 : c];
+  const float vd = a[md ? c - nx : c], vu = a[mu ? c + nx : c];
   float s = 0.f;
-  if (i > 0) s += a[c - 1];
-  if (i < nx - 1) s += a[c + 1];
-  if (j > 0) s += a[c - nx];
-  if (j < ny - 1) s += a[c + nx];
+  s += ml ? vl : 0.f;
+  s += mr ? vr : 0.f;
+  s += md ? vd : 0.f;
+  s += mu ? vu : 0.f;
   return s;
 }
 
@@ -769,9 +774,12 @@ constexpr int kCoarseThreads = 1024;
 // LDS: R, X, T and INF, level l at offset off[l] (off[0] = 0).  R's level 0 holds
 // the right-hand side; the coarser right-hand sides are formed here.  Level 0's
 // result is left in X when nl == 1, else in T.
-__device__ void coarse_cycle(int nl, const int* __restrict__ lnx, const int* __restrict__ lny,
-                             const int* __restrict__ loff, float w, int sweeps, float* R,
-                             float* X, float* T, const uint8_t* INF) {
+// Inlined into its kernels so that R, X, T and INF are addressed as LDS (a called
+// function sees generic pointers and goes through the slower flat path).
+__device__ __forceinline__ void coarse_cycle(int nl, const int* __restrict__ lnx,
+                                             const int* __restrict__ lny,
+                                             const int* __restrict__ loff, float w, int sweeps,
+                                             float* R, float* X, float* T, const uint8_t* INF) {
   for (int l = 0; l < nl - 1; ++l) {
     const int nx = lnx[l], ny = lny[l], n = nx * ny, o = loff[l];
     float *r = R + o, *x = X + o, *t = T + o;
@@ -783,12 +791,20 @@ __device__ void coarse_cycle(int nl, const int* __restrict__ lnx, const int* __r
       if (is_act(inf[q])) {
         int i, j;
         small_divmod(q, nx, inv_nx, i, j);
-        auto x1 = [&](int k) { return is_act(inf[k]) ? w * r[k] * inv_nns(inf[k]) : 0.f; };
+        const bool mk[4] = {i > 0, i < nx - 1, j > 0, j < ny - 1};
+        const int qk[4] = {mk[0] ? q - 1 : q, mk[1] ? q + 1 : q, mk[2] ? q - nx : q,
+                           mk[3] ? q + nx : q};
+        uint8_t fk[4];
+        float rk[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          fk[k] = inf[qk[k]];
+          rk[k] = r[qk[k]];
+        }
         float sn = 0.f;
-        if (i > 0) sn += x1(q - 1);
-        if (i < nx - 1) sn += x1(q + 1);
-        if (j > 0) sn += x1(q - nx);
-        if (j < ny - 1) sn += x1(q + nx);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          sn += (mk[k] && is_act(fk[k])) ? w * rk[k] * inv_nns(fk[k]) : 0.f;
         const float nn = nns_of(inf[q]), xc = w * r[q] * inv_nns(inf[q]);
         v = xc + w * (r[q] - (nn * xc - sn)) * inv_nns(inf[q]);
       }
@@ -812,31 +828,50 @@ __device__ void coarse_cycle(int nl, const int* __restrict__ lnx, const int* __r
       small_divmod(q, cnx, inv_cnx, I, J);
       float v = 0.f;
       if (is_act(INF[co + q])) {
+        // fine rows / columns 2J-1 .. 2J+2 and 2I-1 .. 2I+2, clamped into the level
+        // with weight 0 outside it; every read is issued before the sums
         const float wt[4] = {0.25f, 0.75f, 0.75f, 0.25f};
+        bool rin[4], cin[4];
+        int row[4], col[4];
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-          const int j = 2 * J - 1 + b;
-          if (j < 0 || j >= ny) continue;
+          const int j = 2 * J - 1 + b, i = 2 * I - 1 + b;
+          rin[b] = j >= 0 && j < ny;
+          cin[b] = i >= 0 && i < nx;
+          row[b] = (rin[b] ? j : 0) * nx;
+          col[b] = cin[b] ? i : 0;
+        }
+        float tv[4][4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) tv[b][e] = t[col[e] + row[b]];
+        // coarse neighbours (I -/+ 1, J), (I, J -/+ 1), (I -/+ 1, J -/+ 1); SOLID outside
+        uint8_t cn[3][3];
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+          for (int dx = 0; dx < 3; ++dx) {
+            const int ii = I + dx - 1, jj = J + dy - 1;
+            const bool in = ii >= 0 && ii < cnx && jj >= 0 && jj < cny;
+            const uint8_t fv = INF[co + (in ? ii + jj * cnx : q)];
+            cn[dy][dx] = in ? fv : kWall;
+          }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
           float srow = 0.f;
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int i = 2 * I - 1 + e;
-            if (i >= 0 && i < nx) srow += wt[e] * t[i + j * nx];
-          }
-          v += wt[b] * srow;
+          for (int e = 0; e < 4; ++e) srow += cin[e] ? wt[e] * tv[b][e] : 0.f;
+          v += rin[b] ? wt[b] * srow : 0.f;
         }
-        auto cinf = [&](int ii, int jj) -> uint8_t {
-          return (ii >= 0 && ii < cnx && jj >= 0 && jj < cny) ? INF[co + ii + jj * cnx] : kWall;
-        };
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
           for (int e = 0; e < 2; ++e) {
-            const int i = 2 * I + e, j = 2 * J + b;
-            if (i >= nx || j >= ny) continue;
-            const int dx = e == 0 ? -1 : 1, dy = b == 0 ? -1 : 1;
-            const float m = moved_weight(cinf(I + dx, J), cinf(I, J + dy), cinf(I + dx, J + dy));
-            v += m * t[i + j * nx];
+            // child (2I + e, 2J + b) is tv[b + 1][e + 1]
+            const int dx = e == 0 ? 0 : 2, dy = b == 0 ? 0 : 2;
+            const float m = moved_weight(cn[1][dx], cn[dy][1], cn[dy][dx]);
+            v += (rin[b + 1] && cin[e + 1]) ? m * tv[b + 1][e + 1] : 0.f;
           }
       }
       R[co + q] = v;
@@ -890,13 +925,16 @@ __device__ void coarse_cycle(int nl, const int* __restrict__ lnx, const int* __r
         const int In = (i & 1) ? I + 1 : I - 1;
         const int Jn = (j & 1) ? J + 1 : J - 1;
         const bool iok = In >= 0 && In < cnx, jok = Jn >= 0 && Jn < cny;
-        const float e00 = e[I + J * cnx];
-        const float e10 = iok ? e[In + J * cnx] : 0.f;
-        const float e01 = jok ? e[I + Jn * cnx] : 0.f;
-        const float e11 = iok && jok ? e[In + Jn * cnx] : 0.f;
-        const uint8_t f10 = iok ? cinf[In + J * cnx] : kWall;
-        const uint8_t f01 = jok ? cinf[I + Jn * cnx] : kWall;
-        const uint8_t f11 = iok && jok ? cinf[In + Jn * cnx] : kWall;
+        // neighbours outside the coarse level read (I, J) and are replaced below
+        const int Ic = iok ? In : I, Jc = jok ? Jn : J;
+        const float e00 = e[I + J * cnx], e10r = e[Ic + J * cnx];
+        const float e01r = e[I + Jc * cnx], e11r = e[Ic + Jc * cnx];
+        const uint8_t f10r = cinf[Ic + J * cnx], f01r = cinf[I + Jc * cnx];
+        const uint8_t f11r = cinf[Ic + Jc * cnx];
+        const float e10 = iok ? e10r : 0.f, e01 = jok ? e01r : 0.f;
+        const float e11 = iok && jok ? e11r : 0.f;
+        const uint8_t f10 = iok ? f10r : kWall, f01 = jok ? f01r : kWall;
+        const uint8_t f11 = iok && jok ? f11r : kWall;
         v = x[q] + prolong_value(e00, e10, f10, e01, f01, e11, f11);
       }
       t[q] = v;
