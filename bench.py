@@ -61,6 +61,9 @@ def kernel_bytes(name, nx, ny, P, levels=None):
     levels, whose last one is cycled scene by scene)."""
     C = nx * ny
     cells, lc = levels or mg_levels(nx, ny)
+    # one domain with two or more tile levels: level 0's down kernel also does the
+    # previous iteration's x / residual update (k_mg_xr's bytes), except at iteration 0
+    fused = levels is None and lc >= 2
     table = {
         # particles 16 B read + label 1 B; float4 accumulators read and written once
         "particle_scatter": 17.0 * P + 32.0 * C,
@@ -81,7 +84,10 @@ def kernel_bytes(name, nx, ny, P, levels=None):
             ls = range(3, lc) if tail.endswith("+") else [int(tail)]
             # a "3+" timer covers every launch of levels >= 3: average over them
             ls = list(ls) or [0]
-            return sum(fine_b * cells[l] + coarse_b * cells[l + 1] for l in ls) / len(ls)
+            b = sum(fine_b * cells[l] + coarse_b * cells[l + 1] for l in ls) / len(ls)
+            if fused and pre == "cg_mg_down" and tail == "0":
+                b += table["cg_mg_xr"]
+            return b
     return None
 
 

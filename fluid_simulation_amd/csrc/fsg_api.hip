@@ -370,6 +370,7 @@ int mg_prepare(fsg_domain* h) {
     while ((size_t)m.nx[m.lc] * m.ny[m.lc] > (size_t)kMgCoarseCells) ++m.lc;
     if (m.lc >= m.nlev)
       return fail(h, FSG_E_INVALID, "multigrid solver: grid too elongated for the hierarchy");
+    m.fused = m.lc >= 2;
   }
   // cell offsets of the one-workgroup levels (its LDS layout and its info array)
   MgCoarseArgs& a = m.coarse;
@@ -402,7 +403,7 @@ int mg_prepare(fsg_domain* h) {
   ok &= dalloc(h, &m.part_sum, kMgMaxVecBlocks) == hipSuccess;
   ok &= dalloc(h, &m.part_cnt, kMgMaxVecBlocks) == hipSuccess;
   ok &= dalloc(h, &m.part_bb, kMgMaxVecBlocks) == hipSuccess;
-  ok &= dalloc(h, &m.part_rr, kMgMaxVecBlocks) == hipSuccess;
+  ok &= dalloc(h, &m.part_rr, (size_t)mg_part_rr_size(m)) == hipSuccess;
   ok &= dalloc(h, &m.part_pq, kMgMaxVecBlocks) == hipSuccess;
   ok &= dalloc(h, &m.part_rz, (size_t)std::max(mg_up_blocks(m.nx[0], m.ny[0]), 1)) == hipSuccess;
   ok &= dalloc(h, &m.clk, 2) == hipSuccess;
@@ -497,7 +498,8 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
         timed(h, "cg_mg_xr", [&] { launch_mgb_vectors(h->stream, ba, m, it, h->sc, 2); });
       } else {
         timed(h, "cg_mg_dir", [&] { launch_mg_dir(h->stream, m, it, h->sc); });
-        timed(h, "cg_mg_xr", [&] { launch_mg_xr(h->stream, m, it, h->sc); });
+        // fused form: the update runs in the next iteration's level-0 down kernel
+        if (!m.fused) timed(h, "cg_mg_xr", [&] { launch_mg_xr(h->stream, m, it, h->sc); });
       }
     }
     HIPCHK(h, hipMemcpyAsync(&hs, h->sc, sizeof(hs), hipMemcpyDeviceToHost, h->stream));

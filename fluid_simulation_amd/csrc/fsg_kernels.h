@@ -201,6 +201,9 @@ struct MgHost {
   unsigned long long* clk = nullptr;  // k_mg_coarse's own clock: ticks, launches
   bool clk_on = false;
   uint8_t* tflag = nullptr;  // level-0 tiles (32 x 32) holding an unknown on level 0 or 1
+  // fused form (one domain, two or more tile levels): level 0's down kernel also does
+  // the previous iteration's x and residual update, level 1's runs the stop test
+  bool fused = false;
   int nlev = 0, lc = 0;
   int nx[kMgMaxLevels] = {}, ny[kMgMaxLevels] = {};
   uint8_t* lab[kMgMaxLevels] = {};  // [0]: the domain's labels
@@ -229,6 +232,7 @@ struct MgBatchScale {
 };
 
 int mg_vec_blocks(int C);
+int mg_part_rr_size(const MgHost& m);  // entries of MgHost::part_rr
 int mg_up_blocks(int nx, int ny);
 hipError_t mg_set_lds_limit(size_t bytes);
 // adj_rhs != null: the right-hand side is the adjoint solve's cell field (the

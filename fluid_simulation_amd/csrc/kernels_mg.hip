@@ -493,24 +493,44 @@ struct MgLevel {
   float* xo;  // after the full cycle (level 0: z)
 };
 
-template <bool L0, int NT>
+// Fused form: level 0's down kernel of iteration it >= 1 first finishes iteration
+// it - 1 on its tile, as k_mg_xr would: x_new = x_old + alpha p (FP64), r = b' - K
+// x_new (FP64, stored as FP32) and the block's partial |r|^2.  The residual on the
+// tile's halo is formed from x_new on a halo one cell wider, the same values the
+// neighbouring tiles store.
+struct MgXr {
+  const double* x_old;
+  double* x_new;
+  const float* p;
+  const double* b;
+  const double* part_pq;  // p.Kp partials of iteration it - 1
+  int n_pq;
+  double* part_rr;  // out: one partial |r|^2 per block
+};
+
+template <bool L0, int NT, bool XR = false>
 __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omega, int it,
-                                                 MgCheck chk, const uint8_t* __restrict__ tflag) {
+                                                 MgCheck chk, int check,
+                                                 const uint8_t* __restrict__ tflag, MgXr xr) {
+  static_assert(!XR || L0, "the fused update belongs to level 0");
   __shared__ double sm[NT / 64];
-  if (L0) {
+  if (check) {
     if (mg_check<NT>(chk, it, sm)) return;
   } else if (chk.sc->done) {
     return;
   }
   constexpr int W3 = 38, W2 = 36, W1 = 34;
+  constexpr int W4 = 40;  // fused form: x_new on [-4, 36)
   constexpr int WC = 18;  // coarse cells of the tile and their neighbours
   __shared__ float s_r[W3 * W3], s_x1[W3 * W3], s_x2[W2 * W2], s_res[W1 * W1];
   __shared__ uint8_t s_inf[W3 * W3], s_cinf[WC * WC];
+  __shared__ double s_xn[XR ? W4 * W4 : 1];
   const int tiles_x = (c.nx + 15) >> 4;
   // each XCD takes one band of consecutive tiles, so halo rows are shared in its L2
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int bx = tile % tiles_x, by = tile / tiles_x;
   const int fi0 = bx * 32, fj0 = by * 32;
+  const bool do_xr = XR && it > 0;
   if (tflag && !tflag[tile]) {
     // no unknown in the tile on either level: its outputs are zero; written once per
     // solve (earlier solves may have left other values)
@@ -524,44 +544,147 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
         if (I < c.nx && J < c.ny) c.r[I + J * c.nx] = 0.f;
       }
     }
+    if (do_xr) {
+      // x = r = 0 here: both x buffers once (iterations 1 and 2), r once; every
+      // iteration's stop test sums every block's partial
+      if (it <= 2)
+        for (int t = threadIdx.x; t < 32 * 32; t += NT) {
+          const int i = fi0 + (t & 31), j = fj0 + (t >> 5);
+          if (i < f.nx && j < f.ny) {
+            xr.x_new[i + j * f.nx] = 0.0;
+            if (it == 1) f.r[i + j * f.nx] = 0.f;
+          }
+        }
+      if (threadIdx.x == 0) xr.part_rr[blockIdx.x] = 0.0;
+    }
     return;
   }
-  // every load of the tile is issued before the first LDS store (a load outside the
-  // grid reads cell 0 and is replaced by the boundary value)
-  constexpr int NC = (WC * WC + NT - 1) / NT, NF = (W3 * W3 + NT - 1) / NT;
-  uint8_t cv[NC], fv[NF];
-  float rv[NF];
+  if (do_xr) {
+    // every load of the tile first (a load outside the grid reads cell 0 and is
+    // replaced by the boundary value), then alpha, then the stores to LDS
+    constexpr int NC = (WC * WC + NT - 1) / NT, NF = (W3 * W3 + NT - 1) / NT;
+    constexpr int N4 = (W4 * W4 + NT - 1) / NT;
+    uint8_t cv[NC], fv[NF];
+    double bv[NF], xo[N4];
+    float pv[N4];
 #pragma unroll
-  for (int k = 0; k < NC; ++k) {
-    const int t = threadIdx.x + k * NT;
-    const int I = bx * 16 + t % WC - 1, J = by * 16 + t / WC - 1;
-    const bool in = t < WC * WC && I >= 0 && I < c.nx && J >= 0 && J < c.ny;
-    const uint8_t v = c.inf[in ? I + J * c.nx : 0];
-    cv[k] = in ? v : kWall;
-  }
+    for (int k = 0; k < NC; ++k) {
+      const int t = threadIdx.x + k * NT;
+      const int I = bx * 16 + t % WC - 1, J = by * 16 + t / WC - 1;
+      const bool in = t < WC * WC && I >= 0 && I < c.nx && J >= 0 && J < c.ny;
+      const uint8_t v = c.inf[in ? I + J * c.nx : 0];
+      cv[k] = in ? v : kWall;
+    }
 #pragma unroll
-  for (int k = 0; k < NF; ++k) {
-    const int t = threadIdx.x + k * NT;
-    const int i = fi0 + t % W3 - 3, j = fj0 + t / W3 - 3;
-    const bool in = t < W3 * W3 && i >= 0 && i < f.nx && j >= 0 && j < f.ny;
-    const int q = in ? i + j * f.nx : 0;
-    const uint8_t fq = f.inf[q];
-    const float rq = f.r[q];
-    fv[k] = in ? fq : 0;
-    rv[k] = in ? rq : 0.f;
-  }
+    for (int k = 0; k < NF; ++k) {
+      const int t = threadIdx.x + k * NT;
+      const int i = fi0 + t % W3 - 3, j = fj0 + t / W3 - 3;
+      const bool in = t < W3 * W3 && i >= 0 && i < f.nx && j >= 0 && j < f.ny;
+      const int q = in ? i + j * f.nx : 0;
+      const uint8_t fq = f.inf[q];
+      const double bq = xr.b[q];
+      fv[k] = in ? fq : 0;
+      bv[k] = in ? bq : 0.0;
+    }
 #pragma unroll
-  for (int k = 0; k < NC; ++k) {
-    const int t = threadIdx.x + k * NT;
-    if (t < WC * WC) s_cinf[t] = cv[k];
-  }
+    for (int k = 0; k < N4; ++k) {
+      const int t = threadIdx.x + k * NT;
+      const int i = fi0 + t % W4 - 4, j = fj0 + t / W4 - 4;
+      const bool in = t < W4 * W4 && i >= 0 && i < f.nx && j >= 0 && j < f.ny;
+      const int q = in ? i + j * f.nx : 0;
+      const double xq = xr.x_old[q];
+      const float pq = xr.p[q];
+      xo[k] = in ? xq : 0.0;
+      pv[k] = in ? pq : 0.f;
+    }
+    const double pq = sum_part<NT>(xr.part_pq, xr.n_pq, sm);
+    const double alpha = chk.sc->rho[(it - 1) & 1] / pq;
 #pragma unroll
-  for (int k = 0; k < NF; ++k) {
-    const int t = threadIdx.x + k * NT;
-    if (t < W3 * W3) {
-      s_inf[t] = fv[k];
-      s_r[t] = rv[k];
-      s_x1[t] = is_act(fv[k]) ? omega * rv[k] * inv_nns(fv[k]) : 0.f;
+    for (int k = 0; k < NC; ++k) {
+      const int t = threadIdx.x + k * NT;
+      if (t < WC * WC) s_cinf[t] = cv[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      const int t = threadIdx.x + k * NT;
+      if (t < W3 * W3) s_inf[t] = fv[k];
+    }
+#pragma unroll
+    for (int k = 0; k < N4; ++k) {
+      const int t = threadIdx.x + k * NT;
+      if (t < W4 * W4) s_xn[t] = xo[k] + alpha * (double)pv[k];
+    }
+    __syncthreads();
+    // r = b' - K x_new on [-3, 35) (cells outside the grid hold x = 0 and add 0);
+    // the tile's own cells store x_new and r and add to |r|^2
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      const int t = threadIdx.x + k * NT;
+      if (t >= W3 * W3) break;
+      const int li = t % W3, lj = t / W3;
+      const uint8_t fq = fv[k];
+      const int q4 = (li + 1) + (lj + 1) * W4;
+      const double xc = s_xn[q4];
+      double r64 = 0.0;
+      if (is_act(fq)) {
+        double sn = 0.0;
+        sn += s_xn[q4 - 1];
+        sn += s_xn[q4 + 1];
+        sn += s_xn[q4 - W4];
+        sn += s_xn[q4 + W4];
+        r64 = bv[k] - ((double)(fq & 7) * xc - sn);
+        acc += (li >= 3 && li < 35 && lj >= 3 && lj < 35) ? r64 * r64 : 0.0;
+      }
+      const float rv = (float)r64;
+      s_r[t] = rv;
+      s_x1[t] = is_act(fq) ? omega * rv * inv_nns(fq) : 0.f;
+      const int i = fi0 + li - 3, j = fj0 + lj - 3;
+      if (li >= 3 && li < 35 && lj >= 3 && lj < 35 && i < f.nx && j < f.ny) {
+        xr.x_new[i + j * f.nx] = is_act(fq) ? xc : 0.0;
+        f.r[i + j * f.nx] = rv;
+      }
+    }
+    write_part<NT>(acc, xr.part_rr, sm);
+    if (blockIdx.x == 0 && threadIdx.x == 0) chk.sc->spmv = it;
+  } else {
+    // every load of the tile is issued before the first LDS store (a load outside the
+    // grid reads cell 0 and is replaced by the boundary value)
+    constexpr int NC = (WC * WC + NT - 1) / NT, NF = (W3 * W3 + NT - 1) / NT;
+    uint8_t cv[NC], fv[NF];
+    float rv[NF];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      const int t = threadIdx.x + k * NT;
+      const int I = bx * 16 + t % WC - 1, J = by * 16 + t / WC - 1;
+      const bool in = t < WC * WC && I >= 0 && I < c.nx && J >= 0 && J < c.ny;
+      const uint8_t v = c.inf[in ? I + J * c.nx : 0];
+      cv[k] = in ? v : kWall;
+    }
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      const int t = threadIdx.x + k * NT;
+      const int i = fi0 + t % W3 - 3, j = fj0 + t / W3 - 3;
+      const bool in = t < W3 * W3 && i >= 0 && i < f.nx && j >= 0 && j < f.ny;
+      const int q = in ? i + j * f.nx : 0;
+      const uint8_t fq = f.inf[q];
+      const float rq = f.r[q];
+      fv[k] = in ? fq : 0;
+      rv[k] = in ? rq : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      const int t = threadIdx.x + k * NT;
+      if (t < WC * WC) s_cinf[t] = cv[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      const int t = threadIdx.x + k * NT;
+      if (t < W3 * W3) {
+        s_inf[t] = fv[k];
+        s_r[t] = rv[k];
+        s_x1[t] = is_act(fv[k]) ? omega * rv[k] * inv_nns(fv[k]) : 0.f;
+      }
     }
   }
   __syncthreads();
@@ -1662,10 +1785,20 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
         m.nx[0], m.ny[0], m.inf[0], m.nx[1], m.ny[1], m.inf[1], m.tflag);
 }
 
+static int mg_down_blocks(const MgHost& m, int l) {
+  return ((m.nx[l + 1] + 15) / 16) * ((m.ny[l + 1] + 15) / 16);
+}
+
 MgCheck mg_check_args(const MgHost& m, int it, int max_iter, double tol, CgScalars* sc) {
   const int nb = vec_blocks(m.nx[0] * m.ny[0]);
-  // |r|^2: k_mg_setup1's partials (it == 0) or k_mg_xr's; |b'|^2 from k_mg_setup1
-  return MgCheck{sc, m.part_rr, nb, m.part_bb, nb, tol * tol, max_iter, m.per_scene ? 1 : 0};
+  // |r|^2: k_mg_setup1's partials (it == 0), else k_mg_xr's or, fused, level 0's down
+  // kernel's (one per tile); |b'|^2 from k_mg_setup1
+  const int n_rr = (m.fused && it > 0) ? mg_down_blocks(m, 0) : nb;
+  return MgCheck{sc, m.part_rr, n_rr, m.part_bb, nb, tol * tol, max_iter, m.per_scene ? 1 : 0};
+}
+
+int mg_part_rr_size(const MgHost& m) {
+  return std::max(kMgMaxVecBlocks, m.nlev > 1 ? mg_down_blocks(m, 0) : 1);
 }
 
 static MgLevel mg_level(const MgHost& m, int l) {
@@ -1677,16 +1810,27 @@ static MgLevel mg_level(const MgHost& m, int l) {
 // of this iteration first.
 void launch_mg_down(hipStream_t s, const MgHost& m, int l, int it, int max_iter, double tol,
                     CgScalars* sc) {
-  const int blocks = ((m.nx[l + 1] + 15) / 16) * ((m.ny[l + 1] + 15) / 16);
+  const int blocks = mg_down_blocks(m, l);
   const MgCheck chk = mg_check_args(m, it, max_iter, tol, sc);
+  // the stop test: level 0's kernel, or level 1's in the fused form (level 0 forms
+  // the residual it tests)
+  const int check = l == (m.fused ? 1 : 0) ? 1 : 0;
   // level 0: 256-thread blocks (many tiles); smaller levels: 1024 threads, so that
   // each LDS phase of the few tiles takes one or two passes
-  if (l == 0)
+  if (l == 0 && m.fused) {
+    // iteration it - 1's update: the buffers k_mg_xr would use at that iteration
+    const MgXr xr{m.x[(it + 1) & 1], m.x[it & 1], m.p[it & 1], m.b, m.part_pq,
+                  vec_blocks(m.nx[0] * m.ny[0]), m.part_rr};
+    k_mg_down<true, kT, true><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1),
+                                                    m.coarse.omega, it, chk, 0, m.tflag, xr);
+  } else if (l == 0) {
     k_mg_down<true, kT><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega, it,
-                                              chk, m.tflag);
-  else
+                                              chk, check, m.tflag, MgXr{});
+  } else {
     k_mg_down<false, kTileBig><<<blocks, kTileBig, 0, s>>>(mg_level(m, l), mg_level(m, l + 1),
-                                                           m.coarse.omega, it, chk, nullptr);
+                                                           m.coarse.omega, it, chk, check,
+                                                           nullptr, MgXr{});
+  }
 }
 
 void launch_mg_coarse(hipStream_t s, const MgHost& m, int it, int max_iter, double tol,
