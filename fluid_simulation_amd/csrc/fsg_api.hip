@@ -453,6 +453,8 @@ fsg::SceneCoarseArgs scene_coarse_args(const fsg_domain* h) {
 // CG preconditioned by one V-cycle per iteration (FSG_SOLVER_MG).  Iterations are
 // enqueued in batches (the first sized by the previous solve's count); every
 // kernel returns at once after the device-side stop test has fired.
+// F < 0: the number of unknowns is still on the device (h->n_fluid); it is read back
+// with the stop-test copy, so the solve costs no extra host round trip.
 int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
   if (int rc = mg_prepare(h)) return rc;
   MgHost& m = h->mg;
@@ -503,10 +505,13 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
       }
     }
     HIPCHK(h, hipMemcpyAsync(&hs, h->sc, sizeof(hs), hipMemcpyDeviceToHost, h->stream));
+    if (F < 0) HIPCHK(h, hipMemcpyAsync(&F, h->n_fluid, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     if (hs.done || it > max_iter) break;
     batch = 2;
   }
+  h->last_F = F;
+  h->host_stats.n_fluid = F;
   timed(h, "cg_mg_finish",
         [&] {
           fsg::MgBatchScale bs;
@@ -531,19 +536,22 @@ int solve_pressure(fsg_domain* h) {
     launch_compact(h->stream, h->g, h->label, h->offsets, h->fluid_index, h->cell_of, UF(h), VF(h),
                    h->b, h->info);
   });
-  // the launch geometry of both solver forms depends on the number of unknowns
-  int F = 0;
-  HIPCHK(h, hipMemcpyAsync(&F, h->n_fluid, sizeof(int), hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
   h->mg_solved = false;
-  h->host_stats.n_fluid = F;
   h->host_stats.cg_iterations = 0;
   h->host_stats.cg_error = 0.0;
   h->host_stats.refine_sweeps = 0;
+  const int solver = h->prm.solver;
+  // the multigrid form's launches do not depend on the number of unknowns (a system
+  // without any stops at its first test, and the pressure update reads the count on
+  // the device); it reads the count back with its stop test
+  if (solver == FSG_SOLVER_MG) return mg_solve(h, -1);
+  // the launch geometry of the other forms depends on the number of unknowns
+  int F = 0;
+  HIPCHK(h, hipMemcpyAsync(&F, h->n_fluid, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->host_stats.n_fluid = F;
   h->last_F = F;
   if (F == 0) return FSG_OK;
-  const int solver = h->prm.solver;
-  if (solver == FSG_SOLVER_MG) return mg_solve(h, F);
   if (solver == FSG_SOLVER_PLAIN) {
     const CgParams cp = make_cg_params(h, (float)h->prm.cg_tol);
     timed(h, "cg_init", [&] {
