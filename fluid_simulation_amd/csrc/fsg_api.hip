@@ -197,7 +197,7 @@ struct fsg_domain {
   size_t emit_cap = 0;
   AdjState adj;
   std::vector<Checkpoint> ckpt;
-  bool kclock_on = false;  // k_mg_coarse's in-kernel clock (fsg_set_kernel_clock)
+  int kclock = 0;  // fsg_set_kernel_clock: 1 k_mg_coarse, 2 level 0's down kernel
   bool mg_ready = false;
   bool mg_solved = false;  // the last solve was the multigrid form (its FP64 x is valid)
   int mg_last_iters = 0;
@@ -458,7 +458,16 @@ fsg::SceneCoarseArgs scene_coarse_args(const fsg_domain* h) {
 int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
   if (int rc = mg_prepare(h)) return rc;
   MgHost& m = h->mg;
-  m.clk_on = h->kclock_on;
+  m.clk_on = h->kclock == 1;
+  m.clk_span_on = h->kclock == 2;
+  if (m.clk_span_on && !m.clk_span) {
+    // room for kSpanLaunches launches of level 0's down kernel (held until destroy)
+    constexpr int kSpanLaunches = 4096;
+    m.clk_span_nb = mg_down_blocks0(m);
+    if (m.clk_span_nb > 0 &&
+        dalloc(h, &m.clk_span, 2 * (size_t)m.clk_span_nb * kSpanLaunches) == hipSuccess)
+      m.clk_span_cap = kSpanLaunches;
+  }
   timed(h, "cg_mg_setup", [&] {
     fsg::MgBatchScale bs;
     if (h->batch) bs = h->batch->scale();
@@ -1865,7 +1874,10 @@ int fsg_adjoint_step(fsg_domain* h, int32_t scheme, float dt) {
 
 int fsg_set_kernel_clock(fsg_domain* h, int32_t enable) {
   if (!h) return FSG_E_INVALID;
-  h->kclock_on = enable != 0;
+  if (enable < 0 || enable > 2)
+    return fail(h, FSG_E_INVALID, "fsg_set_kernel_clock: enable is 0, 1 or 2");
+  h->kclock = enable;
+  h->mg.clk_span_n = 0;
   if (h->mg.clk) {
     HIPCHK(h, hipMemsetAsync(h->mg.clk, 0, 2 * sizeof(unsigned long long), h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1877,15 +1889,39 @@ int fsg_get_kernel_clock(fsg_domain* h, double* us_total, int64_t* launches) {
   if (!h || !us_total || !launches) return FSG_E_INVALID;
   *us_total = 0.0;
   *launches = 0;
-  if (!h->mg.clk) return FSG_OK;
-  unsigned long long v[2] = {0, 0};
-  HIPCHK(h, hipMemcpyAsync(v, h->mg.clk, sizeof(v), hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
   int khz = 0;
   HIPCHK(h, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, h->device));
   if (khz <= 0) return fail(h, FSG_E_HIP, "fsg_get_kernel_clock: no device wall clock rate");
-  *us_total = (double)v[0] * 1000.0 / (double)khz;
-  *launches = (int64_t)v[1];
+  unsigned long long ticks = 0;
+  int64_t n = 0;
+  if (h->kclock == 2) {
+    // per launch: latest block end minus earliest block start
+    const MgHost& m = h->mg;
+    if (m.clk_span && m.clk_span_n > 0) {
+      std::vector<unsigned long long> v(2 * (size_t)m.clk_span_nb * m.clk_span_n);
+      HIPCHK(h, hipMemcpyAsync(v.data(), m.clk_span, v.size() * sizeof(v[0]),
+                               hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      for (int k = 0; k < m.clk_span_n; ++k) {
+        const unsigned long long* b = v.data() + 2 * (size_t)m.clk_span_nb * k;
+        unsigned long long t0 = ~0ull, t1 = 0;
+        for (int q = 0; q < m.clk_span_nb; ++q) {
+          t0 = std::min(t0, b[2 * q]);
+          t1 = std::max(t1, b[2 * q + 1]);
+        }
+        ticks += t1 - t0;
+      }
+      n = m.clk_span_n;
+    }
+  } else if (h->mg.clk) {
+    unsigned long long v[2] = {0, 0};
+    HIPCHK(h, hipMemcpyAsync(v, h->mg.clk, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    ticks = v[0];
+    n = (int64_t)v[1];
+  }
+  *us_total = (double)ticks * 1000.0 / (double)khz;
+  *launches = n;
   return FSG_OK;
 }
 

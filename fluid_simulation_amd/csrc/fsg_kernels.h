@@ -200,6 +200,12 @@ struct MgHost {
   bool per_scene = false;  // a batch: stop test and CG scalars per scene (MgbArgs)
   unsigned long long* clk = nullptr;  // k_mg_coarse's own clock: ticks, launches
   bool clk_on = false;
+  // level 0's down kernel clock: per recorded launch and block, (start, end) ticks
+  unsigned long long* clk_span = nullptr;
+  int clk_span_cap = 0;      // launches the buffer holds
+  int clk_span_nb = 0;       // blocks per launch
+  int clk_span_n = 0;        // launches recorded
+  bool clk_span_on = false;
   uint8_t* tflag = nullptr;  // level-0 tiles (32 x 32) holding an unknown on level 0 or 1
   // fused form (one domain, two or more tile levels): level 0's down kernel also does
   // the previous iteration's x and residual update, level 1's runs the stop test
@@ -251,8 +257,10 @@ void launch_mgb_vectors(hipStream_t s, const MgbArgs& a, const MgHost& m, int it
 // removes the plain mean of the FP64 solution over the unknowns (adjoint solves)
 void launch_mg_remove_mean(hipStream_t s, const MgHost& m);
 // one CG iteration: down(l) for l < lc, coarse, up(l) for l = lc-1 .. 0, dir, xr
-void launch_mg_down(hipStream_t s, const MgHost& m, int l, int it, int max_iter, double tol,
+// (launches through launch_mg_down record level 0's block spans while clk_span_on)
+void launch_mg_down(hipStream_t s, MgHost& m, int l, int it, int max_iter, double tol,
                     CgScalars* sc);
+int mg_down_blocks0(const MgHost& m);  // blocks of level 0's down kernel
 void launch_mg_coarse(hipStream_t s, const MgHost& m, int it, int max_iter, double tol,
                       CgScalars* sc);
 // ---- adjoint (reverse-mode) kernels of the PIC / FLIP step, kernels_adj.hip

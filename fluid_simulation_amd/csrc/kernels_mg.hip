@@ -508,15 +508,37 @@ struct MgXr {
   double* part_rr;  // out: one partial |r|^2 per block
 };
 
+// Block span clock (span != null): thread 0 stores the block's start and end ticks
+// of the device wall clock in span[2 b], span[2 b + 1]; the launch's duration is the
+// latest end minus the earliest start over its blocks.
+struct BlockSpan {
+  unsigned long long* span;
+  unsigned long long t0;
+  __device__ __forceinline__ explicit BlockSpan(unsigned long long* s)
+      : span(s), t0(s ? wall_clock64() : 0ull) {}
+  __device__ __forceinline__ void end() const {
+    if (span && threadIdx.x == 0) {
+      span[2 * blockIdx.x] = t0;
+      span[2 * blockIdx.x + 1] = wall_clock64();
+    }
+  }
+};
+
 template <bool L0, int NT, bool XR = false>
 __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omega, int it,
                                                  MgCheck chk, int check,
-                                                 const uint8_t* __restrict__ tflag, MgXr xr) {
+                                                 const uint8_t* __restrict__ tflag, MgXr xr,
+                                                 unsigned long long* span) {
   static_assert(!XR || L0, "the fused update belongs to level 0");
+  const BlockSpan clock(span);
   __shared__ double sm[NT / 64];
   if (check) {
-    if (mg_check<NT>(chk, it, sm)) return;
+    if (mg_check<NT>(chk, it, sm)) {
+      clock.end();
+      return;
+    }
   } else if (chk.sc->done) {
+    clock.end();
     return;
   }
   constexpr int W3 = 38, W2 = 36, W1 = 34;
@@ -557,6 +579,7 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
         }
       if (threadIdx.x == 0) xr.part_rr[blockIdx.x] = 0.0;
     }
+    clock.end();
     return;
   }
   if (do_xr) {
@@ -747,6 +770,10 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
       }
       c.r[cc] = v;
     }
+  }
+  if (span) {
+    __syncthreads();
+    clock.end();
   }
 }
 
@@ -1808,9 +1835,15 @@ static MgLevel mg_level(const MgHost& m, int l) {
 // One CG iteration = V-cycle (z = B r) + direction + update.  Level 0's down kernel
 // (or the one-workgroup kernel when the hierarchy starts there) runs the stop test
 // of this iteration first.
-void launch_mg_down(hipStream_t s, const MgHost& m, int l, int it, int max_iter, double tol,
+int mg_down_blocks0(const MgHost& m) { return m.nlev > 1 ? mg_down_blocks(m, 0) : 0; }
+
+void launch_mg_down(hipStream_t s, MgHost& m, int l, int it, int max_iter, double tol,
                     CgScalars* sc) {
   const int blocks = mg_down_blocks(m, l);
+  unsigned long long* span = nullptr;
+  if (l == 0 && m.clk_span_on && m.clk_span && m.clk_span_n < m.clk_span_cap &&
+      m.clk_span_nb == blocks)
+    span = m.clk_span + 2 * (size_t)blocks * m.clk_span_n++;
   const MgCheck chk = mg_check_args(m, it, max_iter, tol, sc);
   // the stop test: level 0's kernel, or level 1's in the fused form (level 0 forms
   // the residual it tests)
@@ -1822,14 +1855,15 @@ void launch_mg_down(hipStream_t s, const MgHost& m, int l, int it, int max_iter,
     const MgXr xr{m.x[(it + 1) & 1], m.x[it & 1], m.p[it & 1], m.b, m.part_pq,
                   vec_blocks(m.nx[0] * m.ny[0]), m.part_rr};
     k_mg_down<true, kT, true><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1),
-                                                    m.coarse.omega, it, chk, 0, m.tflag, xr);
+                                                    m.coarse.omega, it, chk, 0, m.tflag, xr,
+                                                    span);
   } else if (l == 0) {
     k_mg_down<true, kT><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega, it,
-                                              chk, check, m.tflag, MgXr{});
+                                              chk, check, m.tflag, MgXr{}, span);
   } else {
     k_mg_down<false, kTileBig><<<blocks, kTileBig, 0, s>>>(mg_level(m, l), mg_level(m, l + 1),
                                                            m.coarse.omega, it, chk, check,
-                                                           nullptr, MgXr{});
+                                                           nullptr, MgXr{}, nullptr);
   }
 }
 

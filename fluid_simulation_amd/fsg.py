@@ -416,12 +416,14 @@ class Domain:
         return d
 
 
-    def set_kernel_clock(self, enable: bool):
-        self._chk(self._L.fsg_set_kernel_clock(self._h, 1 if enable else 0),
-                  "fsg_set_kernel_clock")
+    def set_kernel_clock(self, enable: bool, kernel: str = "coarse"):
+        """Device-clock timing of k_mg_coarse (``kernel="coarse"``) or of level 0's
+        down kernel (``"down0"``: earliest block start to latest block end)."""
+        code = {"coarse": 1, "down0": 2}[kernel] if enable else 0
+        self._chk(self._L.fsg_set_kernel_clock(self._h, code), "fsg_set_kernel_clock")
 
     def kernel_clock(self):
-        """(microseconds, launches) of k_mg_coarse by its own device clock."""
+        """(microseconds, launches) of the clocked kernel by the device clock."""
         us, n = C.c_double(), C.c_int64()
         self._chk(self._L.fsg_get_kernel_clock(self._h, C.byref(us), C.byref(n)),
                   "fsg_get_kernel_clock")

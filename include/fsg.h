@@ -274,10 +274,13 @@ int fsg_adjoint_seed_density(fsg_domain *h, const float *rho_bar);
    kernel, 2 only the kernels whose timing name starts with the prefix set by
    fsg_set_timing_prefix (default "cg_": the pressure-solver kernels). */
 int fsg_set_timing(fsg_domain *h, int32_t enable);
-/* The one-workgroup multigrid kernel's own clock (no counterpart in the reference):
-   its start-to-end time per launch from the device wall clock, free of the host's
-   launch gaps that HIP events around a launch include.  enable != 0 clears and
-   starts it, 0 stops it; the total covers every launch since, early exits included. */
+/* A multigrid kernel's own clock (no counterpart in the reference): its duration
+   per launch from the device wall clock, free of the launch gaps that HIP events
+   around a launch include.  enable = 1: the one-workgroup kernel (start to end);
+   enable = 2: level 0's down kernel (earliest block start to latest block end, the
+   first 4096 launches; the span buffer is kept until fsg_destroy); 0 stops it.  A
+   nonzero value clears the total, which then covers every launch, early exits
+   included. */
 int fsg_set_kernel_clock(fsg_domain *h, int32_t enable);
 int fsg_get_kernel_clock(fsg_domain *h, double *us_total, int64_t *launches);
 /* Name prefix selected by timing mode 2 (at most 63 characters). */
