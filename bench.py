@@ -92,19 +92,24 @@ def kernel_bytes(name, nx, ny, P, levels=None):
 
 
 # HBM traffic per launch on the 1024^2 dam break (bytes), from two rocprofv3 --pmc
-# passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) of
-#   bench.py --steps 5 --warmup 2 --plain-steps 0 --profile-steps 1 --no-fp64-check --no-cpu
+# passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, averaged over the run's
+# dispatches) of
+#   bench.py --steps 5 --warmup 2 --plain-steps 0 --profile-steps 1 --no-fp64-check
+#            --no-cpu --batch-scenes 0 --grid-n 0 --adj-n 0
 # scaled by a 1 GiB device copy (bench.py --copy-probe 1024) read by the same counters:
 # FETCH_SIZE reads 0.5 GiB for its 1 GiB read (factor 2.0), WRITE_SIZE 1 GiB (1.0).
 # traffic = (2.0 * FETCH_SIZE + 1.0 * WRITE_SIZE) * 1024.
 MEASURED_TRAFFIC_1024 = {
-    "cg_mg_coarse": (2.0 * 13.6 + 3.6) * 1024,
-    "cg_mg_down0": (2.0 * 2586.1 + 4602.3) * 1024,
-    "cg_mg_up0": (2.0 * 4890.8 + 3710.6) * 1024,
-    "cg_mg_xr": (2.0 * 4242.4 + 11074.2) * 1024,
-    "cg_mg_dir": (2.0 * 2350.7 + 3710.6) * 1024,
-    "particle_scatter": (2.0 * 8038.9 + 39819.9) * 1024,
+    "cg_mg_coarse": (2.0 * 12.1 + 3.6) * 1024,
+    "cg_mg_down0": (2.0 * 2919.2 + 5835.8) * 1024,
+    "cg_mg_up0": (2.0 * 1381.9 + 1238.3) * 1024,
+    "cg_mg_dir": (2.0 * 7036.7 + 3710.6) * 1024,
+    "particle_scatter": (2.0 * 8039.0 + 39819.9) * 1024,
 }
+
+
+# timers that wrap more than one kernel (or one kernel at several levels)
+MULTI_KERNEL_TIMERS = {"cg_mg_setup", "cg_mg_finish", "cg_mg_down3+", "cg_mg_up3+"}
 
 
 def log(*a):
@@ -474,6 +479,9 @@ def main():
     ap.add_argument("--plain-tol", type=float, default=1e-6)
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--roofline-kernel", default=None,
+                    help="timer name of the roofline kernel (default: the one with the most "
+                         "time per step in the profile pass)")
     ap.add_argument("--warm-start", type=int, default=0,
                     help="chosen form: 1 starts each solve from the previous pressure")
     ap.add_argument("--batch-scenes", type=int, default=256)
@@ -527,7 +535,10 @@ def main():
     run_steps(fsg, d, args.warmup, dt)
     prof = kernel_profile(fsg, d, args.profile_steps, dt) if args.profile_steps > 0 else {}
     per_step = {k: ms * n for k, (ms, n) in prof.items()}
-    dominant = max(per_step, key=per_step.get) if per_step else None
+    # the roofline kernel: the one kernel with the most time per step (timers that
+    # cover several kernels or levels are not candidates)
+    single = {k: v for k, v in per_step.items() if k not in MULTI_KERNEL_TIMERS}
+    dominant = args.roofline_kernel or (max(single, key=single.get) if single else None)
     log("kernel profile (ms per step): " +
         ", ".join(f"{k} {v:.3f}" for k, v in sorted(per_step.items(), key=lambda kv: -kv[1])[:12]))
     elapsed, st, tm = timed_segment(fsg, d, args.steps, dt, kernel=dominant)
