@@ -1637,12 +1637,18 @@ __global__ void __launch_bounds__(kT) k_mgb_dir(MgbArgs a, const uint8_t* __rest
 #pragma unroll
   for (int k = 0; k < kMgbRows; ++k) {
     const int c = m.c[k];
-    const bool act = is_act(m.f[k]);
-    const int q[5] = {c, act ? c - 1 : c, act ? c + 1 : c, act ? c - W : c, act ? c + W : c};
+    const int q[5] = {c, c - 1, c + 1, c - W, c + W};
 #pragma unroll
     for (int e = 0; e < 5; ++e) {
-      zv[k][e] = z[q[e]];
-      pv[k][e] = first ? 0.f : p_old[q[e]];
+      zv[k][e] = 0.f;
+      pv[k][e] = 0.f;
+    }
+    if (is_act(m.f[k])) {
+#pragma unroll
+      for (int e = 0; e < 5; ++e) {
+        zv[k][e] = z[q[e]];
+        if (!first) pv[k][e] = p_old[q[e]];
+      }
     }
   }
   double acc = 0.0;
@@ -1679,40 +1685,24 @@ __global__ void __launch_bounds__(kT) k_mgb_xr(MgbArgs a, const uint8_t* __restr
   const int W = a.W;
   const double pq = a.pq[t.s];
   const double alpha = pq != 0.0 ? a.rz[(it & 1) * a.n_scenes + t.s] / pq : 0.0;
-  const MgbCells m = mgb_cells(t, W, inf);
-  double xv[kMgbRows][5], bv[kMgbRows];
-  float pv[kMgbRows][5];
-#pragma unroll
-  for (int k = 0; k < kMgbRows; ++k) {
-    const int c = m.c[k];
-    const bool act = is_act(m.f[k]);
-    const int q[5] = {c, act ? c - 1 : c, act ? c + 1 : c, act ? c - W : c, act ? c + W : c};
-    bv[k] = b[c];
-#pragma unroll
-    for (int e = 0; e < 5; ++e) {
-      xv[k][e] = x_old[q[e]];
-      pv[k][e] = p[q[e]];
-    }
-  }
   double acc = 0.0;
-#pragma unroll
-  for (int k = 0; k < kMgbRows; ++k) {
-    if (!m.in[k]) break;
-    double xn = 0.0;
-    float rv = 0.f;
-    if (is_act(m.f[k])) {
-      double xs[5];
-#pragma unroll
-      for (int e = 0; e < 5; ++e) xs[e] = xv[k][e] + alpha * (double)pv[k][e];
-      xn = xs[0];
-      const double sn = xs[1] + xs[2] + xs[3] + xs[4];
-      const double r64 = bv[k] - ((double)(m.f[k] & 7) * xn - sn);
-      rv = (float)r64;
-      acc += r64 * r64;
+  if (t.col)
+    for (int k = 0, j = t.j0; k < kMgbRows && j < t.jend; ++k, j += 4) {
+      const int c = t.i + j * W;
+      const uint8_t f = inf[c];
+      double xn = 0.0;
+      float rv = 0.f;
+      if (is_act(f)) {
+        auto xv = [&](int q) { return x_old[q] + alpha * (double)p[q]; };
+        xn = xv(c);
+        const double sn = xv(c - 1) + xv(c + 1) + xv(c - W) + xv(c + W);
+        const double r64 = b[c] - ((double)(f & 7) * xn - sn);
+        rv = (float)r64;
+        acc += r64 * r64;
+      }
+      x_new[c] = xn;
+      r[c] = rv;
     }
-    x_new[m.c[k]] = xn;
-    r[m.c[k]] = rv;
-  }
   mgb_add(acc, &a.rr[t.s], sm);
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->spmv = it + 1;
 }
