@@ -95,17 +95,20 @@ struct BatchInfo {
   int stride = 0;
   // per-scene CG: 64 x 16 vector tiles inside the scenes, and the scalars (5 S doubles)
   int4* d_vtile = nullptr;
+  int2* d_vend = nullptr;  // per vector tile: its scene's column and row ends
   int n_vtile = 0;
   double* d_cg = nullptr;
   uint8_t* d_vflag = nullptr;
   fsg::BatchTables tables() const { return fsg::BatchTables{d_scene, d_ptile, n_ptile}; }
-  fsg::MgbArgs cg_args(int W) const {
+  fsg::MgbArgs cg_args(int W, int H) const {
     fsg::MgbArgs a;
     a.tile = d_vtile;
+    a.tend = d_vend;
     a.rect = d_rect;
     a.n_tiles = n_vtile;
     a.n_scenes = S;
     a.W = W;
+    a.C = W * H;
     a.rz = d_cg;
     a.pq = d_cg + 2 * S;
     a.rr = d_cg + 3 * S;
@@ -476,7 +479,7 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
                     h->batch ? &bs : nullptr, adj_rhs);
     if (h->batch) {
       launch_mg_scene_setup(h->stream, scene_coarse_args(h));
-      launch_mgb_init(h->stream, h->batch->cg_args(h->g.nx), h->batch->d_norm, m);
+      launch_mgb_init(h->stream, h->batch->cg_args(h->g.nx, h->g.ny), h->batch->d_norm, m);
     }
   });
   const int max_iter = h->prm.cg_max_iter;
@@ -491,7 +494,7 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
     for (int q = 0; q < batch && it <= max_iter; ++q, ++it) {
       if (h->batch)
         timed(h, "cg_mg_check", [&] {
-          launch_mgb_check(h->stream, h->batch->cg_args(h->g.nx), it, max_iter, tol, h->sc);
+          launch_mgb_check(h->stream, h->batch->cg_args(h->g.nx, h->g.ny), it, max_iter, tol, h->sc);
         });
       for (int l = 0; l < m.lc; ++l)
         timed(h, kDownNames[std::min(l, 3)],
@@ -503,7 +506,7 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
       for (int l = m.lc - 1; l >= 0; --l)
         timed(h, kUpNames[std::min(l, 3)], [&] { launch_mg_up(h->stream, m, l, it, h->sc); });
       if (h->batch) {
-        const fsg::MgbArgs ba = h->batch->cg_args(h->g.nx);
+        const fsg::MgbArgs ba = h->batch->cg_args(h->g.nx, h->g.ny);
         timed(h, "cg_mg_rz", [&] { launch_mgb_vectors(h->stream, ba, m, it, h->sc, 0); });
         timed(h, "cg_mg_dir", [&] { launch_mgb_vectors(h->stream, ba, m, it, h->sc, 1); });
         timed(h, "cg_mg_xr", [&] { launch_mgb_vectors(h->stream, ba, m, it, h->sc, 2); });
@@ -1497,11 +1500,16 @@ int fsg_batch_create(fsg_domain** out, int32_t n_scenes, const int32_t* nx, cons
   ok &= dalloc(h, &b->d_coarse, (size_t)n_scenes) == hipSuccess;
   ok &= dalloc(h, &b->d_sinf, (size_t)n_scenes * b->stride) == hipSuccess;
   std::vector<int4> vtile;
+  std::vector<int2> vend;
   for (int s = 0; s < n_scenes; ++s)
     for (int y0 = 0; y0 < ny[s]; y0 += 16)
-      for (int x0 = 0; x0 < nx[s]; x0 += 64) vtile.push_back(make_int4(ox[s] + x0, oy[s] + y0, s, 0));
+      for (int x0 = 0; x0 < nx[s]; x0 += 64) {
+        vtile.push_back(make_int4(ox[s] + x0, oy[s] + y0, s, 0));
+        vend.push_back(make_int2(ox[s] + nx[s], oy[s] + ny[s]));
+      }
   b->n_vtile = (int)vtile.size();
   ok &= dalloc(h, &b->d_vtile, vtile.size()) == hipSuccess;
+  ok &= dalloc(h, &b->d_vend, vend.size()) == hipSuccess;
   ok &= dalloc(h, &b->d_cg, (size_t)5 * n_scenes) == hipSuccess;
   ok &= dalloc(h, &b->d_vflag, vtile.size()) == hipSuccess;
   if (!ok) return bail(FSG_E_NOMEM, "");
@@ -1514,6 +1522,8 @@ int fsg_batch_create(fsg_domain** out, int32_t n_scenes, const int32_t* nx, cons
       hipMemcpyAsync(b->d_coarse, b->coarse.data(), sizeof(fsg::SceneCoarse) * n_scenes,
                      hipMemcpyHostToDevice, h->stream) != hipSuccess ||
       hipMemcpyAsync(b->d_vtile, vtile.data(), sizeof(int4) * vtile.size(),
+                     hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+      hipMemcpyAsync(b->d_vend, vend.data(), sizeof(int2) * vend.size(),
                      hipMemcpyHostToDevice, h->stream) != hipSuccess ||
       hipStreamSynchronize(h->stream) != hipSuccess)
     return bail(FSG_E_HIP, "");

@@ -187,8 +187,9 @@ struct SceneCoarseArgs {
 // Packed batch: per-scene CG scalars (one CG per scene in lock step).
 struct MgbArgs {
   const int4* tile = nullptr;  // (x0, y0, scene, 0): 64 x 16 tiles that lie in one scene
+  const int2* tend = nullptr;  // per tile: its scene's column and row ends (exclusive)
   const int4* rect = nullptr;  // per scene: (x, y, nx, ny) in the packed grid
-  int n_tiles = 0, n_scenes = 0, W = 0;
+  int n_tiles = 0, n_scenes = 0, W = 0, C = 0;  // C: cells of the packed grid
   double* rz = nullptr;  // [2][S] r.z by iteration parity
   double* pq = nullptr;  // [S]
   double* rr = nullptr;  // [S]

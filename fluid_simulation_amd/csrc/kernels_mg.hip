@@ -1488,16 +1488,19 @@ struct MgbTile {
   bool live;           // the tile holds an unknown (tiles without one skip their work)
 };
 
+// The tile's three table entries are independent loads (one memory round trip); the
+// kernels read them before their stop-flag test so that the two waits overlap.
 __device__ __forceinline__ MgbTile mgb_tile(const MgbArgs& a) {
   const int4 t = a.tile[blockIdx.x];
-  const int4 r = a.rect[t.z];
+  const int2 e = a.tend[blockIdx.x];
+  const uint8_t vf = a.vflag ? a.vflag[blockIdx.x] : 1;
   MgbTile m;
   m.s = t.z;
   m.i = t.x + (threadIdx.x & 63);
   m.j0 = t.y + (threadIdx.x >> 6);
-  m.jend = r.y + r.w;
-  m.col = m.i < r.x + r.z;
-  m.live = a.vflag == nullptr || a.vflag[blockIdx.x] != 0;
+  m.jend = e.y;
+  m.col = m.i < e.x;
+  m.live = vf != 0;
   return m;
 }
 
@@ -1598,8 +1601,8 @@ __global__ void __launch_bounds__(kT) k_mgb_rz(MgbArgs a, const uint8_t* __restr
                                                const float* __restrict__ z, int it,
                                                const CgScalars* sc) {
   __shared__ double sm[kT / 64];
-  if (sc->done) return;
   const MgbTile t = mgb_tile(a);
+  if (sc->done) return;
   if (!t.live) return;  // r = z = 0 there
   const MgbCells m = mgb_cells(t, a.W, inf);
   float rv[kMgbRows], zv[kMgbRows];
@@ -1621,47 +1624,48 @@ __global__ void __launch_bounds__(kT) k_mgb_dir(MgbArgs a, const uint8_t* __rest
                                                 const float* __restrict__ p_old,
                                                 float* __restrict__ p_new, int it,
                                                 const CgScalars* sc) {
+  // p on the tile and a 1-cell halo, staged in LDS: every z and p_old value is read from
+  // memory once per tile, the stencil reads LDS
+  constexpr int LW = 64 + 2, LH = 4 * kMgbRows + 2, NL = (LW * LH + kT - 1) / kT;
   __shared__ double sm[kT / 64];
-  if (sc->done) return;
+  __shared__ float s_p[LW * LH];
   const MgbTile t = mgb_tile(a);
+  if (sc->done) return;
   // a tile without unknowns writes p = 0 into both ping-pong buffers (iterations 0, 1)
   if (!t.live && it >= 2) return;
   const int S = a.n_scenes, W = a.W;
-  const double rzp = it == 0 ? 0.0 : a.rz[((it - 1) & 1) * S + t.s];
-  const float bf = rzp != 0.0 ? (float)(a.rz[(it & 1) * S + t.s] / rzp) : 0.f;
+  const int x0 = t.i - (int)(threadIdx.x & 63), y0 = t.j0 - (int)(threadIdx.x >> 6);
   const bool first = it == 0;
   const MgbCells m = mgb_cells(t, W, inf);
-  // neighbours of an unknown lie in its scene (its border is SOLID: z = p = 0 there);
-  // other cells read their own value only
-  float zv[kMgbRows][5], pv[kMgbRows][5];
+  float zv[NL], pv[NL];
 #pragma unroll
-  for (int k = 0; k < kMgbRows; ++k) {
-    const int c = m.c[k];
-    const int q[5] = {c, c - 1, c + 1, c - W, c + W};
-#pragma unroll
-    for (int e = 0; e < 5; ++e) {
-      zv[k][e] = 0.f;
-      pv[k][e] = 0.f;
-    }
-    if (is_act(m.f[k])) {
-#pragma unroll
-      for (int e = 0; e < 5; ++e) {
-        zv[k][e] = z[q[e]];
-        if (!first) pv[k][e] = p_old[q[e]];
-      }
-    }
+  for (int e = 0; e < NL; ++e) {
+    // halo cells outside the packed grid read a clamped cell; no unknown uses them
+    const int q = threadIdx.x + e * kT;
+    const int g = (x0 - 1 + q % LW) + (y0 - 1 + q / LW) * W;
+    const int gc = g < 0 ? 0 : (g >= a.C ? a.C - 1 : g);
+    zv[e] = z[gc];
+    pv[e] = first ? 0.f : p_old[gc];
   }
+  const double rzp = first ? 0.0 : a.rz[((it - 1) & 1) * S + t.s];
+  const double rzc = a.rz[(it & 1) * S + t.s];
+  const float bf = rzp != 0.0 ? (float)(rzc / rzp) : 0.f;
+#pragma unroll
+  for (int e = 0; e < NL; ++e) {
+    const int q = threadIdx.x + e * kT;
+    if (q < LW * LH) s_p[q] = first ? zv[e] : zv[e] + bf * pv[e];
+  }
+  __syncthreads();
   double acc = 0.0;
 #pragma unroll
   for (int k = 0; k < kMgbRows; ++k) {
     if (!m.in[k]) break;
     float pc = 0.f;
     if (is_act(m.f[k])) {
-      float pn[5];
-#pragma unroll
-      for (int e = 0; e < 5; ++e) pn[e] = first ? zv[k][e] : zv[k][e] + bf * pv[k][e];
-      pc = pn[0];
-      const float sn = pn[1] + pn[2] + pn[3] + pn[4];
+      // neighbours of an unknown lie in its scene (its border is SOLID: z = p = 0 there)
+      const int l = (int)(threadIdx.x & 63) + 1 + ((int)(threadIdx.x >> 6) + 4 * k + 1) * LW;
+      pc = s_p[l];
+      const float sn = s_p[l - 1] + s_p[l + 1] + s_p[l - LW] + s_p[l + LW];
       const float kp = nns_of(m.f[k]) * pc - sn;
       acc += (double)pc * (double)kp;
     }
@@ -1677,32 +1681,59 @@ __global__ void __launch_bounds__(kT) k_mgb_xr(MgbArgs a, const uint8_t* __restr
                                                double* __restrict__ x_new,
                                                const double* __restrict__ b,
                                                float* __restrict__ r, int it, CgScalars* sc) {
+  // x_new on the tile and a 1-cell halo, staged in LDS: every x_old and p value is read
+  // from memory once per tile, the stencil reads LDS
+  constexpr int LW = 64 + 2, LH = 4 * kMgbRows + 2, NL = (LW * LH + kT - 1) / kT;
   __shared__ double sm[kT / 64];
-  if (sc->done) return;
+  __shared__ double s_x[LW * LH];
   const MgbTile t = mgb_tile(a);
+  if (sc->done) return;
   // a tile without unknowns: x[0] = r = 0 from the setup, x[1] = 0 from iteration 0
   if (!t.live && it >= 1) return;
   const int W = a.W;
+  const int x0 = t.i - (int)(threadIdx.x & 63), y0 = t.j0 - (int)(threadIdx.x >> 6);
+  const MgbCells m = mgb_cells(t, W, inf);
+  double bv[kMgbRows], xo[NL];
+  float pv[NL];
+#pragma unroll
+  for (int k = 0; k < kMgbRows; ++k) bv[k] = b[m.c[k]];
+#pragma unroll
+  for (int e = 0; e < NL; ++e) {
+    // halo cells outside the packed grid read a clamped cell; no unknown uses them
+    const int q = threadIdx.x + e * kT;
+    const int g = (x0 - 1 + q % LW) + (y0 - 1 + q / LW) * W;
+    const int gc = g < 0 ? 0 : (g >= a.C ? a.C - 1 : g);
+    xo[e] = x_old[gc];
+    pv[e] = p[gc];
+  }
   const double pq = a.pq[t.s];
-  const double alpha = pq != 0.0 ? a.rz[(it & 1) * a.n_scenes + t.s] / pq : 0.0;
+  const double rz = a.rz[(it & 1) * a.n_scenes + t.s];
+  const double alpha = pq != 0.0 ? rz / pq : 0.0;
+#pragma unroll
+  for (int e = 0; e < NL; ++e) {
+    const int q = threadIdx.x + e * kT;
+    if (q < LW * LH) s_x[q] = xo[e] + alpha * (double)pv[e];
+  }
+  __syncthreads();
   double acc = 0.0;
-  if (t.col)
-    for (int k = 0, j = t.j0; k < kMgbRows && j < t.jend; ++k, j += 4) {
-      const int c = t.i + j * W;
-      const uint8_t f = inf[c];
-      double xn = 0.0;
-      float rv = 0.f;
-      if (is_act(f)) {
-        auto xv = [&](int q) { return x_old[q] + alpha * (double)p[q]; };
-        xn = xv(c);
-        const double sn = xv(c - 1) + xv(c + 1) + xv(c - W) + xv(c + W);
-        const double r64 = b[c] - ((double)(f & 7) * xn - sn);
-        rv = (float)r64;
-        acc += r64 * r64;
-      }
-      x_new[c] = xn;
-      r[c] = rv;
+#pragma unroll
+  for (int k = 0; k < kMgbRows; ++k) {
+    if (!m.in[k]) break;
+    const int c = m.c[k];
+    const uint8_t f = m.f[k];
+    double xn = 0.0;
+    float rv = 0.f;
+    if (is_act(f)) {
+      const int l = (int)(threadIdx.x & 63) + 1 + ((int)(threadIdx.x >> 6) + 4 * k + 1) * LW;
+      xn = s_x[l];
+      const double sn = s_x[l - 1] + s_x[l + 1] + s_x[l - LW] + s_x[l + LW];
+      const double r64 = bv[k] - ((double)(f & 7) * xn - sn);
+      rv = (float)r64;
+      acc += r64 * r64;
     }
+    x_new[c] = xn;
+    r[c] = rv;
+  }
   mgb_add(acc, &a.rr[t.s], sm);
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->spmv = it + 1;
 }
