@@ -1604,17 +1604,21 @@ __global__ void __launch_bounds__(kT) k_mgb_rz(MgbArgs a, const uint8_t* __restr
   const MgbTile t = mgb_tile(a);
   if (sc->done) return;
   if (!t.live) return;  // r = z = 0 there
-  const MgbCells m = mgb_cells(t, a.W, inf);
+  // r = z = 0 at every cell that is not an unknown, so the info bytes are not needed
   float rv[kMgbRows], zv[kMgbRows];
+  bool in[kMgbRows];
 #pragma unroll
   for (int k = 0; k < kMgbRows; ++k) {
-    rv[k] = r[m.c[k]];
-    zv[k] = z[m.c[k]];
+    const int j = t.j0 + 4 * k;
+    in[k] = t.col && j < t.jend;
+    const int c = in[k] ? t.i + j * a.W : 0;
+    rv[k] = r[c];
+    zv[k] = z[c];
   }
   double v = 0.0;
 #pragma unroll
   for (int k = 0; k < kMgbRows; ++k)
-    if (m.in[k] && is_act(m.f[k])) v += (double)rv[k] * (double)zv[k];
+    if (in[k]) v += (double)rv[k] * (double)zv[k];
   mgb_add(v, &a.rz[(it & 1) * a.n_scenes + t.s], sm);
 }
 
