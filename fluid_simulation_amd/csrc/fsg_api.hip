@@ -411,6 +411,9 @@ int mg_prepare(fsg_domain* h) {
   ok &= dalloc(h, &m.part_rz, (size_t)std::max(mg_up_blocks(m.nx[0], m.ny[0]), 1)) == hipSuccess;
   ok &= dalloc(h, &m.clk, 2) == hipSuccess;
   if (m.lc > 0) ok &= dalloc(h, &m.tflag, (size_t)mg_up_blocks(m.nx[0], m.ny[0])) == hipSuccess;
+  if (m.per_scene)
+    for (int l = 1; l < m.lc; ++l)
+      ok &= dalloc(h, &m.tflag_l[l], (size_t)mg_up_blocks(m.nx[l], m.ny[l])) == hipSuccess;
   if (!ok) return fail(h, FSG_E_NOMEM, "multigrid solver: device allocation failed");
   m.lab[0] = h->label;
   a.lc = m.lc;

@@ -208,6 +208,9 @@ struct MgHost {
   int clk_span_n = 0;        // launches recorded
   bool clk_span_on = false;
   uint8_t* tflag = nullptr;  // level-0 tiles (32 x 32) holding an unknown on level 0 or 1
+  // a batch: the same flags for its tile levels 1 .. lc-1 (most of a packed grid is air,
+  // obstacles or gaps on every level)
+  uint8_t* tflag_l[kMgMaxLevels] = {};
   // fused form (one domain, two or more tile levels): level 0's down kernel also does
   // the previous iteration's x and residual update, level 1's runs the stop test
   bool fused = false;

@@ -1832,9 +1832,12 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
     k_mg_coarsen<<<(n + kT - 1) / kT, kT, 0, s>>>(m.nx[l], m.ny[l], m.lab[l], m.inf[l], 0, 0,
                                                   nullptr);
   }
-  if (m.tflag && m.lc > 0)
-    k_mg_tile_flags<<<mg_up_blocks(m.nx[0], m.ny[0]), kT, 0, s>>>(
-        m.nx[0], m.ny[0], m.inf[0], m.nx[1], m.ny[1], m.inf[1], m.tflag);
+  for (int l = 0; l < m.lc; ++l) {
+    uint8_t* flag = l == 0 ? m.tflag : m.tflag_l[l];
+    if (flag)
+      k_mg_tile_flags<<<mg_up_blocks(m.nx[l], m.ny[l]), kT, 0, s>>>(
+          m.nx[l], m.ny[l], m.inf[l], m.nx[l + 1], m.ny[l + 1], m.inf[l + 1], flag);
+  }
 }
 
 static int mg_down_blocks(const MgHost& m, int l) {
@@ -1888,7 +1891,7 @@ void launch_mg_down(hipStream_t s, MgHost& m, int l, int it, int max_iter, doubl
   } else {
     k_mg_down<false, kTileBig><<<blocks, kTileBig, 0, s>>>(mg_level(m, l), mg_level(m, l + 1),
                                                            m.coarse.omega, it, chk, check,
-                                                           nullptr, MgXr{}, nullptr);
+                                                           m.tflag_l[l], MgXr{}, nullptr);
   }
 }
 
@@ -1908,7 +1911,7 @@ void launch_mg_up(hipStream_t s, const MgHost& m, int l, int it, CgScalars* sc) 
   else
     k_mg_up<false, kTileBig><<<blocks, kTileBig, 0, s>>>(mg_level(m, l), mg_level(m, l + 1),
                                                          m.coarse.omega, sc, nullptr, it,
-                                                         nullptr);
+                                                         m.tflag_l[l]);
 }
 
 void launch_mg_dir(hipStream_t s, const MgHost& m, int it, CgScalars* sc) {
