@@ -203,7 +203,8 @@ struct fsg_domain {
   int kclock = 0;  // fsg_set_kernel_clock: 1 k_mg_coarse, 2 level 0's down kernel
   bool mg_ready = false;
   bool mg_solved = false;  // the last solve was the multigrid form (its FP64 x is valid)
-  int mg_last_iters = 0;
+  int mg_last_iters = 0;      // iterations of the last forward multigrid solve
+  int mg_last_iters_adj = 0;  // and of the last adjoint solve (a different system)
   // host-side statistics of the plain solver
   fsg_stats host_stats{};
   std::string err;
@@ -488,7 +489,10 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
   const int max_iter = h->prm.cg_max_iter;
   // a batch runs one CG per scene in lock step, each stopping at |r_s| < tol |b'_s|
   const double tol = h->prm.cg_tol;
-  int it = 0, batch = std::max(h->mg_last_iters + 1, 4);
+  // iterations are enqueued in batches between stop-test reads: first the previous
+  // solve's count (of the same kind) + 1, then doubling batches
+  int& last_iters = adj_rhs ? h->mg_last_iters_adj : h->mg_last_iters;
+  int it = 0, batch = std::max(last_iters + 1, 4), more = 2;
   CgScalars hs{};
   static const char* const kDownNames[4] = {"cg_mg_down0", "cg_mg_down1", "cg_mg_down2",
                                             "cg_mg_down3+"};
@@ -523,7 +527,8 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
     if (F < 0) HIPCHK(h, hipMemcpyAsync(&F, h->n_fluid, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     if (hs.done || it > max_iter) break;
-    batch = 2;
+    batch = more;
+    more = std::min(2 * more, 32);
   }
   h->last_F = F;
   h->host_stats.n_fluid = F;
@@ -534,7 +539,7 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
           launch_mg_finish(h->stream, m, h->prm.project_mean, h->sc, h->xg,
                            h->batch ? &bs : nullptr);
         });
-  h->mg_last_iters = hs.iter;
+  last_iters = hs.iter;
   h->mg_solved = true;
   h->host_stats.cg_iterations = hs.iter;
   h->host_stats.cg_error = hs.err;
