@@ -401,6 +401,10 @@ def adjoint_leg(fsg, scenes, args):
     scale = args.adj_scale
     target, _ = run(torch.randn(2, 16, 16, generator=torch.Generator().manual_seed(1)) * scale)
     c = (torch.randn(2, 16, 16, generator=torch.Generator().manual_seed(0)) * scale)
+    # untimed: the control model's first forward / backward (torch's one-time set-up)
+    cw = torch.zeros(2, 16, 16, requires_grad=True)
+    torch.autograd.backward(list(model(cw)), [torch.ones(len(parts), 2), torch.ones(n, n),
+                                              torch.ones(n, n)])
     c.requires_grad_(True)
     d.synchronize()
     tm0 = time.perf_counter()

@@ -133,11 +133,29 @@ try:  # torch is plumbing here: the wrapper is defined only where torch imports
                     torch.as_tensor(bu, **to), torch.as_tensor(bv, **to),
                     None, None, None, None, None, None)
 
+    class _Gather4(torch.autograd.Function):
+        """out[n] = sum_k w[n, k] c[idx[n, k]]; the backward is one weighted bincount."""
+
+        @staticmethod
+        def forward(ctx, cflat, idx, w):
+            ctx.save_for_backward(idx, w)
+            ctx.size = cflat.numel()
+            return (w * cflat[idx]).sum(1)
+
+        @staticmethod
+        def backward(ctx, g):
+            idx, w = ctx.saved_tensors
+            gc = torch.bincount(idx.reshape(-1), weights=(w * g[:, None]).reshape(-1),
+                                minlength=ctx.size)
+            return gc.to(g.dtype), None, None
+
     class ControlModel:
         """Initial velocities from a (2, K, K) control grid c: the bilinear field
         U(x, y) through control nodes at (a L_x / (K-1), b L_y / (K-1)).  The back
         (and front) buffer samples it at the face centres (u at (i dx, (j + 1/2) dy),
-        v at ((i + 1/2) dx, j dy)) and every particle at its position."""
+        v at ((i + 1/2) dx, j dy)) and every particle at its position.  Each sample is
+        a gather of its 4 control nodes with fixed weights (FP64), so the model and its
+        backward (a scatter-add) are linear in c."""
 
         def __init__(self, nx, ny, length_x, length_y, positions, k=16):
             self.nx, self.ny, self.k = nx, ny, k
@@ -153,6 +171,7 @@ try:  # torch is plumbing here: the wrapper is defined only where torch imports
             self._wp = self._weights(pos[:, 0], pos[:, 1])
 
         def _weights(self, x, y):
+            """(node indices (n, 4), weights (n, 4)) of the bilinear samples at (x, y)."""
             k = self.k
             sx = np.clip(x / self.lx * (k - 1), 0.0, k - 1 - 1e-12)
             sy = np.clip(y / self.ly * (k - 1), 0.0, k - 1 - 1e-12)
@@ -161,18 +180,19 @@ try:  # torch is plumbing here: the wrapper is defined only where torch imports
             idx = np.stack([i0 + j0 * k, i0 + 1 + j0 * k, i0 + (j0 + 1) * k,
                             i0 + 1 + (j0 + 1) * k], 1)
             w = np.stack([(1 - fx) * (1 - fy), fx * (1 - fy), (1 - fx) * fy, fx * fy], 1)
-            n = len(x)
-            rows = np.repeat(np.arange(n), 4)
-            return torch.sparse_coo_tensor(np.stack([rows, idx.ravel()]), w.ravel(),
-                                           (n, k * k), dtype=torch.float64).coalesce()
+            return torch.from_numpy(idx), torch.from_numpy(w)
+
+        @staticmethod
+        def _sample(wt, cflat):
+            return _Gather4.apply(cflat, wt[0], wt[1])
 
         def __call__(self, c, dtype=torch.float32):
             """(particle velocities (P, 2), back u, back v) for controls c (2, K, K)."""
-            cu = c[0].reshape(-1, 1).to(torch.float64)
-            cv = c[1].reshape(-1, 1).to(torch.float64)
-            u = torch.sparse.mm(self._wu, cu).reshape(self.ny, self.nx)
-            v = torch.sparse.mm(self._wv, cv).reshape(self.ny, self.nx)
-            vel = torch.cat([torch.sparse.mm(self._wp, cu), torch.sparse.mm(self._wp, cv)], 1)
+            cu = c[0].reshape(-1).to(torch.float64)
+            cv = c[1].reshape(-1).to(torch.float64)
+            u = self._sample(self._wu, cu).reshape(self.ny, self.nx)
+            v = self._sample(self._wv, cv).reshape(self.ny, self.nx)
+            vel = torch.stack([self._sample(self._wp, cu), self._sample(self._wp, cv)], 1)
             return vel.to(dtype), u.to(dtype), v.to(dtype)
 
 except ImportError:  # pragma: no cover - torch is part of this image

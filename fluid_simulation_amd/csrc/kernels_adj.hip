@@ -45,6 +45,16 @@ __device__ __forceinline__ void splat_add(float* g, int nx, const SplatCorners& 
   unsafeAtomicAdd(&g[s.i1 + s.j1 * nx], s.fi * a1);
 }
 
+// The four bilinear terms of splat_add, handed to `add(i, j, term)`.
+template <typename Add>
+__device__ __forceinline__ void splat_terms2(const SplatCorners& s, float val, Add add) {
+  const float a0 = (1.0f - s.fj) * val, a1 = s.fj * val;
+  add(s.i, s.j, (1.0f - s.fi) * a0);
+  add(s.i1, s.j, s.fi * a0);
+  add(s.i, s.j1, (1.0f - s.fi) * a1);
+  add(s.i1, s.j1, s.fi * a1);
+}
+
 __device__ __forceinline__ float inv_or_div(float a, float d, int pow2, float inv) {
   return pow2 ? a * inv : a / d;
 }
@@ -58,6 +68,9 @@ __device__ __forceinline__ float inv_or_div(float a, float d, int pow2, float in
 // per component (r = 1 for PIC, 0 for FLIP).  In: pbar = (X1bar, V1bar).  Out:
 // pbar = (X0bar, V0bar) of these two phases; s = I^T V1bar on the u / v grids (the
 // adjoint of H' is s, of D is (1-r) s).
+// Cells of a block's LDS splat window (float2 each: the u and v adjoint terms).
+constexpr int kAdjSplatTile = 2048;
+
 __global__ void __launch_bounds__(kAT) k_adj_g2p(Geom g, const float4* __restrict__ x0,
                                                  int64_t np, const float* __restrict__ hu,
                                                  const float* __restrict__ hv,
@@ -67,11 +80,11 @@ __global__ void __launch_bounds__(kAT) k_adj_g2p(Geom g, const float4* __restric
                                                  float4* __restrict__ pbar,
                                                  float* __restrict__ su, float* __restrict__ sv) {
   const int64_t p = gtid();
-  if (p >= np) return;
-  const float4 q = x0[p];
-  const float4 a = pbar[p];
+  const bool live = p < np;
+  const float4 q = live ? x0[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4 a = live ? pbar[p] : make_float4(0.f, 0.f, 0.f, 0.f);
   float vbx = a.z, vby = a.w;
-  if (!(reject && reject[p])) {
+  if (live && !(reject && reject[p])) {
     vbx += dt * a.x;
     vby += dt * a.y;
   }
@@ -80,8 +93,64 @@ __global__ void __launch_bounds__(kAT) k_adj_g2p(Geom g, const float4* __restric
       interp_corners(div_x(q.x, g), div_y(q.y - g.dy * 0.5f, g), nx, ny);
   const SplatCorners cv =
       interp_corners(div_x(q.x - g.dx * 0.5f, g), div_y(q.y, g), nx, ny);
-  splat_add(su, nx, cu, vbx);
-  splat_add(sv, nx, cv, vby);
+  // the splat of vbx into su and vby into sv: summed in an LDS copy of the block's
+  // footprint when it fits (neighbouring particles share cells), then one atomic per
+  // touched value; otherwise straight to the grids
+  int i0 = live ? min(cu.i, cv.i) : INT_MAX, j0 = live ? min(cu.j, cv.j) : INT_MAX;
+  int i1 = live ? max(cu.i1, cv.i1) : INT_MIN, j1 = live ? max(cu.j1, cv.j1) : INT_MIN;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    i0 = min(i0, __shfl_xor(i0, off, 64));
+    j0 = min(j0, __shfl_xor(j0, off, 64));
+    i1 = max(i1, __shfl_xor(i1, off, 64));
+    j1 = max(j1, __shfl_xor(j1, off, 64));
+  }
+  __shared__ int s_box[4][kAT / 64];
+  __shared__ float2 s_tile[kAdjSplatTile];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) {
+    s_box[0][wave] = i0;
+    s_box[1][wave] = j0;
+    s_box[2][wave] = i1;
+    s_box[3][wave] = j1;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int w = 0; w < kAT / 64; ++w) {
+    i0 = min(i0, s_box[0][w]);
+    j0 = min(j0, s_box[1][w]);
+    i1 = max(i1, s_box[2][w]);
+    j1 = max(j1, s_box[3][w]);
+  }
+  const int bw = i1 - i0 + 1, bh = j1 - j0 + 1;
+  const bool tiled = bw > 0 && bh > 0 && (int64_t)bw * bh <= kAdjSplatTile;
+  if (!tiled) {
+    if (live) {
+      splat_add(su, nx, cu, vbx);
+      splat_add(sv, nx, cv, vby);
+    }
+  } else {
+    const int area = bw * bh;
+    for (int c = threadIdx.x; c < area; c += kAT) s_tile[c] = make_float2(0.f, 0.f);
+    __syncthreads();
+    if (live) {
+      float* tile = reinterpret_cast<float*>(s_tile);
+      splat_terms2(cu, vbx, [&](int i, int j, float t) {
+        atomicAdd(tile + 2 * ((i - i0) + (j - j0) * bw), t);
+      });
+      splat_terms2(cv, vby, [&](int i, int j, float t) {
+        atomicAdd(tile + 2 * ((i - i0) + (j - j0) * bw) + 1, t);
+      });
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < area; c += kAT) {
+      const float2 v = s_tile[c];
+      const int cell = (i0 + c % bw) + (j0 + c / bw) * nx;
+      if (v.x != 0.f) unsafeAtomicAdd(su + cell, v.x);
+      if (v.y != 0.f) unsafeAtomicAdd(sv + cell, v.y);
+    }
+  }
+  if (!live) return;
   float ui, uj, vi, vj;
   interp_grad(hu, nx, cu, ui, uj);
   interp_grad(hv, nx, cv, vi, vj);
