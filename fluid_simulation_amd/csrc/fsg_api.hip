@@ -1318,11 +1318,13 @@ int fsg_get_reject_flags(fsg_domain* h, uint8_t* flags) {
 
 int fsg_get_stats(fsg_domain* h, fsg_stats* s) {
   if (!h || !s) return FSG_E_INVALID;
-  if (int rc = check_solver_abort(h)) return rc;
+  // the plain and multigrid forms keep their statistics on the host (read with their
+  // stop tests): no device access, no synchronisation
   if (h->prm.solver == FSG_SOLVER_PLAIN || h->prm.solver == FSG_SOLVER_MG) {
     *s = h->host_stats;
     return FSG_OK;
   }
+  if (int rc = check_solver_abort(h)) return rc;
   CgScalars hs{};
   int F = 0;
   HIPCHK(h, hipMemcpyAsync(&hs, h->sc, sizeof(hs), hipMemcpyDeviceToHost, h->stream));
