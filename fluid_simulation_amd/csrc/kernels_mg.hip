@@ -440,8 +440,9 @@ __global__ void __launch_bounds__(kT) k_mg_coarsen(int fnx, int fny, const uint8
 // Down: level l (fine) -> level l+1.  A block owns 16 x 16 coarse cells (32 x 32
 // fine cells).  In LDS: r and info on the fine halo [-3, 35), x1 = w r / n there,
 // x2 (second sweep) on [-2, 34), the residual on [-1, 33), then the restriction
-// (1D weights 1/4, 3/4, 3/4, 1/4 over fine 2I-1 .. 2I+2).  Level 0 first checks
-// convergence on the partials of |r|^2 (the stop test of the CG iteration).
+// (1D weights 1/4, 3/4, 3/4, 1/4 over fine 2I-1 .. 2I+2).  The kernel given
+// check != 0 first runs the stop test of the CG iteration on the partials of |r|^2:
+// level 0's, or level 1's in the fused form (where level 0 forms the residual).
 struct MgCheck {
   CgScalars* sc;
   const double* part_rr;
