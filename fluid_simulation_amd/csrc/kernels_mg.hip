@@ -545,7 +545,10 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
   constexpr int W3 = 38, W2 = 36, W1 = 34;
   constexpr int W4 = 40;  // fused form: x_new on [-4, 36)
   constexpr int WC = 18;  // coarse cells of the tile and their neighbours
-  __shared__ float s_r[W3 * W3], s_x1[W3 * W3], s_x2[W2 * W2], s_res[W1 * W1];
+  __shared__ float s_r[W3 * W3], s_x1[W3 * W3], s_x2[W2 * W2];
+  // the residual (third phase) reuses x1's storage: x1 is dead once x2 is formed, and
+  // the smaller footprint lets more tiles share a CU
+  float* const s_res = s_x1;
   __shared__ uint8_t s_inf[W3 * W3], s_cinf[WC * WC];
   __shared__ double s_xn[XR ? W4 * W4 : 1];
   const int tiles_x = (c.nx + 15) >> 4;
