@@ -533,6 +533,9 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
   static_assert(!XR || L0, "the fused update belongs to level 0");
   const BlockSpan clock(span);
   __shared__ double sm[NT / 64];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  // the tile flag is read with the stop flag (one memory round trip)
+  const bool empty = tflag && !tflag[tile];
   if (check) {
     if (mg_check<NT>(chk, it, sm)) {
       clock.end();
@@ -553,11 +556,10 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
   __shared__ double s_xn[XR ? W4 * W4 : 1];
   const int tiles_x = (c.nx + 15) >> 4;
   // each XCD takes one band of consecutive tiles, so halo rows are shared in its L2
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int bx = tile % tiles_x, by = tile / tiles_x;
   const int fi0 = bx * 32, fj0 = by * 32;
   const bool do_xr = XR && it > 0;
-  if (tflag && !tflag[tile]) {
+  if (empty) {
     // no unknown in the tile on either level: its outputs are zero; written once per
     // solve (earlier solves may have left other values)
     if (it == 0) {
@@ -789,7 +791,6 @@ template <bool L0, int NT>
 __global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega,
                                                const CgScalars* sc, double* __restrict__ part_rz,
                                                int it, const uint8_t* __restrict__ tflag) {
-  if (sc->done) return;
   constexpr int W2 = 36, W1 = 34, WC = 20;
   __shared__ float s_r[W2 * W2], s_xc[W2 * W2], s_y1[W1 * W1], s_e[WC * WC];
   __shared__ uint8_t s_inf[W2 * W2], s_cinf[WC * WC];
@@ -799,7 +800,10 @@ __global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega,
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int bx = tile % tiles_x, by = tile / tiles_x;
   const int fi0 = bx * 32, fj0 = by * 32;
-  if (tflag && !tflag[tile]) {
+  // the tile flag and the stop flag are read together (one memory round trip)
+  const bool empty = tflag && !tflag[tile];
+  if (sc->done) return;
+  if (empty) {
     // no unknown in the tile: z = 0 there (written once per solve) and r.z adds 0
     if (it == 0)
       for (int t = threadIdx.x; t < 32 * 32; t += NT) {
