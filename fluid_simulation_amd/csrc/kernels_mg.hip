@@ -436,6 +436,58 @@ __global__ void __launch_bounds__(kT) k_mg_coarsen(int fnx, int fny, const uint8
   }
 }
 
+// The last levels of the coarsening (level l0-1 of at most kTailCells cells and
+// below) in one workgroup: level l0-1's labels staged in LDS, then per level the info
+// bytes of the finer level and the labels of the coarser one formed there and written
+// out, and finally the coarsest level's info -- what k_mg_coarsen does one launch per
+// level.
+constexpr int kTailCells = 4096;
+struct MgTailArgs {
+  int l0, nlev;
+  int nx[kMgMaxLevels], ny[kMgMaxLevels];
+  uint8_t* lab[kMgMaxLevels];
+  uint8_t* inf[kMgMaxLevels];
+};
+
+__global__ void __launch_bounds__(1024) k_mg_coarsen_tail(MgTailArgs a) {
+  __shared__ uint8_t s_lab[2][kTailCells];
+  const int lf = a.l0 - 1;
+  {
+    const int n = a.nx[lf] * a.ny[lf];
+    for (int t = threadIdx.x; t < n; t += 1024) s_lab[0][t] = a.lab[lf][t];
+  }
+  __syncthreads();
+  int cur = 0;
+  for (int l = a.l0; l <= a.nlev; ++l) {
+    const int fnx = a.nx[l - 1], fny = a.ny[l - 1], nf = fnx * fny;
+    const uint8_t* flab = s_lab[cur];
+    for (int t = threadIdx.x; t < nf; t += 1024)
+      a.inf[l - 1][t] = make_info(flab, fnx, fny, t % fnx, t / fnx);
+    if (l == a.nlev) break;
+    const int cnx = a.nx[l], cny = a.ny[l], nc = cnx * cny;
+    for (int t = threadIdx.x; t < nc; t += 1024) {
+      const int I = t % cnx, J = t / cnx;
+      bool air = false, liq = false;
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int i = 2 * I + e, j = 2 * J + b;
+          if (i < fnx && j < fny) {
+            const uint8_t L = flab[i + j * fnx];
+            air |= L == kAir;
+            liq |= L == kLiquid;
+          }
+        }
+      const uint8_t L = air ? kAir : (liq ? kLiquid : kSolid);
+      s_lab[cur ^ 1][t] = L;
+      a.lab[l][t] = L;
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+}
+
 // ---------------------------------------------------------------- V-cycle
 // Down: level l (fine) -> level l+1.  A block owns 16 x 16 coarse cells (32 x 32
 // fine cells).  In LDS: r and info on the fine halo [-3, 35), x1 = w r / n there,
@@ -1825,7 +1877,12 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
   k_mg_setup1<<<nb, kT, 0, s>>>(g.nx, g.ny, project_mean, m.inf[0], m.b, warm ? m.x[1] : nullptr,
                                 m.x[0], m.rl[0], m.part_sum, m.part_cnt, nb, m.part_bb,
                                 m.part_rr, sc);
-  for (int l = 1; l < m.nlev; ++l) {
+  // levels coarsened one launch each, down to the first level l0 whose finer level
+  // (>= 1: level 0's info comes from k_mg_setup0) fits the one-workgroup tail
+  int l0 = 2;
+  while (l0 < m.nlev && m.nx[l0 - 1] * m.ny[l0 - 1] > kTailCells) ++l0;
+  if (m.nlev < 2) l0 = m.nlev;
+  for (int l = 1; l < std::min(l0, m.nlev); ++l) {
     const int nf = m.nx[l - 1] * m.ny[l - 1], ncl = m.nx[l] * m.ny[l];
     const int n = nf > ncl ? nf : ncl;
     int blocks = (n + kT - 1) / kT;
@@ -1834,7 +1891,18 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
                                        l - 1 >= 1 ? m.inf[l - 1] : nullptr, m.nx[l], m.ny[l],
                                        m.lab[l]);
   }
-  if (m.nlev > 1) {
+  if (m.nlev > 1 && l0 < m.nlev) {
+    MgTailArgs a{};
+    a.l0 = l0;
+    a.nlev = m.nlev;
+    for (int l = 0; l < m.nlev; ++l) {
+      a.nx[l] = m.nx[l];
+      a.ny[l] = m.ny[l];
+      a.lab[l] = m.lab[l];
+      a.inf[l] = m.inf[l];
+    }
+    k_mg_coarsen_tail<<<1, 1024, 0, s>>>(a);
+  } else if (m.nlev > 1) {
     const int l = m.nlev - 1, n = m.nx[l] * m.ny[l];
     // info of the last level: a coarsening call with no coarse output
     k_mg_coarsen<<<(n + kT - 1) / kT, kT, 0, s>>>(m.nx[l], m.ny[l], m.lab[l], m.inf[l], 0, 0,
