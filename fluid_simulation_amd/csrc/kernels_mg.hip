@@ -1089,7 +1089,7 @@ __device__ __forceinline__ void coarse_cycle(int nl, const int* __restrict__ lnx
     __syncthreads();
   }
   {
-    // coarsest level: <= 16 cells (checked on the host), one wavefront
+    // coarsest level: <= 64 cells (checked on the host), one wavefront
     const int l = nl - 1;
     const int nx = lnx[l], ny = lny[l], n = nx * ny, o = loff[l];
     if (threadIdx.x < 64) {
@@ -1193,6 +1193,14 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, Mg
   const unsigned long long t0 = clk ? wall_clock64() : 0ull;
   __shared__ double sm[2 * 16];
   __shared__ int s_lv[3 * kMgMaxLevels];
+  // the first pass of the info and right-hand side loads is issued before the stop
+  // flag is read, so that both waits overlap
+  const int tot = a.total;  // a multiple of 4
+  const int n0 = a.nx[a.lc] * a.ny[a.lc];
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(a.inf_all);
+  const int q0 = threadIdx.x;
+  const uint32_t inf0 = q0 < tot / 4 ? src[q0] : 0u;
+  const float r0 = q0 < n0 ? a.r_in[q0] : 0.f;
   if (check) {
     if (mg_check<kCoarseThreads>(chk, it, sm)) {
       kclock_end(clk, t0);
@@ -1203,7 +1211,6 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, Mg
     return;
   }
   extern __shared__ float lds[];
-  const int tot = a.total;  // a multiple of 4
   float* R = lds;
   float* X = R + tot;
   float* T = X + tot;
@@ -1216,11 +1223,11 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, Mg
     s_lv[2 * kMgMaxLevels + threadIdx.x] = a.off[l] - a.off[a.lc];
   }
   {
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(a.inf_all);
     uint32_t* dst = reinterpret_cast<uint32_t*>(INF);
-    for (int q = threadIdx.x; q < tot / 4; q += kCoarseThreads) dst[q] = src[q];
-    const int n0 = a.nx[a.lc] * a.ny[a.lc];
-    for (int q = threadIdx.x; q < n0; q += kCoarseThreads) R[q] = a.r_in[q];
+    if (q0 < tot / 4) dst[q0] = inf0;
+    if (q0 < n0) R[q0] = r0;
+    for (int q = q0 + kCoarseThreads; q < tot / 4; q += kCoarseThreads) dst[q] = src[q];
+    for (int q = q0 + kCoarseThreads; q < n0; q += kCoarseThreads) R[q] = a.r_in[q];
   }
   __syncthreads();
   coarse_cycle(nl, s_lv, s_lv + kMgMaxLevels, s_lv + 2 * kMgMaxLevels, a.omega, a.sweeps, R, X,
