@@ -415,6 +415,10 @@ int mg_prepare(fsg_domain* h) {
   if (m.per_scene)
     for (int l = 1; l < m.lc; ++l)
       ok &= dalloc(h, &m.tflag_l[l], (size_t)mg_up_blocks(m.nx[l], m.ny[l])) == hipSuccess;
+  if (!m.per_scene && m.lc >= 2) {
+    ok &= dalloc(h, &m.ticket, 1) == hipSuccess;
+    if (ok) ok &= hipMemsetAsync(m.ticket, 0, sizeof(unsigned), h->stream) == hipSuccess;
+  }
   if (!ok) return fail(h, FSG_E_NOMEM, "multigrid solver: device allocation failed");
   m.lab[0] = h->label;
   a.lc = m.lc;
@@ -508,7 +512,7 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
               [&] { launch_mg_down(h->stream, m, l, it, max_iter, tol, h->sc); });
       if (h->batch)
         timed(h, "cg_mg_coarse", [&] { launch_mg_scene_coarse(h->stream, scene_coarse_args(h), h->sc); });
-      else
+      else if (!mg_tail_coarse(m))
         timed(h, "cg_mg_coarse", [&] { launch_mg_coarse(h->stream, m, it, max_iter, tol, h->sc); });
       for (int l = m.lc - 1; l >= 0; --l)
         timed(h, kUpNames[std::min(l, 3)], [&] { launch_mg_up(h->stream, m, l, it, h->sc); });

@@ -214,6 +214,10 @@ struct MgHost {
   // fused form (one domain, two or more tile levels): level 0's down kernel also does
   // the previous iteration's x and residual update, level 1's runs the stop test
   bool fused = false;
+  // the last tile level's down kernel also runs the one-workgroup stage (its last
+  // workgroup to finish), so no k_mg_coarse launch; ticket: its arrival counter
+  bool coarse_tail = true;
+  unsigned* ticket = nullptr;
   int nlev = 0, lc = 0;
   int nx[kMgMaxLevels] = {}, ny[kMgMaxLevels] = {};
   uint8_t* lab[kMgMaxLevels] = {};  // [0]: the domain's labels
@@ -242,6 +246,7 @@ struct MgBatchScale {
 };
 
 int mg_vec_blocks(int C);
+bool mg_tail_coarse(const MgHost& m);  // the one-workgroup stage runs in the down kernel
 int mg_part_rr_size(const MgHost& m);  // entries of MgHost::part_rr
 int mg_up_blocks(int nx, int ny);
 hipError_t mg_set_lds_limit(size_t bytes);

@@ -561,6 +561,9 @@ struct MgXr {
   double* part_rr;  // out: one partial |r|^2 per block
 };
 
+__device__ __forceinline__ void coarse_stage(const MgCoarseArgs& a, double* part_rz,
+                                             bool coherent);
+
 // Block span clock (span != null): thread 0 stores the block's start and end ticks
 // of the device wall clock in span[2 b], span[2 b + 1]; the launch's duration is the
 // latest end minus the earliest start over its blocks.
@@ -581,10 +584,36 @@ template <bool L0, int NT, bool XR = false>
 __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omega, int it,
                                                  MgCheck chk, int check,
                                                  const uint8_t* __restrict__ tflag, MgXr xr,
-                                                 unsigned long long* span) {
+                                                 unsigned long long* span, int tail,
+                                                 MgCoarseArgs ca, unsigned* __restrict__ ticket) {
   static_assert(!XR || L0, "the fused update belongs to level 0");
   const BlockSpan clock(span);
   __shared__ double sm[NT / 64];
+  __shared__ int s_last;
+  // tail != 0 (the last tile level, 1024 threads): the restriction is stored
+  // device-coherent and the last workgroup to finish runs the one-workgroup stage on
+  // it, in place of a k_mg_coarse launch.  The arrival count is ordered after every
+  // wave's stores, as in a grid barrier, but nobody waits.
+  auto run_tail = [&]() {
+    if (!tail) return;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0)
+      s_last = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+               gridDim.x - 1;
+    __syncthreads();
+    if (s_last) {
+      if (threadIdx.x == 0) st_dev(ticket, 0u);  // ready for the next launch
+      coarse_stage(ca, nullptr, true);
+    }
+  };
+  auto put_r = [&](float* dst, float v) {
+    if (tail)
+      st_dev(dst, v);
+    else
+      *dst = v;
+  };
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   // the tile flag is read with the stop flag (one memory round trip)
   const bool empty = tflag && !tflag[tile];
@@ -621,7 +650,7 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
       }
       for (int t = threadIdx.x; t < 16 * 16; t += NT) {
         const int I = bx * 16 + (t & 15), J = by * 16 + (t >> 4);
-        if (I < c.nx && J < c.ny) c.r[I + J * c.nx] = 0.f;
+        if (I < c.nx && J < c.ny) put_r(c.r + I + J * c.nx, 0.f);
       }
     }
     if (do_xr) {
@@ -637,6 +666,7 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
         }
       if (threadIdx.x == 0) xr.part_rr[blockIdx.x] = 0.0;
     }
+    run_tail();
     clock.end();
     return;
   }
@@ -826,9 +856,10 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
             v += m * s_res[(2 * cj + b) * W1 + 2 * ci + a];
           }
       }
-      c.r[cc] = v;
+      put_r(c.r + cc, v);
     }
   }
+  run_tail();
   if (span) {
     __syncthreads();
     clock.end();
@@ -1186,31 +1217,16 @@ __device__ __forceinline__ void kclock_end(unsigned long long* clk, unsigned lon
 
 // With check != 0 (the hierarchy starts in this kernel) it runs level 0's
 // convergence test first and forms the r.z partial at the end.
-__global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, MgCheck chk, int it,
-                                                             int check,
-                                                             double* __restrict__ part_rz,
-                                                             unsigned long long* clk) {
-  const unsigned long long t0 = clk ? wall_clock64() : 0ull;
+// The one-workgroup stage: load its levels' info and level lc's right-hand side into
+// LDS, cycle, store level lc's result (and, when the hierarchy starts here, the r.z
+// partial).  `coherent`: the right-hand side was written by other workgroups of the
+// same launch with device-coherent stores, so it is read the same way.
+__device__ __forceinline__ void coarse_stage(const MgCoarseArgs& a, double* part_rz,
+                                             bool coherent) {
   __shared__ double sm[2 * 16];
   __shared__ int s_lv[3 * kMgMaxLevels];
-  // the first pass of the info and right-hand side loads is issued before the stop
-  // flag is read, so that both waits overlap
-  const int tot = a.total;  // a multiple of 4
-  const int n0 = a.nx[a.lc] * a.ny[a.lc];
-  const uint32_t* src = reinterpret_cast<const uint32_t*>(a.inf_all);
-  const int q0 = threadIdx.x;
-  const uint32_t inf0 = q0 < tot / 4 ? src[q0] : 0u;
-  const float r0 = q0 < n0 ? a.r_in[q0] : 0.f;
-  if (check) {
-    if (mg_check<kCoarseThreads>(chk, it, sm)) {
-      kclock_end(clk, t0);
-      return;
-    }
-  } else if (chk.sc->done) {
-    kclock_end(clk, t0);
-    return;
-  }
   extern __shared__ float lds[];
+  const int tot = a.total;  // a multiple of 4
   float* R = lds;
   float* X = R + tot;
   float* T = X + tot;
@@ -1223,29 +1239,46 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, Mg
     s_lv[2 * kMgMaxLevels + threadIdx.x] = a.off[l] - a.off[a.lc];
   }
   {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(a.inf_all);
     uint32_t* dst = reinterpret_cast<uint32_t*>(INF);
-    if (q0 < tot / 4) dst[q0] = inf0;
-    if (q0 < n0) R[q0] = r0;
-    for (int q = q0 + kCoarseThreads; q < tot / 4; q += kCoarseThreads) dst[q] = src[q];
-    for (int q = q0 + kCoarseThreads; q < n0; q += kCoarseThreads) R[q] = a.r_in[q];
+    for (int q = threadIdx.x; q < tot / 4; q += kCoarseThreads) dst[q] = src[q];
+    const int n0 = a.nx[a.lc] * a.ny[a.lc];
+    for (int q = threadIdx.x; q < n0; q += kCoarseThreads)
+      R[q] = coherent ? ld_dev(a.r_in + q) : a.r_in[q];
   }
   __syncthreads();
   coarse_cycle(nl, s_lv, s_lv + kMgMaxLevels, s_lv + 2 * kMgMaxLevels, a.omega, a.sweeps, R, X,
                T, INF);
-  {
-    const int n = a.nx[a.lc] * a.ny[a.lc];
-    const float* res = nl == 1 ? X : T;
-    double rz = 0.0;
-    for (int q = threadIdx.x; q < n; q += kCoarseThreads) {
-      a.x_out[q] = res[q];
-      rz += (double)R[q] * (double)res[q];
-    }
-    if (part_rz) {
-      double v[1] = {rz};
-      block_sum<1, kCoarseThreads>(v, sm);
-      if (threadIdx.x == 0) part_rz[0] = v[0];
-    }
+  const int n = a.nx[a.lc] * a.ny[a.lc];
+  const float* res = nl == 1 ? X : T;
+  double rz = 0.0;
+  for (int q = threadIdx.x; q < n; q += kCoarseThreads) {
+    a.x_out[q] = res[q];
+    rz += (double)R[q] * (double)res[q];
   }
+  if (part_rz) {
+    double v[1] = {rz};
+    block_sum<1, kCoarseThreads>(v, sm);
+    if (threadIdx.x == 0) part_rz[0] = v[0];
+  }
+}
+
+__global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(MgCoarseArgs a, MgCheck chk, int it,
+                                                             int check,
+                                                             double* __restrict__ part_rz,
+                                                             unsigned long long* clk) {
+  const unsigned long long t0 = clk ? wall_clock64() : 0ull;
+  __shared__ double sm[2 * 16];
+  if (check) {
+    if (mg_check<kCoarseThreads>(chk, it, sm)) {
+      kclock_end(clk, t0);
+      return;
+    }
+  } else if (chk.sc->done) {
+    kclock_end(clk, t0);
+    return;
+  }
+  coarse_stage(a, part_rz, false);
   if (clk) {
     __syncthreads();
     kclock_end(clk, t0);
@@ -1847,6 +1880,10 @@ hipError_t mg_set_lds_limit(size_t bytes) {
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_coarse),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   if (e != hipSuccess) return e;
+  // the last tile level's down kernel hosts the same stage (its own static LDS aside)
+  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_down<false, kTileBig, false>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) return e;
   return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_scene_coarse),
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
@@ -1967,15 +2004,22 @@ void launch_mg_down(hipStream_t s, MgHost& m, int l, int it, int max_iter, doubl
                   vec_blocks(m.nx[0] * m.ny[0]), m.part_rr};
     k_mg_down<true, kT, true><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1),
                                                     m.coarse.omega, it, chk, 0, m.tflag, xr,
-                                                    span);
+                                                    span, 0, MgCoarseArgs{}, nullptr);
   } else if (l == 0) {
     k_mg_down<true, kT><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega, it,
-                                              chk, check, m.tflag, MgXr{}, span);
+                                              chk, check, m.tflag, MgXr{}, span, 0,
+                                              MgCoarseArgs{}, nullptr);
   } else {
-    k_mg_down<false, kTileBig><<<blocks, kTileBig, 0, s>>>(mg_level(m, l), mg_level(m, l + 1),
-                                                           m.coarse.omega, it, chk, check,
-                                                           m.tflag_l[l], MgXr{}, nullptr);
+    // the last tile level also runs the one-workgroup stage (mg_tail_coarse)
+    const bool tail = l == m.lc - 1 && mg_tail_coarse(m);
+    k_mg_down<false, kTileBig><<<blocks, kTileBig, tail ? m.coarse_lds : 0, s>>>(
+        mg_level(m, l), mg_level(m, l + 1), m.coarse.omega, it, chk, check, m.tflag_l[l],
+        MgXr{}, nullptr, tail ? 1 : 0, m.coarse, m.ticket);
   }
+}
+
+bool mg_tail_coarse(const MgHost& m) {
+  return !m.per_scene && m.lc >= 2 && m.ticket != nullptr && m.coarse_tail;
 }
 
 void launch_mg_coarse(hipStream_t s, const MgHost& m, int it, int max_iter, double tol,

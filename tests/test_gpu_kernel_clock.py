@@ -10,9 +10,12 @@ from fluid_simulation_amd import fsg, scenes
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("kernel,timer", [("coarse", "cg_mg_coarse"), ("down0", "cg_mg_down0")])
-def test_kernel_clock_counts_every_launch(kernel, timer):
-    sc = scenes.dam_break(256)
+# at 64^2 the one-workgroup stage is its own kernel (one tile level); from two tile
+# levels on the last tile level's down kernel runs it
+@pytest.mark.parametrize("kernel,timer,n", [("coarse", "cg_mg_coarse", 64),
+                                            ("down0", "cg_mg_down0", 256)])
+def test_kernel_clock_counts_every_launch(kernel, timer, n):
+    sc = scenes.dam_break(n)
     d = fsg.Domain.from_scene(sc, cg_tol=1e-6, cg_max_iter=200, solver=fsg.SOLVER_MG)
     for _ in range(2):
         d.step(fsg.PICFLIP, sc.dt)
