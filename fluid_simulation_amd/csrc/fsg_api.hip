@@ -464,6 +464,17 @@ fsg::SceneCoarseArgs scene_coarse_args(const fsg_domain* h) {
 // CG preconditioned by one V-cycle per iteration (FSG_SOLVER_MG).  Iterations are
 // enqueued in batches (the first sized by the previous solve's count); every
 // kernel returns at once after the device-side stop test has fired.
+// Room for kSpanLaunches launches of level 0's down kernel in the block-span clock
+// (held until destroy).
+void alloc_clk_span(fsg_domain* h) {
+  constexpr int kSpanLaunches = 4096;
+  MgHost& m = h->mg;
+  m.clk_span_nb = mg_down_blocks0(m);
+  if (m.clk_span_nb > 0 &&
+      dalloc(h, &m.clk_span, 2 * (size_t)m.clk_span_nb * kSpanLaunches) == hipSuccess)
+    m.clk_span_cap = kSpanLaunches;
+}
+
 // F < 0: the number of unknowns is still on the device (h->n_fluid); it is read back
 // with the stop-test copy, so the solve costs no extra host round trip.
 int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
@@ -471,14 +482,7 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
   MgHost& m = h->mg;
   m.clk_on = h->kclock == 1;
   m.clk_span_on = h->kclock == 2;
-  if (m.clk_span_on && !m.clk_span) {
-    // room for kSpanLaunches launches of level 0's down kernel (held until destroy)
-    constexpr int kSpanLaunches = 4096;
-    m.clk_span_nb = mg_down_blocks0(m);
-    if (m.clk_span_nb > 0 &&
-        dalloc(h, &m.clk_span, 2 * (size_t)m.clk_span_nb * kSpanLaunches) == hipSuccess)
-      m.clk_span_cap = kSpanLaunches;
-  }
+  if (m.clk_span_on && !m.clk_span) alloc_clk_span(h);
   timed(h, "cg_mg_setup", [&] {
     fsg::MgBatchScale bs;
     if (h->batch) bs = h->batch->scale();
@@ -1902,6 +1906,8 @@ int fsg_set_kernel_clock(fsg_domain* h, int32_t enable) {
     return fail(h, FSG_E_INVALID, "fsg_set_kernel_clock: enable is 0, 1 or 2");
   h->kclock = enable;
   h->mg.clk_span_n = 0;
+  // allocated here when the hierarchy exists, so that the first clocked solve does not
+  if (enable == 2 && h->mg_ready && !h->mg.clk_span) alloc_clk_span(h);
   if (h->mg.clk) {
     HIPCHK(h, hipMemsetAsync(h->mg.clk, 0, 2 * sizeof(unsigned long long), h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
