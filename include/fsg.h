@@ -276,11 +276,13 @@ int fsg_adjoint_seed_density(fsg_domain *h, const float *rho_bar);
 int fsg_set_timing(fsg_domain *h, int32_t enable);
 /* A multigrid kernel's own clock (no counterpart in the reference): its duration
    per launch from the device wall clock, free of the launch gaps that HIP events
-   around a launch include.  enable = 1: the one-workgroup kernel (start to end);
-   enable = 2: level 0's down kernel (earliest block start to latest block end, the
-   first 4096 launches; the span buffer is kept until fsg_destroy); 0 stops it.  A
-   nonzero value clears the total, which then covers every launch, early exits
-   included. */
+   around a launch include.  enable = 1: the one-workgroup kernel k_mg_coarse (start
+   to end), which runs as its own kernel only where the grid has fewer than two tile
+   levels (elsewhere the last tile level's down kernel runs that stage, and mode 1
+   records no launch); enable = 2: level 0's down kernel (earliest block start to
+   latest block end, the first 4096 launches; the span buffer is kept until
+   fsg_destroy); 0 stops it.  A nonzero value clears the total, which then covers
+   every launch, early exits included. */
 int fsg_set_kernel_clock(fsg_domain *h, int32_t enable);
 int fsg_get_kernel_clock(fsg_domain *h, double *us_total, int64_t *launches);
 /* Name prefix selected by timing mode 2 (at most 63 characters). */
