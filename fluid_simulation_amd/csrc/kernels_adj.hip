@@ -133,13 +133,14 @@ __global__ void __launch_bounds__(kAT) k_adj_g2p(Geom g, const float4* __restric
     const int area = bw * bh;
     for (int c = threadIdx.x; c < area; c += kAT) s_tile[c] = make_float2(0.f, 0.f);
     __syncthreads();
-    if (live) {
+    {
+      // every lane takes part (the quad exchange needs the whole wave)
       float* tile = reinterpret_cast<float*>(s_tile);
       splat_terms2(cu, vbx, [&](int i, int j, float t) {
-        atomicAdd(tile + 2 * ((i - i0) + (j - j0) * bw), t);
+        lds_add_quad(tile, live ? 2 * ((i - i0) + (j - j0) * bw) : -1, live ? t : 0.f);
       });
       splat_terms2(cv, vby, [&](int i, int j, float t) {
-        atomicAdd(tile + 2 * ((i - i0) + (j - j0) * bw) + 1, t);
+        lds_add_quad(tile, live ? 2 * ((i - i0) + (j - j0) * bw) + 1 : -1, live ? t : 0.f);
       });
     }
     __syncthreads();

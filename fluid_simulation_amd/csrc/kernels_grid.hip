@@ -69,29 +69,6 @@ __device__ __forceinline__ void splat_terms(const SplatCorners& s, float value, 
   add(s.i1, s.j1, comp + 1, s.fi * w1);
 }
 
-// LDS float add of (addr, v) per lane, summed first over the lanes of each quad that
-// target the same address (neighbouring particles of one cell share their corners):
-// lanes L and L^1, then L and L^2, compare addresses through DPP quad permutations and
-// the lower lane of an equal pair takes both values.  Every value is added once; a
-// lane left with 0 (or addr < 0) issues no atomic, so the LDS sees up to 4x fewer
-// conflicting atomics.
-__device__ __forceinline__ int quad_xor1(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
-}
-__device__ __forceinline__ int quad_xor2(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
-}
-__device__ __forceinline__ void lds_add_quad(float* tile, int addr, float v) {
-  const int lane = threadIdx.x & 63;
-  const int a1 = quad_xor1(addr);
-  const float v1 = __int_as_float(quad_xor1(__float_as_int(v)));
-  if (a1 == addr) v = (lane & 1) ? 0.f : v + v1;
-  const int a2 = quad_xor2(addr);
-  const float v2 = __int_as_float(quad_xor2(__float_as_int(v)));
-  if (a2 == addr) v = (lane & 2) ? 0.f : v + v2;
-  if (addr >= 0 && v != 0.f) atomicAdd(tile + addr, v);
-}
-
 // Particle block -> (geometry, particles, grid rectangle): the whole grid for one
 // domain; the block's scene (tiles never straddle scenes) for the packed batch.
 struct PartTile {
