@@ -1695,31 +1695,79 @@ __device__ __forceinline__ MgbCells mgb_cells(const MgbTile& t, int W,
   return m;
 }
 
-// r_s . z_s
-__global__ void __launch_bounds__(kT) k_mgb_rz(MgbArgs a, const uint8_t* __restrict__ inf,
-                                               const float* __restrict__ r,
+// r_s . z_s.  A block takes kRzTiles consecutive tiles (the list runs scene by scene,
+// so they mostly share one): their r and z loads are all in flight before the first
+// use, one block sum covers them, and each run of tiles of one scene adds one atomic.
+constexpr int kRzTiles = 4;
+
+__global__ void __launch_bounds__(kT) k_mgb_rz(MgbArgs a, const float* __restrict__ r,
                                                const float* __restrict__ z, int it,
                                                const CgScalars* sc) {
-  __shared__ double sm[kT / 64];
-  const MgbTile t = mgb_tile(a);
-  if (sc->done) return;
-  if (!t.live) return;  // r = z = 0 there
-  // r = z = 0 at every cell that is not an unknown, so the info bytes are not needed
-  float rv[kMgbRows], zv[kMgbRows];
-  bool in[kMgbRows];
+  __shared__ double sm[kRzTiles * (kT / 64)];
+  const int tb = blockIdx.x * kRzTiles;
+  // flags, table entries and the stop flag in one memory round trip (the batch always
+  // has its tile flags)
+  const int done = sc->done;
+  int4 t4[kRzTiles];
+  int2 e[kRzTiles];
+  int vf[kRzTiles];
 #pragma unroll
-  for (int k = 0; k < kMgbRows; ++k) {
-    const int j = t.j0 + 4 * k;
-    in[k] = t.col && j < t.jend;
-    const int c = in[k] ? t.i + j * a.W : 0;
-    rv[k] = r[c];
-    zv[k] = z[c];
+  for (int k = 0; k < kRzTiles; ++k) {
+    const int ti = tb + k < a.n_tiles ? tb + k : a.n_tiles - 1;
+    t4[k] = a.tile[ti];
+    e[k] = a.tend[ti];
+    vf[k] = a.vflag[ti];
   }
-  double v = 0.0;
+  // r = z = 0 on a tile without unknowns.  (x0 >= 0 always holds; the test makes the
+  // early return below wait for the table entries, so that they are loaded before the
+  // branch together with the flags rather than after it.)
+  bool live[kRzTiles];
+  int any = 0;
 #pragma unroll
-  for (int k = 0; k < kMgbRows; ++k)
-    if (in[k]) v += (double)rv[k] * (double)zv[k];
-  mgb_add(v, &a.rz[(it & 1) * a.n_scenes + t.s], sm);
+  for (int k = 0; k < kRzTiles; ++k) {
+    live[k] = (tb + k < a.n_tiles) & (vf[k] != 0) & (t4[k].x >= 0);
+    any |= (int)live[k];
+  }
+  if (done | !any) return;
+  // r = z = 0 at every cell that is not an unknown, so the info bytes are not needed
+  float rv[kRzTiles][kMgbRows], zv[kRzTiles][kMgbRows];
+  bool in[kRzTiles][kMgbRows];
+#pragma unroll
+  for (int k = 0; k < kRzTiles; ++k) {
+    const int i = t4[k].x + (int)(threadIdx.x & 63);
+#pragma unroll
+    for (int q = 0; q < kMgbRows; ++q) {
+      const int j = t4[k].y + (int)(threadIdx.x >> 6) + 4 * q;
+      in[k][q] = live[k] && i < e[k].x && j < e[k].y;
+      const int c = in[k][q] ? i + j * a.W : 0;
+      rv[k][q] = r[c];
+      zv[k][q] = z[c];
+    }
+  }
+  double v[kRzTiles];
+#pragma unroll
+  for (int k = 0; k < kRzTiles; ++k) {
+    v[k] = 0.0;
+#pragma unroll
+    for (int q = 0; q < kMgbRows; ++q)
+      if (in[k][q]) v[k] += (double)rv[k][q] * (double)zv[k][q];
+  }
+  block_sum<kRzTiles, kT>(v, sm);
+  if (threadIdx.x != 0) return;
+  double* rz = a.rz + (it & 1) * a.n_scenes;
+  double acc = 0.0;
+  int s = t4[0].z;
+#pragma unroll
+  for (int k = 0; k < kRzTiles; ++k) {
+    if (tb + k >= a.n_tiles) break;
+    if (t4[k].z != s) {
+      if (acc != 0.0) unsafeAtomicAdd(&rz[s], acc);
+      s = t4[k].z;
+      acc = 0.0;
+    }
+    acc += v[k];
+  }
+  if (acc != 0.0) unsafeAtomicAdd(&rz[s], acc);
 }
 
 // p_s = z + beta_s p_old (beta_s = r.z / previous r.z of the scene), K p, p_s . K p_s
@@ -1853,11 +1901,13 @@ void launch_mgb_check(hipStream_t s, const MgbArgs& a, int it, int max_iter, dou
   k_mgb_check<<<1, kT, 0, s>>>(a, it, max_iter, tol * tol, sc);
 }
 
+// a.vflag is set with the tile table when a batch is packed (k_mgb_rz reads it as given)
 void launch_mgb_vectors(hipStream_t s, const MgbArgs& a, const MgHost& m, int it, CgScalars* sc,
                         int which) {
   if (a.n_tiles <= 0) return;
   if (which == 0)
-    k_mgb_rz<<<a.n_tiles, kT, 0, s>>>(a, m.inf[0], m.rl[0], m.xo[0], it, sc);
+    k_mgb_rz<<<(a.n_tiles + kRzTiles - 1) / kRzTiles, kT, 0, s>>>(a, m.rl[0], m.xo[0], it,
+                                                                   sc);
   else if (which == 1)
     k_mgb_dir<<<a.n_tiles, kT, 0, s>>>(a, m.inf[0], m.xo[0], m.p[it & 1], m.p[(it + 1) & 1], it,
                                         sc);
