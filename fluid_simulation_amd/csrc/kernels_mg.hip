@@ -580,36 +580,38 @@ struct BlockSpan {
   }
 };
 
-template <bool L0, int NT, bool XR = false>
+template <bool L0, int NT, bool XR = false, bool TAIL = false>
 __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omega, int it,
                                                  MgCheck chk, int check,
                                                  const uint8_t* __restrict__ tflag, MgXr xr,
-                                                 unsigned long long* span, int tail,
+                                                 unsigned long long* span,
                                                  MgCoarseArgs ca, unsigned* __restrict__ ticket) {
   static_assert(!XR || L0, "the fused update belongs to level 0");
+  static_assert(!TAIL || !L0, "the one-workgroup stage follows the last tile level");
   const BlockSpan clock(span);
   __shared__ double sm[NT / 64];
   __shared__ int s_last;
-  // tail != 0 (the last tile level, 1024 threads): the restriction is stored
+  // TAIL (the last tile level, 1024 threads): the restriction is stored
   // device-coherent and the last workgroup to finish runs the one-workgroup stage on
   // it, in place of a k_mg_coarse launch.  The arrival count is ordered after every
   // wave's stores, as in a grid barrier, but nobody waits.
   auto run_tail = [&]() {
-    if (!tail) return;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (threadIdx.x == 0)
-      s_last = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
-               gridDim.x - 1;
-    __syncthreads();
-    if (s_last) {
-      if (threadIdx.x == 0) st_dev(ticket, 0u);  // ready for the next launch
-      coarse_stage(ca, nullptr, true);
+    if constexpr (TAIL) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (threadIdx.x == 0)
+        s_last = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
+      __syncthreads();
+      if (s_last) {
+        if (threadIdx.x == 0) st_dev(ticket, 0u);  // ready for the next launch
+        coarse_stage(ca, nullptr, true);
+      }
     }
   };
   auto put_r = [&](float* dst, float v) {
-    if (tail)
+    if constexpr (TAIL)
       st_dev(dst, v);
     else
       *dst = v;
@@ -1931,7 +1933,7 @@ hipError_t mg_set_lds_limit(size_t bytes) {
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   if (e != hipSuccess) return e;
   // the last tile level's down kernel hosts the same stage (its own static LDS aside)
-  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_down<false, kTileBig, false>),
+  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_down<false, kTileBig, false, true>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   if (e != hipSuccess) return e;
   return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_scene_coarse),
@@ -2054,17 +2056,22 @@ void launch_mg_down(hipStream_t s, MgHost& m, int l, int it, int max_iter, doubl
                   vec_blocks(m.nx[0] * m.ny[0]), m.part_rr};
     k_mg_down<true, kT, true><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1),
                                                     m.coarse.omega, it, chk, 0, m.tflag, xr,
-                                                    span, 0, MgCoarseArgs{}, nullptr);
+                                                    span, MgCoarseArgs{}, nullptr);
   } else if (l == 0) {
     k_mg_down<true, kT><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega, it,
-                                              chk, check, m.tflag, MgXr{}, span, 0,
+                                              chk, check, m.tflag, MgXr{}, span,
                                               MgCoarseArgs{}, nullptr);
   } else {
-    // the last tile level also runs the one-workgroup stage (mg_tail_coarse)
-    const bool tail = l == m.lc - 1 && mg_tail_coarse(m);
-    k_mg_down<false, kTileBig><<<blocks, kTileBig, tail ? m.coarse_lds : 0, s>>>(
-        mg_level(m, l), mg_level(m, l + 1), m.coarse.omega, it, chk, check, m.tflag_l[l],
-        MgXr{}, nullptr, tail ? 1 : 0, m.coarse, m.ticket);
+    // the last tile level also runs the one-workgroup stage (mg_tail_coarse): its own
+    // instance, so that the other levels (and every level of a batch) carry none of it
+    if (l == m.lc - 1 && mg_tail_coarse(m))
+      k_mg_down<false, kTileBig, false, true><<<blocks, kTileBig, m.coarse_lds, s>>>(
+          mg_level(m, l), mg_level(m, l + 1), m.coarse.omega, it, chk, check, m.tflag_l[l],
+          MgXr{}, nullptr, m.coarse, m.ticket);
+    else
+      k_mg_down<false, kTileBig><<<blocks, kTileBig, 0, s>>>(
+          mg_level(m, l), mg_level(m, l + 1), m.coarse.omega, it, chk, check, m.tflag_l[l],
+          MgXr{}, nullptr, MgCoarseArgs{}, nullptr);
   }
 }
 
