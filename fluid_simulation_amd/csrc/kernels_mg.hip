@@ -595,7 +595,8 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
   // device-coherent and the last workgroup to finish runs the one-workgroup stage on
   // it, in place of a k_mg_coarse launch.  The arrival count is ordered after every
   // wave's stores, as in a grid barrier, but nobody waits.
-  auto run_tail = [&]() {
+  // inlined: as a call its LDS accesses would go through flat addresses
+  auto run_tail = [&]() __attribute__((always_inline)) {
     if constexpr (TAIL) {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __builtin_amdgcn_s_waitcnt(0);
