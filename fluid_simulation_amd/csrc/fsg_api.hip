@@ -170,6 +170,13 @@ struct fsg_domain {
   float4* pt = nullptr;
   int64_t np = 0, cap = 0;
   uint8_t* reject = nullptr;
+  // the splat's bucket order (one domain): slot per particle and the particles
+  // grouped by bucket (cap each, with pt), particles per bucket and bucket offsets (on
+  // first use)
+  int* prank = nullptr;
+  float4* psorted = nullptr;
+  int* bkt_cnt = nullptr;
+  int* bkt_off = nullptr;
   // pressure system
   int* counts = nullptr;
   int* offsets = nullptr;
@@ -314,10 +321,28 @@ int ph_classify(fsg_domain* h) {
 // normalise [+ prev + gravity + Dirichlet + extension init] on the new front.
 void classify_and_splat(fsg_domain* h) {
   timed(h, "cell_init", [&] { launch_cell_init(h->stream, h->g, h->label, h->solid, h->acc, 1, 1); });
+  // one domain: the splat reads the particles grouped by bucket (their storage order
+  // is kept); a batch keeps its per-scene particle tiles
+  const float4* src = h->pt;
+  if (!h->batch && h->np > 0 && h->np < INT_MAX && h->prank && h->psorted &&
+      fsg::bucket_count(h->g) <= fsg::kMaxBuckets) {
+    if (!h->bkt_cnt) {
+      const size_t nb = (size_t)fsg::bucket_count(h->g);
+      if (dalloc(h, &h->bkt_cnt, nb) != hipSuccess || dalloc(h, &h->bkt_off, nb) != hipSuccess)
+        h->bkt_cnt = h->bkt_off = nullptr;
+    }
+    if (h->bkt_cnt) {
+      timed(h, "bucket_order", [&] {
+        launch_bucket_order(h->stream, h->g, h->pt, h->np, h->bkt_cnt, h->bkt_off, h->prank,
+                            h->psorted);
+      });
+      src = h->psorted;
+    }
+  }
   timed(h, "particle_scatter", [&] {
     fsg::BatchTables bt;
     if (h->batch) bt = h->batch->tables();
-    launch_particle_scatter(h->stream, h->g, h->pt, h->np, h->label, h->acc, 1, 1,
+    launch_particle_scatter(h->stream, h->g, src, h->np, h->label, h->acc, 1, 1,
                             h->batch ? &bt : nullptr);
   });
 }
@@ -875,6 +900,8 @@ int fsg_destroy(fsg_domain* h) {
   if (h->pt) (void)hipFree(h->pt);
   if (h->emit_buf) (void)hipFree(h->emit_buf);
   if (h->reject) (void)hipFree(h->reject);
+  if (h->prank) (void)hipFree(h->prank);
+  if (h->psorted) (void)hipFree(h->psorted);
   if (h->adj.pbar) (void)hipFree(h->adj.pbar);
   if (h->adj.x0) (void)hipFree(h->adj.x0);
   for (Checkpoint& c : h->ckpt)
@@ -949,8 +976,12 @@ static int reserve_particles(fsg_domain* h, int64_t n) {
   while (cap < n) cap *= 2;
   float4* np = nullptr;
   uint8_t* nr = nullptr;
+  int* nrank = nullptr;
+  float4* nsorted = nullptr;
   HIPCHK(h, hipMalloc(&np, (size_t)cap * sizeof(float4)));
   HIPCHK(h, hipMalloc(&nr, (size_t)cap));
+  HIPCHK(h, hipMalloc(&nrank, (size_t)cap * sizeof(int)));
+  HIPCHK(h, hipMalloc(&nsorted, (size_t)cap * sizeof(float4)));
   HIPCHK(h, hipMemsetAsync(nr, 0, (size_t)cap, h->stream));
   if (h->np > 0) {
     HIPCHK(h, hipMemcpyAsync(np, h->pt, (size_t)h->np * sizeof(float4), hipMemcpyDeviceToDevice,
@@ -960,8 +991,12 @@ static int reserve_particles(fsg_domain* h, int64_t n) {
   HIPCHK(h, hipStreamSynchronize(h->stream));
   if (h->pt) (void)hipFree(h->pt);
   if (h->reject) (void)hipFree(h->reject);
+  if (h->prank) (void)hipFree(h->prank);
+  if (h->psorted) (void)hipFree(h->psorted);
   h->pt = np;
   h->reject = nr;
+  h->prank = nrank;
+  h->psorted = nsorted;
   h->cap = cap;
   return FSG_OK;
 }

@@ -34,6 +34,13 @@ void launch_cell_init(hipStream_t s, const Geom& g, uint8_t* label, const uint8_
 void launch_particle_scatter(hipStream_t s, const Geom& g, const float4* pt, int64_t np,
                              uint8_t* label, float4* acc, int do_classify, int do_p2g,
                              const BatchTables* batch = nullptr);
+// The splat's bucket order for one domain (np < 2^31, bucket_count(g) <= kMaxBuckets):
+// cnt (zero on entry, zero again on return) and off hold bucket_count(g) ints, rank np
+// ints; sorted receives the np particles grouped by bucket.
+constexpr int kMaxBuckets = 16384;  // the count kernel's LDS histogram: 64 KiB
+int bucket_count(const Geom& g);
+void launch_bucket_order(hipStream_t s, const Geom& g, const float4* pt, int64_t np, int* cnt,
+                         int* off, int* rank, float4* sorted);
 void launch_face_update(hipStream_t s, const Geom& g, int flags, float* u, float* v,
                         const float4* acc, float* prev_u, float* prev_v, const uint8_t* label,
                         float gx_dt, float gy_dt, float* ext_u, float* ext_v, uint8_t* mxf,

@@ -49,8 +49,13 @@ __global__ void __launch_bounds__(256) k_cell_init(Geom g, uint8_t* __restrict__
 // the initial fill are row-major); the tile is then added to the grid with one
 // atomic per touched component.  Otherwise every term is a global atomic.  Both
 // give the reference's terms; only the order of the float additions differs,
-// which atomic accumulation leaves unspecified anyway.
+// which atomic accumulation leaves unspecified anyway.  For one domain the particles
+// keep their storage order, and the splat reads a copy of them grouped by 16 x 16-cell
+// bucket (k_bucket_count / k_bucket_scan / k_bucket_fill, a counting sort each step):
+// a block's particles then lie in one or two neighbouring buckets and its footprint
+// fits the tile however far the particles have mixed.
 constexpr int kSplatTile = 2048;  // cells (float4 each, 32 KiB of LDS)
+constexpr int kBucketShift = 4;   // buckets of 16 x 16 cells
 
 // Adds the 8 terms of one splat (value and weight, Grid.h:152-184 order of
 // operations) through `add(cell_i, cell_j, component, term)`.
@@ -709,6 +714,90 @@ __global__ void __launch_bounds__(256) k_advect_grid(Geom g, float4* __restrict_
   pt[p] = q;
 }
 
+// ---------------------------------------------------------------- P2G bucket order
+__device__ __forceinline__ int particle_bucket(const Geom& g, float4 q, int bx) {
+  int x = (int)((q.x / g.len_x) * (float)g.nx);
+  int y = (int)((q.y / g.len_y) * (float)g.ny);
+  x = clampi(x, 0, g.nx - 1);
+  y = clampi(y, 0, g.ny - 1);
+  return (x >> kBucketShift) + (y >> kBucketShift) * bx;
+}
+
+// Particles per bucket; each particle keeps its slot in its bucket.  The block counts
+// its particles in an LDS histogram of every bucket first (slots within the block),
+// then takes one range per bucket it touches from the global count: one global atomic
+// per (block, bucket) instead of one per particle on a few hot counters.
+__global__ void __launch_bounds__(256) k_bucket_count(Geom g, const float4* __restrict__ pt,
+                                                      int np, int bx, int nb,
+                                                      int* __restrict__ cnt,
+                                                      int* __restrict__ rank) {
+  extern __shared__ int s_h[];  // nb ints
+  for (int e = threadIdx.x; e < nb; e += 256) s_h[e] = 0;
+  __syncthreads();
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  const bool live = p < np;
+  const int b = live ? particle_bucket(g, pt[p], bx) : 0;
+  const int local = live ? atomicAdd(&s_h[b], 1) : 0;
+  __syncthreads();
+  for (int e = threadIdx.x; e < nb; e += 256) {
+    const int c = s_h[e];
+    if (c) s_h[e] = atomicAdd(&cnt[e], c);
+  }
+  __syncthreads();
+  if (live) rank[p] = s_h[b] + local;
+}
+
+// Exclusive scan of the bucket counts into offsets (one workgroup, 4096 buckets per
+// pass); the counts are cleared for the next step as they are read.
+__global__ void __launch_bounds__(1024) k_bucket_scan(int* __restrict__ cnt,
+                                                      int* __restrict__ off, int nb) {
+  __shared__ int s_w[16];
+  __shared__ int s_carry;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nb; base += 4096) {
+    const int i0 = base + 4 * (int)threadIdx.x;
+    int v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = i0 + k < nb ? cnt[i0 + k] : 0;
+    const int sum = v[0] + v[1] + v[2] + v[3];
+    int x = sum;  // inclusive scan over the wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) s_w[wave] = x;
+    __syncthreads();
+    int e = s_carry + x - sum;
+    for (int w = 0; w < wave; ++w) e += s_w[w];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (i0 + k < nb) {
+        off[i0 + k] = e;
+        cnt[i0 + k] = 0;
+      }
+      e += v[k];
+    }
+    __syncthreads();  // s_carry and s_w read by every thread
+    if (threadIdx.x == 1023) s_carry = e;
+    __syncthreads();
+  }
+}
+
+// sorted[offset of the bucket + slot] = particle (the scattered side is the store).
+__global__ void __launch_bounds__(256) k_bucket_fill(Geom g, const float4* __restrict__ pt,
+                                                     int np, int bx, const int* __restrict__ off,
+                                                     const int* __restrict__ rank,
+                                                     float4* __restrict__ sorted) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= np) return;
+  const float4 q = pt[p];
+  const int d = off[particle_bucket(g, q, bx)] + rank[p];
+  if (d >= 0 && d < np) sorted[d] = q;
+}
+
 // Plain fill used instead of hipMemsetAsync (stays valid inside captured graphs).
 __global__ void __launch_bounds__(256) k_fill_f32(float* __restrict__ a, int64_t n, float val) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -735,6 +824,22 @@ void launch_particle_scatter(hipStream_t s, const Geom& g, const float4* pt, int
   if (np == 0) return;
   k_particle_scatter<false><<<nblocks(np), 256, 0, s>>>(g, pt, np, label, (float*)acc,
                                                         do_classify, do_p2g, BatchTables{});
+}
+
+int bucket_count(const Geom& g) {
+  return ((g.nx + (1 << kBucketShift) - 1) >> kBucketShift) *
+         ((g.ny + (1 << kBucketShift) - 1) >> kBucketShift);
+}
+
+void launch_bucket_order(hipStream_t s, const Geom& g, const float4* pt, int64_t np, int* cnt,
+                         int* off, int* rank, float4* sorted) {
+  if (np <= 0) return;
+  const int bx = (g.nx + (1 << kBucketShift) - 1) >> kBucketShift;
+  const int nb = bucket_count(g);
+  const int n = (int)np;
+  k_bucket_count<<<nblocks(np), 256, (size_t)nb * sizeof(int), s>>>(g, pt, n, bx, nb, cnt, rank);
+  k_bucket_scan<<<1, 1024, 0, s>>>(cnt, off, nb);
+  k_bucket_fill<<<nblocks(np), 256, 0, s>>>(g, pt, n, bx, off, rank, sorted);
 }
 void launch_face_update(hipStream_t s, const Geom& g, int flags, float* u, float* v,
                         const float4* acc, float* prev_u, float* prev_v, const uint8_t* label,
