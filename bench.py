@@ -109,7 +109,7 @@ MEASURED_TRAFFIC_1024 = {
 
 
 # timers that wrap more than one kernel (or one kernel at several levels)
-MULTI_KERNEL_TIMERS = {"cg_mg_setup", "cg_mg_finish", "cg_mg_down3+", "cg_mg_up3+"}
+MULTI_KERNEL_TIMERS = {"cg_mg_setup", "cg_mg_finish", "cg_mg_down3+", "cg_mg_up3+", "bucket_order"}
 
 
 def log(*a):
