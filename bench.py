@@ -55,15 +55,13 @@ def mg_levels(nx, ny, coarse_cells=1024):
 
 
 def kernel_bytes(name, nx, ny, P, levels=None):
-    """Algorithmic HBM bytes of one launch of a kernel, each array read or written
-    once (dense cell grids): nx * ny cells, P particles.  `levels` = (cells per level,
-    first one-workgroup level) overrides the single-grid hierarchy (a batch's packed
-    levels, whose last one is cycled scene by scene)."""
+    """Algorithmic HBM bytes of one launch of a kernel over dense cell grids, each
+    array read or written once: nx * ny cells, P particles.  `levels` = (cells per
+    level, first one-workgroup level) overrides the single-grid hierarchy (a batch's
+    packed levels, whose last one is cycled scene by scene).  Level-0 tile kernels of
+    one domain skip the tiles without an unknown: solve_bytes counts those."""
     C = nx * ny
     cells, lc = levels or mg_levels(nx, ny)
-    # one domain with two or more tile levels: level 0's down kernel also does the
-    # previous iteration's x / residual update (k_mg_xr's bytes), except at iteration 0
-    fused = levels is None and lc >= 2
     table = {
         # particles 16 B read + label 1 B; float4 accumulators read and written once
         "particle_scatter": 17.0 * P + 32.0 * C,
@@ -80,15 +78,57 @@ def kernel_bytes(name, nx, ny, P, levels=None):
         return table[name]
     for pre, fine_b, coarse_b in (("cg_mg_down", 9.0, 5.0), ("cg_mg_up", 13.0, 5.0)):
         if name.startswith(pre):
-            tail = name[len(pre):]
-            ls = range(3, lc) if tail.endswith("+") else [int(tail)]
-            # a "3+" timer covers every launch of levels >= 3: average over them
-            ls = list(ls) or [0]
-            b = sum(fine_b * cells[l] + coarse_b * cells[l + 1] for l in ls) / len(ls)
-            if fused and pre == "cg_mg_down" and tail == "0":
-                b += table["cg_mg_xr"]
+            tail = name[len(pre):].rstrip("+")
+            l = int(tail)
+            b = fine_b * cells[l] + coarse_b * cells[l + 1]
+            if pre == "cg_mg_down" and levels is None and l == lc - 1 and lc >= 2:
+                # the last tile level's down kernel also runs the one-workgroup stage
+                b += table["cg_mg_coarse"]
             return b
     return None
+
+
+# Level-0 tiles of one domain: 32 x 32 cells (and their 16 x 16 level-1 cells).  The
+# level-0 kernels process only the tiles holding an unknown ("live"); the others
+# write their zero outputs in the first iterations of a solve and return.
+TILE = 32 * 32
+
+
+def solve_bytes(name, nx, ny, solves):
+    """Algorithmic bytes of every launch of solver kernel `name` over a list of
+    solves (iterations n, live level-0 tiles L) of one domain (two or more tile
+    levels: the fused form).  A solve of n iterations runs level 0's down kernel n + 1
+    times (iteration 0 without the update, 1 .. n with the previous iteration's x /
+    residual update; the stop test of iteration n fires in level 1's down kernel) and
+    every other kernel n times; launches enqueued after the stop test move nothing."""
+    cells, lc = mg_levels(nx, ny)
+    tiles = ((nx + 31) // 32) * ((ny + 31) // 32)
+    tot = 0.0
+    for n, live in solves:
+        dead = tiles - live
+        if name == "cg_mg_down0" and lc >= 2:
+            # live tile: r, info read and xs written on 1024 cells (9 B), r and info of
+            # the 256 level-1 cells (5 B); with the update also info, p, x_old, b read
+            # and x_new, r written (33 B per cell).  Dead tiles: zero xs and level-1 r
+            # at iteration 0, zero x_new (8 B) and r (4 B) at 1, x_new at 2.
+            plain = 9.0 * TILE + 5.0 * 256
+            tot += live * plain + dead * (4.0 * TILE + 4.0 * 256)
+            tot += n * live * (plain + 33.0 * TILE)
+            tot += dead * ((12.0 * TILE if n >= 1 else 0.0) + (8.0 * TILE if n >= 2 else 0.0))
+        elif name == "cg_mg_up0" and lc >= 1:
+            # live tile: r, xs, info read, z written (13 B), level-1 x and info (5 B);
+            # dead tiles write z = 0 at iteration 0
+            tot += n * live * (13.0 * TILE + 5.0 * 256) + (dead * 4.0 * TILE if n >= 1 else 0.0)
+        elif name == "cg_mg_dir" and lc >= 1:
+            # live tile: info, z, p_old read, p_new written (13 B); dead tiles write
+            # p = 0 at iterations 0 and 1
+            tot += n * live * 13.0 * TILE + dead * 4.0 * TILE * min(n, 2)
+        else:
+            b = kernel_bytes(name, nx, ny, 0)
+            if b is None:
+                return None
+            tot += n * b
+    return tot
 
 
 # HBM traffic per launch on the 1024^2 dam break (bytes), from two rocprofv3 --pmc
@@ -109,7 +149,7 @@ MEASURED_TRAFFIC_1024 = {
 
 
 # timers that wrap more than one kernel (or one kernel at several levels)
-MULTI_KERNEL_TIMERS = {"cg_mg_setup", "cg_mg_finish", "cg_mg_down3+", "cg_mg_up3+", "bucket_order"}
+MULTI_KERNEL_TIMERS = {"cg_mg_setup", "cg_mg_finish", "cg_mg_down7+", "cg_mg_up7+", "bucket_order"}
 
 
 def log(*a):
@@ -128,14 +168,24 @@ def run_steps(fsg, d, n, dt, scheme=None):
         d.step(scheme, dt)
 
 
-def solver_gbs(prof, nx, ny, P, levels=None):
-    """The pressure solver's algorithmic bytes over its kernel time (GB/s)."""
+def solver_gbs(prof, nx, ny, P, levels=None, solves=None):
+    """The pressure solver's algorithmic bytes over its kernel time (GB/s): dense
+    counts for a batch (`levels`), live-tile counts (`solves`: (iterations, live
+    tiles) per solve of the profiled steps) for one domain."""
     tot_b = tot_ms = 0.0
     for k, (ms, n) in prof.items():
-        b = kernel_bytes(k, nx, ny, P, levels) if k.startswith("cg_") else None
-        if b is not None:
-            tot_b += b * n
-            tot_ms += ms * n
+        if not k.startswith("cg_") or k in MULTI_KERNEL_TIMERS:
+            continue
+        if solves is not None:
+            b = solve_bytes(k, nx, ny, solves)
+            if b is not None:
+                tot_b += b
+                tot_ms += ms * n * len(solves)
+        else:
+            b = kernel_bytes(k, nx, ny, P, levels)
+            if b is not None:
+                tot_b += b * n
+                tot_ms += ms * n
     return tot_b / (tot_ms * 1e-3) / 1e9 if tot_ms > 0 else None
 
 
@@ -152,13 +202,19 @@ def batch_levels(sizes, W, H, coarse_cells=4096):
     return [a * b for a, b in lv], la
 
 
-def kernel_profile(fsg, d, steps, dt, scheme=None):
+def kernel_profile(fsg, d, steps, dt, scheme=None, solves=None):
     """Per-kernel HIP-event times over `steps` steps (continuing the trajectory):
-    {name: (ms per launch, launches per step)}."""
+    {name: (ms per launch, launches per step)}; `solves` receives each step's
+    (iterations, live level-0 tiles)."""
     d.synchronize()
     d.reset_timing()
     d.set_timing(1)
-    run_steps(fsg, d, steps, dt, scheme)
+    scheme = fsg.PICFLIP if scheme is None else scheme
+    for _ in range(steps):
+        d.step(scheme, dt)
+        if solves is not None:
+            st = d.stats()
+            solves.append((st.cg_iterations, st.mg_live_tiles))
     d.synchronize()
     tm = d.timing()
     d.set_timing(0)
@@ -167,8 +223,9 @@ def kernel_profile(fsg, d, steps, dt, scheme=None):
 
 def timed_segment(fsg, d, steps, dt, kernel=None, scheme=None):
     """K steps bracketed by synchronisations; with `kernel`, that kernel (and only
-    it) is timed with HIP events on the solver's stream inside the region, and the
-    solver's iterations of every step are recorded (host-side counters, no sync)."""
+    it) is timed with HIP events on the solver's stream inside the region, and each
+    step's solver iterations and live level-0 tiles are recorded (host-side
+    counters, no sync)."""
     d.synchronize()
     d.reset_stats()
     d.reset_timing()
@@ -179,13 +236,14 @@ def timed_segment(fsg, d, steps, dt, kernel=None, scheme=None):
     if clock:  # the kernel's own device clock as well (no launch gaps)
         d.set_kernel_clock(True, clock)
     scheme = fsg.PICFLIP if scheme is None else scheme
-    iters = []
+    solves = []
     d.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         d.step(scheme, dt)
         if kernel:
-            iters.append(d.stats().cg_iterations)
+            st = d.stats()
+            solves.append((st.cg_iterations, st.mg_live_tiles))
     d.synchronize()
     t1 = time.perf_counter()
     st = d.stats()
@@ -196,9 +254,32 @@ def timed_segment(fsg, d, steps, dt, kernel=None, scheme=None):
         if n > 0:
             tm["__clock__"] = (us * 1e-3, n)
     if kernel:
-        tm["__iters__"] = iters
+        tm["__solves__"] = solves
     d.set_timing(0)
     return t1 - t0, st, tm
+
+
+def full_run(fsg, scene, solver, tol, steps):
+    """The configuration's whole run (`steps` steps from the initial state) on a
+    domain of its own: wall time, and every solve's iterations and FP64 relative
+    residual (host-side counters read after each step, no extra synchronisation)."""
+    d = make_domain(fsg, scene, solver, tol, 200 if solver == fsg.SOLVER_MG else 20000)
+    its, errs = [], []
+    d.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        d.step(fsg.PICFLIP, scene.dt)
+        st = d.stats()
+        its.append(st.cg_iterations)
+        errs.append(st.cg_error)
+    d.synchronize()
+    t1 = time.perf_counter()
+    d.close()
+    return {"steps": steps, "seconds": t1 - t0, "steps_per_s": steps / (t1 - t0),
+            "cg_iterations_min": int(min(its)), "cg_iterations_mean": float(np.mean(its)),
+            "cg_iterations_max": int(max(its)), "cg_rel_residual_max": float(max(errs)),
+            "all_solves_within_tol": bool(max(errs) <= tol),
+            "cg_iterations_by_step": {str(k): int(its[k]) for k in range(0, steps, 50)}}
 
 
 def fp64_single_step_error(fsg, scene, solver, tol, state=None):
@@ -486,6 +567,12 @@ def main():
     ap.add_argument("--solver", default="mg", choices=list(SOLVERS))
     ap.add_argument("--tol", type=float, default=1e-6,
                     help="pressure solve: relative residual (FP64 residual for mg / refined)")
+    ap.add_argument("--start", type=int, default=250,
+                    help="first timed step: the trajectory is advanced untimed to it (the "
+                         "configuration's 500-step run is in its developed flow there)")
+    ap.add_argument("--run-steps", type=int, default=500,
+                    help="the configuration's whole run, timed on its own domain")
+    ap.add_argument("--refined-steps", type=int, default=10)
     ap.add_argument("--plain-steps", type=int, default=20)
     ap.add_argument("--plain-tol", type=float, default=1e-6)
     ap.add_argument("--profile-steps", type=int, default=3)
@@ -541,13 +628,29 @@ def main():
     max_iter = 200 if solver == fsg.SOLVER_MG else 20000
     log(f"workload dam_break_{args.n}: {P} particles; solver {args.solver} tol {args.tol:g}")
 
-    # ---- chosen form: warmup, per-kernel profile (untimed), timed region
+    # ---- the configuration's whole run (500 steps from the initial state), on a
+    # domain of its own: average rate, iterations and the largest FP64 residual
+    full = full_run(fsg, scene, solver, args.tol, args.run_steps) if args.run_steps > 0 else None
+    if full:
+        log(f"whole run: {full['steps']} steps at {full['steps_per_s']:.1f} steps/s, "
+            f"{full['cg_iterations_min']}-{full['cg_iterations_max']} iterations "
+            f"(mean {full['cg_iterations_mean']:.1f}), max rel residual "
+            f"{full['cg_rel_residual_max']:.2e}")
+
+    # ---- chosen form: W warmup steps, the trajectory advanced untimed to step
+    # --start (the developed flow), a per-kernel profile, then the timed region
     d = make_domain(fsg, scene, solver, args.tol, max_iter, args.warm_start)
     run_steps(fsg, d, args.warmup, dt)
-    prof = kernel_profile(fsg, d, args.profile_steps, dt) if args.profile_steps > 0 else {}
+    advance = max(args.start - args.warmup - args.profile_steps, 0)
+    run_steps(fsg, d, advance, dt)
+    prof_solves = []
+    prof = (kernel_profile(fsg, d, args.profile_steps, dt, solves=prof_solves)
+            if args.profile_steps > 0 else {})
+    t_start = args.warmup + advance + args.profile_steps
+    state0 = d.save_state() if (args.plain_steps > 0 or args.refined_steps > 0) else None
     per_step = {k: ms * n for k, (ms, n) in prof.items()}
     # the roofline kernel: the one kernel with the most time per step (timers that
-    # cover several kernels or levels are not candidates)
+    # cover several kernels are not candidates)
     single = {k: v for k, v in per_step.items() if k not in MULTI_KERNEL_TIMERS}
     dominant = args.roofline_kernel or (max(single, key=single.get) if single else None)
     log("kernel profile (ms per step): " +
@@ -557,49 +660,87 @@ def main():
     d.close()
     sps = args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
-    log(f"chosen: {sps:.3f} steps/s ({ms_per_step:.3f} ms/step), dominant kernel {dominant}")
+    log(f"chosen: {sps:.3f} steps/s ({ms_per_step:.3f} ms/step) over steps {t_start}.."
+        f"{t_start + args.steps - 1}, dominant kernel {dominant}")
 
-    # roofline of the dominant kernel, timed live in the timed region
+    # roofline of the dominant kernel, timed live in the timed region: HIP events
+    # around each of its launches on the solver's stream (they also count the
+    # dispatch before the kernel starts), and the kernel's own device clock beside
+    one_domain_mg = solver == fsg.SOLVER_MG and mg_levels(args.n, args.n)[1] >= 2
+    solves = tm.get("__solves__", [])
     roof = None
     if dominant and dominant in tm:
-        ev_ms = tm[dominant][0] / max(tm[dominant][1], 1)
-        # the kernel's own device clock where it has one; HIP events around each
-        # launch (on the solver's stream) also count the launch gaps
-        k_ms = tm["__clock__"][0] / tm["__clock__"][1] if "__clock__" in tm else ev_ms
-        k_bytes = kernel_bytes(dominant, args.n, args.n, P)
-        n_launch = tm["__clock__"][1] if "__clock__" in tm else tm[dominant][1]
-        if dominant == "cg_mg_down0" and mg_levels(args.n, args.n)[1] >= 2:
-            # fused form: iteration 0 of a solve has no update; iterations 1 .. it_s
-            # add k_mg_xr's bytes; launches after the stop test exit at once
-            plain = kernel_bytes(dominant, args.n, args.n, P) - 33.0 * args.n * args.n
-            total = sum(plain + it_s * (plain + 33.0 * args.n * args.n)
-                        for it_s in tm["__iters__"])
-            k_bytes = total / n_launch
-        ach = k_bytes / (k_ms * 1e-3) / 1e9 if k_bytes else None
+        ev_ms_total, n_launch = tm[dominant]
+        ev_ms = ev_ms_total / max(n_launch, 1)
+        if one_domain_mg and dominant.startswith("cg_"):
+            total_b = solve_bytes(dominant, args.n, args.n, solves)
+        else:
+            kb = kernel_bytes(dominant, args.n, args.n, P)
+            total_b = kb * n_launch if kb is not None else None
+        k_bytes = total_b / n_launch if total_b is not None else None
+        ach = k_bytes / (ev_ms * 1e-3) / 1e9 if k_bytes else None
         roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": ach / HBM_PEAK_GBS if ach else None,
                 "traffic": MEASURED_TRAFFIC_1024.get(dominant) if args.n == 1024 else None,
-                "kernel": dominant, "kernel_us_per_launch": 1e3 * k_ms,
-                "kernel_us_per_launch_events": 1e3 * ev_ms,
-                "timing": "device clock" if "__clock__" in tm else "HIP events",
-                "launches": n_launch,
+                "kernel": dominant, "timing": "HIP events",
+                "kernel_us_per_launch": 1e3 * ev_ms, "launches": n_launch,
                 "algorithmic_bytes_per_launch": k_bytes}
+        if one_domain_mg and solves:
+            roof["live_level0_tiles_mean"] = float(np.mean([lv for _, lv in solves]))
+            roof["level0_tiles"] = ((args.n + 31) // 32) ** 2
+            roof["cg_iterations_per_solve"] = [n for n, _ in solves]
+        if "__clock__" in tm and k_bytes:
+            ck_ms = tm["__clock__"][0] / tm["__clock__"][1]
+            roof["kernel_us_per_launch_device_clock"] = 1e3 * ck_ms
+            roof["achieved_device_clock"] = k_bytes / (ck_ms * 1e-3) / 1e9
+            roof["frac_device_clock"] = roof["achieved_device_clock"] / HBM_PEAK_GBS
+    # every solver kernel of the profile pass (HIP events per launch, live-tile bytes):
+    # the latency-bound small levels beside the roofline kernel
+    solver_kernels = {}
+    if one_domain_mg and prof_solves:
+        for k, (ms, n) in sorted(prof.items()):
+            if not k.startswith("cg_") or k in MULTI_KERNEL_TIMERS:
+                continue
+            b = solve_bytes(k, args.n, args.n, prof_solves)
+            launches = n * len(prof_solves)
+            if b is None or launches == 0:
+                continue
+            gbs = b / launches / (ms * 1e-3) / 1e9
+            solver_kernels[k] = {"us_per_launch_events": 1e3 * ms, "launches_per_step": n,
+                                 "bytes_per_launch": b / launches, "gbs": gbs,
+                                 "frac": gbs / HBM_PEAK_GBS}
     # "CG HBM GB/s": algorithmic bytes of every pressure-solver kernel over their time
-    cg_gbs = solver_gbs(prof, args.n, args.n, P)
+    cg_gbs = solver_gbs(prof, args.n, args.n, P,
+                        solves=prof_solves if one_domain_mg and prof_solves else None)
 
-    # ---- plain FP32 Jacobi-PCG form over the same trajectory segment
-    plain_sps = None
-    if args.plain_steps > 0 and solver != fsg.SOLVER_PLAIN:
-        dp = make_domain(fsg, scene, fsg.SOLVER_PLAIN, args.plain_tol, 20000)
-        run_steps(fsg, dp, args.warmup + args.profile_steps, dt)
-        pk = min(args.plain_steps, args.steps)
-        e_p, st_p, _ = timed_segment(fsg, dp, pk, dt)
+    # ---- the plain FP32 Jacobi-PCG form and the refined precision form (FP32 PCG
+    # inside FP64 iterative refinement), each from the state at the timed window's
+    # start (the same trajectory segment)
+    def other_form(form, steps, tol):
+        dp = make_domain(fsg, scene, form, tol, 20000)
+        dp.load_state(state0)
+        e_p, st_p, _ = timed_segment(fsg, dp, steps, dt)
         dp.close()
-        plain_sps = pk / e_p
+        return steps / e_p, st_p
+
+    plain_sps = refined = None
+    if args.plain_steps > 0 and solver != fsg.SOLVER_PLAIN:
+        pk = min(args.plain_steps, args.steps)
+        plain_sps, st_p = other_form(fsg.SOLVER_PLAIN, pk, args.plain_tol)
         log(f"plain: {plain_sps:.3f} steps/s over {pk} steps ({st_p.cg_iterations} iterations "
             f"in the last solve)")
     elif solver == fsg.SOLVER_PLAIN:
         plain_sps = sps
+    if args.refined_steps > 0 and solver != fsg.SOLVER_REFINED:
+        rk = min(args.refined_steps, args.steps)
+        r_sps, st_r = other_form(fsg.SOLVER_REFINED, rk, args.tol)
+        refined = {"steps_per_s": r_sps, "steps": rk, "tol": args.tol,
+                   "inner_tol": 1e-3, "cg_iterations_last_solve": st_r.cg_iterations,
+                   "refine_sweeps_last_solve": st_r.refine_sweeps,
+                   "rel_residual_last_solve": st_r.cg_error,
+                   "speedup_vs_plain": r_sps / plain_sps if plain_sps else None}
+        log(f"refined: {r_sps:.3f} steps/s over {rk} steps ({st_r.cg_iterations} inner "
+            f"iterations, {st_r.refine_sweeps} FP64 sweeps in the last solve)")
 
     # ---- single-step error against the FP64 oracle, from the state the timed run
     # reached (at step 0 the projected field is hydrostatic, ~0, and relative
@@ -608,8 +749,14 @@ def main():
     if not args.no_fp64_check:
         t0 = time.perf_counter()
         errs, same_labels = fp64_single_step_error(fsg, scene, solver, args.tol, state)
-        log(f"fp64 check at step {args.warmup + args.profile_steps + args.steps} "
+        log(f"fp64 check at step {t_start + args.steps} "
             f"({time.perf_counter() - t0:.1f} s): {errs} labels equal {same_labels}")
+        if refined is not None:
+            e_r, same_r = fp64_single_step_error(fsg, scene, fsg.SOLVER_REFINED, args.tol, state)
+            refined["rel_err_vs_fp64_single_step"] = e_r
+            refined["max_rel_err_vs_fp64"] = max(e_r.values())
+            refined["labels_equal_fp64"] = same_r
+            log(f"refined fp64 check: max {max(e_r.values()):.2e}, labels equal {same_r}")
 
     # ---- packed batch of scenes; grid-only advection + projection
     batch = batch_leg(fsg, scenes, args) if args.batch_scenes > 0 else None
@@ -645,10 +792,15 @@ def main():
                         f"{P} particles, dt 0.01, density 0.01, 0.95 FLIP",
             "solver": args.solver,
             "tol": args.tol,
+            "timed_steps": [t_start, t_start + args.steps - 1],
+            "untimed_steps_before": t_start,
+            "whole_run": full,
             "plain_tol": args.plain_tol,
             "plain_steps_per_s": plain_sps,
             "speedup_vs_plain": (sps / plain_sps) if plain_sps else None,
+            "refined": refined,
             "cg_hbm_gbs": cg_gbs,
+            "solver_kernels": solver_kernels,
             "cg_iterations_last_solve": st.cg_iterations,
             "rel_err_vs_fp64_single_step": errs,
             "max_rel_err_vs_fp64": max(errs.values()) if errs else None,

@@ -433,7 +433,9 @@ int mg_prepare(fsg_domain* h) {
   ok &= dalloc(h, &m.part_cnt, kMgMaxVecBlocks) == hipSuccess;
   ok &= dalloc(h, &m.part_bb, kMgMaxVecBlocks) == hipSuccess;
   ok &= dalloc(h, &m.part_rr, (size_t)mg_part_rr_size(m)) == hipSuccess;
-  ok &= dalloc(h, &m.part_pq, kMgMaxVecBlocks) == hipSuccess;
+  // k_mg_dir: one partial per vector block, or per level-0 tile where tiles are skipped
+  ok &= dalloc(h, &m.part_pq, (size_t)std::max(kMgMaxVecBlocks, mg_up_blocks(m.nx[0], m.ny[0]))) ==
+        hipSuccess;
   ok &= dalloc(h, &m.part_rz, (size_t)std::max(mg_up_blocks(m.nx[0], m.ny[0]), 1)) == hipSuccess;
   ok &= dalloc(h, &m.clk, 2) == hipSuccess;
   if (m.lc > 0) ok &= dalloc(h, &m.tflag, (size_t)mg_up_blocks(m.nx[0], m.ny[0])) == hipSuccess;
@@ -527,9 +529,13 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
   int& last_iters = adj_rhs ? h->mg_last_iters_adj : h->mg_last_iters;
   int it = 0, batch = std::max(last_iters + 1, 4), more = 2;
   CgScalars hs{};
-  static const char* const kDownNames[4] = {"cg_mg_down0", "cg_mg_down1", "cg_mg_down2",
-                                            "cg_mg_down3+"};
-  static const char* const kUpNames[4] = {"cg_mg_up0", "cg_mg_up1", "cg_mg_up2", "cg_mg_up3+"};
+  // one timer name per level (the last tile level's down kernel also runs the
+  // one-workgroup stage)
+  static const char* const kDownNames[8] = {"cg_mg_down0", "cg_mg_down1", "cg_mg_down2",
+                                            "cg_mg_down3", "cg_mg_down4", "cg_mg_down5",
+                                            "cg_mg_down6", "cg_mg_down7+"};
+  static const char* const kUpNames[8] = {"cg_mg_up0", "cg_mg_up1", "cg_mg_up2", "cg_mg_up3",
+                                          "cg_mg_up4", "cg_mg_up5", "cg_mg_up6", "cg_mg_up7+"};
   for (;;) {
     for (int q = 0; q < batch && it <= max_iter; ++q, ++it) {
       if (h->batch)
@@ -537,14 +543,14 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
           launch_mgb_check(h->stream, h->batch->cg_args(h->g.nx, h->g.ny), it, max_iter, tol, h->sc);
         });
       for (int l = 0; l < m.lc; ++l)
-        timed(h, kDownNames[std::min(l, 3)],
+        timed(h, kDownNames[std::min(l, 7)],
               [&] { launch_mg_down(h->stream, m, l, it, max_iter, tol, h->sc); });
       if (h->batch)
         timed(h, "cg_mg_coarse", [&] { launch_mg_scene_coarse(h->stream, scene_coarse_args(h), h->sc); });
       else if (!mg_tail_coarse(m))
         timed(h, "cg_mg_coarse", [&] { launch_mg_coarse(h->stream, m, it, max_iter, tol, h->sc); });
       for (int l = m.lc - 1; l >= 0; --l)
-        timed(h, kUpNames[std::min(l, 3)], [&] { launch_mg_up(h->stream, m, l, it, h->sc); });
+        timed(h, kUpNames[std::min(l, 7)], [&] { launch_mg_up(h->stream, m, l, it, h->sc); });
       if (h->batch) {
         const fsg::MgbArgs ba = h->batch->cg_args(h->g.nx, h->g.ny);
         timed(h, "cg_mg_rz", [&] { launch_mgb_vectors(h->stream, ba, m, it, h->sc, 0); });
@@ -576,6 +582,7 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
   h->mg_solved = true;
   h->host_stats.cg_iterations = hs.iter;
   h->host_stats.cg_error = hs.err;
+  h->host_stats.mg_live_tiles = hs.live_tiles;
   h->host_stats.total_unknown_iterations += (int64_t)F * hs.iter;
   h->host_stats.total_solves += 1;
   return FSG_OK;

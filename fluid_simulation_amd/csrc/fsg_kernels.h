@@ -96,7 +96,7 @@ struct CgScalars {
   int spmv;              // SpMV applications
   int done;              // 1 once the solve has stopped
   int refine_sweeps;
-  int pad;
+  int live_tiles;        // multigrid form: level-0 32 x 32 tiles holding an unknown
   double b64_norm2;      // refined form: |b|^2 in FP64
   double res64_norm2;    // refined form: |b - A x|^2 in FP64
   unsigned long long total_unknown_iters;
@@ -256,6 +256,7 @@ int mg_vec_blocks(int C);
 bool mg_tail_coarse(const MgHost& m);  // the one-workgroup stage runs in the down kernel
 int mg_part_rr_size(const MgHost& m);  // entries of MgHost::part_rr
 int mg_up_blocks(int nx, int ny);
+int mg_dir_blocks(const MgHost& m);  // blocks of k_mg_dir (partials of p.Kp)
 hipError_t mg_set_lds_limit(size_t bytes);
 // adj_rhs != null: the right-hand side is the adjoint solve's cell field (the
 // pressure's adjoint) instead of the divergence of (uf, vf); see k_mg_adj_rhs0

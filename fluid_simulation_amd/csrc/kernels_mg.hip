@@ -366,6 +366,7 @@ __global__ void __launch_bounds__(kT) k_mg_setup1(int nx, int ny, int project_me
     sc->spmv = 0;
     sc->err = 0.0;
     sc->rho[0] = sc->rho[1] = 0.0;
+    sc->live_tiles = 0;
   }
 }
 
@@ -1440,6 +1441,88 @@ __global__ void __launch_bounds__(kT) k_mg_dir(int nx, int ny, const uint8_t* __
   write_part<kT>(acc, part_pq, sm);
 }
 
+// The same on level 0's 32 x 32 tiles (one block per tile, 4 rows of a column per
+// thread), skipping the tiles that hold no unknown (tflag, as the level-0 tile
+// kernels): their p is zero, written in both ping-pong buffers once per solve
+// (iterations 0 and 1), and their p.Kp partial is 0.
+__global__ void __launch_bounds__(kT) k_mg_dir_tiles(int nx, int ny,
+                                                      const uint8_t* __restrict__ inf,
+                                                      const float* __restrict__ z,
+                                                      const float* __restrict__ p_old,
+                                                      float* __restrict__ p_new, int it,
+                                                      const double* __restrict__ part_rz,
+                                                      int n_rz, double* __restrict__ part_pq,
+                                                      CgScalars* sc,
+                                                      const uint8_t* __restrict__ tflag) {
+  constexpr int R = 32 * 32 / kT;  // cells per thread
+  __shared__ double sm[4];
+  const int tiles_x = (nx + 31) >> 5;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = tile % tiles_x, by = tile / tiles_x;
+  const int i = bx * 32 + (threadIdx.x & 31), j0 = by * 32 + (threadIdx.x >> 5);
+  // the tile flag and the stop flag are read together (one memory round trip)
+  const bool empty = !tflag[tile];
+  if (sc->done) return;
+  if (empty) {
+    if (it <= 1)
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int j = j0 + k * (kT / 32);
+        if (i < nx && j < ny) p_new[i + j * nx] = 0.f;
+      }
+    if (threadIdx.x == 0) part_pq[blockIdx.x] = 0.0;
+    if (blockIdx.x != 0) return;
+  }
+  // block 0 records rho = r.z, whether its tile is empty or not
+  const double rz = sum_part<kT>(part_rz, n_rz, sm);
+  const double beta = it == 0 ? 0.0 : rz / sc->rho[(it - 1) & 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->rho[it & 1] = rz;
+  if (empty) return;
+  const float bf = (float)beta;
+  // every load of the thread's cells first (cells outside the grid read their own
+  // clamped index and are masked), then the stencil
+  Stencil5 st[R];
+  uint8_t f[R];
+  float zv[R][5], pv[R][5];
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int j = j0 + k * (kT / 32);
+    const bool in = i < nx && j < ny;
+    const int c = in ? i + j * nx : 0;
+    st[k] = stencil5(c, in ? i : 0, in ? j : 0, nx, ny, in);
+    f[k] = inf[st[k].q[0]];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      zv[k][q] = z[st[k].q[q]];
+      pv[k][q] = it == 0 ? 0.f : p_old[st[k].q[q]];
+    }
+  }
+  double acc = 0.0;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    if (!st[k].m[0]) continue;
+    float pn[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      const float v = it == 0 ? zv[k][q] : zv[k][q] + bf * pv[k][q];
+      pn[q] = st[k].m[q] ? v : 0.f;
+    }
+    float pc = 0.f;
+    if (is_act(f[k])) {
+      pc = pn[0];
+      float sn = 0.f;
+      sn += pn[1];
+      sn += pn[2];
+      sn += pn[3];
+      sn += pn[4];
+      const float kp = nns_of(f[k]) * pc - sn;
+      acc += (double)pc * (double)kp;
+    }
+    p_new[st[k].q[0]] = pc;
+  }
+  write_part<kT>(acc, part_pq, sm);
+}
+
 // x_new = x_old + alpha p (FP64), r = b' - K x_new (FP64, stored as FP32),
 // partial |r|^2 (FP64).
 __global__ void __launch_bounds__(kT) k_mg_xr(int nx, int ny, const uint8_t* __restrict__ inf,
@@ -1556,10 +1639,12 @@ __global__ void __launch_bounds__(kT) k_mg_finish(int C, const uint8_t* __restri
 
 // Level-0 tiles (32 x 32 fine cells and their 16 x 16 coarse cells) with no unknown on
 // either level: the level-0 tile kernels skip them.  One block per tile.
+// Level 0 (count != null) also counts its live tiles (cleared by k_mg_setup1).
 __global__ void __launch_bounds__(kT) k_mg_tile_flags(int nx, int ny,
                                                       const uint8_t* __restrict__ inf0, int cnx,
                                                       int cny, const uint8_t* __restrict__ inf1,
-                                                      uint8_t* __restrict__ flag) {
+                                                      uint8_t* __restrict__ flag,
+                                                      int* __restrict__ count) {
   const int tiles_x = (nx + 31) >> 5;
   const int bx = blockIdx.x % tiles_x, by = blockIdx.x / tiles_x;
   int any = 0;
@@ -1572,7 +1657,10 @@ __global__ void __launch_bounds__(kT) k_mg_tile_flags(int nx, int ny,
     if (I < cnx && J < cny && is_act(inf1[I + J * cnx])) any = 1;
   }
   any = __syncthreads_or(any);
-  if (threadIdx.x == 0) flag[blockIdx.x] = any ? 1 : 0;
+  if (threadIdx.x == 0) {
+    flag[blockIdx.x] = any ? 1 : 0;
+    if (count && any) atomicAdd(count, 1);
+  }
 }
 
 // ---------------------------------------------------------------- batch: per-scene CG
@@ -2009,7 +2097,8 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
     uint8_t* flag = l == 0 ? m.tflag : m.tflag_l[l];
     if (flag)
       k_mg_tile_flags<<<mg_up_blocks(m.nx[l], m.ny[l]), kT, 0, s>>>(
-          m.nx[l], m.ny[l], m.inf[l], m.nx[l + 1], m.ny[l + 1], m.inf[l + 1], flag);
+          m.nx[l], m.ny[l], m.inf[l], m.nx[l + 1], m.ny[l + 1], m.inf[l + 1], flag,
+          l == 0 ? &sc->live_tiles : nullptr);
   }
 }
 
@@ -2054,7 +2143,7 @@ void launch_mg_down(hipStream_t s, MgHost& m, int l, int it, int max_iter, doubl
   if (l == 0 && m.fused) {
     // iteration it - 1's update: the buffers k_mg_xr would use at that iteration
     const MgXr xr{m.x[(it + 1) & 1], m.x[it & 1], m.p[it & 1], m.b, m.part_pq,
-                  vec_blocks(m.nx[0] * m.ny[0]), m.part_rr};
+                  mg_dir_blocks(m), m.part_rr};
     k_mg_down<true, kT, true><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1),
                                                     m.coarse.omega, it, chk, 0, m.tflag, xr,
                                                     span, MgCoarseArgs{}, nullptr);
@@ -2099,9 +2188,19 @@ void launch_mg_up(hipStream_t s, const MgHost& m, int l, int it, CgScalars* sc) 
                                                          m.tflag_l[l]);
 }
 
+int mg_dir_blocks(const MgHost& m) {
+  return m.tflag ? mg_up_blocks(m.nx[0], m.ny[0]) : vec_blocks(m.nx[0] * m.ny[0]);
+}
+
 void launch_mg_dir(hipStream_t s, const MgHost& m, int it, CgScalars* sc) {
-  const int nb = vec_blocks(m.nx[0] * m.ny[0]);
   const int n_rz = m.lc == 0 ? 1 : mg_up_blocks(m.nx[0], m.ny[0]);
+  if (m.tflag) {
+    k_mg_dir_tiles<<<mg_dir_blocks(m), kT, 0, s>>>(m.nx[0], m.ny[0], m.inf[0], m.xo[0],
+                                                   m.p[it & 1], m.p[(it + 1) & 1], it, m.part_rz,
+                                                   n_rz, m.part_pq, sc, m.tflag);
+    return;
+  }
+  const int nb = vec_blocks(m.nx[0] * m.ny[0]);
   k_mg_dir<<<nb, kT, 0, s>>>(m.nx[0], m.ny[0], m.inf[0], m.xo[0], m.p[it & 1], m.p[(it + 1) & 1],
                              it, m.part_rz, n_rz, m.part_pq, sc);
 }
@@ -2109,7 +2208,8 @@ void launch_mg_dir(hipStream_t s, const MgHost& m, int it, CgScalars* sc) {
 void launch_mg_xr(hipStream_t s, const MgHost& m, int it, CgScalars* sc) {
   const int nb = vec_blocks(m.nx[0] * m.ny[0]);
   k_mg_xr<<<nb, kT, 0, s>>>(m.nx[0], m.ny[0], m.inf[0], m.p[(it + 1) & 1], m.x[it & 1],
-                            m.x[(it + 1) & 1], m.b, m.rl[0], it, m.part_pq, nb, m.part_rr, sc);
+                            m.x[(it + 1) & 1], m.b, m.rl[0], it, m.part_pq, mg_dir_blocks(m),
+                            m.part_rr, sc);
 }
 
 void launch_mg_remove_mean(hipStream_t s, const MgHost& m) {

@@ -7,9 +7,12 @@ the GPU.  There is no CPU fallback: if libfsg.so is missing or cannot be loaded,
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import sys
 import threading
+import weakref
 
 import numpy as np
 
@@ -17,6 +20,18 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfsg.so")
 _LIB = None
 _LOCK = threading.Lock()
+# Handles still open.  They are destroyed by an atexit hook (registered by load()),
+# which runs while the HIP runtime is intact; a handle's __del__ that runs later,
+# during interpreter finalisation, makes no device call.
+_LIVE: "weakref.WeakSet" = weakref.WeakSet()
+
+
+def _close_live():
+    for d in list(_LIVE):
+        try:
+            d.close()
+        except Exception:
+            pass
 
 OK = 0
 LIQUID, AIR, SOLID = 0, 1, 2
@@ -69,7 +84,7 @@ class Params(C.Structure):
 class Stats(C.Structure):
     _fields_ = [
         ("n_fluid", C.c_int32), ("cg_iterations", C.c_int32), ("cg_error", C.c_double),
-        ("refine_sweeps", C.c_int32), ("reserved0", C.c_int32),
+        ("refine_sweeps", C.c_int32), ("mg_live_tiles", C.c_int32),
         ("total_unknown_iterations", C.c_int64), ("total_solves", C.c_int64),
         ("reserved", C.c_int64 * 4),
     ]
@@ -159,6 +174,7 @@ def load(path: str | None = None):
             f.argtypes = args
             f.restype = res
         _LIB = L
+        atexit.register(_close_live)
         return L
 
 
@@ -186,13 +202,20 @@ class Domain:
         if rc != OK:
             raise FsgError(f"fsg_create failed ({rc})")
         self.nx, self.ny = nx, ny
+        _LIVE.add(self)
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             self._L.fsg_destroy(self._h)
             self._h = C.c_void_p()
+        _LIVE.discard(self)
 
     def __del__(self):
+        # at interpreter exit the atexit hook has already closed every handle; one
+        # collected later (finalisation) must not call into HIP, whose runtime may be
+        # torn down by then
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -497,6 +520,7 @@ class Batch(Domain):
                                       ny.ctypes.data, delta, density, pic_ratio)
         if rc != OK:
             raise FsgError(f"fsg_batch_create failed ({rc})")
+        _LIVE.add(self)
         self.sizes = [(int(a), int(b)) for a, b in zip(nx, ny)]
         self.n_scenes = len(self.sizes)
         geo = self.geometry()

@@ -131,7 +131,8 @@ typedef struct fsg_stats {
   int32_t cg_iterations;  /* iterations of the last solve (Eigen's count) */
   double cg_error;        /* |r|/|b| reported by the last solve */
   int32_t refine_sweeps;  /* FP64 sweeps of the last refined solve */
-  int32_t reserved0;
+  int32_t mg_live_tiles;  /* multigrid form: level-0 32 x 32 tiles holding an unknown in
+                             the last solve (the tiles its level-0 kernels process) */
   int64_t total_unknown_iterations; /* sum over solves of n_fluid * SpMV count */
   int64_t total_solves;
   int64_t reserved[4];

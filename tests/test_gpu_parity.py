@@ -276,6 +276,41 @@ def test_full_step(state, scheme):
     assert max(errs) <= 1e-4
 
 
+def _fp64_pair(state):
+    """The state in the FP64 oracle (pressure to |r| / |b| <= 1e-13) and on the GPU
+    with the multigrid-preconditioned solver to 1e-10."""
+    sc, sim = state
+    from oracle import OracleSim
+    o = OracleSim(sim.nx, sim.ny, sc.length_x, sc.length_y, sc.density, sc.pic_ratio, "f64")
+    copy_state(sim, o)
+    o.set_params(cg_tol=1e-13, cg_max_iter=200000)
+    g = gpu_from_oracle(sim, sc, cg_tol=1e-10, cg_max_iter=500, solver=fsg.SOLVER_MG)
+    return sc, o, g
+
+
+def _step_errs(g, o):
+    u, v = g.velocity(fsg.VEL_FRONT)
+    uo, vo = o.vel("front")
+    ub, vb = g.velocity(fsg.VEL_BACK)
+    ubo, vbo = o.vel("back")
+    return [rel_err(g.particles(), o.particles), rel_err(u, uo), rel_err(v, vo),
+            rel_err(ub, ubo), rel_err(vb, vbo)]
+
+
+@pytest.mark.parametrize("scheme", [fsg.PIC, fsg.FLIP, fsg.PICFLIP, fsg.SEMI_LAGRANGIAN])
+def test_full_step_mg_against_fp64(state, scheme):
+    """One whole step with the multigrid-preconditioned solver against the FP64
+    oracle: labels bit-exact; particles, front and back u / v within 1e-5."""
+    sc, o, g = _fp64_pair(state)
+    o.step(scheme, sc.dt)
+    g.step(scheme, sc.dt)
+    assert np.array_equal(g.labels(), o.label)
+    errs = _step_errs(g, o)
+    print(f"scheme {scheme} (MG vs FP64): particles {errs[0]:.2e} front u/v {errs[1]:.2e}/"
+          f"{errs[2]:.2e} back u/v {errs[3]:.2e}/{errs[4]:.2e}")
+    assert max(errs) <= 1e-5
+
+
 def test_full_step_shuffled_particles_ragged_grid():
     """The step's splat reads the particles grouped by 16 x 16-cell bucket. Here their
     storage order is shuffled (every block's particles spread over the whole grid) on a
@@ -295,6 +330,23 @@ def test_full_step_shuffled_particles_ragged_grid():
     errs = [rel_err(g.particles(), o.particles), rel_err(u, uo), rel_err(v, vo)]
     print(f"shuffled 100^2: particles {errs[0]:.2e} front u/v {errs[1]:.2e}/{errs[2]:.2e}")
     assert max(errs) <= 1e-4
+
+
+def test_full_step_shuffled_particles_mg_against_fp64():
+    """The shuffled-storage-order step of the test above with the multigrid solver,
+    against the FP64 oracle: labels bit-exact, the rest within 1e-5, storage order
+    kept."""
+    sc, sim = oracle_after(100, steps=2)
+    perm = np.random.default_rng(7).permutation(len(sim.particles))
+    sim.set_particles(np.ascontiguousarray(sim.particles[perm]))
+    sc, o, g = _fp64_pair((sc, sim))
+    o.step(PICFLIP, sc.dt)
+    g.step(fsg.PICFLIP, sc.dt)
+    assert np.array_equal(g.labels(), o.label)
+    errs = _step_errs(g, o)
+    print(f"shuffled 100^2 (MG vs FP64): particles {errs[0]:.2e} front u/v {errs[1]:.2e}/"
+          f"{errs[2]:.2e} back u/v {errs[3]:.2e}/{errs[4]:.2e}")
+    assert max(errs) <= 1e-5
 
 
 def test_no_fluid_keeps_buffers_unswapped():
