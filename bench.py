@@ -156,10 +156,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# multigrid iterations replayed from captured HIP graphs (--graph)
+GRAPH = 1
+
+
 def make_domain(fsg, scene, solver, tol, max_iter, warm=0):
     return fsg.Domain.from_scene(scene, cg_tol=tol, cg_max_iter=max_iter, solver=solver,
                                  refine_inner_tol=1e-3, refine_max_outer=20, check_every=64,
-                                 warm_start=warm)
+                                 warm_start=warm, graph=GRAPH)
 
 
 def run_steps(fsg, d, n, dt, scheme=None):
@@ -339,7 +343,7 @@ def batch_leg(fsg, scenes, args):
     scs = scenes.dam_break_batch(args.batch_scenes, args.batch_lo, args.batch_hi)
     cells = sum(sc.nx * sc.ny for sc in scs)
     parts = sum(len(sc.particles) for sc in scs)
-    b = fsg.Batch.from_scenes(scs, cg_tol=args.tol, cg_max_iter=200)
+    b = fsg.Batch.from_scenes(scs, cg_tol=args.tol, cg_max_iter=200, graph=GRAPH)
     log(f"batch: {len(scs)} scenes, {cells} cells, {parts} particles, packed {b.nx}x{b.ny}")
     run_steps(fsg, b, args.batch_warmup, scs[0].dt)
     prof = kernel_profile(fsg, b, 2, scs[0].dt)
@@ -425,7 +429,7 @@ def grid_leg(fsg, scenes, args):
     reference's one-sided wall rule makes this closed box inject fluid through its
     walls and overflow within a few steps)."""
     sc = scenes.grid_only(args.grid_n)
-    d = fsg.Domain.from_scene(sc, cg_tol=args.tol, cg_max_iter=200, solver=fsg.SOLVER_MG,
+    d = fsg.Domain.from_scene(sc, cg_tol=args.tol, cg_max_iter=200, solver=fsg.SOLVER_MG, graph=GRAPH,
                               integrator=2, project_mean=1, closed_walls=1)
     run_steps(fsg, d, 2, sc.dt, fsg.GRID_ADVECT)
     prof = kernel_profile(fsg, d, 2, sc.dt, fsg.GRID_ADVECT)
@@ -580,6 +584,9 @@ def main():
     ap.add_argument("--roofline-kernel", default=None,
                     help="timer name of the roofline kernel (default: the one with the most "
                          "time per step in the profile pass)")
+    ap.add_argument("--graph", type=int, default=1,
+                    help="multigrid iterations replayed from captured HIP graphs (1) or "
+                         "launched kernel by kernel (0)")
     ap.add_argument("--warm-start", type=int, default=0,
                     help="chosen form: 1 starts each solve from the previous pressure")
     ap.add_argument("--batch-scenes", type=int, default=256)
@@ -604,6 +611,8 @@ def main():
                     help="only copy a MIB-sized device buffer 5 times (calibrates the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    global GRAPH
+    GRAPH = args.graph
     if args.copy_probe:
         import torch
         n = args.copy_probe * (1 << 20) // 4
@@ -791,6 +800,7 @@ def main():
             "workload": f"dam_break_{args.n}: stepPICFLIP on a {args.n}x{args.n} MAC grid, "
                         f"{P} particles, dt 0.01, density 0.01, 0.95 FLIP",
             "solver": args.solver,
+            "graph": args.graph,
             "tol": args.tol,
             "timed_steps": [t_start, t_start + args.steps - 1],
             "untimed_steps_before": t_start,

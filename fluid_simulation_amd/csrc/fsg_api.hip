@@ -139,6 +139,22 @@ struct AdjState {
   int last_iters = 0;
 };
 
+// The multigrid solve's iterations captured as HIP graphs, kIters per graph:
+// graph c runs iterations [c kIters, (c + 1) kIters) (the last one stops at the
+// iteration cap).  Their launch arguments depend only on the iteration index and on
+// the hierarchy's buffers, fixed once the hierarchy is built, and on the tolerance
+// and the cap (fsg_set_params drops the graphs when those change).
+struct MgGraphs {
+  static constexpr int kIters = 2;
+  std::vector<hipGraphExec_t> exec;
+  void clear() {
+    for (hipGraphExec_t e : exec)
+      if (e) (void)hipGraphExecDestroy(e);
+    exec.clear();
+  }
+  ~MgGraphs() { clear(); }
+};
+
 // A checkpoint of the carried state: particles, front and back velocities.
 struct Checkpoint {
   float4* pt = nullptr;
@@ -207,6 +223,7 @@ struct fsg_domain {
   size_t emit_cap = 0;
   AdjState adj;
   std::vector<Checkpoint> ckpt;
+  MgGraphs mg_graphs;  // captured multigrid iterations (fsg_params.graph)
   int kclock = 0;  // fsg_set_kernel_clock: 1 k_mg_coarse, 2 level 0's down kernel
   bool mg_ready = false;
   bool mg_solved = false;  // the last solve was the multigrid form (its FP64 x is valid)
@@ -536,31 +553,67 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
                                             "cg_mg_down6", "cg_mg_down7+"};
   static const char* const kUpNames[8] = {"cg_mg_up0", "cg_mg_up1", "cg_mg_up2", "cg_mg_up3",
                                           "cg_mg_up4", "cg_mg_up5", "cg_mg_up6", "cg_mg_up7+"};
-  for (;;) {
-    for (int q = 0; q < batch && it <= max_iter; ++q, ++it) {
-      if (h->batch)
-        timed(h, "cg_mg_check", [&] {
-          launch_mgb_check(h->stream, h->batch->cg_args(h->g.nx, h->g.ny), it, max_iter, tol, h->sc);
-        });
-      for (int l = 0; l < m.lc; ++l)
-        timed(h, kDownNames[std::min(l, 7)],
-              [&] { launch_mg_down(h->stream, m, l, it, max_iter, tol, h->sc); });
-      if (h->batch)
-        timed(h, "cg_mg_coarse", [&] { launch_mg_scene_coarse(h->stream, scene_coarse_args(h), h->sc); });
-      else if (!mg_tail_coarse(m))
-        timed(h, "cg_mg_coarse", [&] { launch_mg_coarse(h->stream, m, it, max_iter, tol, h->sc); });
-      for (int l = m.lc - 1; l >= 0; --l)
-        timed(h, kUpNames[std::min(l, 7)], [&] { launch_mg_up(h->stream, m, l, it, h->sc); });
-      if (h->batch) {
-        const fsg::MgbArgs ba = h->batch->cg_args(h->g.nx, h->g.ny);
-        timed(h, "cg_mg_rz", [&] { launch_mgb_vectors(h->stream, ba, m, it, h->sc, 0); });
-        timed(h, "cg_mg_dir", [&] { launch_mgb_vectors(h->stream, ba, m, it, h->sc, 1); });
-        timed(h, "cg_mg_xr", [&] { launch_mgb_vectors(h->stream, ba, m, it, h->sc, 2); });
-      } else {
-        timed(h, "cg_mg_dir", [&] { launch_mg_dir(h->stream, m, it, h->sc); });
-        // fused form: the update runs in the next iteration's level-0 down kernel
-        if (!m.fused) timed(h, "cg_mg_xr", [&] { launch_mg_xr(h->stream, m, it, h->sc); });
+  // one CG iteration: the V-cycle's kernels, the direction (and the x / residual
+  // update where it is not fused into the next iteration's level-0 kernel)
+  auto enqueue_iteration = [&](int it) {
+    if (h->batch)
+      timed(h, "cg_mg_check", [&] {
+        launch_mgb_check(h->stream, h->batch->cg_args(h->g.nx, h->g.ny), it, max_iter, tol, h->sc);
+      });
+    for (int l = 0; l < m.lc; ++l)
+      timed(h, kDownNames[std::min(l, 7)],
+            [&] { launch_mg_down(h->stream, m, l, it, max_iter, tol, h->sc); });
+    if (h->batch)
+      timed(h, "cg_mg_coarse", [&] { launch_mg_scene_coarse(h->stream, scene_coarse_args(h), h->sc); });
+    else if (!mg_tail_coarse(m))
+      timed(h, "cg_mg_coarse", [&] { launch_mg_coarse(h->stream, m, it, max_iter, tol, h->sc); });
+    for (int l = m.lc - 1; l >= 0; --l)
+      timed(h, kUpNames[std::min(l, 7)], [&] { launch_mg_up(h->stream, m, l, it, h->sc); });
+    if (h->batch) {
+      const fsg::MgbArgs ba = h->batch->cg_args(h->g.nx, h->g.ny);
+      timed(h, "cg_mg_rz", [&] { launch_mgb_vectors(h->stream, ba, m, it, h->sc, 0); });
+      timed(h, "cg_mg_dir", [&] { launch_mgb_vectors(h->stream, ba, m, it, h->sc, 1); });
+      timed(h, "cg_mg_xr", [&] { launch_mgb_vectors(h->stream, ba, m, it, h->sc, 2); });
+    } else {
+      timed(h, "cg_mg_dir", [&] { launch_mg_dir(h->stream, m, it, h->sc); });
+      // fused form: the update runs in the next iteration's level-0 down kernel
+      if (!m.fused) timed(h, "cg_mg_xr", [&] { launch_mg_xr(h->stream, m, it, h->sc); });
+    }
+  };
+  // Graph replay: graph c holds iterations [c K, (c + 1) K) (up to the cap), captured
+  // on its first use.  Per-kernel timing and the kernel clocks need every launch.
+  const bool use_graph = h->prm.graph && !h->timer.on && !m.clk_on && !m.clk_span_on;
+  constexpr int K = MgGraphs::kIters;
+  auto graph_of = [&](int c, hipGraphExec_t* out) -> int {
+    std::vector<hipGraphExec_t>& ex = h->mg_graphs.exec;
+    if ((int)ex.size() <= c) ex.resize(c + 1, nullptr);
+    if (!ex[c]) {
+      hipGraph_t g = nullptr;
+      HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      for (int q = c * K; q < std::min((c + 1) * K, max_iter + 1); ++q) enqueue_iteration(q);
+      HIPCHK(h, hipStreamEndCapture(h->stream, &g));
+      const hipError_t e = hipGraphInstantiate(&ex[c], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        ex[c] = nullptr;
+        HIPCHK(h, e);
       }
+    }
+    *out = ex[c];
+    return FSG_OK;
+  };
+  for (;;) {
+    const int target = std::min(it + batch, max_iter + 1);
+    if (use_graph) {
+      while (it < target) {
+        const int c = it / K;
+        hipGraphExec_t e = nullptr;
+        if (int rc = graph_of(c, &e)) return rc;
+        HIPCHK(h, hipGraphLaunch(e, h->stream));
+        it = std::min((c + 1) * K, max_iter + 1);
+      }
+    } else {
+      for (; it < target; ++it) enqueue_iteration(it);
     }
     HIPCHK(h, hipMemcpyAsync(&hs, h->sc, sizeof(hs), hipMemcpyDeviceToHost, h->stream));
     if (F < 0) HIPCHK(h, hipMemcpyAsync(&F, h->n_fluid, sizeof(int), hipMemcpyDeviceToHost, h->stream));
@@ -826,6 +879,7 @@ void fsg_params_default(fsg_params* p) {
   p->refine_inner_tol = 1e-3;
   p->refine_max_outer = 20;
   p->record_ext_info = 0;
+  p->graph = 1;
 }
 
 int fsg_create(fsg_domain** out, int32_t nx, int32_t ny, float length_x, float length_y,
@@ -901,6 +955,7 @@ int fsg_create(fsg_domain** out, int32_t nx, int32_t ny, float length_x, float l
 int fsg_destroy(fsg_domain* h) {
   if (!h) return FSG_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  h->mg_graphs.clear();
   delete h->batch;  // its device arrays are in h->allocations
   h->timer.collect();
   for (void* p : h->allocations) (void)hipFree(p);
@@ -924,11 +979,14 @@ int fsg_set_params(fsg_domain* h, const fsg_params* p) {
   if (!h || !p) return FSG_E_INVALID;
   if (p->cg_max_iter < 0 || p->ext_layers < 0 || p->ext_layers > 14 || p->integrator < 0 ||
       p->integrator > 3 || p->solver < 0 || p->solver > 3 || !(p->cg_tol >= 0.0) ||
-      !std::isfinite(p->particle_pressure_k) || p->closed_walls < 0 || p->closed_walls > 1)
+      !std::isfinite(p->particle_pressure_k) || p->closed_walls < 0 || p->closed_walls > 1 ||
+      p->graph < 0 || p->graph > 1)
     return fail(h, FSG_E_INVALID, "fsg_set_params: value out of range");
   if ((p->solver == FSG_SOLVER_PERSISTENT || p->solver == FSG_SOLVER_REFINED) &&
       h->max_coop_blocks <= 0)
     return fail(h, FSG_E_INVALID, "persistent solver unavailable on this device");
+  // the captured iterations hold the tolerance and the iteration cap
+  if (p->cg_tol != h->prm.cg_tol || p->cg_max_iter != h->prm.cg_max_iter) h->mg_graphs.clear();
   h->prm = *p;
   return FSG_OK;
 }

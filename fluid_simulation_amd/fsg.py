@@ -77,7 +77,7 @@ class Params(C.Structure):
         ("refine_inner_tol", C.c_double), ("refine_max_outer", C.c_int32),
         ("record_ext_info", C.c_int32), ("pcg_blocks", C.c_int32), ("pcg_kpt", C.c_int32),
         ("warm_start", C.c_int32), ("particle_pressure_k", C.c_float),
-        ("closed_walls", C.c_int32),
+        ("closed_walls", C.c_int32), ("graph", C.c_int32),
     ]
 
 

@@ -123,6 +123,10 @@ typedef struct fsg_params {
                              one-sided rule lets the update, which takes SOLID
                              neighbours as p = 0, inject fluid through the walls, and
                              the flow grows without bound; 1 keeps such runs bounded. */
+  int32_t graph;          /* multigrid solver: 1 (default) replays its iterations from
+                             HIP graphs captured once per handle (two iterations per
+                             graph); 0 launches every kernel.  Per-kernel timing and
+                             the kernel clocks launch every kernel. */
 } fsg_params;
 
 /* Counters of the last pressure solve and running totals. */
