@@ -70,6 +70,19 @@ def rollout_adjoint(dom: "fsg.Domain", scheme: int, dt: float, steps: int, every
     return reverse_pass(dom, scheme, dt, steps, every, seed_fn)
 
 
+def rollout_decisions(dom: "fsg.Domain", scheme: int, dt: float, steps: int):
+    """Step the domain ``steps`` times from its state and return each step's discrete
+    decisions: (labels, face decisions, reject flags) per step.  Needs
+    ``record_decisions = 1``; with ``deterministic = 1`` they are the decisions every
+    rerun from the same state (a checkpointed recompute) takes, i.e. the branch of
+    the piecewise-smooth rollout the adjoint differentiates."""
+    recs = []
+    for _ in range(steps):
+        dom.step(scheme, dt)
+        recs.append((dom.labels(), dom.step_decisions(), dom.reject_flags()))
+    return recs
+
+
 def density_loss_seed(target: np.ndarray):
     """seed_fn for the loss sum((rho - target)^2) (FP64 sum) of the particle density."""
     target = np.asarray(target, np.float64)

@@ -179,6 +179,8 @@ struct fsg_domain {
   uint8_t* solid_buf = nullptr;
   float* phi = nullptr;      // optional
   float4* acc = nullptr;
+  long long* acc_fx = nullptr;  // fixed-point P2G / density sums (fsg_params.deterministic)
+  uint8_t* dec = nullptr;       // the last step's decisions (fsg_params.record_decisions)
   uint8_t* mask[4] = {nullptr, nullptr, nullptr, nullptr};  // u0 u1 v0 v1
   int mask_front = 0;
   uint8_t *info_u = nullptr, *info_v = nullptr;
@@ -337,7 +339,12 @@ int ph_classify(fsg_domain* h) {
 // Fused classify + P2G splat (one read of the particles), then the per-face
 // normalise [+ prev + gravity + Dirichlet + extension init] on the new front.
 void classify_and_splat(fsg_domain* h) {
-  timed(h, "cell_init", [&] { launch_cell_init(h->stream, h->g, h->label, h->solid, h->acc, 1, 1); });
+  // deterministic form: the splat sums in fixed point (zero between steps) and the
+  // conversion writes every accumulator, so they are not cleared here
+  long long* fx = h->prm.deterministic ? h->acc_fx : nullptr;
+  timed(h, "cell_init", [&] {
+    launch_cell_init(h->stream, h->g, h->label, h->solid, h->acc, 1, fx ? 0 : 1);
+  });
   // one domain: the splat reads the particles grouped by bucket (their storage order
   // is kept); a batch keeps its per-scene particle tiles
   const float4* src = h->pt;
@@ -360,8 +367,12 @@ void classify_and_splat(fsg_domain* h) {
     fsg::BatchTables bt;
     if (h->batch) bt = h->batch->tables();
     launch_particle_scatter(h->stream, h->g, src, h->np, h->label, h->acc, 1, 1,
-                            h->batch ? &bt : nullptr);
+                            h->batch ? &bt : nullptr, fx);
   });
+  if (fx)
+    timed(h, "fx_to_float", [&] {
+      launch_fx_to_float(h->stream, 4 * (int64_t)ncell(h), fx, reinterpret_cast<float*>(h->acc));
+    });
 }
 
 void face_update(fsg_domain* h, int flags, float dt, bool record_info) {
@@ -371,7 +382,8 @@ void face_update(fsg_domain* h, int flags, float dt, bool record_info) {
   timed(h, "face_update", [&] {
     launch_face_update(h->stream, h->g, flags, UF(h), VF(h), h->acc, h->u_prev, h->v_prev, h->label,
                        gx_dt, gy_dt, UB(h), VB(h), MXF(h), MXB(h), MYF(h), MYB(h),
-                       record_info ? h->info_u : nullptr, record_info ? h->info_v : nullptr);
+                       record_info ? h->info_u : nullptr, record_info ? h->info_v : nullptr,
+                       (flags & FU_DIRICHLET) && h->prm.record_decisions ? h->dec : nullptr);
   });
 }
 
@@ -779,7 +791,8 @@ int ph_pressure(fsg_domain* h, float dt, bool dirichlet, bool diff) {
                            VB(h),
                            dt_over_density, dirichlet ? (h->prm.closed_walls ? 2 : 1) : 0,
                            diff ? 1 : 0, h->u_prev, h->v_prev,
-                           h->u_diff, h->v_diff, h->adj.taping ? h->adj.zmask : nullptr);
+                           h->u_diff, h->v_diff, h->adj.taping ? h->adj.zmask : nullptr,
+                           dirichlet && h->prm.record_decisions ? h->dec : nullptr);
   });
   swap_vel(h);
   return FSG_OK;
@@ -980,8 +993,26 @@ int fsg_set_params(fsg_domain* h, const fsg_params* p) {
   if (p->cg_max_iter < 0 || p->ext_layers < 0 || p->ext_layers > 14 || p->integrator < 0 ||
       p->integrator > 3 || p->solver < 0 || p->solver > 3 || !(p->cg_tol >= 0.0) ||
       !std::isfinite(p->particle_pressure_k) || p->closed_walls < 0 || p->closed_walls > 1 ||
-      p->graph < 0 || p->graph > 1)
+      p->graph < 0 || p->graph > 1 || p->deterministic < 0 || p->deterministic > 1 ||
+      p->record_decisions < 0 || p->record_decisions > 1)
     return fail(h, FSG_E_INVALID, "fsg_set_params: value out of range");
+  const size_t C = ncell(h);
+  // fixed-point sums: 4 per cell (the P2G accumulators; the density uses the first C),
+  // zero between uses
+  if (p->deterministic && !h->acc_fx) {
+    long long* fx = nullptr;
+    if (dalloc(h, &fx, 4 * C) != hipSuccess)
+      return fail(h, FSG_E_NOMEM, "fsg_set_params: deterministic sums: allocation failed");
+    HIPCHK(h, hipMemsetAsync(fx, 0, 4 * C * sizeof(long long), h->stream));
+    h->acc_fx = fx;
+  }
+  if (p->record_decisions && !h->dec) {
+    uint8_t* d = nullptr;
+    if (dalloc(h, &d, C) != hipSuccess)
+      return fail(h, FSG_E_NOMEM, "fsg_set_params: decision record: allocation failed");
+    HIPCHK(h, hipMemsetAsync(d, 0, C, h->stream));
+    h->dec = d;
+  }
   if ((p->solver == FSG_SOLVER_PERSISTENT || p->solver == FSG_SOLVER_REFINED) &&
       h->max_coop_blocks <= 0)
     return fail(h, FSG_E_INVALID, "persistent solver unavailable on this device");
@@ -1276,14 +1307,21 @@ int fsg_run_phase(fsg_domain* h, int32_t phase, float dt) {
   const bool rec = true;
   switch (phase) {
     case FSG_PH_CLASSIFY: ph_classify(h); break;
-    case FSG_PH_P2G:
-      timed(h, "cell_init", [&] { launch_cell_init(h->stream, h->g, h->label, h->solid, h->acc, 0, 1); });
-      timed(h, "particle_scatter", [&] {
-        launch_particle_scatter(h->stream, h->g, h->pt, h->np, h->label, h->acc, 0, 1);
+    case FSG_PH_P2G: {
+      long long* fx = h->prm.deterministic ? h->acc_fx : nullptr;
+      timed(h, "cell_init", [&] {
+        launch_cell_init(h->stream, h->g, h->label, h->solid, h->acc, 0, fx ? 0 : 1);
       });
+      timed(h, "particle_scatter", [&] {
+        launch_particle_scatter(h->stream, h->g, h->pt, h->np, h->label, h->acc, 0, 1, nullptr,
+                                fx);
+      });
+      if (fx)
+        launch_fx_to_float(h->stream, 4 * (int64_t)ncell(h), fx, reinterpret_cast<float*>(h->acc));
       swap_vel(h);  // the normalise writes the (old) back, which is now the front
       face_update(h, FU_P2G, dt, false);
       break;
+    }
     case FSG_PH_SAVE_PREV: face_update(h, FU_PREV, dt, false); break;
     case FSG_PH_GRAVITY: face_update(h, FU_GRAVITY, dt, false); break;
     case FSG_PH_DIRICHLET: face_update(h, FU_DIRICHLET, dt, false); break;
@@ -1412,6 +1450,15 @@ int fsg_get_ext_info(fsg_domain* h, uint8_t* iu, uint8_t* iv) {
   if (!h) return FSG_E_INVALID;
   if (iu) HIPCHK(h, hipMemcpyAsync(iu, h->info_u, ncell(h), hipMemcpyDeviceToHost, h->stream));
   if (iv) HIPCHK(h, hipMemcpyAsync(iv, h->info_v, ncell(h), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+int fsg_get_step_decisions(fsg_domain* h, uint8_t* dec) {
+  if (!h || !dec) return FSG_E_INVALID;
+  if (!h->prm.record_decisions || !h->dec)
+    return fail(h, FSG_E_INVALID, "fsg_get_step_decisions: fsg_params.record_decisions is off");
+  HIPCHK(h, hipMemcpyAsync(dec, h->dec, ncell(h), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return FSG_OK;
 }
@@ -1905,8 +1952,9 @@ int fsg_particle_density(fsg_domain* h, float* rho) {
   if (!h || h->batch || !rho) return FSG_E_INVALID;
   if (int rc = adj_prepare(h)) return rc;
   const size_t C = ncell(h);
-  launch_fill(h->stream, h->adj.su, (int64_t)C, 0.f);
-  launch_density(h->stream, h->g, h->pt, h->np, h->adj.su);
+  long long* fx = h->prm.deterministic ? h->acc_fx : nullptr;
+  if (!fx) launch_fill(h->stream, h->adj.su, (int64_t)C, 0.f);
+  launch_density(h->stream, h->g, h->pt, h->np, h->adj.su, fx);
   HIPCHK(h, hipMemcpyAsync(rho, h->adj.su, C * 4, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return FSG_OK;

@@ -79,8 +79,6 @@ __device__ __forceinline__ float vel_y_at(const float* __restrict__ v, const Geo
   return interp_scaled(v, g.nx, g.ny, div_x(xx, g), div_y(y, g));
 }
 
-// Corners and fractions of Grid<T>::addToValueInterpolated (include/Grid.h:
-// 152-184) at scaled coordinates: i+1 taken before the clamp.
 // LDS float add of (addr, v) per lane, summed first over the lanes of each quad that
 // target the same address (neighbouring particles of one cell share their corners):
 // lanes L and L^1, then L and L^2, compare addresses through DPP quad permutations and
@@ -104,11 +102,46 @@ __device__ __forceinline__ void lds_add_quad(float* tile, int addr, float v) {
   if (addr >= 0 && v != 0.f) atomicAdd(tile + addr, v);
 }
 
+// Fixed-point splat terms (deterministic accumulation): t -> rint(t * 2^40) as a
+// 64-bit integer, and a sum back to float through one double.
+constexpr float kFxScale = 1099511627776.0f;  // 2^40
+__device__ __forceinline__ long long fx_term(float t) {
+  return (long long)__builtin_rintf(t * kFxScale);
+}
+__device__ __forceinline__ float fx_value(long long v) {
+  return (float)((double)v * (1.0 / 1099511627776.0));
+}
+
+// lds_add_quad for 64-bit integer terms: integer sums are exact, so the result does
+// not depend on which lane adds first.
+__device__ __forceinline__ long long quad_xor1_64(long long v) {
+  const unsigned lo = (unsigned)quad_xor1((int)(unsigned)(unsigned long long)v);
+  const unsigned hi = (unsigned)quad_xor1((int)(unsigned)((unsigned long long)v >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ long long quad_xor2_64(long long v) {
+  const unsigned lo = (unsigned)quad_xor2((int)(unsigned)(unsigned long long)v);
+  const unsigned hi = (unsigned)quad_xor2((int)(unsigned)((unsigned long long)v >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ void lds_add_quad_fx(unsigned long long* tile, int addr, long long v) {
+  const int lane = threadIdx.x & 63;
+  const int a1 = quad_xor1(addr);
+  const long long v1 = quad_xor1_64(v);
+  if (a1 == addr) v = (lane & 1) ? 0ll : v + v1;
+  const int a2 = quad_xor2(addr);
+  const long long v2 = quad_xor2_64(v);
+  if (a2 == addr) v = (lane & 2) ? 0ll : v + v2;
+  if (addr >= 0 && v != 0ll) atomicAdd(tile + addr, (unsigned long long)v);
+}
+
 struct SplatCorners {
   int i, j, i1, j1;
   float fi, fj;
 };
 
+// Corners and fractions of Grid<T>::addToValueInterpolated (include/Grid.h:
+// 152-184) at scaled coordinates: i+1 taken before the clamp.
 __device__ __forceinline__ SplatCorners splat_corners(float xs, float ys, int nx, int ny) {
   SplatCorners s;
   s.i = (int)xs;

@@ -31,9 +31,12 @@ enum { FU_P2G = 1, FU_PREV = 2, FU_GRAVITY = 4, FU_DIRICHLET = 8, FU_EXTINIT = 1
 
 void launch_cell_init(hipStream_t s, const Geom& g, uint8_t* label, const uint8_t* solid,
                       float4* acc, int do_labels, int do_acc);
+// acc_fx set: the splat accumulates in 64-bit fixed point there (4 per cell, zero on
+// entry; launch_fx_to_float turns the sums into acc and zeroes them again).
 void launch_particle_scatter(hipStream_t s, const Geom& g, const float4* pt, int64_t np,
                              uint8_t* label, float4* acc, int do_classify, int do_p2g,
-                             const BatchTables* batch = nullptr);
+                             const BatchTables* batch = nullptr, long long* acc_fx = nullptr);
+void launch_fx_to_float(hipStream_t s, int64_t n, long long* fx, float* out);
 // The splat's bucket order for one domain (np < 2^31, bucket_count(g) <= kMaxBuckets):
 // cnt (zero on entry, zero again on return) and off hold bucket_count(g) ints, rank np
 // ints; sorted receives the np particles grouped by bucket.
@@ -45,7 +48,7 @@ void launch_face_update(hipStream_t s, const Geom& g, int flags, float* u, float
                         const float4* acc, float* prev_u, float* prev_v, const uint8_t* label,
                         float gx_dt, float gy_dt, float* ext_u, float* ext_v, uint8_t* mxf,
                         uint8_t* mxb, uint8_t* myf, uint8_t* myb, uint8_t* info_u,
-                        uint8_t* info_v);
+                        uint8_t* info_v, uint8_t* dec = nullptr);
 void launch_extend_sweep(hipStream_t s, const Geom& g, const uint8_t* label, float* ub,
                          float* vb, const uint8_t* mxf, uint8_t* mxb, const uint8_t* myf,
                          uint8_t* myb, uint8_t* info_u, uint8_t* info_v, int layer);
@@ -56,7 +59,7 @@ void launch_pressure_update(hipStream_t s, const Geom& g, const int* n_fluid,
                             const int* counts, float ppk, float* au, float* av,
                             float* bu, float* bv, float dt_over_density, int do_dirichlet,
                             int do_diff, const float* prev_u, const float* prev_v, float* diff_u,
-                            float* diff_v, uint8_t* zmask = nullptr);
+                            float* diff_v, uint8_t* zmask = nullptr, uint8_t* dec = nullptr);
 void launch_diff(hipStream_t s, const Geom& g, const float* u, const float* v, const float* pu,
                  const float* pv, float* du, float* dv);
 void launch_particle_gather(hipStream_t s, const Geom& g, float4* pt, int64_t np, const float* u,
@@ -304,7 +307,10 @@ void launch_adj_pre(hipStream_t s, const Geom& g, const uint8_t* label, const ui
                     int closed, float* qu, float* qv, float* nu, float* nv);
 void launch_adj_p2g(hipStream_t s, const Geom& g, const float4* x0, int64_t np, const float* g1u,
                     const float* g1v, const float* qu, const float* qv, float4* pbar);
-void launch_density(hipStream_t s, const Geom& g, const float4* pt, int64_t np, float* rho);
+// rho_fx set (C int64, zero on entry and on return): fixed-point accumulation, and
+// rho is overwritten with the sums; otherwise the terms are added to rho.
+void launch_density(hipStream_t s, const Geom& g, const float4* pt, int64_t np, float* rho,
+                    long long* rho_fx = nullptr);
 void launch_density_adj(hipStream_t s, const Geom& g, const float4* pt, int64_t np,
                         const float* rb, float4* pbar);
 

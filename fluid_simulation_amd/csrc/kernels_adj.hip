@@ -429,17 +429,27 @@ __global__ void __launch_bounds__(kAT) k_adj_p2g(Geom g, const float4* __restric
 // ---------------------------------------------------------------- density loss
 // Particle density on the cell grid: every particle splats weight 1 at (x / dx,
 // y / dy) with Grid::addToValueInterpolated's weights.
+// FX: the terms are added as 64-bit fixed-point integers (fx_term) into rho_fx, so
+// the density does not depend on the order of the atomics (fsg_params.deterministic).
+template <bool FX>
 __global__ void __launch_bounds__(kAT) k_density(Geom g, const float4* __restrict__ pt,
-                                                 int64_t np, float* __restrict__ rho) {
+                                                 int64_t np, float* __restrict__ rho,
+                                                 unsigned long long* __restrict__ rho_fx) {
   const int64_t p = gtid();
   if (p >= np) return;
   const float4 q = pt[p];
   const SplatCorners s = splat_corners(div_x(q.x, g), div_y(q.y, g), g.nx, g.ny);
   const float a0 = (1.0f - s.fj), a1 = s.fj;
-  unsafeAtomicAdd(&rho[s.i + s.j * g.nx], (1.0f - s.fi) * a0);
-  unsafeAtomicAdd(&rho[s.i1 + s.j * g.nx], s.fi * a0);
-  unsafeAtomicAdd(&rho[s.i + s.j1 * g.nx], (1.0f - s.fi) * a1);
-  unsafeAtomicAdd(&rho[s.i1 + s.j1 * g.nx], s.fi * a1);
+  auto add = [&](int c, float t) {
+    if (FX)
+      atomicAdd(rho_fx + c, (unsigned long long)fx_term(t));
+    else
+      unsafeAtomicAdd(&rho[c], t);
+  };
+  add(s.i + s.j * g.nx, (1.0f - s.fi) * a0);
+  add(s.i1 + s.j * g.nx, s.fi * a0);
+  add(s.i + s.j1 * g.nx, (1.0f - s.fi) * a1);
+  add(s.i1 + s.j1 * g.nx, s.fi * a1);
 }
 
 // pbar = (d rho / d X)^T rhobar, zero velocity adjoints.
@@ -516,8 +526,16 @@ void launch_adj_p2g(hipStream_t s, const Geom& g, const float4* x0, int64_t np, 
   if (np > 0) k_adj_p2g<<<nblk(np), kAT, 0, s>>>(g, x0, np, g1u, g1v, qu, qv, pbar);
 }
 
-void launch_density(hipStream_t s, const Geom& g, const float4* pt, int64_t np, float* rho) {
-  if (np > 0) k_density<<<nblk(np), kAT, 0, s>>>(g, pt, np, rho);
+void launch_density(hipStream_t s, const Geom& g, const float4* pt, int64_t np, float* rho,
+                    long long* rho_fx) {
+  if (rho_fx) {
+    if (np > 0)
+      k_density<true><<<nblk(np), kAT, 0, s>>>(g, pt, np, rho,
+                                               reinterpret_cast<unsigned long long*>(rho_fx));
+    launch_fx_to_float(s, (int64_t)g.nx * g.ny, rho_fx, rho);
+    return;
+  }
+  if (np > 0) k_density<false><<<nblk(np), kAT, 0, s>>>(g, pt, np, rho, nullptr);
 }
 
 void launch_density_adj(hipStream_t s, const Geom& g, const float4* pt, int64_t np,

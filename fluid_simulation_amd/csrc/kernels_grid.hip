@@ -55,6 +55,7 @@ __global__ void __launch_bounds__(256) k_cell_init(Geom g, uint8_t* __restrict__
 // a block's particles then lie in one or two neighbouring buckets and its footprint
 // fits the tile however far the particles have mixed.
 constexpr int kSplatTile = 2048;  // cells (float4 each, 32 KiB of LDS)
+constexpr int kSplatTileFx = 1024;  // cells in fixed point (4 x int64 each, 32 KiB)
 constexpr int kBucketShift = 4;   // buckets of 16 x 16 cells
 
 // Adds the 8 terms of one splat (value and weight, Grid.h:152-184 order of
@@ -105,17 +106,26 @@ __device__ __forceinline__ PartTile part_tile(const Geom& g0, int64_t np, const 
   return pt;
 }
 
-template <bool BATCH>
+// FX (fsg_params.deterministic): every term t is accumulated as the integer
+// fx_term(t) = rint(t * 2^40) with 64-bit integer adds (LDS and global), which are
+// associative: the sums, and everything computed from them, do not depend on the
+// order in which the particles' terms arrive.  Terms of magnitude >= 2^-17 are
+// represented exactly (a float has 24 significant bits), so each sum is the exact
+// sum of the float terms, rounded once when it is converted back (k_fx_to_float);
+// smaller terms are rounded to 2^-40.  Range: |sum| < 2^23 per face component.
+template <bool BATCH, bool FX>
 __global__ void __launch_bounds__(256) k_particle_scatter(Geom g0, const float4* __restrict__ pt,
                                                           int64_t np, uint8_t* __restrict__ label,
-                                                          float* __restrict__ acc, int do_classify,
-                                                          int do_p2g, BatchTables tb) {
+                                                          void* __restrict__ acc_out,
+                                                          int do_classify, int do_p2g,
+                                                          BatchTables tb) {
   const PartTile T = part_tile<BATCH>(g0, np, tb);
   const Geom& g = T.g;
   const int64_t p = T.p;
   const bool live = p < T.np_end;
   label += T.off;
-  acc += 4 * (int64_t)T.off;
+  float* acc = reinterpret_cast<float*>(acc_out) + 4 * (int64_t)T.off;
+  unsigned long long* accx = reinterpret_cast<unsigned long long*>(acc_out) + 4 * (int64_t)T.off;
   const int pitch = T.pitch;
   const float4 q = live ? pt[p] : make_float4(0.f, 0.f, 0.f, 0.f);
   if (do_classify && live) {
@@ -144,6 +154,7 @@ __global__ void __launch_bounds__(256) k_particle_scatter(Geom g0, const float4*
     j1 = max(j1, __shfl_xor(j1, off, 64));
   }
   __shared__ int s_box[4][4];
+  // the block's LDS copy of its footprint: float4 per cell, or 4 x int64 (FX)
   __shared__ float4 s_tile[kSplatTile];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (lane == 0) {
@@ -158,17 +169,39 @@ __global__ void __launch_bounds__(256) k_particle_scatter(Geom g0, const float4*
   i1 = max(max(s_box[2][0], s_box[2][1]), max(s_box[2][2], s_box[2][3]));
   j1 = max(max(s_box[3][0], s_box[3][1]), max(s_box[3][2], s_box[3][3]));
   const int bw = i1 - i0 + 1, bh = j1 - j0 + 1;
-  const bool tiled = bw > 0 && bh > 0 && (int64_t)bw * bh <= kSplatTile;
+  const bool tiled = bw > 0 && bh > 0 && (int64_t)bw * bh <= (FX ? kSplatTileFx : kSplatTile);
   if (!tiled) {
     if (!live) return;
     auto gadd = [&](int i, int j, int comp, float v) {
-      unsafeAtomicAdd(acc + 4 * (i + j * pitch) + comp, v);
+      if (FX)
+        atomicAdd(accx + 4 * (i + j * pitch) + comp, (unsigned long long)fx_term(v));
+      else
+        unsafeAtomicAdd(acc + 4 * (i + j * pitch) + comp, v);
     };
     splat_terms(su, q.z, 0, gadd);
     splat_terms(sv, q.w, 2, gadd);
     return;
   }
   const int area = bw * bh;
+  if (FX) {
+    unsigned long long* tile = reinterpret_cast<unsigned long long*>(s_tile);
+    for (int c = threadIdx.x; c < 4 * area; c += 256) tile[c] = 0ull;
+    __syncthreads();
+    auto ladd = [&](int i, int j, int comp, float v) {
+      lds_add_quad_fx(tile, live ? 4 * ((i - i0) + (j - j0) * bw) + comp : -1,
+                      live ? fx_term(v) : 0ll);
+    };
+    splat_terms(su, q.z, 0, ladd);
+    splat_terms(sv, q.w, 2, ladd);
+    __syncthreads();
+    for (int c = threadIdx.x; c < 4 * area; c += 256) {
+      const unsigned long long v = tile[c];
+      const int cell = c >> 2;
+      if (v != 0ull)
+        atomicAdd(accx + 4 * ((i0 + cell % bw) + (j0 + cell / bw) * pitch) + (c & 3), v);
+    }
+    return;
+  }
   for (int c = threadIdx.x; c < area; c += 256) s_tile[c] = make_float4(0.f, 0.f, 0.f, 0.f);
   __syncthreads();
   {
@@ -205,22 +238,31 @@ __global__ void __launch_bounds__(256) k_particle_scatter(Geom g0, const float4*
 //               branch writes the x buffer, :527)
 // The weight test `w > 0.000001` compares the float weight with a double constant;
 // the float test w > 1e-6f selects the same floats (no float lies in between).
+// dec (optional, fsg_params.record_decisions): the step's discrete decisions per cell,
+// bit 0 / 1 u / v normalised by P2G, bit 2 / 3 u / v zeroed by this Dirichlet pass.
 __global__ void __launch_bounds__(256) k_face_update(
     Geom g, int flags, float* __restrict__ u, float* __restrict__ v,
     const float4* __restrict__ acc, float* __restrict__ prev_u, float* __restrict__ prev_v,
     const uint8_t* __restrict__ label, float gx_dt, float gy_dt, float* __restrict__ ext_u,
     float* __restrict__ ext_v, uint8_t* __restrict__ mxf, uint8_t* __restrict__ mxb,
     uint8_t* __restrict__ myf, uint8_t* __restrict__ myb, uint8_t* __restrict__ info_u,
-    uint8_t* __restrict__ info_v) {
+    uint8_t* __restrict__ info_v, uint8_t* __restrict__ dec) {
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int c = t * 256 + threadIdx.x;
   if (c >= g.nx * g.ny) return;
   const int i = c % g.nx, j = c / g.nx;
   float uu = u[c], vv = v[c];
+  unsigned dbits = 0;
   if (flags & FU_P2G) {
     const float4 a = acc[c];
-    if (a.y > 0.000001f) uu = a.x / a.y;
-    if (a.w > 0.000001f) vv = a.z / a.w;
+    if (a.y > 0.000001f) {
+      uu = a.x / a.y;
+      dbits |= 1;
+    }
+    if (a.w > 0.000001f) {
+      vv = a.z / a.w;
+      dbits |= 2;
+    }
   }
   if (flags & FU_PREV) {
     prev_u[c] = uu;
@@ -234,14 +276,19 @@ __global__ void __launch_bounds__(256) k_face_update(
     vv = vv + gy_dt;
   }
   if (flags & FU_DIRICHLET) {
+    bool zu, zv;
     if (flags & FU_CLOSED) {
-      if (Lim == kSolid || L == kSolid) uu = 0.f;
-      if (Ljm == kSolid || L == kSolid) vv = 0.f;
+      zu = Lim == kSolid || L == kSolid;
+      zv = Ljm == kSolid || L == kSolid;
     } else {
-      if ((Lim == kSolid && uu < 0.f) || (L == kSolid && uu > 0.f)) uu = 0.f;
-      if ((Ljm == kSolid && vv < 0.f) || (L == kSolid && vv > 0.f)) vv = 0.f;
+      zu = (Lim == kSolid && uu < 0.f) || (L == kSolid && uu > 0.f);
+      zv = (Ljm == kSolid && vv < 0.f) || (L == kSolid && vv > 0.f);
     }
+    if (zu) uu = 0.f;
+    if (zv) vv = 0.f;
+    dbits |= (zu ? 4u : 0u) | (zv ? 8u : 0u);
   }
+  if (dec) dec[c] = (uint8_t)dbits;
   if (flags & FU_EXTINIT) {
     const bool vu = (L == kLiquid || Lim == kLiquid);
     const bool vv_ok = (L == kLiquid || Ljm == kLiquid);
@@ -345,7 +392,8 @@ __global__ void __launch_bounds__(256) k_pressure_update(
     float ppk, float* __restrict__ au, float* __restrict__ av,
     float* __restrict__ bu, float* __restrict__ bv, float dt_over_density, int do_dirichlet,
     int do_diff, const float* __restrict__ prev_u, const float* __restrict__ prev_v,
-    float* __restrict__ diff_u, float* __restrict__ diff_v, uint8_t* __restrict__ zmask) {
+    float* __restrict__ diff_u, float* __restrict__ diff_v, uint8_t* __restrict__ zmask,
+    uint8_t* __restrict__ dec) {
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int c = t * 256 + threadIdx.x;
   if (c >= g.nx * g.ny) return;
@@ -401,6 +449,8 @@ __global__ void __launch_bounds__(256) k_pressure_update(
   if (zv) nv = 0.f;
   // the adjoint's record of the Dirichlet decisions (bit 0: u zeroed, bit 1: v)
   if (zmask) zmask[c] = (uint8_t)((zu ? 1 : 0) | (zv ? 2 : 0));
+  // the step's decision record: bit 4 / 5 u / v zeroed by this (second) Dirichlet pass
+  if (dec) dec[c] = (uint8_t)(dec[c] | (zu ? 16 : 0) | (zv ? 32 : 0));
   bu[c] = nu;
   bv[c] = nv;
   if (do_diff) {
@@ -814,16 +864,37 @@ void launch_cell_init(hipStream_t s, const Geom& g, uint8_t* label, const uint8_
 }
 void launch_particle_scatter(hipStream_t s, const Geom& g, const float4* pt, int64_t np,
                              uint8_t* label, float4* acc, int do_classify, int do_p2g,
-                             const BatchTables* batch) {
+                             const BatchTables* batch, long long* acc_fx) {
   if (batch) {
     if (batch->n_ptile == 0) return;
-    k_particle_scatter<true><<<batch->n_ptile, 256, 0, s>>>(g, pt, np, label, (float*)acc,
-                                                            do_classify, do_p2g, *batch);
+    if (acc_fx)
+      k_particle_scatter<true, true><<<batch->n_ptile, 256, 0, s>>>(g, pt, np, label, acc_fx,
+                                                                    do_classify, do_p2g, *batch);
+    else
+      k_particle_scatter<true, false><<<batch->n_ptile, 256, 0, s>>>(g, pt, np, label, acc,
+                                                                     do_classify, do_p2g, *batch);
     return;
   }
   if (np == 0) return;
-  k_particle_scatter<false><<<nblocks(np), 256, 0, s>>>(g, pt, np, label, (float*)acc,
-                                                        do_classify, do_p2g, BatchTables{});
+  if (acc_fx)
+    k_particle_scatter<false, true><<<nblocks(np), 256, 0, s>>>(g, pt, np, label, acc_fx,
+                                                                do_classify, do_p2g, BatchTables{});
+  else
+    k_particle_scatter<false, false><<<nblocks(np), 256, 0, s>>>(g, pt, np, label, acc,
+                                                                 do_classify, do_p2g, BatchTables{});
+}
+
+// Fixed-point sums -> float: out[k] = (float)(fx[k] * 2^-40) (one rounding from the
+// double, exact below 2^53), and fx[k] := 0 for the next accumulation.  n values.
+__global__ void __launch_bounds__(256) k_fx_to_float(int64_t n, long long* __restrict__ fx,
+                                                     float* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  out[k] = fx_value(fx[k]);
+  fx[k] = 0;
+}
+void launch_fx_to_float(hipStream_t s, int64_t n, long long* fx, float* out) {
+  if (n > 0) k_fx_to_float<<<nblocks(n), 256, 0, s>>>(n, fx, out);
 }
 
 int bucket_count(const Geom& g) {
@@ -845,10 +916,10 @@ void launch_face_update(hipStream_t s, const Geom& g, int flags, float* u, float
                         const float4* acc, float* prev_u, float* prev_v, const uint8_t* label,
                         float gx_dt, float gy_dt, float* ext_u, float* ext_v, uint8_t* mxf,
                         uint8_t* mxb, uint8_t* myf, uint8_t* myb, uint8_t* info_u,
-                        uint8_t* info_v) {
+                        uint8_t* info_v, uint8_t* dec) {
   k_face_update<<<nblocks((int64_t)g.nx * g.ny), 256, 0, s>>>(
       g, flags, u, v, acc, prev_u, prev_v, label, gx_dt, gy_dt, ext_u, ext_v, mxf, mxb, myf,
-      myb, info_u, info_v);
+      myb, info_u, info_v, dec);
 }
 void launch_extend_sweep(hipStream_t s, const Geom& g, const uint8_t* label, float* ub,
                          float* vb, const uint8_t* mxf, uint8_t* mxb, const uint8_t* myf,
@@ -865,10 +936,10 @@ void launch_pressure_update(hipStream_t s, const Geom& g, const int* n_fluid,
                             const int* counts, float ppk, float* au, float* av,
                             float* bu, float* bv, float dt_over_density, int do_dirichlet,
                             int do_diff, const float* prev_u, const float* prev_v, float* diff_u,
-                            float* diff_v, uint8_t* zmask) {
+                            float* diff_v, uint8_t* zmask, uint8_t* dec) {
   k_pressure_update<<<nblocks((int64_t)g.nx * g.ny), 256, 0, s>>>(
       g, n_fluid, label, xg, x64, counts, ppk, au, av, bu, bv, dt_over_density, do_dirichlet,
-      do_diff, prev_u, prev_v, diff_u, diff_v, zmask);
+      do_diff, prev_u, prev_v, diff_u, diff_v, zmask, dec);
 }
 void launch_count_particles(hipStream_t s, const Geom& g, const float4* pt, int64_t np,
                             int* counts) {

@@ -61,7 +61,7 @@ EXPORTS = [
     "fsg_batch_get_labels", "fsg_checkpoint_save", "fsg_checkpoint_load",
     "fsg_checkpoint_clear", "fsg_adjoint_set", "fsg_adjoint_get", "fsg_adjoint_step",
     "fsg_adjoint_last_iterations", "fsg_particle_density", "fsg_adjoint_seed_density",
-    "fsg_set_kernel_clock", "fsg_get_kernel_clock",
+    "fsg_set_kernel_clock", "fsg_get_kernel_clock", "fsg_get_step_decisions",
 ]
 
 
@@ -77,7 +77,8 @@ class Params(C.Structure):
         ("refine_inner_tol", C.c_double), ("refine_max_outer", C.c_int32),
         ("record_ext_info", C.c_int32), ("pcg_blocks", C.c_int32), ("pcg_kpt", C.c_int32),
         ("warm_start", C.c_int32), ("particle_pressure_k", C.c_float),
-        ("closed_walls", C.c_int32), ("graph", C.c_int32),
+        ("closed_walls", C.c_int32), ("graph", C.c_int32), ("deterministic", C.c_int32),
+        ("record_decisions", C.c_int32),
     ]
 
 
@@ -168,6 +169,7 @@ def load(path: str | None = None):
             "fsg_adjoint_seed_density": ([H, C.c_void_p], C.c_int),
             "fsg_set_kernel_clock": ([H, C.c_int32], C.c_int),
             "fsg_get_kernel_clock": ([H, P(C.c_double), P(C.c_int64)], C.c_int),
+            "fsg_get_step_decisions": ([H, C.c_void_p], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -358,6 +360,12 @@ class Domain:
         a = np.zeros(n, np.uint8)
         self._chk(self._L.fsg_get_reject_flags(self._h, a.ctypes.data), "fsg_get_reject_flags")
         return a
+
+    def step_decisions(self):
+        """The last step's discrete decisions per cell (fsg_params.record_decisions):
+        bit 0 / 1 u / v normalised by P2G, bit 2 / 3 u / v zeroed by the first
+        Dirichlet pass, bit 4 / 5 by the second."""
+        return self._grid("fsg_get_step_decisions", np.uint8)
 
     def stats(self) -> Stats:
         s = Stats()

@@ -127,6 +127,16 @@ typedef struct fsg_params {
                              HIP graphs captured once per handle (two iterations per
                              graph); 0 launches every kernel.  Per-kernel timing and
                              the kernel clocks launch every kernel. */
+  int32_t deterministic;  /* 0 (default): the P2G splat and the particle density add
+                             their float terms with float atomics, in no fixed order
+                             (like any parallel sum; the reference sums in particle
+                             order); 1: the terms are added as 64-bit fixed-point
+                             integers (2^-40 units, |sum| < 2^23 per face), so every
+                             step is bit-reproducible: a rerun from the same state
+                             (a checkpointed recompute) takes the same trajectory */
+  int32_t record_decisions; /* 1: each PIC / FLIP / PICFLIP / semi-Lagrangian step
+                               records its discrete decisions per cell for
+                               fsg_get_step_decisions */
 } fsg_params;
 
 /* Counters of the last pressure solve and running totals. */
@@ -205,6 +215,12 @@ int fsg_get_valid_masks(fsg_domain *h, uint8_t *mask_u, uint8_t *mask_v);
 int fsg_get_ext_info(fsg_domain *h, uint8_t *info_u, uint8_t *info_v);
 /* per particle: 1 where advectAndEnsureOutsideObstacles stepped back */
 int fsg_get_reject_flags(fsg_domain *h, uint8_t *flags);
+/* The discrete decisions of the last step (fsg_params.record_decisions = 1), one byte
+ * per cell: bit 0 / 1 u / v normalised by the P2G weight test w > 1e-6
+ * (src/FluidSolver.cpp:898-917), bit 2 / 3 u / v zeroed by the first enforceDirichlet
+ * (:233), bit 4 / 5 by the second (:240).  With the labels and the reject flags they
+ * fix the branch of the piecewise-smooth step a gradient is taken on. */
+int fsg_get_step_decisions(fsg_domain *h, uint8_t *dec);
 int fsg_get_stats(fsg_domain *h, fsg_stats *s);
 int fsg_reset_stats(fsg_domain *h);
 

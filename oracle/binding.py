@@ -49,6 +49,8 @@ def _state_type(R):
             ("n_fluid", C.c_int32), ("cg_iterations", C.c_int32),
             ("cg_error", C.c_double), ("particle_pressure_k", R),
             ("closed_walls", C.c_int32),
+            ("replay_label", U8), ("replay_face", U8), ("replay_reject", U8),
+            ("record_face", U8), ("dirichlet_pass", C.c_int32),
         ]
     return State
 
@@ -188,6 +190,37 @@ class OracleSim:
     def set_params(self, **kw):
         for k, v in kw.items():
             setattr(self.s, k, v)
+
+    # -- decision replay --------------------------------------------------------
+    def record_faces(self, on=True):
+        """Record each step's face decisions in ``self.face_record`` (one byte per cell:
+        bit 0 / 1 u / v normalised by P2G, bit 2 / 3 zeroed by the first Dirichlet
+        pass, bit 4 / 5 by the second)."""
+        if on:
+            self.face_record = np.zeros((self.ny, self.nx), np.uint8)
+            self.s.record_face = _ptr(self.face_record, C.c_uint8)
+        else:
+            self.face_record = None
+            self.s.record_face = C.POINTER(C.c_uint8)()
+
+    def set_replay(self, label=None, face=None, reject=None):
+        """The next steps take these decisions instead of computing them: labels
+        (ny, nx), face decisions (ny, nx; bits as in record_faces), reject flags (one
+        per particle).  None leaves that decision to the step."""
+        sh = (self.ny, self.nx)
+        self._replay = []
+        for name, a, n in (("replay_label", label, sh), ("replay_face", face, sh),
+                           ("replay_reject", reject, None)):
+            if a is None:
+                setattr(self.s, name, C.POINTER(C.c_uint8)())
+                continue
+            a = np.ascontiguousarray(a, dtype=np.uint8)
+            if n is not None:
+                a = a.reshape(n)
+            elif a.size != len(self.particles):
+                raise ValueError("replayed reject flags: one per particle")
+            self._replay.append(a)
+            setattr(self.s, name, _ptr(a, C.c_uint8))
 
     # -- running --------------------------------------------------------------
     def run_phase(self, phase, dt=0.0):

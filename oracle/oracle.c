@@ -24,9 +24,13 @@
  *   interp / splat      exact on affine (interp) fields; partition of unity (splat)
  *   rk3                 stability polynomial of a linear field = cubic Taylor poly
  *   g2p / advect        closed forms on affine fields; rejection decision table
- *   full step           composition of the above (the stale-buffer carry-over of
- *                       src/FluidSolver.cpp:902-918,439-482 is reproduced by
- *                       construction; no independent check of the composition)
+ *   full step           the buffer state machine of src/FluidSolver.cpp:211-251 by
+ *                       invariants derived from the source (prev = the brute-force
+ *                       P2G grid over the stale back buffer; non-union faces; the
+ *                       no-fluid early return)
+ *   decision replay     (test support, no counterpart in the reference) a step that
+ *                       replays the decisions it recorded itself is the same step,
+ *                       bit for bit; a replayed decision that differs changes it
  */
 #include <float.h>
 #include <math.h>
@@ -141,6 +145,10 @@ static void clear_labels(STATE *s) {
  * cell = (int)((x / lengthX) * sizeX), clamped; the border (and static solids)
  * are re-marked SOLID afterwards. */
 static void classify_particles(STATE *s) {
+  if (s->replay_label) { /* decision replay: the recorded labels */
+    memcpy(s->label, s->replay_label, (size_t)s->nx * s->ny);
+    return;
+  }
   clear_labels(s);
   const REAL len_x = (REAL)s->nx * s->dx; /* GridInterface::lengthX, Grid.h:54 */
   const REAL len_y = (REAL)s->ny * s->dy;
@@ -199,8 +207,14 @@ static void p2g(STATE *s) {
   }
   REAL *ub = U_B(s), *vb = V_B(s);
   for (size_t c = 0; c < C; ++c) {
-    if (s->acc_wu[c] > 0.000001) ub[c] = s->acc_u[c] / s->acc_wu[c];
-    if (s->acc_wv[c] > 0.000001) vb[c] = s->acc_v[c] / s->acc_wv[c];
+    int nu = s->acc_wu[c] > 0.000001, nv = s->acc_wv[c] > 0.000001;
+    if (s->replay_face) { /* decision replay (a zero weight cannot be normalised) */
+      nu = (s->replay_face[c] & 1) && s->acc_wu[c] != 0;
+      nv = (s->replay_face[c] & 2) && s->acc_wv[c] != 0;
+    }
+    if (nu) ub[c] = s->acc_u[c] / s->acc_wu[c];
+    if (nv) vb[c] = s->acc_v[c] / s->acc_wv[c];
+    if (s->record_face) s->record_face[c] = (uint8_t)(nu | (nv << 1));
   }
   swap_vel(s);
 }
@@ -239,22 +253,32 @@ static void gravity(STATE *s, REAL dt) {
  * the reference): every face with a SOLID cell on either side is zeroed. */
 static void dirichlet(STATE *s) {
   REAL *u = U_F(s), *v = V_F(s);
+  /* decision bits of this pass: the first pass of a step 4 / 8, later ones 16 / 32 */
+  const int k = s->dirichlet_pass > 0 ? 1 : 0;
+  const uint8_t bu = (uint8_t)(4 << (2 * k)), bv = (uint8_t)(8 << (2 * k));
+  s->dirichlet_pass++;
   for (int j = 0; j < s->ny; ++j)
     for (int i = 0; i < s->nx; ++i) {
       int im = clampi(i - 1, 0, s->nx - 1);
       int jm = clampi(j - 1, 0, s->ny - 1);
       size_t c = IDX(s, i, j);
+      int zu, zv;
       if (s->closed_walls) {
-        if (cell(s, im, j) == ORC_SOLID || cell(s, i, j) == ORC_SOLID) u[c] = 0;
-        if (cell(s, i, jm) == ORC_SOLID || cell(s, i, j) == ORC_SOLID) v[c] = 0;
-        continue;
+        zu = cell(s, im, j) == ORC_SOLID || cell(s, i, j) == ORC_SOLID;
+        zv = cell(s, i, jm) == ORC_SOLID || cell(s, i, j) == ORC_SOLID;
+      } else {
+        zu = (cell(s, im, j) == ORC_SOLID && u[c] < 0) ||
+             (cell(s, i, j) == ORC_SOLID && u[c] > 0);
+        zv = (cell(s, i, jm) == ORC_SOLID && v[c] < 0) ||
+             (cell(s, i, j) == ORC_SOLID && v[c] > 0);
       }
-      if ((cell(s, im, j) == ORC_SOLID && u[c] < 0) ||
-          (cell(s, i, j) == ORC_SOLID && u[c] > 0))
-        u[c] = 0;
-      if ((cell(s, i, jm) == ORC_SOLID && v[c] < 0) ||
-          (cell(s, i, j) == ORC_SOLID && v[c] > 0))
-        v[c] = 0;
+      if (s->replay_face) { /* decision replay */
+        zu = (s->replay_face[c] & bu) != 0;
+        zv = (s->replay_face[c] & bv) != 0;
+      }
+      if (zu) u[c] = 0;
+      if (zv) v[c] = 0;
+      if (s->record_face) s->record_face[c] |= (uint8_t)((zu ? bu : 0) | (zv ? bv : 0));
     }
 }
 
@@ -618,7 +642,9 @@ static void advect_particles(STATE *s, REAL dt, int reject) {
     if (reject) {
       int x = (int)(q[0] / s->dx);
       int y = (int)(q[1] / s->dy);
-      if (cell(s, x, y) == ORC_SOLID) {
+      int pushed = cell(s, x, y) == ORC_SOLID;
+      if (s->replay_reject) pushed = s->replay_reject[p] != 0; /* decision replay */
+      if (pushed) {
         q[0] += q[2] * -dt;
         q[1] += q[3] * -dt;
         flag = 1;
@@ -843,6 +869,8 @@ int F(orc_run_phase)(STATE *s, int32_t phase, REAL dt) {
  * plus the grid-only gather scheme (labels from the level set, gather advection,
  * gravity, Dirichlet, projection, Dirichlet). */
 int F(orc_step)(STATE *s, int32_t scheme, REAL dt) {
+  s->dirichlet_pass = 0;
+  if (s->record_face) memset(s->record_face, 0, (size_t)s->nx * s->ny);
   switch (scheme) {
     case ORC_SCHEME_SEMI_LAGRANGIAN:
       classify_particles(s);

@@ -70,6 +70,16 @@ enum {
     double cg_error;                                                                \
     R particle_pressure_k;    /* 0 (FluidSolver.cpp:443) */                         \
     int32_t closed_walls;     /* 1: Dirichlet zeroes every face next to SOLID */    \
+    /* Decision replay (all NULL: the reference's own decisions).  Set, the next  \
+     * step takes its discrete decisions from these records instead of computing  \
+     * them: labels (per cell), reject flags (per particle) and per cell a byte  \
+     * of face decisions: bit 0 / 1 u / v normalised by P2G, bit 2 / 3 u / v     \
+     * zeroed by the first Dirichlet pass, bit 4 / 5 by the second. */          \
+    const uint8_t *replay_label;                                                    \
+    const uint8_t *replay_face;                                                     \
+    const uint8_t *replay_reject;                                                   \
+    uint8_t *record_face;     /* optional: the face decisions this step took */     \
+    int32_t dirichlet_pass;   /* Dirichlet passes so far in the current step */     \
   } orc_state##SFX;                                                                 \
   void orc_defaults##SFX(orc_state##SFX *s, int32_t nx, int32_t ny, R length_x,     \
                          R length_y, R density, R pic_ratio);                       \

@@ -1,0 +1,109 @@
+"""The differentiable configuration's gradient check, at 64 x 64: a closed all-fluid
+box, initial velocities from a (2, 16, 16) control grid bilinearly upsampled, 64
+PICFLIP steps (src/FluidSolver.cpp:211-251) with checkpoints every 8, loss = the L2
+distance between the final particle-splat density and a target from another control.
+
+The rollout runs deterministic (fixed-point splats), so the checkpointed recompute of
+the reverse pass retraces the forward run bit for bit, and records its discrete
+decisions (labels, P2G weight tests, Dirichlet passes, obstacle rejections).  The
+adjoint's directional derivatives along 16 seeded unit directions in control space
+are checked against central differences of the FP64 oracle replaying those decisions:
+the derivative of the branch of the piecewise-smooth rollout the GPU run took, which
+the FP32 differences of the GPU itself cannot resolve at eps = 1e-3."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from fluid_simulation_amd import adjoint, fsg, scenes
+
+pytestmark = pytest.mark.gpu
+
+N, STEPS, EVERY, DT, SCALE, DIRS = 64, 64, 8, 0.01, 4.0, 16
+SCHEME_ORACLE_PICFLIP = 3
+
+
+@pytest.fixture(scope="module")
+def c5():
+    import torch
+    n = N
+    jj, ii = np.meshgrid(np.arange(1, n - 1), np.arange(1, n - 1), indexing="ij")
+    parts = scenes.jittered_particles(ii.ravel(), jj.ravel(), n, 1.0, 1.0, 0)
+    model = adjoint.ControlModel(n, n, float(n), float(n), parts[:, :2])
+    d = fsg.Domain(n, n, float(n), float(n), DT, 0.05)
+    d.set_params(solver=fsg.SOLVER_MG, cg_tol=1e-10, cg_max_iter=500, project_mean=1,
+                 closed_walls=1, deterministic=1, record_decisions=1)
+
+    def start(c):
+        vel, u, v = model(c)
+        adjoint.set_state(d, np.concatenate([parts[:, :2], vel.detach().numpy()], 1),
+                          u.detach().numpy(), v.detach().numpy())
+
+    c1 = torch.randn(2, 16, 16, generator=torch.Generator().manual_seed(1)) * SCALE
+    start(c1)
+    for _ in range(STEPS):
+        d.step(fsg.PICFLIP, DT)
+    target = d.density().astype(np.float64)
+    c0 = torch.randn(2, 16, 16, generator=torch.Generator().manual_seed(0)) * SCALE
+    c0.requires_grad_(True)
+    vel, u, v = model(c0)
+    pos = torch.tensor(parts[:, :2])
+    loss = adjoint.Rollout.apply(pos, vel, u, v, d, fsg.PICFLIP, DT, STEPS, EVERY, target)
+    loss.backward()
+    grad = c0.grad.detach().numpy().astype(np.float64)
+    c0 = c0.detach()
+    # the forward run again, step by step, recording its decisions
+    start(c0)
+    recs = adjoint.rollout_decisions(d, fsg.PICFLIP, DT, STEPS)
+    rho = d.density().astype(np.float64)
+    d.close()
+    return dict(parts=parts, model=model, target=target, c0=c0, grad=grad,
+                loss=float(loss), loss_again=float(((rho - target) ** 2).sum()), recs=recs)
+
+
+def test_c5_rollout_is_reproducible(c5):
+    """The deterministic rollout: the recorded rerun's loss equals the checkpointed
+    forward's bit for bit, and every step took a decision worth replaying."""
+    assert c5["loss_again"] == c5["loss"]
+    assert len(c5["recs"]) == STEPS
+    assert any(r[2].any() for r in c5["recs"]), "no obstacle rejection in the run"
+
+
+def test_c5_gradient_against_fp64_replayed_differences(c5):
+    import torch
+    from oracle.replay import ReplayRollout
+    parts, model, target, c0, grad = (c5[k] for k in ("parts", "model", "target", "c0", "grad"))
+    rr = ReplayRollout(N, N, float(N), float(N), DT, 0.05, c5["recs"], cg_tol=1e-13,
+                       closed_walls=1, project_mean=1)
+
+    def state(c):
+        vel, u, v = model(torch.as_tensor(c), dtype=torch.float64)
+        p = np.concatenate([parts[:, :2].astype(np.float64), vel.numpy()], 1)
+        return p, u.numpy(), v.numpy()
+
+    base = c0.numpy().astype(np.float64)
+    l64 = rr.density_losses([state(base)], SCHEME_ORACLE_PICFLIP, DT, target)[0]
+    print(f"loss: GPU FP32 {c5['loss']:.9e}, FP64 replay {l64:.9e} "
+          f"(rel {abs(l64 - c5['loss']) / abs(l64):.2e})", flush=True)
+    rng = np.random.default_rng(16)
+    dirs = []
+    for _ in range(DIRS):
+        dv = rng.standard_normal((2, 16, 16))
+        dirs.append(dv / np.linalg.norm(dv))
+    worst = {}
+    for eps in (1e-3, 1e-5):
+        states = []
+        for dv in dirs:
+            states += [state(base + eps * dv), state(base - eps * dv)]
+        ls = rr.density_losses(states, SCHEME_ORACLE_PICFLIP, DT, target)
+        rels = []
+        for k, dv in enumerate(dirs):
+            fd = (ls[2 * k] - ls[2 * k + 1]) / (2 * eps)
+            ad = float((grad * dv).sum())
+            rels.append(abs(ad - fd) / max(abs(fd), abs(ad), 1e-30))
+            print(f"eps {eps:g} dir {k:2d}: adjoint {ad:+.6e} fp64 {fd:+.6e} rel {rels[-1]:.2e}",
+                  flush=True)
+        worst[eps] = max(rels)
+        print(f"eps {eps:g}: median rel {np.median(rels):.2e}, max {max(rels):.2e}", flush=True)
+    assert worst[1e-3] <= 1e-3
+    assert worst[1e-5] <= 1e-3
