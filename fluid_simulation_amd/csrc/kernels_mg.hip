@@ -221,20 +221,37 @@ __global__ void __launch_bounds__(kT) k_mg_setup0(Geom g, const uint8_t* __restr
   write_part<kT>(cnt, part_cnt, sm);
 }
 
+// AIR neighbours of cell (i, j) (Dirichlet links of its row: an unknown with one
+// makes the system non-singular).
+__device__ __forceinline__ int air_links(const uint8_t* __restrict__ lab, int nx, int ny, int i,
+                                         int j) {
+  const int c = i + j * nx;
+  int n = 0;
+  n += i > 0 && lab[c - 1] == kAir;
+  n += i < nx - 1 && lab[c + 1] == kAir;
+  n += j > 0 && lab[c - nx] == kAir;
+  n += j < ny - 1 && lab[c + nx] == kAir;
+  return n;
+}
+
 // Adjoint solves (the step's reverse pass): the right-hand side is a given cell
 // field y (the adjoint of the pressure) instead of the divergence.  With the mean
 // removed the forward solution is x = S A^+ P b (P: remove the mean of b, S: the
-// D-weighted constant of k_mg_finish), so the adjoint is P A^+ S^T y with
-// S^T y = y - n (sum y) / (sum n) over the unknowns (n: non-SOLID neighbours).
-// Pass 1: info and the partials of sum y and sum n (into part_y / part_n).
+// D-weighted constant of k_mg_finish, applied only when the system is singular, i.e.
+// no unknown has an AIR neighbour), so the adjoint is P A^+ S^T y with
+// S^T y = y - n (sum y) / (sum n) over the unknowns (n: non-SOLID neighbours), or
+// P A^-1 y for a non-singular system.
+// Pass 1: info and the partials of sum y, sum n and the AIR links (part_y / part_n /
+// part_d).
 __global__ void __launch_bounds__(kT) k_mg_adj_rhs0(Geom g, const uint8_t* __restrict__ label,
                                                     const double* __restrict__ y,
                                                     uint8_t* __restrict__ inf,
                                                     double* __restrict__ part_y,
-                                                    double* __restrict__ part_n) {
+                                                    double* __restrict__ part_n,
+                                                    double* __restrict__ part_d) {
   __shared__ double sm[4];
   const int nx = g.nx, ny = g.ny, C = nx * ny;
-  double sy = 0.0, sn = 0.0;
+  double sy = 0.0, sn = 0.0, sd = 0.0;
   CellWalk w(blockIdx.x * kT + threadIdx.x, gridDim.x * kT, nx);
   for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT, w.next()) {
     const uint8_t f = make_info(label, nx, ny, w.i, w.j);
@@ -242,10 +259,12 @@ __global__ void __launch_bounds__(kT) k_mg_adj_rhs0(Geom g, const uint8_t* __res
     if (is_act(f)) {
       sy += y[c];
       sn += (double)(f & 7);
+      sd += (double)air_links(label, nx, ny, w.i, w.j);
     }
   }
   write_part<kT>(sy, part_y, sm);
   write_part<kT>(sn, part_n, sm);
+  write_part<kT>(sd, part_d, sm);
 }
 
 // Pass 2: b' = -dx^2 S^T y at the unknowns (S^T = I without the mean removal), and
@@ -254,13 +273,14 @@ __global__ void __launch_bounds__(kT) k_mg_adj_rhs1(int C, double scale, int pro
                                                     const uint8_t* __restrict__ inf,
                                                     const double* __restrict__ y,
                                                     const double* __restrict__ part_y,
-                                                    const double* __restrict__ part_n, int n_part,
+                                                    const double* __restrict__ part_n,
+                                                    const double* __restrict__ part_d, int n_part,
                                                     double* __restrict__ b,
                                                     double* __restrict__ part_sum,
                                                     double* __restrict__ part_cnt) {
   __shared__ double sm[4];
   double shift = 0.0;
-  if (project_mean) {
+  if (project_mean && sum_part<kT>(part_d, n_part, sm) == 0.0) {
     const double a = sum_part<kT>(part_y, n_part, sm);
     const double n = sum_part<kT>(part_n, n_part, sm);
     shift = n > 0.0 ? a / n : 0.0;
@@ -1586,44 +1606,59 @@ __global__ void __launch_bounds__(kT) k_mg_xr(int nx, int ny, const uint8_t* __r
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->spmv = it + 1;
 }
 
-// Partials of sum(n_i x_i) and sum(n_i) over the unknowns (n_i = K_ii).
-__global__ void __launch_bounds__(kT) k_mg_wsum(int C, const uint8_t* __restrict__ inf,
+// Partials of sum(n_i x_i), sum(n_i) (n_i = K_ii) and of the AIR links over the
+// unknowns: n_i minus the unknown neighbours (a non-SOLID neighbour of an unknown is
+// an unknown or AIR).
+__global__ void __launch_bounds__(kT) k_mg_wsum(int nx, int ny, const uint8_t* __restrict__ inf,
                                                  const double* __restrict__ x0,
                                                  const double* __restrict__ x1,
                                                  const CgScalars* sc, double* __restrict__ part_nx,
-                                                 double* __restrict__ part_n) {
+                                                 double* __restrict__ part_n,
+                                                 double* __restrict__ part_d) {
   __shared__ double sm[4];
+  const int C = nx * ny;
   const double* x = (sc->iter & 1) ? x1 : x0;
-  double a = 0.0, n = 0.0;
-  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT) {
+  double a = 0.0, n = 0.0, d = 0.0;
+  CellWalk w(blockIdx.x * kT + threadIdx.x, gridDim.x * kT, nx);
+  for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT, w.next()) {
     const uint8_t f = inf[c];
     if (is_act(f)) {
       a += (double)(f & 7) * x[c];
       n += (double)(f & 7);
+      int na = 0;
+      na += w.i > 0 && is_act(inf[c - 1]);
+      na += w.i < nx - 1 && is_act(inf[c + 1]);
+      na += w.j > 0 && is_act(inf[c - nx]);
+      na += w.j < ny - 1 && is_act(inf[c + nx]);
+      d += (double)((int)(f & 7) - na);
     }
   }
   write_part<kT>(a, part_nx, sm);
   write_part<kT>(n, part_n, sm);
+  write_part<kT>(d, part_d, sm);
 }
 
 // Pressure grid for the velocity update: x at unknowns, 0 elsewhere.  With the
-// mean of b removed (all-Neumann systems, defined up to a constant) the constant
-// is fixed as Jacobi-preconditioned CG from x0 = 0 fixes it: every Krylov vector of
-// D^-1 K has zero D-weighted sum, so that solve ends with sum(n_i x_i) = 0.  The
-// constant matters: the update treats SOLID neighbours as p = 0.
+// mean of b removed and no unknown next to AIR (an all-Neumann system, defined up to
+// a constant) the constant is fixed as Jacobi-preconditioned CG from x0 = 0 fixes
+// it: every Krylov vector of D^-1 K then has zero D-weighted sum, so that solve ends
+// with sum(n_i x_i) = 0.  The constant matters: the update treats SOLID neighbours
+// as p = 0.  With an AIR neighbour the system is non-singular and x is kept as
+// solved (part_d: the AIR-link partials).
 __global__ void __launch_bounds__(kT) k_mg_finish(int C, const uint8_t* __restrict__ inf,
                                                    const double* __restrict__ x0,
                                                    const double* __restrict__ x1,
                                                    const CgScalars* sc,
                                                    const double* __restrict__ part_nx,
-                                                   const double* __restrict__ part_n, int n_part,
+                                                   const double* __restrict__ part_n,
+                                                   const double* __restrict__ part_d, int n_part,
                                                    double* __restrict__ xout,
                                                    float* __restrict__ xg,
                                                    const int* __restrict__ sid,
                                                    const double* __restrict__ scene_norm) {
   __shared__ double sm[4];
   double shift = 0.0;
-  if (part_nx) {
+  if (part_nx && sum_part<kT>(part_d, n_part, sm) == 0.0) {
     const double a = sum_part<kT>(part_nx, n_part, sm);
     const double n = sum_part<kT>(part_n, n_part, sm);
     shift = n > 0.0 ? a / n : 0.0;
@@ -2044,9 +2079,10 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
   const int C = g.nx * g.ny, nb = vec_blocks(C);
   if (adj_rhs) {
     // part_bb / part_rr are free until k_mg_setup1 writes them
-    k_mg_adj_rhs0<<<nb, kT, 0, s>>>(g, label, adj_rhs, m.inf[0], m.part_bb, m.part_rr);
+    k_mg_adj_rhs0<<<nb, kT, 0, s>>>(g, label, adj_rhs, m.inf[0], m.part_bb, m.part_rr,
+                                    m.part_pq);
     k_mg_adj_rhs1<<<nb, kT, 0, s>>>(C, -(double)g.dx * (double)g.dx, project_mean, m.inf[0],
-                                    adj_rhs, m.part_bb, m.part_rr, nb, m.b, m.part_sum,
+                                    adj_rhs, m.part_bb, m.part_rr, m.part_pq, nb, m.b, m.part_sum,
                                     m.part_cnt);
   } else {
     k_mg_setup0<<<nb, kT, 0, s>>>(g, label, uf, vf, m.inf[0], m.b, m.part_sum, m.part_cnt);
@@ -2059,7 +2095,11 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
   // x[0] holds the previous solve's solution (k_mg_finish); x[1] is free until the
   // first update of this solve
   if (warm) k_mg_warm<<<nb, kT, 0, s>>>(C, m.inf[0], m.x[0], m.x[1]);
-  k_mg_setup1<<<nb, kT, 0, s>>>(g.nx, g.ny, project_mean, m.inf[0], m.b, warm ? m.x[1] : nullptr,
+  // an adjoint right-hand side is not mean-projected: S^T y already has zero sum for a
+  // singular system, and a non-singular one solves A y = x-bar itself (P follows the
+  // solve)
+  k_mg_setup1<<<nb, kT, 0, s>>>(g.nx, g.ny, adj_rhs ? 0 : project_mean, m.inf[0], m.b,
+                                warm ? m.x[1] : nullptr,
                                 m.x[0], m.rl[0], m.part_sum, m.part_cnt, nb, m.part_bb,
                                 m.part_rr, sc);
   // levels coarsened one launch each, down to the first level l0 whose finer level
@@ -2224,13 +2264,14 @@ void launch_mg_finish(hipStream_t s, const MgHost& m, int project_mean, CgScalar
   const int* sid = batch ? batch->sid : nullptr;
   const double* norm = batch ? batch->norm : nullptr;
   if (project_mean) {
-    // part_sum / part_cnt are free again after the setup
-    k_mg_wsum<<<nb, kT, 0, s>>>(C, m.inf[0], m.x[0], m.x[1], sc, m.part_sum, m.part_cnt);
-    k_mg_finish<<<nb, kT, 0, s>>>(C, m.inf[0], m.x[0], m.x[1], sc, m.part_sum, m.part_cnt, nb,
-                                  m.x[0], xg, sid, norm);
+    // part_sum / part_cnt are free again after the setup, part_bb after iteration 0
+    k_mg_wsum<<<nb, kT, 0, s>>>(m.nx[0], m.ny[0], m.inf[0], m.x[0], m.x[1], sc, m.part_sum,
+                                m.part_cnt, m.part_bb);
+    k_mg_finish<<<nb, kT, 0, s>>>(C, m.inf[0], m.x[0], m.x[1], sc, m.part_sum, m.part_cnt,
+                                  m.part_bb, nb, m.x[0], xg, sid, norm);
   } else {
-    k_mg_finish<<<nb, kT, 0, s>>>(C, m.inf[0], m.x[0], m.x[1], sc, nullptr, nullptr, 0, m.x[0],
-                                  xg, sid, norm);
+    k_mg_finish<<<nb, kT, 0, s>>>(C, m.inf[0], m.x[0], m.x[1], sc, nullptr, nullptr, nullptr, 0,
+                                  m.x[0], xg, sid, norm);
   }
 }
 

@@ -26,6 +26,9 @@ _LOCK = threading.Lock()
 _LIVE: "weakref.WeakSet" = weakref.WeakSet()
 
 
+_finalizing = sys.is_finalizing
+
+
 def _close_live():
     for d in list(_LIVE):
         try:
@@ -215,10 +218,11 @@ class Domain:
     def __del__(self):
         # at interpreter exit the atexit hook has already closed every handle; one
         # collected later (finalisation) must not call into HIP, whose runtime may be
-        # torn down by then
-        if sys.is_finalizing():
-            return
+        # torn down by then (nor can it rely on module globals such as sys, which
+        # finalisation may already have cleared: any failure here is ignored)
         try:
+            if _finalizing():
+                return
             self.close()
         except Exception:
             pass

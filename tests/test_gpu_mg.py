@@ -102,6 +102,28 @@ def test_mg_closed_box_project_mean():
     assert max(rel_err(u, ud), rel_err(v, vd)) <= 1e-5
 
 
+def test_mg_project_mean_with_air_is_nonsingular():
+    """A SOLID box of LIQUID with a few AIR cells inside: with the mean of b removed
+    (project_mean) the system is still non-singular (the AIR neighbours are Dirichlet
+    links), so its solution is unique and no constant may be added: the pressure
+    itself, not only its deviation from the mean, matches the FP64 oracle's
+    Jacobi-PCG from x0 = 0, and so does the velocity update."""
+    n = 96
+    lab = np.full((n, n), LIQ, np.uint8)
+    lab[0, :] = lab[-1, :] = SOL
+    lab[:, 0] = lab[:, -1] = SOL
+    lab[40, 50] = lab[41, 50] = lab[70, 20] = AIR
+    d, g = _solve_pair(lab, 9, tol=1e-9, project_mean=1)
+    liq = lab == LIQ
+    pg, pd = g.pressure()[liq].astype(np.float64), d.pressure[liq]
+    ep = rel_err(pg, pd)
+    u, v = g.velocity(fsg.VEL_FRONT)
+    ud, vd = d.vel("front")
+    print(f"box with AIR: p {ep:.2e} u {rel_err(u, ud):.2e} v {rel_err(v, vd):.2e}")
+    assert ep <= 1e-5
+    assert max(rel_err(u, ud), rel_err(v, vd)) <= 1e-5
+
+
 def test_mg_deterministic_and_no_fluid():
     lab = irregular_labels(150, 90, 11)
     _, g1 = _solve_pair(lab, 11)
