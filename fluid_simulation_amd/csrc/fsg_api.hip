@@ -262,30 +262,6 @@ hipError_t dalloc(fsg_domain* h, T** p, size_t n) {
   return hipMemsetAsync(q, 0, n * sizeof(T) + 16, h->stream);
 }
 
-bool is_pow2(float x) {
-  if (!(x > 0.f) || !isfinite(x)) return false;
-  int e;
-  const float m = frexpf(x, &e);
-  return m == 0.5f;
-}
-
-// MacGrid geometry: spacing = length / size in float (src/MacGrid.cpp:8), lengths
-// = size * spacing (GridInterface::lengthX, include/Grid.h:54-55).
-Geom make_geom(int nx, int ny, float length_x, float length_y) {
-  Geom g{};
-  g.nx = nx;
-  g.ny = ny;
-  g.dx = length_x / (float)nx;
-  g.dy = length_y / (float)ny;
-  g.pow2_x = is_pow2(g.dx);
-  g.pow2_y = is_pow2(g.dy);
-  g.inv_dx = g.pow2_x ? 1.0f / g.dx : 0.f;
-  g.inv_dy = g.pow2_y ? 1.0f / g.dy : 0.f;
-  g.len_x = (float)nx * g.dx;
-  g.len_y = (float)ny * g.dy;
-  return g;
-}
-
 // Timed launch helper: wraps `body` in events when timing is on.
 template <typename F>
 void timed(fsg_domain* h, const char* name, F&& body) {

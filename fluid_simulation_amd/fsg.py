@@ -65,6 +65,7 @@ EXPORTS = [
     "fsg_checkpoint_clear", "fsg_adjoint_set", "fsg_adjoint_get", "fsg_adjoint_step",
     "fsg_adjoint_last_iterations", "fsg_particle_density", "fsg_adjoint_seed_density",
     "fsg_set_kernel_clock", "fsg_get_kernel_clock", "fsg_get_step_decisions",
+    "fsg_p2g_noalloc",
 ]
 
 

@@ -314,6 +314,27 @@ int fsg_get_timing(fsg_domain *h, int32_t cap, const char **names, double *ms,
                    int64_t *launches);
 int fsg_reset_timing(fsg_domain *h);
 
+/* ---- an entry point that allocates nothing ----------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The P2G transfer of fsg_run_phase(FSG_PH_P2G)
+ * (transferVelocityToGridSpread, src/FluidSolver.cpp:873-919) on caller-owned
+ * device buffers: n particles (device AoS x, y, vx, vy) splatted with the
+ * deterministic fixed-point sums of fsg_params.deterministic = 1, then
+ * back := sum / weight at the faces whose weight exceeds 1e-6; other faces keep their
+ * values (the caller swaps the buffers, as MacGrid::swapVelocityBuffers does).  The
+ * grid is nx x ny cells of length_x / nx by length_y / ny, like fsg_create's.  The
+ * sums live in the library's static device scratch of FSG_NOALLOC_MAX_CELLS cells
+ * (32 MiB per GPU, held until the process ends): bigger grids, negative sizes and
+ * null buffers return FSG_E_INVALID before any device call.  Every call on one GPU
+ * shares that scratch, so calls that overlap on one GPU race; calls on one stream do
+ * not.  Work is enqueued on `stream` (a hipStream_t; NULL: the null stream) and the
+ * call returns without synchronising. */
+#define FSG_NOALLOC_MAX_CELLS (1 << 20)
+int fsg_p2g_noalloc(const float *particles_xyuv, int64_t n, int32_t nx, int32_t ny,
+                    float length_x, float length_y, float *back_u, float *back_v,
+                    void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,23 @@ def test_params_struct_layout_and_defaults():
     assert p.deterministic == 0 and p.record_decisions == 0
 
 
+def test_noalloc_rejects_bad_sizes_before_any_device_call():
+    """fsg_p2g_noalloc checks its sizes and buffers before its first HIP call (so this
+    runs without a GPU): a grid above FSG_NOALLOC_MAX_CELLS, a negative count and
+    null buffers return FSG_E_INVALID."""
+    f = fsg.load().fsg_p2g_noalloc
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                  ctypes.c_float, ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_void_p]
+    f.restype = ctypes.c_int
+    dummy = ctypes.c_void_p(16)
+    assert f(dummy, 1, 1025, 1024, 1025.0, 1024.0, dummy, dummy, None) == -1
+    assert f(dummy, -1, 64, 64, 64.0, 64.0, dummy, dummy, None) == -1
+    assert f(None, 10, 64, 64, 64.0, 64.0, dummy, dummy, None) == -1
+    assert f(dummy, 10, 64, 64, 64.0, 64.0, None, dummy, None) == -1
+    assert f(dummy, 10, 64, 64, 0.0, 64.0, dummy, dummy, None) == -1
+
+
 def test_version_string():
     assert b"gfx950" in fsg.load().fsg_version()
 
