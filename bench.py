@@ -156,14 +156,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-# multigrid iterations replayed from captured HIP graphs (--graph)
+# multigrid iterations replayed from captured HIP graphs (--graph); the multigrid
+# smoother (--mg-smoother: 0 damped Jacobi, 1 two-step Chebyshev)
 GRAPH = 1
+SMOOTHER = 1
 
 
 def make_domain(fsg, scene, solver, tol, max_iter, warm=0):
     return fsg.Domain.from_scene(scene, cg_tol=tol, cg_max_iter=max_iter, solver=solver,
                                  refine_inner_tol=1e-3, refine_max_outer=20, check_every=64,
-                                 warm_start=warm, graph=GRAPH)
+                                 warm_start=warm, graph=GRAPH, mg_smoother=SMOOTHER)
 
 
 def run_steps(fsg, d, n, dt, scheme=None):
@@ -343,7 +345,8 @@ def batch_leg(fsg, scenes, args):
     scs = scenes.dam_break_batch(args.batch_scenes, args.batch_lo, args.batch_hi)
     cells = sum(sc.nx * sc.ny for sc in scs)
     parts = sum(len(sc.particles) for sc in scs)
-    b = fsg.Batch.from_scenes(scs, cg_tol=args.tol, cg_max_iter=200, graph=GRAPH)
+    b = fsg.Batch.from_scenes(scs, cg_tol=args.tol, cg_max_iter=200, graph=GRAPH,
+                              mg_smoother=SMOOTHER)
     log(f"batch: {len(scs)} scenes, {cells} cells, {parts} particles, packed {b.nx}x{b.ny}")
     run_steps(fsg, b, args.batch_warmup, scs[0].dt)
     prof = kernel_profile(fsg, b, 2, scs[0].dt)
@@ -430,7 +433,7 @@ def grid_leg(fsg, scenes, args):
     walls and overflow within a few steps)."""
     sc = scenes.grid_only(args.grid_n)
     d = fsg.Domain.from_scene(sc, cg_tol=args.tol, cg_max_iter=200, solver=fsg.SOLVER_MG, graph=GRAPH,
-                              integrator=2, project_mean=1, closed_walls=1)
+                              integrator=2, project_mean=1, closed_walls=1, mg_smoother=SMOOTHER)
     run_steps(fsg, d, 2, sc.dt, fsg.GRID_ADVECT)
     prof = kernel_profile(fsg, d, 2, sc.dt, fsg.GRID_ADVECT)
     per_step = {k: ms * n for k, (ms, n) in prof.items()}
@@ -451,16 +454,101 @@ def grid_leg(fsg, scenes, args):
     }
 
 
+def replay_gradient_check(fsg, scenes, n, steps, every, n_dirs, eps, scale, tol, workers):
+    """The differentiable configuration at n^2 (closed all-fluid box, (2, 16, 16)
+    controls x scale, ``steps`` PICFLIP steps, checkpoints every ``every``): the
+    adjoint gradient (deterministic rollout) along ``n_dirs`` seeded unit directions
+    against central FP64 differences (step ``eps``) of the oracle replaying the GPU
+    run's decisions (labels, P2G weight tests, Dirichlet passes, rejections), which
+    follows the branch of the piecewise-smooth rollout the GPU run took.  Runs after
+    the timed legs; the oracle is the checker here, not the thing measured."""
+    import torch
+    from fluid_simulation_amd import adjoint
+    from oracle.replay import ReplayRollout
+    t0 = time.perf_counter()
+    dt = 0.01
+    jj, ii = np.meshgrid(np.arange(1, n - 1), np.arange(1, n - 1), indexing="ij")
+    parts = scenes.jittered_particles(ii.ravel(), jj.ravel(), n, 1.0, 1.0, 0)
+    model = adjoint.ControlModel(n, n, float(n), float(n), parts[:, :2])
+    d = fsg.Domain(n, n, float(n), float(n), dt, 0.05)
+    d.set_params(solver=fsg.SOLVER_MG, cg_tol=tol, cg_max_iter=500, project_mean=1,
+                 closed_walls=1, deterministic=1, record_decisions=1, mg_smoother=SMOOTHER)
+
+    def start(c):
+        vel, u, v = model(c)
+        adjoint.set_state(d, np.concatenate([parts[:, :2], vel.detach().numpy()], 1),
+                          u.detach().numpy(), v.detach().numpy())
+
+    start(torch.randn(2, 16, 16, generator=torch.Generator().manual_seed(1)) * scale)
+    for _ in range(steps):
+        d.step(fsg.PICFLIP, dt)
+    target = d.density().astype(np.float64)
+    c0 = torch.randn(2, 16, 16, generator=torch.Generator().manual_seed(0)) * scale
+    cg = c0.clone().requires_grad_(True)
+    vel, u, v = model(cg)
+    loss = adjoint.Rollout.apply(torch.tensor(parts[:, :2]), vel, u, v, d, fsg.PICFLIP, dt,
+                                 steps, every, target)
+    loss.backward()
+    grad = cg.grad.numpy().astype(np.float64)
+    start(c0)
+    recs = adjoint.rollout_decisions(d, fsg.PICFLIP, dt, steps)
+    d.close()
+    rr = ReplayRollout(n, n, float(n), float(n), dt, 0.05, recs, cg_tol=1e-13, closed_walls=1,
+                       project_mean=1)
+
+    def state(c):
+        pv, uu, vv = model(torch.as_tensor(c), dtype=torch.float64)
+        return (np.concatenate([parts[:, :2].astype(np.float64), pv.numpy()], 1), uu.numpy(),
+                vv.numpy())
+
+    base = c0.numpy().astype(np.float64)
+    rng = np.random.default_rng(16)
+    dirs = []
+    for _ in range(n_dirs):
+        dv = rng.standard_normal((2, 16, 16))
+        dirs.append(dv / np.linalg.norm(dv))
+    states = [state(base)]
+    for dv in dirs:
+        states += [state(base + eps * dv), state(base - eps * dv)]
+    ls = rr.density_losses(states, 3, dt, target, workers=workers)
+    gnorm = float(np.linalg.norm(grad))
+    rows = []
+    for k, dv in enumerate(dirs):
+        fd = (ls[1 + 2 * k] - ls[2 + 2 * k]) / (2 * eps)
+        ad = float((grad * dv).sum())
+        rows.append((ad, fd, abs(ad - fd) / max(abs(fd), abs(ad), 1e-30), abs(ad - fd) / gnorm))
+    out = {
+        "workload": f"closed all-fluid {n}^2 box, {len(parts)} particles, {steps} PICFLIP "
+                    f"steps, checkpoints every {every}, deterministic rollout",
+        "eps": eps, "directions": n_dirs,
+        "loss_gpu_fp32": float(loss), "loss_fp64_replay": ls[0],
+        "loss_rel_diff": abs(ls[0] - float(loss)) / abs(ls[0]),
+        "rel_median": float(np.median([r[2] for r in rows])),
+        "rel_max": float(max(r[2] for r in rows)),
+        "rel_to_grad_norm_max": float(max(r[3] for r in rows)),
+        "grad_norm": gnorm,
+        "seconds": time.perf_counter() - t0,
+        "oracle_threads": workers,
+    }
+    log(f"adjoint vs FP64 replayed differences ({n}^2, {steps} steps, {n_dirs} directions, "
+        f"eps {eps:g}): median rel {out['rel_median']:.2e}, max {out['rel_max']:.2e}, max rel to "
+        f"|grad| {out['rel_to_grad_norm_max']:.2e}; loss FP32 vs FP64 replay "
+        f"{out['loss_rel_diff']:.1e} ({out['seconds']:.1f} s)")
+    return out
+
+
 def adjoint_leg(fsg, scenes, args):
     """The differentiable form: a closed all-fluid ``--adj-n``^2 box (4 particles per
     interior cell), initial velocities from a (2, 16, 16) control grid bilinearly
     upsampled, ``--adj-steps`` PICFLIP steps with checkpoints every ``--adj-every``,
     loss = sum((particle density - target)^2) with the target from control seed 1 and
-    the evaluation at seed 0.  Times the checkpointed forward and the reverse pass,
-    and compares the gradient with central FP32 differences (each ``--adj-eps``) along
-    ``--adj-dirs`` seeded unit directions, counting the label changes between each
-    pair of runs (a discrete decision the differences cross), beside the loss's
-    run-to-run spread at fixed controls (the differences' noise floor)."""
+    the evaluation at seed 0, deterministic (fixed-point) splats.  Times the
+    checkpointed forward and the reverse pass, and compares the gradient with central
+    FP32 differences of the GPU rollout (each ``--adj-eps``) along ``--adj-dirs``
+    seeded unit directions, counting the label changes between each pair of runs (a
+    discrete decision the differences cross), beside the loss's run-to-run spread at
+    fixed controls (zero for the deterministic rollout).  The well-posed check, against
+    FP64 differences replaying the run's decisions, runs at ``--adj-check-n``^2."""
     import torch
     from fluid_simulation_amd import adjoint
     n, steps, every, dt = args.adj_n, args.adj_steps, args.adj_every, 0.01
@@ -469,7 +557,7 @@ def adjoint_leg(fsg, scenes, args):
     model = adjoint.ControlModel(n, n, float(n), float(n), parts[:, :2])
     d = fsg.Domain(n, n, float(n), float(n), dt, 0.05)
     d.set_params(solver=fsg.SOLVER_MG, cg_tol=args.tol, cg_max_iter=200, project_mean=1,
-                 closed_walls=1)
+                 closed_walls=1, deterministic=1, mg_smoother=SMOOTHER)
     pos = torch.tensor(parts[:, :2].astype(np.float32))
 
     def run(c, record=False):
@@ -511,7 +599,7 @@ def adjoint_leg(fsg, scenes, args):
         f"{1e3 * (t0 - tm0):.1f} ms, forward {1e3 * (t1 - t0):.1f} ms, "
         f"reverse {1e3 * (t2 - t1):.1f} ms, control model backward {1e3 * (t3 - t2):.1f} ms, "
         f"loss {loss.item():.6e}, |grad| {np.linalg.norm(grad):.3e}")
-    # run-to-run spread of the loss at the same controls (float atomics in P2G)
+    # run-to-run spread of the loss at the same controls (zero: fixed-point splats)
     with torch.no_grad():
         l_a = float((((run(c)[0]) - target) ** 2).sum())
         l_b = float((((run(c)[0]) - target) ** 2).sum())
@@ -545,7 +633,13 @@ def adjoint_leg(fsg, scenes, args):
             "rel_max_without_label_changes": float(max(clean)) if clean else None,
         }
     d.close()
+    check = None
+    if args.adj_check_n > 0 and args.adj_check_dirs > 0:
+        check = replay_gradient_check(fsg, scenes, args.adj_check_n, steps, every,
+                                      args.adj_check_dirs, 1e-3, scale, 1e-10,
+                                      args.adj_check_threads)
     return {
+        "fp64_replay_check": check,
         "workload": f"closed all-fluid {n}^2 box, {len(parts)} particles, controls (2, 16, 16) "
                     f"x {scale:g} bilinearly upsampled, {steps} PICFLIP steps dt {dt}, "
                     f"checkpoints every {every}, loss sum((density - target)^2)",
@@ -587,6 +681,8 @@ def main():
     ap.add_argument("--graph", type=int, default=1,
                     help="multigrid iterations replayed from captured HIP graphs (1) or "
                          "launched kernel by kernel (0)")
+    ap.add_argument("--mg-smoother", type=int, default=1,
+                    help="multigrid smoothing: 0 damped Jacobi (0.8), 1 two-step Chebyshev")
     ap.add_argument("--warm-start", type=int, default=0,
                     help="chosen form: 1 starts each solve from the previous pressure")
     ap.add_argument("--batch-scenes", type=int, default=256)
@@ -603,6 +699,10 @@ def main():
     ap.add_argument("--adj-every", type=int, default=8)
     ap.add_argument("--adj-dirs", type=int, default=16)
     ap.add_argument("--adj-scale", type=float, default=4.0)
+    ap.add_argument("--adj-check-n", type=int, default=128,
+                    help="grid of the FP64 replayed gradient check (0: off)")
+    ap.add_argument("--adj-check-dirs", type=int, default=8)
+    ap.add_argument("--adj-check-threads", type=int, default=8)
     ap.add_argument("--adj-eps", type=lambda t: [float(x) for x in t.split(",")],
                     default=[1e-3, 3e-2], help="comma-separated FD steps in control space")
     ap.add_argument("--no-fp64-check", action="store_true")
@@ -611,8 +711,9 @@ def main():
                     help="only copy a MIB-sized device buffer 5 times (calibrates the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
-    global GRAPH
+    global GRAPH, SMOOTHER
     GRAPH = args.graph
+    SMOOTHER = args.mg_smoother
     if args.copy_probe:
         import torch
         n = args.copy_probe * (1 << 20) // 4
@@ -801,6 +902,7 @@ def main():
                         f"{P} particles, dt 0.01, density 0.01, 0.95 FLIP",
             "solver": args.solver,
             "graph": args.graph,
+            "mg_smoother": ["damped Jacobi 0.8", "two-step Chebyshev [0.4, 2]"][args.mg_smoother],
             "tol": args.tol,
             "timed_steps": [t_start, t_start + args.steps - 1],
             "untimed_steps_before": t_start,

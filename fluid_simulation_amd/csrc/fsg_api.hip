@@ -455,7 +455,7 @@ int mg_prepare(fsg_domain* h) {
   m.lab[0] = h->label;
   a.lc = m.lc;
   a.nlev = m.nlev;
-  a.omega = 0.8f;
+  a.omega = a.omega2 = 0.8f;  // set per solve from fsg_params.mg_smoother
   a.sweeps = 2 * std::max(m.nx[m.nlev - 1], m.ny[m.nlev - 1]) + 8;
   for (int l = 0; l < m.nlev; ++l) {
     a.nx[l] = m.nx[l];
@@ -490,6 +490,7 @@ fsg::SceneCoarseArgs scene_coarse_args(const fsg_domain* h) {
   a.r_in = m.rl[b.la];
   a.x_out = m.xo[b.la];
   a.omega = m.coarse.omega;
+  a.omega2 = m.coarse.omega2;
   return a;
 }
 
@@ -512,6 +513,14 @@ void alloc_clk_span(fsg_domain* h) {
 int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
   if (int rc = mg_prepare(h)) return rc;
   MgHost& m = h->mg;
+  // smoothing weights: damped Jacobi (0.8, 0.8) or the two-step Chebyshev smoother on
+  // [0.4, 2]: 1 / (1.2 + 0.8 cos(pi / 4)), 1 / (1.2 - 0.8 cos(pi / 4))
+  if (h->prm.mg_smoother == 1) {
+    m.coarse.omega = (float)(1.0 / (1.2 + 0.8 * 0.70710678118654752));
+    m.coarse.omega2 = (float)(1.0 / (1.2 - 0.8 * 0.70710678118654752));
+  } else {
+    m.coarse.omega = m.coarse.omega2 = 0.8f;
+  }
   m.clk_on = h->kclock == 1;
   m.clk_span_on = h->kclock == 2;
   if (m.clk_span_on && !m.clk_span) alloc_clk_span(h);
@@ -869,6 +878,7 @@ void fsg_params_default(fsg_params* p) {
   p->refine_max_outer = 20;
   p->record_ext_info = 0;
   p->graph = 1;
+  p->mg_smoother = 1;
 }
 
 int fsg_create(fsg_domain** out, int32_t nx, int32_t ny, float length_x, float length_y,
@@ -970,7 +980,8 @@ int fsg_set_params(fsg_domain* h, const fsg_params* p) {
       p->integrator > 3 || p->solver < 0 || p->solver > 3 || !(p->cg_tol >= 0.0) ||
       !std::isfinite(p->particle_pressure_k) || p->closed_walls < 0 || p->closed_walls > 1 ||
       p->graph < 0 || p->graph > 1 || p->deterministic < 0 || p->deterministic > 1 ||
-      p->record_decisions < 0 || p->record_decisions > 1)
+      p->record_decisions < 0 || p->record_decisions > 1 || p->mg_smoother < 0 ||
+      p->mg_smoother > 1)
     return fail(h, FSG_E_INVALID, "fsg_set_params: value out of range");
   const size_t C = ncell(h);
   // fixed-point sums: 4 per cell (the P2G accumulators; the density uses the first C),
@@ -993,7 +1004,9 @@ int fsg_set_params(fsg_domain* h, const fsg_params* p) {
       h->max_coop_blocks <= 0)
     return fail(h, FSG_E_INVALID, "persistent solver unavailable on this device");
   // the captured iterations hold the tolerance and the iteration cap
-  if (p->cg_tol != h->prm.cg_tol || p->cg_max_iter != h->prm.cg_max_iter) h->mg_graphs.clear();
+  if (p->cg_tol != h->prm.cg_tol || p->cg_max_iter != h->prm.cg_max_iter ||
+      p->mg_smoother != h->prm.mg_smoother)
+    h->mg_graphs.clear();
   h->prm = *p;
   return FSG_OK;
 }

@@ -191,9 +191,13 @@ constexpr int kMgMaxLevels = 16;
 constexpr int kMgMaxVecBlocks = 1024;  // blocks of the vector kernels (partials per kernel)
 constexpr int kMgCoarseCells = 1024;   // levels at or below this size run in one workgroup
 
+// Smoothing: every level's pre- and post-smoothing is two relaxation sweeps
+// x += w (r - K x) / n, the first with omega, the second with omega2 (damped Jacobi:
+// both 0.8; the two-step Chebyshev smoother: the reciprocals of the roots of the
+// degree-2 Chebyshev polynomial on [0.4, 2], the high end of the spectrum of D^-1 K).
 struct MgCoarseArgs {
   int lc, nlev, total, sweeps;
-  float omega;
+  float omega, omega2;
   int nx[kMgMaxLevels], ny[kMgMaxLevels], off[kMgMaxLevels];
   const uint8_t* inf_all;  // info of levels lc .. nlev-1, contiguous (offsets off[l] - off[lc])
   const float* r_in;  // right-hand side of level lc
@@ -217,7 +221,7 @@ struct SceneCoarseArgs {
   const uint8_t* lab = nullptr;  // packed labels of level la
   const float* r_in = nullptr;   // packed right-hand side of level la
   float* x_out = nullptr;        // packed cycle result of level la
-  float omega = 0.8f;
+  float omega = 0.8f, omega2 = 0.8f;
 };
 
 // Packed batch: per-scene CG scalars (one CG per scene in lock step).

@@ -602,7 +602,8 @@ struct BlockSpan {
 };
 
 template <bool L0, int NT, bool XR = false, bool TAIL = false>
-__global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omega, int it,
+__global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omega, float omega2,
+                                                 int it,
                                                  MgCheck chk, int check,
                                                  const uint8_t* __restrict__ tflag, MgXr xr,
                                                  unsigned long long* span,
@@ -830,7 +831,7 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
     if (is_act(fv)) {
       const float n = nns_of(fv), x = s_x1[q];
       const float kx = n * x - (s_x1[q - 1] + s_x1[q + 1] + s_x1[q - W3] + s_x1[q + W3]);
-      v = x + omega * (s_r[q] - kx) * inv_nns(fv);
+      v = x + omega2 * (s_r[q] - kx) * inv_nns(fv);
     }
     s_x2[t] = v;
   }
@@ -895,7 +896,7 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
 // there, the first post-sweep on [-1, 33), the second on the tile, written to xo.
 // Level 0 also forms the partial of r.z (z = xo) for the next direction.
 template <bool L0, int NT>
-__global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega,
+__global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega, float omega2,
                                                const CgScalars* sc, double* __restrict__ part_rz,
                                                int it, const uint8_t* __restrict__ tflag) {
   constexpr int W2 = 36, W1 = 34, WC = 20;
@@ -1006,7 +1007,7 @@ __global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega,
     if (is_act(fv)) {
       const float n = nns_of(fv), x = s_y1[q];
       const float kx = n * x - (s_y1[q - 1] + s_y1[q + 1] + s_y1[q - W1] + s_y1[q + W1]);
-      v = x + omega * (s_r[q2] - kx) * inv_nns(fv);
+      v = x + omega2 * (s_r[q2] - kx) * inv_nns(fv);
     }
     f.xo[i + j * f.nx] = v;
     if (L0) rz += (double)s_r[q2] * (double)v;
@@ -1034,6 +1035,7 @@ __device__ __forceinline__ float nbr_sum(const float* a, int c, int i, int j, in
 }
 
 constexpr int kCoarseThreads = 1024;
+constexpr float kCoarsestOmega = 0.8f;  // damped Jacobi on the coarsest level (<= 64 cells)
 
 // The one-workgroup V-cycle over the levels 0 .. nl-1 of a small hierarchy held in
 // LDS: R, X, T and INF, level l at offset off[l] (off[0] = 0).  R's level 0 holds
@@ -1043,8 +1045,9 @@ constexpr int kCoarseThreads = 1024;
 // function sees generic pointers and goes through the slower flat path).
 __device__ __forceinline__ void coarse_cycle(int nl, const int* __restrict__ lnx,
                                              const int* __restrict__ lny,
-                                             const int* __restrict__ loff, float w, int sweeps,
-                                             float* R, float* X, float* T, const uint8_t* INF) {
+                                             const int* __restrict__ loff, float w, float w2,
+                                             int sweeps, float* R, float* X, float* T,
+                                             const uint8_t* INF) {
   for (int l = 0; l < nl - 1; ++l) {
     const int nx = lnx[l], ny = lny[l], n = nx * ny, o = loff[l];
     float *r = R + o, *x = X + o, *t = T + o;
@@ -1071,7 +1074,7 @@ __device__ __forceinline__ void coarse_cycle(int nl, const int* __restrict__ lnx
         for (int k = 0; k < 4; ++k)
           sn += (mk[k] && is_act(fk[k])) ? w * rk[k] * inv_nns(fk[k]) : 0.f;
         const float nn = nns_of(inf[q]), xc = w * r[q] * inv_nns(inf[q]);
-        v = xc + w * (r[q] - (nn * xc - sn)) * inv_nns(inf[q]);
+        v = xc + w2 * (r[q] - (nn * xc - sn)) * inv_nns(inf[q]);
       }
       x[q] = v;
     }
@@ -1165,7 +1168,7 @@ __device__ __forceinline__ void coarse_cycle(int nl, const int* __restrict__ lnx
         if (i < nx - 1) s += xr;
         if (j > 0) s += xd;
         if (j < ny - 1) s += xu;
-        x = is_act(f) ? x + w * (r - (nn * x - s)) * rn : 0.f;
+        x = is_act(f) ? x + kCoarsestOmega * (r - (nn * x - s)) * rn : 0.f;
       }
       if (in) X[o + q] = x;
     }
@@ -1222,7 +1225,7 @@ __device__ __forceinline__ void coarse_cycle(int nl, const int* __restrict__ lnx
         const float nn = nns_of(inf[q]);
         int i, j;
         small_divmod(q, nx, inv_nx, i, j);
-        v = x[q] + w * (r[q] - (nn * x[q] - nbr_sum(x, q, i, j, nx, ny))) * inv_nns(inf[q]);
+        v = x[q] + w2 * (r[q] - (nn * x[q] - nbr_sum(x, q, i, j, nx, ny))) * inv_nns(inf[q]);
       }
       t[q] = v;
     }
@@ -1271,7 +1274,8 @@ __device__ __forceinline__ void coarse_stage(const MgCoarseArgs& a, double* part
       R[q] = coherent ? ld_dev(a.r_in + q) : a.r_in[q];
   }
   __syncthreads();
-  coarse_cycle(nl, s_lv, s_lv + kMgMaxLevels, s_lv + 2 * kMgMaxLevels, a.omega, a.sweeps, R, X,
+  coarse_cycle(nl, s_lv, s_lv + kMgMaxLevels, s_lv + 2 * kMgMaxLevels, a.omega, a.omega2,
+               a.sweeps, R, X,
                T, INF);
   const int n = a.nx[a.lc] * a.ny[a.lc];
   const float* res = nl == 1 ? X : T;
@@ -1384,7 +1388,7 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_scene_coarse(SceneCoarseA
   __syncthreads();
   const int cl = t.nl - 1;
   const int sweeps = 2 * (t.nx[cl] > t.ny[cl] ? t.nx[cl] : t.ny[cl]) + 8;
-  coarse_cycle(t.nl, t.nx, t.ny, t.off, a.omega, sweeps, R, X, T, INF);
+  coarse_cycle(t.nl, t.nx, t.ny, t.off, a.omega, a.omega2, sweeps, R, X, T, INF);
   const float* res = t.nl == 1 ? X : T;
   for (int q = threadIdx.x; q < n0; q += kCoarseThreads)
     a.x_out[(size_t)(t.cx + q % nx) + (size_t)(t.cy + q / nx) * a.pitch] = res[q];
@@ -2185,10 +2189,10 @@ void launch_mg_down(hipStream_t s, MgHost& m, int l, int it, int max_iter, doubl
     const MgXr xr{m.x[(it + 1) & 1], m.x[it & 1], m.p[it & 1], m.b, m.part_pq,
                   mg_dir_blocks(m), m.part_rr};
     k_mg_down<true, kT, true><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1),
-                                                    m.coarse.omega, it, chk, 0, m.tflag, xr,
+                                                    m.coarse.omega, m.coarse.omega2, it, chk, 0, m.tflag, xr,
                                                     span, MgCoarseArgs{}, nullptr);
   } else if (l == 0) {
-    k_mg_down<true, kT><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega, it,
+    k_mg_down<true, kT><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega, m.coarse.omega2, it,
                                               chk, check, m.tflag, MgXr{}, span,
                                               MgCoarseArgs{}, nullptr);
   } else {
@@ -2196,11 +2200,11 @@ void launch_mg_down(hipStream_t s, MgHost& m, int l, int it, int max_iter, doubl
     // instance, so that the other levels (and every level of a batch) carry none of it
     if (l == m.lc - 1 && mg_tail_coarse(m))
       k_mg_down<false, kTileBig, false, true><<<blocks, kTileBig, m.coarse_lds, s>>>(
-          mg_level(m, l), mg_level(m, l + 1), m.coarse.omega, it, chk, check, m.tflag_l[l],
+          mg_level(m, l), mg_level(m, l + 1), m.coarse.omega, m.coarse.omega2, it, chk, check, m.tflag_l[l],
           MgXr{}, nullptr, m.coarse, m.ticket);
     else
       k_mg_down<false, kTileBig><<<blocks, kTileBig, 0, s>>>(
-          mg_level(m, l), mg_level(m, l + 1), m.coarse.omega, it, chk, check, m.tflag_l[l],
+          mg_level(m, l), mg_level(m, l + 1), m.coarse.omega, m.coarse.omega2, it, chk, check, m.tflag_l[l],
           MgXr{}, nullptr, MgCoarseArgs{}, nullptr);
   }
 }
@@ -2220,11 +2224,11 @@ void launch_mg_coarse(hipStream_t s, const MgHost& m, int it, int max_iter, doub
 void launch_mg_up(hipStream_t s, const MgHost& m, int l, int it, CgScalars* sc) {
   const int blocks = mg_up_blocks(m.nx[l], m.ny[l]);
   if (l == 0)
-    k_mg_up<true, kT><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega, sc,
+    k_mg_up<true, kT><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega, m.coarse.omega2, sc,
                                             m.part_rz, it, m.tflag);
   else
     k_mg_up<false, kTileBig><<<blocks, kTileBig, 0, s>>>(mg_level(m, l), mg_level(m, l + 1),
-                                                         m.coarse.omega, sc, nullptr, it,
+                                                         m.coarse.omega, m.coarse.omega2, sc, nullptr, it,
                                                          m.tflag_l[l]);
 }
 

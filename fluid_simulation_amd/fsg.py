@@ -82,7 +82,7 @@ class Params(C.Structure):
         ("record_ext_info", C.c_int32), ("pcg_blocks", C.c_int32), ("pcg_kpt", C.c_int32),
         ("warm_start", C.c_int32), ("particle_pressure_k", C.c_float),
         ("closed_walls", C.c_int32), ("graph", C.c_int32), ("deterministic", C.c_int32),
-        ("record_decisions", C.c_int32),
+        ("record_decisions", C.c_int32), ("mg_smoother", C.c_int32),
     ]
 
 

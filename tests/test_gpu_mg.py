@@ -38,7 +38,7 @@ def irregular_labels(nx, ny, seed, liquid_frac=0.7, n_obst=3):
     return lab
 
 
-def _solve_pair(lab, seed, tol=1e-10, project_mean=0, length=None):
+def _solve_pair(lab, seed, tol=1e-10, project_mean=0, length=None, smoother=1):
     from oracle import OracleSim
     ny, nx = lab.shape
     lx, ly = (float(nx), float(ny)) if length is None else length
@@ -52,7 +52,8 @@ def _solve_pair(lab, seed, tol=1e-10, project_mean=0, length=None):
     d.set_vel(np.zeros_like(u), np.zeros_like(v), "back")
     d.run_phase("PRESSURE", 0.01)
     g = fsg.Domain(nx, ny, lx, ly, 0.01, 0.05)
-    g.set_params(solver=fsg.SOLVER_MG, cg_tol=tol, cg_max_iter=500, project_mean=project_mean)
+    g.set_params(solver=fsg.SOLVER_MG, cg_tol=tol, cg_max_iter=500, project_mean=project_mean,
+                 mg_smoother=smoother)
     g.set_labels(lab)
     g.set_velocity(u, v, fsg.VEL_FRONT)
     g.set_velocity(np.zeros_like(u), np.zeros_like(v), fsg.VEL_BACK)
@@ -61,11 +62,12 @@ def _solve_pair(lab, seed, tol=1e-10, project_mean=0, length=None):
     return d, g
 
 
+@pytest.mark.parametrize("smoother", [1, 0], ids=["chebyshev", "jacobi"])
 @pytest.mark.parametrize("nx,ny,seed", [(64, 64, 0), (77, 133, 1), (200, 50, 2), (7, 5, 3),
                                         (513, 257, 4), (1030, 20, 5)])
-def test_mg_irregular_against_fp64(nx, ny, seed):
+def test_mg_irregular_against_fp64(nx, ny, seed, smoother):
     lab = irregular_labels(nx, ny, seed)
-    d, g = _solve_pair(lab, seed)
+    d, g = _solve_pair(lab, seed, smoother=smoother)
     st = g.stats()
     liq = lab == LIQ
     ep = rel_err(g.pressure()[liq], d.pressure[liq])
