@@ -11,6 +11,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
+#include <initializer_list>
 #include <map>
 #include <new>
 #include <string>
@@ -139,13 +141,14 @@ struct AdjState {
   int last_iters = 0;
 };
 
-// The multigrid solve's iterations captured as HIP graphs, kIters per graph:
-// graph c runs iterations [c kIters, (c + 1) kIters) (the last one stops at the
-// iteration cap).  Their launch arguments depend only on the iteration index and on
-// the hierarchy's buffers, fixed once the hierarchy is built, and on the tolerance
-// and the cap (fsg_set_params drops the graphs when those change).
+// The multigrid solve's iterations captured as HIP graphs, `iters` per graph
+// (fsg_params.graph_iters): graph c runs iterations [c iters, (c + 1) iters) (the last
+// one stops at the iteration cap).  Their launch arguments depend only on the
+// iteration index and on the hierarchy's buffers, fixed once the hierarchy is built,
+// and on the tolerance, the cap and the smoothing settings (fsg_set_params drops the
+// graphs when those change).
 struct MgGraphs {
-  static constexpr int kIters = 2;
+  int iters = 2;
   std::vector<hipGraphExec_t> exec;
   void clear() {
     for (hipGraphExec_t e : exec)
@@ -491,6 +494,7 @@ fsg::SceneCoarseArgs scene_coarse_args(const fsg_domain* h) {
   a.x_out = m.xo[b.la];
   a.omega = m.coarse.omega;
   a.omega2 = m.coarse.omega2;
+  a.trunc = m.coarse.trunc;
   return a;
 }
 
@@ -521,6 +525,8 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
   } else {
     m.coarse.omega = m.coarse.omega2 = 0.8f;
   }
+  m.coarse.trunc = h->prm.mg_coarse_sweeps;
+  m.fused = !h->batch && m.lc >= 2 && h->prm.mg_fuse_update != 0;
   m.clk_on = h->kclock == 1;
   m.clk_span_on = h->kclock == 2;
   if (m.clk_span_on && !m.clk_span) alloc_clk_span(h);
@@ -580,7 +586,11 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
   // Graph replay: graph c holds iterations [c K, (c + 1) K) (up to the cap), captured
   // on its first use.  Per-kernel timing and the kernel clocks need every launch.
   const bool use_graph = h->prm.graph && !h->timer.on && !m.clk_on && !m.clk_span_on;
-  constexpr int K = MgGraphs::kIters;
+  if (h->mg_graphs.iters != std::max(1, h->prm.graph_iters)) {
+    h->mg_graphs.clear();
+    h->mg_graphs.iters = std::max(1, h->prm.graph_iters);
+  }
+  const int K = h->mg_graphs.iters;
   auto graph_of = [&](int c, hipGraphExec_t* out) -> int {
     std::vector<hipGraphExec_t>& ex = h->mg_graphs.exec;
     if ((int)ex.size() <= c) ex.resize(c + 1, nullptr);
@@ -879,6 +889,9 @@ void fsg_params_default(fsg_params* p) {
   p->record_ext_info = 0;
   p->graph = 1;
   p->mg_smoother = 1;
+  p->mg_coarse_sweeps = 0;
+  p->graph_iters = 2;
+  p->mg_fuse_update = 1;
 }
 
 int fsg_create(fsg_domain** out, int32_t nx, int32_t ny, float length_x, float length_y,
@@ -981,7 +994,9 @@ int fsg_set_params(fsg_domain* h, const fsg_params* p) {
       !std::isfinite(p->particle_pressure_k) || p->closed_walls < 0 || p->closed_walls > 1 ||
       p->graph < 0 || p->graph > 1 || p->deterministic < 0 || p->deterministic > 1 ||
       p->record_decisions < 0 || p->record_decisions > 1 || p->mg_smoother < 0 ||
-      p->mg_smoother > 1)
+      p->mg_smoother > 1 || p->mg_coarse_sweeps < 0 || p->mg_coarse_sweeps > 64 ||
+      p->graph_iters < 1 || p->graph_iters > 16 || p->mg_fuse_update < 0 ||
+      p->mg_fuse_update > 1)
     return fail(h, FSG_E_INVALID, "fsg_set_params: value out of range");
   const size_t C = ncell(h);
   // fixed-point sums: 4 per cell (the P2G accumulators; the density uses the first C),
@@ -1005,7 +1020,8 @@ int fsg_set_params(fsg_domain* h, const fsg_params* p) {
     return fail(h, FSG_E_INVALID, "persistent solver unavailable on this device");
   // the captured iterations hold the tolerance and the iteration cap
   if (p->cg_tol != h->prm.cg_tol || p->cg_max_iter != h->prm.cg_max_iter ||
-      p->mg_smoother != h->prm.mg_smoother)
+      p->mg_smoother != h->prm.mg_smoother || p->mg_coarse_sweeps != h->prm.mg_coarse_sweeps ||
+      p->mg_fuse_update != h->prm.mg_fuse_update)
     h->mg_graphs.clear();
   h->prm = *p;
   return FSG_OK;
@@ -1855,6 +1871,142 @@ int fsg_checkpoint_clear(fsg_domain* h) {
     if (c.pt) (void)hipFree(c.pt);
     c.pt = nullptr;
     c.np = c.cap = 0;
+  }
+  return FSG_OK;
+}
+
+// ---------------------------------------------------------------- autotuner
+// The multigrid solve's settings (smoother, coarse solve, fused update, iterations per
+// graph) chosen by timing whole steps from one saved state: every candidate runs one
+// untimed step (graph capture), then `steps` timed steps, each from the saved state,
+// which is restored at the end (particles, the four velocity grids, labels, reject
+// flags, statistics), so the handle continues as if the call had not been made,
+// with the fastest candidate's settings.
+struct TuneState {
+  float4* pt = nullptr;
+  float* g[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  uint8_t* label = nullptr;
+  uint8_t* reject = nullptr;
+  int64_t np = 0;
+  std::vector<void*> mem;
+  ~TuneState() {
+    for (void* p : mem) (void)hipFree(p);
+  }
+};
+
+static int tune_save(fsg_domain* h, TuneState& t) {
+  const size_t C = ncell(h);
+  t.np = h->np;
+  auto get = [&](size_t bytes, void** out) -> hipError_t {
+    const hipError_t e = hipMalloc(out, bytes ? bytes : 16);
+    if (e == hipSuccess) t.mem.push_back(*out);
+    return e;
+  };
+  HIPCHK(h, get((size_t)h->np * sizeof(float4), reinterpret_cast<void**>(&t.pt)));
+  for (float*& q : t.g) HIPCHK(h, get(C * 4, reinterpret_cast<void**>(&q)));
+  HIPCHK(h, get(C, reinterpret_cast<void**>(&t.label)));
+  HIPCHK(h, get((size_t)h->np, reinterpret_cast<void**>(&t.reject)));
+  const float* src[6] = {UF(h), VF(h), UB(h), VB(h), h->u_prev, h->v_prev};
+  if (h->np > 0) {
+    HIPCHK(h, hipMemcpyAsync(t.pt, h->pt, (size_t)h->np * sizeof(float4),
+                             hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(t.reject, h->reject, (size_t)h->np, hipMemcpyDeviceToDevice,
+                             h->stream));
+  }
+  for (int k = 0; k < 6; ++k)
+    HIPCHK(h, hipMemcpyAsync(t.g[k], src[k], C * 4, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(t.label, h->label, C, hipMemcpyDeviceToDevice, h->stream));
+  return FSG_OK;
+}
+
+static int tune_load(fsg_domain* h, const TuneState& t, bool with_diff) {
+  const size_t C = ncell(h);
+  if (t.np > 0) {
+    HIPCHK(h, hipMemcpyAsync(h->pt, t.pt, (size_t)t.np * sizeof(float4),
+                             hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->reject, t.reject, (size_t)t.np, hipMemcpyDeviceToDevice,
+                             h->stream));
+  }
+  float* dst[6] = {UF(h), VF(h), UB(h), VB(h), h->u_prev, h->v_prev};
+  for (int k = 0; k < (with_diff ? 6 : 4); ++k)
+    HIPCHK(h, hipMemcpyAsync(dst[k], t.g[k], C * 4, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->label, t.label, C, hipMemcpyDeviceToDevice, h->stream));
+  return FSG_OK;
+}
+
+int fsg_autotune(fsg_domain* h, int32_t scheme, float dt, int32_t steps, fsg_tune* out) {
+  if (!h || steps < 1 || steps > 64) return FSG_E_INVALID;
+  if (h->prm.solver != FSG_SOLVER_MG)
+    return fail(h, FSG_E_INVALID, "fsg_autotune: tunes the multigrid solver (FSG_SOLVER_MG)");
+  if (h->np == 0 && scheme != FSG_SCHEME_GRID_ADVECT)
+    return fail(h, FSG_E_INVALID, "fsg_autotune: no particles to step");
+  struct Cand {
+    int smoother, coarse, fuse, iters;
+  };
+  std::vector<Cand> cands;
+  const fsg_params p0 = h->prm;
+  const bool can_fuse = !h->batch;
+  for (int sm : {1, 0})
+    for (int co : {0, 8, 16})
+      for (int fu : can_fuse ? std::initializer_list<int>{1, 0} : std::initializer_list<int>{1})
+        for (int gi : {2, 4}) cands.push_back({sm, co, fu, gi});
+  TuneState saved;
+  if (int rc = tune_save(h, saved)) return rc;
+  const fsg_stats stats0 = h->host_stats;
+  const int last0 = h->mg_last_iters;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  double best = 1e300, cur = -1.0;
+  int best_k = 0, best_iters = 0;
+  for (size_t k = 0; k < cands.size(); ++k) {
+    const Cand& c = cands[k];
+    fsg_params q = p0;
+    q.mg_smoother = c.smoother;
+    q.mg_coarse_sweeps = c.coarse;
+    q.mg_fuse_update = c.fuse;
+    q.graph_iters = c.iters;
+    if (int rc = fsg_set_params(h, &q)) return rc;
+    double t_sum = 0.0;
+    int its = 0;
+    for (int s = -1; s < steps; ++s) {  // s = -1: the untimed step (graph capture)
+      if (int rc = tune_load(h, saved, false)) return rc;
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      const auto t0 = std::chrono::steady_clock::now();
+      if (int rc = fsg_step(h, scheme, dt)) return rc;
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      const auto t1 = std::chrono::steady_clock::now();
+      if (s >= 0) t_sum += std::chrono::duration<double, std::milli>(t1 - t0).count();
+      its = h->host_stats.cg_iterations;
+    }
+    const double ms = t_sum / steps;
+    if (c.smoother == p0.mg_smoother && c.coarse == p0.mg_coarse_sweeps &&
+        c.fuse == p0.mg_fuse_update && c.iters == p0.graph_iters)
+      cur = ms;
+    if (ms < best) {
+      best = ms;
+      best_k = (int)k;
+      best_iters = its;
+    }
+  }
+  fsg_params q = p0;
+  q.mg_smoother = cands[best_k].smoother;
+  q.mg_coarse_sweeps = cands[best_k].coarse;
+  q.mg_fuse_update = cands[best_k].fuse;
+  q.graph_iters = cands[best_k].iters;
+  if (int rc = fsg_set_params(h, &q)) return rc;
+  if (int rc = tune_load(h, saved, true)) return rc;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->host_stats = stats0;
+  h->mg_last_iters = last0;
+  if (out) {
+    memset(out, 0, sizeof(*out));
+    out->mg_smoother = q.mg_smoother;
+    out->mg_coarse_sweeps = q.mg_coarse_sweeps;
+    out->mg_fuse_update = q.mg_fuse_update;
+    out->graph_iters = q.graph_iters;
+    out->candidates = (int32_t)cands.size();
+    out->cg_iterations = best_iters;
+    out->ms_per_step = best;
+    out->ms_per_step_before = cur;
   }
   return FSG_OK;
 }

@@ -198,6 +198,7 @@ constexpr int kMgCoarseCells = 1024;   // levels at or below this size run in on
 struct MgCoarseArgs {
   int lc, nlev, total, sweeps;
   float omega, omega2;
+  int trunc;  // 0: V-cycle over levels lc .. nlev-1; k > 0: k Jacobi sweeps on level lc only
   int nx[kMgMaxLevels], ny[kMgMaxLevels], off[kMgMaxLevels];
   const uint8_t* inf_all;  // info of levels lc .. nlev-1, contiguous (offsets off[l] - off[lc])
   const float* r_in;  // right-hand side of level lc
@@ -222,6 +223,7 @@ struct SceneCoarseArgs {
   const float* r_in = nullptr;   // packed right-hand side of level la
   float* x_out = nullptr;        // packed cycle result of level la
   float omega = 0.8f, omega2 = 0.8f;
+  int trunc = 0;  // as MgCoarseArgs::trunc, on each scene's first level
 };
 
 // Packed batch: per-scene CG scalars (one CG per scene in lock step).

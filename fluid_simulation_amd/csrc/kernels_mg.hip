@@ -1047,7 +1047,35 @@ __device__ __forceinline__ void coarse_cycle(int nl, const int* __restrict__ lnx
                                              const int* __restrict__ lny,
                                              const int* __restrict__ loff, float w, float w2,
                                              int sweeps, float* R, float* X, float* T,
-                                             const uint8_t* INF) {
+                                             const uint8_t* INF, int trunc) {
+  if (trunc > 0) {
+    // truncated: `trunc` damped-Jacobi sweeps from zero on the first level only (a
+    // polynomial in D^-1 K, so the preconditioner stays symmetric positive definite),
+    // alternating between X and T so that the last one lands in T
+    const int nx = lnx[0], ny = lny[0], n = nx * ny;
+    const float inv_nx = 1.0f / (float)nx;
+    for (int sw = 0; sw < trunc; ++sw) {
+      const float* src = ((trunc - sw) & 1) ? X : T;
+      float* dst = ((trunc - sw) & 1) ? T : X;
+      for (int q = threadIdx.x; q < n; q += kCoarseThreads) {
+        float v = 0.f;
+        if (is_act(INF[q])) {
+          const float rn = inv_nns(INF[q]);
+          if (sw == 0) {
+            v = kCoarsestOmega * R[q] * rn;
+          } else {
+            int i, j;
+            small_divmod(q, nx, inv_nx, i, j);
+            const float x = src[q];
+            v = x + kCoarsestOmega * (R[q] - (nns_of(INF[q]) * x - nbr_sum(src, q, i, j, nx, ny))) * rn;
+          }
+        }
+        dst[q] = v;
+      }
+      __syncthreads();
+    }
+    return;
+  }
   for (int l = 0; l < nl - 1; ++l) {
     const int nx = lnx[l], ny = lny[l], n = nx * ny, o = loff[l];
     float *r = R + o, *x = X + o, *t = T + o;
@@ -1275,10 +1303,9 @@ __device__ __forceinline__ void coarse_stage(const MgCoarseArgs& a, double* part
   }
   __syncthreads();
   coarse_cycle(nl, s_lv, s_lv + kMgMaxLevels, s_lv + 2 * kMgMaxLevels, a.omega, a.omega2,
-               a.sweeps, R, X,
-               T, INF);
+               a.sweeps, R, X, T, INF, a.trunc);
   const int n = a.nx[a.lc] * a.ny[a.lc];
-  const float* res = nl == 1 ? X : T;
+  const float* res = nl == 1 && a.trunc == 0 ? X : T;
   double rz = 0.0;
   for (int q = threadIdx.x; q < n; q += kCoarseThreads) {
     a.x_out[q] = res[q];
@@ -1388,8 +1415,8 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_scene_coarse(SceneCoarseA
   __syncthreads();
   const int cl = t.nl - 1;
   const int sweeps = 2 * (t.nx[cl] > t.ny[cl] ? t.nx[cl] : t.ny[cl]) + 8;
-  coarse_cycle(t.nl, t.nx, t.ny, t.off, a.omega, a.omega2, sweeps, R, X, T, INF);
-  const float* res = t.nl == 1 ? X : T;
+  coarse_cycle(t.nl, t.nx, t.ny, t.off, a.omega, a.omega2, sweeps, R, X, T, INF, a.trunc);
+  const float* res = t.nl == 1 && a.trunc == 0 ? X : T;
   for (int q = threadIdx.x; q < n0; q += kCoarseThreads)
     a.x_out[(size_t)(t.cx + q % nx) + (size_t)(t.cy + q / nx) * a.pitch] = res[q];
 }

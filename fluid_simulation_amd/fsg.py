@@ -65,7 +65,7 @@ EXPORTS = [
     "fsg_checkpoint_clear", "fsg_adjoint_set", "fsg_adjoint_get", "fsg_adjoint_step",
     "fsg_adjoint_last_iterations", "fsg_particle_density", "fsg_adjoint_seed_density",
     "fsg_set_kernel_clock", "fsg_get_kernel_clock", "fsg_get_step_decisions",
-    "fsg_p2g_noalloc",
+    "fsg_p2g_noalloc", "fsg_autotune",
 ]
 
 
@@ -83,6 +83,18 @@ class Params(C.Structure):
         ("warm_start", C.c_int32), ("particle_pressure_k", C.c_float),
         ("closed_walls", C.c_int32), ("graph", C.c_int32), ("deterministic", C.c_int32),
         ("record_decisions", C.c_int32), ("mg_smoother", C.c_int32),
+        ("mg_coarse_sweeps", C.c_int32), ("graph_iters", C.c_int32),
+        ("mg_fuse_update", C.c_int32),
+    ]
+
+
+class Tune(C.Structure):
+    _fields_ = [
+        ("mg_smoother", C.c_int32), ("mg_coarse_sweeps", C.c_int32),
+        ("mg_fuse_update", C.c_int32), ("graph_iters", C.c_int32),
+        ("candidates", C.c_int32), ("cg_iterations", C.c_int32),
+        ("ms_per_step", C.c_double), ("ms_per_step_before", C.c_double),
+        ("reserved", C.c_int64 * 4),
     ]
 
 
@@ -174,6 +186,7 @@ def load(path: str | None = None):
             "fsg_set_kernel_clock": ([H, C.c_int32], C.c_int),
             "fsg_get_kernel_clock": ([H, P(C.c_double), P(C.c_int64)], C.c_int),
             "fsg_get_step_decisions": ([H, C.c_void_p], C.c_int),
+            "fsg_autotune": ([H, C.c_int32, C.c_float, C.c_int32, P(Tune)], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -365,6 +378,13 @@ class Domain:
         a = np.zeros(n, np.uint8)
         self._chk(self._L.fsg_get_reject_flags(self._h, a.ctypes.data), "fsg_get_reject_flags")
         return a
+
+    def autotune(self, scheme: int, dt: float, steps: int = 3) -> dict:
+        """fsg_autotune: time every candidate multigrid configuration over ``steps``
+        steps from the current state, keep the fastest, restore the state."""
+        t = Tune()
+        self._chk(self._L.fsg_autotune(self._h, scheme, dt, steps, C.byref(t)), "fsg_autotune")
+        return {k: getattr(t, k) for k, _ in Tune._fields_ if k != "reserved"}
 
     def step_decisions(self):
         """The last step's discrete decisions per cell (fsg_params.record_decisions):

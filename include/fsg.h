@@ -144,6 +144,17 @@ typedef struct fsg_params {
                              polynomial on [0.4, 2] (the high end of the spectrum of
                              D^-1 A): 11.9 instead of 13.4 iterations per solve over the
                              1024^2 dam break's 500 steps */
+  int32_t mg_coarse_sweeps; /* multigrid solver's coarse solve (the levels of <= 1024
+                               cells, one workgroup): 0 (default) a V-cycle down to 4 x 4
+                               cells; k in 1..64: k damped-Jacobi sweeps on the first of
+                               those levels only (cheaper; as good where AIR cells pin
+                               the coarse problem, much weaker for all-Neumann boxes) */
+  int32_t graph_iters;    /* multigrid solver with graph = 1: iterations per captured
+                             graph, 1..16 (default 2) */
+  int32_t mg_fuse_update; /* multigrid solver, one domain of two or more tile levels:
+                             1 (default) the CG update of x and the residual runs inside
+                             the next iteration's level-0 down kernel; 0 in a kernel of
+                             its own */
 } fsg_params;
 
 /* Counters of the last pressure solve and running totals. */
@@ -320,6 +331,25 @@ int fsg_set_timing_prefix(fsg_domain *h, const char *prefix);
 int fsg_get_timing(fsg_domain *h, int32_t cap, const char **names, double *ms,
                    int64_t *launches);
 int fsg_reset_timing(fsg_domain *h);
+
+/* ---- autotuner (no counterpart in the reference) -----------------------------
+ * Chooses the multigrid solver's settings for the handle's workload by timing whole
+ * steps (scheme, dt) from its current state: every combination of mg_smoother
+ * {1, 0}, mg_coarse_sweeps {0, 8, 16}, mg_fuse_update {1, 0} (one domain) and
+ * graph_iters {2, 4}; each runs one untimed step, then `steps` (1..64) timed steps,
+ * each from the saved state.  The fastest is set on the handle, and the state
+ * (particles, velocity grids, labels, reject flags, statistics) is restored, so the
+ * next fsg_step continues the trajectory the call found.  Needs FSG_SOLVER_MG;
+ * allocates a copy of the state for the duration of the call. */
+typedef struct fsg_tune {
+  int32_t mg_smoother, mg_coarse_sweeps, mg_fuse_update, graph_iters; /* chosen */
+  int32_t candidates;       /* configurations timed */
+  int32_t cg_iterations;    /* the chosen configuration's iterations (last timed step) */
+  double ms_per_step;       /* the chosen configuration's mean step time */
+  double ms_per_step_before;/* the handle's configuration before the call (if a candidate) */
+  int64_t reserved[4];
+} fsg_tune;
+int fsg_autotune(fsg_domain *h, int32_t scheme, float dt, int32_t steps, fsg_tune *out);
 
 /* ---- an entry point that allocates nothing ----------------------------------
  * No counterpart in the reference: for callers that cannot allocate device memory

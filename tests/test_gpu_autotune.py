@@ -1,0 +1,60 @@
+"""fsg_autotune: it times the multigrid solver's candidate settings over whole steps from
+the handle's state, keeps the fastest and restores the state, so the next step equals
+the step of a handle that was given the chosen settings directly (bit for bit, in the
+deterministic form), for one domain and for a packed batch of scenes."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from fluid_simulation_amd import fsg, scenes
+
+pytestmark = pytest.mark.gpu
+
+TUNED = ("mg_smoother", "mg_coarse_sweeps", "mg_fuse_update", "graph_iters")
+
+
+def _state(d):
+    return (d.particles(), *d.velocity(fsg.VEL_FRONT), *d.velocity(fsg.VEL_BACK), d.labels())
+
+
+def test_autotune_restores_the_state_and_keeps_the_fastest():
+    sc = scenes.dam_break(192)
+    kw = dict(solver=fsg.SOLVER_MG, cg_tol=1e-6, cg_max_iter=200, deterministic=1)
+    a = fsg.Domain.from_scene(sc, **kw)
+    b = fsg.Domain.from_scene(sc, **kw)
+    for d in (a, b):
+        for _ in range(30):
+            d.step(fsg.PICFLIP, sc.dt)
+    before = _state(a)
+    it_before = a.stats().cg_iterations
+    t = a.autotune(fsg.PICFLIP, sc.dt, steps=2)
+    print(f"autotune: {t}", flush=True)
+    assert t["candidates"] == 24 and t["ms_per_step"] > 0
+    assert t["ms_per_step"] <= t["ms_per_step_before"]
+    after = _state(a)
+    assert all(np.array_equal(x, y) for x, y in zip(before, after))
+    assert a.stats().cg_iterations == it_before
+    b.set_params(**{k: t[k] for k in TUNED})
+    for d in (a, b):
+        d.step(fsg.PICFLIP, sc.dt)
+    for x, y in zip(_state(a), _state(b)):
+        assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
+    a.close()
+    b.close()
+
+
+def test_autotune_batch():
+    scs = scenes.dam_break_batch(12, 48, 96)
+    bt = fsg.Batch.from_scenes(scs, cg_tol=1e-6, cg_max_iter=200, deterministic=1)
+    for _ in range(3):
+        bt.step(fsg.PICFLIP, 0.01)
+    lab = bt.labels()
+    t = bt.autotune(fsg.PICFLIP, 0.01, steps=1)
+    print(f"batch autotune: {t}", flush=True)
+    assert t["candidates"] == 12 and t["mg_fuse_update"] == 1
+    assert np.array_equal(bt.labels(), lab)
+    bt.step(fsg.PICFLIP, 0.01)
+    st = bt.stats()
+    assert st.cg_error <= 1e-6
+    bt.close()

@@ -38,7 +38,7 @@ def irregular_labels(nx, ny, seed, liquid_frac=0.7, n_obst=3):
     return lab
 
 
-def _solve_pair(lab, seed, tol=1e-10, project_mean=0, length=None, smoother=1):
+def _solve_pair(lab, seed, tol=1e-10, project_mean=0, length=None, smoother=1, **gpu_params):
     from oracle import OracleSim
     ny, nx = lab.shape
     lx, ly = (float(nx), float(ny)) if length is None else length
@@ -53,7 +53,7 @@ def _solve_pair(lab, seed, tol=1e-10, project_mean=0, length=None, smoother=1):
     d.run_phase("PRESSURE", 0.01)
     g = fsg.Domain(nx, ny, lx, ly, 0.01, 0.05)
     g.set_params(solver=fsg.SOLVER_MG, cg_tol=tol, cg_max_iter=500, project_mean=project_mean,
-                 mg_smoother=smoother)
+                 mg_smoother=smoother, **gpu_params)
     g.set_labels(lab)
     g.set_velocity(u, v, fsg.VEL_FRONT)
     g.set_velocity(np.zeros_like(u), np.zeros_like(v), fsg.VEL_BACK)
@@ -80,6 +80,23 @@ def test_mg_irregular_against_fp64(nx, ny, seed, smoother):
     assert st.cg_error <= 1e-10
     assert st.cg_iterations <= 60
     assert ep <= 1e-6 and eu <= 1e-6 and ev <= 1e-6
+
+
+@pytest.mark.parametrize("sweeps,fuse", [(8, 1), (16, 0), (0, 0)])
+def test_mg_coarse_and_fusion_options_against_fp64(sweeps, fuse):
+    """The truncated coarse solve (mg_coarse_sweeps: Jacobi sweeps on the first
+    one-workgroup level instead of its V-cycle) and the unfused update kernel
+    (mg_fuse_update = 0) change the preconditioner and the launch sequence, not the
+    system: the same FP64 residual bound and the same solution."""
+    lab = irregular_labels(513, 257, 4)
+    d, g = _solve_pair(lab, 4, mg_coarse_sweeps=sweeps, mg_fuse_update=fuse)
+    st = g.stats()
+    liq = lab == LIQ
+    ep = rel_err(g.pressure()[liq], d.pressure[liq])
+    print(f"coarse sweeps {sweeps} fuse {fuse}: {st.cg_iterations} iterations, "
+          f"|r|/|b| {st.cg_error:.2e}, p {ep:.2e}")
+    assert st.cg_error <= 1e-10
+    assert ep <= 1e-6
 
 
 def test_mg_closed_box_project_mean():
