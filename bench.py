@@ -148,6 +148,9 @@ MEASURED_TRAFFIC_1024 = {
 }
 
 
+# a kernel moving less than this per launch is bound by launch and barrier latency
+ROOFLINE_MIN_BYTES = 1e6
+
 # timers that wrap more than one kernel (or one kernel at several levels)
 MULTI_KERNEL_TIMERS = {"cg_mg_setup", "cg_mg_finish", "cg_mg_down7+", "cg_mg_up7+", "bucket_order"}
 
@@ -160,6 +163,8 @@ def log(*a):
 # smoother (--mg-smoother: 0 damped Jacobi, 1 two-step Chebyshev)
 GRAPH = 1
 SMOOTHER = 1
+# the multigrid settings fsg_autotune chooses
+TUNED = ("mg_smoother", "mg_coarse_sweeps", "mg_fuse_update", "graph_iters")
 
 
 def make_domain(fsg, scene, solver, tol, max_iter, warm=0):
@@ -265,11 +270,14 @@ def timed_segment(fsg, d, steps, dt, kernel=None, scheme=None):
     return t1 - t0, st, tm
 
 
-def full_run(fsg, scene, solver, tol, steps):
+def full_run(fsg, scene, solver, tol, steps, tuned=None):
     """The configuration's whole run (`steps` steps from the initial state) on a
-    domain of its own: wall time, and every solve's iterations and FP64 relative
-    residual (host-side counters read after each step, no extra synchronisation)."""
+    domain of its own, with the autotuner's settings if given: wall time, and every
+    solve's iterations and FP64 relative residual (host-side counters read after each
+    step, no extra synchronisation)."""
     d = make_domain(fsg, scene, solver, tol, 200 if solver == fsg.SOLVER_MG else 20000)
+    if tuned:
+        d.set_params(**tuned)
     its, errs = [], []
     d.synchronize()
     t0 = time.perf_counter()
@@ -349,6 +357,12 @@ def batch_leg(fsg, scenes, args):
                               mg_smoother=SMOOTHER)
     log(f"batch: {len(scs)} scenes, {cells} cells, {parts} particles, packed {b.nx}x{b.ny}")
     run_steps(fsg, b, args.batch_warmup, scs[0].dt)
+    tune = None
+    if args.autotune:
+        t0 = time.perf_counter()
+        tune = b.autotune(fsg.PICFLIP, scs[0].dt, steps=1)
+        tune["seconds"] = time.perf_counter() - t0
+        log(f"batch autotune: {tune}")
     prof = kernel_profile(fsg, b, 2, scs[0].dt)
     per_step = {k: ms * n for k, (ms, n) in prof.items()}
     elapsed, st, _ = timed_segment(fsg, b, args.batch_steps, scs[0].dt)
@@ -411,6 +425,7 @@ def batch_leg(fsg, scenes, args):
                     f"{args.batch_hi}, 1-4 obstacles each, {cells} cells, {parts} particles, "
                     f"stepPICFLIP dt 0.01, 0.95 FLIP, per-scene rel tol {args.tol:g}",
         "steps": args.batch_steps,
+        "autotune": tune,
         "steps_per_s": sps,
         "ms_per_step": 1e3 / sps,
         "scene_steps_per_s": sps * len(scs),
@@ -435,6 +450,9 @@ def grid_leg(fsg, scenes, args):
     d = fsg.Domain.from_scene(sc, cg_tol=args.tol, cg_max_iter=200, solver=fsg.SOLVER_MG, graph=GRAPH,
                               integrator=2, project_mean=1, closed_walls=1, mg_smoother=SMOOTHER)
     run_steps(fsg, d, 2, sc.dt, fsg.GRID_ADVECT)
+    tune = d.autotune(fsg.GRID_ADVECT, sc.dt, steps=2) if args.autotune else None
+    if tune:
+        log(f"grid-only autotune: {tune}")
     prof = kernel_profile(fsg, d, 2, sc.dt, fsg.GRID_ADVECT)
     per_step = {k: ms * n for k, (ms, n) in prof.items()}
     elapsed, st, _ = timed_segment(fsg, d, args.grid_steps, sc.dt, scheme=fsg.GRID_ADVECT)
@@ -446,6 +464,7 @@ def grid_leg(fsg, scenes, args):
                     f"projection, fully fluid {args.grid_n}^2, closed walls, "
                     f"dt {sc.dt:.4g}, rel tol {args.tol:g}",
         "steps": args.grid_steps,
+        "autotune": tune,
         "steps_per_s": sps,
         "ms_per_step": 1e3 / sps,
         "cg_iterations_last_solve": st.cg_iterations,
@@ -454,7 +473,7 @@ def grid_leg(fsg, scenes, args):
     }
 
 
-def replay_gradient_check(fsg, scenes, n, steps, every, n_dirs, eps, scale, tol, workers):
+def replay_gradient_check(fsg, scenes, n, steps, every, n_dirs, eps_list, scale, tol, workers):
     """The differentiable configuration at n^2 (closed all-fluid box, (2, 16, 16)
     controls x scale, ``steps`` PICFLIP steps, checkpoints every ``every``): the
     adjoint gradient (deterministic rollout) along ``n_dirs`` seeded unit directions
@@ -507,33 +526,38 @@ def replay_gradient_check(fsg, scenes, n, steps, every, n_dirs, eps, scale, tol,
     for _ in range(n_dirs):
         dv = rng.standard_normal((2, 16, 16))
         dirs.append(dv / np.linalg.norm(dv))
-    states = [state(base)]
-    for dv in dirs:
-        states += [state(base + eps * dv), state(base - eps * dv)]
-    ls = rr.density_losses(states, 3, dt, target, workers=workers)
+    l0 = rr.density_losses([state(base)], 3, dt, target, workers=1)[0]
     gnorm = float(np.linalg.norm(grad))
-    rows = []
-    for k, dv in enumerate(dirs):
-        fd = (ls[1 + 2 * k] - ls[2 + 2 * k]) / (2 * eps)
-        ad = float((grad * dv).sum())
-        rows.append((ad, fd, abs(ad - fd) / max(abs(fd), abs(ad), 1e-30), abs(ad - fd) / gnorm))
     out = {
         "workload": f"closed all-fluid {n}^2 box, {len(parts)} particles, {steps} PICFLIP "
                     f"steps, checkpoints every {every}, deterministic rollout",
-        "eps": eps, "directions": n_dirs,
-        "loss_gpu_fp32": float(loss), "loss_fp64_replay": ls[0],
-        "loss_rel_diff": abs(ls[0] - float(loss)) / abs(ls[0]),
-        "rel_median": float(np.median([r[2] for r in rows])),
-        "rel_max": float(max(r[2] for r in rows)),
-        "rel_to_grad_norm_max": float(max(r[3] for r in rows)),
-        "grad_norm": gnorm,
-        "seconds": time.perf_counter() - t0,
-        "oracle_threads": workers,
+        "directions": n_dirs,
+        "loss_gpu_fp32": float(loss.detach()), "loss_fp64_replay": l0,
+        "loss_rel_diff": abs(l0 - float(loss.detach())) / abs(l0),
+        "grad_norm": gnorm, "oracle_threads": workers, "by_eps": {},
     }
-    log(f"adjoint vs FP64 replayed differences ({n}^2, {steps} steps, {n_dirs} directions, "
-        f"eps {eps:g}): median rel {out['rel_median']:.2e}, max {out['rel_max']:.2e}, max rel to "
-        f"|grad| {out['rel_to_grad_norm_max']:.2e}; loss FP32 vs FP64 replay "
-        f"{out['loss_rel_diff']:.1e} ({out['seconds']:.1f} s)")
+    for eps in eps_list:
+        states = []
+        for dv in dirs:
+            states += [state(base + eps * dv), state(base - eps * dv)]
+        ls = rr.density_losses(states, 3, dt, target, workers=workers)
+        rows = []
+        for k, dv in enumerate(dirs):
+            fd = (ls[2 * k] - ls[2 * k + 1]) / (2 * eps)
+            ad = float((grad * dv).sum())
+            rows.append((ad, fd, abs(ad - fd) / max(abs(fd), abs(ad), 1e-30),
+                         abs(ad - fd) / gnorm))
+        r = {"rel_median": float(np.median([q[2] for q in rows])),
+             "rel_max": float(max(q[2] for q in rows)),
+             "rel_to_grad_norm_max": float(max(q[3] for q in rows)),
+             "directions": [{"adjoint": q[0], "fd": q[1]} for q in rows]}
+        out["by_eps"][f"{eps:g}"] = r
+        log(f"adjoint vs FP64 replayed differences ({n}^2, {steps} steps, {n_dirs} directions, "
+            f"eps {eps:g}): median rel {r['rel_median']:.2e}, max {r['rel_max']:.2e}, max rel "
+            f"to |grad| {r['rel_to_grad_norm_max']:.2e}")
+    out["seconds"] = time.perf_counter() - t0
+    log(f"replayed check: loss FP32 vs FP64 replay {out['loss_rel_diff']:.1e} "
+        f"({out['seconds']:.1f} s)")
     return out
 
 
@@ -636,7 +660,7 @@ def adjoint_leg(fsg, scenes, args):
     check = None
     if args.adj_check_n > 0 and args.adj_check_dirs > 0:
         check = replay_gradient_check(fsg, scenes, args.adj_check_n, steps, every,
-                                      args.adj_check_dirs, 1e-3, scale, 1e-10,
+                                      args.adj_check_dirs, (1e-5, 1e-3), scale, 1e-10,
                                       args.adj_check_threads)
     return {
         "fp64_replay_check": check,
@@ -681,6 +705,9 @@ def main():
     ap.add_argument("--graph", type=int, default=1,
                     help="multigrid iterations replayed from captured HIP graphs (1) or "
                          "launched kernel by kernel (0)")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="1: fsg_autotune chooses the multigrid settings (untimed) before the "
+                         "timed region of each workload")
     ap.add_argument("--mg-smoother", type=int, default=1,
                     help="multigrid smoothing: 0 damped Jacobi (0.8), 1 two-step Chebyshev")
     ap.add_argument("--warm-start", type=int, default=0,
@@ -738,45 +765,77 @@ def main():
     max_iter = 200 if solver == fsg.SOLVER_MG else 20000
     log(f"workload dam_break_{args.n}: {P} particles; solver {args.solver} tol {args.tol:g}")
 
-    # ---- the configuration's whole run (500 steps from the initial state), on a
-    # domain of its own: average rate, iterations and the largest FP64 residual
-    full = full_run(fsg, scene, solver, args.tol, args.run_steps) if args.run_steps > 0 else None
-    if full:
-        log(f"whole run: {full['steps']} steps at {full['steps_per_s']:.1f} steps/s, "
-            f"{full['cg_iterations_min']}-{full['cg_iterations_max']} iterations "
-            f"(mean {full['cg_iterations_mean']:.1f}), max rel residual "
-            f"{full['cg_rel_residual_max']:.2e}")
-
     # ---- chosen form: W warmup steps, the trajectory advanced untimed to step
-    # --start (the developed flow), a per-kernel profile, then the timed region
+    # --start (the developed flow), the autotuner (untimed; it restores the state), a
+    # per-kernel profile, then the timed region
     d = make_domain(fsg, scene, solver, args.tol, max_iter, args.warm_start)
     run_steps(fsg, d, args.warmup, dt)
     advance = max(args.start - args.warmup - args.profile_steps, 0)
     run_steps(fsg, d, advance, dt)
+    tune = None
+    if args.autotune and solver == fsg.SOLVER_MG:
+        t0 = time.perf_counter()
+        tune = d.autotune(fsg.PICFLIP, dt, steps=3)
+        tune["seconds"] = time.perf_counter() - t0
+        log(f"autotune at step {args.warmup + advance}: {tune}")
+    tuned = {k: tune[k] for k in TUNED} if tune else None
     prof_solves = []
     prof = (kernel_profile(fsg, d, args.profile_steps, dt, solves=prof_solves)
             if args.profile_steps > 0 else {})
     t_start = args.warmup + advance + args.profile_steps
     state0 = d.save_state() if (args.plain_steps > 0 or args.refined_steps > 0) else None
     per_step = {k: ms * n for k, (ms, n) in prof.items()}
-    # the roofline kernel: the one kernel with the most time per step (timers that
-    # cover several kernels are not candidates)
+    # the roofline kernel: the one kernel with the most time per step among those that
+    # move at least ROOFLINE_MIN_BYTES of algorithmic bytes per launch (timers that
+    # cover several kernels are not candidates); below that a kernel is bound by launch
+    # and barrier latency, not by HBM: those are reported beside it ("latency_bound")
     single = {k: v for k, v in per_step.items() if k not in MULTI_KERNEL_TIMERS}
-    dominant = args.roofline_kernel or (max(single, key=single.get) if single else None)
+    one_domain_mg = solver == fsg.SOLVER_MG and mg_levels(args.n, args.n)[1] >= 2
+
+    def launch_bytes(k):
+        if one_domain_mg and k.startswith("cg_") and prof_solves:
+            b = solve_bytes(k, args.n, args.n, prof_solves)
+            n = prof[k][1] * len(prof_solves)
+            return b / n if b is not None and n else None
+        return kernel_bytes(k, args.n, args.n, P)
+
+    hbm = {k: v for k, v in single.items() if (launch_bytes(k) or 0.0) >= ROOFLINE_MIN_BYTES}
+    latency = sorted(((k, v) for k, v in single.items() if k not in hbm), key=lambda kv: -kv[1])
+    pool = hbm or single
+    dominant = args.roofline_kernel or (max(pool, key=pool.get) if pool else None)
     log("kernel profile (ms per step): " +
         ", ".join(f"{k} {v:.3f}" for k, v in sorted(per_step.items(), key=lambda kv: -kv[1])[:12]))
-    elapsed, st, tm = timed_segment(fsg, d, args.steps, dt, kernel=dominant)
+    # the timed region: the multigrid iterations replayed from their graphs, nothing
+    # instrumented
+    elapsed, st, _ = timed_segment(fsg, d, args.steps, dt)
     state = d.save_state() if not args.no_fp64_check else None
-    d.close()
     sps = args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
     log(f"chosen: {sps:.3f} steps/s ({ms_per_step:.3f} ms/step) over steps {t_start}.."
         f"{t_start + args.steps - 1}, dominant kernel {dominant}")
+    # the roofline kernel timed live over the next K steps of the same trajectory: HIP
+    # events around each of its launches on the solver's stream (per-kernel events
+    # launch every kernel, not the captured graphs: this segment's own rate is
+    # reported beside, not as the value)
+    elapsed_k, _, tm = timed_segment(fsg, d, args.steps, dt, kernel=dominant)
+    d.close()
+    log(f"instrumented segment (eager launches, {dominant} timed): "
+        f"{args.steps / elapsed_k:.3f} steps/s")
 
-    # roofline of the dominant kernel, timed live in the timed region: HIP events
-    # around each of its launches on the solver's stream (they also count the
+    # ---- the configuration's whole run (500 steps from the initial state), on a
+    # domain of its own with the tuned settings: average rate, iterations and the
+    # largest FP64 residual
+    full = (full_run(fsg, scene, solver, args.tol, args.run_steps, tuned)
+            if args.run_steps > 0 else None)
+    if full:
+        log(f"whole run: {full['steps']} steps at {full['steps_per_s']:.1f} steps/s, "
+            f"{full['cg_iterations_min']}-{full['cg_iterations_max']} iterations "
+            f"(mean {full['cg_iterations_mean']:.1f}), max rel residual "
+            f"{full['cg_rel_residual_max']:.2e}")
+
+    # roofline of the dominant kernel, timed live over the instrumented segment: HIP
+    # events around each of its launches on the solver's stream (they also count the
     # dispatch before the kernel starts), and the kernel's own device clock beside
-    one_domain_mg = solver == fsg.SOLVER_MG and mg_levels(args.n, args.n)[1] >= 2
     solves = tm.get("__solves__", [])
     roof = None
     if dominant and dominant in tm:
@@ -905,6 +964,8 @@ def main():
             "mg_smoother": ["damped Jacobi 0.8", "two-step Chebyshev [0.4, 2]"][args.mg_smoother],
             "tol": args.tol,
             "timed_steps": [t_start, t_start + args.steps - 1],
+            "autotune": tune,
+            "instrumented_segment_steps_per_s": args.steps / elapsed_k,
             "untimed_steps_before": t_start,
             "whole_run": full,
             "plain_tol": args.plain_tol,
@@ -913,6 +974,9 @@ def main():
             "refined": refined,
             "cg_hbm_gbs": cg_gbs,
             "solver_kernels": solver_kernels,
+            "latency_bound_kernels": {k: {"ms_per_step": round(v, 4),
+                                          "bytes_per_launch": launch_bytes(k)}
+                                      for k, v in latency[:6]},
             "cg_iterations_last_solve": st.cg_iterations,
             "rel_err_vs_fp64_single_step": errs,
             "max_rel_err_vs_fp64": max(errs.values()) if errs else None,
