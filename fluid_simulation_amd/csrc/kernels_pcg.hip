@@ -359,7 +359,7 @@ __global__ void __launch_bounds__(kPcgThreads) k_pcg(PersistentArgs a) {
   }
 }
 
-int pcg_max_kpt() { return 8; }
+int pcg_max_kpt() { return 4; }
 
 size_t pcg_lds_bytes(int F, int grid, int nx) {
   const int per = (F + grid - 1) / grid;
@@ -374,8 +374,12 @@ hipError_t launch_pcg(hipStream_t s, const PersistentArgs& a, int grid, int kpt)
   switch (kpt) {
     case 1: f = ref ? (const void*)k_pcg<1, true> : (const void*)k_pcg<1, false>; break;
     case 2: f = ref ? (const void*)k_pcg<2, true> : (const void*)k_pcg<2, false>; break;
-    case 4: f = ref ? (const void*)k_pcg<4, true> : (const void*)k_pcg<4, false>; break;
-    case 8: f = ref ? (const void*)k_pcg<8, true> : (const void*)k_pcg<8, false>; break;
+    // 4 unknowns per thread only in the FP32 form: the refined form's FP64 state
+    // exceeds the 128 VGPRs of a 1024-thread block there (it would spill)
+    case 4:
+      if (ref) return hipErrorInvalidValue;
+      f = (const void*)k_pcg<4, false>;
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipLaunchCooperativeKernel(f, dim3(grid), dim3(kPcgThreads), args, lds, s);
@@ -383,9 +387,8 @@ hipError_t launch_pcg(hipStream_t s, const PersistentArgs& a, int grid, int kpt)
 
 hipError_t pcg_set_lds_limit(size_t max_bytes) {
   const void* fs[] = {(const void*)k_pcg<1, false>, (const void*)k_pcg<2, false>,
-                      (const void*)k_pcg<4, false>, (const void*)k_pcg<8, false>,
-                      (const void*)k_pcg<1, true>,  (const void*)k_pcg<2, true>,
-                      (const void*)k_pcg<4, true>,  (const void*)k_pcg<8, true>};
+                      (const void*)k_pcg<4, false>, (const void*)k_pcg<1, true>,
+                      (const void*)k_pcg<2, true>};
   for (const void* f : fs) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)max_bytes);

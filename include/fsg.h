@@ -109,7 +109,8 @@ typedef struct fsg_params {
   int32_t refine_max_outer;/* refined solver: maximum FP64 refinement sweeps */
   int32_t record_ext_info; /* 1: fsg_step also records the extension sources */
   int32_t pcg_blocks;     /* persistent solvers: blocks (0: automatic) */
-  int32_t pcg_kpt;        /* persistent solvers: unknowns per thread, 1/2/4/8 (0: automatic) */
+  int32_t pcg_kpt;        /* persistent solvers: unknowns per thread, 1/2/4 (refined form:
+                             1/2), 0: automatic */
   int32_t warm_start;     /* persistent, refined and multigrid solvers: 1 starts CG
                              from the previous solve's pressure (same stopping test
                              |r| < tol |b|); 0 (default) starts from x0 = 0 like
