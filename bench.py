@@ -182,10 +182,12 @@ def run_steps(fsg, d, n, dt, scheme=None):
         d.step(scheme, dt)
 
 
-def solver_gbs(prof, nx, ny, P, levels=None, solves=None):
-    """The pressure solver's algorithmic bytes over its kernel time (GB/s): dense
-    counts for a batch (`levels`), live-tile counts (`solves`: (iterations, live
-    tiles) per solve of the profiled steps) for one domain."""
+def solver_gbs(prof, nx, ny, P, levels=None, solves=None, live_frac=1.0):
+    """The pressure solver's algorithmic bytes over its kernel time (GB/s): live-tile
+    counts (`solves`: (iterations, live tiles) per solve of the profiled steps) for one
+    domain; for a batch (`levels`) the level-0 and vector kernels' dense counts scaled
+    by the fraction of level-0 tiles that hold an unknown (`live_frac`: the others are
+    skipped), the smaller levels' dense."""
     tot_b = tot_ms = 0.0
     for k, (ms, n) in prof.items():
         if not k.startswith("cg_") or k in MULTI_KERNEL_TIMERS:
@@ -198,6 +200,8 @@ def solver_gbs(prof, nx, ny, P, levels=None, solves=None):
         else:
             b = kernel_bytes(k, nx, ny, P, levels)
             if b is not None:
+                if k in ("cg_mg_down0", "cg_mg_up0", "cg_mg_xr", "cg_mg_dir", "cg_mg_rz"):
+                    b *= live_frac
                 tot_b += b * n
                 tot_ms += ms * n
     return tot_b / (tot_ms * 1e-3) / 1e9 if tot_ms > 0 else None
@@ -368,11 +372,15 @@ def batch_leg(fsg, scenes, args):
         log(f"batch autotune: {tune}")
     prof = kernel_profile(fsg, b, 2, scs[0].dt)
     per_step = {k: ms * n for k, (ms, n) in prof.items()}
+    st_p = b.stats()
+    unknowns, live = st_p.n_fluid, st_p.mg_live_tiles
     elapsed, st, _ = timed_segment(fsg, b, args.batch_steps, scs[0].dt)
     W, H = b.nx, b.ny
     b.close()
     sps = args.batch_steps / elapsed
-    gbs = solver_gbs(prof, W, H, parts, batch_levels([(sc.nx, sc.ny) for sc in scs], W, H))
+    live_frac = live / (((W + 31) // 32) * ((H + 31) // 32))
+    gbs = solver_gbs(prof, W, H, parts, batch_levels([(sc.nx, sc.ny) for sc in scs], W, H),
+                     live_frac=live_frac)
     log(f"batch: {sps:.3f} steps/s = {sps * len(scs):.1f} scene-steps/s "
         f"({st.cg_iterations} MG iterations in the last solve)")
     seq = None
@@ -418,10 +426,12 @@ def batch_leg(fsg, scenes, args):
                "sample": f"first step of the first {k} scenes, FP32 oracle, Jacobi-PCG rel "
                          f"tol 1e-6, 1 thread; the batch rate is extrapolated by cells"}
         log(f"batch cpu oracle: {k / t_cpu:.3f} scene-steps/s over {k} scenes ({t_cpu:.1f} s)")
-    # the per-scene CG vector kernels: algorithmic bytes over the scenes' cells (each
-    # array read or written once) per launch, over their HIP-event time (which also
-    # counts launch gaps, so this is a lower bound on the bandwidth)
-    vec_bytes = {"cg_mg_xr": 33.0 * cells, "cg_mg_dir": 13.0 * cells, "cg_mg_rz": 9.0 * cells}
+    # the per-scene CG vector kernels: algorithmic bytes per unknown (each array read or
+    # written once; the cells of their tiles that are not unknowns move bytes that are
+    # not counted) per launch, over their HIP-event time (which also counts launch
+    # gaps, so this is a lower bound on the bandwidth)
+    vec_bytes = {"cg_mg_xr": 33.0 * unknowns, "cg_mg_dir": 13.0 * unknowns,
+                 "cg_mg_rz": 9.0 * unknowns}
     vec_gbs = {k: vec_bytes[k] / (prof[k][0] * 1e-3) / 1e9 for k in vec_bytes if k in prof}
     return {
         "workload": f"dam_break_batch: {len(scs)} non-square scenes, sides {args.batch_lo}.."
@@ -435,7 +445,9 @@ def batch_leg(fsg, scenes, args):
         "cell_steps_per_s": sps * cells,
         "cg_iterations_last_solve": st.cg_iterations,
         "cg_hbm_gbs": gbs,
-        "cg_vector_kernel_gbs": vec_gbs,
+        "cg_unknowns": unknowns,
+        "level0_live_tile_fraction": live_frac,
+        "cg_vector_kernel_gbs_per_unknown_bytes": vec_gbs,
         "one_scene_at_a_time": seq,
         "cpu_baseline": cpu,
         "speedup_vs_one_at_a_time": (sps * cells / seq["cell_steps_per_s"]) if seq else None,
