@@ -167,7 +167,7 @@ def log(*a):
 GRAPH = 1
 SMOOTHER = 1
 # the multigrid settings fsg_autotune chooses
-TUNED = ("mg_smoother", "mg_coarse_sweeps", "mg_fuse_update", "graph_iters")
+TUNED = ("mg_smoother", "mg_coarse_sweeps", "mg_fuse_update", "graph_iters", "splat_order")
 
 
 def make_domain(fsg, scene, solver, tol, max_iter, warm=0):
@@ -277,14 +277,13 @@ def timed_segment(fsg, d, steps, dt, kernel=None, scheme=None):
     return t1 - t0, st, tm
 
 
-def full_run(fsg, scene, solver, tol, steps, tuned=None):
+def full_run(fsg, scene, solver, tol, steps, autotune=False):
     """The configuration's whole run (`steps` steps from the initial state) on a
-    domain of its own, with the autotuner's settings if given: wall time, and every
-    solve's iterations and FP64 relative residual (host-side counters read after each
-    step, no extra synchronisation)."""
+    domain of its own, autotuned (untimed) at its initial state if asked: wall time,
+    and every solve's iterations and FP64 relative residual (host-side counters read
+    after each step, no extra synchronisation)."""
     d = make_domain(fsg, scene, solver, tol, 200 if solver == fsg.SOLVER_MG else 20000)
-    if tuned:
-        d.set_params(**tuned)
+    tune = d.autotune(fsg.PICFLIP, scene.dt, steps=3) if autotune else None
     its, errs = [], []
     d.synchronize()
     t0 = time.perf_counter()
@@ -300,7 +299,8 @@ def full_run(fsg, scene, solver, tol, steps, tuned=None):
             "cg_iterations_min": int(min(its)), "cg_iterations_mean": float(np.mean(its)),
             "cg_iterations_max": int(max(its)), "cg_rel_residual_max": float(max(errs)),
             "all_solves_within_tol": bool(max(errs) <= tol),
-            "cg_iterations_by_step": {str(k): int(its[k]) for k in range(0, steps, 50)}}
+            "cg_iterations_by_step": {str(k): int(its[k]) for k in range(0, steps, 50)},
+            "autotune": tune}
 
 
 def fp64_single_step_error(fsg, scene, solver, tol, state=None):
@@ -793,7 +793,6 @@ def main():
         tune = d.autotune(fsg.PICFLIP, dt, steps=3)
         tune["seconds"] = time.perf_counter() - t0
         log(f"autotune at step {args.warmup + advance}: {tune}")
-    tuned = {k: tune[k] for k in TUNED} if tune else None
     prof_solves = []
     prof = (kernel_profile(fsg, d, args.profile_steps, dt, solves=prof_solves)
             if args.profile_steps > 0 else {})
@@ -838,9 +837,10 @@ def main():
         f"{args.steps / elapsed_k:.3f} steps/s")
 
     # ---- the configuration's whole run (500 steps from the initial state), on a
-    # domain of its own with the tuned settings: average rate, iterations and the
+    # domain of its own, tuned at its own start: average rate, iterations and the
     # largest FP64 residual
-    full = (full_run(fsg, scene, solver, args.tol, args.run_steps, tuned)
+    full = (full_run(fsg, scene, solver, args.tol, args.run_steps,
+                     bool(args.autotune) and solver == fsg.SOLVER_MG)
             if args.run_steps > 0 else None)
     if full:
         log(f"whole run: {full['steps']} steps at {full['steps_per_s']:.1f} steps/s, "

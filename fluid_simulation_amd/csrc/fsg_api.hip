@@ -85,6 +85,8 @@ struct BatchInfo {
   fsg::BatchScene* d_scene = nullptr;
   int2* d_ptile = nullptr;
   int n_ptile = 0;
+  int n_buckets = 0;          // the splat's 16 x 16-cell buckets over all scenes
+  int max_scene_buckets = 0;  // the most buckets of one scene
   int4* d_rect = nullptr;
   int* d_sid = nullptr;
   double* d_norm = nullptr;
@@ -324,10 +326,27 @@ void classify_and_splat(fsg_domain* h) {
   timed(h, "cell_init", [&] {
     launch_cell_init(h->stream, h->g, h->label, h->solid, h->acc, 1, fx ? 0 : 1);
   });
-  // one domain: the splat reads the particles grouped by bucket (their storage order
-  // is kept); a batch keeps its per-scene particle tiles
+  // the splat reads the particles grouped by bucket (their storage order is kept); a
+  // batch groups each scene's particles within the scene's own range
   const float4* src = h->pt;
-  if (!h->batch && h->np > 0 && h->np < INT_MAX && h->prank && h->psorted &&
+  const bool bucketed = h->prm.splat_order == 1;
+  if (bucketed && h->batch && h->np > 0 && h->np < INT_MAX && h->prank && h->psorted &&
+      h->batch->max_scene_buckets <= fsg::kMaxBuckets) {
+    BatchInfo& b = *h->batch;
+    if (!h->bkt_cnt) {
+      if (dalloc(h, &h->bkt_cnt, (size_t)b.n_buckets) != hipSuccess ||
+          dalloc(h, &h->bkt_off, (size_t)b.n_buckets) != hipSuccess)
+        h->bkt_cnt = h->bkt_off = nullptr;
+    }
+    if (h->bkt_cnt) {
+      timed(h, "bucket_order", [&] {
+        launch_bucket_order_batch(h->stream, h->pt, h->np, h->bkt_cnt, h->bkt_off, h->prank,
+                                  h->psorted, b.tables(), b.n_buckets, b.max_scene_buckets);
+      });
+      src = h->psorted;
+    }
+  }
+  if (bucketed && !h->batch && h->np > 0 && h->np < INT_MAX && h->prank && h->psorted &&
       fsg::bucket_count(h->g) <= fsg::kMaxBuckets) {
     if (!h->bkt_cnt) {
       const size_t nb = (size_t)fsg::bucket_count(h->g);
@@ -892,6 +911,7 @@ void fsg_params_default(fsg_params* p) {
   p->mg_coarse_sweeps = 0;
   p->graph_iters = 2;
   p->mg_fuse_update = 1;
+  p->splat_order = 1;
 }
 
 int fsg_create(fsg_domain** out, int32_t nx, int32_t ny, float length_x, float length_y,
@@ -996,7 +1016,7 @@ int fsg_set_params(fsg_domain* h, const fsg_params* p) {
       p->record_decisions < 0 || p->record_decisions > 1 || p->mg_smoother < 0 ||
       p->mg_smoother > 1 || p->mg_coarse_sweeps < 0 || p->mg_coarse_sweeps > 64 ||
       p->graph_iters < 1 || p->graph_iters > 16 || p->mg_fuse_update < 0 ||
-      p->mg_fuse_update > 1)
+      p->mg_fuse_update > 1 || p->splat_order < 0 || p->splat_order > 1)
     return fail(h, FSG_E_INVALID, "fsg_set_params: value out of range");
   const size_t C = ncell(h);
   // fixed-point sums: 4 per cell (the P2G accumulators; the density uses the first C),
@@ -1640,7 +1660,10 @@ int fsg_batch_create(fsg_domain** out, int32_t n_scenes, const int32_t* nx, cons
         b->solid[c] = border ? 1 : 0;
       }
     rect[s] = make_int4(ox[s], oy[s], nx[s], ny[s]);
-    table[s] = fsg::BatchScene{gs, 0, 0, ox[s] + oy[s] * W, W};
+    const int bbx = (nx[s] + 15) >> 4, bby = (ny[s] + 15) >> 4;
+    table[s] = fsg::BatchScene{gs, 0, 0, ox[s] + oy[s] * W, W, b->n_buckets, bbx};
+    b->n_buckets += bbx * bby;
+    b->max_scene_buckets = std::max(b->max_scene_buckets, bbx * bby);
     // the scene's own multigrid levels: its rectangle of level la, halved until 4 x 4
     fsg::SceneCoarse c{};
     c.cx = ox[s] >> la;
@@ -1941,22 +1964,26 @@ int fsg_autotune(fsg_domain* h, int32_t scheme, float dt, int32_t steps, fsg_tun
   if (h->np == 0 && scheme != FSG_SCHEME_GRID_ADVECT)
     return fail(h, FSG_E_INVALID, "fsg_autotune: no particles to step");
   struct Cand {
-    int smoother, coarse, fuse, iters;
+    int smoother, coarse, fuse, iters, order;
   };
   std::vector<Cand> cands;
   const fsg_params p0 = h->prm;
   const bool can_fuse = !h->batch;
+  const bool has_particles = h->np > 0;
   for (int sm : {1, 0})
     for (int co : {0, 8, 16})
       for (int fu : can_fuse ? std::initializer_list<int>{1, 0} : std::initializer_list<int>{1})
-        for (int gi : {2, 4}) cands.push_back({sm, co, fu, gi});
+        for (int gi : {2, 4})
+          for (int so : has_particles ? std::initializer_list<int>{1, 0}
+                                      : std::initializer_list<int>{p0.splat_order})
+            cands.push_back({sm, co, fu, gi, so});
   TuneState saved;
   if (int rc = tune_save(h, saved)) return rc;
   const fsg_stats stats0 = h->host_stats;
   const int last0 = h->mg_last_iters;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   double best = 1e300, cur = -1.0;
-  int best_k = 0, best_iters = 0;
+  int best_k = 0, best_iters = 0, cur_k = -1, cur_iters = 0;
   for (size_t k = 0; k < cands.size(); ++k) {
     const Cand& c = cands[k];
     fsg_params q = p0;
@@ -1964,6 +1991,7 @@ int fsg_autotune(fsg_domain* h, int32_t scheme, float dt, int32_t steps, fsg_tun
     q.mg_coarse_sweeps = c.coarse;
     q.mg_fuse_update = c.fuse;
     q.graph_iters = c.iters;
+    q.splat_order = c.order;
     if (int rc = fsg_set_params(h, &q)) return rc;
     double t_sum = 0.0;
     int its = 0;
@@ -1978,20 +2006,34 @@ int fsg_autotune(fsg_domain* h, int32_t scheme, float dt, int32_t steps, fsg_tun
       its = h->host_stats.cg_iterations;
     }
     const double ms = t_sum / steps;
-    if (c.smoother == p0.mg_smoother && c.coarse == p0.mg_coarse_sweeps &&
-        c.fuse == p0.mg_fuse_update && c.iters == p0.graph_iters)
+    const bool is_cur = c.smoother == p0.mg_smoother && c.coarse == p0.mg_coarse_sweeps &&
+                        c.fuse == p0.mg_fuse_update && c.iters == p0.graph_iters &&
+                        c.order == p0.splat_order;
+    if (is_cur) {
       cur = ms;
+      cur_k = (int)k;
+      cur_iters = its;
+    }
     if (ms < best) {
       best = ms;
       best_k = (int)k;
       best_iters = its;
     }
   }
+  // the handle's own configuration stays unless a candidate beats it by more than
+  // kTuneMargin (the spread of a few timed steps)
+  constexpr double kTuneMargin = 0.02;
+  if (cur_k >= 0 && best > cur * (1.0 - kTuneMargin)) {
+    best = cur;
+    best_k = cur_k;
+    best_iters = cur_iters;
+  }
   fsg_params q = p0;
   q.mg_smoother = cands[best_k].smoother;
   q.mg_coarse_sweeps = cands[best_k].coarse;
   q.mg_fuse_update = cands[best_k].fuse;
   q.graph_iters = cands[best_k].iters;
+  q.splat_order = cands[best_k].order;
   if (int rc = fsg_set_params(h, &q)) return rc;
   if (int rc = tune_load(h, saved, true)) return rc;
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -2003,6 +2045,7 @@ int fsg_autotune(fsg_domain* h, int32_t scheme, float dt, int32_t steps, fsg_tun
     out->mg_coarse_sweeps = q.mg_coarse_sweeps;
     out->mg_fuse_update = q.mg_fuse_update;
     out->graph_iters = q.graph_iters;
+    out->splat_order = q.splat_order;
     out->candidates = (int32_t)cands.size();
     out->cg_iterations = best_iters;
     out->ms_per_step = best;

@@ -43,6 +43,8 @@ struct BatchScene {
   int64_t np;
   int cell_off;
   int pitch;
+  int bucket_base;  // the splat's bucket order: the scene's first bucket over all scenes
+  int bucket_bx;    // and its buckets per row (16 x 16 cells each)
 };
 // Particle tile t (256 particles) belongs to scene ptile[t].x, local tile ptile[t].y.
 struct BatchTables {
@@ -70,6 +72,13 @@ constexpr int kMaxBuckets = 16384;  // the count kernel's LDS histogram: 64 KiB
 int bucket_count(const Geom& g);
 void launch_bucket_order(hipStream_t s, const Geom& g, const float4* pt, int64_t np, int* cnt,
                          int* off, int* rank, float4* sorted);
+// A packed batch: every scene's particles grouped by the 16 x 16-cell buckets of its own
+// grid (BatchScene::bucket_base / bucket_bx), within the scene's own particle range, so
+// the batch's particle tiles address the sorted copy unchanged.  n_buckets over all
+// scenes; max_scene_buckets sizes the count kernel's LDS histogram.
+void launch_bucket_order_batch(hipStream_t s, const float4* pt, int64_t np, int* cnt, int* off,
+                               int* rank, float4* sorted, const BatchTables& tb, int n_buckets,
+                               int max_scene_buckets);
 void launch_face_update(hipStream_t s, const Geom& g, int flags, float* u, float* v,
                         const float4* acc, float* prev_u, float* prev_v, const uint8_t* label,
                         float gx_dt, float gy_dt, float* ext_u, float* ext_v, uint8_t* mxf,

@@ -777,21 +777,66 @@ __device__ __forceinline__ int particle_bucket(const Geom& g, float4 q, int bx) 
 // its particles in an LDS histogram of every bucket first (slots within the block),
 // then takes one range per bucket it touches from the global count: one global atomic
 // per (block, bucket) instead of one per particle on a few hot counters.
-__global__ void __launch_bounds__(256) k_bucket_count(Geom g, const float4* __restrict__ pt,
-                                                      int np, int bx, int nb,
+// The slots within a block follow the particles' storage order (a stable rank): each
+// wave groups its lanes by bucket with ballots (one round per distinct bucket among its
+// lanes), and the block's four waves take their ranges of each bucket in wave order.
+// Particles that were neighbours in storage stay neighbours in the copy, which keeps
+// the splat's quad aggregation effective on particles that have not mixed yet.
+// BATCH: a block is one particle tile of one scene (its own geometry); the histogram
+// holds that scene's buckets, which start at its bucket_base in cnt.
+template <bool BATCH>
+__global__ void __launch_bounds__(256) k_bucket_count(Geom g0, const float4* __restrict__ pt,
+                                                      int np, int bx0, int nb0,
                                                       int* __restrict__ cnt,
-                                                      int* __restrict__ rank) {
+                                                      int* __restrict__ rank, BatchTables tb) {
   extern __shared__ int s_h[];  // nb ints
+  const PartTile T = part_tile<BATCH>(g0, np, tb);
+  int bx = bx0, nb = nb0, base = 0;
+  if (BATCH) {
+    const BatchScene sc = tb.scene[tb.ptile[xcd_tile(blockIdx.x, gridDim.x)].x];
+    bx = sc.bucket_bx;
+    nb = sc.bucket_bx * ((sc.g.ny + (1 << kBucketShift) - 1) >> kBucketShift);
+    base = sc.bucket_base;
+  }
   for (int e = threadIdx.x; e < nb; e += 256) s_h[e] = 0;
   __syncthreads();
-  const int p = blockIdx.x * 256 + threadIdx.x;
-  const bool live = p < np;
-  const int b = live ? particle_bucket(g, pt[p], bx) : 0;
-  const int local = live ? atomicAdd(&s_h[b], 1) : 0;
+  const int64_t p = T.p;
+  const bool live = p < T.np_end;
+  const int b = live ? particle_bucket(T.g, pt[p], bx) : 0;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  // this lane's rank among the wave's lanes of its bucket, its group's size and leader
+  int wrank = 0, wcount = 0, lead = lane;
+  bool done = !live;
+  for (;;) {
+    const uint64_t pending = __ballot(!done);
+    if (pending == 0ull) break;
+    const int first = __ffsll((long long)pending) - 1;
+    const int fb = __shfl(b, first, 64);
+    const bool mine = !done && b == fb;
+    const uint64_t same = __ballot(mine);
+    if (mine) {
+      wrank = __popcll(same & below);
+      wcount = __popcll(same);
+      lead = first;
+      done = true;
+    }
+  }
+  int start = 0;
+  for (int w = 0; w < 4; ++w) {
+    // the group leaders of wave w take their range; buckets are distinct within a wave
+    if (wave == w && live && wrank == 0) {
+      start = s_h[b];
+      s_h[b] = start + wcount;
+    }
+    __syncthreads();
+  }
+  const int gstart = __shfl(start, lead, 64);
+  const int local = gstart + wrank;
   __syncthreads();
   for (int e = threadIdx.x; e < nb; e += 256) {
     const int c = s_h[e];
-    if (c) s_h[e] = atomicAdd(&cnt[e], c);
+    if (c) s_h[e] = atomicAdd(&cnt[base + e], c);
   }
   __syncthreads();
   if (live) rank[p] = s_h[b] + local;
@@ -837,14 +882,25 @@ __global__ void __launch_bounds__(1024) k_bucket_scan(int* __restrict__ cnt,
 }
 
 // sorted[offset of the bucket + slot] = particle (the scattered side is the store).
-__global__ void __launch_bounds__(256) k_bucket_fill(Geom g, const float4* __restrict__ pt,
-                                                     int np, int bx, const int* __restrict__ off,
+// BATCH: the offsets run over all scenes' buckets in scene order, and the scenes'
+// particle ranges are contiguous in scene order, so every scene's particles land in
+// its own range.
+template <bool BATCH>
+__global__ void __launch_bounds__(256) k_bucket_fill(Geom g0, const float4* __restrict__ pt,
+                                                     int np, int bx0, const int* __restrict__ off,
                                                      const int* __restrict__ rank,
-                                                     float4* __restrict__ sorted) {
-  const int p = blockIdx.x * 256 + threadIdx.x;
-  if (p >= np) return;
+                                                     float4* __restrict__ sorted, BatchTables tb) {
+  const PartTile T = part_tile<BATCH>(g0, np, tb);
+  int bx = bx0, base = 0;
+  if (BATCH) {
+    const BatchScene sc = tb.scene[tb.ptile[xcd_tile(blockIdx.x, gridDim.x)].x];
+    bx = sc.bucket_bx;
+    base = sc.bucket_base;
+  }
+  const int64_t p = T.p;
+  if (p >= T.np_end) return;
   const float4 q = pt[p];
-  const int d = off[particle_bucket(g, q, bx)] + rank[p];
+  const int d = off[base + particle_bucket(T.g, q, bx)] + rank[p];
   if (d >= 0 && d < np) sorted[d] = q;
 }
 
@@ -908,9 +964,21 @@ void launch_bucket_order(hipStream_t s, const Geom& g, const float4* pt, int64_t
   const int bx = (g.nx + (1 << kBucketShift) - 1) >> kBucketShift;
   const int nb = bucket_count(g);
   const int n = (int)np;
-  k_bucket_count<<<nblocks(np), 256, (size_t)nb * sizeof(int), s>>>(g, pt, n, bx, nb, cnt, rank);
+  k_bucket_count<false><<<nblocks(np), 256, (size_t)nb * sizeof(int), s>>>(g, pt, n, bx, nb, cnt,
+                                                                            rank, BatchTables{});
   k_bucket_scan<<<1, 1024, 0, s>>>(cnt, off, nb);
-  k_bucket_fill<<<nblocks(np), 256, 0, s>>>(g, pt, n, bx, off, rank, sorted);
+  k_bucket_fill<false><<<nblocks(np), 256, 0, s>>>(g, pt, n, bx, off, rank, sorted, BatchTables{});
+}
+
+void launch_bucket_order_batch(hipStream_t s, const float4* pt, int64_t np, int* cnt, int* off,
+                               int* rank, float4* sorted, const BatchTables& tb, int n_buckets,
+                               int max_scene_buckets) {
+  if (np <= 0 || tb.n_ptile == 0) return;
+  const int n = (int)np;
+  k_bucket_count<true><<<tb.n_ptile, 256, (size_t)max_scene_buckets * sizeof(int), s>>>(
+      Geom{}, pt, n, 0, 0, cnt, rank, tb);
+  k_bucket_scan<<<1, 1024, 0, s>>>(cnt, off, n_buckets);
+  k_bucket_fill<true><<<tb.n_ptile, 256, 0, s>>>(Geom{}, pt, n, 0, off, rank, sorted, tb);
 }
 void launch_face_update(hipStream_t s, const Geom& g, int flags, float* u, float* v,
                         const float4* acc, float* prev_u, float* prev_v, const uint8_t* label,

@@ -84,7 +84,7 @@ class Params(C.Structure):
         ("closed_walls", C.c_int32), ("graph", C.c_int32), ("deterministic", C.c_int32),
         ("record_decisions", C.c_int32), ("mg_smoother", C.c_int32),
         ("mg_coarse_sweeps", C.c_int32), ("graph_iters", C.c_int32),
-        ("mg_fuse_update", C.c_int32),
+        ("mg_fuse_update", C.c_int32), ("splat_order", C.c_int32),
     ]
 
 
@@ -92,9 +92,9 @@ class Tune(C.Structure):
     _fields_ = [
         ("mg_smoother", C.c_int32), ("mg_coarse_sweeps", C.c_int32),
         ("mg_fuse_update", C.c_int32), ("graph_iters", C.c_int32),
-        ("candidates", C.c_int32), ("cg_iterations", C.c_int32),
+        ("splat_order", C.c_int32), ("candidates", C.c_int32), ("cg_iterations", C.c_int32),
         ("ms_per_step", C.c_double), ("ms_per_step_before", C.c_double),
-        ("reserved", C.c_int64 * 4),
+        ("reserved", C.c_int64 * 3),
     ]
 
 

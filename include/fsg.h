@@ -156,6 +156,12 @@ typedef struct fsg_params {
                              1 (default) the CG update of x and the residual runs inside
                              the next iteration's level-0 down kernel; 0 in a kernel of
                              its own */
+  int32_t splat_order;    /* P2G / classify splat: 1 (default) reads a copy of the
+                             particles grouped by 16 x 16-cell bucket (a counting sort
+                             each step, storage order kept within a bucket; the
+                             particles' own order is untouched); 0 reads them in
+                             storage order (cheaper while particles that are neighbours
+                             in storage are neighbours in space) */
 } fsg_params;
 
 /* Counters of the last pressure solve and running totals. */
@@ -334,21 +340,23 @@ int fsg_get_timing(fsg_domain *h, int32_t cap, const char **names, double *ms,
 int fsg_reset_timing(fsg_domain *h);
 
 /* ---- autotuner (no counterpart in the reference) -----------------------------
- * Chooses the multigrid solver's settings for the handle's workload by timing whole
- * steps (scheme, dt) from its current state: every combination of mg_smoother
- * {1, 0}, mg_coarse_sweeps {0, 8, 16}, mg_fuse_update {1, 0} (one domain) and
- * graph_iters {2, 4}; each runs one untimed step, then `steps` (1..64) timed steps,
- * each from the saved state.  The fastest is set on the handle, and the state
+ * Chooses the step's settings for the handle's workload by timing whole steps
+ * (scheme, dt) from its current state: every combination of mg_smoother {1, 0},
+ * mg_coarse_sweeps {0, 8, 16}, mg_fuse_update {1, 0} (one domain), graph_iters {2, 4}
+ * and splat_order {1, 0} (with particles); each runs one untimed step, then `steps`
+ * (1..64) timed steps, each from the saved state.  The fastest is set on the handle
+ * unless the handle's own configuration is within 2 % of it, and the state
  * (particles, velocity grids, labels, reject flags, statistics) is restored, so the
  * next fsg_step continues the trajectory the call found.  Needs FSG_SOLVER_MG;
  * allocates a copy of the state for the duration of the call. */
 typedef struct fsg_tune {
   int32_t mg_smoother, mg_coarse_sweeps, mg_fuse_update, graph_iters; /* chosen */
+  int32_t splat_order;                                                /* chosen */
   int32_t candidates;       /* configurations timed */
   int32_t cg_iterations;    /* the chosen configuration's iterations (last timed step) */
   double ms_per_step;       /* the chosen configuration's mean step time */
   double ms_per_step_before;/* the handle's configuration before the call (if a candidate) */
-  int64_t reserved[4];
+  int64_t reserved[3];
 } fsg_tune;
 int fsg_autotune(fsg_domain *h, int32_t scheme, float dt, int32_t steps, fsg_tune *out);
 
