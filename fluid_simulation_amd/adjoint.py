@@ -1,9 +1,10 @@
-"""Reverse-mode differentiation of PIC / FLIP rollouts on the GPU.
+"""Reverse-mode differentiation of semi-Lagrangian, PIC and FLIP rollouts on the GPU.
 
 The adjoint kernels (``fsg_adjoint_step``) carry the adjoint state of one step
-(per particle d/d(x, y, vx, vy) and the back velocity buffer, the only grid a step
-reads from its start state) from the step's end to its start, with the step's
-discrete decisions held at the forward run's.  ``rollout_adjoint`` drives a whole
+(per particle d/d(x, y, vx, vy) and the front and back velocity buffers' d/d(u, v))
+from the step's end to its start, with the step's discrete decisions held at the
+forward run's.  A PIC / FLIP step reads only the back buffer of its start state, a
+semi-Lagrangian step only the front.  ``rollout_adjoint`` drives a whole
 run: checkpoints every ``every`` steps in the forward pass; in the reverse pass each
 segment is recomputed from its checkpoint, its per-step states kept, and the steps'
 adjoints applied from the last.
@@ -40,8 +41,9 @@ def reverse_pass(dom: "fsg.Domain", scheme: int, dt: float, steps: int, every: i
     ``seed_fn(dom)`` (it sets the adjoint state, e.g. through
     ``adjoint_seed_density``, and returns the loss), then recompute each segment from
     its checkpoint, keep its per-step states and apply the steps' adjoints from the
-    last.  Returns (loss, pbar (n, 4), bbar_u, bbar_v) at the initial state; the
-    domain is left at the final state."""
+    last.  Returns (loss, pbar (n, 4), gu, gv) at the initial state, gu / gv the
+    adjoint of the velocity arrays ``set_state`` writes to both buffers (the back and
+    the front buffer's adjoints summed); the domain is left at the final state."""
     every = max(1, int(every))
     nseg = math.ceil(steps / every) if steps > 0 else 0
     final_slot = nseg + every
@@ -59,8 +61,9 @@ def reverse_pass(dom: "fsg.Domain", scheme: int, dt: float, steps: int, every: i
             dom.checkpoint_load(nseg + (s - s0))
             dom.adjoint_step(scheme, dt)
     pbar, bu, bv = dom.adjoint_get()
+    fu, fv = dom.adjoint_get_front()
     dom.checkpoint_load(final_slot)
-    return loss, pbar, bu, bv
+    return loss, pbar, bu + fu, bv + fv
 
 
 def rollout_adjoint(dom: "fsg.Domain", scheme: int, dt: float, steps: int, every: int,
@@ -103,8 +106,9 @@ def state_of(dom: "fsg.Domain"):
 
 
 def set_state(dom: "fsg.Domain", particles, back_u, back_v):
-    """The carried state a step reads: particles and the back buffer (the front is
-    overwritten before it is read; it is set to the back values)."""
+    """The carried state a step reads: particles and one velocity field, written to
+    both buffers (a PIC / FLIP step reads the back buffer, a semi-Lagrangian step the
+    front)."""
     dom.set_particles(np.asarray(particles, np.float32))
     dom.set_velocity(back_u, back_v, fsg.VEL_BACK)
     dom.set_velocity(back_u, back_v, fsg.VEL_FRONT)
@@ -115,7 +119,8 @@ try:  # torch is plumbing here: the wrapper is defined only where torch imports
 
     class Rollout(torch.autograd.Function):
         """loss = sum((density(final particles) - target)^2) after ``steps`` steps from
-        (positions, velocities, back u, back v); gradients by the adjoint kernels."""
+        (positions, velocities, grid u, grid v) (the grid written to both buffers);
+        gradients by the adjoint kernels."""
 
         @staticmethod
         def forward(ctx, pos, vel, back_u, back_v, dom, scheme, dt, steps, every, target):

@@ -132,6 +132,8 @@ struct AdjState {
   float4* pbar = nullptr;  // per particle: (x bar, y bar, vx bar, vy bar)
   float4* x0 = nullptr;    // tape: the particles at the step's start
   float *bu = nullptr, *bv = nullptr;  // adjoint of the back buffer
+  float *fu = nullptr, *fv = nullptr;  // adjoint of the front buffer
+  float *f0u = nullptr, *f0v = nullptr;  // tape: the front at a semi-Lagrangian step's start
   float *su = nullptr, *sv = nullptr, *eu = nullptr, *ev = nullptr, *e2u = nullptr,
         *e2v = nullptr, *g4u = nullptr, *g4v = nullptr, *g1u = nullptr, *g1v = nullptr,
         *qu = nullptr, *qv = nullptr;
@@ -848,16 +850,25 @@ int step_flip_family(fsg_domain* h, int scheme, float dt) {
   return FSG_OK;
 }
 
+// advectVelocitySemiLagrangian (src/FluidSolver.cpp:709-772): back := 0, then the
+// scatter (fixed-point sums written whole to the back buffer when deterministic)
+void ph_sl_scatter(fsg_domain* h, float dt) {
+  long long* fx = h->prm.deterministic ? h->acc_fx : nullptr;
+  if (!fx)
+    timed(h, "fill", [&] {
+      launch_fill(h->stream, UB(h), (int64_t)ncell(h), 0.f);
+      launch_fill(h->stream, VB(h), (int64_t)ncell(h), 0.f);
+    });
+  timed(h, "sl_scatter", [&] {
+    launch_sl_scatter(h->stream, h->g, h->label, UF(h), VF(h), UB(h), VB(h), h->prm.integrator, dt,
+                      fx);
+  });
+}
+
 int step_semi_lagrangian(fsg_domain* h, float dt) {
   int rc = ph_classify(h);
   if (rc) return rc;
-  timed(h, "fill", [&] {
-    launch_fill(h->stream, UB(h), (int64_t)ncell(h), 0.f);
-    launch_fill(h->stream, VB(h), (int64_t)ncell(h), 0.f);
-  });
-  timed(h, "sl_scatter", [&] {
-    launch_sl_scatter(h->stream, h->g, h->label, UF(h), VF(h), UB(h), VB(h), h->prm.integrator, dt);
-  });
+  ph_sl_scatter(h, dt);
   face_update(h, FU_GRAVITY | FU_DIRICHLET, dt, false);
   rc = ph_pressure(h, dt, true, false);
   if (rc) return rc;
@@ -1398,14 +1409,7 @@ int fsg_run_phase(fsg_domain* h, int32_t phase, float dt) {
       break;
     }
     case FSG_PH_SL_SCATTER:
-      timed(h, "fill", [&] {
-        launch_fill(h->stream, UB(h), (int64_t)ncell(h), 0.f);
-        launch_fill(h->stream, VB(h), (int64_t)ncell(h), 0.f);
-      });
-      timed(h, "sl_scatter", [&] {
-        launch_sl_scatter(h->stream, h->g, h->label, UF(h), VF(h), UB(h), VB(h), h->prm.integrator,
-                          dt);
-      });
+      ph_sl_scatter(h, dt);
       break;
     case FSG_PH_SL_GATHER:
       timed(h, "sl_gather", [&] {
@@ -2059,8 +2063,8 @@ static int adj_prepare(fsg_domain* h) {
   AdjState& a = h->adj;
   const size_t C = ncell(h);
   if (!a.grids) {
-    float** grids[] = {&a.bu, &a.bv, &a.su, &a.sv, &a.eu, &a.ev, &a.e2u, &a.e2v,
-                       &a.g4u, &a.g4v, &a.g1u, &a.g1v, &a.qu, &a.qv};
+    float** grids[] = {&a.bu, &a.bv, &a.fu, &a.fv, &a.f0u, &a.f0v, &a.su, &a.sv, &a.eu,
+                       &a.ev, &a.e2u, &a.e2v, &a.g4u, &a.g4v, &a.g1u, &a.g1v, &a.qu, &a.qv};
     for (float** g : grids) HIPCHK(h, dalloc(h, g, C));
     HIPCHK(h, dalloc(h, &a.y, C));
     HIPCHK(h, dalloc(h, &a.zmask, C));
@@ -2113,6 +2117,35 @@ int fsg_adjoint_set(fsg_domain* h, const float* pbar_xyuv, const float* bbar_u,
     HIPCHK(h, hipMemcpyAsync(a.bv, bbar_v, C * 4, hipMemcpyHostToDevice, h->stream));
   else
     launch_fill(h->stream, a.bv, (int64_t)C, 0.f);
+  launch_fill(h->stream, a.fu, (int64_t)C, 0.f);
+  launch_fill(h->stream, a.fv, (int64_t)C, 0.f);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+int fsg_adjoint_set_front(fsg_domain* h, const float* fbar_u, const float* fbar_v) {
+  if (!h || h->batch) return FSG_E_INVALID;
+  if (int rc = adj_prepare(h)) return rc;
+  AdjState& a = h->adj;
+  const size_t C = ncell(h);
+  if (fbar_u)
+    HIPCHK(h, hipMemcpyAsync(a.fu, fbar_u, C * 4, hipMemcpyHostToDevice, h->stream));
+  else
+    launch_fill(h->stream, a.fu, (int64_t)C, 0.f);
+  if (fbar_v)
+    HIPCHK(h, hipMemcpyAsync(a.fv, fbar_v, C * 4, hipMemcpyHostToDevice, h->stream));
+  else
+    launch_fill(h->stream, a.fv, (int64_t)C, 0.f);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSG_OK;
+}
+
+int fsg_adjoint_get_front(fsg_domain* h, float* fbar_u, float* fbar_v) {
+  if (!h || h->batch || !h->adj.grids) return FSG_E_INVALID;
+  const AdjState& a = h->adj;
+  const size_t C = ncell(h);
+  if (fbar_u) HIPCHK(h, hipMemcpyAsync(fbar_u, a.fu, C * 4, hipMemcpyDeviceToHost, h->stream));
+  if (fbar_v) HIPCHK(h, hipMemcpyAsync(fbar_v, a.fv, C * 4, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return FSG_OK;
 }
@@ -2151,19 +2184,76 @@ int fsg_adjoint_seed_density(fsg_domain* h, const float* rho_bar) {
   const size_t C = ncell(h);
   HIPCHK(h, hipMemcpyAsync(a.su, rho_bar, C * 4, hipMemcpyHostToDevice, h->stream));
   launch_density_adj(h->stream, h->g, h->pt, h->np, a.su, a.pbar);
-  launch_fill(h->stream, a.bu, (int64_t)C, 0.f);
-  launch_fill(h->stream, a.bv, (int64_t)C, 0.f);
+  for (float* z : {a.bu, a.bv, a.fu, a.fv}) launch_fill(h->stream, z, (int64_t)C, 0.f);
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return FSG_OK;
 }
 
+// The semi-Lagrangian step's adjoint (the particles' start copy already taped): the
+// front at the step's start taped, the step run with the second Dirichlet pass's
+// decisions recorded, then in reverse: the particles' advection through the new front,
+// the pressure update and its solve (as for the PIC / FLIP family, with no FLIP
+// difference), gravity + the first Dirichlet pass and the velocity scatter.  The
+// front's adjoint is the carried grid state; the back is zeroed before the step reads
+// it, so its adjoint at the start is zero.
+static int adjoint_step_sl(fsg_domain* h, float dt) {
+  AdjState& a = h->adj;
+  const int64_t np = h->np;
+  const size_t C = ncell(h);
+  HIPCHK(h, hipMemcpyAsync(a.f0u, UF(h), C * 4, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(a.f0v, VF(h), C * 4, hipMemcpyDeviceToDevice, h->stream));
+  a.taping = true;
+  int rc = step_semi_lagrangian(h, dt);
+  a.taping = false;
+  if (rc) return rc;
+  if (h->last_F == 0)
+    return fail(h, FSG_E_INVALID,
+                "fsg_adjoint_step: no LIQUID cell (the no-fluid step has no adjoint here)");
+  const fsg_stats fwd = h->host_stats;
+  const float dt_over_density = dt / h->density;
+  HIPCHK(h, hipMemcpyAsync(a.su, a.fu, C * 4, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(a.sv, a.fv, C * 4, hipMemcpyDeviceToDevice, h->stream));
+  timed(h, "adj_advect_grid", [&] {
+    launch_adj_advect_grid(h->stream, h->g, a.x0, np, UF(h), VF(h), h->prm.integrator, dt, a.pbar,
+                           a.su, a.sv);
+  });
+  // the pressure update: eu = the source's adjoint through the direct term, g4u = the
+  // scatter's (faces the update keeps)
+  timed(h, "adj_post", [&] {
+    launch_adj_post(h->stream, h->g, h->label, a.zmask, 1.0f, 0, a.su, a.sv, nullptr, nullptr,
+                    a.eu, a.ev, a.g4u, a.g4v, a.g1u, a.g1v);
+    launch_adj_pgrad(h->stream, h->g, h->label, a.eu, a.ev, dt_over_density, a.y);
+  });
+  rc = mg_solve(h, h->last_F, a.y);
+  if (rc) return rc;
+  a.last_iters = h->host_stats.cg_iterations;
+  h->host_stats = fwd;
+  // divergence adjoint, plus the back buffer after the step (the source itself)
+  timed(h, "adj_div", [&] {
+    if (h->prm.project_mean) launch_mg_remove_mean(h->stream, h->mg);
+    launch_adj_div(h->stream, h->g, h->mg.inf[0], h->mg.x[0], a.bu, a.bv, a.eu, a.ev);
+  });
+  timed(h, "adj_sl", [&] {
+    launch_adj_sl(h->stream, h->g, h->label, a.f0u, a.f0v, a.eu, a.ev, a.g4u, a.g4v,
+                  h->prm.gravity_x * dt, h->prm.gravity_y * dt, h->prm.closed_walls,
+                  h->prm.integrator, dt, a.fu, a.fv);
+    launch_fill(h->stream, a.bu, (int64_t)C, 0.f);
+    launch_fill(h->stream, a.bv, (int64_t)C, 0.f);
+  });
+  HIPCHK(h, hipGetLastError());
+  return FSG_OK;
+}
+
 // One step forward from the handle's state, taped, then its adjoint: the adjoint
-// state (particles, back buffer) is carried from the step's end to its start.
+// state (particles, front and back buffers) is carried from the step's end to its
+// start.
 int fsg_adjoint_step(fsg_domain* h, int32_t scheme, float dt) {
   if (!h) return FSG_E_INVALID;
   if (h->batch) return fail(h, FSG_E_INVALID, "fsg_adjoint_step: not available for a batch");
-  if (scheme != FSG_SCHEME_PIC && scheme != FSG_SCHEME_FLIP && scheme != FSG_SCHEME_PICFLIP)
-    return fail(h, FSG_E_INVALID, "fsg_adjoint_step: PIC, FLIP or PICFLIP only");
+  if (scheme != FSG_SCHEME_PIC && scheme != FSG_SCHEME_FLIP && scheme != FSG_SCHEME_PICFLIP &&
+      scheme != FSG_SCHEME_SEMI_LAGRANGIAN)
+    return fail(h, FSG_E_INVALID,
+                "fsg_adjoint_step: SEMI_LAGRANGIAN, PIC, FLIP or PICFLIP only");
   if (h->prm.solver != FSG_SOLVER_MG || h->prm.warm_start != 0)
     return fail(h, FSG_E_INVALID,
                 "fsg_adjoint_step: needs FSG_SOLVER_MG with warm_start = 0 (a replayable solve)");
@@ -2175,6 +2265,7 @@ int fsg_adjoint_step(fsg_domain* h, int32_t scheme, float dt) {
   if (np > 0)
     HIPCHK(h, hipMemcpyAsync(a.x0, h->pt, (size_t)np * sizeof(float4), hipMemcpyDeviceToDevice,
                              h->stream));
+  if (scheme == FSG_SCHEME_SEMI_LAGRANGIAN) return adjoint_step_sl(h, dt);
   a.taping = true;
   int rc = step_flip_family(h, scheme, dt);
   a.taping = false;
@@ -2195,8 +2286,10 @@ int fsg_adjoint_step(fsg_domain* h, int32_t scheme, float dt) {
   });
   // pressure update: direct term, pressure adjoint, adjoint solve, divergence adjoint
   timed(h, "adj_post", [&] {
-    launch_adj_post(h->stream, h->g, h->label, a.zmask, r, flip ? 1 : 0, a.su, a.sv, a.eu, a.ev,
-                    a.g4u, a.g4v, a.g1u, a.g1v);
+    // the front after the step (H') also carries the front adjoint of the step after
+    // it (non-zero when a semi-Lagrangian step follows)
+    launch_adj_post(h->stream, h->g, h->label, a.zmask, r, flip ? 1 : 0, a.su, a.sv, a.fu, a.fv,
+                    a.eu, a.ev, a.g4u, a.g4v, a.g1u, a.g1v);
     launch_adj_pgrad(h->stream, h->g, h->label, a.eu, a.ev, dt_over_density, a.y);
   });
   rc = mg_solve(h, h->last_F, a.y);
@@ -2227,6 +2320,9 @@ int fsg_adjoint_step(fsg_domain* h, int32_t scheme, float dt) {
                    h->u_prev, h->v_prev, h->acc, h->prm.gravity_x * dt, h->prm.gravity_y * dt,
                    h->prm.closed_walls, a.qu, a.qv, a.bu, a.bv);
     launch_adj_p2g(h->stream, h->g, a.x0, np, h->u_prev, h->v_prev, a.qu, a.qv, a.pbar);
+    // the step overwrites its start front before reading it
+    launch_fill(h->stream, a.fu, (int64_t)C, 0.f);
+    launch_fill(h->stream, a.fv, (int64_t)C, 0.f);
   });
   HIPCHK(h, hipGetLastError());
   return FSG_OK;  // stream-ordered; fsg_adjoint_get synchronises

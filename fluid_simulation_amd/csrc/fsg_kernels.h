@@ -101,8 +101,11 @@ void launch_particle_gather(hipStream_t s, const Geom& g, float4* pt, int64_t np
                             const float* v, const float* du, const float* dv, float pic_ratio,
                             int g2p, int move, float dt, const uint8_t* label, uint8_t* reject,
                             const BatchTables* batch = nullptr);
+// fx != null: the deterministic form (2 C fixed-point sums, zero on entry and on exit;
+// the back buffer is written whole, no zero fill needed)
 void launch_sl_scatter(hipStream_t s, const Geom& g, const uint8_t* label, const float* uf,
-                       const float* vf, float* ub, float* vb, int integrator, float dt);
+                       const float* vf, float* ub, float* vb, int integrator, float dt,
+                       long long* fx = nullptr);
 void launch_sl_gather(hipStream_t s, const Geom& g, const uint8_t* label, const float* uf,
                       const float* vf, float* ub, float* vb, int integrator, float dt);
 void launch_advect_grid(hipStream_t s, const Geom& g, float4* pt, int64_t np, const float* uf,
@@ -329,8 +332,9 @@ void launch_adj_g2p(hipStream_t s, const Geom& g, const float4* x0, int64_t np, 
                     const float* hv, const float* du, const float* dv, float r, float dt,
                     const uint8_t* reject, float4* pbar, float* su, float* sv);
 void launch_adj_post(hipStream_t s, const Geom& g, const uint8_t* label, const uint8_t* zmask,
-                     float r, int flip, const float* su, const float* sv, float* eu, float* ev,
-                     float* g4u, float* g4v, float* g1u, float* g1v);
+                     float r, int flip, const float* su, const float* sv, const float* fu,
+                     const float* fv, float* eu, float* ev, float* g4u, float* g4v, float* g1u,
+                     float* g1v);
 void launch_adj_pgrad(hipStream_t s, const Geom& g, const uint8_t* label, const float* eu,
                       const float* ev, float dt_over_density, double* pbar);
 void launch_adj_div(hipStream_t s, const Geom& g, const uint8_t* inf, const double* bbar,
@@ -354,6 +358,17 @@ void launch_density(hipStream_t s, const Geom& g, const float4* pt, int64_t np, 
                     long long* rho_fx = nullptr);
 void launch_density_adj(hipStream_t s, const Geom& g, const float4* pt, int64_t np,
                         const float* rb, float4* pbar);
+// the semi-Lagrangian step's adjoint: particle advection through the grid (pbar in /
+// out, the grid's adjoint added to su, sv), then gravity + the first Dirichlet pass and
+// the velocity scatter (the front's adjoint at the step's start into fu, fv from the
+// adjoint abu, abv of the pressure update's source and sbu, sbv of the scatter)
+void launch_adj_advect_grid(hipStream_t s, const Geom& g, const float4* x0, int64_t np,
+                            const float* uf, const float* vf, int integrator, float dt,
+                            float4* pbar, float* su, float* sv);
+void launch_adj_sl(hipStream_t s, const Geom& g, const uint8_t* label, const float* f0u,
+                   const float* f0v, const float* abu, const float* abv, const float* sbu,
+                   const float* sbv, float gx_dt, float gy_dt, int closed, int integrator,
+                   float dt, float* fu, float* fv);
 
 // a batch: every scene's own levels (labels / info once per solve, a cycle per iteration)
 void launch_mg_scene_setup(hipStream_t s, const SceneCoarseArgs& a);

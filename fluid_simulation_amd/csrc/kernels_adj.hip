@@ -173,12 +173,16 @@ __global__ void __launch_bounds__(kAT) k_adj_g2p(Geom g, const float4* __restric
 // ---------------------------------------------------------------- pressure update
 // H = E - dt/density grad p at union faces U (L(i,j) | L(i-1,j) | L(i,j-1) LIQUID,
 // clamped), G4 elsewhere; H' = Dirichlet(H) with the zeroed faces recorded in zmask
-// (bit 0 u, bit 1 v) by the taped forward.  In: s (adjoint of H').  Out: ebar =
-// adjoint of E through the direct term, g4bar, and g1bar = -(1-r) s from the diff.
+// (bit 0 u, bit 1 v) by the taped forward.  In: s (the G2P's adjoint of H', through
+// H' itself and the diff D = H' - G1) and f (optional: H''s adjoint as the front
+// after the step, read by a following semi-Lagrangian step).  Out: ebar = adjoint of
+// E through the direct term, g4bar, and g1bar = -(1-r) s from the diff.
 __global__ void __launch_bounds__(kAT) k_adj_post(Geom g, const uint8_t* __restrict__ label,
                                                   const uint8_t* __restrict__ zmask, float r,
                                                   int flip, const float* __restrict__ su,
                                                   const float* __restrict__ sv,
+                                                  const float* __restrict__ fu,
+                                                  const float* __restrict__ fv,
                                                   float* __restrict__ eu, float* __restrict__ ev,
                                                   float* __restrict__ g4u,
                                                   float* __restrict__ g4v,
@@ -191,8 +195,8 @@ __global__ void __launch_bounds__(kAT) k_adj_post(Geom g, const uint8_t* __restr
   g1u[c] = flip ? -(1.0f - r) * s_u : 0.f;
   g1v[c] = flip ? -(1.0f - r) * s_v : 0.f;
   const uint8_t z = zmask[c];
-  const float hbu = (z & 1) ? 0.f : s_u;
-  const float hbv = (z & 2) ? 0.f : s_v;
+  const float hbu = (z & 1) ? 0.f : s_u + (fu ? fu[c] : 0.f);
+  const float hbv = (z & 2) ? 0.f : s_v + (fv ? fv[c] : 0.f);
   const uint8_t L = label[c];
   const uint8_t Lim = label[clampi(i - 1, 0, g.nx - 1) + j * g.nx];
   const uint8_t Ljm = label[i + clampi(j - 1, 0, g.ny - 1) * g.nx];
@@ -469,6 +473,200 @@ __global__ void __launch_bounds__(kAT) k_density_adj(Geom g, const float4* __res
                         inv_or_div(dfj, g.dy, g.pow2_y, g.inv_dy), 0.f, 0.f);
 }
 
+// ---------------------------------------------------------------- semi-Lagrangian
+// FluidSolver::stepSemiLagrangian (src/FluidSolver.cpp:99-134) with its decisions
+// held (labels, both Dirichlet passes).  Names:
+//   F0      the front at the step's start (taped copy), the carried velocity state
+//   S       advectVelocitySemiLagrangian's scatter into the zeroed back (:709-772)
+//   A       F0 + gravity at LIQUID cells (:276-295), then the first Dirichlet pass:
+//           the pressure update's source and the back buffer after the step
+//   F1      the front after the step: Dirichlet(A - dt/density grad p) at faces next
+//           to LIQUID, Dirichlet(S) elsewhere (the pressure update keeps the back)
+//   X1      X0 + the ODE displacement through F1 (advectParticlesWithGrid, :774-791)
+// The particles' velocities are not read or written by the step: their adjoints pass.
+
+// Adjoint of one velocity sample f(x, y) = (u(x, y), v(x, y)) of VelField::at with
+// adjoint (kbx, kby): position adjoint added to (xbar, ybar), the transposed bilinear
+// samples added to the grids' adjoints (au, av) by atomics.
+__device__ __forceinline__ void vel_at_adj(const Geom& g, const float* __restrict__ u,
+                                           const float* __restrict__ v, float x, float y,
+                                           float kbx, float kby, float& xbar, float& ybar,
+                                           float* au, float* av) {
+  const SplatCorners cu = interp_corners(div_x(x, g), div_y(y - g.dy * 0.5f, g), g.nx, g.ny);
+  const SplatCorners cv = interp_corners(div_x(x - g.dx * 0.5f, g), div_y(y, g), g.nx, g.ny);
+  float ui, uj, vi, vj;
+  interp_grad(u, g.nx, cu, ui, uj);
+  interp_grad(v, g.nx, cv, vi, vj);
+  xbar += inv_or_div(kbx * ui + kby * vi, g.dx, g.pow2_x, g.inv_dx);
+  ybar += inv_or_div(kbx * uj + kby * vj, g.dy, g.pow2_y, g.inv_dy);
+  if (kbx != 0.f) splat_add(au, g.nx, cu, kbx);
+  if (kby != 0.f) splat_add(av, g.nx, cv, kby);
+}
+
+// Reverse of ode_disp (fsg_device.h): the displacement (ddx, ddy) of one ODE step
+// from (x, y) through (u, v) with adjoint (dbx, dby).  The stages are recomputed with
+// ode_disp's own float expressions, then taken in reverse order; each stage's sample
+// gives its position adjoint (carried to the stages it was formed from) and adds its
+// transposed sample to (au, av).  Returns the start position's adjoint.
+__device__ __forceinline__ void ode_disp_adj(const Geom& g, const float* __restrict__ u,
+                                             const float* __restrict__ v, int integrator,
+                                             float x, float y, float h, float dbx, float dby,
+                                             float& xbar, float& ybar, float* au, float* av) {
+  const VelField f{u, v, g};
+  xbar = 0.f;
+  ybar = 0.f;
+  if (integrator == kRK4) {
+    // f1 at x, f2 = f3 at x + h/2, f4 at x + h (scalar shifts): each stage is a
+    // function of x alone, weighted h/6, 4h/6, h/6
+    const float hh = h / 2.0f;
+    const float w1 = h / 6.0f, w2 = 4.0f * h / 6.0f;
+    vel_at_adj(g, u, v, x, y, w1 * dbx, w1 * dby, xbar, ybar, au, av);
+    vel_at_adj(g, u, v, x + hh, y + hh, w2 * dbx, w2 * dby, xbar, ybar, au, av);
+    vel_at_adj(g, u, v, x + h, y + h, w1 * dbx, w1 * dby, xbar, ybar, au, av);
+    return;
+  }
+  if (integrator == kEuler) {
+    vel_at_adj(g, u, v, x + h, y + h, h * dbx, h * dby, xbar, ybar, au, av);
+    return;
+  }
+  float k1x, k1y;
+  f.at(x, y, k1x, k1y);
+  if (integrator == kRK2) {
+    const float mx = x + k1x * h * 0.5f, my = y + k1y * h * 0.5f;
+    float mxb = 0.f, myb = 0.f;
+    vel_at_adj(g, u, v, mx, my, h * dbx, h * dby, mxb, myb, au, av);
+    // m = x + k1 h/2
+    xbar += mxb;
+    ybar += myb;
+    vel_at_adj(g, u, v, x, y, 0.5f * h * mxb, 0.5f * h * myb, xbar, ybar, au, av);
+    return;
+  }
+  // RK3: k2 at a = x + k1 h/2, k3 at b = x + k2 h 3/4, d = (2 k1 + 3 k2 + 4 k3) h/9
+  const float ax = x + k1x * h * 1.0f / 2.0f, ay = y + k1y * h * 1.0f / 2.0f;
+  float k2x, k2y;
+  f.at(ax, ay, k2x, k2y);
+  const float bx = x + k2x * h * 3.0f / 4.0f, by = y + k2y * h * 3.0f / 4.0f;
+  const float c = h / 9.0f;
+  float bxb = 0.f, byb = 0.f;
+  vel_at_adj(g, u, v, bx, by, 4.0f * c * dbx, 4.0f * c * dby, bxb, byb, au, av);
+  xbar += bxb;
+  ybar += byb;
+  const float k2bx = 3.0f * c * dbx + 0.75f * h * bxb, k2by = 3.0f * c * dby + 0.75f * h * byb;
+  float axb = 0.f, ayb = 0.f;
+  vel_at_adj(g, u, v, ax, ay, k2bx, k2by, axb, ayb, au, av);
+  xbar += axb;
+  ybar += ayb;
+  const float k1bx = 2.0f * c * dbx + 0.5f * h * axb, k1by = 2.0f * c * dby + 0.5f * h * ayb;
+  vel_at_adj(g, u, v, x, y, k1bx, k1by, xbar, ybar, au, av);
+}
+
+// advectParticlesWithGrid: X1 = X0 + disp(F1, X0, dt).  pbar (X1bar, V1bar) in,
+// (X0bar, V1bar) out; F1's adjoint from the samples added to (su, sv).
+__global__ void __launch_bounds__(kAT) k_adj_advect_grid(Geom g, const float4* __restrict__ x0,
+                                                         int64_t np, const float* __restrict__ uf,
+                                                         const float* __restrict__ vf,
+                                                         int integrator, float dt,
+                                                         float4* __restrict__ pbar,
+                                                         float* __restrict__ su,
+                                                         float* __restrict__ sv) {
+  const int64_t p = gtid();
+  if (p >= np) return;
+  const float4 q = x0[p];
+  const float4 a = pbar[p];
+  float xb, yb;
+  ode_disp_adj(g, uf, vf, integrator, q.x, q.y, dt, a.x, a.y, xb, yb, su, sv);
+  pbar[p] = make_float4(a.x + xb, a.y + yb, a.z, a.w);
+}
+
+// Gravity and the first Dirichlet pass: A = Z1(F0 + g dt at LIQUID cells), the zeroing
+// decisions recomputed from the taped F0 exactly as k_face_update takes them.
+// F0bar := Abar where the face was not zeroed (the scatter's terms are added after).
+__global__ void __launch_bounds__(kAT) k_adj_sl_dirichlet(Geom g, const uint8_t* __restrict__ label,
+                                                          const float* __restrict__ f0u,
+                                                          const float* __restrict__ f0v,
+                                                          const float* __restrict__ abu,
+                                                          const float* __restrict__ abv,
+                                                          float gx_dt, float gy_dt, int closed,
+                                                          float* __restrict__ fu,
+                                                          float* __restrict__ fv) {
+  const int64_t c = gtid();
+  if (c >= (int64_t)g.nx * g.ny) return;
+  const int i = (int)(c % g.nx), j = (int)(c / g.nx);
+  const uint8_t L = label[c];
+  const uint8_t Lim = label[clampi(i - 1, 0, g.nx - 1) + j * g.nx];
+  const uint8_t Ljm = label[i + clampi(j - 1, 0, g.ny - 1) * g.nx];
+  float uu = f0u[c], vv = f0v[c];
+  if (L == kLiquid) {
+    uu = uu + gx_dt;
+    vv = vv + gy_dt;
+  }
+  bool zu, zv;
+  if (closed) {
+    zu = Lim == kSolid || L == kSolid;
+    zv = Ljm == kSolid || L == kSolid;
+  } else {
+    zu = (Lim == kSolid && uu < 0.f) || (L == kSolid && uu > 0.f);
+    zv = (Ljm == kSolid && vv < 0.f) || (L == kSolid && vv > 0.f);
+  }
+  fu[c] = zu ? 0.f : abu[c];
+  fv[c] = zv ? 0.f : abv[c];
+}
+
+// The scatter S = sum over faces next to LIQUID of splat(b_f, val_f), b_f = x_f +
+// disp(F0, x_f, -dt), val_f = F0 sampled at x_f.  With Sbar: val_f's adjoint is
+// Sbar gathered with the splat's weights at b_f, b_f's is val_f times the weights'
+// derivative; both go back to F0 (the sample at x_f, the ODE stages of the trace)
+// into (fu, fv) by atomics.
+__global__ void __launch_bounds__(kAT) k_adj_sl_scatter(Geom g, const uint8_t* __restrict__ label,
+                                                        const float* __restrict__ f0u,
+                                                        const float* __restrict__ f0v,
+                                                        const float* __restrict__ sbu,
+                                                        const float* __restrict__ sbv,
+                                                        int integrator, float dt,
+                                                        float* __restrict__ fu,
+                                                        float* __restrict__ fv) {
+  const int64_t c = gtid();
+  if (c >= (int64_t)g.nx * g.ny) return;
+  const int i = (int)(c % g.nx), j = (int)(c / g.nx);
+  const int nx = g.nx, ny = g.ny;
+  const uint8_t L = label[c];
+  const uint8_t Lim = label[clampi(i - 1, 0, nx - 1) + j * nx];
+  const uint8_t Ljm = label[i + clampi(j - 1, 0, ny - 1) * nx];
+  const VelField f{f0u, f0v, g};
+  for (int comp = 0; comp < 2; ++comp) {
+    const bool on = comp == 0 ? (L == kLiquid || Lim == kLiquid) : (L == kLiquid || Ljm == kLiquid);
+    if (!on) continue;
+    const float xp = comp == 0 ? (float)i * g.dx : ((float)i + 0.5f) * g.dx;
+    const float yp = comp == 0 ? ((float)j + 0.5f) * g.dy : (float)j * g.dy;
+    float ddx, ddy;
+    ode_disp(f, integrator, xp, yp, -dt, ddx, ddy);
+    const float xb = xp + ddx, yb = yp + ddy;
+    const float* sb = comp == 0 ? sbu : sbv;
+    const SplatCorners s = comp == 0
+        ? splat_corners(div_x(xb, g), div_y(yb - g.dy * 0.5f, g), nx, ny)
+        : splat_corners(div_x(xb - g.dx * 0.5f, g), div_y(yb, g), nx, ny);
+    const float s00 = sb[s.i + s.j * nx], s10 = sb[s.i1 + s.j * nx];
+    const float s01 = sb[s.i + s.j1 * nx], s11 = sb[s.i1 + s.j1 * nx];
+    const float valbar = (1.0f - s.fj) * ((1.0f - s.fi) * s00 + s.fi * s10) +
+                         s.fj * ((1.0f - s.fi) * s01 + s.fi * s11);
+    float dfi, dfj;
+    interp_grad(sb, nx, s, dfi, dfj);
+    const float val = comp == 0 ? vel_x_at(f0u, g, xp, yp) : vel_y_at(f0v, g, xp, yp);
+    const float bbx = inv_or_div(val * dfi, g.dx, g.pow2_x, g.inv_dx);
+    const float bby = inv_or_div(val * dfj, g.dy, g.pow2_y, g.inv_dy);
+    if (valbar != 0.f) {
+      const SplatCorners o = comp == 0
+          ? interp_corners(div_x(xp, g), div_y(yp - g.dy * 0.5f, g), nx, ny)
+          : interp_corners(div_x(xp - g.dx * 0.5f, g), div_y(yp, g), nx, ny);
+      splat_add(comp == 0 ? fu : fv, nx, o, valbar);
+    }
+    if (bbx != 0.f || bby != 0.f) {
+      float xs, ys;  // the trace starts at the fixed face position: its adjoint is dropped
+      ode_disp_adj(g, f0u, f0v, integrator, xp, yp, -dt, bbx, bby, xs, ys, fu, fv);
+    }
+  }
+}
+
 // ---------------------------------------------------------------- launchers
 static inline unsigned nblk(int64_t n) { return (unsigned)((n + kAT - 1) / kAT); }
 
@@ -480,10 +678,11 @@ void launch_adj_g2p(hipStream_t s, const Geom& g, const float4* x0, int64_t np, 
 }
 
 void launch_adj_post(hipStream_t s, const Geom& g, const uint8_t* label, const uint8_t* zmask,
-                     float r, int flip, const float* su, const float* sv, float* eu, float* ev,
-                     float* g4u, float* g4v, float* g1u, float* g1v) {
-  k_adj_post<<<nblk((int64_t)g.nx * g.ny), kAT, 0, s>>>(g, label, zmask, r, flip, su, sv, eu, ev,
-                                                        g4u, g4v, g1u, g1v);
+                     float r, int flip, const float* su, const float* sv, const float* fu,
+                     const float* fv, float* eu, float* ev, float* g4u, float* g4v, float* g1u,
+                     float* g1v) {
+  k_adj_post<<<nblk((int64_t)g.nx * g.ny), kAT, 0, s>>>(g, label, zmask, r, flip, su, sv, fu, fv,
+                                                        eu, ev, g4u, g4v, g1u, g1v);
 }
 
 void launch_adj_pgrad(hipStream_t s, const Geom& g, const uint8_t* label, const float* eu,
@@ -541,6 +740,23 @@ void launch_density(hipStream_t s, const Geom& g, const float4* pt, int64_t np, 
 void launch_density_adj(hipStream_t s, const Geom& g, const float4* pt, int64_t np,
                         const float* rb, float4* pbar) {
   if (np > 0) k_density_adj<<<nblk(np), kAT, 0, s>>>(g, pt, np, rb, pbar);
+}
+
+void launch_adj_advect_grid(hipStream_t s, const Geom& g, const float4* x0, int64_t np,
+                            const float* uf, const float* vf, int integrator, float dt,
+                            float4* pbar, float* su, float* sv) {
+  if (np > 0)
+    k_adj_advect_grid<<<nblk(np), kAT, 0, s>>>(g, x0, np, uf, vf, integrator, dt, pbar, su, sv);
+}
+
+void launch_adj_sl(hipStream_t s, const Geom& g, const uint8_t* label, const float* f0u,
+                   const float* f0v, const float* abu, const float* abv, const float* sbu,
+                   const float* sbv, float gx_dt, float gy_dt, int closed, int integrator,
+                   float dt, float* fu, float* fv) {
+  const unsigned nb = nblk((int64_t)g.nx * g.ny);
+  k_adj_sl_dirichlet<<<nb, kAT, 0, s>>>(g, label, f0u, f0v, abu, abv, gx_dt, gy_dt, closed, fu,
+                                        fv);
+  k_adj_sl_scatter<<<nb, kAT, 0, s>>>(g, label, f0u, f0v, sbu, sbv, integrator, dt, fu, fv);
 }
 
 }  // namespace fsg

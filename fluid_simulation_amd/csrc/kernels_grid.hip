@@ -680,10 +680,30 @@ __global__ void __launch_bounds__(256) k_particle_gather(
 // 709-772): for each face next to LIQUID, backtrace its position by -dt through the
 // front field, sample the face's own velocity at its own position and splat it
 // into the back buffer (zeroed beforehand) at the backtraced point.  No swap.
+// FX (fsg_params.deterministic): the terms are summed as 2^-40 fixed point in fx
+// (u sums at [0, C), v sums at [C, 2C)), converted to the back buffer afterwards, so
+// the sums do not depend on the order of the atomics.
+template <bool FX>
+__device__ __forceinline__ void sl_splat(float* g, unsigned long long* fx, int nx, int ny,
+                                         float xs, float ys, float value) {
+  if (!FX) {
+    splat_scaled(g, nx, ny, xs, ys, value);
+    return;
+  }
+  const SplatCorners s = splat_corners(xs, ys, nx, ny);
+  const float v0 = (1.0f - s.fj) * value, v1 = s.fj * value;
+  atomicAdd(fx + s.i + s.j * nx, (unsigned long long)fx_term((1.0f - s.fi) * v0));
+  atomicAdd(fx + s.i1 + s.j * nx, (unsigned long long)fx_term(s.fi * v0));
+  atomicAdd(fx + s.i + s.j1 * nx, (unsigned long long)fx_term((1.0f - s.fi) * v1));
+  atomicAdd(fx + s.i1 + s.j1 * nx, (unsigned long long)fx_term(s.fi * v1));
+}
+
+template <bool FX>
 __global__ void __launch_bounds__(256) k_sl_scatter(Geom g, const uint8_t* __restrict__ label,
                                                     const float* __restrict__ uf,
                                                     const float* __restrict__ vf,
                                                     float* __restrict__ ub, float* __restrict__ vb,
+                                                    unsigned long long* __restrict__ fx,
                                                     int integrator, float dt) {
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int c = t * 256 + threadIdx.x;
@@ -700,7 +720,7 @@ __global__ void __launch_bounds__(256) k_sl_scatter(Geom g, const uint8_t* __res
     ode_disp(f, integrator, xp, yp, -dt, ddx, ddy);
     const float xb = xp + ddx, yb = yp + ddy;
     const float val = vel_x_at(uf, g, xp, yp);
-    splat_scaled(ub, g.nx, g.ny, div_x(xb, g), div_y(yb - g.dy * 0.5f, g), val);
+    sl_splat<FX>(ub, fx, g.nx, g.ny, div_x(xb, g), div_y(yb - g.dy * 0.5f, g), val);
   }
   if (L == kLiquid || Ljm == kLiquid) {
     const float xp = ((float)i + 0.5f) * g.dx;
@@ -709,7 +729,8 @@ __global__ void __launch_bounds__(256) k_sl_scatter(Geom g, const uint8_t* __res
     ode_disp(f, integrator, xp, yp, -dt, ddx, ddy);
     const float xb = xp + ddx, yb = yp + ddy;
     const float val = vel_y_at(vf, g, xp, yp);
-    splat_scaled(vb, g.nx, g.ny, div_x(xb - g.dx * 0.5f, g), div_y(yb, g), val);
+    sl_splat<FX>(vb, fx ? fx + (int64_t)g.nx * g.ny : nullptr, g.nx, g.ny,
+                 div_x(xb - g.dx * 0.5f, g), div_y(yb, g), val);
   }
 }
 
@@ -1045,9 +1066,18 @@ void launch_particle_gather(hipStream_t s, const Geom& g, float4* pt, int64_t np
                                                        move, dt, label, reject, BatchTables{});
 }
 void launch_sl_scatter(hipStream_t s, const Geom& g, const uint8_t* label, const float* uf,
-                       const float* vf, float* ub, float* vb, int integrator, float dt) {
-  k_sl_scatter<<<nblocks((int64_t)g.nx * g.ny), 256, 0, s>>>(g, label, uf, vf, ub, vb,
-                                                              integrator, dt);
+                       const float* vf, float* ub, float* vb, int integrator, float dt,
+                       long long* fx) {
+  const int64_t C = (int64_t)g.nx * g.ny;
+  if (fx) {
+    k_sl_scatter<true><<<nblocks(C), 256, 0, s>>>(
+        g, label, uf, vf, ub, vb, reinterpret_cast<unsigned long long*>(fx), integrator, dt);
+    launch_fx_to_float(s, C, fx, ub);
+    launch_fx_to_float(s, C, fx + C, vb);
+    return;
+  }
+  k_sl_scatter<false><<<nblocks(C), 256, 0, s>>>(g, label, uf, vf, ub, vb, nullptr, integrator,
+                                                 dt);
 }
 void launch_sl_gather(hipStream_t s, const Geom& g, const uint8_t* label, const float* uf,
                       const float* vf, float* ub, float* vb, int integrator, float dt) {

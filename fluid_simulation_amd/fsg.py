@@ -64,6 +64,7 @@ EXPORTS = [
     "fsg_batch_get_labels", "fsg_checkpoint_save", "fsg_checkpoint_load",
     "fsg_checkpoint_clear", "fsg_adjoint_set", "fsg_adjoint_get", "fsg_adjoint_step",
     "fsg_adjoint_last_iterations", "fsg_particle_density", "fsg_adjoint_seed_density",
+    "fsg_adjoint_set_front", "fsg_adjoint_get_front",
     "fsg_set_kernel_clock", "fsg_get_kernel_clock", "fsg_get_step_decisions",
     "fsg_p2g_noalloc", "fsg_autotune",
 ]
@@ -180,6 +181,8 @@ def load(path: str | None = None):
             "fsg_adjoint_set": ([H, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
             "fsg_adjoint_get": ([H, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
             "fsg_adjoint_step": ([H, C.c_int32, C.c_float], C.c_int),
+            "fsg_adjoint_set_front": ([H, C.c_void_p, C.c_void_p], C.c_int),
+            "fsg_adjoint_get_front": ([H, C.c_void_p, C.c_void_p], C.c_int),
             "fsg_adjoint_last_iterations": ([H, P(C.c_int32)], C.c_int),
             "fsg_particle_density": ([H, C.c_void_p], C.c_int),
             "fsg_adjoint_seed_density": ([H, C.c_void_p], C.c_int),
@@ -518,6 +521,22 @@ class Domain:
         self._chk(self._L.fsg_adjoint_get(self._h, pbar.ctypes.data if n else None,
                                           bu.ctypes.data, bv.ctypes.data), "fsg_adjoint_get")
         return pbar, bu, bv
+
+    def adjoint_set_front(self, fbar_u=None, fbar_v=None):
+        """The front buffer's adjoint, (ny, nx) u and v (None = zeros)."""
+        keep = [None if a is None else
+                np.ascontiguousarray(a, np.float32).reshape(self.ny, self.nx)
+                for a in (fbar_u, fbar_v)]
+        self._chk(self._L.fsg_adjoint_set_front(
+            self._h, *(None if a is None else a.ctypes.data for a in keep)),
+            "fsg_adjoint_set_front")
+
+    def adjoint_get_front(self):
+        fu = np.zeros((self.ny, self.nx), np.float32)
+        fv = np.zeros((self.ny, self.nx), np.float32)
+        self._chk(self._L.fsg_adjoint_get_front(self._h, fu.ctypes.data, fv.ctypes.data),
+                  "fsg_adjoint_get_front")
+        return fu, fv
 
     def adjoint_step(self, scheme: int, dt: float):
         self._chk(self._L.fsg_adjoint_step(self._h, scheme, dt), "fsg_adjoint_step")

@@ -287,32 +287,39 @@ int fsg_batch_get_velocity(fsg_domain *h, int32_t scene, int32_t which, float *u
 int fsg_batch_get_labels(fsg_domain *h, int32_t scene, uint8_t *labels);
 
 /* ---- checkpoints and reverse mode (no counterpart in the reference) -----------
-   The carried state of the PIC / FLIP family is the particles plus the front and
-   back velocity buffers; a checkpoint slot (0 <= slot < 65536) keeps a device copy.
-   The reverse pass of a run: for each step from the last, load (or recompute) the
-   state at the step's start and call fsg_adjoint_step, which steps forward once
-   with a tape and carries the adjoint state (per particle d/d(x, y, vx, vy); the
-   back buffer's d/d(u, v): the only grid a step reads from its start state) from
-   the step's end to its start.  Discrete decisions (labels, weight tests,
-   Dirichlet, extension masks, obstacle rejection) are held at the forward run's;
-   the adjoint of the pressure projection is a second multigrid-preconditioned
-   solve on the same operator.  Needs FSG_SOLVER_MG with warm_start = 0 and at least
-   one LIQUID cell. */
+   The carried state of a step is the particles plus the front and back velocity
+   buffers; a checkpoint slot (0 <= slot < 65536) keeps a device copy.  The reverse
+   pass of a run: for each step from the last, load (or recompute) the state at the
+   step's start and call fsg_adjoint_step, which steps forward once with a tape and
+   carries the adjoint state (per particle d/d(x, y, vx, vy); the front and back
+   buffers' d/d(u, v)) from the step's end to its start.  A PIC / FLIP / PICFLIP step
+   reads only the back buffer of its start state (the front is overwritten first), a
+   semi-Lagrangian step (FluidSolver::stepSemiLagrangian, src/FluidSolver.cpp:99-134)
+   only the front (the back is zeroed first): the other buffer's adjoint is zero at
+   the step's start.  Discrete decisions (labels, weight tests, Dirichlet, extension
+   masks, obstacle rejection) are held at the forward run's; the adjoint of the
+   pressure projection is a second multigrid-preconditioned solve on the same
+   operator.  Needs FSG_SOLVER_MG with warm_start = 0 and at least one LIQUID cell. */
 int fsg_checkpoint_save(fsg_domain *h, int32_t slot);
 int fsg_checkpoint_load(fsg_domain *h, int32_t slot);
 int fsg_checkpoint_clear(fsg_domain *h);
 /* adjoint state: float[4*n] per particle (x, y, vx, vy) and the back buffer's
-   (ny x nx) u and v; null arguments set zeros / are skipped */
+   (ny x nx) u and v; null arguments set zeros / are skipped.  fsg_adjoint_set also
+   zeroes the front buffer's adjoint. */
 int fsg_adjoint_set(fsg_domain *h, const float *pbar_xyuv, const float *bbar_u,
                     const float *bbar_v);
 int fsg_adjoint_get(fsg_domain *h, float *pbar_xyuv, float *bbar_u, float *bbar_v);
+/* the front buffer's adjoint (ny x nx u and v); null arguments set zeros / are skipped */
+int fsg_adjoint_set_front(fsg_domain *h, const float *fbar_u, const float *fbar_v);
+int fsg_adjoint_get_front(fsg_domain *h, float *fbar_u, float *fbar_v);
 int fsg_adjoint_step(fsg_domain *h, int32_t scheme, float dt);
 /* iterations of the last adjoint pressure solve */
 int fsg_adjoint_last_iterations(const fsg_domain *h, int32_t *iters);
 /* particle density: every particle splats weight 1 at (x/dx, y/dy) with
    Grid::addToValueInterpolated's weights (include/Grid.h:152-184), ny x nx */
 int fsg_particle_density(fsg_domain *h, float *rho);
-/* adjoint state := (d density / d positions)^T rho_bar, zero elsewhere */
+/* adjoint state := (d density / d positions)^T rho_bar, zero elsewhere (both
+   buffers' adjoints included) */
 int fsg_adjoint_seed_density(fsg_domain *h, const float *rho_bar);
 
 /* ---- timing --------------------------------------------------------------- */

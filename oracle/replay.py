@@ -8,7 +8,8 @@ particle state and the back buffer: its discrete decisions (the labels of
 classifyCells, src/FluidDomain.cpp:150-180; the P2G weight test,
 src/FluidSolver.cpp:898-917; the two enforceDirichlet passes, :297-321; the obstacle
 rejection of advectAndEnsureOutsideObstacles, src/MarkerParticleSet.cpp:48-62)
-select the smooth piece.  The adjoint kernels differentiate the piece the GPU run
+select the smooth piece.  A semi-Lagrangian step (:99-134) is one of the particles
+and the front buffer, its decisions the labels and the two Dirichlet passes.  The adjoint kernels differentiate the piece the GPU run
 took.  An FP64 rollout that takes the same decisions stays on that piece, so its
 central differences are the derivative the adjoint must match, even where an FP64
 rollout of its own would take a different branch within a few steps.
@@ -54,8 +55,8 @@ class ReplayRollout:
 
     def run(self, particles, back_u, back_v, scheme, dt):
         """The final oracle state after the replayed steps from (particles (P, 4),
-        back u, back v); the front buffer starts equal to the back one (a step
-        overwrites the front before reading it)."""
+        back u, back v); the front buffer starts equal to the back one (a PIC / FLIP
+        step reads the back, a semi-Lagrangian step the front)."""
         o = OracleSim(*self.geom, precision="f64")
         o.set_params(**self.params)
         o.set_particles(np.asarray(particles, np.float64))

@@ -32,7 +32,10 @@ class MockDomain:
         self.t = self.slots[slot]
 
     def adjoint_get(self):
-        return np.zeros((1, 4), np.float32), np.zeros((2, 2)), np.zeros((2, 2))
+        return np.zeros((1, 4), np.float32), np.full((2, 2), 1.0), np.full((2, 2), 2.0)
+
+    def adjoint_get_front(self):
+        return np.full((2, 2), 10.0), np.full((2, 2), 20.0)
 
 
 @pytest.mark.parametrize("steps,every", [(1, 1), (5, 2), (8, 8), (64, 8), (7, 3), (6, 10)])
@@ -44,8 +47,10 @@ def test_reverse_pass_visits_every_step_once_from_the_last(steps, every):
         seen["at"] = d.t
         return 1.25
 
-    loss, *_ = adjoint.rollout_adjoint(dom, 3, 0.01, steps, every, seed)
+    loss, _, gu, gv = adjoint.rollout_adjoint(dom, 3, 0.01, steps, every, seed)
     assert loss == 1.25
+    # the velocity arrays set_state writes to both buffers: both adjoints summed
+    assert np.all(gu == 11.0) and np.all(gv == 22.0)
     assert seen["at"] == steps                     # seeded at the final state
     assert dom.adjoint_at == list(reversed(range(steps)))
     assert dom.t == steps                          # left at the final state

@@ -1,9 +1,12 @@
-"""Reverse mode (fsg_adjoint_step, fsg_checkpoint_*): the GPU adjoint of PIC / FLIP /
-PICFLIP steps (src/FluidSolver.cpp:136-251) against directional derivatives of the
-FP64 oracle by central finite differences (eps = 1e-6 in FP64, where the step is
-smooth: no label, weight-test, Dirichlet, extension or rejection decision changes
-between the two runs), and the torch.autograd wrapper end to end against FP64
-differences of the same control -> rollout -> loss pipeline run by the oracle."""
+"""Reverse mode (fsg_adjoint_step, fsg_checkpoint_*): the GPU adjoint of semi-Lagrangian,
+PIC, FLIP and PICFLIP steps (src/FluidSolver.cpp:99-251) against directional
+derivatives of the FP64 oracle by central finite differences (eps = 1e-6 in FP64,
+where the step is smooth: no label, weight-test, Dirichlet, extension or rejection
+decision changes between the two runs), and the torch.autograd wrapper end to end
+against FP64 differences of the same control -> rollout -> loss pipeline run by the
+oracle.  The functional reads the final particles, their density and both velocity
+buffers, so a semi-Lagrangian step (which reads the front of its start state) and a
+PIC / FLIP step (which reads the back) are both seen through every output."""
 from __future__ import annotations
 
 import numpy as np
@@ -13,7 +16,7 @@ from fluid_simulation_amd import adjoint, fsg, scenes
 
 pytestmark = pytest.mark.gpu
 
-SCHEME_ORACLE = {fsg.PIC: 1, fsg.FLIP: 2, fsg.PICFLIP: 3}
+SCHEME_ORACLE = {fsg.SEMI_LAGRANGIAN: 0, fsg.PIC: 1, fsg.FLIP: 2, fsg.PICFLIP: 3}
 
 
 def density_np(parts, nx, ny, dx, dy):
@@ -59,7 +62,9 @@ def make_case(kind, seed):
     w = dict(rho=rng.standard_normal((sc.ny, sc.nx)),
              p=rng.standard_normal((len(parts), 4)),
              bu=rng.standard_normal((sc.ny, sc.nx)),
-             bv=rng.standard_normal((sc.ny, sc.nx)))
+             bv=rng.standard_normal((sc.ny, sc.nx)),
+             fu=rng.standard_normal((sc.ny, sc.nx)),
+             fv=rng.standard_normal((sc.ny, sc.nx)))
     return sc, params, parts, bu, bv, w
 
 
@@ -76,9 +81,10 @@ def oracle_functional(sc, params, parts, bu, bv, scheme, steps, w):
         labels.append(o.label.copy())
     P = np.asarray(o.particles, np.float64)
     BU, BV = (np.asarray(a, np.float64) for a in o.vel("back"))
+    FU, FV = (np.asarray(a, np.float64) for a in o.vel("front"))
     rho = density_np(P, sc.nx, sc.ny, sc.length_x / sc.nx, sc.length_y / sc.ny)
     val = (w["rho"] * rho).sum() + (w["p"] * P).sum() + (w["bu"] * BU).sum() + \
-        (w["bv"] * BV).sum()
+        (w["bv"] * BV).sum() + (w["fu"] * FU).sum() + (w["fv"] * FV).sum()
     return float(val), labels
 
 
@@ -91,6 +97,7 @@ def gpu_gradient(sc, params, parts, bu, bv, scheme, steps, every, w):
         dom.adjoint_seed_density(w["rho"].astype(np.float32))
         pb, _, _ = dom.adjoint_get()
         dom.adjoint_set(pb + w["p"].astype(np.float32), w["bu"], w["bv"])
+        dom.adjoint_set_front(w["fu"], w["fv"])
         return 0.0
 
     _, pbar, gbu, gbv = adjoint.rollout_adjoint(d, scheme, sc.dt, steps, every, seed)
@@ -103,6 +110,10 @@ def gpu_gradient(sc, params, parts, bu, bv, scheme, steps, every, w):
 # FP32 trajectory itself drifts from the FP64 one and the gradients, taken at slightly
 # different states, agree to ~5e-4 (the differentiable form's bar is 1e-3).
 @pytest.mark.parametrize("kind,scheme,steps,every,bar", [
+    ("dam", fsg.SEMI_LAGRANGIAN, 1, 1, 1e-4),
+    ("dam", fsg.SEMI_LAGRANGIAN, 3, 2, 1e-4),
+    ("box", fsg.SEMI_LAGRANGIAN, 1, 1, 1e-4),
+    ("boxair", fsg.SEMI_LAGRANGIAN, 3, 2, 1e-4),
     ("dam", fsg.PICFLIP, 1, 1, 1e-4),
     ("dam", fsg.FLIP, 1, 1, 1e-4),
     ("dam", fsg.PIC, 1, 1, 1e-4),
@@ -139,14 +150,95 @@ def test_adjoint_against_fp64_finite_differences(kind, scheme, steps, every, bar
     assert worst <= bar
 
 
+@pytest.mark.parametrize("integrator", [1, 2, 3], ids=["euler", "rk2", "rk4"])
+def test_semi_lagrangian_adjoint_other_integrators(integrator):
+    """The ODE steps' adjoints other than RK3 (FluidSolver::getAdvectedPosition with
+    include/OdeSolver.h's Euler and RK4, and the midpoint RK2), through both the
+    particles' advection and the velocity scatter's backtrace, on 2 steps."""
+    sc, params, parts, bu, bv, w = make_case("dam", 6)
+    params = dict(params, integrator=integrator)
+    print(f"semi-Lagrangian adjoint, integrator {integrator}", flush=True)
+    pbar, gbu, gbv, _ = gpu_gradient(sc, params, parts, bu, bv, fsg.SEMI_LAGRANGIAN, 2, 2, w)
+    rng = np.random.default_rng(12)
+    eps = 1e-6
+    for k in range(3):
+        dp = rng.standard_normal(parts.shape)
+        dbu = rng.standard_normal(bu.shape)
+        dbv = rng.standard_normal(bv.shape)
+        adj = float((pbar * dp).sum() + (gbu * dbu).sum() + (gbv * dbv).sum())
+        fp, lp = oracle_functional(sc, params, parts + eps * dp, bu + eps * dbu, bv + eps * dbv,
+                                   fsg.SEMI_LAGRANGIAN, 2, w)
+        fm, lm = oracle_functional(sc, params, parts - eps * dp, bu - eps * dbu, bv - eps * dbv,
+                                   fsg.SEMI_LAGRANGIAN, 2, w)
+        assert sum(int((a != b).sum()) for a, b in zip(lp, lm)) == 0
+        fd = (fp - fm) / (2 * eps)
+        rel = abs(adj - fd) / max(abs(fd), abs(adj), 1e-30)
+        print(f"direction {k}: adjoint {adj:.8e} fd {fd:.8e} rel {rel:.2e}")
+        assert rel <= 1e-4
+
+
+def test_mixed_schemes_carry_both_buffers():
+    """A rollout that alternates semi-Lagrangian and PICFLIP steps: the front buffer's
+    adjoint that a semi-Lagrangian step leaves is taken up by the PICFLIP step before
+    it (whose end front the semi-Lagrangian step reads), the back buffer's the other
+    way round."""
+    sc, params, parts, bu, bv, w = make_case("dam", 7)
+    schemes = [fsg.PICFLIP, fsg.SEMI_LAGRANGIAN, fsg.PICFLIP, fsg.SEMI_LAGRANGIAN]
+    d = fsg.Domain(sc.nx, sc.ny, sc.length_x, sc.length_y, sc.density, sc.pic_ratio)
+    d.set_params(solver=fsg.SOLVER_MG, cg_tol=1e-12, cg_max_iter=500, **params)
+    adjoint.set_state(d, parts, bu, bv)
+    for k, sch in enumerate(schemes):
+        d.checkpoint_save(k)
+        d.step(sch, sc.dt)
+    d.adjoint_seed_density(w["rho"].astype(np.float32))
+    pb, _, _ = d.adjoint_get()
+    d.adjoint_set(pb + w["p"].astype(np.float32), w["bu"], w["bv"])
+    d.adjoint_set_front(w["fu"], w["fv"])
+    for k in reversed(range(len(schemes))):
+        d.checkpoint_load(k)
+        d.adjoint_step(schemes[k], sc.dt)
+    pbar, gbu, gbv = (a.astype(np.float64) for a in d.adjoint_get())
+    fu, fv = d.adjoint_get_front()
+    gbu, gbv = gbu + fu, gbv + fv
+    d.close()
+
+    def functional(p, u, v):
+        from oracle import OracleSim
+        o = OracleSim(sc.nx, sc.ny, sc.length_x, sc.length_y, sc.density, sc.pic_ratio, "f64")
+        o.set_params(cg_tol=1e-14, cg_max_iter=100000, **params)
+        o.set_particles(p.astype(np.float64))
+        o.set_vel(u, v, "back")
+        o.set_vel(u, v, "front")
+        for sch in schemes:
+            o.step(SCHEME_ORACLE[sch], sc.dt)
+        P = np.asarray(o.particles, np.float64)
+        rho = density_np(P, sc.nx, sc.ny, sc.length_x / sc.nx, sc.length_y / sc.ny)
+        (BU, BV), (FU, FV) = o.vel("back"), o.vel("front")
+        return float((w["rho"] * rho).sum() + (w["p"] * P).sum() + (w["bu"] * BU).sum() +
+                     (w["bv"] * BV).sum() + (w["fu"] * FU).sum() + (w["fv"] * FV).sum())
+
+    rng = np.random.default_rng(13)
+    eps = 1e-6
+    for k in range(3):
+        dp = rng.standard_normal(parts.shape)
+        dbu = rng.standard_normal(bu.shape)
+        dbv = rng.standard_normal(bv.shape)
+        adj = float((pbar * dp).sum() + (gbu * dbu).sum() + (gbv * dbv).sum())
+        fd = (functional(parts + eps * dp, bu + eps * dbu, bv + eps * dbv) -
+              functional(parts - eps * dp, bu - eps * dbu, bv - eps * dbv)) / (2 * eps)
+        rel = abs(adj - fd) / max(abs(fd), abs(adj), 1e-30)
+        print(f"mixed direction {k}: adjoint {adj:.8e} fd {fd:.8e} rel {rel:.2e}")
+        assert rel <= 1e-4
+
+
 def test_adjoint_rejects_unsupported_use():
     sc = scenes.dam_break(16)
     d = fsg.Domain.from_scene(sc, solver=fsg.SOLVER_PLAIN)
     with pytest.raises(fsg.FsgError, match="FSG_SOLVER_MG"):
         d.adjoint_step(fsg.PICFLIP, 0.01)
     d.set_params(solver=fsg.SOLVER_MG)
-    with pytest.raises(fsg.FsgError, match="PIC, FLIP or PICFLIP"):
-        d.adjoint_step(fsg.SEMI_LAGRANGIAN, 0.01)
+    with pytest.raises(fsg.FsgError, match="SEMI_LAGRANGIAN, PIC, FLIP or PICFLIP"):
+        d.adjoint_step(fsg.GRID_ADVECT, 0.01)
     empty = fsg.Domain(16, 16, 16.0, 16.0, 0.01, 0.05)
     empty.set_params(solver=fsg.SOLVER_MG)
     with pytest.raises(fsg.FsgError, match="no LIQUID cell"):

@@ -66,7 +66,7 @@ def c5():
     rho = d.density().astype(np.float64)
     d.close()
     return dict(parts=parts, model=model, target=target, c0=c0, grad=grad,
-                loss=float(loss), loss_again=float(((rho - target) ** 2).sum()), recs=recs)
+                loss=float(loss.detach()), loss_again=float(((rho - target) ** 2).sum()), recs=recs)
 
 
 def test_c5_rollout_is_reproducible(c5):
