@@ -167,7 +167,8 @@ def log(*a):
 GRAPH = 1
 SMOOTHER = 1
 # the multigrid settings fsg_autotune chooses
-TUNED = ("mg_smoother", "mg_coarse_sweeps", "mg_fuse_update", "graph_iters", "splat_order")
+TUNED = ("mg_smoother", "mg_coarse_sweeps", "mg_fuse_update", "graph_iters", "splat_order",
+         "mg_tile_threads")
 
 
 def make_domain(fsg, scene, solver, tol, max_iter, warm=0):

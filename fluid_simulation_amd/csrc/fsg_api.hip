@@ -548,6 +548,7 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
   }
   m.coarse.trunc = h->prm.mg_coarse_sweeps;
   m.fused = !h->batch && m.lc >= 2 && h->prm.mg_fuse_update != 0;
+  m.l0_threads = h->prm.mg_tile_threads;
   m.clk_on = h->kclock == 1;
   m.clk_span_on = h->kclock == 2;
   if (m.clk_span_on && !m.clk_span) alloc_clk_span(h);
@@ -923,6 +924,7 @@ void fsg_params_default(fsg_params* p) {
   p->graph_iters = 2;
   p->mg_fuse_update = 1;
   p->splat_order = 1;
+  p->mg_tile_threads = 256;
 }
 
 int fsg_create(fsg_domain** out, int32_t nx, int32_t ny, float length_x, float length_y,
@@ -1027,7 +1029,8 @@ int fsg_set_params(fsg_domain* h, const fsg_params* p) {
       p->record_decisions < 0 || p->record_decisions > 1 || p->mg_smoother < 0 ||
       p->mg_smoother > 1 || p->mg_coarse_sweeps < 0 || p->mg_coarse_sweeps > 64 ||
       p->graph_iters < 1 || p->graph_iters > 16 || p->mg_fuse_update < 0 ||
-      p->mg_fuse_update > 1 || p->splat_order < 0 || p->splat_order > 1)
+      p->mg_fuse_update > 1 || p->splat_order < 0 || p->splat_order > 1 ||
+      (p->mg_tile_threads != 256 && p->mg_tile_threads != 512 && p->mg_tile_threads != 1024))
     return fail(h, FSG_E_INVALID, "fsg_set_params: value out of range");
   const size_t C = ncell(h);
   // fixed-point sums: 4 per cell (the P2G accumulators; the density uses the first C),
@@ -1052,7 +1055,7 @@ int fsg_set_params(fsg_domain* h, const fsg_params* p) {
   // the captured iterations hold the tolerance and the iteration cap
   if (p->cg_tol != h->prm.cg_tol || p->cg_max_iter != h->prm.cg_max_iter ||
       p->mg_smoother != h->prm.mg_smoother || p->mg_coarse_sweeps != h->prm.mg_coarse_sweeps ||
-      p->mg_fuse_update != h->prm.mg_fuse_update)
+      p->mg_fuse_update != h->prm.mg_fuse_update || p->mg_tile_threads != h->prm.mg_tile_threads)
     h->mg_graphs.clear();
   h->prm = *p;
   return FSG_OK;
@@ -1968,7 +1971,7 @@ int fsg_autotune(fsg_domain* h, int32_t scheme, float dt, int32_t steps, fsg_tun
   if (h->np == 0 && scheme != FSG_SCHEME_GRID_ADVECT)
     return fail(h, FSG_E_INVALID, "fsg_autotune: no particles to step");
   struct Cand {
-    int smoother, coarse, fuse, iters, order;
+    int smoother, coarse, fuse, iters, order, threads;
   };
   std::vector<Cand> cands;
   const fsg_params p0 = h->prm;
@@ -1980,7 +1983,7 @@ int fsg_autotune(fsg_domain* h, int32_t scheme, float dt, int32_t steps, fsg_tun
         for (int gi : {2, 4})
           for (int so : has_particles ? std::initializer_list<int>{1, 0}
                                       : std::initializer_list<int>{p0.splat_order})
-            cands.push_back({sm, co, fu, gi, so});
+            for (int nt : {256, 512}) cands.push_back({sm, co, fu, gi, so, nt});
   TuneState saved;
   if (int rc = tune_save(h, saved)) return rc;
   const fsg_stats stats0 = h->host_stats;
@@ -1996,6 +1999,7 @@ int fsg_autotune(fsg_domain* h, int32_t scheme, float dt, int32_t steps, fsg_tun
     q.mg_fuse_update = c.fuse;
     q.graph_iters = c.iters;
     q.splat_order = c.order;
+    q.mg_tile_threads = c.threads;
     if (int rc = fsg_set_params(h, &q)) return rc;
     double t_sum = 0.0;
     int its = 0;
@@ -2012,7 +2016,7 @@ int fsg_autotune(fsg_domain* h, int32_t scheme, float dt, int32_t steps, fsg_tun
     const double ms = t_sum / steps;
     const bool is_cur = c.smoother == p0.mg_smoother && c.coarse == p0.mg_coarse_sweeps &&
                         c.fuse == p0.mg_fuse_update && c.iters == p0.graph_iters &&
-                        c.order == p0.splat_order;
+                        c.order == p0.splat_order && c.threads == p0.mg_tile_threads;
     if (is_cur) {
       cur = ms;
       cur_k = (int)k;
@@ -2038,6 +2042,7 @@ int fsg_autotune(fsg_domain* h, int32_t scheme, float dt, int32_t steps, fsg_tun
   q.mg_fuse_update = cands[best_k].fuse;
   q.graph_iters = cands[best_k].iters;
   q.splat_order = cands[best_k].order;
+  q.mg_tile_threads = cands[best_k].threads;
   if (int rc = fsg_set_params(h, &q)) return rc;
   if (int rc = tune_load(h, saved, true)) return rc;
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -2050,6 +2055,7 @@ int fsg_autotune(fsg_domain* h, int32_t scheme, float dt, int32_t steps, fsg_tun
     out->mg_fuse_update = q.mg_fuse_update;
     out->graph_iters = q.graph_iters;
     out->splat_order = q.splat_order;
+    out->mg_tile_threads = q.mg_tile_threads;
     out->candidates = (int32_t)cands.size();
     out->cg_iterations = best_iters;
     out->ms_per_step = best;

@@ -268,6 +268,7 @@ struct MgHost {
   // fused form (one domain, two or more tile levels): level 0's down kernel also does
   // the previous iteration's x and residual update, level 1's runs the stop test
   bool fused = false;
+  int l0_threads = 256;  // threads per level-0 tile of the down / up kernels (256/512/1024)
   // the last tile level's down kernel also runs the one-workgroup stage (its last
   // workgroup to finish), so no k_mg_coarse launch; ticket: its arrival counter
   bool coarse_tail = true;

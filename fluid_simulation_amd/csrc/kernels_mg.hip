@@ -42,6 +42,8 @@
 // every block and the solve is deterministic.
 #include <float.h>
 
+#include <type_traits>
+
 #include "fsg_common.h"
 #include "fsg_device.h"
 #include "fsg_kernels.h"
@@ -2209,19 +2211,32 @@ void launch_mg_down(hipStream_t s, MgHost& m, int l, int it, int max_iter, doubl
   // the stop test: level 0's kernel, or level 1's in the fused form (level 0 forms
   // the residual it tests)
   const int check = l == (m.fused ? 1 : 0) ? 1 : 0;
-  // level 0: 256-thread blocks (many tiles); smaller levels: 1024 threads, so that
-  // each LDS phase of the few tiles takes one or two passes
-  if (l == 0 && m.fused) {
-    // iteration it - 1's update: the buffers k_mg_xr would use at that iteration
-    const MgXr xr{m.x[(it + 1) & 1], m.x[it & 1], m.p[it & 1], m.b, m.part_pq,
-                  mg_dir_blocks(m), m.part_rr};
-    k_mg_down<true, kT, true><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1),
-                                                    m.coarse.omega, m.coarse.omega2, it, chk, 0, m.tflag, xr,
-                                                    span, MgCoarseArgs{}, nullptr);
-  } else if (l == 0) {
-    k_mg_down<true, kT><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega, m.coarse.omega2, it,
-                                              chk, check, m.tflag, MgXr{}, span,
-                                              MgCoarseArgs{}, nullptr);
+  // level 0: m.l0_threads per tile (256 by default: many tiles); smaller levels: 1024
+  // threads, so that each LDS phase of the few tiles takes one or two passes
+  if (l == 0) {
+    // fused form: iteration it - 1's update, the buffers k_mg_xr would use then
+    const MgXr xr = m.fused ? MgXr{m.x[(it + 1) & 1], m.x[it & 1], m.p[it & 1], m.b, m.part_pq,
+                                   mg_dir_blocks(m), m.part_rr}
+                            : MgXr{};
+    const int ck = m.fused ? 0 : check;
+    auto go = [&](auto nt) {
+      constexpr int NT = decltype(nt)::value;
+      if (m.fused)
+        k_mg_down<true, NT, true><<<blocks, NT, 0, s>>>(mg_level(m, 0), mg_level(m, 1),
+                                                        m.coarse.omega, m.coarse.omega2, it, chk,
+                                                        ck, m.tflag, xr, span, MgCoarseArgs{},
+                                                        nullptr);
+      else
+        k_mg_down<true, NT><<<blocks, NT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega,
+                                                  m.coarse.omega2, it, chk, ck, m.tflag, xr, span,
+                                                  MgCoarseArgs{}, nullptr);
+    };
+    if (m.l0_threads == 1024)
+      go(std::integral_constant<int, 1024>{});
+    else if (m.l0_threads == 512)
+      go(std::integral_constant<int, 512>{});
+    else
+      go(std::integral_constant<int, kT>{});
   } else {
     // the last tile level also runs the one-workgroup stage (mg_tail_coarse): its own
     // instance, so that the other levels (and every level of a batch) carry none of it
@@ -2250,9 +2265,17 @@ void launch_mg_coarse(hipStream_t s, const MgHost& m, int it, int max_iter, doub
 
 void launch_mg_up(hipStream_t s, const MgHost& m, int l, int it, CgScalars* sc) {
   const int blocks = mg_up_blocks(m.nx[l], m.ny[l]);
-  if (l == 0)
-    k_mg_up<true, kT><<<blocks, kT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega, m.coarse.omega2, sc,
-                                            m.part_rz, it, m.tflag);
+  auto go0 = [&](auto nt) {
+    constexpr int NT = decltype(nt)::value;
+    k_mg_up<true, NT><<<blocks, NT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega,
+                                            m.coarse.omega2, sc, m.part_rz, it, m.tflag);
+  };
+  if (l == 0 && m.l0_threads == 1024)
+    go0(std::integral_constant<int, 1024>{});
+  else if (l == 0 && m.l0_threads == 512)
+    go0(std::integral_constant<int, 512>{});
+  else if (l == 0)
+    go0(std::integral_constant<int, kT>{});
   else
     k_mg_up<false, kTileBig><<<blocks, kTileBig, 0, s>>>(mg_level(m, l), mg_level(m, l + 1),
                                                          m.coarse.omega, m.coarse.omega2, sc, nullptr, it,

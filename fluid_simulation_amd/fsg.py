@@ -86,6 +86,7 @@ class Params(C.Structure):
         ("record_decisions", C.c_int32), ("mg_smoother", C.c_int32),
         ("mg_coarse_sweeps", C.c_int32), ("graph_iters", C.c_int32),
         ("mg_fuse_update", C.c_int32), ("splat_order", C.c_int32),
+        ("mg_tile_threads", C.c_int32),
     ]
 
 
@@ -94,6 +95,7 @@ class Tune(C.Structure):
         ("mg_smoother", C.c_int32), ("mg_coarse_sweeps", C.c_int32),
         ("mg_fuse_update", C.c_int32), ("graph_iters", C.c_int32),
         ("splat_order", C.c_int32), ("candidates", C.c_int32), ("cg_iterations", C.c_int32),
+        ("mg_tile_threads", C.c_int32),
         ("ms_per_step", C.c_double), ("ms_per_step_before", C.c_double),
         ("reserved", C.c_int64 * 3),
     ]

@@ -162,6 +162,10 @@ typedef struct fsg_params {
                              particles' own order is untouched); 0 reads them in
                              storage order (cheaper while particles that are neighbours
                              in storage are neighbours in space) */
+  int32_t mg_tile_threads;/* multigrid solver: threads per 32 x 32 level-0 tile of the
+                             down / up kernels, 256 (default), 512 or 1024 (more waves
+                             per tile hide more memory latency where few tiles hold
+                             unknowns; fewer leave room for more tiles per CU) */
 } fsg_params;
 
 /* Counters of the last pressure solve and running totals. */
@@ -361,6 +365,7 @@ typedef struct fsg_tune {
   int32_t splat_order;                                                /* chosen */
   int32_t candidates;       /* configurations timed */
   int32_t cg_iterations;    /* the chosen configuration's iterations (last timed step) */
+  int32_t mg_tile_threads;  /* chosen */
   double ms_per_step;       /* the chosen configuration's mean step time */
   double ms_per_step_before;/* the handle's configuration before the call (if a candidate) */
   int64_t reserved[3];

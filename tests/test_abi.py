@@ -26,9 +26,8 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_params_struct_layout_and_defaults():
-    # layout: 2 floats, double, 6 int32, double, 5 int32, 1 float, 9 int32 (+ 4 bytes of
-    # tail padding to the double alignment)
-    assert ctypes.sizeof(fsg.Params) == 4 * 2 + 8 + 4 * 6 + 8 + 4 * 5 + 4 + 4 * 9 + 4
+    # layout: 2 floats, double, 6 int32, double, 5 int32, 1 float, 10 int32
+    assert ctypes.sizeof(fsg.Params) == 4 * 2 + 8 + 4 * 6 + 8 + 4 * 5 + 4 + 4 * 10
     assert fsg.Params.particle_pressure_k.offset == 68
     assert fsg.Params.closed_walls.offset == 72
     assert fsg.Params.graph.offset == 76
@@ -36,6 +35,7 @@ def test_params_struct_layout_and_defaults():
     assert fsg.Params.record_decisions.offset == 84
     assert fsg.Params.mg_smoother.offset == 88
     assert fsg.Params.graph_iters.offset == 96
+    assert fsg.Params.mg_tile_threads.offset == 108
     p = fsg.default_params()
     assert p.gravity_y == ctypes.c_float(-9.82).value      # src/FluidSolver.cpp:232
     assert p.cg_max_iter == 100                              # src/FluidSolver.cpp:81
@@ -47,6 +47,7 @@ def test_params_struct_layout_and_defaults():
     assert p.deterministic == 0 and p.record_decisions == 0 and p.mg_smoother == 1
     assert p.mg_coarse_sweeps == 0 and p.graph_iters == 2 and p.mg_fuse_update == 1
     assert p.splat_order == 1
+    assert p.mg_tile_threads == 256
 
 
 def test_noalloc_rejects_bad_sizes_before_any_device_call():

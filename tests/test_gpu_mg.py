@@ -99,6 +99,22 @@ def test_mg_coarse_and_fusion_options_against_fp64(sweeps, fuse):
     assert ep <= 1e-6
 
 
+@pytest.mark.parametrize("threads", [512, 1024])
+@pytest.mark.parametrize("fuse", [1, 0])
+def test_mg_level0_tile_threads_against_fp64(threads, fuse):
+    """Level-0 tiles of 512 or 1024 threads (mg_tile_threads) in the fused and the
+    unfused form: the same system solved to the same FP64 residual bound."""
+    lab = irregular_labels(513, 257, 4)
+    d, g = _solve_pair(lab, 4, mg_tile_threads=threads, mg_fuse_update=fuse)
+    st = g.stats()
+    liq = lab == LIQ
+    ep = rel_err(g.pressure()[liq], d.pressure[liq])
+    print(f"tile threads {threads} fuse {fuse}: {st.cg_iterations} iterations, "
+          f"|r|/|b| {st.cg_error:.2e}, p {ep:.2e}")
+    assert st.cg_error <= 1e-10
+    assert ep <= 1e-6
+
+
 def test_mg_closed_box_project_mean():
     """All-LIQUID interior in a SOLID box: K is singular (constants); with the mean
     of b removed the pressure is defined up to a constant and the velocity update
