@@ -278,25 +278,35 @@ def timed_segment(fsg, d, steps, dt, kernel=None, scheme=None):
     return t1 - t0, st, tm
 
 
-def full_run(fsg, scene, solver, tol, steps, autotune=False):
+def full_run(fsg, scene, solver, tol, steps, autotune=False, tune_every=100):
     """The configuration's whole run (`steps` steps from the initial state) on a
-    domain of its own, autotuned (untimed) at its initial state if asked: wall time,
-    and every solve's iterations and FP64 relative residual (host-side counters read
-    after each step, no extra synchronisation)."""
+    domain of its own, autotuned (untimed) at its initial state and every `tune_every`
+    steps if asked (the flow changes what is fastest: the particles leave their
+    initial lattice order within ~40 steps, which decides the splat order): wall time
+    of the steps, and every solve's iterations and FP64 relative residual (host-side
+    counters read after each step, no extra synchronisation)."""
     d = make_domain(fsg, scene, solver, tol, 200 if solver == fsg.SOLVER_MG else 20000)
-    tune = d.autotune(fsg.PICFLIP, scene.dt, steps=3) if autotune else None
-    its, errs = [], []
-    d.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        d.step(fsg.PICFLIP, scene.dt)
-        st = d.stats()
-        its.append(st.cg_iterations)
-        errs.append(st.cg_error)
-    d.synchronize()
-    t1 = time.perf_counter()
+    its, errs, tunes = [], [], {}
+    seconds = tune_s = 0.0
+    seg = tune_every if autotune else max(steps, 1)
+    for s0 in range(0, steps, seg):
+        if autotune:
+            t0 = time.perf_counter()
+            tunes[str(s0)] = d.autotune(fsg.PICFLIP, scene.dt, steps=3)
+            tune_s += time.perf_counter() - t0
+        d.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(min(seg, steps - s0)):
+            d.step(fsg.PICFLIP, scene.dt)
+            st = d.stats()
+            its.append(st.cg_iterations)
+            errs.append(st.cg_error)
+        d.synchronize()
+        seconds += time.perf_counter() - t0
     d.close()
-    return {"steps": steps, "seconds": t1 - t0, "steps_per_s": steps / (t1 - t0),
+    tune = ({k: {q: t[q] for q in TUNED} for k, t in tunes.items()} | {"seconds": tune_s}
+            if autotune else None)
+    return {"steps": steps, "seconds": seconds, "steps_per_s": steps / seconds,
             "cg_iterations_min": int(min(its)), "cg_iterations_mean": float(np.mean(its)),
             "cg_iterations_max": int(max(its)), "cg_rel_residual_max": float(max(errs)),
             "all_solves_within_tol": bool(max(errs) <= tol),
