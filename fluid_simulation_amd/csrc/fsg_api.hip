@@ -549,6 +549,7 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
   m.coarse.trunc = h->prm.mg_coarse_sweeps;
   m.fused = !h->batch && m.lc >= 2 && h->prm.mg_fuse_update != 0;
   m.l0_threads = h->prm.mg_tile_threads;
+  m.level_sweeps = h->prm.mg_level_sweeps;
   m.clk_on = h->kclock == 1;
   m.clk_span_on = h->kclock == 2;
   if (m.clk_span_on && !m.clk_span) alloc_clk_span(h);
@@ -925,6 +926,7 @@ void fsg_params_default(fsg_params* p) {
   p->mg_fuse_update = 1;
   p->splat_order = 1;
   p->mg_tile_threads = 256;
+  p->mg_level_sweeps = 4;
 }
 
 int fsg_create(fsg_domain** out, int32_t nx, int32_t ny, float length_x, float length_y,
@@ -1030,7 +1032,8 @@ int fsg_set_params(fsg_domain* h, const fsg_params* p) {
       p->mg_smoother > 1 || p->mg_coarse_sweeps < 0 || p->mg_coarse_sweeps > 64 ||
       p->graph_iters < 1 || p->graph_iters > 16 || p->mg_fuse_update < 0 ||
       p->mg_fuse_update > 1 || p->splat_order < 0 || p->splat_order > 1 ||
-      (p->mg_tile_threads != 256 && p->mg_tile_threads != 512 && p->mg_tile_threads != 1024))
+      (p->mg_tile_threads != 256 && p->mg_tile_threads != 512 && p->mg_tile_threads != 1024) ||
+      (p->mg_level_sweeps != 2 && p->mg_level_sweeps != 4))
     return fail(h, FSG_E_INVALID, "fsg_set_params: value out of range");
   const size_t C = ncell(h);
   // fixed-point sums: 4 per cell (the P2G accumulators; the density uses the first C),
@@ -1055,7 +1058,8 @@ int fsg_set_params(fsg_domain* h, const fsg_params* p) {
   // the captured iterations hold the tolerance and the iteration cap
   if (p->cg_tol != h->prm.cg_tol || p->cg_max_iter != h->prm.cg_max_iter ||
       p->mg_smoother != h->prm.mg_smoother || p->mg_coarse_sweeps != h->prm.mg_coarse_sweeps ||
-      p->mg_fuse_update != h->prm.mg_fuse_update || p->mg_tile_threads != h->prm.mg_tile_threads)
+      p->mg_fuse_update != h->prm.mg_fuse_update || p->mg_tile_threads != h->prm.mg_tile_threads ||
+      p->mg_level_sweeps != h->prm.mg_level_sweeps)
     h->mg_graphs.clear();
   h->prm = *p;
   return FSG_OK;

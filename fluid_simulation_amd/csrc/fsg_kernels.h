@@ -269,6 +269,9 @@ struct MgHost {
   // the previous iteration's x and residual update, level 1's runs the stop test
   bool fused = false;
   int l0_threads = 256;  // threads per level-0 tile of the down / up kernels (256/512/1024)
+  // sweeps before and after the coarse correction on the tile levels below level 0: 2
+  // (the level's omega, omega2) or 4 (degree-4 Chebyshev weights)
+  int level_sweeps = 2;
   // the last tile level's down kernel also runs the one-workgroup stage (its last
   // workgroup to finish), so no k_mg_coarse launch; ticket: its arrival counter
   bool coarse_tail = true;

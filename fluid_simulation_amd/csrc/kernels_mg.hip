@@ -42,6 +42,9 @@
 // every block and the solve is deterministic.
 #include <float.h>
 
+#include <algorithm>
+#include <cmath>
+
 #include <type_traits>
 
 #include "fsg_common.h"
@@ -587,6 +590,13 @@ struct MgXr {
 __device__ __forceinline__ void coarse_stage(const MgCoarseArgs& a, double* part_rz,
                                              bool coherent);
 
+// Smoothing weights of the tile levels below level 0 when they take S = 4 sweeps
+// before and after the coarse correction (sweep k uses w[k - 1]); with S = 2 every
+// level uses the kernels' omega, omega2.
+struct MgSweeps {
+  float w[4];
+};
+
 // Block span clock (span != null): thread 0 stores the block's start and end ticks
 // of the device wall clock in span[2 b], span[2 b + 1]; the launch's duration is the
 // latest end minus the earliest start over its blocks.
@@ -603,15 +613,20 @@ struct BlockSpan {
   }
 };
 
-template <bool L0, int NT, bool XR = false, bool TAIL = false>
+// S sweeps (2, or 4 on the levels below level 0): the right-hand side and info on
+// [-(S + 1), 33 + S), sweep k on a ring one cell narrower than sweep k - 1, the last on
+// [-2, 34); the sweeps alternate between the x1 and x2 buffers.
+template <bool L0, int NT, bool XR = false, bool TAIL = false, int S = 2>
 __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omega, float omega2,
                                                  int it,
                                                  MgCheck chk, int check,
                                                  const uint8_t* __restrict__ tflag, MgXr xr,
                                                  unsigned long long* span,
-                                                 MgCoarseArgs ca, unsigned* __restrict__ ticket) {
+                                                 MgCoarseArgs ca, unsigned* __restrict__ ticket,
+                                                 MgSweeps sw) {
   static_assert(!XR || L0, "the fused update belongs to level 0");
   static_assert(!TAIL || !L0, "the one-workgroup stage follows the last tile level");
+  static_assert(S == 2 || (S == 4 && !L0), "level 0 takes two sweeps; smaller levels 2 or 4");
   const BlockSpan clock(span);
   __shared__ double sm[NT / 64];
   __shared__ int s_last;
@@ -653,13 +668,15 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
     clock.end();
     return;
   }
-  constexpr int W3 = 38, W2 = 36, W1 = 34;
+  constexpr int W3 = 34 + 2 * S, W2 = W3 - 2, W1 = 34;  // S = 2: 38, 36, 34
   constexpr int W4 = 40;  // fused form: x_new on [-4, 36)
   constexpr int WC = 18;  // coarse cells of the tile and their neighbours
   __shared__ float s_r[W3 * W3], s_x1[W3 * W3], s_x2[W2 * W2];
-  // the residual (third phase) reuses x1's storage: x1 is dead once x2 is formed, and
-  // the smaller footprint lets more tiles share a CU
+  // the residual reuses x1's storage: x1 is dead once the next sweep is formed (with
+  // S = 4, the third sweep's x1-sized buffer is dead once the fourth is), and the
+  // smaller footprint lets more tiles share a CU
   float* const s_res = s_x1;
+  const float wk[4] = {S == 4 ? sw.w[0] : omega, S == 4 ? sw.w[1] : omega2, sw.w[2], sw.w[3]};
   __shared__ uint8_t s_inf[W3 * W3], s_cinf[WC * WC];
   __shared__ double s_xn[XR ? W4 * W4 : 1];
   const int tiles_x = (c.nx + 15) >> 4;
@@ -802,7 +819,7 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
 #pragma unroll
     for (int k = 0; k < NF; ++k) {
       const int t = threadIdx.x + k * NT;
-      const int i = fi0 + t % W3 - 3, j = fj0 + t / W3 - 3;
+      const int i = fi0 + t % W3 - (S + 1), j = fj0 + t / W3 - (S + 1);
       const bool in = t < W3 * W3 && i >= 0 && i < f.nx && j >= 0 && j < f.ny;
       const int q = in ? i + j * f.nx : 0;
       const uint8_t fq = f.inf[q];
@@ -821,32 +838,44 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
       if (t < W3 * W3) {
         s_inf[t] = fv[k];
         s_r[t] = rv[k];
-        s_x1[t] = is_act(fv[k]) ? omega * rv[k] * inv_nns(fv[k]) : 0.f;
+        s_x1[t] = is_act(fv[k]) ? wk[0] * rv[k] * inv_nns(fv[k]) : 0.f;
       }
     }
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < W2 * W2; t += NT) {
-    const int q = (t % W2 + 1) + (t / W2 + 1) * W3;
-    const uint8_t fv = s_inf[q];
-    float v = 0.f;
-    if (is_act(fv)) {
-      const float n = nns_of(fv), x = s_x1[q];
-      const float kx = n * x - (s_x1[q - 1] + s_x1[q + 1] + s_x1[q - W3] + s_x1[q + W3]);
-      v = x + omega2 * (s_r[q] - kx) * inv_nns(fv);
+  // sweeps 2 .. S: from the buffer of window win (offset d inside the right-hand
+  // side's window) to one of window win - 2; the last lands in s_x2 ([-2, 34))
+#pragma unroll
+  for (int k = 2; k <= S; ++k) {
+    const int win = W3 - 2 * (k - 2), wout = win - 2, d = k - 2;
+    const float* xin = (k & 1) ? s_x2 : s_x1;
+    float* xout = (k & 1) ? s_x1 : s_x2;
+    const float wgt = wk[k - 1];
+    for (int t = threadIdx.x; t < wout * wout; t += NT) {
+      const int li = t % wout + 1, lj = t / wout + 1;
+      const int q = li + lj * win;
+      const int qr = (li + d) + (lj + d) * W3;
+      const uint8_t fv = s_inf[qr];
+      float v = 0.f;
+      if (is_act(fv)) {
+        const float n = nns_of(fv), x = xin[q];
+        const float kx = n * x - (xin[q - 1] + xin[q + 1] + xin[q - win] + xin[q + win]);
+        v = x + wgt * (s_r[qr] - kx) * inv_nns(fv);
+      }
+      xout[t] = v;
     }
-    s_x2[t] = v;
+    __syncthreads();
   }
-  __syncthreads();
   for (int t = threadIdx.x; t < W1 * W1; t += NT) {
     const int li = t % W1, lj = t / W1;  // fine offset li - 1, lj - 1
-    const int q = (li + 1) + (lj + 1) * W2;
-    const int q3 = (li + 2) + (lj + 2) * W3;
+    constexpr int WS = 36;               // the last sweep's window, [-2, 34)
+    const int q = (li + 1) + (lj + 1) * WS;
+    const int q3 = (li + S) + (lj + S) * W3;
     const uint8_t fv = s_inf[q3];
     const float x = s_x2[q];
     float res = 0.f;
     if (is_act(fv)) {
-      const float kx = nns_of(fv) * x - (s_x2[q - 1] + s_x2[q + 1] + s_x2[q - W2] + s_x2[q + W2]);
+      const float kx = nns_of(fv) * x - (s_x2[q - 1] + s_x2[q + 1] + s_x2[q - WS] + s_x2[q + WS]);
       res = s_r[q3] - kx;
     }
     s_res[t] = res;
@@ -894,17 +923,25 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
 }
 
 // Up: level l+1 -> level l.  A block owns 32 x 32 fine cells.  In LDS: the coarse
-// solution around the tile, r and info on [-2, 34), the corrected x = xs + P e
-// there, the first post-sweep on [-1, 33), the second on the tile, written to xo.
+// solution around the tile, r and info on [-S, 32 + S), the corrected x = xs + P e
+// there, then S post-sweeps, each on a ring one cell narrower than the one before (the
+// last on the tile, written to xo), alternating between the xc and y buffers.  S = 2
+// (level 0), or 2 or 4 on the levels below (sweep k uses w[k - 1]).
 // Level 0 also forms the partial of r.z (z = xo) for the next direction.
-template <bool L0, int NT>
+template <bool L0, int NT, int S = 2>
 __global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega, float omega2,
                                                const CgScalars* sc, double* __restrict__ part_rz,
-                                               int it, const uint8_t* __restrict__ tflag) {
-  constexpr int W2 = 36, W1 = 34, WC = 20;
+                                               int it, const uint8_t* __restrict__ tflag,
+                                               MgSweeps sw) {
+  static_assert(S == 2 || (S == 4 && !L0), "level 0 takes two sweeps; smaller levels 2 or 4");
+  constexpr int W2 = 32 + 2 * S, W1 = W2 - 2;  // S = 2: 36, 34
+  // coarse cells from fi0 / 2 - (S / 2 + 1): every coarse neighbour of the fine cells
+  // [-S, 32 + S)
+  constexpr int WC = 18 + S, CO = S / 2 + 1;
   __shared__ float s_r[W2 * W2], s_xc[W2 * W2], s_y1[W1 * W1], s_e[WC * WC];
   __shared__ uint8_t s_inf[W2 * W2], s_cinf[WC * WC];
   __shared__ double sm[NT / 64];
+  const float wk[4] = {S == 4 ? sw.w[0] : omega, S == 4 ? sw.w[1] : omega2, sw.w[2], sw.w[3]};
   const int tiles_x = (f.nx + 31) >> 5;
   // each XCD takes one band of consecutive tiles, so halo rows are shared in its L2
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -923,7 +960,7 @@ __global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega,
     if (L0 && threadIdx.x == 0) part_rz[blockIdx.x] = 0.0;
     return;
   }
-  const int ci0 = (fi0 >> 1) - 2, cj0 = (fj0 >> 1) - 2;  // coarse frame origin
+  const int ci0 = (fi0 >> 1) - CO, cj0 = (fj0 >> 1) - CO;  // coarse frame origin
   // every load of the tile is issued before the first LDS store (a load outside the
   // grid reads cell 0 and is replaced by the boundary value)
   constexpr int NC = (WC * WC + NT - 1) / NT, NF = (W2 * W2 + NT - 1) / NT;
@@ -943,7 +980,7 @@ __global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega,
 #pragma unroll
   for (int k = 0; k < NF; ++k) {
     const int t = threadIdx.x + k * NT;
-    const int i = fi0 + t % W2 - 2, j = fj0 + t / W2 - 2;
+    const int i = fi0 + t % W2 - S, j = fj0 + t / W2 - S;
     const bool in = t < W2 * W2 && i >= 0 && i < f.nx && j >= 0 && j < f.ny;
     const int q = in ? i + j * f.nx : 0;
     const uint8_t fq = f.inf[q];
@@ -975,8 +1012,8 @@ __global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega,
       s_xc[t] = 0.f;
       continue;
     }
-    const int i = fi0 + t % W2 - 2, j = fj0 + t / W2 - 2;
-    const int I = i >> 1, J = j >> 1;  // floor for the -2 .. -1 halo too
+    const int i = fi0 + t % W2 - S, j = fj0 + t / W2 - S;
+    const int I = i >> 1, J = j >> 1;  // floor for the halo left of / below 0 too
     const int In = (i & 1) ? I + 1 : I - 1;
     const int Jn = (j & 1) ? J + 1 : J - 1;
     const int q00 = (I - ci0) + (J - cj0) * WC, q10 = (In - ci0) + (J - cj0) * WC;
@@ -985,31 +1022,45 @@ __global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega,
                              s_cinf[q11]);
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < W1 * W1; t += NT) {
-    const int q = (t % W1 + 1) + (t / W1 + 1) * W2;
-    const uint8_t fv = s_inf[q];
-    float v = 0.f;
-    if (is_act(fv)) {
-      const float n = nns_of(fv), x = s_xc[q];
-      const float kx = n * x - (s_xc[q - 1] + s_xc[q + 1] + s_xc[q - W2] + s_xc[q + W2]);
-      v = x + omega * (s_r[q] - kx) * inv_nns(fv);
+  // post-sweeps 1 .. S-1: from the buffer of window win (offset d inside the
+  // right-hand side's window) to one of window win - 2
+#pragma unroll
+  for (int k = 1; k < S; ++k) {
+    const int win = W2 - 2 * (k - 1), wout = win - 2, d = k - 1;
+    const float* xin = (k & 1) ? s_xc : s_y1;
+    float* xout = (k & 1) ? s_y1 : s_xc;
+    const float wgt = wk[k - 1];
+    for (int t = threadIdx.x; t < wout * wout; t += NT) {
+      const int li = t % wout + 1, lj = t / wout + 1;
+      const int q = li + lj * win;
+      const int qr = (li + d) + (lj + d) * W2;
+      const uint8_t fv = s_inf[qr];
+      float v = 0.f;
+      if (is_act(fv)) {
+        const float n = nns_of(fv), x = xin[q];
+        const float kx = n * x - (xin[q - 1] + xin[q + 1] + xin[q - win] + xin[q + win]);
+        v = x + wgt * (s_r[qr] - kx) * inv_nns(fv);
+      }
+      xout[t] = v;
     }
-    s_y1[t] = v;
+    __syncthreads();
   }
-  __syncthreads();
+  // the last post-sweep, on the tile, from the window [-1, 33) (in y1: S even)
+  constexpr int WL = 34;
   double rz = 0.0;
+  const float wl = wk[S - 1];
   for (int t = threadIdx.x; t < 32 * 32; t += NT) {
     const int oi = t & 31, oj = t >> 5;
     const int i = fi0 + oi, j = fj0 + oj;
     if (i >= f.nx || j >= f.ny) continue;
-    const int q = (oi + 1) + (oj + 1) * W1;
-    const int q2 = (oi + 2) + (oj + 2) * W2;
+    const int q = (oi + 1) + (oj + 1) * WL;
+    const int q2 = (oi + S) + (oj + S) * W2;
     const uint8_t fv = s_inf[q2];
     float v = 0.f;
     if (is_act(fv)) {
       const float n = nns_of(fv), x = s_y1[q];
-      const float kx = n * x - (s_y1[q - 1] + s_y1[q + 1] + s_y1[q - W1] + s_y1[q + W1]);
-      v = x + omega2 * (s_r[q2] - kx) * inv_nns(fv);
+      const float kx = n * x - (s_y1[q - 1] + s_y1[q + 1] + s_y1[q - WL] + s_y1[q + WL]);
+      v = x + wl * (s_r[q2] - kx) * inv_nns(fv);
     }
     f.xo[i + j * f.nx] = v;
     if (L0) rz += (double)s_r[q2] * (double)v;
@@ -2093,6 +2144,9 @@ hipError_t mg_set_lds_limit(size_t bytes) {
   e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_down<false, kTileBig, false, true>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_down<false, kTileBig, false, true, 4>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) return e;
   return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_scene_coarse),
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
@@ -2195,6 +2249,21 @@ static MgLevel mg_level(const MgHost& m, int l) {
   return MgLevel{m.nx[l], m.ny[l], m.inf[l], m.rl[l], m.xs[l], m.xo[l]};
 }
 
+// The tile levels' four-sweep weights: the reciprocal roots of the degree-4 Chebyshev
+// polynomial on [0.25, 2] (the high end of the spectrum of D^-1 K), smallest first
+// (every partial product of the factors stays within [-1, 1] on [0, 2]).
+static MgSweeps mg_sweeps(const MgHost& m) {
+  MgSweeps w{};
+  if (m.level_sweeps != 4) return w;
+  const double lo = 0.25, hi = 2.0, pi = 3.14159265358979323846;
+  double r[4];
+  for (int k = 0; k < 4; ++k)
+    r[k] = 1.0 / ((hi + lo) / 2 + (hi - lo) / 2 * std::cos(pi * (2 * k + 1) / 8.0));
+  std::sort(r, r + 4);
+  for (int k = 0; k < 4; ++k) w.w[k] = (float)r[k];
+  return w;
+}
+
 // One CG iteration = V-cycle (z = B r) + direction + update.  Level 0's down kernel
 // (or the one-workgroup kernel when the hierarchy starts there) runs the stop test
 // of this iteration first.
@@ -2225,11 +2294,11 @@ void launch_mg_down(hipStream_t s, MgHost& m, int l, int it, int max_iter, doubl
         k_mg_down<true, NT, true><<<blocks, NT, 0, s>>>(mg_level(m, 0), mg_level(m, 1),
                                                         m.coarse.omega, m.coarse.omega2, it, chk,
                                                         ck, m.tflag, xr, span, MgCoarseArgs{},
-                                                        nullptr);
+                                                        nullptr, MgSweeps{});
       else
         k_mg_down<true, NT><<<blocks, NT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega,
                                                   m.coarse.omega2, it, chk, ck, m.tflag, xr, span,
-                                                  MgCoarseArgs{}, nullptr);
+                                                  MgCoarseArgs{}, nullptr, MgSweeps{});
     };
     if (m.l0_threads == 1024)
       go(std::integral_constant<int, 1024>{});
@@ -2239,15 +2308,24 @@ void launch_mg_down(hipStream_t s, MgHost& m, int l, int it, int max_iter, doubl
       go(std::integral_constant<int, kT>{});
   } else {
     // the last tile level also runs the one-workgroup stage (mg_tail_coarse): its own
-    // instance, so that the other levels (and every level of a batch) carry none of it
-    if (l == m.lc - 1 && mg_tail_coarse(m))
-      k_mg_down<false, kTileBig, false, true><<<blocks, kTileBig, m.coarse_lds, s>>>(
-          mg_level(m, l), mg_level(m, l + 1), m.coarse.omega, m.coarse.omega2, it, chk, check, m.tflag_l[l],
-          MgXr{}, nullptr, m.coarse, m.ticket);
+    // instance, so that the other levels (and every level of a batch) carry none of it;
+    // m.level_sweeps (2 or 4) sweeps before the coarse correction
+    const MgSweeps sw = mg_sweeps(m);
+    auto go = [&](auto ns) {
+      constexpr int S = decltype(ns)::value;
+      if (l == m.lc - 1 && mg_tail_coarse(m))
+        k_mg_down<false, kTileBig, false, true, S><<<blocks, kTileBig, m.coarse_lds, s>>>(
+            mg_level(m, l), mg_level(m, l + 1), m.coarse.omega, m.coarse.omega2, it, chk, check,
+            m.tflag_l[l], MgXr{}, nullptr, m.coarse, m.ticket, sw);
+      else
+        k_mg_down<false, kTileBig, false, false, S><<<blocks, kTileBig, 0, s>>>(
+            mg_level(m, l), mg_level(m, l + 1), m.coarse.omega, m.coarse.omega2, it, chk, check,
+            m.tflag_l[l], MgXr{}, nullptr, MgCoarseArgs{}, nullptr, sw);
+    };
+    if (m.level_sweeps == 4)
+      go(std::integral_constant<int, 4>{});
     else
-      k_mg_down<false, kTileBig><<<blocks, kTileBig, 0, s>>>(
-          mg_level(m, l), mg_level(m, l + 1), m.coarse.omega, m.coarse.omega2, it, chk, check, m.tflag_l[l],
-          MgXr{}, nullptr, MgCoarseArgs{}, nullptr);
+      go(std::integral_constant<int, 2>{});
   }
 }
 
@@ -2268,7 +2346,8 @@ void launch_mg_up(hipStream_t s, const MgHost& m, int l, int it, CgScalars* sc) 
   auto go0 = [&](auto nt) {
     constexpr int NT = decltype(nt)::value;
     k_mg_up<true, NT><<<blocks, NT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega,
-                                            m.coarse.omega2, sc, m.part_rz, it, m.tflag);
+                                            m.coarse.omega2, sc, m.part_rz, it, m.tflag,
+                                            MgSweeps{});
   };
   if (l == 0 && m.l0_threads == 1024)
     go0(std::integral_constant<int, 1024>{});
@@ -2276,10 +2355,14 @@ void launch_mg_up(hipStream_t s, const MgHost& m, int l, int it, CgScalars* sc) 
     go0(std::integral_constant<int, 512>{});
   else if (l == 0)
     go0(std::integral_constant<int, kT>{});
+  else if (m.level_sweeps == 4)
+    k_mg_up<false, kTileBig, 4><<<blocks, kTileBig, 0, s>>>(mg_level(m, l), mg_level(m, l + 1),
+                                                            m.coarse.omega, m.coarse.omega2, sc,
+                                                            nullptr, it, m.tflag_l[l], mg_sweeps(m));
   else
     k_mg_up<false, kTileBig><<<blocks, kTileBig, 0, s>>>(mg_level(m, l), mg_level(m, l + 1),
                                                          m.coarse.omega, m.coarse.omega2, sc, nullptr, it,
-                                                         m.tflag_l[l]);
+                                                         m.tflag_l[l], MgSweeps{});
 }
 
 int mg_dir_blocks(const MgHost& m) {

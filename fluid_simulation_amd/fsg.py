@@ -86,7 +86,7 @@ class Params(C.Structure):
         ("record_decisions", C.c_int32), ("mg_smoother", C.c_int32),
         ("mg_coarse_sweeps", C.c_int32), ("graph_iters", C.c_int32),
         ("mg_fuse_update", C.c_int32), ("splat_order", C.c_int32),
-        ("mg_tile_threads", C.c_int32),
+        ("mg_tile_threads", C.c_int32), ("mg_level_sweeps", C.c_int32),
     ]
 
 
@@ -97,7 +97,8 @@ class Tune(C.Structure):
         ("splat_order", C.c_int32), ("candidates", C.c_int32), ("cg_iterations", C.c_int32),
         ("mg_tile_threads", C.c_int32),
         ("ms_per_step", C.c_double), ("ms_per_step_before", C.c_double),
-        ("reserved", C.c_int64 * 3),
+        ("mg_level_sweeps", C.c_int32), ("reserved32", C.c_int32),
+        ("reserved", C.c_int64 * 2),
     ]
 
 
@@ -389,7 +390,7 @@ class Domain:
         steps from the current state, keep the fastest, restore the state."""
         t = Tune()
         self._chk(self._L.fsg_autotune(self._h, scheme, dt, steps, C.byref(t)), "fsg_autotune")
-        return {k: getattr(t, k) for k, _ in Tune._fields_ if k != "reserved"}
+        return {k: getattr(t, k) for k, _ in Tune._fields_ if not k.startswith("reserved")}
 
     def step_decisions(self):
         """The last step's discrete decisions per cell (fsg_params.record_decisions):

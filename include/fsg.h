@@ -166,6 +166,13 @@ typedef struct fsg_params {
                              down / up kernels, 256 (default), 512 or 1024 (more waves
                              per tile hide more memory latency where few tiles hold
                              unknowns; fewer leave room for more tiles per CU) */
+  int32_t mg_level_sweeps;/* multigrid solver: smoothing sweeps before and after the
+                             coarse correction on the tile levels below level 0 (level 0
+                             and the one-workgroup levels keep two): 4 (default), the
+                             degree-4 Chebyshev smoother on [0.25, 2] (those levels are
+                             bound by launch and barrier latency, so the extra sweeps
+                             cost little and save iterations: 12.4 -> 10 per solve over
+                             the 1024^2 dam break), or 2, the mg_smoother weights */
 } fsg_params;
 
 /* Counters of the last pressure solve and running totals. */
@@ -368,7 +375,9 @@ typedef struct fsg_tune {
   int32_t mg_tile_threads;  /* chosen */
   double ms_per_step;       /* the chosen configuration's mean step time */
   double ms_per_step_before;/* the handle's configuration before the call (if a candidate) */
-  int64_t reserved[3];
+  int32_t mg_level_sweeps;  /* chosen */
+  int32_t reserved32;
+  int64_t reserved[2];
 } fsg_tune;
 int fsg_autotune(fsg_domain *h, int32_t scheme, float dt, int32_t steps, fsg_tune *out);
 

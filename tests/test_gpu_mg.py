@@ -195,3 +195,20 @@ def test_graph_replay_matches_eager_launches():
         print(f"solve {k}: {ig} iterations (graph) / {ie} (eager)")
         assert ig == ie
         assert np.array_equal(pg.view(np.uint32), pe.view(np.uint32))
+
+
+@pytest.mark.parametrize("sweeps", [2, 4])
+@pytest.mark.parametrize("nx,ny,seed", [(513, 257, 21), (1030, 20, 22), (300, 300, 23)])
+def test_mg_level_sweeps_against_fp64(nx, ny, seed, sweeps):
+    """Two or four smoothing sweeps (mg_level_sweeps; four: the degree-4 Chebyshev
+    weights on [0.25, 2]) on the tile levels below level 0: a different symmetric
+    preconditioner, the same system solved to the same FP64 residual bound."""
+    lab = irregular_labels(nx, ny, seed)
+    d, g = _solve_pair(lab, seed, mg_level_sweeps=sweeps)
+    st = g.stats()
+    liq = lab == LIQ
+    ep = rel_err(g.pressure()[liq], d.pressure[liq])
+    print(f"level sweeps {sweeps}: {st.cg_iterations} iterations, |r|/|b| {st.cg_error:.2e}, "
+          f"p vs fp64 {ep:.2e}")
+    assert st.cg_error <= 1e-10
+    assert ep <= 1e-6
