@@ -926,7 +926,7 @@ void fsg_params_default(fsg_params* p) {
   p->mg_fuse_update = 1;
   p->splat_order = 1;
   p->mg_tile_threads = 256;
-  p->mg_level_sweeps = 4;
+  p->mg_level_sweeps = 5;
 }
 
 int fsg_create(fsg_domain** out, int32_t nx, int32_t ny, float length_x, float length_y,
@@ -1033,7 +1033,7 @@ int fsg_set_params(fsg_domain* h, const fsg_params* p) {
       p->graph_iters < 1 || p->graph_iters > 16 || p->mg_fuse_update < 0 ||
       p->mg_fuse_update > 1 || p->splat_order < 0 || p->splat_order > 1 ||
       (p->mg_tile_threads != 256 && p->mg_tile_threads != 512 && p->mg_tile_threads != 1024) ||
-      (p->mg_level_sweeps != 2 && p->mg_level_sweeps != 4))
+      p->mg_level_sweeps < 2 || p->mg_level_sweeps > 6)
     return fail(h, FSG_E_INVALID, "fsg_set_params: value out of range");
   const size_t C = ncell(h);
   // fixed-point sums: 4 per cell (the P2G accumulators; the density uses the first C),
@@ -1975,19 +1975,20 @@ int fsg_autotune(fsg_domain* h, int32_t scheme, float dt, int32_t steps, fsg_tun
   if (h->np == 0 && scheme != FSG_SCHEME_GRID_ADVECT)
     return fail(h, FSG_E_INVALID, "fsg_autotune: no particles to step");
   struct Cand {
-    int smoother, coarse, fuse, iters, order, threads;
+    int sweeps, coarse, fuse, iters, order, threads;
   };
   std::vector<Cand> cands;
   const fsg_params p0 = h->prm;
   const bool can_fuse = !h->batch;
   const bool has_particles = h->np > 0;
-  for (int sm : {1, 0})
+  // the smoother of level 0 and of the one-workgroup levels stays the handle's
+  for (int ls : {5, 3, 2})
     for (int co : {0, 8, 16})
       for (int fu : can_fuse ? std::initializer_list<int>{1, 0} : std::initializer_list<int>{1})
         for (int gi : {2, 4})
           for (int so : has_particles ? std::initializer_list<int>{1, 0}
                                       : std::initializer_list<int>{p0.splat_order})
-            for (int nt : {256, 512}) cands.push_back({sm, co, fu, gi, so, nt});
+            for (int nt : {256, 512}) cands.push_back({ls, co, fu, gi, so, nt});
   TuneState saved;
   if (int rc = tune_save(h, saved)) return rc;
   const fsg_stats stats0 = h->host_stats;
@@ -1998,7 +1999,7 @@ int fsg_autotune(fsg_domain* h, int32_t scheme, float dt, int32_t steps, fsg_tun
   for (size_t k = 0; k < cands.size(); ++k) {
     const Cand& c = cands[k];
     fsg_params q = p0;
-    q.mg_smoother = c.smoother;
+    q.mg_level_sweeps = c.sweeps;
     q.mg_coarse_sweeps = c.coarse;
     q.mg_fuse_update = c.fuse;
     q.graph_iters = c.iters;
@@ -2018,7 +2019,7 @@ int fsg_autotune(fsg_domain* h, int32_t scheme, float dt, int32_t steps, fsg_tun
       its = h->host_stats.cg_iterations;
     }
     const double ms = t_sum / steps;
-    const bool is_cur = c.smoother == p0.mg_smoother && c.coarse == p0.mg_coarse_sweeps &&
+    const bool is_cur = c.sweeps == p0.mg_level_sweeps && c.coarse == p0.mg_coarse_sweeps &&
                         c.fuse == p0.mg_fuse_update && c.iters == p0.graph_iters &&
                         c.order == p0.splat_order && c.threads == p0.mg_tile_threads;
     if (is_cur) {
@@ -2041,7 +2042,7 @@ int fsg_autotune(fsg_domain* h, int32_t scheme, float dt, int32_t steps, fsg_tun
     best_iters = cur_iters;
   }
   fsg_params q = p0;
-  q.mg_smoother = cands[best_k].smoother;
+  q.mg_level_sweeps = cands[best_k].sweeps;
   q.mg_coarse_sweeps = cands[best_k].coarse;
   q.mg_fuse_update = cands[best_k].fuse;
   q.graph_iters = cands[best_k].iters;
@@ -2060,6 +2061,7 @@ int fsg_autotune(fsg_domain* h, int32_t scheme, float dt, int32_t steps, fsg_tun
     out->graph_iters = q.graph_iters;
     out->splat_order = q.splat_order;
     out->mg_tile_threads = q.mg_tile_threads;
+    out->mg_level_sweeps = q.mg_level_sweeps;
     out->candidates = (int32_t)cands.size();
     out->cg_iterations = best_iters;
     out->ms_per_step = best;

@@ -201,6 +201,7 @@ hipError_t pcg_set_lds_limit(size_t max_bytes);
 // multigrid-preconditioned CG with FP64 solution and residual (dense cell grids)
 constexpr int kMgMaxLevels = 16;
 constexpr int kMgMaxVecBlocks = 1024;  // blocks of the vector kernels (partials per kernel)
+constexpr int kMgMaxSweeps = 6;     // smoothing sweeps of the tile levels below level 0, at most
 constexpr int kMgCoarseCells = 1024;   // levels at or below this size run in one workgroup
 
 // Smoothing: every level's pre- and post-smoothing is two relaxation sweeps

@@ -590,11 +590,11 @@ struct MgXr {
 __device__ __forceinline__ void coarse_stage(const MgCoarseArgs& a, double* part_rz,
                                              bool coherent);
 
-// Smoothing weights of the tile levels below level 0 when they take S = 4 sweeps
+// Smoothing weights of the tile levels below level 0 when they take S > 2 sweeps
 // before and after the coarse correction (sweep k uses w[k - 1]); with S = 2 every
 // level uses the kernels' omega, omega2.
 struct MgSweeps {
-  float w[4];
+  float w[kMgMaxSweeps];
 };
 
 // Block span clock (span != null): thread 0 stores the block's start and end ticks
@@ -613,9 +613,9 @@ struct BlockSpan {
   }
 };
 
-// S sweeps (2, or 4 on the levels below level 0): the right-hand side and info on
-// [-(S + 1), 33 + S), sweep k on a ring one cell narrower than sweep k - 1, the last on
-// [-2, 34); the sweeps alternate between the x1 and x2 buffers.
+// S sweeps (2 on level 0, 2 .. kMgMaxSweeps on the levels below): the right-hand side
+// and info on [-(S + 1), 33 + S), sweep k on a ring one cell narrower than sweep k - 1,
+// the last on [-2, 34); the sweeps alternate between the x1 and x2 buffers.
 template <bool L0, int NT, bool XR = false, bool TAIL = false, int S = 2>
 __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omega, float omega2,
                                                  int it,
@@ -626,7 +626,8 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
                                                  MgSweeps sw) {
   static_assert(!XR || L0, "the fused update belongs to level 0");
   static_assert(!TAIL || !L0, "the one-workgroup stage follows the last tile level");
-  static_assert(S == 2 || (S == 4 && !L0), "level 0 takes two sweeps; smaller levels 2 or 4");
+  static_assert(S == 2 || (S > 2 && S <= kMgMaxSweeps && !L0),
+                "level 0 takes two sweeps; smaller levels 2 .. kMgMaxSweeps");
   const BlockSpan clock(span);
   __shared__ double sm[NT / 64];
   __shared__ int s_last;
@@ -672,11 +673,17 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
   constexpr int W4 = 40;  // fused form: x_new on [-4, 36)
   constexpr int WC = 18;  // coarse cells of the tile and their neighbours
   __shared__ float s_r[W3 * W3], s_x1[W3 * W3], s_x2[W2 * W2];
-  // the residual reuses x1's storage: x1 is dead once the next sweep is formed (with
-  // S = 4, the third sweep's x1-sized buffer is dead once the fourth is), and the
-  // smaller footprint lets more tiles share a CU
-  float* const s_res = s_x1;
-  const float wk[4] = {S == 4 ? sw.w[0] : omega, S == 4 ? sw.w[1] : omega2, sw.w[2], sw.w[3]};
+  // the last sweep lands in x2 (S even) or x1 (S odd); the residual reuses the other
+  // buffer, dead by then, so the smaller footprint lets more tiles share a CU
+  float* const s_res = (S & 1) ? s_x2 : s_x1;
+  const float* const s_xs = (S & 1) ? s_x1 : s_x2;
+  float wk[kMgMaxSweeps];
+#pragma unroll
+  for (int k = 0; k < kMgMaxSweeps; ++k) wk[k] = sw.w[k];
+  if (S == 2) {
+    wk[0] = omega;
+    wk[1] = omega2;
+  }
   __shared__ uint8_t s_inf[W3 * W3], s_cinf[WC * WC];
   __shared__ double s_xn[XR ? W4 * W4 : 1];
   const int tiles_x = (c.nx + 15) >> 4;
@@ -844,7 +851,7 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
   }
   __syncthreads();
   // sweeps 2 .. S: from the buffer of window win (offset d inside the right-hand
-  // side's window) to one of window win - 2; the last lands in s_x2 ([-2, 34))
+  // side's window) to one of window win - 2; the last lands in s_xs ([-2, 34))
 #pragma unroll
   for (int k = 2; k <= S; ++k) {
     const int win = W3 - 2 * (k - 2), wout = win - 2, d = k - 2;
@@ -872,10 +879,10 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
     const int q = (li + 1) + (lj + 1) * WS;
     const int q3 = (li + S) + (lj + S) * W3;
     const uint8_t fv = s_inf[q3];
-    const float x = s_x2[q];
+    const float x = s_xs[q];
     float res = 0.f;
     if (is_act(fv)) {
-      const float kx = nns_of(fv) * x - (s_x2[q - 1] + s_x2[q + 1] + s_x2[q - WS] + s_x2[q + WS]);
+      const float kx = nns_of(fv) * x - (s_xs[q - 1] + s_xs[q + 1] + s_xs[q - WS] + s_xs[q + WS]);
       res = s_r[q3] - kx;
     }
     s_res[t] = res;
@@ -926,14 +933,15 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
 // solution around the tile, r and info on [-S, 32 + S), the corrected x = xs + P e
 // there, then S post-sweeps, each on a ring one cell narrower than the one before (the
 // last on the tile, written to xo), alternating between the xc and y buffers.  S = 2
-// (level 0), or 2 or 4 on the levels below (sweep k uses w[k - 1]).
+// (level 0), or 2 .. kMgMaxSweeps on the levels below (sweep k uses w[k - 1]).
 // Level 0 also forms the partial of r.z (z = xo) for the next direction.
 template <bool L0, int NT, int S = 2>
 __global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega, float omega2,
                                                const CgScalars* sc, double* __restrict__ part_rz,
                                                int it, const uint8_t* __restrict__ tflag,
                                                MgSweeps sw) {
-  static_assert(S == 2 || (S == 4 && !L0), "level 0 takes two sweeps; smaller levels 2 or 4");
+  static_assert(S == 2 || (S > 2 && S <= kMgMaxSweeps && !L0),
+                "level 0 takes two sweeps; smaller levels 2 .. kMgMaxSweeps");
   constexpr int W2 = 32 + 2 * S, W1 = W2 - 2;  // S = 2: 36, 34
   // coarse cells from fi0 / 2 - (S / 2 + 1): every coarse neighbour of the fine cells
   // [-S, 32 + S)
@@ -941,7 +949,13 @@ __global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega,
   __shared__ float s_r[W2 * W2], s_xc[W2 * W2], s_y1[W1 * W1], s_e[WC * WC];
   __shared__ uint8_t s_inf[W2 * W2], s_cinf[WC * WC];
   __shared__ double sm[NT / 64];
-  const float wk[4] = {S == 4 ? sw.w[0] : omega, S == 4 ? sw.w[1] : omega2, sw.w[2], sw.w[3]};
+  float wk[kMgMaxSweeps];
+#pragma unroll
+  for (int k = 0; k < kMgMaxSweeps; ++k) wk[k] = sw.w[k];
+  if (S == 2) {
+    wk[0] = omega;
+    wk[1] = omega2;
+  }
   const int tiles_x = (f.nx + 31) >> 5;
   // each XCD takes one band of consecutive tiles, so halo rows are shared in its L2
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -1045,8 +1059,10 @@ __global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega,
     }
     __syncthreads();
   }
-  // the last post-sweep, on the tile, from the window [-1, 33) (in y1: S even)
+  // the last post-sweep, on the tile, from the window [-1, 33) (in y1 for S even,
+  // in xc for S odd)
   constexpr int WL = 34;
+  const float* const s_yl = (S & 1) ? s_xc : s_y1;
   double rz = 0.0;
   const float wl = wk[S - 1];
   for (int t = threadIdx.x; t < 32 * 32; t += NT) {
@@ -1058,8 +1074,8 @@ __global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega,
     const uint8_t fv = s_inf[q2];
     float v = 0.f;
     if (is_act(fv)) {
-      const float n = nns_of(fv), x = s_y1[q];
-      const float kx = n * x - (s_y1[q - 1] + s_y1[q + 1] + s_y1[q - WL] + s_y1[q + WL]);
+      const float n = nns_of(fv), x = s_yl[q];
+      const float kx = n * x - (s_yl[q - 1] + s_yl[q + 1] + s_yl[q - WL] + s_yl[q + WL]);
       v = x + wl * (s_r[q2] - kx) * inv_nns(fv);
     }
     f.xo[i + j * f.nx] = v;
@@ -2144,9 +2160,13 @@ hipError_t mg_set_lds_limit(size_t bytes) {
   e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_down<false, kTileBig, false, true>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_down<false, kTileBig, false, true, 4>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  if (e != hipSuccess) return e;
+  for (const void* f : {reinterpret_cast<const void*>(&k_mg_down<false, kTileBig, false, true, 3>),
+                        reinterpret_cast<const void*>(&k_mg_down<false, kTileBig, false, true, 4>),
+                        reinterpret_cast<const void*>(&k_mg_down<false, kTileBig, false, true, 5>),
+                        reinterpret_cast<const void*>(&k_mg_down<false, kTileBig, false, true, 6>)}) {
+    e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+  }
   return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_scene_coarse),
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
@@ -2249,18 +2269,22 @@ static MgLevel mg_level(const MgHost& m, int l) {
   return MgLevel{m.nx[l], m.ny[l], m.inf[l], m.rl[l], m.xs[l], m.xo[l]};
 }
 
-// The tile levels' four-sweep weights: the reciprocal roots of the degree-4 Chebyshev
-// polynomial on [0.25, 2] (the high end of the spectrum of D^-1 K), smallest first
-// (every partial product of the factors stays within [-1, 1] on [0, 2]).
+// The tile levels' weights for S > 2 sweeps: the reciprocal roots of the degree-S
+// Chebyshev polynomial on [lo_S, 2] (the high end of the spectrum of D^-1 K), smallest
+// first (every partial product of the factors stays within [-1, 1] on [0, 2]).  lo_S
+// per degree from a model of the solve over developed dam-break domains (fewest
+// iterations): 0.2, 0.2, 0.05, 0.05 for S = 3 .. 6.
 static MgSweeps mg_sweeps(const MgHost& m) {
   MgSweeps w{};
-  if (m.level_sweeps != 4) return w;
-  const double lo = 0.25, hi = 2.0, pi = 3.14159265358979323846;
-  double r[4];
-  for (int k = 0; k < 4; ++k)
-    r[k] = 1.0 / ((hi + lo) / 2 + (hi - lo) / 2 * std::cos(pi * (2 * k + 1) / 8.0));
-  std::sort(r, r + 4);
-  for (int k = 0; k < 4; ++k) w.w[k] = (float)r[k];
+  const int S = m.level_sweeps;
+  if (S <= 2 || S > kMgMaxSweeps) return w;
+  static const double kLo[kMgMaxSweeps + 1] = {0, 0, 0.4, 0.2, 0.2, 0.05, 0.05};
+  const double lo = kLo[S], hi = 2.0, pi = 3.14159265358979323846;
+  double r[kMgMaxSweeps];
+  for (int k = 0; k < S; ++k)
+    r[k] = 1.0 / ((hi + lo) / 2 + (hi - lo) / 2 * std::cos(pi * (2 * k + 1) / (2.0 * S)));
+  std::sort(r, r + S);
+  for (int k = 0; k < S; ++k) w.w[k] = (float)r[k];
   return w;
 }
 
@@ -2322,10 +2346,13 @@ void launch_mg_down(hipStream_t s, MgHost& m, int l, int it, int max_iter, doubl
             mg_level(m, l), mg_level(m, l + 1), m.coarse.omega, m.coarse.omega2, it, chk, check,
             m.tflag_l[l], MgXr{}, nullptr, MgCoarseArgs{}, nullptr, sw);
     };
-    if (m.level_sweeps == 4)
-      go(std::integral_constant<int, 4>{});
-    else
-      go(std::integral_constant<int, 2>{});
+    switch (m.level_sweeps) {
+      case 3: go(std::integral_constant<int, 3>{}); break;
+      case 4: go(std::integral_constant<int, 4>{}); break;
+      case 5: go(std::integral_constant<int, 5>{}); break;
+      case 6: go(std::integral_constant<int, 6>{}); break;
+      default: go(std::integral_constant<int, 2>{}); break;
+    }
   }
 }
 
@@ -2355,14 +2382,22 @@ void launch_mg_up(hipStream_t s, const MgHost& m, int l, int it, CgScalars* sc) 
     go0(std::integral_constant<int, 512>{});
   else if (l == 0)
     go0(std::integral_constant<int, kT>{});
-  else if (m.level_sweeps == 4)
-    k_mg_up<false, kTileBig, 4><<<blocks, kTileBig, 0, s>>>(mg_level(m, l), mg_level(m, l + 1),
-                                                            m.coarse.omega, m.coarse.omega2, sc,
-                                                            nullptr, it, m.tflag_l[l], mg_sweeps(m));
-  else
-    k_mg_up<false, kTileBig><<<blocks, kTileBig, 0, s>>>(mg_level(m, l), mg_level(m, l + 1),
-                                                         m.coarse.omega, m.coarse.omega2, sc, nullptr, it,
-                                                         m.tflag_l[l], MgSweeps{});
+  else {
+    auto go = [&](auto ns) {
+      constexpr int S = decltype(ns)::value;
+      k_mg_up<false, kTileBig, S><<<blocks, kTileBig, 0, s>>>(mg_level(m, l), mg_level(m, l + 1),
+                                                              m.coarse.omega, m.coarse.omega2, sc,
+                                                              nullptr, it, m.tflag_l[l],
+                                                              mg_sweeps(m));
+    };
+    switch (m.level_sweeps) {
+      case 3: go(std::integral_constant<int, 3>{}); break;
+      case 4: go(std::integral_constant<int, 4>{}); break;
+      case 5: go(std::integral_constant<int, 5>{}); break;
+      case 6: go(std::integral_constant<int, 6>{}); break;
+      default: go(std::integral_constant<int, 2>{}); break;
+    }
+  }
 }
 
 int mg_dir_blocks(const MgHost& m) {

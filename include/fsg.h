@@ -168,11 +168,13 @@ typedef struct fsg_params {
                              unknowns; fewer leave room for more tiles per CU) */
   int32_t mg_level_sweeps;/* multigrid solver: smoothing sweeps before and after the
                              coarse correction on the tile levels below level 0 (level 0
-                             and the one-workgroup levels keep two): 4 (default), the
-                             degree-4 Chebyshev smoother on [0.25, 2] (those levels are
-                             bound by launch and barrier latency, so the extra sweeps
-                             cost little and save iterations: 12.4 -> 10 per solve over
-                             the 1024^2 dam break), or 2, the mg_smoother weights */
+                             and the one-workgroup levels keep two), 2 .. 6: 2 uses the
+                             mg_smoother weights; S > 2 the degree-S Chebyshev weights
+                             on [lo_S, 2] (lo = 0.2, 0.2, 0.05, 0.05 for S = 3 .. 6).
+                             Those levels are bound by launch and barrier latency, so
+                             the extra sweeps cost little and save iterations: 5
+                             (default) takes the developed 1024^2 dam break from 13 to 9
+                             iterations per solve */
 } fsg_params;
 
 /* Counters of the last pressure solve and running totals. */

@@ -11,7 +11,7 @@ from fluid_simulation_amd import fsg, scenes
 
 pytestmark = pytest.mark.gpu
 
-TUNED = ("mg_smoother", "mg_coarse_sweeps", "mg_fuse_update", "graph_iters", "splat_order",
+TUNED = ("mg_level_sweeps", "mg_coarse_sweeps", "mg_fuse_update", "graph_iters", "splat_order",
          "mg_tile_threads")
 
 
@@ -31,7 +31,7 @@ def test_autotune_restores_the_state_and_keeps_the_fastest():
     it_before = a.stats().cg_iterations
     t = a.autotune(fsg.PICFLIP, sc.dt, steps=2)
     print(f"autotune: {t}", flush=True)
-    assert t["candidates"] == 96 and t["ms_per_step"] > 0
+    assert t["candidates"] == 144 and t["ms_per_step"] > 0
     assert t["ms_per_step"] <= t["ms_per_step_before"]
     after = _state(a)
     assert all(np.array_equal(x, y) for x, y in zip(before, after))
@@ -53,7 +53,7 @@ def test_autotune_batch():
     lab = bt.labels()
     t = bt.autotune(fsg.PICFLIP, 0.01, steps=1)
     print(f"batch autotune: {t}", flush=True)
-    assert t["candidates"] == 48 and t["mg_fuse_update"] == 1
+    assert t["candidates"] == 72 and t["mg_fuse_update"] == 1
     assert np.array_equal(bt.labels(), lab)
     bt.step(fsg.PICFLIP, 0.01)
     st = bt.stats()

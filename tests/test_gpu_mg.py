@@ -197,11 +197,11 @@ def test_graph_replay_matches_eager_launches():
         assert np.array_equal(pg.view(np.uint32), pe.view(np.uint32))
 
 
-@pytest.mark.parametrize("sweeps", [2, 4])
+@pytest.mark.parametrize("sweeps", [2, 3, 4, 5, 6])
 @pytest.mark.parametrize("nx,ny,seed", [(513, 257, 21), (1030, 20, 22), (300, 300, 23)])
 def test_mg_level_sweeps_against_fp64(nx, ny, seed, sweeps):
-    """Two or four smoothing sweeps (mg_level_sweeps; four: the degree-4 Chebyshev
-    weights on [0.25, 2]) on the tile levels below level 0: a different symmetric
+    """2 .. 6 smoothing sweeps (mg_level_sweeps; more than two: the degree-S
+    Chebyshev weights) on the tile levels below level 0: a different symmetric
     preconditioner, the same system solved to the same FP64 residual bound."""
     lab = irregular_labels(nx, ny, seed)
     d, g = _solve_pair(lab, seed, mg_level_sweeps=sweeps)
