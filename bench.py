@@ -167,7 +167,7 @@ def log(*a):
 GRAPH = 1
 SMOOTHER = 1
 # the multigrid settings fsg_autotune chooses
-TUNED = ("mg_level_sweeps", "mg_coarse_sweeps", "mg_fuse_update", "graph_iters", "splat_order",
+TUNED = ("mg_level_sweeps", "mg_level0_sweeps", "mg_coarse_sweeps", "mg_fuse_update", "graph_iters", "splat_order",
          "mg_tile_threads")
 
 
@@ -990,6 +990,7 @@ def main():
             "mg_smoother": ["damped Jacobi 0.8", "two-step Chebyshev [0.4, 2]"][args.mg_smoother]
                            + " on level 0 and the one-workgroup levels",
             "mg_level_sweeps": (tune or {}).get("mg_level_sweeps", 5),
+            "mg_level0_sweeps": (tune or {}).get("mg_level0_sweeps", 3),
             "tol": args.tol,
             "timed_steps": [t_start, t_start + args.steps - 1],
             "autotune": tune,

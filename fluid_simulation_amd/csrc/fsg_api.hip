@@ -550,6 +550,7 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
   m.fused = !h->batch && m.lc >= 2 && h->prm.mg_fuse_update != 0;
   m.l0_threads = h->prm.mg_tile_threads;
   m.level_sweeps = h->prm.mg_level_sweeps;
+  m.level0_sweeps = h->prm.mg_level0_sweeps;
   m.clk_on = h->kclock == 1;
   m.clk_span_on = h->kclock == 2;
   if (m.clk_span_on && !m.clk_span) alloc_clk_span(h);
@@ -927,6 +928,7 @@ void fsg_params_default(fsg_params* p) {
   p->splat_order = 1;
   p->mg_tile_threads = 256;
   p->mg_level_sweeps = 5;
+  p->mg_level0_sweeps = 3;
 }
 
 int fsg_create(fsg_domain** out, int32_t nx, int32_t ny, float length_x, float length_y,
@@ -1033,7 +1035,8 @@ int fsg_set_params(fsg_domain* h, const fsg_params* p) {
       p->graph_iters < 1 || p->graph_iters > 16 || p->mg_fuse_update < 0 ||
       p->mg_fuse_update > 1 || p->splat_order < 0 || p->splat_order > 1 ||
       (p->mg_tile_threads != 256 && p->mg_tile_threads != 512 && p->mg_tile_threads != 1024) ||
-      p->mg_level_sweeps < 2 || p->mg_level_sweeps > 6)
+      p->mg_level_sweeps < 2 || p->mg_level_sweeps > 6 || p->mg_level0_sweeps < 2 ||
+      p->mg_level0_sweeps > 3)
     return fail(h, FSG_E_INVALID, "fsg_set_params: value out of range");
   const size_t C = ncell(h);
   // fixed-point sums: 4 per cell (the P2G accumulators; the density uses the first C),
@@ -1059,7 +1062,8 @@ int fsg_set_params(fsg_domain* h, const fsg_params* p) {
   if (p->cg_tol != h->prm.cg_tol || p->cg_max_iter != h->prm.cg_max_iter ||
       p->mg_smoother != h->prm.mg_smoother || p->mg_coarse_sweeps != h->prm.mg_coarse_sweeps ||
       p->mg_fuse_update != h->prm.mg_fuse_update || p->mg_tile_threads != h->prm.mg_tile_threads ||
-      p->mg_level_sweeps != h->prm.mg_level_sweeps)
+      p->mg_level_sweeps != h->prm.mg_level_sweeps ||
+      p->mg_level0_sweeps != h->prm.mg_level0_sweeps)
     h->mg_graphs.clear();
   h->prm = *p;
   return FSG_OK;
@@ -1975,20 +1979,22 @@ int fsg_autotune(fsg_domain* h, int32_t scheme, float dt, int32_t steps, fsg_tun
   if (h->np == 0 && scheme != FSG_SCHEME_GRID_ADVECT)
     return fail(h, FSG_E_INVALID, "fsg_autotune: no particles to step");
   struct Cand {
-    int sweeps, coarse, fuse, iters, order, threads;
+    int sweeps, sweeps0, coarse, fuse, iters, order, threads;
   };
   std::vector<Cand> cands;
   const fsg_params p0 = h->prm;
   const bool can_fuse = !h->batch;
   const bool has_particles = h->np > 0;
-  // the smoother of level 0 and of the one-workgroup levels stays the handle's
-  for (int ls : {5, 3, 2})
+  // smoothing sweeps (levels below 0, level 0): three presets; the smoother weights of
+  // the two-sweep levels stay the handle's
+  static const int kSweeps[3][2] = {{5, 3}, {5, 2}, {2, 2}};
+  for (const auto& ls : kSweeps)
     for (int co : {0, 8, 16})
       for (int fu : can_fuse ? std::initializer_list<int>{1, 0} : std::initializer_list<int>{1})
         for (int gi : {2, 4})
           for (int so : has_particles ? std::initializer_list<int>{1, 0}
                                       : std::initializer_list<int>{p0.splat_order})
-            for (int nt : {256, 512}) cands.push_back({ls, co, fu, gi, so, nt});
+            for (int nt : {256, 512}) cands.push_back({ls[0], ls[1], co, fu, gi, so, nt});
   TuneState saved;
   if (int rc = tune_save(h, saved)) return rc;
   const fsg_stats stats0 = h->host_stats;
@@ -2000,6 +2006,7 @@ int fsg_autotune(fsg_domain* h, int32_t scheme, float dt, int32_t steps, fsg_tun
     const Cand& c = cands[k];
     fsg_params q = p0;
     q.mg_level_sweeps = c.sweeps;
+    q.mg_level0_sweeps = c.sweeps0;
     q.mg_coarse_sweeps = c.coarse;
     q.mg_fuse_update = c.fuse;
     q.graph_iters = c.iters;
@@ -2019,7 +2026,8 @@ int fsg_autotune(fsg_domain* h, int32_t scheme, float dt, int32_t steps, fsg_tun
       its = h->host_stats.cg_iterations;
     }
     const double ms = t_sum / steps;
-    const bool is_cur = c.sweeps == p0.mg_level_sweeps && c.coarse == p0.mg_coarse_sweeps &&
+    const bool is_cur = c.sweeps == p0.mg_level_sweeps && c.sweeps0 == p0.mg_level0_sweeps &&
+                        c.coarse == p0.mg_coarse_sweeps &&
                         c.fuse == p0.mg_fuse_update && c.iters == p0.graph_iters &&
                         c.order == p0.splat_order && c.threads == p0.mg_tile_threads;
     if (is_cur) {
@@ -2043,6 +2051,7 @@ int fsg_autotune(fsg_domain* h, int32_t scheme, float dt, int32_t steps, fsg_tun
   }
   fsg_params q = p0;
   q.mg_level_sweeps = cands[best_k].sweeps;
+  q.mg_level0_sweeps = cands[best_k].sweeps0;
   q.mg_coarse_sweeps = cands[best_k].coarse;
   q.mg_fuse_update = cands[best_k].fuse;
   q.graph_iters = cands[best_k].iters;
@@ -2062,6 +2071,7 @@ int fsg_autotune(fsg_domain* h, int32_t scheme, float dt, int32_t steps, fsg_tun
     out->splat_order = q.splat_order;
     out->mg_tile_threads = q.mg_tile_threads;
     out->mg_level_sweeps = q.mg_level_sweeps;
+    out->mg_level0_sweeps = q.mg_level0_sweeps;
     out->candidates = (int32_t)cands.size();
     out->cg_iterations = best_iters;
     out->ms_per_step = best;

@@ -273,6 +273,7 @@ struct MgHost {
   // sweeps before and after the coarse correction on the tile levels below level 0: 2
   // (the level's omega, omega2) or 4 (degree-4 Chebyshev weights)
   int level_sweeps = 2;
+  int level0_sweeps = 2;  // the same on level 0: 2 (omega, omega2) or 3 (degree-3 Chebyshev)
   // the last tile level's down kernel also runs the one-workgroup stage (its last
   // workgroup to finish), so no k_mg_coarse launch; ticket: its arrival counter
   bool coarse_tail = true;

@@ -626,8 +626,8 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
                                                  MgSweeps sw) {
   static_assert(!XR || L0, "the fused update belongs to level 0");
   static_assert(!TAIL || !L0, "the one-workgroup stage follows the last tile level");
-  static_assert(S == 2 || (S > 2 && S <= kMgMaxSweeps && !L0),
-                "level 0 takes two sweeps; smaller levels 2 .. kMgMaxSweeps");
+  static_assert(S >= 2 && S <= kMgMaxSweeps && (!L0 || S <= 3),
+                "level 0 takes two or three sweeps; smaller levels 2 .. kMgMaxSweeps");
   const BlockSpan clock(span);
   __shared__ double sm[NT / 64];
   __shared__ int s_last;
@@ -670,7 +670,7 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
     return;
   }
   constexpr int W3 = 34 + 2 * S, W2 = W3 - 2, W1 = 34;  // S = 2: 38, 36, 34
-  constexpr int W4 = 40;  // fused form: x_new on [-4, 36)
+  constexpr int W4 = W3 + 2;  // fused form: x_new on [-(S + 2), 34 + S)
   constexpr int WC = 18;  // coarse cells of the tile and their neighbours
   __shared__ float s_r[W3 * W3], s_x1[W3 * W3], s_x2[W2 * W2];
   // the last sweep lands in x2 (S even) or x1 (S odd); the residual reuses the other
@@ -740,7 +740,7 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
 #pragma unroll
     for (int k = 0; k < NF; ++k) {
       const int t = threadIdx.x + k * NT;
-      const int i = fi0 + t % W3 - 3, j = fj0 + t / W3 - 3;
+      const int i = fi0 + t % W3 - (S + 1), j = fj0 + t / W3 - (S + 1);
       const bool in = t < W3 * W3 && i >= 0 && i < f.nx && j >= 0 && j < f.ny;
       const int q = in ? i + j * f.nx : 0;
       const uint8_t fq = f.inf[q];
@@ -751,7 +751,7 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
 #pragma unroll
     for (int k = 0; k < N4; ++k) {
       const int t = threadIdx.x + k * NT;
-      const int i = fi0 + t % W4 - 4, j = fj0 + t / W4 - 4;
+      const int i = fi0 + t % W4 - (S + 2), j = fj0 + t / W4 - (S + 2);
       const bool in = t < W4 * W4 && i >= 0 && i < f.nx && j >= 0 && j < f.ny;
       const int q = in ? i + j * f.nx : 0;
       const double xq = xr.x_old[q];
@@ -777,8 +777,8 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
       if (t < W4 * W4) s_xn[t] = xo[k] + alpha * (double)pv[k];
     }
     __syncthreads();
-    // r = b' - K x_new on [-3, 35) (cells outside the grid hold x = 0 and add 0);
-    // the tile's own cells store x_new and r and add to |r|^2
+    // r = b' - K x_new on [-(S + 1), 33 + S) (cells outside the grid hold x = 0 and
+    // add 0); the tile's own cells store x_new and r and add to |r|^2
     double acc = 0.0;
 #pragma unroll
     for (int k = 0; k < NF; ++k) {
@@ -796,13 +796,13 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
         sn += s_xn[q4 - W4];
         sn += s_xn[q4 + W4];
         r64 = bv[k] - ((double)(fq & 7) * xc - sn);
-        acc += (li >= 3 && li < 35 && lj >= 3 && lj < 35) ? r64 * r64 : 0.0;
+        acc += (li > S && li < 33 + S && lj > S && lj < 33 + S) ? r64 * r64 : 0.0;
       }
       const float rv = (float)r64;
       s_r[t] = rv;
-      s_x1[t] = is_act(fq) ? omega * rv * inv_nns(fq) : 0.f;
-      const int i = fi0 + li - 3, j = fj0 + lj - 3;
-      if (li >= 3 && li < 35 && lj >= 3 && lj < 35 && i < f.nx && j < f.ny) {
+      s_x1[t] = is_act(fq) ? wk[0] * rv * inv_nns(fq) : 0.f;
+      const int i = fi0 + li - (S + 1), j = fj0 + lj - (S + 1);
+      if (li > S && li < 33 + S && lj > S && lj < 33 + S && i < f.nx && j < f.ny) {
         xr.x_new[i + j * f.nx] = is_act(fq) ? xc : 0.0;
         f.r[i + j * f.nx] = rv;
       }
@@ -940,8 +940,8 @@ __global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega,
                                                const CgScalars* sc, double* __restrict__ part_rz,
                                                int it, const uint8_t* __restrict__ tflag,
                                                MgSweeps sw) {
-  static_assert(S == 2 || (S > 2 && S <= kMgMaxSweeps && !L0),
-                "level 0 takes two sweeps; smaller levels 2 .. kMgMaxSweeps");
+  static_assert(S >= 2 && S <= kMgMaxSweeps && (!L0 || S <= 3),
+                "level 0 takes two or three sweeps; smaller levels 2 .. kMgMaxSweeps");
   constexpr int W2 = 32 + 2 * S, W1 = W2 - 2;  // S = 2: 36, 34
   // coarse cells from fi0 / 2 - (S / 2 + 1): every coarse neighbour of the fine cells
   // [-S, 32 + S)
@@ -2274,9 +2274,8 @@ static MgLevel mg_level(const MgHost& m, int l) {
 // first (every partial product of the factors stays within [-1, 1] on [0, 2]).  lo_S
 // per degree from a model of the solve over developed dam-break domains (fewest
 // iterations): 0.2, 0.2, 0.05, 0.05 for S = 3 .. 6.
-static MgSweeps mg_sweeps(const MgHost& m) {
+static MgSweeps cheb_sweeps(int S) {
   MgSweeps w{};
-  const int S = m.level_sweeps;
   if (S <= 2 || S > kMgMaxSweeps) return w;
   static const double kLo[kMgMaxSweeps + 1] = {0, 0, 0.4, 0.2, 0.2, 0.05, 0.05};
   const double lo = kLo[S], hi = 2.0, pi = 3.14159265358979323846;
@@ -2287,6 +2286,9 @@ static MgSweeps mg_sweeps(const MgHost& m) {
   for (int k = 0; k < S; ++k) w.w[k] = (float)r[k];
   return w;
 }
+static MgSweeps mg_sweeps(const MgHost& m) { return cheb_sweeps(m.level_sweeps); }
+// level 0: three sweeps take the degree-3 weights on [0.2, 2]
+static MgSweeps mg_sweeps0(const MgHost& m) { return cheb_sweeps(m.level0_sweeps); }
 
 // One CG iteration = V-cycle (z = B r) + direction + update.  Level 0's down kernel
 // (or the one-workgroup kernel when the hierarchy starts there) runs the stop test
@@ -2312,24 +2314,30 @@ void launch_mg_down(hipStream_t s, MgHost& m, int l, int it, int max_iter, doubl
                                    mg_dir_blocks(m), m.part_rr}
                             : MgXr{};
     const int ck = m.fused ? 0 : check;
-    auto go = [&](auto nt) {
-      constexpr int NT = decltype(nt)::value;
+    const MgSweeps sw0 = mg_sweeps0(m);
+    auto go = [&](auto nt, auto ns) {
+      constexpr int NT = decltype(nt)::value, S = decltype(ns)::value;
       if (m.fused)
-        k_mg_down<true, NT, true><<<blocks, NT, 0, s>>>(mg_level(m, 0), mg_level(m, 1),
-                                                        m.coarse.omega, m.coarse.omega2, it, chk,
-                                                        ck, m.tflag, xr, span, MgCoarseArgs{},
-                                                        nullptr, MgSweeps{});
+        k_mg_down<true, NT, true, false, S><<<blocks, NT, 0, s>>>(
+            mg_level(m, 0), mg_level(m, 1), m.coarse.omega, m.coarse.omega2, it, chk, ck, m.tflag,
+            xr, span, MgCoarseArgs{}, nullptr, sw0);
       else
-        k_mg_down<true, NT><<<blocks, NT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega,
-                                                  m.coarse.omega2, it, chk, ck, m.tflag, xr, span,
-                                                  MgCoarseArgs{}, nullptr, MgSweeps{});
+        k_mg_down<true, NT, false, false, S><<<blocks, NT, 0, s>>>(
+            mg_level(m, 0), mg_level(m, 1), m.coarse.omega, m.coarse.omega2, it, chk, ck, m.tflag,
+            xr, span, MgCoarseArgs{}, nullptr, sw0);
+    };
+    auto go_s = [&](auto nt) {
+      if (m.level0_sweeps == 3)
+        go(nt, std::integral_constant<int, 3>{});
+      else
+        go(nt, std::integral_constant<int, 2>{});
     };
     if (m.l0_threads == 1024)
-      go(std::integral_constant<int, 1024>{});
+      go_s(std::integral_constant<int, 1024>{});
     else if (m.l0_threads == 512)
-      go(std::integral_constant<int, 512>{});
+      go_s(std::integral_constant<int, 512>{});
     else
-      go(std::integral_constant<int, kT>{});
+      go_s(std::integral_constant<int, kT>{});
   } else {
     // the last tile level also runs the one-workgroup stage (mg_tail_coarse): its own
     // instance, so that the other levels (and every level of a batch) carry none of it;
@@ -2372,9 +2380,14 @@ void launch_mg_up(hipStream_t s, const MgHost& m, int l, int it, CgScalars* sc) 
   const int blocks = mg_up_blocks(m.nx[l], m.ny[l]);
   auto go0 = [&](auto nt) {
     constexpr int NT = decltype(nt)::value;
-    k_mg_up<true, NT><<<blocks, NT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega,
-                                            m.coarse.omega2, sc, m.part_rz, it, m.tflag,
-                                            MgSweeps{});
+    if (m.level0_sweeps == 3)
+      k_mg_up<true, NT, 3><<<blocks, NT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega,
+                                                 m.coarse.omega2, sc, m.part_rz, it, m.tflag,
+                                                 mg_sweeps0(m));
+    else
+      k_mg_up<true, NT><<<blocks, NT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega,
+                                              m.coarse.omega2, sc, m.part_rz, it, m.tflag,
+                                              MgSweeps{});
   };
   if (l == 0 && m.l0_threads == 1024)
     go0(std::integral_constant<int, 1024>{});

@@ -87,6 +87,7 @@ class Params(C.Structure):
         ("mg_coarse_sweeps", C.c_int32), ("graph_iters", C.c_int32),
         ("mg_fuse_update", C.c_int32), ("splat_order", C.c_int32),
         ("mg_tile_threads", C.c_int32), ("mg_level_sweeps", C.c_int32),
+        ("mg_level0_sweeps", C.c_int32),
     ]
 
 
@@ -97,7 +98,7 @@ class Tune(C.Structure):
         ("splat_order", C.c_int32), ("candidates", C.c_int32), ("cg_iterations", C.c_int32),
         ("mg_tile_threads", C.c_int32),
         ("ms_per_step", C.c_double), ("ms_per_step_before", C.c_double),
-        ("mg_level_sweeps", C.c_int32), ("reserved32", C.c_int32),
+        ("mg_level_sweeps", C.c_int32), ("mg_level0_sweeps", C.c_int32),
         ("reserved", C.c_int64 * 2),
     ]
 

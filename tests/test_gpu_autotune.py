@@ -11,7 +11,7 @@ from fluid_simulation_amd import fsg, scenes
 
 pytestmark = pytest.mark.gpu
 
-TUNED = ("mg_level_sweeps", "mg_coarse_sweeps", "mg_fuse_update", "graph_iters", "splat_order",
+TUNED = ("mg_level_sweeps", "mg_level0_sweeps", "mg_coarse_sweeps", "mg_fuse_update", "graph_iters", "splat_order",
          "mg_tile_threads")
 
 
