@@ -208,6 +208,10 @@ struct fsg_domain {
   int* n_fluid = nullptr;
   int* fluid_index = nullptr;
   int* cell_of = nullptr;
+  // true: the last pressure solve compacted the system (fluid_index, and b per unknown
+  // index); false (multigrid form): b holds the reference's b per cell, and the fluid
+  // indices follow from the labels
+  bool compacted = false;
   float* b = nullptr;
   uint8_t* info = nullptr;
   float* r = nullptr;
@@ -460,6 +464,7 @@ int mg_prepare(fsg_domain* h) {
   ok &= dalloc(h, &m.p[1], C) == hipSuccess;
   ok &= dalloc(h, &m.part_sum, kMgMaxVecBlocks) == hipSuccess;
   ok &= dalloc(h, &m.part_cnt, kMgMaxVecBlocks) == hipSuccess;
+  ok &= dalloc(h, &m.part_liq, kMgMaxVecBlocks) == hipSuccess;
   ok &= dalloc(h, &m.part_bb, kMgMaxVecBlocks) == hipSuccess;
   ok &= dalloc(h, &m.part_rr, (size_t)mg_part_rr_size(m)) == hipSuccess;
   // k_mg_dir: one partial per vector block, or per level-0 tile where tiles are skipped
@@ -559,7 +564,8 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
     if (h->batch) bs = h->batch->scale();
     launch_mg_setup(h->stream, h->g, m, h->label, UF(h), VF(h), h->prm.project_mean,
                     h->prm.warm_start != 0 && !h->batch && !adj_rhs, h->sc,
-                    h->batch ? &bs : nullptr, adj_rhs);
+                    h->batch ? &bs : nullptr, adj_rhs, adj_rhs ? nullptr : h->b,
+                    adj_rhs ? nullptr : h->n_fluid);
     if (h->batch) {
       launch_mg_scene_setup(h->stream, scene_coarse_args(h));
       launch_mgb_init(h->stream, h->batch->cg_args(h->g.nx, h->g.ny), h->batch->d_norm, m);
@@ -673,13 +679,6 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
 }
 
 int solve_pressure(fsg_domain* h) {
-  const int nb = (int)((ncell(h) + kScanChunk - 1) / kScanChunk);
-  timed(h, "count_liquid", [&] { launch_count_liquid(h->stream, h->g, h->label, h->counts); });
-  timed(h, "scan_counts", [&] { launch_scan_counts(h->stream, nb, h->counts, h->offsets, h->n_fluid); });
-  timed(h, "compact", [&] {
-    launch_compact(h->stream, h->g, h->label, h->offsets, h->fluid_index, h->cell_of, UF(h), VF(h),
-                   h->b, h->info);
-  });
   h->mg_solved = false;
   h->host_stats.cg_iterations = 0;
   h->host_stats.cg_error = 0.0;
@@ -687,8 +686,17 @@ int solve_pressure(fsg_domain* h) {
   const int solver = h->prm.solver;
   // the multigrid form's launches do not depend on the number of unknowns (a system
   // without any stops at its first test, and the pressure update reads the count on
-  // the device); it reads the count back with its stop test
+  // the device); it reads the count back with its stop test.  Its set-up forms the
+  // LIQUID count and the reference's b per cell itself: no compaction.
+  h->compacted = solver != FSG_SOLVER_MG;
   if (solver == FSG_SOLVER_MG) return mg_solve(h, -1);
+  const int nb = (int)((ncell(h) + kScanChunk - 1) / kScanChunk);
+  timed(h, "count_liquid", [&] { launch_count_liquid(h->stream, h->g, h->label, h->counts); });
+  timed(h, "scan_counts", [&] { launch_scan_counts(h->stream, nb, h->counts, h->offsets, h->n_fluid); });
+  timed(h, "compact", [&] {
+    launch_compact(h->stream, h->g, h->label, h->offsets, h->fluid_index, h->cell_of, UF(h), VF(h),
+                   h->b, h->info);
+  });
   // the launch geometry of the other forms depends on the number of unknowns
   int F = 0;
   HIPCHK(h, hipMemcpyAsync(&F, h->n_fluid, sizeof(int), hipMemcpyDeviceToHost, h->stream));
@@ -1444,13 +1452,31 @@ int fsg_run_phase(fsg_domain* h, int32_t phase, float dt) {
   return FSG_OK;
 }
 
+// The fluid index of every cell (src/FluidSolver.cpp:331-346): the compaction's, or,
+// where the last solve did not compact (the multigrid form), the row-major running
+// count of the LIQUID cells of the labels, which is what the compaction forms.
+static int host_fluid_indices(fsg_domain* h, int32_t* idx) {
+  const size_t C = ncell(h);
+  if (h->compacted) {
+    HIPCHK(h, hipMemcpyAsync(idx, h->fluid_index, C * sizeof(int), hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return FSG_OK;
+  }
+  std::vector<uint8_t> lab(C);
+  HIPCHK(h, hipMemcpyAsync(lab.data(), h->label, C, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  int32_t k = 0;
+  for (size_t c = 0; c < C; ++c) idx[c] = lab[c] == kLiquid ? k++ : -1;
+  return FSG_OK;
+}
+
 int fsg_get_pressure(fsg_domain* h, float* p) {
   if (!h || check_grid_ptr(h, p)) return FSG_E_INVALID;
   const size_t C = ncell(h);
   std::vector<int> idx(C);
+  if (int rc = host_fluid_indices(h, idx.data())) return rc;
   HIPCHK(h, hipMemcpyAsync(p, h->xg, C * sizeof(float), hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(h, hipMemcpyAsync(idx.data(), h->fluid_index, C * sizeof(int), hipMemcpyDeviceToHost,
-                           h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   for (size_t c = 0; c < C; ++c)
     if (idx[c] < 0) p[c] = 0.f;
@@ -1460,6 +1486,11 @@ int fsg_get_pressure(fsg_domain* h, float* p) {
 int fsg_get_rhs(fsg_domain* h, float* bout) {
   if (!h || check_grid_ptr(h, bout)) return FSG_E_INVALID;
   const size_t C = ncell(h);
+  if (!h->compacted) {  // b per cell, 0 off the LIQUID cells
+    HIPCHK(h, hipMemcpyAsync(bout, h->b, C * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return FSG_OK;
+  }
   std::vector<int> idx(C);
   std::vector<float> bc(C);
   HIPCHK(h, hipMemcpyAsync(idx.data(), h->fluid_index, C * sizeof(int), hipMemcpyDeviceToHost,
@@ -1472,10 +1503,7 @@ int fsg_get_rhs(fsg_domain* h, float* bout) {
 
 int fsg_get_fluid_indices(fsg_domain* h, int32_t* idx) {
   if (!h || check_grid_ptr(h, idx)) return FSG_E_INVALID;
-  HIPCHK(h, hipMemcpyAsync(idx, h->fluid_index, ncell(h) * sizeof(int), hipMemcpyDeviceToHost,
-                           h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  return FSG_OK;
+  return host_fluid_indices(h, idx);
 }
 
 int fsg_get_valid_masks(fsg_domain* h, uint8_t* mu, uint8_t* mv) {

@@ -290,6 +290,7 @@ struct MgHost {
   float* p[2] = {nullptr, nullptr};   // search direction (ping-pong)
   double *part_sum = nullptr, *part_cnt = nullptr, *part_bb = nullptr, *part_rr = nullptr,
          *part_pq = nullptr, *part_rz = nullptr;
+  double* part_liq = nullptr;  // LIQUID cells per setup block (the solve without compaction)
   MgCoarseArgs coarse{};
   size_t coarse_lds = 0;
 };
@@ -312,11 +313,15 @@ int mg_up_blocks(int nx, int ny);
 int mg_dir_blocks(const MgHost& m);  // blocks of k_mg_dir (partials of p.Kp)
 hipError_t mg_set_lds_limit(size_t bytes);
 // adj_rhs != null: the right-hand side is the adjoint solve's cell field (the
-// pressure's adjoint) instead of the divergence of (uf, vf); see k_mg_adj_rhs0
+// pressure's adjoint) instead of the divergence of (uf, vf); see k_mg_adj_rhs0.
+// rhs, n_fluid (optional, divergence right-hand side only): the reference's FP32 b at
+// every LIQUID cell (0 elsewhere) and the number of LIQUID cells, the outputs the
+// compaction of the other solver forms gives, so the multigrid form skips it.
 void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_t* label,
                      const float* uf, const float* vf, int project_mean, int warm,
                      CgScalars* sc, const MgBatchScale* batch = nullptr,
-                     const double* adj_rhs = nullptr);
+                     const double* adj_rhs = nullptr, float* rhs = nullptr,
+                     int* n_fluid = nullptr);
 // a batch's per-scene CG: init after the setup; the stop test at the start of each
 // iteration; the vector kernels (which: 0 r.z, 1 direction, 2 x / r update)
 void launch_mgb_init(hipStream_t s, const MgbArgs& a, const double* norm, const MgHost& m);

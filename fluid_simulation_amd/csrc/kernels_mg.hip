@@ -194,36 +194,50 @@ __device__ __forceinline__ void write_part(double val, double* __restrict__ part
 // ---------------------------------------------------------------- setup
 // Level-0 info, b' = -dx^2 * div (the divergence exactly as the compact path forms
 // it), and per-block partials (sum b', number of unknowns).
+// rhs (optional): the reference's b (src/FluidSolver.cpp:413, unscaled FP32) at every
+// LIQUID cell, 0 elsewhere, and part_liq the block's count of LIQUID cells (the
+// reference's number of fluid indices, :331-346): what the compaction of the other
+// solver forms would give, so the multigrid form needs no compaction.
 __global__ void __launch_bounds__(kT) k_mg_setup0(Geom g, const uint8_t* __restrict__ label,
                                                    const float* __restrict__ uf,
                                                    const float* __restrict__ vf,
                                                    uint8_t* __restrict__ inf,
                                                    double* __restrict__ b,
                                                    double* __restrict__ part_sum,
-                                                   double* __restrict__ part_cnt) {
+                                                   double* __restrict__ part_cnt,
+                                                   float* __restrict__ rhs,
+                                                   double* __restrict__ part_liq) {
   __shared__ double sm[4];
   const int nx = g.nx, ny = g.ny, C = nx * ny;
   const double scale = -(double)g.dx * (double)g.dx;
-  double s = 0.0, cnt = 0.0;
+  double s = 0.0, cnt = 0.0, liq = 0.0;
   CellWalk w(blockIdx.x * kT + threadIdx.x, gridDim.x * kT, nx);
   for (int c = blockIdx.x * kT + threadIdx.x; c < C; c += gridDim.x * kT, w.next()) {
     const int i = w.i, j = w.j;
     const uint8_t f = make_info(label, nx, ny, i, j);
     inf[c] = f;
+    const bool fluid = label[c] == kLiquid;
     double bv = 0.0;
-    if (is_act(f)) {
+    float bf = 0.f;
+    if (fluid) {
       const float uxp = i < nx - 1 ? uf[c + 1] : 0.f;
       const float vyp = j < ny - 1 ? vf[c + nx] : 0.f;
       const float dvx = div_x(uxp - uf[c], g);
       const float dvy = div_y(vyp - vf[c], g);
-      bv = scale * (double)(dvx + dvy);
+      bf = dvx + dvy;
+      liq += 1.0;
+    }
+    if (is_act(f)) {
+      bv = scale * (double)bf;
       s += bv;
       cnt += 1.0;
     }
     b[c] = bv;
+    if (rhs) rhs[c] = bf;
   }
   write_part<kT>(s, part_sum, sm);
   write_part<kT>(cnt, part_cnt, sm);
+  if (part_liq) write_part<kT>(liq, part_liq, sm);
 }
 
 // AIR neighbours of cell (i, j) (Dirichlet links of its row: an unknown with one
@@ -347,9 +361,16 @@ __global__ void __launch_bounds__(kT) k_mg_setup1(int nx, int ny, int project_me
                                                    const double* __restrict__ part_sum,
                                                    const double* __restrict__ part_cnt, int n_part,
                                                    double* __restrict__ part_bb,
-                                                   double* __restrict__ part_rr, CgScalars* sc) {
+                                                   double* __restrict__ part_rr, CgScalars* sc,
+                                                   const double* __restrict__ part_liq,
+                                                   int* __restrict__ n_fluid) {
   __shared__ double sm[4];
   const int C = nx * ny;
+  // the number of LIQUID cells (k_mg_setup0's partials), when the compaction did not run
+  if (part_liq && blockIdx.x == 0) {
+    const double n = sum_part<kT>(part_liq, n_part, sm);
+    if (threadIdx.x == 0) *n_fluid = (int)n;
+  }
   double mean = 0.0;
   if (project_mean) {
     const double s = sum_part<kT>(part_sum, n_part, sm);
@@ -2182,7 +2203,8 @@ void launch_mg_scene_coarse(hipStream_t s, const SceneCoarseArgs& a, CgScalars* 
 
 void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_t* label,
                      const float* uf, const float* vf, int project_mean, int warm,
-                     CgScalars* sc, const MgBatchScale* batch, const double* adj_rhs) {
+                     CgScalars* sc, const MgBatchScale* batch, const double* adj_rhs,
+                     float* rhs, int* n_fluid) {
   const int C = g.nx * g.ny, nb = vec_blocks(C);
   if (adj_rhs) {
     // part_bb / part_rr are free until k_mg_setup1 writes them
@@ -2192,7 +2214,8 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
                                     adj_rhs, m.part_bb, m.part_rr, m.part_pq, nb, m.b, m.part_sum,
                                     m.part_cnt);
   } else {
-    k_mg_setup0<<<nb, kT, 0, s>>>(g, label, uf, vf, m.inf[0], m.b, m.part_sum, m.part_cnt);
+    k_mg_setup0<<<nb, kT, 0, s>>>(g, label, uf, vf, m.inf[0], m.b, m.part_sum, m.part_cnt, rhs,
+                                  rhs ? m.part_liq : nullptr);
   }
   if (batch) {
     k_mg_scene_norm<<<batch->n_scenes, 1024, 0, s>>>(m.b, g.nx, batch->rect, batch->norm,
@@ -2208,7 +2231,7 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
   k_mg_setup1<<<nb, kT, 0, s>>>(g.nx, g.ny, adj_rhs ? 0 : project_mean, m.inf[0], m.b,
                                 warm ? m.x[1] : nullptr,
                                 m.x[0], m.rl[0], m.part_sum, m.part_cnt, nb, m.part_bb,
-                                m.part_rr, sc);
+                                m.part_rr, sc, rhs && !adj_rhs ? m.part_liq : nullptr, n_fluid);
   // levels coarsened one launch each, down to the first level l0 whose finer level
   // (>= 1: level 0's info comes from k_mg_setup0) fits the one-workgroup tail
   int l0 = 2;
