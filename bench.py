@@ -134,20 +134,20 @@ def solve_bytes(name, nx, ny, solves):
 # HBM traffic per launch on the 1024^2 dam break (bytes), from two rocprofv3 --pmc
 # passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, averaged over the run's
 # dispatches) of
-#   bench.py --graph 0 --steps 5 --warmup 2 --start 60 --run-steps 0 --plain-steps 0
+#   bench.py --graph 0 --steps 5 --warmup 2 --start 250 --run-steps 0 --plain-steps 0
 #            --refined-steps 0 --profile-steps 1 --no-fp64-check --no-cpu
 #            --batch-scenes 0 --grid-steps 0 --adj-steps 0 --autotune 0
 # scaled by a 1 GiB device copy (bench.py --copy-probe 1024) read by the same counters:
 # FETCH_SIZE reads 0.5 GiB for its 1 GiB read (factor 2.0), WRITE_SIZE 1 GiB (1.0).
 # traffic = (2.0 * FETCH_SIZE + 1.0 * WRITE_SIZE) * 1024.
 MEASURED_TRAFFIC_1024 = {
-    "cg_mg_down0": (2.0 * 2921.9 + 6021.1) * 1024,
-    "cg_mg_up0": (2.0 * 1383.9 + 1264.1) * 1024,
-    "cg_mg_dir": (2.0 * 1152.2 + 1568.8) * 1024,
-    "cg_mg_down4": (2.0 * 57.3 + 21.6) * 1024,
-    "particle_scatter": (2.0 * 8055.8 + 50440.7) * 1024,
-    "particle_gather": (2.0 * 10087.6 + 16676.3) * 1024,
-    "face_update": (2.0 * 12817.1 + 28672.3) * 1024,
+    "cg_mg_down0": (2.0 * 2966.4 + 6284.4) * 1024,
+    "cg_mg_up0": (2.0 * 1399.2 + 1301.4) * 1024,
+    "cg_mg_dir": (2.0 * 1133.1 + 1646.4) * 1024,
+    "cg_mg_down4": (2.0 * 61.4 + 20.9) * 1024,
+    "particle_scatter": (2.0 * 8064.1 + 44602.8) * 1024,
+    "particle_gather": (2.0 * 10185.7 + 16676.3) * 1024,
+    "face_update": (2.0 * 12819.0 + 28672.4) * 1024,
 }
 
 
