@@ -462,9 +462,11 @@ int mg_prepare(fsg_domain* h) {
   ok &= dalloc(h, &m.x[1], C) == hipSuccess;
   ok &= dalloc(h, &m.p[0], C) == hipSuccess;
   ok &= dalloc(h, &m.p[1], C) == hipSuccess;
-  ok &= dalloc(h, &m.part_sum, kMgMaxVecBlocks) == hipSuccess;
-  ok &= dalloc(h, &m.part_cnt, kMgMaxVecBlocks) == hipSuccess;
-  ok &= dalloc(h, &m.part_liq, kMgMaxVecBlocks) == hipSuccess;
+  // the set-up's partials: one per vector block, or per level-0 tile (k_mg_setup0t)
+  const size_t n_setup = (size_t)std::max(kMgMaxVecBlocks, mg_up_blocks(m.nx[0], m.ny[0]));
+  ok &= dalloc(h, &m.part_sum, n_setup) == hipSuccess;
+  ok &= dalloc(h, &m.part_cnt, n_setup) == hipSuccess;
+  ok &= dalloc(h, &m.part_liq, n_setup) == hipSuccess;
   ok &= dalloc(h, &m.part_bb, kMgMaxVecBlocks) == hipSuccess;
   ok &= dalloc(h, &m.part_rr, (size_t)mg_part_rr_size(m)) == hipSuccess;
   // k_mg_dir: one partial per vector block, or per level-0 tile where tiles are skipped

@@ -363,9 +363,20 @@ __global__ void __launch_bounds__(kT) k_mg_setup1(int nx, int ny, int project_me
                                                    double* __restrict__ part_bb,
                                                    double* __restrict__ part_rr, CgScalars* sc,
                                                    const double* __restrict__ part_liq,
-                                                   int* __restrict__ n_fluid) {
+                                                   int* __restrict__ n_fluid,
+                                                   const uint8_t* __restrict__ tflag,
+                                                   int n_tiles) {
   __shared__ double sm[4];
   const int C = nx * ny;
+  // the live level-0 tiles, where k_mg_setup0t set their flags (else k_mg_tile_flags
+  // counts them after this kernel)
+  if (tflag && blockIdx.x == 0) {
+    double n = 0.0;
+    for (int t = threadIdx.x; t < n_tiles; t += kT) n += tflag[t] ? 1.0 : 0.0;
+    double v[1] = {n};
+    block_sum<1, kT>(v, sm);
+    if (threadIdx.x == 0) sc->live_tiles = (int)v[0];
+  }
   // the number of LIQUID cells (k_mg_setup0's partials), when the compaction did not run
   if (part_liq && blockIdx.x == 0) {
     const double n = sum_part<kT>(part_liq, n_part, sm);
@@ -412,7 +423,7 @@ __global__ void __launch_bounds__(kT) k_mg_setup1(int nx, int ny, int project_me
     sc->spmv = 0;
     sc->err = 0.0;
     sc->rho[0] = sc->rho[1] = 0.0;
-    sc->live_tiles = 0;
+    if (!tflag) sc->live_tiles = 0;
   }
 }
 
@@ -496,7 +507,7 @@ struct MgTailArgs {
   uint8_t* inf[kMgMaxLevels];
 };
 
-__global__ void __launch_bounds__(1024) k_mg_coarsen_tail(MgTailArgs a) {
+__device__ __forceinline__ void coarsen_tail(const MgTailArgs& a) {
   __shared__ uint8_t s_lab[2][kTailCells];
   const int lf = a.l0 - 1;
   {
@@ -532,6 +543,126 @@ __global__ void __launch_bounds__(1024) k_mg_coarsen_tail(MgTailArgs a) {
     }
     __syncthreads();
     cur ^= 1;
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_mg_coarsen_tail(MgTailArgs a) { coarsen_tail(a); }
+
+// One domain, the divergence right-hand side: level 0's set-up by 32 x 32 tiles
+// (k_mg_setup0's outputs, with one partial per tile), the tile's flag (a LIQUID cell in
+// it: every unknown of level 0 or 1 lies under one, so the level-0 tile kernels skip the
+// others, as k_mg_tile_flags' exact flags let them), and the labels of the levels
+// 1 .. nl whose cells lie inside one tile (2^nl <= 32), coarsened in LDS by the rule of
+// k_mg_coarsen (children outside the grid take no part).
+struct MgTileLabels {
+  int nl;
+  int nx[6], ny[6];
+  uint8_t* lab[6];
+};
+__global__ void __launch_bounds__(kT) k_mg_setup0t(Geom g, const uint8_t* __restrict__ label,
+                                                    const float* __restrict__ uf,
+                                                    const float* __restrict__ vf,
+                                                    uint8_t* __restrict__ inf,
+                                                    double* __restrict__ b,
+                                                    double* __restrict__ part_sum,
+                                                    double* __restrict__ part_cnt,
+                                                    float* __restrict__ rhs,
+                                                    double* __restrict__ part_liq,
+                                                    uint8_t* __restrict__ tflag, MgTileLabels L) {
+  constexpr uint8_t kNone = 0xFF;  // a cell outside the grid
+  __shared__ double sm[3 * (kT / 64)];
+  __shared__ uint8_t s_l[2][32 * 32];
+  const int nx = g.nx, ny = g.ny;
+  const int tiles_x = (nx + 31) >> 5;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int i0 = (tile % tiles_x) * 32, j0 = (tile / tiles_x) * 32;
+  const double scale = -(double)g.dx * (double)g.dx;
+  double acc[3] = {0.0, 0.0, 0.0};  // sum b', unknowns, LIQUID cells
+  int any = 0;
+#pragma unroll
+  for (int k = 0; k < 32 * 32 / kT; ++k) {
+    const int t = threadIdx.x + k * kT;
+    const int i = i0 + (t & 31), j = j0 + (t >> 5);
+    uint8_t lab = kNone;
+    if (i < nx && j < ny) {
+      const int c = i + j * nx;
+      const uint8_t f = make_info(label, nx, ny, i, j);
+      inf[c] = f;
+      lab = label[c];
+      const bool fluid = lab == kLiquid;
+      double bv = 0.0;
+      float bf = 0.f;
+      if (fluid) {
+        const float uxp = i < nx - 1 ? uf[c + 1] : 0.f;
+        const float vyp = j < ny - 1 ? vf[c + nx] : 0.f;
+        const float dvx = div_x(uxp - uf[c], g);
+        const float dvy = div_y(vyp - vf[c], g);
+        bf = dvx + dvy;
+        acc[2] += 1.0;
+        any = 1;
+      }
+      if (is_act(f)) {
+        bv = scale * (double)bf;
+        acc[0] += bv;
+        acc[1] += 1.0;
+      }
+      b[c] = bv;
+      rhs[c] = bf;
+    }
+    s_l[0][t] = lab;
+  }
+  // the partials in k_mg_setup0's order of magnitude (one per tile, fixed order)
+  block_sum<3, kT>(acc, sm);
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0) {
+    part_sum[blockIdx.x] = acc[0];
+    part_cnt[blockIdx.x] = acc[1];
+    part_liq[blockIdx.x] = acc[2];
+    tflag[tile] = any ? 1 : 0;
+  }
+  for (int l = 1; l <= L.nl; ++l) {
+    const int w = 32 >> l, wf = 2 * w;
+    const uint8_t* src = s_l[(l - 1) & 1];
+    uint8_t* dst = s_l[l & 1];
+    for (int t = threadIdx.x; t < w * w; t += kT) {
+      const int I = t % w, J = t / w;
+      bool air = false, liq = false;
+#pragma unroll
+      for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+        for (int aa = 0; aa < 2; ++aa) {
+          const uint8_t v = src[(2 * I + aa) + (2 * J + bb) * wf];
+          air |= v == kAir;
+          liq |= v == kLiquid;
+        }
+      const int Ig = (i0 >> l) + I, Jg = (j0 >> l) + J;
+      const bool in = Ig < L.nx[l] && Jg < L.ny[l];
+      const uint8_t v = in ? (air ? kAir : (liq ? kLiquid : kSolid)) : kNone;
+      dst[t] = v;
+      if (in) L.lab[l][Ig + Jg * L.nx[l]] = v;
+    }
+    __syncthreads();
+  }
+}
+
+// The info bytes of levels 1 .. nli (their labels complete), by every block but the
+// last, which runs the coarsening tail (levels l0-1 and below; l0 - 1 > nli).
+struct MgInfoArgs {
+  int nli;
+  int nx[kMgMaxLevels], ny[kMgMaxLevels];
+  const uint8_t* lab[kMgMaxLevels];
+  uint8_t* inf[kMgMaxLevels];
+};
+__global__ void __launch_bounds__(1024) k_mg_level_info(MgInfoArgs a, MgTailArgs tail) {
+  if (blockIdx.x == gridDim.x - 1) {
+    coarsen_tail(tail);
+    return;
+  }
+  const int stride = (gridDim.x - 1) * 1024;
+  for (int l = 1; l <= a.nli; ++l) {
+    const int nx = a.nx[l], ny = a.ny[l], n = nx * ny;
+    for (int t = blockIdx.x * 1024 + threadIdx.x; t < n; t += stride)
+      a.inf[l][t] = make_info(a.lab[l], nx, ny, t % nx, t / nx);
   }
 }
 
@@ -2206,6 +2337,18 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
                      CgScalars* sc, const MgBatchScale* batch, const double* adj_rhs,
                      float* rhs, int* n_fluid) {
   const int C = g.nx * g.ny, nb = vec_blocks(C);
+  // levels coarsened one launch each, down to the first level l0 whose finer level
+  // (>= 1: level 0's info comes from k_mg_setup0) fits the one-workgroup tail
+  int l0 = 2;
+  while (l0 < m.nlev && m.nx[l0 - 1] * m.ny[l0 - 1] > kTailCells) ++l0;
+  if (m.nlev < 2) l0 = m.nlev;
+  // one domain's forward solve: level 0's set-up by tiles, which also forms the tile
+  // flags and the labels of levels 1 .. lk (k_mg_setup0t), then the info of the levels
+  // and the coarsening tail in one launch (k_mg_level_info)
+  const bool tiled = !adj_rhs && !batch && rhs && m.tflag && m.nlev >= 3 && l0 < m.nlev;
+  const int n_tiles = mg_up_blocks(g.nx, g.ny);
+  const int lk = tiled ? std::min(std::min(5, l0 - 1), m.nlev - 1) : 0;
+  int n_part = nb;
   if (adj_rhs) {
     // part_bb / part_rr are free until k_mg_setup1 writes them
     k_mg_adj_rhs0<<<nb, kT, 0, s>>>(g, label, adj_rhs, m.inf[0], m.part_bb, m.part_rr,
@@ -2213,6 +2356,17 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
     k_mg_adj_rhs1<<<nb, kT, 0, s>>>(C, -(double)g.dx * (double)g.dx, project_mean, m.inf[0],
                                     adj_rhs, m.part_bb, m.part_rr, m.part_pq, nb, m.b, m.part_sum,
                                     m.part_cnt);
+  } else if (tiled) {
+    MgTileLabels L{};
+    L.nl = lk;
+    for (int l = 1; l <= lk; ++l) {
+      L.nx[l] = m.nx[l];
+      L.ny[l] = m.ny[l];
+      L.lab[l] = m.lab[l];
+    }
+    k_mg_setup0t<<<n_tiles, kT, 0, s>>>(g, label, uf, vf, m.inf[0], m.b, m.part_sum, m.part_cnt,
+                                         rhs, m.part_liq, m.tflag, L);
+    n_part = n_tiles;
   } else {
     k_mg_setup0<<<nb, kT, 0, s>>>(g, label, uf, vf, m.inf[0], m.b, m.part_sum, m.part_cnt, rhs,
                                   rhs ? m.part_liq : nullptr);
@@ -2230,14 +2384,11 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
   // solve)
   k_mg_setup1<<<nb, kT, 0, s>>>(g.nx, g.ny, adj_rhs ? 0 : project_mean, m.inf[0], m.b,
                                 warm ? m.x[1] : nullptr,
-                                m.x[0], m.rl[0], m.part_sum, m.part_cnt, nb, m.part_bb,
-                                m.part_rr, sc, rhs && !adj_rhs ? m.part_liq : nullptr, n_fluid);
-  // levels coarsened one launch each, down to the first level l0 whose finer level
-  // (>= 1: level 0's info comes from k_mg_setup0) fits the one-workgroup tail
-  int l0 = 2;
-  while (l0 < m.nlev && m.nx[l0 - 1] * m.ny[l0 - 1] > kTailCells) ++l0;
-  if (m.nlev < 2) l0 = m.nlev;
-  for (int l = 1; l < std::min(l0, m.nlev); ++l) {
+                                m.x[0], m.rl[0], m.part_sum, m.part_cnt, n_part, m.part_bb,
+                                m.part_rr, sc, rhs && !adj_rhs ? m.part_liq : nullptr, n_fluid,
+                                tiled ? m.tflag : nullptr, n_tiles);
+  // the levels not formed by k_mg_setup0t, one launch each, down to level l0 - 1
+  for (int l = std::max(1, lk + 1); l < std::min(l0, m.nlev); ++l) {
     const int nf = m.nx[l - 1] * m.ny[l - 1], ncl = m.nx[l] * m.ny[l];
     const int n = nf > ncl ? nf : ncl;
     int blocks = (n + kT - 1) / kT;
@@ -2246,7 +2397,31 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
                                        l - 1 >= 1 ? m.inf[l - 1] : nullptr, m.nx[l], m.ny[l],
                                        m.lab[l]);
   }
-  if (m.nlev > 1 && l0 < m.nlev) {
+  if (tiled) {
+    // the info of levels 1 .. lk - 1 (lk .. l0 - 2 came with the launches above) and the
+    // tail from l0 in one launch
+    MgInfoArgs ia{};
+    ia.nli = lk - 1;
+    int cells = 0;
+    for (int l = 1; l <= ia.nli; ++l) cells += m.nx[l] * m.ny[l];
+    for (int l = 0; l < m.nlev; ++l) {
+      ia.nx[l] = m.nx[l];
+      ia.ny[l] = m.ny[l];
+      ia.lab[l] = m.lab[l];
+      ia.inf[l] = m.inf[l];
+    }
+    MgTailArgs a{};
+    a.l0 = l0;
+    a.nlev = m.nlev;
+    for (int l = 0; l < m.nlev; ++l) {
+      a.nx[l] = m.nx[l];
+      a.ny[l] = m.ny[l];
+      a.lab[l] = m.lab[l];
+      a.inf[l] = m.inf[l];
+    }
+    const int blocks = std::min(std::max((cells + 1023) / 1024, 1), 1024) + 1;
+    k_mg_level_info<<<blocks, 1024, 0, s>>>(ia, a);
+  } else if (m.nlev > 1 && l0 < m.nlev) {
     MgTailArgs a{};
     a.l0 = l0;
     a.nlev = m.nlev;
@@ -2263,7 +2438,7 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
     k_mg_coarsen<<<(n + kT - 1) / kT, kT, 0, s>>>(m.nx[l], m.ny[l], m.lab[l], m.inf[l], 0, 0,
                                                   nullptr);
   }
-  for (int l = 0; l < m.lc; ++l) {
+  for (int l = tiled ? 1 : 0; l < m.lc; ++l) {
     uint8_t* flag = l == 0 ? m.tflag : m.tflag_l[l];
     if (flag)
       k_mg_tile_flags<<<mg_up_blocks(m.nx[l], m.ny[l]), kT, 0, s>>>(
