@@ -224,6 +224,8 @@ struct fsg_domain {
   double* partials = nullptr;
   unsigned* bar = nullptr;
   CgScalars* sc = nullptr;
+  HostScalars* hsc = nullptr;  // pinned host memory: the solve loop's readback
+  bool mg_finish_pending = false;  // the next pressure update finishes the MG solve
   int max_coop_blocks = 0;  // co-resident 1024-thread blocks of the persistent solver
   size_t pcg_lds_max = 0;   // dynamic LDS the persistent solver may use per block
   int last_F = 0, last_grid = 0, last_kpt = 0;
@@ -329,13 +331,30 @@ void classify_and_splat(fsg_domain* h) {
   // deterministic form: the splat sums in fixed point (zero between steps) and the
   // conversion writes every accumulator, so they are not cleared here
   long long* fx = h->prm.deterministic ? h->acc_fx : nullptr;
-  timed(h, "cell_init", [&] {
-    launch_cell_init(h->stream, h->g, h->label, h->solid, h->acc, 1, fx ? 0 : 1);
-  });
   // the splat reads the particles grouped by bucket (their storage order is kept); a
   // batch groups each scene's particles within the scene's own range
   const float4* src = h->pt;
   const bool bucketed = h->prm.splat_order == 1;
+  bool single = bucketed && !h->batch && h->np > 0 && h->np < INT_MAX && h->prank &&
+                h->psorted && fsg::bucket_count(h->g) <= fsg::kMaxBuckets;
+  if (single && !h->bkt_cnt) {
+    const size_t nb = (size_t)fsg::bucket_count(h->g);
+    if (dalloc(h, &h->bkt_cnt, nb) != hipSuccess || dalloc(h, &h->bkt_off, nb) != hipSuccess)
+      h->bkt_cnt = h->bkt_off = nullptr;
+  }
+  single = single && h->bkt_cnt;
+  if (single) {
+    // one domain: the bucket count kernel also does the cell set-up
+    timed(h, "bucket_order", [&] {
+      launch_bucket_order(h->stream, h->g, h->pt, h->np, h->bkt_cnt, h->bkt_off, h->prank,
+                          h->psorted, fsg::CellInit{h->label, h->solid, fx ? nullptr : h->acc});
+    });
+    src = h->psorted;
+  } else {
+    timed(h, "cell_init", [&] {
+      launch_cell_init(h->stream, h->g, h->label, h->solid, h->acc, 1, fx ? 0 : 1);
+    });
+  }
   if (bucketed && h->batch && h->np > 0 && h->np < INT_MAX && h->prank && h->psorted &&
       h->batch->max_scene_buckets <= fsg::kMaxBuckets) {
     BatchInfo& b = *h->batch;
@@ -348,21 +367,6 @@ void classify_and_splat(fsg_domain* h) {
       timed(h, "bucket_order", [&] {
         launch_bucket_order_batch(h->stream, h->pt, h->np, h->bkt_cnt, h->bkt_off, h->prank,
                                   h->psorted, b.tables(), b.n_buckets, b.max_scene_buckets);
-      });
-      src = h->psorted;
-    }
-  }
-  if (bucketed && !h->batch && h->np > 0 && h->np < INT_MAX && h->prank && h->psorted &&
-      fsg::bucket_count(h->g) <= fsg::kMaxBuckets) {
-    if (!h->bkt_cnt) {
-      const size_t nb = (size_t)fsg::bucket_count(h->g);
-      if (dalloc(h, &h->bkt_cnt, nb) != hipSuccess || dalloc(h, &h->bkt_off, nb) != hipSuccess)
-        h->bkt_cnt = h->bkt_off = nullptr;
-    }
-    if (h->bkt_cnt) {
-      timed(h, "bucket_order", [&] {
-        launch_bucket_order(h->stream, h->g, h->pt, h->np, h->bkt_cnt, h->bkt_off, h->prank,
-                            h->psorted);
       });
       src = h->psorted;
     }
@@ -542,7 +546,10 @@ void alloc_clk_span(fsg_domain* h) {
 
 // F < 0: the number of unknowns is still on the device (h->n_fluid); it is read back
 // with the stop-test copy, so the solve costs no extra host round trip.
-int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
+// defer_finish: the caller's pressure update does the finish (MgFinishFuse) where it can
+// (one domain, no mean projection); h->mg_finish_pending says whether it must.
+int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr, bool defer_finish = false) {
+  h->mg_finish_pending = false;
   if (int rc = mg_prepare(h)) return rc;
   MgHost& m = h->mg;
   // smoothing weights: damped Jacobi (0.8, 0.8) or the two-step Chebyshev smoother on
@@ -654,22 +661,28 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr) {
     } else {
       for (; it < target; ++it) enqueue_iteration(it);
     }
-    HIPCHK(h, hipMemcpyAsync(&hs, h->sc, sizeof(hs), hipMemcpyDeviceToHost, h->stream));
-    if (F < 0) HIPCHK(h, hipMemcpyAsync(&F, h->n_fluid, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    if (!h->hsc)
+      HIPCHK(h, hipHostMalloc((void**)&h->hsc, sizeof(HostScalars), hipHostMallocCoherent));
+    launch_read_scalars(h->stream, h->sc, F < 0 ? h->n_fluid : nullptr, h->hsc);
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    memcpy(&hs, &h->hsc->sc, sizeof(hs));
+    if (F < 0) F = h->hsc->n_fluid;
     if (hs.done || it > max_iter) break;
     batch = more;
     more = std::min(2 * more, 32);
   }
   h->last_F = F;
   h->host_stats.n_fluid = F;
-  timed(h, "cg_mg_finish",
-        [&] {
-          fsg::MgBatchScale bs;
-          if (h->batch) bs = h->batch->scale();
-          launch_mg_finish(h->stream, m, h->prm.project_mean, h->sc, h->xg,
-                           h->batch ? &bs : nullptr);
-        });
+  if (defer_finish && !adj_rhs && !h->batch && !h->prm.project_mean)
+    h->mg_finish_pending = true;
+  else
+    timed(h, "cg_mg_finish",
+          [&] {
+            fsg::MgBatchScale bs;
+            if (h->batch) bs = h->batch->scale();
+            launch_mg_finish(h->stream, m, h->prm.project_mean, h->sc, h->xg,
+                             h->batch ? &bs : nullptr);
+          });
   last_iters = hs.iter;
   h->mg_solved = true;
   h->host_stats.cg_iterations = hs.iter;
@@ -691,7 +704,7 @@ int solve_pressure(fsg_domain* h) {
   // the device); it reads the count back with its stop test.  Its set-up forms the
   // LIQUID count and the reference's b per cell itself: no compaction.
   h->compacted = solver != FSG_SOLVER_MG;
-  if (solver == FSG_SOLVER_MG) return mg_solve(h, -1);
+  if (solver == FSG_SOLVER_MG) return mg_solve(h, -1, nullptr, true);
   const int nb = (int)((ncell(h) + kScanChunk - 1) / kScanChunk);
   timed(h, "count_liquid", [&] { launch_count_liquid(h->stream, h->g, h->label, h->counts); });
   timed(h, "scan_counts", [&] { launch_scan_counts(h->stream, nb, h->counts, h->offsets, h->n_fluid); });
@@ -814,6 +827,9 @@ int ph_pressure(fsg_domain* h, float dt, bool dirichlet, bool diff) {
           [&] { launch_count_particles(h->stream, h->g, h->pt, h->np, counts); });
   }
   const float dt_over_density = dt / h->density;
+  fsg::MgFinishFuse fin;
+  if (h->mg_finish_pending) fin = fsg::MgFinishFuse{h->mg.inf[0], h->mg.x[1], h->sc};
+  h->mg_finish_pending = false;
   timed(h, "pressure_update", [&] {
     launch_pressure_update(h->stream, h->g, h->n_fluid, h->label, h->xg,
                            h->mg_solved ? h->mg.x[0] : nullptr, counts, ppk, UF(h), VF(h), UB(h),
@@ -821,7 +837,7 @@ int ph_pressure(fsg_domain* h, float dt, bool dirichlet, bool diff) {
                            dt_over_density, dirichlet ? (h->prm.closed_walls ? 2 : 1) : 0,
                            diff ? 1 : 0, h->u_prev, h->v_prev,
                            h->u_diff, h->v_diff, h->adj.taping ? h->adj.zmask : nullptr,
-                           dirichlet && h->prm.record_decisions ? h->dec : nullptr);
+                           dirichlet && h->prm.record_decisions ? h->dec : nullptr, fin);
   });
   swap_vel(h);
   return FSG_OK;
@@ -1023,6 +1039,7 @@ int fsg_destroy(fsg_domain* h) {
   if (h->reject) (void)hipFree(h->reject);
   if (h->prank) (void)hipFree(h->prank);
   if (h->psorted) (void)hipFree(h->psorted);
+  if (h->hsc) (void)hipHostFree(h->hsc);
   if (h->adj.pbar) (void)hipFree(h->adj.pbar);
   if (h->adj.x0) (void)hipFree(h->adj.x0);
   for (Checkpoint& c : h->ckpt)

@@ -59,6 +59,13 @@ enum { FU_P2G = 1, FU_PREV = 2, FU_GRAVITY = 4, FU_DIRICHLET = 8, FU_EXTINIT = 1
 
 void launch_cell_init(hipStream_t s, const Geom& g, uint8_t* label, const uint8_t* solid,
                       float4* acc, int do_labels, int do_acc);
+// The cell set-up of a step: label (if set) := SOLID on the border and where solid is
+// set, else AIR; acc (if set) := 0.
+struct CellInit {
+  uint8_t* label = nullptr;
+  const uint8_t* solid = nullptr;
+  float4* acc = nullptr;
+};
 // acc_fx set: the splat accumulates in 64-bit fixed point there (4 per cell, zero on
 // entry; launch_fx_to_float turns the sums into acc and zeroes them again).
 void launch_particle_scatter(hipStream_t s, const Geom& g, const float4* pt, int64_t np,
@@ -67,11 +74,12 @@ void launch_particle_scatter(hipStream_t s, const Geom& g, const float4* pt, int
 void launch_fx_to_float(hipStream_t s, int64_t n, long long* fx, float* out);
 // The splat's bucket order for one domain (np < 2^31, bucket_count(g) <= kMaxBuckets):
 // cnt (zero on entry, zero again on return) and off hold bucket_count(g) ints, rank np
-// ints; sorted receives the np particles grouped by bucket.
+// ints; sorted receives the np particles grouped by bucket.  ci: the step's cell set-up
+// (launch_cell_init's work), done by the count kernel as well.
 constexpr int kMaxBuckets = 16384;  // the count kernel's LDS histogram: 64 KiB
 int bucket_count(const Geom& g);
 void launch_bucket_order(hipStream_t s, const Geom& g, const float4* pt, int64_t np, int* cnt,
-                         int* off, int* rank, float4* sorted);
+                         int* off, int* rank, float4* sorted, const CellInit& ci = CellInit{});
 // A packed batch: every scene's particles grouped by the 16 x 16-cell buckets of its own
 // grid (BatchScene::bucket_base / bucket_bx), within the scene's own particle range, so
 // the batch's particle tiles address the sorted copy unchanged.  n_buckets over all
@@ -89,12 +97,22 @@ void launch_extend_sweep(hipStream_t s, const Geom& g, const uint8_t* label, flo
                          uint8_t* myb, uint8_t* info_u, uint8_t* info_v, int layer);
 void launch_classify_levelset(hipStream_t s, const Geom& g, const float* phi,
                               const uint8_t* solid, uint8_t* label);
+struct CgScalars;
+// The multigrid solve's last step, done by the pressure update (single domain, no mean
+// projection): x = x[iter & 1] at the unknowns (inf0 & kMgUnknown), 0 elsewhere, into
+// x64 (the solve's x[0]) and xg, as the separate finish kernel would.
+struct MgFinishFuse {
+  const uint8_t* inf0 = nullptr;
+  const double* x1 = nullptr;
+  const CgScalars* sc = nullptr;
+};
 void launch_pressure_update(hipStream_t s, const Geom& g, const int* n_fluid,
-                            const uint8_t* label, float* xg, const double* x64,
+                            const uint8_t* label, float* xg, double* x64,
                             const int* counts, float ppk, float* au, float* av,
                             float* bu, float* bv, float dt_over_density, int do_dirichlet,
                             int do_diff, const float* prev_u, const float* prev_v, float* diff_u,
-                            float* diff_v, uint8_t* zmask = nullptr, uint8_t* dec = nullptr);
+                            float* diff_v, uint8_t* zmask = nullptr, uint8_t* dec = nullptr,
+                            const MgFinishFuse& fin = MgFinishFuse{});
 void launch_diff(hipStream_t s, const Geom& g, const float* u, const float* v, const float* pu,
                  const float* pv, float* du, float* dv);
 void launch_particle_gather(hipStream_t s, const Geom& g, float4* pt, int64_t np, const float* u,
@@ -125,6 +143,10 @@ void launch_advect_level_set(hipStream_t s, const Geom& g, const float* phi, flo
 // ---------------------------------------------------------------- pressure
 constexpr int kScanChunk = 2048;  // cells per block of the liquid count / compaction
 
+// Info bit of the multigrid form's level-0 cells: the cell is an unknown (a LIQUID cell
+// with a non-SOLID neighbour).
+constexpr uint8_t kMgUnknown = 0x80;
+
 // Device-resident scalars of one solve.
 struct CgScalars {
   double rhs_norm2;      // |b|^2 (after the optional mean removal)
@@ -144,6 +166,17 @@ struct CgScalars {
   long long total_solves;
   double rho[2];         // multigrid form: r.z of the current / previous direction
 };
+
+// The scalars a solve's host loop reads back after each batch of iterations, written by
+// one small kernel into pinned host memory (one launch instead of two device-to-host
+// copies).  n_fluid is -1 when not asked for.
+struct HostScalars {
+  CgScalars sc;
+  int n_fluid;
+  int pad;
+};
+void launch_read_scalars(hipStream_t s, const CgScalars* sc, const int* n_fluid,
+                         HostScalars* out);
 
 struct CgParams {
   int max_iter;

@@ -16,22 +16,22 @@ namespace fsg {
 // ---------------------------------------------------------------- cell init
 // MacGrid::clearCellTypeBuffer (src/MacGrid.cpp:32-50) with optional static solids,
 // fused with zeroing the P2G accumulators (src/FluidSolver.cpp:877-886).
-__global__ void __launch_bounds__(256) k_cell_init(Geom g, uint8_t* __restrict__ label,
-                                                   const uint8_t* __restrict__ solid,
-                                                   float4* __restrict__ acc, int do_labels,
-                                                   int do_acc) {
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
-  const int c = t * 256 + threadIdx.x;
-  const int n = g.nx * g.ny;
-  if (c >= n) return;
-  if (do_labels) {
+__device__ __forceinline__ void cell_init_one(const Geom& g, const CellInit& ci, int c) {
+  if (ci.label) {
     const int i = c % g.nx, j = c / g.nx;
     const bool border = (i == 0 || j == 0 || i == g.nx - 1 || j == g.ny - 1);
     uint8_t l = border ? kSolid : kAir;
-    if (solid && solid[c]) l = kSolid;
-    label[c] = l;
+    if (ci.solid && ci.solid[c]) l = kSolid;
+    ci.label[c] = l;
   }
-  if (do_acc) acc[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (ci.acc) ci.acc[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ void __launch_bounds__(256) k_cell_init(Geom g, CellInit ci) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int c = t * 256 + threadIdx.x;
+  if (c >= g.nx * g.ny) return;
+  cell_init_one(g, ci, c);
 }
 
 // ------------------------------------------------------- particle scatter pass
@@ -388,12 +388,12 @@ __global__ void __launch_bounds__(256) k_classify_levelset(Geom g, const float* 
 // cost ~1e-4 relative accuracy that the FP64 solve has.
 __global__ void __launch_bounds__(256) k_pressure_update(
     Geom g, const int* __restrict__ n_fluid, const uint8_t* __restrict__ label,
-    float* __restrict__ xg, const double* __restrict__ x64, const int* __restrict__ counts,
+    float* __restrict__ xg, double* __restrict__ x64, const int* __restrict__ counts,
     float ppk, float* __restrict__ au, float* __restrict__ av,
     float* __restrict__ bu, float* __restrict__ bv, float dt_over_density, int do_dirichlet,
     int do_diff, const float* __restrict__ prev_u, const float* __restrict__ prev_v,
     float* __restrict__ diff_u, float* __restrict__ diff_v, uint8_t* __restrict__ zmask,
-    uint8_t* __restrict__ dec) {
+    uint8_t* __restrict__ dec, MgFinishFuse fin) {
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int c = t * 256 + threadIdx.x;
   if (c >= g.nx * g.ny) return;
@@ -402,6 +402,19 @@ __global__ void __launch_bounds__(256) k_pressure_update(
   const int cjm = i + clampi(j - 1, 0, g.ny - 1) * g.nx;
   const uint8_t L = label[c], Lim = label[cim], Ljm = label[cjm];
   const float a_u = au[c], a_v = av[c];
+  // fused finish: the solution is x[iter & 1] at the unknowns.  x64 may be that array
+  // itself: a thread writes only its own cell, and reads its neighbours' only where they
+  // are unknowns, whose values it leaves unchanged.
+  const double* xs = x64;
+  if (fin.inf0) {
+    xs = (fin.sc->iter & 1) ? fin.x1 : x64;
+    const double v = (fin.inf0[c] & kMgUnknown) ? xs[c] : 0.0;
+    x64[c] = v;
+    xg[c] = (float)v;
+  }
+  auto pval = [&](int cc) -> double {
+    return !fin.inf0 || (fin.inf0[cc] & kMgUnknown) ? xs[cc] : 0.0;
+  };
   float nu, nv;
   if (*n_fluid > 0) {
     nu = bu[c];
@@ -416,9 +429,9 @@ __global__ void __launch_bounds__(256) k_pressure_update(
       pp_jm = ppk * (float)(n2 > 9 ? n2 - 8 : 0);
     }
     if (x64 && (L == kLiquid || Lim == kLiquid || Ljm == kLiquid)) {
-      const double p = L == kLiquid ? x64[c] + (double)pp : 0.0;
-      const double p_im = Lim == kLiquid ? x64[cim] + (double)pp_im : 0.0;
-      const double p_jm = Ljm == kLiquid ? x64[cjm] + (double)pp_jm : 0.0;
+      const double p = L == kLiquid ? pval(c) + (double)pp : 0.0;
+      const double p_im = Lim == kLiquid ? pval(cim) + (double)pp_im : 0.0;
+      const double p_jm = Ljm == kLiquid ? pval(cjm) + (double)pp_jm : 0.0;
       nu = (float)((double)a_u - (double)dt_over_density * (p - p_im) / (double)g.dx);
       nv = (float)((double)a_v - (double)dt_over_density * (p - p_jm) / (double)g.dy);
     } else if (L == kLiquid || Lim == kLiquid || Ljm == kLiquid) {
@@ -804,63 +817,92 @@ __device__ __forceinline__ int particle_bucket(const Geom& g, float4 q, int bx) 
 // Particles that were neighbours in storage stay neighbours in the copy, which keeps
 // the splat's quad aggregation effective on particles that have not mixed yet.
 // BATCH: a block is one particle tile of one scene (its own geometry); the histogram
-// holds that scene's buckets, which start at its bucket_base in cnt.
-template <bool BATCH>
+// holds that scene's buckets, which start at its bucket_base in cnt.  A single domain's
+// block takes PPT rounds of 256 consecutive particles (the histogram's clearing and
+// flush shared by PPT x 256 particles), and with ci set also the cell set-up of
+// k_cell_init, its cells shared out over the blocks (the splat reads both afterwards).
+template <bool BATCH, int PPT>
 __global__ void __launch_bounds__(256) k_bucket_count(Geom g0, const float4* __restrict__ pt,
                                                       int np, int bx0, int nb0,
                                                       int* __restrict__ cnt,
-                                                      int* __restrict__ rank, BatchTables tb) {
+                                                      int* __restrict__ rank, BatchTables tb,
+                                                      CellInit ci) {
+  static_assert(!BATCH || PPT == 1, "a batch block is one particle tile");
   extern __shared__ int s_h[];  // nb ints
-  const PartTile T = part_tile<BATCH>(g0, np, tb);
   int bx = bx0, nb = nb0, base = 0;
+  Geom g = g0;
+  int64_t p0, np_end;
   if (BATCH) {
+    const PartTile T = part_tile<true>(g0, np, tb);
     const BatchScene sc = tb.scene[tb.ptile[xcd_tile(blockIdx.x, gridDim.x)].x];
     bx = sc.bucket_bx;
     nb = sc.bucket_bx * ((sc.g.ny + (1 << kBucketShift) - 1) >> kBucketShift);
     base = sc.bucket_base;
+    g = T.g;
+    p0 = T.p - threadIdx.x;
+    np_end = T.np_end;
+  } else {
+    p0 = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * (256 * PPT);
+    np_end = np;
+    if (ci.label || ci.acc) {
+      const int C = g0.nx * g0.ny;
+      for (int c = blockIdx.x * 256 + threadIdx.x; c < C; c += gridDim.x * 256)
+        cell_init_one(g0, ci, c);
+    }
   }
   for (int e = threadIdx.x; e < nb; e += 256) s_h[e] = 0;
   __syncthreads();
-  const int64_t p = T.p;
-  const bool live = p < T.np_end;
-  const int b = live ? particle_bucket(T.g, pt[p], bx) : 0;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-  // this lane's rank among the wave's lanes of its bucket, its group's size and leader
-  int wrank = 0, wcount = 0, lead = lane;
-  bool done = !live;
-  for (;;) {
-    const uint64_t pending = __ballot(!done);
-    if (pending == 0ull) break;
-    const int first = __ffsll((long long)pending) - 1;
-    const int fb = __shfl(b, first, 64);
-    const bool mine = !done && b == fb;
-    const uint64_t same = __ballot(mine);
-    if (mine) {
-      wrank = __popcll(same & below);
-      wcount = __popcll(same);
-      lead = first;
-      done = true;
-    }
+  int b[PPT], local[PPT];
+#pragma unroll
+  for (int r = 0; r < PPT; ++r) {
+    const int64_t p = p0 + r * 256 + threadIdx.x;
+    const bool live = p < np_end;
+    b[r] = live ? particle_bucket(g, pt[p], bx) : 0;
   }
-  int start = 0;
-  for (int w = 0; w < 4; ++w) {
-    // the group leaders of wave w take their range; buckets are distinct within a wave
-    if (wave == w && live && wrank == 0) {
-      start = s_h[b];
-      s_h[b] = start + wcount;
+#pragma unroll
+  for (int r = 0; r < PPT; ++r) {
+    // round r: this lane's rank among the wave's lanes of its bucket, its group's size
+    // and leader; rounds, waves and lanes in storage order
+    const bool live = p0 + r * 256 + threadIdx.x < np_end;
+    int wrank = 0, wcount = 0, lead = lane;
+    bool done = !live;
+    for (;;) {
+      const uint64_t pending = __ballot(!done);
+      if (pending == 0ull) break;
+      const int first = __ffsll((long long)pending) - 1;
+      const int fb = __shfl(b[r], first, 64);
+      const bool mine = !done && b[r] == fb;
+      const uint64_t same = __ballot(mine);
+      if (mine) {
+        wrank = __popcll(same & below);
+        wcount = __popcll(same);
+        lead = first;
+        done = true;
+      }
     }
-    __syncthreads();
+    int start = 0;
+    for (int w = 0; w < 4; ++w) {
+      // the group leaders of wave w take their range; buckets are distinct within a wave
+      if (wave == w && live && wrank == 0) {
+        start = s_h[b[r]];
+        s_h[b[r]] = start + wcount;
+      }
+      __syncthreads();
+    }
+    local[r] = __shfl(start, lead, 64) + wrank;
   }
-  const int gstart = __shfl(start, lead, 64);
-  const int local = gstart + wrank;
-  __syncthreads();
   for (int e = threadIdx.x; e < nb; e += 256) {
     const int c = s_h[e];
     if (c) s_h[e] = atomicAdd(&cnt[base + e], c);
   }
   __syncthreads();
-  if (live) rank[p] = s_h[b] + local;
+#pragma unroll
+  for (int r = 0; r < PPT; ++r) {
+    const int64_t p = p0 + r * 256 + threadIdx.x;
+    if (p < np_end) rank[p] = s_h[b[r]] + local[r];
+  }
 }
 
 // Exclusive scan of the bucket counts into offsets (one workgroup, 4096 buckets per
@@ -936,8 +978,9 @@ static inline int nblocks(int64_t n) { return (int)((n + 255) / 256); }
 
 void launch_cell_init(hipStream_t s, const Geom& g, uint8_t* label, const uint8_t* solid,
                       float4* acc, int do_labels, int do_acc) {
-  k_cell_init<<<nblocks((int64_t)g.nx * g.ny), 256, 0, s>>>(g, label, solid, acc, do_labels,
-                                                             do_acc);
+  if (!do_labels && !do_acc) return;
+  const CellInit ci{do_labels ? label : nullptr, solid, do_acc ? acc : nullptr};
+  k_cell_init<<<nblocks((int64_t)g.nx * g.ny), 256, 0, s>>>(g, ci);
 }
 void launch_particle_scatter(hipStream_t s, const Geom& g, const float4* pt, int64_t np,
                              uint8_t* label, float4* acc, int do_classify, int do_p2g,
@@ -980,13 +1023,14 @@ int bucket_count(const Geom& g) {
 }
 
 void launch_bucket_order(hipStream_t s, const Geom& g, const float4* pt, int64_t np, int* cnt,
-                         int* off, int* rank, float4* sorted) {
+                         int* off, int* rank, float4* sorted, const CellInit& ci) {
   if (np <= 0) return;
   const int bx = (g.nx + (1 << kBucketShift) - 1) >> kBucketShift;
   const int nb = bucket_count(g);
   const int n = (int)np;
-  k_bucket_count<false><<<nblocks(np), 256, (size_t)nb * sizeof(int), s>>>(g, pt, n, bx, nb, cnt,
-                                                                            rank, BatchTables{});
+  constexpr int kPPT = 4;
+  k_bucket_count<false, kPPT><<<nblocks((np + kPPT - 1) / kPPT), 256, (size_t)nb * sizeof(int),
+                                s>>>(g, pt, n, bx, nb, cnt, rank, BatchTables{}, ci);
   k_bucket_scan<<<1, 1024, 0, s>>>(cnt, off, nb);
   k_bucket_fill<false><<<nblocks(np), 256, 0, s>>>(g, pt, n, bx, off, rank, sorted, BatchTables{});
 }
@@ -996,8 +1040,8 @@ void launch_bucket_order_batch(hipStream_t s, const float4* pt, int64_t np, int*
                                int max_scene_buckets) {
   if (np <= 0 || tb.n_ptile == 0) return;
   const int n = (int)np;
-  k_bucket_count<true><<<tb.n_ptile, 256, (size_t)max_scene_buckets * sizeof(int), s>>>(
-      Geom{}, pt, n, 0, 0, cnt, rank, tb);
+  k_bucket_count<true, 1><<<tb.n_ptile, 256, (size_t)max_scene_buckets * sizeof(int), s>>>(
+      Geom{}, pt, n, 0, 0, cnt, rank, tb, CellInit{});
   k_bucket_scan<<<1, 1024, 0, s>>>(cnt, off, n_buckets);
   k_bucket_fill<true><<<tb.n_ptile, 256, 0, s>>>(Geom{}, pt, n, 0, off, rank, sorted, tb);
 }
@@ -1021,14 +1065,15 @@ void launch_classify_levelset(hipStream_t s, const Geom& g, const float* phi,
   k_classify_levelset<<<nblocks((int64_t)g.nx * g.ny), 256, 0, s>>>(g, phi, solid, label);
 }
 void launch_pressure_update(hipStream_t s, const Geom& g, const int* n_fluid,
-                            const uint8_t* label, float* xg, const double* x64,
+                            const uint8_t* label, float* xg, double* x64,
                             const int* counts, float ppk, float* au, float* av,
                             float* bu, float* bv, float dt_over_density, int do_dirichlet,
                             int do_diff, const float* prev_u, const float* prev_v, float* diff_u,
-                            float* diff_v, uint8_t* zmask, uint8_t* dec) {
+                            float* diff_v, uint8_t* zmask, uint8_t* dec,
+                            const MgFinishFuse& fin) {
   k_pressure_update<<<nblocks((int64_t)g.nx * g.ny), 256, 0, s>>>(
       g, n_fluid, label, xg, x64, counts, ppk, au, av, bu, bv, dt_over_density, do_dirichlet,
-      do_diff, prev_u, prev_v, diff_u, diff_v, zmask, dec);
+      do_diff, prev_u, prev_v, diff_u, diff_v, zmask, dec, fin);
 }
 void launch_count_particles(hipStream_t s, const Geom& g, const float4* pt, int64_t np,
                             int* counts) {

@@ -57,7 +57,7 @@ namespace {
 
 constexpr int kT = 256;         // threads of the vector kernels and the level-0 tiles
 constexpr int kTileBig = 1024;  // threads of the tiles of the smaller levels
-constexpr uint8_t kAct = 0x80;    // info bit: the cell is an unknown
+constexpr uint8_t kAct = kMgUnknown;  // info bit: the cell is an unknown
 constexpr uint8_t kWall = 0x40;   // info bit: SOLID (also used for cells outside the grid)
 
 __device__ __forceinline__ bool is_act(uint8_t f) { return (f & kAct) != 0; }
@@ -2639,6 +2639,21 @@ void launch_mg_remove_mean(hipStream_t s, const MgHost& m) {
   const int C = m.nx[0] * m.ny[0], nb = vec_blocks(C);
   k_mg_mean_part<<<nb, kT, 0, s>>>(C, m.inf[0], m.x[0], m.part_sum, m.part_cnt);
   k_mg_mean_sub<<<nb, kT, 0, s>>>(C, m.inf[0], m.x[0], m.part_sum, m.part_cnt, nb);
+}
+
+__global__ void __launch_bounds__(64) k_read_scalars(const CgScalars* __restrict__ sc,
+                                                    const int* __restrict__ n_fluid,
+                                                    HostScalars* __restrict__ out) {
+  constexpr int kW = (int)(sizeof(CgScalars) / 4);
+  static_assert(sizeof(CgScalars) % 4 == 0 && kW < 64, "one word per lane");
+  const int t = threadIdx.x;
+  if (t < kW) reinterpret_cast<int*>(&out->sc)[t] = reinterpret_cast<const int*>(sc)[t];
+  if (t == 63) out->n_fluid = n_fluid ? *n_fluid : -1;
+  __threadfence_system();
+}
+void launch_read_scalars(hipStream_t s, const CgScalars* sc, const int* n_fluid,
+                         HostScalars* out) {
+  k_read_scalars<<<1, 64, 0, s>>>(sc, n_fluid, out);
 }
 
 void launch_mg_finish(hipStream_t s, const MgHost& m, int project_mean, CgScalars* sc,
