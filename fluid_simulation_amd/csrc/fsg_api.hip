@@ -147,17 +147,23 @@ struct AdjState {
 
 // The multigrid solve's iterations captured as HIP graphs, `iters` per graph
 // (fsg_params.graph_iters): graph c runs iterations [c iters, (c + 1) iters) (the last
-// one stops at the iteration cap).  Their launch arguments depend only on the
+// one stops at the iteration cap); iters == 0: one graph per solve length (solve, unit
+// below).  Their launch arguments depend only on the
 // iteration index and on the hierarchy's buffers, fixed once the hierarchy is built,
 // and on the tolerance, the cap and the smoothing settings (fsg_set_params drops the
 // graphs when those change).
 struct MgGraphs {
   int iters = 2;
   std::vector<hipGraphExec_t> exec;
+  // iters == 0: solve[L] runs a whole solve of L iterations (through the stop test of
+  // iteration L), unit[q] the rest of iteration q and the head of q + 1
+  std::vector<hipGraphExec_t> solve, unit;
   void clear() {
-    for (hipGraphExec_t e : exec)
-      if (e) (void)hipGraphExecDestroy(e);
-    exec.clear();
+    for (std::vector<hipGraphExec_t>* v : {&exec, &solve, &unit}) {
+      for (hipGraphExec_t e : *v)
+        if (e) (void)hipGraphExecDestroy(e);
+      v->clear();
+    }
   }
   ~MgGraphs() { clear(); }
 };
@@ -244,6 +250,8 @@ struct fsg_domain {
   bool mg_solved = false;  // the last solve was the multigrid form (its FP64 x is valid)
   int mg_last_iters = 0;      // iterations of the last forward multigrid solve
   int mg_last_iters_adj = 0;  // and of the last adjoint solve (a different system)
+  int mg_prev_iters = 0;      // the same for the solves before those
+  int mg_prev_iters_adj = 0;
   // host-side statistics of the plain solver
   fsg_stats host_stats{};
   std::string err;
@@ -586,6 +594,7 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr, bool defer_f
   // iterations are enqueued in batches between stop-test reads: first the previous
   // solve's count (of the same kind) + 1, then doubling batches
   int& last_iters = adj_rhs ? h->mg_last_iters_adj : h->mg_last_iters;
+  int& prev_iters = adj_rhs ? h->mg_prev_iters_adj : h->mg_prev_iters;
   int it = 0, batch = std::max(last_iters + 1, 4), more = 2;
   CgScalars hs{};
   // one timer name per level (the last tile level's down kernel also runs the
@@ -596,15 +605,28 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr, bool defer_f
   static const char* const kUpNames[8] = {"cg_mg_up0", "cg_mg_up1", "cg_mg_up2", "cg_mg_up3",
                                           "cg_mg_up4", "cg_mg_up5", "cg_mg_up6", "cg_mg_up7+"};
   // one CG iteration: the V-cycle's kernels, the direction (and the x / residual
-  // update where it is not fused into the next iteration's level-0 kernel)
-  auto enqueue_iteration = [&](int it) {
-    if (h->batch)
+  // update where it is not fused into the next iteration's level-0 kernel).  Its head
+  // is its kernels up to and including the stop test (a batch's check kernel; level 0's
+  // down kernel, or levels 0-1 in the fused form; the whole iteration where the
+  // hierarchy has no tile level), its tail the rest: once the solve has stopped in
+  // iteration q's head, the kernels after it return at once.
+  const int head_levels = h->batch ? 0 : m.fused ? 2 : 1;
+  const bool whole_head = !h->batch && m.lc < head_levels;
+  auto enqueue_part = [&](int it, bool head) {
+    if (h->batch && head)
       timed(h, "cg_mg_check", [&] {
         launch_mgb_check(h->stream, h->batch->cg_args(h->g.nx, h->g.ny), it, max_iter, tol, h->sc);
       });
-    for (int l = 0; l < m.lc; ++l)
-      timed(h, kDownNames[std::min(l, 7)],
-            [&] { launch_mg_down(h->stream, m, l, it, max_iter, tol, h->sc); });
+    if (!whole_head)
+      for (int l = 0; l < m.lc; ++l)
+        if ((l < head_levels) == head)
+          timed(h, kDownNames[std::min(l, 7)],
+                [&] { launch_mg_down(h->stream, m, l, it, max_iter, tol, h->sc); });
+    if (head != whole_head) return;
+    if (whole_head)
+      for (int l = 0; l < m.lc; ++l)
+        timed(h, kDownNames[std::min(l, 7)],
+              [&] { launch_mg_down(h->stream, m, l, it, max_iter, tol, h->sc); });
     if (h->batch)
       timed(h, "cg_mg_coarse", [&] { launch_mg_scene_coarse(h->stream, scene_coarse_args(h), h->sc); });
     else if (!mg_tail_coarse(m))
@@ -622,33 +644,86 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr, bool defer_f
       if (!m.fused) timed(h, "cg_mg_xr", [&] { launch_mg_xr(h->stream, m, it, h->sc); });
     }
   };
+  auto enqueue_iteration = [&](int it) {
+    enqueue_part(it, true);
+    enqueue_part(it, false);
+  };
   // Graph replay: graph c holds iterations [c K, (c + 1) K) (up to the cap), captured
   // on its first use.  Per-kernel timing and the kernel clocks need every launch.
   const bool use_graph = h->prm.graph && !h->timer.on && !m.clk_on && !m.clk_span_on;
-  if (h->mg_graphs.iters != std::max(1, h->prm.graph_iters)) {
+  if (h->mg_graphs.iters != h->prm.graph_iters) {
     h->mg_graphs.clear();
-    h->mg_graphs.iters = std::max(1, h->prm.graph_iters);
+    h->mg_graphs.iters = h->prm.graph_iters;
   }
-  const int K = h->mg_graphs.iters;
+  const int K = std::max(1, h->mg_graphs.iters);
+  // captures what body enqueues into *slot once
+  auto capture = [&](hipGraphExec_t* slot, auto&& body) -> int {
+    if (*slot) return FSG_OK;
+    hipGraph_t g = nullptr;
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    body();
+    HIPCHK(h, hipStreamEndCapture(h->stream, &g));
+    const hipError_t e = hipGraphInstantiate(slot, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      *slot = nullptr;
+      HIPCHK(h, e);
+    }
+    return FSG_OK;
+  };
   auto graph_of = [&](int c, hipGraphExec_t* out) -> int {
     std::vector<hipGraphExec_t>& ex = h->mg_graphs.exec;
     if ((int)ex.size() <= c) ex.resize(c + 1, nullptr);
-    if (!ex[c]) {
-      hipGraph_t g = nullptr;
-      HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-      for (int q = c * K; q < std::min((c + 1) * K, max_iter + 1); ++q) enqueue_iteration(q);
-      HIPCHK(h, hipStreamEndCapture(h->stream, &g));
-      const hipError_t e = hipGraphInstantiate(&ex[c], g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      if (e != hipSuccess) {
-        ex[c] = nullptr;
-        HIPCHK(h, e);
-      }
-    }
+    if (int rc = capture(&ex[c], [&] {
+          for (int q = c * K; q < std::min((c + 1) * K, max_iter + 1); ++q) enqueue_iteration(q);
+        }))
+      return rc;
     *out = ex[c];
     return FSG_OK;
   };
-  for (;;) {
+  auto read_back = [&]() -> int {
+    if (!h->hsc)
+      HIPCHK(h, hipHostMalloc((void**)&h->hsc, sizeof(HostScalars), hipHostMallocCoherent));
+    launch_read_scalars(h->stream, h->sc, F < 0 ? h->n_fluid : nullptr, h->hsc);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    memcpy(&hs, &h->hsc->sc, sizeof(hs));
+    if (F < 0) F = h->hsc->n_fluid;
+    return FSG_OK;
+  };
+  if (use_graph && h->mg_graphs.iters == 0) {
+    // one graph for a solve as long as the longer of the previous two (L iterations,
+    // through the stop test of iteration L), then one graph per further iteration until
+    // the test passes (a further graph costs a host round trip, a shorter solve than L
+    // the kernels of its remaining iterations, which return at once)
+    const int L = std::min(last_iters > 0 ? std::max(last_iters, prev_iters) : 4, max_iter);
+    std::vector<hipGraphExec_t>& sv = h->mg_graphs.solve;
+    if ((int)sv.size() <= L) sv.resize(L + 1, nullptr);
+    if (int rc = capture(&sv[L], [&] {
+          enqueue_part(0, true);
+          for (int q = 0; q < L; ++q) {
+            enqueue_part(q, false);
+            enqueue_part(q + 1, true);
+          }
+        }))
+      return rc;
+    HIPCHK(h, hipGraphLaunch(sv[L], h->stream));
+    it = L;
+    for (;;) {
+      if (int rc = read_back()) return rc;
+      if (hs.done || it >= max_iter) break;
+      std::vector<hipGraphExec_t>& uv = h->mg_graphs.unit;
+      if ((int)uv.size() <= it) uv.resize(it + 1, nullptr);
+      const int q = it;
+      if (int rc = capture(&uv[q], [&] {
+            enqueue_part(q, false);
+            enqueue_part(q + 1, true);
+          }))
+        return rc;
+      HIPCHK(h, hipGraphLaunch(uv[q], h->stream));
+      ++it;
+    }
+  }
+  while (!(use_graph && h->mg_graphs.iters == 0)) {
     const int target = std::min(it + batch, max_iter + 1);
     if (use_graph) {
       while (it < target) {
@@ -661,12 +736,7 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr, bool defer_f
     } else {
       for (; it < target; ++it) enqueue_iteration(it);
     }
-    if (!h->hsc)
-      HIPCHK(h, hipHostMalloc((void**)&h->hsc, sizeof(HostScalars), hipHostMallocCoherent));
-    launch_read_scalars(h->stream, h->sc, F < 0 ? h->n_fluid : nullptr, h->hsc);
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    memcpy(&hs, &h->hsc->sc, sizeof(hs));
-    if (F < 0) F = h->hsc->n_fluid;
+    if (int rc = read_back()) return rc;
     if (hs.done || it > max_iter) break;
     batch = more;
     more = std::min(2 * more, 32);
@@ -683,6 +753,7 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr, bool defer_f
             launch_mg_finish(h->stream, m, h->prm.project_mean, h->sc, h->xg,
                              h->batch ? &bs : nullptr);
           });
+  prev_iters = last_iters;
   last_iters = hs.iter;
   h->mg_solved = true;
   h->host_stats.cg_iterations = hs.iter;
@@ -949,7 +1020,7 @@ void fsg_params_default(fsg_params* p) {
   p->graph = 1;
   p->mg_smoother = 1;
   p->mg_coarse_sweeps = 0;
-  p->graph_iters = 2;
+  p->graph_iters = 0;
   p->mg_fuse_update = 1;
   p->splat_order = 1;
   p->mg_tile_threads = 256;
@@ -1059,7 +1130,7 @@ int fsg_set_params(fsg_domain* h, const fsg_params* p) {
       p->graph < 0 || p->graph > 1 || p->deterministic < 0 || p->deterministic > 1 ||
       p->record_decisions < 0 || p->record_decisions > 1 || p->mg_smoother < 0 ||
       p->mg_smoother > 1 || p->mg_coarse_sweeps < 0 || p->mg_coarse_sweeps > 64 ||
-      p->graph_iters < 1 || p->graph_iters > 16 || p->mg_fuse_update < 0 ||
+      p->graph_iters < 0 || p->graph_iters > 16 || p->mg_fuse_update < 0 ||
       p->mg_fuse_update > 1 || p->splat_order < 0 || p->splat_order > 1 ||
       (p->mg_tile_threads != 256 && p->mg_tile_threads != 512 && p->mg_tile_threads != 1024) ||
       p->mg_level_sweeps < 2 || p->mg_level_sweeps > 6 || p->mg_level0_sweeps < 2 ||
@@ -2038,14 +2109,14 @@ int fsg_autotune(fsg_domain* h, int32_t scheme, float dt, int32_t steps, fsg_tun
   for (const auto& ls : kSweeps)
     for (int co : {0, 8, 16})
       for (int fu : can_fuse ? std::initializer_list<int>{1, 0} : std::initializer_list<int>{1})
-        for (int gi : {2, 4})
+        for (int gi : {0, 2})
           for (int so : has_particles ? std::initializer_list<int>{1, 0}
                                       : std::initializer_list<int>{p0.splat_order})
             for (int nt : {256, 512}) cands.push_back({ls[0], ls[1], co, fu, gi, so, nt});
   TuneState saved;
   if (int rc = tune_save(h, saved)) return rc;
   const fsg_stats stats0 = h->host_stats;
-  const int last0 = h->mg_last_iters;
+  const int last0 = h->mg_last_iters, prev0 = h->mg_prev_iters;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   double best = 1e300, cur = -1.0;
   int best_k = 0, best_iters = 0, cur_k = -1, cur_iters = 0;
@@ -2109,6 +2180,7 @@ int fsg_autotune(fsg_domain* h, int32_t scheme, float dt, int32_t steps, fsg_tun
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->host_stats = stats0;
   h->mg_last_iters = last0;
+  h->mg_prev_iters = prev0;
   if (out) {
     memset(out, 0, sizeof(*out));
     out->mg_smoother = q.mg_smoother;

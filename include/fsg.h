@@ -151,7 +151,10 @@ typedef struct fsg_params {
                                those levels only (cheaper; as good where AIR cells pin
                                the coarse problem, much weaker for all-Neumann boxes) */
   int32_t graph_iters;    /* multigrid solver with graph = 1: iterations per captured
-                             graph, 1..16 (default 2) */
+                             graph, 1..16; 0 (default): one graph per solve, as long as
+                             the longer of the previous two solves, then one graph per
+                             further iteration (fewer iterations enqueued after the
+                             stop test, whose kernels return at once) */
   int32_t mg_fuse_update; /* multigrid solver, one domain of two or more tile levels:
                              1 (default) the CG update of x and the residual runs inside
                              the next iteration's level-0 down kernel; 0 in a kernel of
@@ -365,9 +368,11 @@ int fsg_reset_timing(fsg_domain *h);
 
 /* ---- autotuner (no counterpart in the reference) -----------------------------
  * Chooses the step's settings for the handle's workload by timing whole steps
- * (scheme, dt) from its current state: every combination of mg_smoother {1, 0},
- * mg_coarse_sweeps {0, 8, 16}, mg_fuse_update {1, 0} (one domain), graph_iters {2, 4}
- * and splat_order {1, 0} (with particles); each runs one untimed step, then `steps`
+ * (scheme, dt) from its current state: every combination of the smoothing sweeps
+ * (mg_level_sweeps, mg_level0_sweeps) {(5, 3), (5, 2), (2, 2)}, mg_coarse_sweeps
+ * {0, 8, 16}, mg_fuse_update {1, 0} (one domain), graph_iters {0, 2}, splat_order
+ * {1, 0} (with particles) and mg_tile_threads {256, 512} (144 for one domain with
+ * particles, 72 for a batch); each runs one untimed step, then `steps`
  * (1..64) timed steps, each from the saved state.  The fastest is set on the handle
  * unless the handle's own configuration is within 2 % of it, and the state
  * (particles, velocity grids, labels, reject flags, statistics) is restored, so the

@@ -103,3 +103,27 @@ def test_batch_rejects_unsupported_use():
     b.step(fsg.PICFLIP, 0.01)
     with pytest.raises(fsg.FsgError):
         b.scene_particles(3)
+
+
+def test_batch_solve_graphs_match_eager_launches():
+    """A batch's solves replayed from one graph per solve length (graph_iters 0) step
+    its scenes as kernel-by-kernel launches do (3 PICFLIP steps of 6 scenes): the same
+    iterations and labels, particles and velocities within 1e-6 (the per-scene dot
+    products are summed by FP64 atomics, so two runs differ in the last bits)."""
+    scs = scenes.dam_break_batch(6, 32, 80, seed=5)
+    kw = dict(cg_tol=1e-8, cg_max_iter=400, deterministic=1)
+    bs = [fsg.Batch.from_scenes(scs, graph=1, graph_iters=0, **kw),
+          fsg.Batch.from_scenes(scs, graph=0, **kw)]
+    for step in range(3):
+        its = []
+        for b in bs:
+            b.step(fsg.PICFLIP, 0.01)
+            its.append(b.stats().cg_iterations)
+        print(f"step {step}: iterations {its}", flush=True)
+        assert its[0] == its[1]
+    for s in range(len(scs)):
+        assert np.array_equal(bs[0].scene_labels(s), bs[1].scene_labels(s))
+        assert rel_err(bs[0].scene_particles(s), bs[1].scene_particles(s)) <= 1e-6
+        assert max(_vel_errs(*bs[0].scene_velocity(s), *bs[1].scene_velocity(s))) <= 1e-6
+    for b in bs:
+        b.close()

@@ -173,15 +173,19 @@ def test_mg_deterministic_and_no_fluid():
     assert np.array_equal(u, ud.astype(np.float32)) and np.array_equal(v, vd.astype(np.float32))
 
 
-def test_graph_replay_matches_eager_launches():
+@pytest.mark.parametrize("graph_iters,fuse", [(2, 1), (0, 1), (0, 0), (3, 0)])
+def test_graph_replay_matches_eager_launches(graph_iters, fuse):
     """The multigrid iterations replayed from captured HIP graphs (the default) give
     the pressure of the same solve launched kernel by kernel, bit for bit, over
     several solves of different right-hand sides on one handle (every replay
-    compared, with eager work and synchronisations in between)."""
+    compared, with eager work and synchronisations in between): graphs of graph_iters
+    iterations, or (0) one graph per solve length followed by one per further
+    iteration, for the fused and the separate update."""
     sc, sim = oracle_after(96, steps=3)
     rng = np.random.default_rng(21)
     doms = [gpu_from_oracle(sim, sc, cg_tol=1e-9, cg_max_iter=300, solver=fsg.SOLVER_MG,
-                            graph=g) for g in (1, 0)]
+                            graph=g, graph_iters=graph_iters, mg_fuse_update=fuse)
+            for g in (1, 0)]
     u0, v0 = sim.vel("front")
     for k in range(4):
         du = (rng.standard_normal(u0.shape) * 0.1 * k).astype(np.float32)
