@@ -487,6 +487,8 @@ int mg_prepare(fsg_domain* h) {
   ok &= dalloc(h, &m.part_rz, (size_t)std::max(mg_up_blocks(m.nx[0], m.ny[0]), 1)) == hipSuccess;
   ok &= dalloc(h, &m.clk, 2) == hipSuccess;
   if (m.lc > 0) ok &= dalloc(h, &m.tflag, (size_t)mg_up_blocks(m.nx[0], m.ny[0])) == hipSuccess;
+  if (m.lc > 0 && !h->batch)
+    ok &= dalloc(h, &m.tmap_buf, (size_t)mg_up_blocks(m.nx[0], m.ny[0])) == hipSuccess;
   if (m.per_scene)
     for (int l = 1; l < m.lc; ++l)
       ok &= dalloc(h, &m.tflag_l[l], (size_t)mg_up_blocks(m.nx[l], m.ny[l])) == hipSuccess;
@@ -573,6 +575,7 @@ int mg_solve(fsg_domain* h, int F, const double* adj_rhs = nullptr, bool defer_f
   m.l0_threads = h->prm.mg_tile_threads;
   m.level_sweeps = h->prm.mg_level_sweeps;
   m.level0_sweeps = h->prm.mg_level0_sweeps;
+  m.tmap = h->prm.mg_spread_tiles ? m.tmap_buf : nullptr;
   m.clk_on = h->kclock == 1;
   m.clk_span_on = h->kclock == 2;
   if (m.clk_span_on && !m.clk_span) alloc_clk_span(h);
@@ -1021,6 +1024,7 @@ void fsg_params_default(fsg_params* p) {
   p->mg_smoother = 1;
   p->mg_coarse_sweeps = 0;
   p->graph_iters = 0;
+  p->mg_spread_tiles = 1;
   p->mg_fuse_update = 1;
   p->splat_order = 1;
   p->mg_tile_threads = 256;
@@ -1134,7 +1138,7 @@ int fsg_set_params(fsg_domain* h, const fsg_params* p) {
       p->mg_fuse_update > 1 || p->splat_order < 0 || p->splat_order > 1 ||
       (p->mg_tile_threads != 256 && p->mg_tile_threads != 512 && p->mg_tile_threads != 1024) ||
       p->mg_level_sweeps < 2 || p->mg_level_sweeps > 6 || p->mg_level0_sweeps < 2 ||
-      p->mg_level0_sweeps > 3)
+      p->mg_level0_sweeps > 3 || p->mg_spread_tiles < 0 || p->mg_spread_tiles > 1)
     return fail(h, FSG_E_INVALID, "fsg_set_params: value out of range");
   const size_t C = ncell(h);
   // fixed-point sums: 4 per cell (the P2G accumulators; the density uses the first C),
@@ -1161,7 +1165,8 @@ int fsg_set_params(fsg_domain* h, const fsg_params* p) {
       p->mg_smoother != h->prm.mg_smoother || p->mg_coarse_sweeps != h->prm.mg_coarse_sweeps ||
       p->mg_fuse_update != h->prm.mg_fuse_update || p->mg_tile_threads != h->prm.mg_tile_threads ||
       p->mg_level_sweeps != h->prm.mg_level_sweeps ||
-      p->mg_level0_sweeps != h->prm.mg_level0_sweeps)
+      p->mg_level0_sweeps != h->prm.mg_level0_sweeps ||
+      p->mg_spread_tiles != h->prm.mg_spread_tiles)
     h->mg_graphs.clear();
   h->prm = *p;
   return FSG_OK;

@@ -296,6 +296,10 @@ struct MgHost {
   int clk_span_n = 0;        // launches recorded
   bool clk_span_on = false;
   uint8_t* tflag = nullptr;  // level-0 tiles (32 x 32) holding an unknown on level 0 or 1
+  // one domain: the level-0 tile kernels' block -> tile map (k_mg_tile_map), formed per
+  // solve from tflag: tile, bit 31 set where the tile is empty
+  int* tmap = nullptr;
+  int* tmap_buf = nullptr;  // its storage (tmap is null where the map is not used)
   // a batch: the same flags for its tile levels 1 .. lc-1 (most of a packed grid is air,
   // obstacles or gaps on every level)
   uint8_t* tflag_l[kMgMaxLevels] = {};

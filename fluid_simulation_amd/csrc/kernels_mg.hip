@@ -58,6 +58,7 @@ namespace {
 constexpr int kT = 256;         // threads of the vector kernels and the level-0 tiles
 constexpr int kTileBig = 1024;  // threads of the tiles of the smaller levels
 constexpr uint8_t kAct = kMgUnknown;  // info bit: the cell is an unknown
+constexpr int kTileEmptyBit = (int)0x80000000u;  // k_mg_tile_map: the tile holds no unknown
 constexpr uint8_t kWall = 0x40;   // info bit: SOLID (also used for cells outside the grid)
 
 __device__ __forceinline__ bool is_act(uint8_t f) { return (f & kAct) != 0; }
@@ -647,18 +648,103 @@ __global__ void __launch_bounds__(kT) k_mg_setup0t(Geom g, const uint8_t* __rest
 
 // The info bytes of levels 1 .. nli (their labels complete), by every block but the
 // last, which runs the coarsening tail (levels l0-1 and below; l0 - 1 > nli).
+// Block -> tile map of one domain's level-0 tile kernels.  Workgroup b runs on XCD
+// b % 8 (workgroups are handed to the XCDs in turn).  xcd_tile gives each XCD one band
+// of consecutive tiles, but a dam break's unknowns fill the tiles of a few bands only:
+// 5 of the 8 XCDs held all 278 live tiles of the developed 1024^2 flow, 52 each.  Here
+// the live tiles (row-major order) are dealt out in 8 contiguous runs of
+// lp = ceil(live / 8), run k to the first blocks of XCD k (33 per XCD there), and the
+// empty tiles take the remaining blocks in order.  tmap[b] = tile, bit 31 set where the
+// tile is empty.  Where a run would not fit an XCD's blocks (almost every tile live) the
+// map is xcd_tile's.  One 1024-thread workgroup.
+__device__ __forceinline__ int live_blocks_before(int b, int n_live, int lp) {
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int cnt = min(max(n_live - k * lp, 0), lp);
+    const int have = b > k ? (b - k + 7) / 8 : 0;
+    s += min(cnt, have);
+  }
+  return s;
+}
+
+__device__ __forceinline__ void tile_map_block(int nb, const uint8_t* __restrict__ tflag,
+                                               int* __restrict__ tmap) {
+  __shared__ int s_w[16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int n_live = 0;
+  for (int base = 0; base < nb; base += 1024) {
+    const int t = base + threadIdx.x;
+    n_live += __syncthreads_count(t < nb && tflag[t] != 0);
+  }
+  const int per = nb / 8, lp = (n_live + 7) / 8;
+  if (nb < 8 || lp > per) {
+    for (int b = threadIdx.x; b < nb; b += 1024) {
+      const int t = xcd_tile(b, nb);
+      tmap[b] = t | (tflag[t] ? 0 : kTileEmptyBit);
+    }
+    return;
+  }
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  int carry = 0;  // live tiles before this chunk
+  for (int base = 0; base < nb; base += 1024) {
+    const int t = base + threadIdx.x;
+    const bool live = t < nb && tflag[t] != 0;
+    const uint64_t m = __ballot(live);
+    if (lane == 0) s_w[wave] = __popcll(m);
+    __syncthreads();
+    int q = carry + __popcll(m & below);  // live tiles before t
+    int chunk = 0;
+    for (int w = 0; w < 16; ++w) {
+      if (w < wave) q += s_w[w];
+      chunk += s_w[w];
+    }
+    __syncthreads();  // s_w is rewritten by the next chunk
+    if (t < nb) {
+      if (live) {
+        tmap[(q % lp) * 8 + q / lp] = t;
+      } else {
+        // the (t - q)-th block that takes no live tile
+        const int e = t - q;
+        int lo = 0, hi = nb - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (mid + 1 - live_blocks_before(mid + 1, n_live, lp) > e)
+            hi = mid;
+          else
+            lo = mid + 1;
+        }
+        tmap[lo] = t | kTileEmptyBit;
+      }
+    }
+    carry += chunk;
+  }
+}
+__global__ void __launch_bounds__(1024) k_mg_tile_map(int nb, const uint8_t* __restrict__ tflag,
+                                                      int* __restrict__ tmap) {
+  tile_map_block(nb, tflag, tmap);
+}
+
 struct MgInfoArgs {
   int nli;
   int nx[kMgMaxLevels], ny[kMgMaxLevels];
   const uint8_t* lab[kMgMaxLevels];
   uint8_t* inf[kMgMaxLevels];
+  // set: the next-to-last block forms the level-0 tile map from tflag (nb tiles)
+  int nb_map;
+  const uint8_t* tflag;
+  int* tmap;
 };
 __global__ void __launch_bounds__(1024) k_mg_level_info(MgInfoArgs a, MgTailArgs tail) {
   if (blockIdx.x == gridDim.x - 1) {
     coarsen_tail(tail);
     return;
   }
-  const int stride = (gridDim.x - 1) * 1024;
+  if (a.tmap && blockIdx.x == gridDim.x - 2) {
+    tile_map_block(a.nb_map, a.tflag, a.tmap);
+    return;
+  }
+  const int stride = (gridDim.x - (a.tmap ? 2 : 1)) * 1024;
   for (int l = 1; l <= a.nli; ++l) {
     const int nx = a.nx[l], ny = a.ny[l], n = nx * ny;
     for (int t = blockIdx.x * 1024 + threadIdx.x; t < n; t += stride)
@@ -775,7 +861,7 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
                                                  const uint8_t* __restrict__ tflag, MgXr xr,
                                                  unsigned long long* span,
                                                  MgCoarseArgs ca, unsigned* __restrict__ ticket,
-                                                 MgSweeps sw) {
+                                                 MgSweeps sw, const int* __restrict__ tmap) {
   static_assert(!XR || L0, "the fused update belongs to level 0");
   static_assert(!TAIL || !L0, "the one-workgroup stage follows the last tile level");
   static_assert(S >= 2 && S <= kMgMaxSweeps && (!L0 || S <= 3),
@@ -809,9 +895,10 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
     else
       *dst = v;
   };
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  // the tile flag is read with the stop flag (one memory round trip)
-  const bool empty = tflag && !tflag[tile];
+  // the tile (and its flag) are read with the stop flag (one memory round trip)
+  const int tm = tmap ? tmap[blockIdx.x] : 0;
+  const int tile = tmap ? tm & ~kTileEmptyBit : xcd_tile(blockIdx.x, gridDim.x);
+  const bool empty = tmap ? tm < 0 : tflag && !tflag[tile];
   if (check) {
     if (mg_check<NT>(chk, it, sm)) {
       clock.end();
@@ -1091,7 +1178,7 @@ template <bool L0, int NT, int S = 2>
 __global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega, float omega2,
                                                const CgScalars* sc, double* __restrict__ part_rz,
                                                int it, const uint8_t* __restrict__ tflag,
-                                               MgSweeps sw) {
+                                               MgSweeps sw, const int* __restrict__ tmap) {
   static_assert(S >= 2 && S <= kMgMaxSweeps && (!L0 || S <= 3),
                 "level 0 takes two or three sweeps; smaller levels 2 .. kMgMaxSweeps");
   constexpr int W2 = 32 + 2 * S, W1 = W2 - 2;  // S = 2: 36, 34
@@ -1110,11 +1197,13 @@ __global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega,
   }
   const int tiles_x = (f.nx + 31) >> 5;
   // each XCD takes one band of consecutive tiles, so halo rows are shared in its L2
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  // (or the live tiles' share of them, tmap); the tile, its flag and the stop flag are
+  // read together (one memory round trip)
+  const int tm = tmap ? tmap[blockIdx.x] : 0;
+  const int tile = tmap ? tm & ~kTileEmptyBit : xcd_tile(blockIdx.x, gridDim.x);
+  const bool empty = tmap ? tm < 0 : tflag && !tflag[tile];
   const int bx = tile % tiles_x, by = tile / tiles_x;
   const int fi0 = bx * 32, fj0 = by * 32;
-  // the tile flag and the stop flag are read together (one memory round trip)
-  const bool empty = tflag && !tflag[tile];
   if (sc->done) return;
   if (empty) {
     // no unknown in the tile: z = 0 there (written once per solve) and r.z adds 0
@@ -1725,15 +1814,17 @@ __global__ void __launch_bounds__(kT) k_mg_dir_tiles(int nx, int ny,
                                                       const double* __restrict__ part_rz,
                                                       int n_rz, double* __restrict__ part_pq,
                                                       CgScalars* sc,
-                                                      const uint8_t* __restrict__ tflag) {
+                                                      const uint8_t* __restrict__ tflag,
+                                                      const int* __restrict__ tmap) {
   constexpr int R = 32 * 32 / kT;  // cells per thread
   __shared__ double sm[4];
   const int tiles_x = (nx + 31) >> 5;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  // the tile, its flag and the stop flag are read together (one memory round trip)
+  const int tm = tmap ? tmap[blockIdx.x] : 0;
+  const int tile = tmap ? tm & ~kTileEmptyBit : xcd_tile(blockIdx.x, gridDim.x);
+  const bool empty = tmap ? tm < 0 : !tflag[tile];
   const int bx = tile % tiles_x, by = tile / tiles_x;
   const int i = bx * 32 + (threadIdx.x & 31), j0 = by * 32 + (threadIdx.x >> 5);
-  // the tile flag and the stop flag are read together (one memory round trip)
-  const bool empty = !tflag[tile];
   if (sc->done) return;
   if (empty) {
     if (it <= 1)
@@ -2419,7 +2510,13 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
       a.lab[l] = m.lab[l];
       a.inf[l] = m.inf[l];
     }
-    const int blocks = std::min(std::max((cells + 1023) / 1024, 1), 1024) + 1;
+    // the level-0 tile flags came with k_mg_setup0t: the map is formed here too
+    if (m.tmap && !batch) {
+      ia.nb_map = mg_up_blocks(m.nx[0], m.ny[0]);
+      ia.tflag = m.tflag;
+      ia.tmap = m.tmap;
+    }
+    const int blocks = std::min(std::max((cells + 1023) / 1024, 1), 1024) + (ia.tmap ? 2 : 1);
     k_mg_level_info<<<blocks, 1024, 0, s>>>(ia, a);
   } else if (m.nlev > 1 && l0 < m.nlev) {
     MgTailArgs a{};
@@ -2445,6 +2542,8 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
           m.nx[l], m.ny[l], m.inf[l], m.nx[l + 1], m.ny[l + 1], m.inf[l + 1], flag,
           l == 0 ? &sc->live_tiles : nullptr);
   }
+  if (m.tmap && m.tflag && !batch && m.lc > 0 && !tiled)
+    k_mg_tile_map<<<1, 1024, 0, s>>>(mg_up_blocks(m.nx[0], m.ny[0]), m.tflag, m.tmap);
 }
 
 static int mg_down_blocks(const MgHost& m, int l) {
@@ -2518,11 +2617,11 @@ void launch_mg_down(hipStream_t s, MgHost& m, int l, int it, int max_iter, doubl
       if (m.fused)
         k_mg_down<true, NT, true, false, S><<<blocks, NT, 0, s>>>(
             mg_level(m, 0), mg_level(m, 1), m.coarse.omega, m.coarse.omega2, it, chk, ck, m.tflag,
-            xr, span, MgCoarseArgs{}, nullptr, sw0);
+            xr, span, MgCoarseArgs{}, nullptr, sw0, m.tmap);
       else
         k_mg_down<true, NT, false, false, S><<<blocks, NT, 0, s>>>(
             mg_level(m, 0), mg_level(m, 1), m.coarse.omega, m.coarse.omega2, it, chk, ck, m.tflag,
-            xr, span, MgCoarseArgs{}, nullptr, sw0);
+            xr, span, MgCoarseArgs{}, nullptr, sw0, m.tmap);
     };
     auto go_s = [&](auto nt) {
       if (m.level0_sweeps == 3)
@@ -2546,11 +2645,11 @@ void launch_mg_down(hipStream_t s, MgHost& m, int l, int it, int max_iter, doubl
       if (l == m.lc - 1 && mg_tail_coarse(m))
         k_mg_down<false, kTileBig, false, true, S><<<blocks, kTileBig, m.coarse_lds, s>>>(
             mg_level(m, l), mg_level(m, l + 1), m.coarse.omega, m.coarse.omega2, it, chk, check,
-            m.tflag_l[l], MgXr{}, nullptr, m.coarse, m.ticket, sw);
+            m.tflag_l[l], MgXr{}, nullptr, m.coarse, m.ticket, sw, nullptr);
       else
         k_mg_down<false, kTileBig, false, false, S><<<blocks, kTileBig, 0, s>>>(
             mg_level(m, l), mg_level(m, l + 1), m.coarse.omega, m.coarse.omega2, it, chk, check,
-            m.tflag_l[l], MgXr{}, nullptr, MgCoarseArgs{}, nullptr, sw);
+            m.tflag_l[l], MgXr{}, nullptr, MgCoarseArgs{}, nullptr, sw, nullptr);
     };
     switch (m.level_sweeps) {
       case 3: go(std::integral_constant<int, 3>{}); break;
@@ -2581,11 +2680,11 @@ void launch_mg_up(hipStream_t s, const MgHost& m, int l, int it, CgScalars* sc) 
     if (m.level0_sweeps == 3)
       k_mg_up<true, NT, 3><<<blocks, NT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega,
                                                  m.coarse.omega2, sc, m.part_rz, it, m.tflag,
-                                                 mg_sweeps0(m));
+                                                 mg_sweeps0(m), m.tmap);
     else
       k_mg_up<true, NT><<<blocks, NT, 0, s>>>(mg_level(m, 0), mg_level(m, 1), m.coarse.omega,
                                               m.coarse.omega2, sc, m.part_rz, it, m.tflag,
-                                              MgSweeps{});
+                                              MgSweeps{}, m.tmap);
   };
   if (l == 0 && m.l0_threads == 1024)
     go0(std::integral_constant<int, 1024>{});
@@ -2599,7 +2698,7 @@ void launch_mg_up(hipStream_t s, const MgHost& m, int l, int it, CgScalars* sc) 
       k_mg_up<false, kTileBig, S><<<blocks, kTileBig, 0, s>>>(mg_level(m, l), mg_level(m, l + 1),
                                                               m.coarse.omega, m.coarse.omega2, sc,
                                                               nullptr, it, m.tflag_l[l],
-                                                              mg_sweeps(m));
+                                                              mg_sweeps(m), nullptr);
     };
     switch (m.level_sweeps) {
       case 3: go(std::integral_constant<int, 3>{}); break;
@@ -2620,7 +2719,7 @@ void launch_mg_dir(hipStream_t s, const MgHost& m, int it, CgScalars* sc) {
   if (m.tflag) {
     k_mg_dir_tiles<<<mg_dir_blocks(m), kT, 0, s>>>(m.nx[0], m.ny[0], m.inf[0], m.xo[0],
                                                    m.p[it & 1], m.p[(it + 1) & 1], it, m.part_rz,
-                                                   n_rz, m.part_pq, sc, m.tflag);
+                                                   n_rz, m.part_pq, sc, m.tflag, m.tmap);
     return;
   }
   const int nb = vec_blocks(m.nx[0] * m.ny[0]);

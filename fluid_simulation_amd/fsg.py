@@ -87,7 +87,7 @@ class Params(C.Structure):
         ("mg_coarse_sweeps", C.c_int32), ("graph_iters", C.c_int32),
         ("mg_fuse_update", C.c_int32), ("splat_order", C.c_int32),
         ("mg_tile_threads", C.c_int32), ("mg_level_sweeps", C.c_int32),
-        ("mg_level0_sweeps", C.c_int32),
+        ("mg_level0_sweeps", C.c_int32), ("mg_spread_tiles", C.c_int32),
     ]
 
 

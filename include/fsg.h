@@ -182,6 +182,10 @@ typedef struct fsg_params {
                              weights) or 3 (default: the degree-3 Chebyshev weights on
                              [0.2, 2]; one more LDS phase in each level-0 kernel, about
                              7 % fewer iterations on the dam break) */
+  int32_t mg_spread_tiles;/* multigrid solver, one domain: 1 (default) the level-0 tile
+                             kernels deal the tiles holding unknowns out evenly over the
+                             XCDs (a block -> tile map formed per solve); 0 each XCD
+                             takes one band of consecutive tiles, live or not */
 } fsg_params;
 
 /* Counters of the last pressure solve and running totals. */

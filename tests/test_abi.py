@@ -26,8 +26,10 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_params_struct_layout_and_defaults():
-    # layout: 2 floats, double, 6 int32, double, 5 int32, 1 float, 12 int32
-    assert ctypes.sizeof(fsg.Params) == 4 * 2 + 8 + 4 * 6 + 8 + 4 * 5 + 4 + 4 * 12
+    # layout: 2 floats, double, 6 int32, double, 5 int32, 1 float, 13 int32, padded to
+    # the doubles' 8-byte alignment
+    assert ctypes.sizeof(fsg.Params) == 4 * 2 + 8 + 4 * 6 + 8 + 4 * 5 + 4 + 4 * 13 + 4
+    assert fsg.Params.mg_spread_tiles.offset == 120
     assert fsg.Params.mg_level_sweeps.offset == 112
     assert fsg.Params.particle_pressure_k.offset == 68
     assert fsg.Params.closed_walls.offset == 72
@@ -49,6 +51,7 @@ def test_params_struct_layout_and_defaults():
     assert p.mg_coarse_sweeps == 0 and p.graph_iters == 0 and p.mg_fuse_update == 1
     assert p.splat_order == 1
     assert p.mg_tile_threads == 256 and p.mg_level_sweeps == 5 and p.mg_level0_sweeps == 3
+    assert p.mg_spread_tiles == 1
 
 
 def test_noalloc_rejects_bad_sizes_before_any_device_call():

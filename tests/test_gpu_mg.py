@@ -217,3 +217,31 @@ def test_mg_level_sweeps_against_fp64(nx, ny, seed, sweeps, sweeps0):
           f"|r|/|b| {st.cg_error:.2e}, p vs fp64 {ep:.2e}")
     assert st.cg_error <= 1e-10
     assert ep <= 1e-6
+
+
+@pytest.mark.parametrize("nx,ny,seed", [(520, 390, 31), (1030, 257, 32)])
+def test_spread_tiles_against_fp64(nx, ny, seed):
+    """The level-0 tile kernels with the live tiles dealt out evenly over the XCDs
+    (mg_spread_tiles 1, a block -> tile map formed per solve) and with one band of
+    consecutive tiles per XCD (0): the same system solved to the same FP64 residual
+    bound, on a grid whose unknowns fill only some tiles (a LIQUID block in one corner
+    and a few pools elsewhere; the tile count is not a multiple of 8)."""
+    rng = np.random.default_rng(seed)
+    lab = np.full((ny, nx), AIR, np.uint8)
+    lab[0, :] = lab[-1, :] = lab[:, 0] = lab[:, -1] = SOL
+    lab[1:ny * 3 // 4, 1:nx * 2 // 5] = LIQ
+    for _ in range(4):
+        i, j = int(rng.integers(nx // 2, nx - 40)), int(rng.integers(1, ny - 40))
+        lab[j:j + 30, i:i + 30] = LIQ
+    out = []
+    for spread in (1, 0):
+        d, g = _solve_pair(lab, seed, mg_spread_tiles=spread)
+        st = g.stats()
+        liq = lab == LIQ
+        ep = rel_err(g.pressure()[liq], d.pressure[liq])
+        print(f"spread {spread}: {st.cg_iterations} iterations, {st.mg_live_tiles} live tiles, "
+              f"|r|/|b| {st.cg_error:.2e}, p vs fp64 {ep:.2e}")
+        assert st.cg_error <= 1e-10
+        assert ep <= 1e-6
+        out.append(g.pressure())
+    assert rel_err(out[0], out[1]) <= 1e-7
