@@ -899,6 +899,42 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
   const int tm = tmap ? tmap[blockIdx.x] : 0;
   const int tile = tmap ? tm & ~kTileEmptyBit : xcd_tile(blockIdx.x, gridDim.x);
   const bool empty = tmap ? tm < 0 : tflag && !tflag[tile];
+  constexpr int W3 = 34 + 2 * S, W2 = W3 - 2, W1 = 34;  // S = 2: 38, 36, 34
+  constexpr int W4 = W3 + 2;  // fused form: x_new on [-(S + 2), 34 + S)
+  constexpr int WC = 18;  // coarse cells of the tile and their neighbours
+  const int tiles_x = (c.nx + 15) >> 4;
+  // each XCD takes one band of consecutive tiles, so halo rows are shared in its L2
+  const int bx = tile % tiles_x, by = tile / tiles_x;
+  const int fi0 = bx * 32, fj0 = by * 32;
+  const bool do_xr = XR && it > 0;
+  // Without the fused update, every load of a tile holding unknowns is issued before
+  // the stop test's reads (their round trips overlap; a finished solve drops them) and
+  // before the first LDS store (a load outside the grid reads cell 0 and is replaced
+  // by the boundary value).
+  constexpr int NC = (WC * WC + NT - 1) / NT, NF = (W3 * W3 + NT - 1) / NT;
+  uint8_t cv[NC], fv[NF];
+  float rv[NF];
+  if (!empty && !do_xr) {
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      const int t = threadIdx.x + k * NT;
+      const int I = bx * 16 + t % WC - 1, J = by * 16 + t / WC - 1;
+      const bool in = t < WC * WC && I >= 0 && I < c.nx && J >= 0 && J < c.ny;
+      const uint8_t v = c.inf[in ? I + J * c.nx : 0];
+      cv[k] = in ? v : kWall;
+    }
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      const int t = threadIdx.x + k * NT;
+      const int i = fi0 + t % W3 - (S + 1), j = fj0 + t / W3 - (S + 1);
+      const bool in = t < W3 * W3 && i >= 0 && i < f.nx && j >= 0 && j < f.ny;
+      const int q = in ? i + j * f.nx : 0;
+      const uint8_t fq = f.inf[q];
+      const float rq = f.r[q];
+      fv[k] = in ? fq : 0;
+      rv[k] = in ? rq : 0.f;
+    }
+  }
   if (check) {
     if (mg_check<NT>(chk, it, sm)) {
       clock.end();
@@ -908,9 +944,6 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
     clock.end();
     return;
   }
-  constexpr int W3 = 34 + 2 * S, W2 = W3 - 2, W1 = 34;  // S = 2: 38, 36, 34
-  constexpr int W4 = W3 + 2;  // fused form: x_new on [-(S + 2), 34 + S)
-  constexpr int WC = 18;  // coarse cells of the tile and their neighbours
   __shared__ float s_r[W3 * W3], s_x1[W3 * W3], s_x2[W2 * W2];
   // the last sweep lands in x2 (S even) or x1 (S odd); the residual reuses the other
   // buffer, dead by then, so the smaller footprint lets more tiles share a CU
@@ -925,11 +958,6 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
   }
   __shared__ uint8_t s_inf[W3 * W3], s_cinf[WC * WC];
   __shared__ double s_xn[XR ? W4 * W4 : 1];
-  const int tiles_x = (c.nx + 15) >> 4;
-  // each XCD takes one band of consecutive tiles, so halo rows are shared in its L2
-  const int bx = tile % tiles_x, by = tile / tiles_x;
-  const int fi0 = bx * 32, fj0 = by * 32;
-  const bool do_xr = XR && it > 0;
   if (empty) {
     // no unknown in the tile on either level: its outputs are zero; written once per
     // solve (earlier solves may have left other values)
@@ -963,9 +991,7 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
   if (do_xr) {
     // every load of the tile first (a load outside the grid reads cell 0 and is
     // replaced by the boundary value), then alpha, then the stores to LDS
-    constexpr int NC = (WC * WC + NT - 1) / NT, NF = (W3 * W3 + NT - 1) / NT;
     constexpr int N4 = (W4 * W4 + NT - 1) / NT;
-    uint8_t cv[NC], fv[NF];
     double bv[NF], xo[N4];
     float pv[N4];
 #pragma unroll
@@ -1049,30 +1075,7 @@ __global__ void __launch_bounds__(NT) k_mg_down(MgLevel f, MgLevel c, float omeg
     write_part<NT>(acc, xr.part_rr, sm);
     if (blockIdx.x == 0 && threadIdx.x == 0) chk.sc->spmv = it;
   } else {
-    // every load of the tile is issued before the first LDS store (a load outside the
-    // grid reads cell 0 and is replaced by the boundary value)
-    constexpr int NC = (WC * WC + NT - 1) / NT, NF = (W3 * W3 + NT - 1) / NT;
-    uint8_t cv[NC], fv[NF];
-    float rv[NF];
-#pragma unroll
-    for (int k = 0; k < NC; ++k) {
-      const int t = threadIdx.x + k * NT;
-      const int I = bx * 16 + t % WC - 1, J = by * 16 + t / WC - 1;
-      const bool in = t < WC * WC && I >= 0 && I < c.nx && J >= 0 && J < c.ny;
-      const uint8_t v = c.inf[in ? I + J * c.nx : 0];
-      cv[k] = in ? v : kWall;
-    }
-#pragma unroll
-    for (int k = 0; k < NF; ++k) {
-      const int t = threadIdx.x + k * NT;
-      const int i = fi0 + t % W3 - (S + 1), j = fj0 + t / W3 - (S + 1);
-      const bool in = t < W3 * W3 && i >= 0 && i < f.nx && j >= 0 && j < f.ny;
-      const int q = in ? i + j * f.nx : 0;
-      const uint8_t fq = f.inf[q];
-      const float rq = f.r[q];
-      fv[k] = in ? fq : 0;
-      rv[k] = in ? rq : 0.f;
-    }
+    // the tile's loads were issued before the stop test
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
       const int t = threadIdx.x + k * NT;
@@ -1204,6 +1207,39 @@ __global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega,
   const bool empty = tmap ? tm < 0 : tflag && !tflag[tile];
   const int bx = tile % tiles_x, by = tile / tiles_x;
   const int fi0 = bx * 32, fj0 = by * 32;
+  const int ci0 = (fi0 >> 1) - CO, cj0 = (fj0 >> 1) - CO;  // coarse frame origin
+  // every load of a tile holding unknowns is issued before the stop flag's value is
+  // known (one memory round trip for both; the loads of a finished solve are dropped)
+  // and before the first LDS store (a load outside the grid reads cell 0 and is
+  // replaced by the boundary value)
+  constexpr int NC = (WC * WC + NT - 1) / NT, NF = (W2 * W2 + NT - 1) / NT;
+  uint8_t cv[NC], fv[NF];
+  float ev[NC], rv[NF], xv[NF];
+  if (!empty) {
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      const int t = threadIdx.x + k * NT;
+      const int I = ci0 + t % WC, J = cj0 + t / WC;
+      const bool in = t < WC * WC && I >= 0 && I < c.nx && J >= 0 && J < c.ny;
+      const int q = in ? I + J * c.nx : 0;
+      const float e = c.xo[q];
+      const uint8_t ci = c.inf[q];
+      ev[k] = in ? e : 0.f;
+      cv[k] = in ? ci : kWall;
+    }
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      const int t = threadIdx.x + k * NT;
+      const int i = fi0 + t % W2 - S, j = fj0 + t / W2 - S;
+      const bool in = t < W2 * W2 && i >= 0 && i < f.nx && j >= 0 && j < f.ny;
+      const int q = in ? i + j * f.nx : 0;
+      const uint8_t fq = f.inf[q];
+      const float rq = f.r[q], xq = f.xs[q];
+      fv[k] = in ? fq : 0;
+      rv[k] = in ? rq : 0.f;
+      xv[k] = in ? xq : 0.f;
+    }
+  }
   if (sc->done) return;
   if (empty) {
     // no unknown in the tile: z = 0 there (written once per solve) and r.z adds 0
@@ -1214,35 +1250,6 @@ __global__ void __launch_bounds__(NT) k_mg_up(MgLevel f, MgLevel c, float omega,
       }
     if (L0 && threadIdx.x == 0) part_rz[blockIdx.x] = 0.0;
     return;
-  }
-  const int ci0 = (fi0 >> 1) - CO, cj0 = (fj0 >> 1) - CO;  // coarse frame origin
-  // every load of the tile is issued before the first LDS store (a load outside the
-  // grid reads cell 0 and is replaced by the boundary value)
-  constexpr int NC = (WC * WC + NT - 1) / NT, NF = (W2 * W2 + NT - 1) / NT;
-  uint8_t cv[NC], fv[NF];
-  float ev[NC], rv[NF], xv[NF];
-#pragma unroll
-  for (int k = 0; k < NC; ++k) {
-    const int t = threadIdx.x + k * NT;
-    const int I = ci0 + t % WC, J = cj0 + t / WC;
-    const bool in = t < WC * WC && I >= 0 && I < c.nx && J >= 0 && J < c.ny;
-    const int q = in ? I + J * c.nx : 0;
-    const float e = c.xo[q];
-    const uint8_t ci = c.inf[q];
-    ev[k] = in ? e : 0.f;
-    cv[k] = in ? ci : kWall;
-  }
-#pragma unroll
-  for (int k = 0; k < NF; ++k) {
-    const int t = threadIdx.x + k * NT;
-    const int i = fi0 + t % W2 - S, j = fj0 + t / W2 - S;
-    const bool in = t < W2 * W2 && i >= 0 && i < f.nx && j >= 0 && j < f.ny;
-    const int q = in ? i + j * f.nx : 0;
-    const uint8_t fq = f.inf[q];
-    const float rq = f.r[q], xq = f.xs[q];
-    fv[k] = in ? fq : 0;
-    rv[k] = in ? rq : 0.f;
-    xv[k] = in ? xq : 0.f;
   }
 #pragma unroll
   for (int k = 0; k < NC; ++k) {
