@@ -132,8 +132,8 @@ def solve_bytes(name, nx, ny, solves):
 
 
 # HBM traffic per launch on the 1024^2 dam break (bytes), from two rocprofv3 --pmc
-# passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, averaged over the run's
-# dispatches) of
+# passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, averaged over the dispatches
+# of the last 8 steps, 250-257) of
 #   bench.py --graph 0 --steps 5 --warmup 2 --start 250 --run-steps 0 --plain-steps 0
 #            --refined-steps 0 --profile-steps 1 --no-fp64-check --no-cpu
 #            --batch-scenes 0 --grid-steps 0 --adj-steps 0 --autotune 0
@@ -141,13 +141,13 @@ def solve_bytes(name, nx, ny, solves):
 # FETCH_SIZE reads 0.5 GiB for its 1 GiB read (factor 2.0), WRITE_SIZE 1 GiB (1.0).
 # traffic = (2.0 * FETCH_SIZE + 1.0 * WRITE_SIZE) * 1024.
 MEASURED_TRAFFIC_1024 = {
-    "cg_mg_down0": (2.0 * 2966.4 + 6284.4) * 1024,
-    "cg_mg_up0": (2.0 * 1399.2 + 1301.4) * 1024,
-    "cg_mg_dir": (2.0 * 1133.1 + 1646.4) * 1024,
-    "cg_mg_down4": (2.0 * 61.4 + 20.9) * 1024,
-    "particle_scatter": (2.0 * 8064.1 + 44602.8) * 1024,
-    "particle_gather": (2.0 * 10185.7 + 16676.3) * 1024,
-    "face_update": (2.0 * 12819.0 + 28672.4) * 1024,
+    "cg_mg_down0": (2.0 * 3327.3 + 6436.8) * 1024,
+    "cg_mg_up0": (2.0 * 1836.4 + 1343.8) * 1024,
+    "cg_mg_dir": (2.0 * 1270.6 + 1676.0) * 1024,
+    "cg_mg_down4": (2.0 * 62.1 + 21.5) * 1024,
+    "particle_scatter": (2.0 * 8065.7 + 50236.7) * 1024,
+    "particle_gather": (2.0 * 10294.8 + 16676.3) * 1024,
+    "face_update": (2.0 * 12821.0 + 28672.1) * 1024,
 }
 
 
