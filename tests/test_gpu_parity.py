@@ -83,6 +83,31 @@ def test_p2g_within_1e5(state):
     _assert_vel_equal(g, o, "back")
 
 
+def test_p2g_large_velocities_within_1e5(state):
+    """The splat sums a block's terms in 64-bit fixed point in LDS unless one of its
+    particles moves faster than the fixed point's range allows (|v| > 16384): those
+    blocks sum in float.  A state with every 1000th particle 1e5 times faster (so most
+    blocks take the fixed point, the others the float sums) still splats to within 1e-5
+    of the oracle."""
+    sc, sim = state
+    from oracle import OracleSim
+    o = OracleSim(sim.nx, sim.ny, sc.length_x, sc.length_y, sc.density, sc.pic_ratio, "f32")
+    copy_state(sim, o)
+    p = o.particles.copy()
+    p[::1000, 2:] *= np.float32(1e5)
+    o.set_particles(p)
+    g = gpu_from_oracle(o, sc)
+    for s in (o, g):
+        s.run_phase("CLASSIFY")
+        s.run_phase("P2G")
+    u, v = g.velocity(fsg.VEL_FRONT)
+    uo, vo = o.vel("front")
+    eu, ev = rel_err(u, uo), rel_err(v, vo)
+    print(f"p2g (fast particles) rel err u {eu:.2e} v {ev:.2e}, max |u| {np.abs(uo).max():.3g}")
+    assert np.abs(p[:, 2:]).max() > 16384
+    assert eu <= 1e-5 and ev <= 1e-5
+
+
 @pytest.mark.parametrize("phase", ["SAVE_PREV", "GRAVITY", "DIRICHLET", "DIFF"])
 def test_face_phases_bit_exact(state, phase):
     sc, o, g = _pair(state)
