@@ -86,15 +86,16 @@ def test_p2g_within_1e5(state):
 def test_p2g_large_velocities_within_1e5(state):
     """The splat sums a block's terms in 64-bit fixed point in LDS unless one of its
     particles moves faster than the fixed point's range allows (|v| > 16384): those
-    blocks sum in float.  A state with every 1000th particle 1e5 times faster (so most
-    blocks take the fixed point, the others the float sums) still splats to within 1e-5
-    of the oracle."""
+    blocks sum in float.  A state whose every 1000th particle moves at (3e4, -2.5e4) (so
+    most blocks take the fixed point, the others the float sums) still splats to within
+    1e-5 of the oracle."""
     sc, sim = state
     from oracle import OracleSim
     o = OracleSim(sim.nx, sim.ny, sc.length_x, sc.length_y, sc.density, sc.pic_ratio, "f32")
     copy_state(sim, o)
     p = o.particles.copy()
-    p[::1000, 2:] *= np.float32(1e5)
+    p[::1000, 2] = np.float32(3.0e4)
+    p[::1000, 3] = np.float32(-2.5e4)
     o.set_particles(p)
     g = gpu_from_oracle(o, sc)
     for s in (o, g):
