@@ -560,6 +560,17 @@ struct MgTileLabels {
   int nx[6], ny[6];
   uint8_t* lab[6];
 };
+// Set: k_mg_setup0t also does k_mg_setup1's work for a cold start without the mean
+// projection (x0 = 0, the FP32 residual r0 = b', one |b'|^2 partial per tile in both
+// part_bb and part_rr, zero partials from n_tiles to nb, the solve's scalars reset).
+struct MgSetupFold {
+  double* x0 = nullptr;
+  float* r0 = nullptr;
+  double* part_bb = nullptr;
+  double* part_rr = nullptr;
+  int nb = 0;
+  CgScalars* sc = nullptr;
+};
 __global__ void __launch_bounds__(kT) k_mg_setup0t(Geom g, const uint8_t* __restrict__ label,
                                                     const float* __restrict__ uf,
                                                     const float* __restrict__ vf,
@@ -569,16 +580,17 @@ __global__ void __launch_bounds__(kT) k_mg_setup0t(Geom g, const uint8_t* __rest
                                                     double* __restrict__ part_cnt,
                                                     float* __restrict__ rhs,
                                                     double* __restrict__ part_liq,
-                                                    uint8_t* __restrict__ tflag, MgTileLabels L) {
+                                                    uint8_t* __restrict__ tflag, MgTileLabels L,
+                                                    MgSetupFold fold) {
   constexpr uint8_t kNone = 0xFF;  // a cell outside the grid
-  __shared__ double sm[3 * (kT / 64)];
+  __shared__ double sm[4 * (kT / 64)];
   __shared__ uint8_t s_l[2][32 * 32];
   const int nx = g.nx, ny = g.ny;
   const int tiles_x = (nx + 31) >> 5;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int i0 = (tile % tiles_x) * 32, j0 = (tile / tiles_x) * 32;
   const double scale = -(double)g.dx * (double)g.dx;
-  double acc[3] = {0.0, 0.0, 0.0};  // sum b', unknowns, LIQUID cells
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};  // sum b', unknowns, LIQUID cells, |b'|^2
   int any = 0;
 #pragma unroll
   for (int k = 0; k < 32 * 32 / kT; ++k) {
@@ -609,17 +621,40 @@ __global__ void __launch_bounds__(kT) k_mg_setup0t(Geom g, const uint8_t* __rest
       }
       b[c] = bv;
       rhs[c] = bf;
+      if (fold.x0) {
+        fold.x0[c] = 0.0;
+        fold.r0[c] = (float)bv;
+        acc[3] += bv * bv;
+      }
     }
     s_l[0][t] = lab;
   }
   // the partials in k_mg_setup0's order of magnitude (one per tile, fixed order)
-  block_sum<3, kT>(acc, sm);
+  block_sum<4, kT>(acc, sm);
   any = __syncthreads_or(any);
   if (threadIdx.x == 0) {
     part_sum[blockIdx.x] = acc[0];
     part_cnt[blockIdx.x] = acc[1];
     part_liq[blockIdx.x] = acc[2];
     tflag[tile] = any ? 1 : 0;
+    if (fold.x0) {
+      fold.part_bb[blockIdx.x] = acc[3];
+      fold.part_rr[blockIdx.x] = acc[3];
+    }
+  }
+  if (fold.x0 && blockIdx.x == 0) {
+    for (int e = (int)gridDim.x + threadIdx.x; e < fold.nb; e += kT) {
+      fold.part_bb[e] = 0.0;
+      fold.part_rr[e] = 0.0;
+    }
+    if (threadIdx.x == 0) {
+      CgScalars* sc = fold.sc;
+      sc->done = 0;
+      sc->iter = 0;
+      sc->spmv = 0;
+      sc->err = 0.0;
+      sc->rho[0] = sc->rho[1] = 0.0;
+    }
   }
   for (int l = 1; l <= L.nl; ++l) {
     const int w = 32 >> l, wf = 2 * w;
@@ -646,8 +681,6 @@ __global__ void __launch_bounds__(kT) k_mg_setup0t(Geom g, const uint8_t* __rest
   }
 }
 
-// The info bytes of levels 1 .. nli (their labels complete), by every block but the
-// last, which runs the coarsening tail (levels l0-1 and below; l0 - 1 > nli).
 // Block -> tile map of one domain's level-0 tile kernels.  Workgroup b runs on XCD
 // b % 8 (workgroups are handed to the XCDs in turn).  xcd_tile gives each XCD one band
 // of consecutive tiles, but a dam break's unknowns fill the tiles of a few bands only:
@@ -725,6 +758,9 @@ __global__ void __launch_bounds__(1024) k_mg_tile_map(int nb, const uint8_t* __r
   tile_map_block(nb, tflag, tmap);
 }
 
+// The info bytes of levels 1 .. nli (their labels complete), by every block but the
+// last, which runs the coarsening tail (levels l0-1 and below; l0 - 1 > nli) and, where
+// k_mg_setup0t did k_mg_setup1's work, the solve's LIQUID count and live level-0 tiles.
 struct MgInfoArgs {
   int nli;
   int nx[kMgMaxLevels], ny[kMgMaxLevels];
@@ -734,10 +770,31 @@ struct MgInfoArgs {
   int nb_map;
   const uint8_t* tflag;
   int* tmap;
+  // set (setup folded): *n_fluid = the sum of n_part part_liq partials, and
+  // sc->live_tiles = the tiles flagged in tflag0 (n_tiles)
+  const double* part_liq;
+  int n_part;
+  int* n_fluid;
+  const uint8_t* tflag0;
+  int n_tiles;
+  CgScalars* sc;
 };
 __global__ void __launch_bounds__(1024) k_mg_level_info(MgInfoArgs a, MgTailArgs tail) {
   if (blockIdx.x == gridDim.x - 1) {
     coarsen_tail(tail);
+    if (a.part_liq) {
+      __shared__ double sm[16];
+      const double n = sum_part<1024>(a.part_liq, a.n_part, sm);
+      int live = 0;
+      for (int base = 0; base < a.n_tiles; base += 1024) {
+        const int t = base + threadIdx.x;
+        live += __syncthreads_count(t < a.n_tiles && a.tflag0[t] != 0);
+      }
+      if (threadIdx.x == 0) {
+        *a.n_fluid = (int)n;
+        a.sc->live_tiles = live;
+      }
+    }
     return;
   }
   if (a.tmap && blockIdx.x == gridDim.x - 2) {
@@ -2445,6 +2502,10 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
   // and the coarsening tail in one launch (k_mg_level_info)
   const bool tiled = !adj_rhs && !batch && rhs && m.tflag && m.nlev >= 3 && l0 < m.nlev;
   const int n_tiles = mg_up_blocks(g.nx, g.ny);
+  // a cold start without the mean projection: k_mg_setup0t does k_mg_setup1's per-cell
+  // work (its tile partials in the vector kernels' nb slots) and k_mg_level_info the
+  // counts; k_mg_setup1 does not run
+  const bool folded = tiled && !project_mean && !warm && n_tiles <= nb;
   const int lk = tiled ? std::min(std::min(5, l0 - 1), m.nlev - 1) : 0;
   int n_part = nb;
   if (adj_rhs) {
@@ -2462,8 +2523,10 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
       L.ny[l] = m.ny[l];
       L.lab[l] = m.lab[l];
     }
+    MgSetupFold fold;
+    if (folded) fold = MgSetupFold{m.x[0], m.rl[0], m.part_bb, m.part_rr, nb, sc};
     k_mg_setup0t<<<n_tiles, kT, 0, s>>>(g, label, uf, vf, m.inf[0], m.b, m.part_sum, m.part_cnt,
-                                         rhs, m.part_liq, m.tflag, L);
+                                         rhs, m.part_liq, m.tflag, L, fold);
     n_part = n_tiles;
   } else {
     k_mg_setup0<<<nb, kT, 0, s>>>(g, label, uf, vf, m.inf[0], m.b, m.part_sum, m.part_cnt, rhs,
@@ -2480,11 +2543,12 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
   // an adjoint right-hand side is not mean-projected: S^T y already has zero sum for a
   // singular system, and a non-singular one solves A y = x-bar itself (P follows the
   // solve)
-  k_mg_setup1<<<nb, kT, 0, s>>>(g.nx, g.ny, adj_rhs ? 0 : project_mean, m.inf[0], m.b,
-                                warm ? m.x[1] : nullptr,
-                                m.x[0], m.rl[0], m.part_sum, m.part_cnt, n_part, m.part_bb,
-                                m.part_rr, sc, rhs && !adj_rhs ? m.part_liq : nullptr, n_fluid,
-                                tiled ? m.tflag : nullptr, n_tiles);
+  if (!folded)
+    k_mg_setup1<<<nb, kT, 0, s>>>(g.nx, g.ny, adj_rhs ? 0 : project_mean, m.inf[0], m.b,
+                                  warm ? m.x[1] : nullptr,
+                                  m.x[0], m.rl[0], m.part_sum, m.part_cnt, n_part, m.part_bb,
+                                  m.part_rr, sc, rhs && !adj_rhs ? m.part_liq : nullptr, n_fluid,
+                                  tiled ? m.tflag : nullptr, n_tiles);
   // the levels not formed by k_mg_setup0t, one launch each, down to level l0 - 1
   for (int l = std::max(1, lk + 1); l < std::min(l0, m.nlev); ++l) {
     const int nf = m.nx[l - 1] * m.ny[l - 1], ncl = m.nx[l] * m.ny[l];
@@ -2522,6 +2586,14 @@ void launch_mg_setup(hipStream_t s, const Geom& g, const MgHost& m, const uint8_
       ia.nb_map = mg_up_blocks(m.nx[0], m.ny[0]);
       ia.tflag = m.tflag;
       ia.tmap = m.tmap;
+    }
+    if (folded) {
+      ia.part_liq = m.part_liq;
+      ia.n_part = n_tiles;
+      ia.n_fluid = n_fluid;
+      ia.tflag0 = m.tflag;
+      ia.n_tiles = n_tiles;
+      ia.sc = sc;
     }
     const int blocks = std::min(std::max((cells + 1023) / 1024, 1), 1024) + (ia.tmap ? 2 : 1);
     k_mg_level_info<<<blocks, 1024, 0, s>>>(ia, a);
