@@ -342,15 +342,23 @@ __global__ void __launch_bounds__(256) k_extend_sweep(
   if (c >= nx * g.ny) return;
   const int i = c % nx, j = c / nx;
   if (i < 1 || j < 1 || i > nx - 2 || j > g.ny - 2) return;
-  const uint8_t L = label[c];
+  // every load first (one memory round trip): the neighbours' values are used only where
+  // their front mask is 1, and this sweep writes only faces whose front mask is 0
+  const uint8_t L = label[c], Lim = label[c - 1], Ljm = label[c - nx];
+  const uint8_t mx0 = mxf[c], mx1 = mxf[c - 1], mx2 = mxf[c - nx], mx4 = mxf[c + nx],
+                mx8 = mxf[c + 1];
+  const uint8_t my0 = myf[c], my1 = myf[c - 1], my2 = myf[c - nx], my4 = myf[c + nx],
+                my8 = myf[c + 1];
+  const float u1 = ub[c - 1], u2 = ub[c - nx], u4 = ub[c + nx], u8 = ub[c + 1];
+  const float v1 = vb[c - 1], v2 = vb[c - nx], v4 = vb[c + nx], v8 = vb[c + 1];
   if (L == kSolid) return;
-  if (mxf[c] == 0 && label[c - 1] != kSolid) {
+  if (mx0 == 0 && Lim != kSolid) {
     float acc = 0.f;
     int n = 0, bits = 0;
-    if (mxf[c - 1] == 1) { acc += ub[c - 1]; n++; bits |= 1; }
-    if (mxf[c - nx] == 1) { acc += ub[c - nx]; n++; bits |= 2; }
-    if (mxf[c + nx] == 1) { acc += ub[c + nx]; n++; bits |= 4; }
-    if (mxf[c + 1] == 1) { acc += ub[c + 1]; n++; bits |= 8; }
+    if (mx1 == 1) { acc += u1; n++; bits |= 1; }
+    if (mx2 == 1) { acc += u2; n++; bits |= 2; }
+    if (mx4 == 1) { acc += u4; n++; bits |= 4; }
+    if (mx8 == 1) { acc += u8; n++; bits |= 8; }
     if (n > 0) {
       acc /= (float)n;
       ub[c] = acc;
@@ -358,13 +366,13 @@ __global__ void __launch_bounds__(256) k_extend_sweep(
       if (info_u) info_u[c] = (uint8_t)((layer << 4) | bits);
     }
   }
-  if (myf[c] == 0 && label[c - nx] != kSolid) {
+  if (my0 == 0 && Ljm != kSolid) {
     float acc = 0.f;
     int n = 0, bits = 0;
-    if (myf[c - 1] == 1) { acc += vb[c - 1]; n++; bits |= 1; }
-    if (myf[c - nx] == 1) { acc += vb[c - nx]; n++; bits |= 2; }
-    if (myf[c + nx] == 1) { acc += vb[c + nx]; n++; bits |= 4; }
-    if (myf[c + 1] == 1) { acc += vb[c + 1]; n++; bits |= 8; }
+    if (my1 == 1) { acc += v1; n++; bits |= 1; }
+    if (my2 == 1) { acc += v2; n++; bits |= 2; }
+    if (my4 == 1) { acc += v4; n++; bits |= 4; }
+    if (my8 == 1) { acc += v8; n++; bits |= 8; }
     if (n > 0) {
       acc /= (float)n;
       vb[c] = acc;
