@@ -2649,11 +2649,12 @@ static MgLevel mg_level(const MgHost& m, int l) {
 // Chebyshev polynomial on [lo_S, 2] (the high end of the spectrum of D^-1 K), smallest
 // first (every partial product of the factors stays within [-1, 1] on [0, 2]).  lo_S
 // per degree from a model of the solve over developed dam-break domains (fewest
-// iterations): 0.2, 0.2, 0.05, 0.05 for S = 3 .. 6.
+// iterations): 0.2, 0.2, 0.03, 0.05 for S = 3 .. 6 (S = 5: 48 iterations over six
+// developed 1024^2 dam-break solves against 50 with 0.05).
 static MgSweeps cheb_sweeps(int S) {
   MgSweeps w{};
   if (S <= 2 || S > kMgMaxSweeps) return w;
-  static const double kLo[kMgMaxSweeps + 1] = {0, 0, 0.4, 0.2, 0.2, 0.05, 0.05};
+  static const double kLo[kMgMaxSweeps + 1] = {0, 0, 0.4, 0.2, 0.2, 0.03, 0.05};
   const double lo = kLo[S], hi = 2.0, pi = 3.14159265358979323846;
   double r[kMgMaxSweeps];
   for (int k = 0; k < S; ++k)

@@ -173,7 +173,7 @@ typedef struct fsg_params {
                              coarse correction on the tile levels below level 0 (level 0
                              and the one-workgroup levels keep two), 2 .. 6: 2 uses the
                              mg_smoother weights; S > 2 the degree-S Chebyshev weights
-                             on [lo_S, 2] (lo = 0.2, 0.2, 0.05, 0.05 for S = 3 .. 6).
+                             on [lo_S, 2] (lo = 0.2, 0.2, 0.03, 0.05 for S = 3 .. 6).
                              Those levels are bound by launch and barrier latency, so
                              the extra sweeps cost little and save iterations: 5
                              (default) takes the developed 1024^2 dam break from 13 to 9
