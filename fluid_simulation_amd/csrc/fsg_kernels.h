@@ -308,7 +308,7 @@ struct MgHost {
   bool fused = false;
   int l0_threads = 256;  // threads per level-0 tile of the down / up kernels (256/512/1024)
   // sweeps before and after the coarse correction on the tile levels below level 0: 2
-  // (the level's omega, omega2) or 4 (degree-4 Chebyshev weights)
+  // (the level's omega, omega2) or 3 .. kMgMaxSweeps (degree-S Chebyshev weights)
   int level_sweeps = 2;
   int level0_sweeps = 2;  // the same on level 0: 2 (omega, omega2) or 3 (degree-3 Chebyshev)
   // the last tile level's down kernel also runs the one-workgroup stage (its last
