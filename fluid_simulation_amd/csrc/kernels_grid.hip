@@ -58,7 +58,9 @@ __global__ void __launch_bounds__(256) k_cell_init(Geom g, CellInit ci) {
 // footprint fits kSplatTileFx cells: LDS integer atomics issue about twice as fast as
 // float ones on gfx950 (the developed 1024^2 dam break's splat 99 -> ~50 us).  The
 // float form then adds each fixed-point sum to the grid as one float (one rounding
-// instead of one per term); the deterministic form adds the integers.  A block whose
+// instead of one per term); the deterministic form adds the integers.  The integer
+// adds go to LDS one per lane: summing a quad's equal addresses first (DPP, as the
+// float sums do) cost more than it saved on bucket-ordered particles.  A block whose
 // terms could overflow the fixed point (|velocity| > kFxMaxVel: 256 terms of at most
 // 2^14 stay below 2^23 = 2^63 / 2^40), or is not finite, sums in float instead.
 constexpr int kSplatTile = 2048;  // cells (float4 each, 32 KiB of LDS)
@@ -203,8 +205,9 @@ __global__ void __launch_bounds__(256) k_particle_scatter(Geom g0, const float4*
     for (int c = threadIdx.x; c < 4 * area; c += 256) tile[c] = 0ull;
     __syncthreads();
     auto ladd = [&](int i, int j, int comp, float v) {
-      lds_add_quad_fx(tile, live ? 4 * ((i - i0) + (j - j0) * bw) + comp : -1,
-                      live ? fx_term(v) : 0ll);
+      const long long t = fx_term(v);
+      if (live && t != 0ll)
+        atomicAdd(tile + 4 * ((i - i0) + (j - j0) * bw) + comp, (unsigned long long)t);
     };
     splat_terms(su, q.z, 0, ladd);
     splat_terms(sv, q.w, 2, ladd);
