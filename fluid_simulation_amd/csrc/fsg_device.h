@@ -112,29 +112,6 @@ __device__ __forceinline__ float fx_value(long long v) {
   return (float)((double)v * (1.0 / 1099511627776.0));
 }
 
-// lds_add_quad for 64-bit integer terms: integer sums are exact, so the result does
-// not depend on which lane adds first.
-__device__ __forceinline__ long long quad_xor1_64(long long v) {
-  const unsigned lo = (unsigned)quad_xor1((int)(unsigned)(unsigned long long)v);
-  const unsigned hi = (unsigned)quad_xor1((int)(unsigned)((unsigned long long)v >> 32));
-  return (long long)(((unsigned long long)hi << 32) | lo);
-}
-__device__ __forceinline__ long long quad_xor2_64(long long v) {
-  const unsigned lo = (unsigned)quad_xor2((int)(unsigned)(unsigned long long)v);
-  const unsigned hi = (unsigned)quad_xor2((int)(unsigned)((unsigned long long)v >> 32));
-  return (long long)(((unsigned long long)hi << 32) | lo);
-}
-__device__ __forceinline__ void lds_add_quad_fx(unsigned long long* tile, int addr, long long v) {
-  const int lane = threadIdx.x & 63;
-  const int a1 = quad_xor1(addr);
-  const long long v1 = quad_xor1_64(v);
-  if (a1 == addr) v = (lane & 1) ? 0ll : v + v1;
-  const int a2 = quad_xor2(addr);
-  const long long v2 = quad_xor2_64(v);
-  if (a2 == addr) v = (lane & 2) ? 0ll : v + v2;
-  if (addr >= 0 && v != 0ll) atomicAdd(tile + addr, (unsigned long long)v);
-}
-
 struct SplatCorners {
   int i, j, i1, j1;
   float fi, fj;
