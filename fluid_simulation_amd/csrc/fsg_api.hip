@@ -1027,7 +1027,7 @@ void fsg_params_default(fsg_params* p) {
   p->mg_spread_tiles = 1;
   p->mg_fuse_update = 1;
   p->splat_order = 1;
-  p->mg_tile_threads = 256;
+  p->mg_tile_threads = 512;
   p->mg_level_sweeps = 5;
   p->mg_level0_sweeps = 3;
 }

@@ -166,7 +166,7 @@ typedef struct fsg_params {
                              storage order (cheaper while particles that are neighbours
                              in storage are neighbours in space) */
   int32_t mg_tile_threads;/* multigrid solver: threads per 32 x 32 level-0 tile of the
-                             down / up kernels, 256 (default), 512 or 1024 (more waves
+                             down / up kernels, 256, 512 (default) or 1024 (more waves
                              per tile hide more memory latency where few tiles hold
                              unknowns; fewer leave room for more tiles per CU) */
   int32_t mg_level_sweeps;/* multigrid solver: smoothing sweeps before and after the

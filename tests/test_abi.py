@@ -50,7 +50,7 @@ def test_params_struct_layout_and_defaults():
     assert p.deterministic == 0 and p.record_decisions == 0 and p.mg_smoother == 1
     assert p.mg_coarse_sweeps == 0 and p.graph_iters == 0 and p.mg_fuse_update == 1
     assert p.splat_order == 1
-    assert p.mg_tile_threads == 256 and p.mg_level_sweeps == 5 and p.mg_level0_sweeps == 3
+    assert p.mg_tile_threads == 512 and p.mg_level_sweeps == 5 and p.mg_level0_sweeps == 3
     assert p.mg_spread_tiles == 1
 
 
