@@ -141,9 +141,9 @@ def solve_bytes(name, nx, ny, solves):
 # FETCH_SIZE reads 0.5 GiB for its 1 GiB read (factor 2.0), WRITE_SIZE 1 GiB (1.0).
 # traffic = (2.0 * FETCH_SIZE + 1.0 * WRITE_SIZE) * 1024.
 MEASURED_TRAFFIC_1024 = {
-    "cg_mg_down0": (2.0 * 3323.8 + 6283.6) * 1024,
-    "cg_mg_up0": (2.0 * 1837.2 + 1309.8) * 1024,
-    "cg_mg_dir": (2.0 * 1308.5 + 1637.5) * 1024,
+    "cg_mg_down0": (2.0 * 3201.4 + 6303.8) * 1024,
+    "cg_mg_up0": (2.0 * 1828.3 + 1309.8) * 1024,
+    "cg_mg_dir": (2.0 * 1229.6 + 1637.5) * 1024,
     "cg_mg_down4": (2.0 * 66.2 + 20.8) * 1024,
     "particle_scatter": (2.0 * 8065.4 + 18469.3) * 1024,
     "particle_gather": (2.0 * 10294.5 + 16676.3) * 1024,
