@@ -994,7 +994,9 @@ def main():
             "solver": args.solver,
             "graph": args.graph,
             "mg_smoother": ["damped Jacobi 0.8", "two-step Chebyshev [0.4, 2]"][args.mg_smoother]
-                           + " on level 0 and the one-workgroup levels",
+                           + " on the levels that take two sweeps (the one-workgroup levels;"
+                           " level 0 takes mg_level0_sweeps, the tile levels below it"
+                           " mg_level_sweeps, degree-S Chebyshev where S > 2)",
             "mg_level_sweeps": (tune or {}).get("mg_level_sweeps", 5),
             "mg_level0_sweeps": (tune or {}).get("mg_level0_sweeps", 3),
             "tol": args.tol,
